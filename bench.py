@@ -1,0 +1,177 @@
+"""Closest-hit throughput on the diced-patch heightfield scene.
+
+Workload (one step): one closest-hit trace of a 2^22-ray stream against the
+64x64 bilinear-patch heightfield (psychopath_cpp_amd.configs.heightfield_scene,
+split-diced at dice_rate 1.0): 2^21 primary rays (1024x1024, 2 samples per
+pixel, Halton sample streams) followed by 2^21 cosine-distributed Lambert
+first-bounce rays from their hits (sample dimensions 9 and 10).  The stream
+is generated on the GPU and is resident in HBM before timing starts; it is
+traced in its 56-byte-per-ray form (origin, direction, max_t, time, six
+footprint terms) and writes 16 bytes per ray (t, instance, u, v).
+
+Prints one JSON line.  `roofline` prices the trace kernel against HBM
+(72 algorithmic bytes per ray, 8 TB/s); `cpu_baseline` times the oracle's
+restatement of the reference's breadth-first CPU tracer on a sample of the
+same stream, on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+BYTES_PER_RAY = 72          # 56 B ray in + 16 B hit out
+HBM_PEAK_GBS = 8000.0       # MI355X HBM3E
+STREAM_RAYS = 1 << 22
+W, H, SPP, SEED = 1024, 1024, 2, 1
+METRIC = "closest-hit Mrays/s + %HBM roofline on diced-patch scene"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_stream(ctx, scene):
+    import torch
+    from psychopath_cpp_amd import gpu
+    prim, _ = gpu.camera_rays(ctx, scene, W, H, SPP, SEED)
+    hits = gpu.trace_world(ctx, scene, prim)
+    bounce, valid = gpu.bounce_rays(ctx, scene, prim, hits, W, SPP, SEED, 9, 10)
+    idx = torch.nonzero(valid, as_tuple=False).flatten()
+    nb = STREAM_RAYS - prim.shape[1]
+    if idx.numel() == 0:
+        raise RuntimeError("no valid bounce rays")
+    # every primary ray of this camera lands on the field; if some bounce is
+    # invalid, the valid ones are repeated cyclically to keep 2^21 rays
+    idx = idx.repeat((nb + idx.numel() - 1) // idx.numel())[:nb]
+    wrays = torch.cat([prim, bounce[:, idx]], dim=1).contiguous()
+    return wrays, int(valid.sum().item())
+
+
+def cpu_baseline(scene_py, wrays, sample):
+    from oracle import ora
+    threads = min(16, os.cpu_count() or 1)
+    sub = wrays[:, :sample].cpu().numpy()
+    w = ora.wrays_from_planes(sub)
+    o = ora.OracleScene(scene_py)
+    t0 = time.perf_counter()
+    inter = o.trace(w, threads=threads, chunk=65536)
+    dt = time.perf_counter() - t0
+    return sample / dt / 1e6, threads, inter, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-sample", type=int, default=1 << 21)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--dice-depth", type=int, default=-1)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box has one GPU; "
+              "the path shards by image tiles or ray ranges, see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import torch
+    from psychopath_cpp_amd import configs, gpu
+
+    log("setting up heightfield scene")
+    ctx = gpu.Context(0)
+    scene_py = configs.heightfield_scene()
+    t0 = time.perf_counter()
+    scene = gpu.GpuScene(ctx, scene_py, dice_depth=args.dice_depth)
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - t0
+    info = scene.info
+    log(f"scene: {info.n_leaf_instances} leaves, {info.n_bvh_nodes} BVH4 nodes, dice depth {info.dice_depth}, "
+        f"{info.dice_cache_bytes / 2**20:.0f} MiB dice cache, built in {build_s:.2f}s")
+    wrays, n_valid = make_stream(ctx, scene)
+    rays = gpu.world_to_rays(ctx, wrays)
+    hits = gpu.HitBuffers(STREAM_RAYS, rays.device, aux=False)
+    torch.cuda.synchronize()
+    log(f"stream: {rays.shape[1]} rays ({n_valid} valid bounces)")
+
+    for _ in range(args.warmup):
+        gpu.trace_rays(ctx, scene, rays, hits=hits)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_wall = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        gpu.trace_rays(ctx, scene, rays, hits=hits)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_wall
+    ms = ev0.elapsed_time(ev1) / args.steps
+    mrays = STREAM_RAYS / (ms / 1e3) / 1e6
+    achieved = BYTES_PER_RAY * STREAM_RAYS / (ms / 1e3) / 1e9
+    hit_frac = float((hits.inst != -1).float().mean().item())
+    log(f"trace: {ms:.3f} ms/step (HIP events), wall {wall / args.steps * 1e3:.3f} ms/step, hit fraction {hit_frac:.4f}")
+
+    cpu = None
+    if not args.no_cpu:
+        sample = min(args.cpu_sample, STREAM_RAYS)
+        log(f"cpu baseline: oracle breadth-first tracer on {sample} rays")
+        cpu_mrays, threads, inter, dt = cpu_baseline(scene_py, wrays, sample)
+        log(f"cpu baseline: {cpu_mrays:.3f} Mrays/s on {threads} threads ({dt:.1f}s)")
+        # spot parity on the same sample
+        import numpy as np
+        ids = scene.instance_ids()
+        gi = hits.inst[:sample].cpu().numpy().view(np.uint32)
+        gt = hits.t[:sample].cpu().numpy()
+        ghit = gi != 0xFFFFFFFF
+        ok = (ghit == (inter["hit"] != 0))
+        both = ghit & (inter["hit"] != 0)
+        ok[both] &= ids[gi[both]] == inter["id"][both]
+        ok[both] &= np.abs(gt[both] - inter["t"][both]) <= 1e-5 * np.abs(inter["t"][both])
+        log(f"parity on the cpu sample: {ok.mean():.6f} of rays match")
+        cpu = {"value": round(cpu_mrays, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+               "sample": f"first {sample} rays of the same 2^22 stream (primary rays, 1024x1024x2 spp), "
+                         f"oracle restatement of the reference breadth-first BVH4 + split-dice tracer, "
+                         f"65536-ray chunks, {threads} threads"}
+
+    out = {
+        "metric": METRIC,
+        "value": round(mrays, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "heightfield 64x64 bilinear patches over 10x10 (8 seeded sines), dice_rate 1.0; "
+                        "2^22-ray closest-hit stream = 2^21 primary (1024x1024, 2 spp) + 2^21 cosine first-bounce",
+            "rays_per_step": STREAM_RAYS,
+            "dice_depth": int(info.dice_depth),
+            "leaf_instances": int(info.n_leaf_instances),
+            "bytes_per_ray": BYTES_PER_RAY,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": None,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,226 @@
+/*
+ * psy_gpu.h -- C ABI of the MI355X ray-stream library (libpsy_gpu.so).
+ *
+ * Plain C, POD structs, device pointers as void/float/uint32_t pointers,
+ * int status codes with psy_last_error().  No torch or HIP types appear in
+ * the signatures; a `stream` argument is a hipStream_t passed as void*
+ * (NULL = the null stream).
+ *
+ * Every entry point below names the reference interface it stands in for
+ * (paths relative to the psychopath_cpp source tree).
+ *
+ * Conventions carried over from the reference:
+ *   - matrices are row-major float[16] in row-vector convention (p' = p*M),
+ *     the layout of Matrix44::data (math/matrix.hpp:18-22, :99-106);
+ *   - an instance's transforms map its parent space to its local space;
+ *     n motion samples are evenly spaced over time in [0,1]
+ *     (scene/assembly.hpp:26-65, utils/utils.hpp:76-91);
+ *   - ray SoA "planes": plane p of an n-ray buffer starts at base + p*n.
+ */
+#ifndef PSY_GPU_H
+#define PSY_GPU_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PSY_ABI_VERSION 1
+
+/* ---- status codes ---------------------------------------------------- */
+enum psy_status {
+    PSY_OK = 0,
+    PSY_ERR_INVALID = 1,      /* bad argument / inconsistent description */
+    PSY_ERR_HIP = 2,          /* a HIP runtime call failed                */
+    PSY_ERR_UNSUPPORTED = 3,  /* valid input the device path cannot take  */
+    PSY_ERR_NO_DEVICE = 4     /* no GPU visible                            */
+};
+
+/* ---- scene description (host memory, read during psy_scene_create) --- */
+
+/* Object kinds (object/object.hpp:21-138, lights/sphere_light.hpp,
+ * lights/rectangle_light.hpp). */
+enum psy_object_kind {
+    PSY_OBJ_SPHERE = 1,          /* per time sample: cx cy cz r                  (4)  */
+    PSY_OBJ_BILINEAR = 2,        /* per time sample: v1 v2 v3 v4, xyz each      (12) */
+    PSY_OBJ_BICUBIC = 3,         /* per time sample: 16 control points, xyz     (48) */
+    PSY_OBJ_SPHERE_LIGHT = 4,    /* per time sample: cx cy cz r  R G B            (7) */
+    PSY_OBJ_RECTANGLE_LIGHT = 5  /* per time sample: width height  R G B          (5) */
+};
+
+typedef struct psy_object_desc {
+    int32_t kind;          /* enum psy_object_kind              */
+    int32_t time_samples;  /* >= 1                              */
+    const float* data;     /* time_samples * floats-per-kind    */
+} psy_object_desc;
+
+enum psy_instance_type { PSY_INSTANCE_OBJECT = 0, PSY_INSTANCE_ASSEMBLY = 1 };
+
+/* One Instance (scene/assembly.hpp:26-65).  shader_index indexes the
+ * scene-wide shader table (-1 = none, i.e. inherit from the parent). */
+typedef struct psy_instance_desc {
+    int32_t type;             /* enum psy_instance_type                      */
+    int32_t data_index;       /* object index or child-assembly index        */
+    int32_t transform_index;  /* first transform in the assembly's list     */
+    int32_t transform_count;  /* 0 = untransformed                           */
+    int32_t shader_index;     /* -1 = none                                   */
+} psy_instance_desc;
+
+/* Surface shaders (shading/surface_shader.hpp:28-53). */
+enum psy_shader_kind { PSY_SHADER_LAMBERT = 1, PSY_SHADER_EMIT = 2 };
+typedef struct psy_shader_desc {
+    int32_t kind;
+    float color[3];  /* linear RGB with white point rgb<1,1,1> */
+} psy_shader_desc;
+
+/* One Assembly (scene/assembly.hpp:74-330).  Child assemblies are indices
+ * into psy_scene_desc.assemblies; entry 0 of that array is the root. */
+typedef struct psy_assembly_desc {
+    const psy_object_desc* objects;     int32_t n_objects;
+    const psy_instance_desc* instances; int32_t n_instances;
+    const float* transforms;            int32_t n_transforms;  /* 16 floats each */
+    const int32_t* assemblies;          int32_t n_assemblies;  /* scene indices  */
+} psy_assembly_desc;
+
+/* Camera (basics/camera.hpp:14-105): per-sample lists, fov in radians. */
+typedef struct psy_camera_desc {
+    const float* transforms; int32_t n_transforms;  /* camera-to-world, 16 floats */
+    const float* fovs;       int32_t n_fovs;
+    const float* aperture_radii;  int32_t n_aperture_radii;
+    const float* focus_distances; int32_t n_focus_distances;
+} psy_camera_desc;
+
+typedef struct psy_scene_desc {
+    const psy_assembly_desc* assemblies; int32_t n_assemblies;  /* [0] = root */
+    const psy_shader_desc* shaders;      int32_t n_shaders;
+    psy_camera_desc camera;
+    float background[3];
+    float dice_rate;       /* Config::dice_rate (config.cpp:7), reference default 0.25 */
+    float min_upoly_size;  /* Config::min_upoly_size (config.cpp:8), default 1e-5       */
+} psy_scene_desc;
+
+/* ---- ray / hit streams (device memory) -------------------------------- */
+
+/* World rays, WorldRay (basics/ray.hpp:162-185): 20 planes of n entries.
+ * Planes 0-2 o, 3-5 d, 6-8 odx, 9-11 ody, 12-14 ddx, 15-17 ddy, 18 time,
+ * 19 type (uint32 bit pattern: PSY_RAY_OCCLUSION = WorldRay::OCCLUSION). */
+#define PSY_WRAY_PLANES 20
+#define PSY_RAY_OCCLUSION 32u   /* WorldRay::OCCLUSION (basics/ray.hpp:177) */
+
+/* Object-space rays with their footprint already derived, i.e. the fields
+ * of Ray that WorldRay::to_ray() fills (basics/ray.hpp:24-36, :207-252):
+ * 15 planes.  0-2 o, 3-5 d, 6 max_t, 7 time, 8 owx, 9 owy, 10 dwx, 11 dwy,
+ * 12 fwx, 13 fwy, 14 flags (uint32 bit pattern, bit 30 = occlusion as in
+ * Ray::id_and_flags).  Valid only for scenes whose patch instances are
+ * untransformed; otherwise pass world rays. */
+#define PSY_RAY_PLANES 15
+
+/* Hit record, the closest-hit part of Intersection
+ * (basics/intersection.hpp:22-49).  All device pointers; `aux` may be NULL.
+ *   t     : hit distance (+inf on a miss)
+ *   inst  : flattened leaf-instance index, 0xffffffff on a miss
+ *   u, v  : surface parameters (0 for lights)
+ *   aux   : patch leaf extent max_dim (offset = n*max_dim*1.74), 0 else   */
+typedef struct psy_hits {
+    float* t;
+    uint32_t* inst;
+    float* u;
+    float* v;
+    float* aux;
+} psy_hits;
+
+#define PSY_TRACE_OCCLUSION_ONLY 1u  /* any-hit: stop at the first hit */
+
+/* ---- opaque handles ----------------------------------------------------- */
+typedef struct psy_ctx psy_ctx;
+typedef struct psy_scene psy_scene;
+
+typedef struct psy_scene_info {
+    int64_t n_leaf_instances;    /* flattened (assembly hierarchy expanded)  */
+    int64_t n_objects;
+    int64_t n_bvh_nodes;
+    int64_t n_patches;
+    int32_t dice_depth;          /* cached split depth of the dice cache     */
+    int32_t bvh_stack_need;      /* worst-case traversal stack entries       */
+    int64_t device_bytes;        /* scene bytes resident in HBM              */
+    int64_t dice_cache_bytes;
+} psy_scene_info;
+
+/* ---- context ------------------------------------------------------------ */
+/* Replaces: the per-worker Tracer construction (tracer/tracer.hpp:57-63);
+ * the GPU context owns device memory, scene uploads and kernels. */
+int psy_ctx_create(int device, psy_ctx** out);
+int psy_ctx_destroy(psy_ctx* ctx);
+const char* psy_last_error(void);
+int psy_abi_version(void);
+
+/* ---- scene -------------------------------------------------------------- */
+/* Replaces: Scene::finalize -> Assembly::finalize -> BVH4::build /
+ * LightTree::build (scene/scene.hpp:50-52, scene/assembly.hpp:206-238,
+ * accel/bvh4.cpp:17-159).  Flattens the assembly hierarchy, builds the
+ * instance BVH and the patch dice cache in HBM.  dice_depth < 0 picks it
+ * from the scene. */
+int psy_scene_create(psy_ctx* ctx, const psy_scene_desc* desc, int dice_depth,
+                     psy_scene** out);
+int psy_scene_destroy(psy_scene* scene);
+int psy_scene_get_info(const psy_scene* scene, psy_scene_info* info);
+/* InstanceID (basics/instance_id.hpp:9-40) of every flattened leaf, as the
+ * reference's Tracer would report it in Intersection::id (host memory). */
+int psy_scene_instance_ids(const psy_scene* scene, uint64_t* out, int64_t n);
+
+/* ---- tracing ------------------------------------------------------------ */
+/* Replaces: uint32_t Tracer::trace(const WorldRay*, const WorldRay*,
+ *           Intersection*, Intersection*)  (tracer/tracer.hpp:76,
+ *           tracer/tracer.cpp:38-121).  Closest hit per ray (or any hit
+ *           with PSY_TRACE_OCCLUSION_ONLY / per-ray occlusion type). */
+int psy_trace_world(psy_ctx* ctx, const psy_scene* scene,
+                    const float* world_rays, int64_t n,
+                    const psy_hits* hits, uint32_t flags, void* stream);
+/* Same, for rays whose footprint was already derived (WorldRay::to_ray,
+ * basics/ray.hpp:310-323); the 56-byte-per-ray form used for streams. */
+int psy_trace_rays(psy_ctx* ctx, const psy_scene* scene,
+                   const float* rays, int64_t n,
+                   const psy_hits* hits, uint32_t flags, void* stream);
+/* WorldRay::to_ray over a stream: world rays -> object rays (planes as
+ * PSY_RAY_PLANES).  Replaces tracer/tracer.cpp:47-51. */
+int psy_world_to_rays(psy_ctx* ctx, const float* world_rays, int64_t n,
+                      float* rays, void* stream);
+
+/* ---- sampling ----------------------------------------------------------- */
+/* Replaces: Halton::sample(dim, index) (sampling/halton.cpp:2154-2671). */
+int psy_halton(psy_ctx* ctx, uint32_t dim, const uint32_t* index, int64_t n,
+               float* out, void* stream);
+/* Replaces: Sampler(x,y,s,seed) + Sampler::get_sample(dim)
+ * (sampling/image_sampler.hpp:16-38): out[i*n_dims + k] = sample of dim
+ * dim0+k for pixel sample i of the tile (x0..x0+w, y0..y0+h, spp). */
+int psy_pixel_samples(psy_ctx* ctx, int32_t x0, int32_t y0, int32_t w, int32_t h,
+                      int32_t spp, uint32_t seed, uint32_t dim0, int32_t n_dims,
+                      float* out, void* stream);
+
+/* ---- stream generation (integrator steps) ------------------------------ */
+/* Camera rays: PathTraceIntegrator::init_path + next_ray_for_path step 0
+ * (integrator/path_trace_integrator.cpp:91-125, basics/camera.hpp:72-104)
+ * for every pixel sample of the image, ordered (y, x, s) row-major.
+ * Writes world rays (PSY_WRAY_PLANES) and the hero wavelength per ray. */
+int psy_camera_rays(psy_ctx* ctx, const psy_scene* scene, int32_t width,
+                    int32_t height, int32_t spp, uint32_t seed,
+                    float* world_rays, float* wavelength, void* stream);
+/* Lambert bounce rays from the closest hits of `in_rays`
+ * (integrator/path_trace_integrator.cpp:183-209,
+ *  shading/surface_closure.hpp:233-331).  Sample dimensions
+ * dim_u, dim_v of each path's Sampler; rays that missed (or hit a light)
+ * are written with d = 0 and flagged in `valid` (uint32 0/1).  pix_*:
+ * the generating pixel sample (same ordering as psy_camera_rays). */
+int psy_bounce_rays(psy_ctx* ctx, const psy_scene* scene,
+                    const float* in_world_rays, const psy_hits* in_hits,
+                    int64_t n, int32_t width, int32_t spp, uint32_t seed,
+                    uint32_t dim_u, uint32_t dim_v,
+                    float* out_world_rays, uint32_t* valid, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PSY_GPU_H */

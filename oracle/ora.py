@@ -1,0 +1,204 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes bindings of the oracle (liboracle.so)
+and of oracle/_ref/libpsyref.so (the reference's own self-contained
+sources).  Imported only by tests/, the smoke check and bench.py's
+CPU-baseline leg, as the checker -- never by the product.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libpsyref.so")
+REFERENCE_TREE = "/root/reference"
+
+WRAY_DTYPE = np.dtype([("o", "<f4", 3), ("d", "<f4", 3), ("odx", "<f4", 3), ("ody", "<f4", 3),
+                       ("ddx", "<f4", 3), ("ddy", "<f4", 3), ("time", "<f4"), ("type", "<u4")])
+INTER_DTYPE = np.dtype([("hit", "<i4"), ("backfacing", "<i4"), ("id", "<u8"), ("t", "<f4"),
+                        ("light_pdf", "<f4"), ("space", "<f4", 16), ("u", "<f4"), ("v", "<f4"),
+                        ("p", "<f4", 3), ("dpdu", "<f4", 3), ("dpdv", "<f4", 3), ("n", "<f4", 3),
+                        ("dndu", "<f4", 3), ("dndv", "<f4", 3), ("offset", "<f4", 3),
+                        ("closure", "<i4"), ("closure_col", "<f4", 3), ("closure_prob", "<f4"),
+                        ("patch_dim", "<f4")], align=True)
+RAY_DTYPE = np.dtype([("o", "<f4", 3), ("d", "<f4", 3), ("max_t", "<f4"), ("time", "<f4"),
+                      ("owx", "<f4"), ("owy", "<f4"), ("dwx", "<f4"), ("dwy", "<f4"), ("fwx", "<f4"),
+                      ("fwy", "<f4"), ("flags", "<u4")])
+
+_lib = None
+_ref = None
+
+
+def build(with_ref=None):
+    """Compile the oracle (and oracle/_ref when the reference tree exists)."""
+    subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
+    if with_ref is None:
+        with_ref = os.path.isdir(REFERENCE_TREE)
+    if with_ref:
+        subprocess.run(["make", "-s", "-C", HERE, "ref", f"REF={REFERENCE_TREE}"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build(with_ref=False)
+        L = C.CDLL(LIB)
+        vp, i64, u32 = C.c_void_p, C.c_int64, C.c_uint32
+        L.ora_halton.restype = C.c_float
+        L.ora_halton.argtypes = [u32, u32]
+        L.ora_halton_batch.argtypes = [u32, vp, i64, vp]
+        L.ora_hash_u32.restype = u32
+        L.ora_hash_u32.argtypes = [u32, u32]
+        L.ora_sampler_offset.restype = u32
+        L.ora_sampler_offset.argtypes = [u32, u32, u32, u32]
+        L.ora_sampler_get.restype = C.c_float
+        L.ora_sampler_get.argtypes = [u32, u32]
+        L.ora_matrix_inverse.argtypes = [vp, vp]
+        L.ora_bbox_ray.restype = C.c_int
+        L.ora_bbox_ray.argtypes = [vp, vp, vp, vp, C.c_float, vp, vp]
+        L.ora_bbox4_ray.restype = C.c_uint
+        L.ora_bbox4_ray.argtypes = [vp, vp, vp, C.c_float, vp]
+        L.ora_world_to_rays.argtypes = [vp, i64, vp]
+        L.ora_vec3_length.restype = C.c_float
+        L.ora_vec3_length.argtypes = [vp]
+        L.ora_vec3_normalize.argtypes = [vp, vp]
+        L.ora_vec3_dot.restype = C.c_float
+        L.ora_vec3_dot.argtypes = [vp, vp]
+        L.ora_vec3_cross.argtypes = [vp, vp, vp]
+        L.ora_partition_ints.restype = C.c_int
+        L.ora_partition_ints.argtypes = [vp, C.c_int, C.c_int, vp]
+        L.ora_scene_create.restype = vp
+        L.ora_scene_create.argtypes = [vp]
+        L.ora_scene_destroy.argtypes = [vp]
+        L.ora_scene_bvh4_nodes.restype = i64
+        L.ora_scene_bvh4_nodes.argtypes = [vp, C.c_int]
+        L.ora_trace.restype = i64
+        L.ora_trace.argtypes = [vp, vp, i64, vp, C.c_int, i64]
+        L.ora_trace_brute.restype = i64
+        L.ora_trace_brute.argtypes = [vp, vp, i64, vp, C.c_int, i64]
+        L.ora_camera_rays.argtypes = [vp, C.c_int, C.c_int, C.c_int, u32, i64, i64, vp, vp]
+        L.ora_bounce_rays.argtypes = [vp, vp, vp, i64, i64, C.c_int, C.c_int, u32, u32, u32, vp, vp]
+        _lib = L
+    return _lib
+
+
+def ref():
+    """oracle/_ref/libpsyref.so or None when it was not built."""
+    global _ref
+    if _ref is None:
+        if not os.path.exists(REF_LIB):
+            return None
+        R = C.CDLL(REF_LIB)
+        R.psyref_halton.restype = C.c_float
+        R.psyref_halton.argtypes = [C.c_uint32, C.c_uint32]
+        R.psyref_sobol.restype = C.c_double
+        R.psyref_sobol.argtypes = [C.c_uint32, C.c_uint64, C.c_uint64]
+        R.psyref_sobol_matrix.restype = C.c_uint64
+        R.psyref_sobol_matrix.argtypes = [C.c_uint32]
+        R.psyref_sobol_dims.restype = C.c_uint32
+        R.psyref_hash_u32.restype = C.c_uint32
+        R.psyref_hash_u32.argtypes = [C.c_uint32, C.c_uint32]
+        R.psyref_invert44.restype = C.c_float
+        R.psyref_invert44.argtypes = [C.c_void_p]
+        R.psyref_spectrum_xyz_to_p.restype = C.c_float
+        R.psyref_spectrum_xyz_to_p.argtypes = [C.c_float, C.c_void_p]
+        _ref = R
+    return _ref
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def halton(dim, index):
+    index = np.ascontiguousarray(index, np.uint32)
+    out = np.empty(index.shape, np.float32)
+    lib().ora_halton_batch(dim, _p(index), index.size, _p(out))
+    return out
+
+
+def matrix_inverse(m):
+    m = np.ascontiguousarray(m, np.float32).reshape(16)
+    out = np.empty(16, np.float32)
+    lib().ora_matrix_inverse(_p(m), _p(out))
+    return out.reshape(4, 4)
+
+
+def world_to_rays(wrays):
+    out = np.empty(wrays.shape[0], RAY_DTYPE)
+    lib().ora_world_to_rays(_p(wrays), wrays.shape[0], _p(out))
+    return out
+
+
+class OracleScene:
+    def __init__(self, scene):
+        self._desc = scene.to_desc() if hasattr(scene, "to_desc") else scene
+        self.handle = lib().ora_scene_create(self._desc.ptr())
+        if not self.handle:
+            raise RuntimeError("ora_scene_create failed")
+
+    def close(self):
+        if self.handle:
+            lib().ora_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def trace(self, wrays, threads=1, chunk=65536, brute=False):
+        wrays = np.ascontiguousarray(wrays, WRAY_DTYPE)
+        out = np.zeros(wrays.shape[0], INTER_DTYPE)
+        fn = lib().ora_trace_brute if brute else lib().ora_trace
+        fn(self.handle, _p(wrays), wrays.shape[0], _p(out), int(threads), int(chunk))
+        return out
+
+    def camera_rays(self, width, height, spp, seed, first=0, count=None):
+        n = width * height * spp if count is None else count
+        out = np.zeros(n, WRAY_DTYPE)
+        wl = np.zeros(n, np.float32)
+        lib().ora_camera_rays(self.handle, width, height, spp, seed, first, n, _p(out), _p(wl))
+        return out, wl
+
+    def bounce_rays(self, wrays, inters, width, spp, seed, first=0, dim_u=9, dim_v=10):
+        n = wrays.shape[0]
+        out = np.zeros(n, WRAY_DTYPE)
+        valid = np.zeros(n, np.uint32)
+        lib().ora_bounce_rays(self.handle, _p(np.ascontiguousarray(wrays)), _p(np.ascontiguousarray(inters)),
+                              first, n, width, spp, seed, dim_u, dim_v, _p(out), _p(valid))
+        return out, valid
+
+
+def wrays_from_planes(planes):
+    """(20, n) float32 plane layout -> WRAY_DTYPE records."""
+    planes = np.asarray(planes, np.float32)
+    n = planes.shape[1]
+    w = np.zeros(n, WRAY_DTYPE)
+    w["o"] = planes[0:3].T
+    w["d"] = planes[3:6].T
+    w["odx"] = planes[6:9].T
+    w["ody"] = planes[9:12].T
+    w["ddx"] = planes[12:15].T
+    w["ddy"] = planes[15:18].T
+    w["time"] = planes[18]
+    w["type"] = planes[19].view(np.uint32)
+    return w
+
+
+def planes_from_wrays(w):
+    """WRAY_DTYPE records -> (20, n) float32 plane layout."""
+    n = w.shape[0]
+    p = np.zeros((20, n), np.float32)
+    p[0:3] = w["o"].T
+    p[3:6] = w["d"].T
+    p[6:9] = w["odx"].T
+    p[9:12] = w["ody"].T
+    p[12:15] = w["ddx"].T
+    p[15:18] = w["ddy"].T
+    p[18] = w["time"]
+    p[19] = w["type"].astype(np.uint32).view(np.float32)
+    return p

@@ -1,0 +1,107 @@
+/*
+ * psy_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * The oracle: a plain-C CPU restatement of the reference's breadth-first
+ * ray-stream hot path (psychopath_cpp), used only by tests/, the smoke
+ * check and the CPU-baseline leg of bench.py, as the checker.  The product
+ * library (libpsy_gpu.so) never links or calls it.
+ *
+ * Each function cites the reference file:line it restates in its
+ * definition.  Scene input is the product's own psy_scene_desc, so both
+ * sides consume identical inputs.
+ */
+#ifndef PSY_ORACLE_H
+#define PSY_ORACLE_H
+
+#include <stdint.h>
+#include "../include/psy_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* WorldRay (basics/ray.hpp:162-185), AoS */
+typedef struct {
+    float o[3], d[3], odx[3], ody[3], ddx[3], ddy[3];
+    float time;
+    uint32_t type;   /* PSY_RAY_OCCLUSION = occlusion ray */
+} ora_wray;
+
+/* Intersection (basics/intersection.hpp:22-49), flattened to POD */
+typedef struct {
+    int32_t hit;
+    int32_t backfacing;
+    uint64_t id;          /* InstanceID::id */
+    float t;
+    float light_pdf;
+    float space[16];      /* Transform::to */
+    float u, v;
+    float p[3], dpdu[3], dpdv[3], n[3], dndu[3], dndv[3];
+    float offset[3];
+    int32_t closure;      /* 0 none, PSY_SHADER_LAMBERT, PSY_SHADER_EMIT */
+    float closure_col[3];
+    float closure_prob;
+    float patch_dim;      /* max_dim of the accepted patch leaf (0 if none) */
+} ora_inter;
+
+/* Ray with derived footprint (the product's PSY_RAY_PLANES fields) */
+typedef struct {
+    float o[3], d[3], max_t, time, owx, owy, dwx, dwy, fwx, fwy;
+    uint32_t flags;
+} ora_ray_out;
+
+/* sampling */
+float ora_halton(uint32_t dim, uint32_t index);
+void ora_halton_batch(uint32_t dim, const uint32_t* index, int64_t n, float* out);
+uint32_t ora_hash_u32(uint32_t n, uint32_t seed);
+uint32_t ora_sampler_offset(uint32_t x, uint32_t y, uint32_t n, uint32_t seed);
+float ora_sampler_get(uint32_t offset, uint32_t dim);
+
+/* math KAT helpers */
+void ora_matrix_inverse(const float in[16], float out[16]);
+int ora_bbox_ray(const float bmin[3], const float bmax[3], const float o[3],
+                 const float d[3], float max_t, float* h0, float* h1);
+unsigned ora_bbox4_ray(const float bounds[24], const float o[3], const float d[3],
+                       float max_t, float near_out[4]);
+void ora_world_to_rays(const ora_wray* w, int64_t n, ora_ray_out* out);
+float ora_vec3_length(const float v[3]);
+void ora_vec3_normalize(const float v[3], float out[3]);
+float ora_vec3_dot(const float a[3], const float b[3]);
+void ora_vec3_cross(const float a[3], const float b[3], float out[3]);
+int ora_partition_ints(int* v, int n, int key, int* calls);
+
+/* scene */
+typedef struct ora_scene ora_scene;
+ora_scene* ora_scene_create(const psy_scene_desc* desc);
+void ora_scene_destroy(ora_scene* s);
+int64_t ora_scene_bvh4_nodes(const ora_scene* s, int assembly);
+
+/* Tracer::trace (tracer/tracer.cpp:38-121) over n world rays.  Rays are
+ * processed in chunks of `chunk` rays (one Tracer per chunk, as one
+ * Tracer per worker bucket in the reference), spread over `threads`
+ * threads.  Returns the number of rays traced. */
+int64_t ora_trace(const ora_scene* s, const ora_wray* rays, int64_t n,
+                  ora_inter* out, int threads, int64_t chunk);
+/* The same with the BVH replaced by an exhaustive instance loop. */
+int64_t ora_trace_brute(const ora_scene* s, const ora_wray* rays, int64_t n,
+                        ora_inter* out, int threads, int64_t chunk);
+
+/* Camera rays for every pixel sample, ordered (y, x, s):
+ * init_path + next_ray_for_path step 0
+ * (integrator/path_trace_integrator.cpp:91-125). */
+void ora_camera_rays(const ora_scene* s, int width, int height, int spp,
+                     uint32_t seed, int64_t first, int64_t count,
+                     ora_wray* out, float* wavelength);
+
+/* Lambert bounce rays from hits (integrator/path_trace_integrator.cpp:
+ * 183-209).  valid[i] = 0 where the hit is missing or not Lambert. */
+void ora_bounce_rays(const ora_scene* s, const ora_wray* in, const ora_inter* hits,
+                     int64_t first, int64_t count, int width, int spp,
+                     uint32_t seed, uint32_t dim_u, uint32_t dim_v,
+                     ora_wray* out, uint32_t* valid);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,95 @@
+"""ctypes mirror of include/psy_gpu.h (structs and constants).
+
+Shared by the GPU bindings (psychopath_cpp_amd.gpu) and the test oracle's
+bindings, which take the same scene description.
+"""
+import ctypes as C
+
+PSY_OK = 0
+PSY_ERR_INVALID = 1
+PSY_ERR_HIP = 2
+PSY_ERR_UNSUPPORTED = 3
+PSY_ERR_NO_DEVICE = 4
+
+PSY_OBJ_SPHERE = 1
+PSY_OBJ_BILINEAR = 2
+PSY_OBJ_BICUBIC = 3
+PSY_OBJ_SPHERE_LIGHT = 4
+PSY_OBJ_RECTANGLE_LIGHT = 5
+
+PSY_INSTANCE_OBJECT = 0
+PSY_INSTANCE_ASSEMBLY = 1
+
+PSY_SHADER_LAMBERT = 1
+PSY_SHADER_EMIT = 2
+
+PSY_WRAY_PLANES = 20
+PSY_RAY_PLANES = 15
+PSY_RAY_OCCLUSION = 32
+PSY_RAY_CAMERA = 1
+PSY_RAY_R_DIFFUSE = 2
+PSY_TRACE_OCCLUSION_ONLY = 1
+
+FLOATS_PER_KIND = {
+    PSY_OBJ_SPHERE: 4,
+    PSY_OBJ_BILINEAR: 12,
+    PSY_OBJ_BICUBIC: 48,
+    PSY_OBJ_SPHERE_LIGHT: 7,
+    PSY_OBJ_RECTANGLE_LIGHT: 5,
+}
+
+MISS = 0xFFFFFFFF
+
+
+class ObjectDesc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("time_samples", C.c_int32), ("data", C.POINTER(C.c_float))]
+
+
+class InstanceDesc(C.Structure):
+    _fields_ = [("type", C.c_int32), ("data_index", C.c_int32), ("transform_index", C.c_int32),
+                ("transform_count", C.c_int32), ("shader_index", C.c_int32)]
+
+
+class ShaderDesc(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("color", C.c_float * 3)]
+
+
+class AssemblyDesc(C.Structure):
+    _fields_ = [("objects", C.POINTER(ObjectDesc)), ("n_objects", C.c_int32),
+                ("instances", C.POINTER(InstanceDesc)), ("n_instances", C.c_int32),
+                ("transforms", C.POINTER(C.c_float)), ("n_transforms", C.c_int32),
+                ("assemblies", C.POINTER(C.c_int32)), ("n_assemblies", C.c_int32)]
+
+
+class CameraDesc(C.Structure):
+    _fields_ = [("transforms", C.POINTER(C.c_float)), ("n_transforms", C.c_int32),
+                ("fovs", C.POINTER(C.c_float)), ("n_fovs", C.c_int32),
+                ("aperture_radii", C.POINTER(C.c_float)), ("n_aperture_radii", C.c_int32),
+                ("focus_distances", C.POINTER(C.c_float)), ("n_focus_distances", C.c_int32)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("assemblies", C.POINTER(AssemblyDesc)), ("n_assemblies", C.c_int32),
+                ("shaders", C.POINTER(ShaderDesc)), ("n_shaders", C.c_int32),
+                ("camera", CameraDesc), ("background", C.c_float * 3),
+                ("dice_rate", C.c_float), ("min_upoly_size", C.c_float)]
+
+
+class Hits(C.Structure):
+    _fields_ = [("t", C.c_void_p), ("inst", C.c_void_p), ("u", C.c_void_p), ("v", C.c_void_p),
+                ("aux", C.c_void_p)]
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [("n_leaf_instances", C.c_int64), ("n_objects", C.c_int64), ("n_bvh_nodes", C.c_int64),
+                ("n_patches", C.c_int64), ("dice_depth", C.c_int32), ("bvh_stack_need", C.c_int32),
+                ("device_bytes", C.c_int64), ("dice_cache_bytes", C.c_int64)]
+
+
+# Every symbol the C ABI exports (include/psy_gpu.h).
+EXPORTED_SYMBOLS = [
+    "psy_ctx_create", "psy_ctx_destroy", "psy_last_error", "psy_abi_version",
+    "psy_scene_create", "psy_scene_destroy", "psy_scene_get_info", "psy_scene_instance_ids",
+    "psy_trace_world", "psy_trace_rays", "psy_world_to_rays",
+    "psy_halton", "psy_pixel_samples", "psy_camera_rays", "psy_bounce_rays",
+]

@@ -1,0 +1,802 @@
+// psy_capi.cpp -- host side of libpsy_gpu.so: the C ABI declared in
+// include/psy_gpu.h, scene flattening, the instance BVH4 builder, uploads to
+// HBM, Halton table generation.
+//
+// Scene preparation replaces Scene::finalize / Assembly::finalize /
+// BVH4::build / LightTree::build (scene/assembly.hpp:206-238,
+// accel/bvh4.cpp:17-159).  The reference keeps a per-assembly BVH4 and
+// re-derives rays per instance while recursing (tracer/tracer.cpp:125-219).
+// Here the assembly tree is flattened once: every leaf object instance gets
+// the reference's merged transform list (math/transform.hpp:241-272) and
+// InstanceID (basics/instance_id.hpp:9-40), and one BVH4 is built over all
+// leaves.  A ray at a leaf is re-derived from the world ray with the merged
+// transform exactly as the reference does, so closest hits are unchanged.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/psy_gpu.h"
+#include "psy_device.hpp"
+
+using namespace psy;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIP_OK(expr)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(PSY_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int floats_per_kind(int kind) {
+    switch (kind) {
+        case PSY_OBJ_SPHERE: return 4;
+        case PSY_OBJ_BILINEAR: return 12;
+        case PSY_OBJ_BICUBIC: return 48;
+        case PSY_OBJ_SPHERE_LIGHT: return 7;
+        case PSY_OBJ_RECTANGLE_LIGHT: return 5;
+        default: return 0;
+    }
+}
+
+// ---- device buffer ---------------------------------------------------------
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    ~DBuf() { if (p) (void)hipFree(p); }
+    int alloc(size_t b) {
+        bytes = b;
+        if (b == 0) return 0;
+        hipError_t e = hipMalloc(&p, b);
+        if (e != hipSuccess) { p = nullptr; return fail(PSY_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e)); }
+        return 0;
+    }
+    int upload(const void* src, size_t b) {
+        int rc = alloc(b);
+        if (rc) return rc;
+        if (b) HIP_OK(hipMemcpy(p, src, b, hipMemcpyHostToDevice));
+        return 0;
+    }
+};
+
+// ---- Halton tables (sampling/halton.py:27-87 generation rule) --------------
+struct HaltonTables {
+    std::vector<HaltonDim> dims;
+    std::vector<uint16_t> perm;
+};
+
+HaltonTables make_halton_tables() {
+    HaltonTables t;
+    const int D = 256;
+    std::vector<uint32_t> primes;
+    uint32_t cand = 1;
+    while ((int)primes.size() < D) {
+        ++cand;
+        bool prime = true;
+        for (uint32_t k = 2; k * k <= cand; ++k)
+            if (cand % k == 0) { prime = false; break; }
+        if (prime) primes.push_back(cand);
+    }
+    const uint32_t maxb = primes.back();
+    std::vector<std::vector<uint16_t>> faure(maxb + 1);
+    for (uint32_t b = 0; b <= maxb; ++b) {
+        if (b < 2) { faure[b] = {0}; continue; }
+        if (b == 2) { faure[b] = {0, 1}; continue; }
+        faure[b].resize(b);
+        if (b & 1) {
+            const uint32_t c = (b - 1) / 2;
+            for (uint32_t i = 0; i < b; ++i) {
+                if (i == c) { faure[b][i] = (uint16_t)c; continue; }
+                const uint32_t f = faure[b - 1][i - (i > c ? 1 : 0)];
+                faure[b][i] = (uint16_t)(f + (f >= c ? 1 : 0));
+            }
+        } else {
+            const uint32_t c = b / 2;
+            for (uint32_t i = 0; i < b; ++i)
+                faure[b][i] = (uint16_t)(i < c ? 2 * faure[c][i] : 2 * faure[c][i - c] + 1);
+        }
+    }
+    t.dims.assign(D, HaltonDim{});
+    for (int d = 1; d < D; ++d) {
+        HaltonDim& h = t.dims[d];
+        const uint64_t base = primes[d];
+        uint32_t digits = 1;
+        uint64_t pow_base = base;
+        while (pow_base * base <= 500) { pow_base *= base; ++digits; }
+        uint64_t max_power = pow_base;
+        while (max_power * pow_base < (1ull << 32)) max_power *= pow_base;
+        h.perm_offset = (uint32_t)t.perm.size();
+        h.pow_base = (uint32_t)pow_base;
+        for (uint64_t j = 0; j < pow_base; ++j) {
+            uint64_t idx = j, result = 0;
+            for (uint32_t k = 0; k < digits; ++k) {
+                const uint64_t rem = idx % base;
+                idx /= base;
+                result = result * base + faure[base][rem];
+            }
+            t.perm.push_back((uint16_t)result);
+        }
+        uint64_t power = max_power / pow_base, div = 1;
+        uint32_t n = 0;
+        h.div[n] = 1; h.power[n] = (uint32_t)power; ++n;
+        while (power / pow_base > 1) {
+            div *= pow_base; power /= pow_base;
+            h.div[n] = (uint32_t)div; h.power[n] = (uint32_t)power; ++n;
+        }
+        h.div[n] = (uint32_t)(div * pow_base); h.power[n] = 1; ++n;
+        h.n_terms = n;
+        h.scale = 0x1.fffffep-1 / (double)max_power;
+    }
+    return t;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+struct psy_ctx {
+    int device = 0;
+    DBuf halton_dims, halton_perm;
+};
+
+struct psy_scene {
+    psy_ctx* ctx = nullptr;
+    DevScene dev{};
+    DevCamera cam{};
+    DBuf bvh, inst, obj, odata, xf, dice, shaders;
+    int stack_need = 1;
+    bool transformed_patches = false;
+    std::vector<uint64_t> ids;
+    psy_scene_info info{};
+};
+
+extern "C" {
+
+const char* psy_last_error(void) { return g_err.c_str(); }
+int psy_abi_version(void) { return PSY_ABI_VERSION; }
+
+int psy_ctx_create(int device, psy_ctx** out) {
+    if (!out) return fail(PSY_ERR_INVALID, "psy_ctx_create: out is NULL");
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+        return fail(PSY_ERR_NO_DEVICE, "psy_ctx_create: no HIP device visible");
+    if (device < 0 || device >= count) return fail(PSY_ERR_INVALID, "psy_ctx_create: bad device index");
+    HIP_OK(hipSetDevice(device));
+    std::unique_ptr<psy_ctx> c(new psy_ctx());
+    c->device = device;
+    HaltonTables t = make_halton_tables();
+    int rc = c->halton_dims.upload(t.dims.data(), t.dims.size() * sizeof(HaltonDim));
+    if (rc) return rc;
+    rc = c->halton_perm.upload(t.perm.data(), t.perm.size() * sizeof(uint16_t));
+    if (rc) return rc;
+    *out = c.release();
+    return PSY_OK;
+}
+
+int psy_ctx_destroy(psy_ctx* ctx) {
+    if (!ctx) return PSY_OK;
+    (void)hipSetDevice(ctx->device);
+    delete ctx;
+    return PSY_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Flattening
+// ---------------------------------------------------------------------------
+namespace {
+
+M44 load_m44(const float* f) {
+    M44 m;
+    std::memcpy(m.m, f, sizeof(float) * 16);
+    return m;
+}
+
+// merge(Transform* dest, a, b) (math/transform.hpp:241-272), a = parent
+std::vector<M44> merge_frames(const std::vector<M44>& a, const M44* b, size_t nb) {
+    const size_t na = a.size();
+    std::vector<M44> d;
+    if (na == nb) {
+        for (size_t i = 0; i < na; ++i) d.push_back(m_mul(a[i], b[i]));
+    } else if (na > nb) {
+        if (nb == 0) d = a;
+        else {
+            const float inv_s = 1.0f / (float)(na - 1);
+            for (size_t i = 0; i < na; ++i) d.push_back(m_mul(a[i], m_lerp_seq((float)i * inv_s, b, (int)nb)));
+        }
+    } else {
+        if (na == 0) d.assign(b, b + nb);
+        else {
+            const float inv_s = 1.0f / (float)(nb - 1);
+            for (size_t i = 0; i < nb; ++i) d.push_back(m_mul(b[i], m_lerp_seq((float)i * inv_s, a.data(), (int)na)));
+        }
+    }
+    return d;
+}
+
+uint32_t intlog2(uint32_t v) {
+    uint32_t r, shift;
+    r = (uint32_t)(v > 0xFFFF) << 4; v >>= r;
+    shift = (uint32_t)(v > 0xFF) << 3; v >>= shift; r |= shift;
+    shift = (uint32_t)(v > 0xF) << 2; v >>= shift; r |= shift;
+    shift = (uint32_t)(v > 0x3) << 1; v >>= shift; r |= shift;
+    r |= (v >> 1);
+    return r;
+}
+uint32_t upper_pow2(uint32_t v) {
+    v--; v |= v >> 1; v |= v >> 2; v |= v >> 4; v |= v >> 8; v |= v >> 16; v++;
+    return v;
+}
+
+struct FlatLeaf {
+    int32_t object;   // global object index
+    int32_t frame;    // index into frames (-1 none)
+    int32_t shader;
+    uint64_t id;
+};
+
+struct Flattener {
+    const psy_scene_desc* d;
+    std::vector<int32_t> obj_base;           // first global object of each assembly
+    std::vector<std::vector<M44>> frames;
+    std::vector<FlatLeaf> leaves;
+    int depth_guard = 0;
+    std::string err;
+
+    bool run(int a, int frame, uint64_t prefix, int shader) {
+        if (++depth_guard > 64) { err = "assembly nesting deeper than 64 (cycle?)"; return false; }
+        const psy_assembly_desc& ad = d->assemblies[a];
+        const int bits = (int)intlog2(upper_pow2((uint32_t)ad.n_instances));
+        for (int i = 0; i < ad.n_instances; ++i) {
+            const psy_instance_desc& in = ad.instances[i];
+            const uint64_t id = (prefix << bits) | ((uint64_t)i & (uint64_t)((1 << bits) - 1));
+            int fr = frame;
+            if (in.transform_count > 0) {
+                if (in.transform_index < 0 || in.transform_index + in.transform_count > ad.n_transforms) {
+                    err = "instance transform range out of bounds";
+                    return false;
+                }
+                std::vector<M44> xs;
+                for (int k = 0; k < in.transform_count; ++k)
+                    xs.push_back(load_m44(ad.transforms + 16 * (in.transform_index + k)));
+                const std::vector<M44> parent = frame >= 0 ? frames[frame] : std::vector<M44>();
+                frames.push_back(merge_frames(parent, xs.data(), xs.size()));
+                fr = (int)frames.size() - 1;
+            }
+            const int sh = in.shader_index >= 0 ? in.shader_index : shader;
+            if (in.shader_index >= d->n_shaders) { err = "shader index out of range"; return false; }
+            if (in.type == PSY_INSTANCE_OBJECT) {
+                if (in.data_index < 0 || in.data_index >= ad.n_objects) { err = "object index out of range"; return false; }
+                leaves.push_back(FlatLeaf{obj_base[a] + in.data_index, fr, sh, id});
+            } else if (in.type == PSY_INSTANCE_ASSEMBLY) {
+                if (in.data_index < 0 || in.data_index >= ad.n_assemblies) { err = "sub-assembly index out of range"; return false; }
+                const int sub = ad.assemblies[in.data_index];
+                if (sub <= 0 || sub >= d->n_assemblies) { err = "sub-assembly scene index out of range"; return false; }
+                if (!run(sub, fr, id, sh)) return false;
+            } else {
+                err = "unknown instance type";
+                return false;
+            }
+        }
+        --depth_guard;
+        return true;
+    }
+};
+
+// ---- BVH4 builder: binned-SAH binary tree collapsed to 4-wide --------------
+struct Box {
+    float lo[3] = {INFINITY, INFINITY, INFINITY};
+    float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    void grow(const Box& b) {
+        for (int k = 0; k < 3; ++k) { lo[k] = std::min(lo[k], b.lo[k]); hi[k] = std::max(hi[k], b.hi[k]); }
+    }
+    void growp(const float* p) {
+        for (int k = 0; k < 3; ++k) { lo[k] = std::min(lo[k], p[k]); hi[k] = std::max(hi[k], p[k]); }
+    }
+    float area() const {
+        const float x = hi[0] - lo[0], y = hi[1] - lo[1], z = hi[2] - lo[2];
+        if (!(x >= 0) || !(y >= 0) || !(z >= 0)) return 0.0f;
+        return 2.0f * (x * y + x * z + y * z);
+    }
+};
+
+struct B2 {
+    Box box;
+    int left = -1, right = -1;
+    int leaf = -1;
+};
+
+struct Builder2 {
+    std::vector<B2> nodes;
+    std::vector<Box> boxes;
+    std::vector<float> cent;  // 3 per prim
+    std::vector<int> idx;
+
+    int build(int begin, int end) {
+        const int me = (int)nodes.size();
+        nodes.emplace_back();
+        Box b, cb;
+        for (int i = begin; i < end; ++i) { b.grow(boxes[idx[i]]); cb.growp(&cent[3 * idx[i]]); }
+        nodes[me].box = b;
+        if (end - begin == 1) { nodes[me].leaf = idx[begin]; return me; }
+        int axis = 0;
+        float ext[3];
+        for (int k = 0; k < 3; ++k) ext[k] = cb.hi[k] - cb.lo[k];
+        if (ext[1] > ext[axis]) axis = 1;
+        if (ext[2] > ext[axis]) axis = 2;
+        int mid = (begin + end) / 2;
+        if (ext[axis] > 0.0f) {
+            const int NB = 16;
+            Box bb[NB];
+            int bc[NB] = {0};
+            const float s = (float)NB / ext[axis];
+            auto bin_of = [&](int p) {
+                int k = (int)((cent[3 * p + axis] - cb.lo[axis]) * s);
+                return std::min(NB - 1, std::max(0, k));
+            };
+            for (int i = begin; i < end; ++i) { const int k = bin_of(idx[i]); bb[k].grow(boxes[idx[i]]); bc[k]++; }
+            float best = INFINITY;
+            int best_k = -1;
+            Box lacc[NB];
+            int lcnt[NB];
+            Box acc; int cnt = 0;
+            for (int k = 0; k < NB; ++k) { acc.grow(bb[k]); cnt += bc[k]; lacc[k] = acc; lcnt[k] = cnt; }
+            Box racc; int rcnt = 0;
+            for (int k = NB - 1; k > 0; --k) {
+                racc.grow(bb[k]); rcnt += bc[k];
+                const int lc = lcnt[k - 1];
+                if (lc == 0 || rcnt == 0) continue;
+                const float cost = lacc[k - 1].area() * lc + racc.area() * rcnt;
+                if (cost < best) { best = cost; best_k = k; }
+            }
+            if (best_k > 0) {
+                auto it = std::partition(idx.begin() + begin, idx.begin() + end,
+                                         [&](int p) { return bin_of(p) < best_k; });
+                mid = (int)(it - idx.begin());
+                if (mid == begin || mid == end) mid = (begin + end) / 2;
+            } else {
+                std::nth_element(idx.begin() + begin, idx.begin() + mid, idx.begin() + end,
+                                 [&](int a, int c) { return cent[3 * a + axis] < cent[3 * c + axis]; });
+            }
+        }
+        const int l = build(begin, mid);
+        const int r = build(mid, end);
+        nodes[me].left = l;
+        nodes[me].right = r;
+        return me;
+    }
+};
+
+struct Collapser {
+    const std::vector<B2>* n2;
+    std::vector<BvhNode> out;
+    int max_need = 0;
+
+    // returns child ref, and the traversal stack need of the subtree
+    int32_t emit(int b2, int* need) {
+        const B2& nd = (*n2)[b2];
+        if (nd.leaf >= 0) { *need = 0; return ~nd.leaf; }
+        std::vector<int> kids = {nd.left, nd.right};
+        while (kids.size() < 4) {
+            int best = -1;
+            float ba = -1.0f;
+            for (int k = 0; k < (int)kids.size(); ++k) {
+                const B2& c = (*n2)[kids[k]];
+                if (c.leaf >= 0) continue;
+                const float a = c.box.area();
+                if (a > ba) { ba = a; best = k; }
+            }
+            if (best < 0) break;
+            const int c = kids[best];
+            kids.erase(kids.begin() + best);
+            kids.push_back((*n2)[c].left);
+            kids.push_back((*n2)[c].right);
+        }
+        const int me = (int)out.size();
+        out.emplace_back();
+        int32_t refs[4];
+        int sub_need = 0;
+        for (int k = 0; k < 4; ++k) {
+            if (k < (int)kids.size()) {
+                int cn = 0;
+                refs[k] = emit(kids[k], &cn);
+                sub_need = std::max(sub_need, cn);
+            } else {
+                refs[k] = CHILD_EMPTY;
+            }
+        }
+        BvhNode& o = out[me];
+        for (int k = 0; k < 4; ++k) {
+            o.child[k] = refs[k];
+            if (k < (int)kids.size()) {
+                const Box& b = (*n2)[kids[k]].box;
+                o.lo_x[k] = b.lo[0]; o.hi_x[k] = b.hi[0];
+                o.lo_y[k] = b.lo[1]; o.hi_y[k] = b.hi[1];
+                o.lo_z[k] = b.lo[2]; o.hi_z[k] = b.hi[2];
+            } else {
+                o.lo_x[k] = o.lo_y[k] = o.lo_z[k] = INFINITY;
+                o.hi_x[k] = o.hi_y[k] = o.hi_z[k] = -INFINITY;
+            }
+        }
+        const int kc = (int)kids.size();
+        *need = std::max(kc, kc - 1 + sub_need);
+        return me;
+    }
+};
+
+// exact inverse in double for conservative world bounds
+bool invert_d(const M44& m, double inv[16]) {
+    double a[16];
+    for (int i = 0; i < 16; ++i) a[i] = m.m[i];
+    double r[16];
+    r[0] = a[5] * a[10] * a[15] - a[5] * a[11] * a[14] - a[9] * a[6] * a[15] + a[9] * a[7] * a[14] + a[13] * a[6] * a[11] - a[13] * a[7] * a[10];
+    r[4] = -a[4] * a[10] * a[15] + a[4] * a[11] * a[14] + a[8] * a[6] * a[15] - a[8] * a[7] * a[14] - a[12] * a[6] * a[11] + a[12] * a[7] * a[10];
+    r[8] = a[4] * a[9] * a[15] - a[4] * a[11] * a[13] - a[8] * a[5] * a[15] + a[8] * a[7] * a[13] + a[12] * a[5] * a[11] - a[12] * a[7] * a[9];
+    r[12] = -a[4] * a[9] * a[14] + a[4] * a[10] * a[13] + a[8] * a[5] * a[14] - a[8] * a[6] * a[13] - a[12] * a[5] * a[10] + a[12] * a[6] * a[9];
+    r[1] = -a[1] * a[10] * a[15] + a[1] * a[11] * a[14] + a[9] * a[2] * a[15] - a[9] * a[3] * a[14] - a[13] * a[2] * a[11] + a[13] * a[3] * a[10];
+    r[5] = a[0] * a[10] * a[15] - a[0] * a[11] * a[14] - a[8] * a[2] * a[15] + a[8] * a[3] * a[14] + a[12] * a[2] * a[11] - a[12] * a[3] * a[10];
+    r[9] = -a[0] * a[9] * a[15] + a[0] * a[11] * a[13] + a[8] * a[1] * a[15] - a[8] * a[3] * a[13] - a[12] * a[1] * a[11] + a[12] * a[3] * a[9];
+    r[13] = a[0] * a[9] * a[14] - a[0] * a[10] * a[13] - a[8] * a[1] * a[14] + a[8] * a[2] * a[13] + a[12] * a[1] * a[10] - a[12] * a[2] * a[9];
+    r[2] = a[1] * a[6] * a[15] - a[1] * a[7] * a[14] - a[5] * a[2] * a[15] + a[5] * a[3] * a[14] + a[13] * a[2] * a[7] - a[13] * a[3] * a[6];
+    r[6] = -a[0] * a[6] * a[15] + a[0] * a[7] * a[14] + a[4] * a[2] * a[15] - a[4] * a[3] * a[14] - a[12] * a[2] * a[7] + a[12] * a[3] * a[6];
+    r[10] = a[0] * a[5] * a[15] - a[0] * a[7] * a[13] - a[4] * a[1] * a[15] + a[4] * a[3] * a[13] + a[12] * a[1] * a[7] - a[12] * a[3] * a[5];
+    r[14] = -a[0] * a[5] * a[14] + a[0] * a[6] * a[13] + a[4] * a[1] * a[14] - a[4] * a[2] * a[13] - a[12] * a[1] * a[6] + a[12] * a[2] * a[5];
+    r[3] = -a[1] * a[6] * a[11] + a[1] * a[7] * a[10] + a[5] * a[2] * a[11] - a[5] * a[3] * a[10] - a[9] * a[2] * a[7] + a[9] * a[3] * a[6];
+    r[7] = a[0] * a[6] * a[11] - a[0] * a[7] * a[10] - a[4] * a[2] * a[11] + a[4] * a[3] * a[10] + a[8] * a[2] * a[7] - a[8] * a[3] * a[6];
+    r[11] = -a[0] * a[5] * a[11] + a[0] * a[7] * a[9] + a[4] * a[1] * a[11] - a[4] * a[3] * a[9] - a[8] * a[1] * a[7] + a[8] * a[3] * a[5];
+    r[15] = a[0] * a[5] * a[10] - a[0] * a[6] * a[9] - a[4] * a[1] * a[10] + a[4] * a[2] * a[9] + a[8] * a[1] * a[6] - a[8] * a[2] * a[5];
+    const double det = a[0] * r[0] + a[1] * r[4] + a[2] * r[8] + a[3] * r[12];
+    if (det == 0.0 || !std::isfinite(det)) return false;
+    for (int i = 0; i < 16; ++i) inv[i] = r[i] / det;
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int psy_scene_create(psy_ctx* ctx, const psy_scene_desc* d, int dice_depth, psy_scene** out) {
+    if (!ctx || !d || !out) return fail(PSY_ERR_INVALID, "psy_scene_create: NULL argument");
+    *out = nullptr;
+    if (d->n_assemblies < 1 || !d->assemblies) return fail(PSY_ERR_INVALID, "psy_scene_create: no root assembly");
+    if (d->n_shaders < 0 || (d->n_shaders > 0 && !d->shaders)) return fail(PSY_ERR_INVALID, "psy_scene_create: bad shader table");
+    HIP_OK(hipSetDevice(ctx->device));
+    std::unique_ptr<psy_scene> sc(new psy_scene());
+    sc->ctx = ctx;
+
+    // ---- objects (global numbering: assemblies in order) -------------------
+    Flattener fl;
+    fl.d = d;
+    std::vector<DevObject> objs;
+    std::vector<float> odata;
+    std::vector<Box> obox;  // union over time samples
+    for (int a = 0; a < d->n_assemblies; ++a) {
+        const psy_assembly_desc& ad = d->assemblies[a];
+        fl.obj_base.push_back((int32_t)objs.size());
+        for (int i = 0; i < ad.n_objects; ++i) {
+            const psy_object_desc& od = ad.objects[i];
+            const int nf = floats_per_kind(od.kind);
+            if (nf == 0) return fail(PSY_ERR_INVALID, "psy_scene_create: unknown object kind");
+            if (od.time_samples < 1 || !od.data) return fail(PSY_ERR_INVALID, "psy_scene_create: object without data");
+            if ((od.kind == PSY_OBJ_BILINEAR || od.kind == PSY_OBJ_BICUBIC) && od.time_samples != 1)
+                return fail(PSY_ERR_UNSUPPORTED, "psy_scene_create: deforming patches (time_samples > 1) are not supported on the device path");
+            DevObject o{};
+            o.kind = od.kind;
+            o.ts = od.time_samples;
+            o.data = (int32_t)odata.size();
+            o.dice_levels = 0;
+            o.dice = 0;
+            odata.insert(odata.end(), od.data, od.data + (size_t)nf * od.time_samples);
+            Box b;
+            for (int t = 0; t < od.time_samples; ++t) {
+                const float* p = od.data + (size_t)t * nf;
+                float lo[3], hi[3];
+                switch (od.kind) {
+                    case PSY_OBJ_SPHERE:
+                    case PSY_OBJ_SPHERE_LIGHT:
+                        for (int k = 0; k < 3; ++k) { lo[k] = p[k] - p[3]; hi[k] = p[k] + p[3]; }
+                        break;
+                    case PSY_OBJ_RECTANGLE_LIGHT:
+                        lo[0] = -p[0] * 0.5f; hi[0] = p[0] * 0.5f;
+                        lo[1] = -p[1] * 0.5f; hi[1] = p[1] * 0.5f;
+                        lo[2] = 0.0f; hi[2] = 0.0f;
+                        break;
+                    default: {
+                        const int np = nf / 3;
+                        for (int k = 0; k < 3; ++k) { lo[k] = p[k]; hi[k] = p[k]; }
+                        for (int q = 1; q < np; ++q)
+                            for (int k = 0; k < 3; ++k) { lo[k] = std::min(lo[k], p[3 * q + k]); hi[k] = std::max(hi[k], p[3 * q + k]); }
+                    }
+                }
+                b.growp(lo);
+                b.growp(hi);
+            }
+            obox.push_back(b);
+            objs.push_back(o);
+        }
+    }
+    if (!fl.run(0, -1, 0, -1)) return fail(PSY_ERR_INVALID, "psy_scene_create: " + fl.err);
+
+    // ---- leaf instances, transform pool, world bounds ----------------------
+    std::vector<M44> xf_pool;
+    std::vector<int32_t> frame_off(fl.frames.size());
+    for (size_t f = 0; f < fl.frames.size(); ++f) {
+        frame_off[f] = (int32_t)xf_pool.size();
+        xf_pool.insert(xf_pool.end(), fl.frames[f].begin(), fl.frames[f].end());
+    }
+    const size_t nl = fl.leaves.size();
+    if (nl > (size_t)INT32_MAX / 2) return fail(PSY_ERR_UNSUPPORTED, "psy_scene_create: too many instances");
+    std::vector<DevInstance> insts(nl);
+    Builder2 b2;
+    b2.boxes.resize(nl);
+    b2.cent.resize(3 * nl);
+    b2.idx.resize(nl);
+    for (size_t i = 0; i < nl; ++i) {
+        const FlatLeaf& L = fl.leaves[i];
+        DevInstance& I = insts[i];
+        I.object = L.object;
+        I.kind = objs[L.object].kind;
+        I.shader = L.shader;
+        I.xf_offset = L.frame >= 0 ? frame_off[L.frame] : 0;
+        I.xf_count = L.frame >= 0 ? (int32_t)fl.frames[L.frame].size() : 0;
+        if (I.xf_count > 0 && (I.kind == PSY_OBJ_BILINEAR || I.kind == PSY_OBJ_BICUBIC)) sc->transformed_patches = true;
+        const Box& ob = obox[L.object];
+        Box wb;
+        if (I.xf_count == 0) {
+            wb = ob;
+        } else {
+            const std::vector<M44>& fr = fl.frames[L.frame];
+            const int steps = fr.size() > 1 ? 32 : 0;
+            for (int s = 0; s <= steps; ++s) {
+                const float t = steps ? (float)s / steps : 0.0f;
+                const M44 m = m_lerp_seq(t, fr.data(), (int)fr.size());
+                double inv[16];
+                if (!invert_d(m, inv)) return fail(PSY_ERR_INVALID, "psy_scene_create: singular instance transform");
+                for (int c = 0; c < 8; ++c) {
+                    const double pc[3] = {(c & 1) ? ob.hi[0] : ob.lo[0], (c & 2) ? ob.hi[1] : ob.lo[1],
+                                          (c & 4) ? ob.hi[2] : ob.lo[2]};
+                    double r[4];
+                    for (int j = 0; j < 4; ++j) r[j] = pc[0] * inv[0 * 4 + j] + pc[1] * inv[1 * 4 + j] + pc[2] * inv[2 * 4 + j] + inv[3 * 4 + j];
+                    const float q[3] = {(float)(r[0] / r[3]), (float)(r[1] / r[3]), (float)(r[2] / r[3])};
+                    wb.growp(q);
+                }
+            }
+            float ext = 0.0f;
+            for (int k = 0; k < 3; ++k) ext = std::max(ext, std::max(std::fabs(wb.lo[k]), std::fabs(wb.hi[k])));
+            const float pad = (fr.size() > 1 ? 2e-2f * std::max(ext, 1e-3f) : 1e-5f * ext) + 1e-6f;
+            for (int k = 0; k < 3; ++k) { wb.lo[k] -= pad; wb.hi[k] += pad; }
+        }
+        b2.boxes[i] = wb;
+        for (int k = 0; k < 3; ++k) b2.cent[3 * i + k] = 0.5f * (wb.lo[k] + wb.hi[k]);
+        b2.idx[i] = (int)i;
+        sc->ids.push_back(L.id);
+    }
+    std::vector<BvhNode> bvh;
+    int32_t root = CHILD_EMPTY;
+    int stack_need = 1;
+    if (nl == 1) {
+        root = ~0;
+    } else if (nl > 1) {
+        b2.nodes.reserve(2 * nl);
+        b2.build(0, (int)nl);
+        Collapser cl;
+        cl.n2 = &b2.nodes;
+        int need = 0;
+        root = cl.emit(0, &need);
+        stack_need = std::max(1, need) + 1;
+        bvh.swap(cl.out);
+    }
+    sc->stack_need = stack_need;
+
+    // ---- dice cache --------------------------------------------------------
+    std::vector<int32_t> patch_ids;
+    for (size_t i = 0; i < objs.size(); ++i)
+        if (objs[i].kind == PSY_OBJ_BILINEAR || objs[i].kind == PSY_OBJ_BICUBIC) patch_ids.push_back((int32_t)i);
+    int levels = dice_depth;
+    if (levels < 0) {
+        const double budget = 1024.0 * 1024.0 * 1024.0;  // 1 GiB of split-tree nodes
+        levels = 14;
+        while (levels > 0 && (double)(((int64_t)1 << (levels + 1)) - 1) * 32.0 * (double)patch_ids.size() > budget) --levels;
+    }
+    if (levels > 20) return fail(PSY_ERR_INVALID, "psy_scene_create: dice_depth above 20");
+    const int64_t per = ((int64_t)1 << (levels + 1)) - 1;
+    for (size_t k = 0; k < patch_ids.size(); ++k) {
+        objs[patch_ids[k]].dice_levels = levels;
+        objs[patch_ids[k]].dice = (int64_t)k * per;
+    }
+
+    // ---- upload ------------------------------------------------------------
+    int rc;
+    if ((rc = sc->bvh.upload(bvh.data(), bvh.size() * sizeof(BvhNode)))) return rc;
+    if ((rc = sc->inst.upload(insts.data(), insts.size() * sizeof(DevInstance)))) return rc;
+    if ((rc = sc->obj.upload(objs.data(), objs.size() * sizeof(DevObject)))) return rc;
+    if ((rc = sc->odata.upload(odata.data(), odata.size() * sizeof(float)))) return rc;
+    if ((rc = sc->xf.upload(xf_pool.data(), xf_pool.size() * sizeof(M44)))) return rc;
+    std::vector<DevShader> sh(d->n_shaders);
+    for (int i = 0; i < d->n_shaders; ++i) {
+        sh[i].kind = d->shaders[i].kind;
+        for (int k = 0; k < 3; ++k) sh[i].color[k] = d->shaders[i].color[k];
+    }
+    if ((rc = sc->shaders.upload(sh.data(), sh.size() * sizeof(DevShader)))) return rc;
+    const size_t dice_bytes = (size_t)per * patch_ids.size() * 32;
+    if ((rc = sc->dice.alloc(dice_bytes))) return rc;
+    if (!patch_ids.empty()) {
+        DBuf pid;
+        if ((rc = pid.upload(patch_ids.data(), patch_ids.size() * sizeof(int32_t)))) return rc;
+        rc = launch_build_dice((const float*)sc->odata.p, (const DevObject*)sc->obj.p, (const int32_t*)pid.p,
+                               (int)patch_ids.size(), levels, (float4*)sc->dice.p, nullptr);
+        if (rc) return fail(rc, "psy_scene_create: dice cache build launch failed");
+        HIP_OK(hipDeviceSynchronize());
+    }
+
+    DevScene& s = sc->dev;
+    s.bvh = (const BvhNode*)sc->bvh.p;
+    s.n_bvh = (int32_t)bvh.size();
+    s.root = root;
+    s.inst = (const DevInstance*)sc->inst.p;
+    s.obj = (const DevObject*)sc->obj.p;
+    s.odata = (const float*)sc->odata.p;
+    s.xf = (const M44*)sc->xf.p;
+    s.dice = (const float4*)sc->dice.p;
+    s.shaders = (const DevShader*)sc->shaders.p;
+    s.dice_rate = d->dice_rate;
+    s.min_upoly = d->min_upoly_size;
+    s.n_inst = (int32_t)nl;
+
+    // ---- camera (basics/camera.hpp:23-69) ----------------------------------
+    const psy_camera_desc& c = d->camera;
+    DevCamera& cam = sc->cam;
+    if (c.n_transforms > MAX_CAM_SAMPLES || c.n_fovs > MAX_CAM_SAMPLES || c.n_aperture_radii > MAX_CAM_SAMPLES ||
+        c.n_focus_distances > MAX_CAM_SAMPLES)
+        return fail(PSY_ERR_UNSUPPORTED, "psy_scene_create: more than 8 camera motion samples");
+    cam.n_xf = std::max(1, (int)c.n_transforms);
+    if (c.n_transforms == 0) cam.xf[0] = m_identity();
+    for (int i = 0; i < c.n_transforms; ++i) cam.xf[i] = load_m44(c.transforms + 16 * i);
+    cam.n_tfov = std::max(1, (int)c.n_fovs);
+    cam.tfov[0] = 1.0f;
+    for (int i = 0; i < c.n_fovs; ++i) {
+        const float f = c.fovs[i];
+        cam.tfov[i] = (float)(std::sin((double)(f / 2)) / std::cos((double)(f / 2)));
+    }
+    bool focal_ok = c.n_aperture_radii > 0 && c.n_focus_distances > 0;
+    for (int i = 0; focal_ok && i < c.n_focus_distances; ++i) if (c.focus_distances[i] <= 0.0f) focal_ok = false;
+    if (focal_ok) {
+        cam.n_ap = c.n_aperture_radii;
+        for (int i = 0; i < cam.n_ap; ++i) cam.aperture[i] = c.aperture_radii[i];
+        cam.n_fd = c.n_focus_distances;
+        for (int i = 0; i < cam.n_fd; ++i) cam.focus[i] = c.focus_distances[i];
+    } else {
+        cam.n_ap = 1; cam.aperture[0] = 0.0f;
+        cam.n_fd = 1; cam.focus[0] = 1.0f;
+    }
+
+    psy_scene_info& info = sc->info;
+    info.n_leaf_instances = (int64_t)nl;
+    info.n_objects = (int64_t)objs.size();
+    info.n_bvh_nodes = (int64_t)bvh.size();
+    info.n_patches = (int64_t)patch_ids.size();
+    info.dice_depth = levels;
+    info.bvh_stack_need = stack_need;
+    info.dice_cache_bytes = (int64_t)dice_bytes;
+    info.device_bytes = (int64_t)(sc->bvh.bytes + sc->inst.bytes + sc->obj.bytes + sc->odata.bytes + sc->xf.bytes +
+                                  sc->shaders.bytes + dice_bytes);
+    *out = sc.release();
+    return PSY_OK;
+}
+
+int psy_scene_destroy(psy_scene* s) {
+    if (!s) return PSY_OK;
+    (void)hipSetDevice(s->ctx->device);
+    delete s;
+    return PSY_OK;
+}
+
+int psy_scene_get_info(const psy_scene* s, psy_scene_info* info) {
+    if (!s || !info) return fail(PSY_ERR_INVALID, "psy_scene_get_info: NULL argument");
+    *info = s->info;
+    return PSY_OK;
+}
+
+int psy_scene_instance_ids(const psy_scene* s, uint64_t* out, int64_t n) {
+    if (!s || !out) return fail(PSY_ERR_INVALID, "psy_scene_instance_ids: NULL argument");
+    if (n < (int64_t)s->ids.size()) return fail(PSY_ERR_INVALID, "psy_scene_instance_ids: output too small");
+    std::memcpy(out, s->ids.data(), s->ids.size() * sizeof(uint64_t));
+    return PSY_OK;
+}
+
+static int check_hits(const psy_hits* h, int64_t n) {
+    if (!h) return fail(PSY_ERR_INVALID, "hit buffer struct is NULL");
+    if (n > 0 && (!h->t || !h->inst || !h->u || !h->v))
+        return fail(PSY_ERR_INVALID, "hit buffers must be non-NULL (aux may be NULL)");
+    return PSY_OK;
+}
+
+static DevHits to_dev(const psy_hits* h) { return DevHits{h->t, h->inst, h->u, h->v, h->aux}; }
+
+int psy_trace_world(psy_ctx* ctx, const psy_scene* s, const float* rays, int64_t n, const psy_hits* hits,
+                    uint32_t flags, void* stream) {
+    if (!ctx || !s || (!rays && n > 0)) return fail(PSY_ERR_INVALID, "psy_trace_world: NULL argument");
+    if (n < 0) return fail(PSY_ERR_INVALID, "psy_trace_world: negative ray count");
+    if (int rc = check_hits(hits, n)) return rc;
+    HIP_OK(hipSetDevice(ctx->device));
+    int rc = launch_trace(s->dev, true, (flags & PSY_TRACE_OCCLUSION_ONLY) != 0, s->stack_need, rays, n, to_dev(hits),
+                          (hipStream_t)stream);
+    return rc ? fail(rc, "psy_trace_world: launch failed") : PSY_OK;
+}
+
+int psy_trace_rays(psy_ctx* ctx, const psy_scene* s, const float* rays, int64_t n, const psy_hits* hits,
+                   uint32_t flags, void* stream) {
+    if (!ctx || !s || (!rays && n > 0)) return fail(PSY_ERR_INVALID, "psy_trace_rays: NULL argument");
+    if (n < 0) return fail(PSY_ERR_INVALID, "psy_trace_rays: negative ray count");
+    if (s->transformed_patches)
+        return fail(PSY_ERR_UNSUPPORTED, "psy_trace_rays: scene has transformed patch instances; their footprint needs world rays (psy_trace_world)");
+    if (int rc = check_hits(hits, n)) return rc;
+    HIP_OK(hipSetDevice(ctx->device));
+    int rc = launch_trace(s->dev, false, (flags & PSY_TRACE_OCCLUSION_ONLY) != 0, s->stack_need, rays, n,
+                          to_dev(hits), (hipStream_t)stream);
+    return rc ? fail(rc, "psy_trace_rays: launch failed") : PSY_OK;
+}
+
+int psy_world_to_rays(psy_ctx* ctx, const float* wrays, int64_t n, float* rays, void* stream) {
+    if (!ctx || ((!wrays || !rays) && n > 0) || n < 0) return fail(PSY_ERR_INVALID, "psy_world_to_rays: bad argument");
+    HIP_OK(hipSetDevice(ctx->device));
+    int rc = launch_world_to_rays(wrays, n, rays, (hipStream_t)stream);
+    return rc ? fail(rc, "psy_world_to_rays: launch failed") : PSY_OK;
+}
+
+int psy_halton(psy_ctx* ctx, uint32_t dim, const uint32_t* index, int64_t n, float* out, void* stream) {
+    if (!ctx || ((!index || !out) && n > 0) || n < 0) return fail(PSY_ERR_INVALID, "psy_halton: bad argument");
+    HIP_OK(hipSetDevice(ctx->device));
+    int rc = launch_halton((const HaltonDim*)ctx->halton_dims.p, (const uint16_t*)ctx->halton_perm.p, dim, index, n,
+                           out, (hipStream_t)stream);
+    return rc ? fail(rc, "psy_halton: launch failed") : PSY_OK;
+}
+
+int psy_pixel_samples(psy_ctx* ctx, int32_t x0, int32_t y0, int32_t w, int32_t h, int32_t spp, uint32_t seed,
+                      uint32_t dim0, int32_t n_dims, float* out, void* stream) {
+    if (!ctx || w < 0 || h < 0 || spp < 1 || n_dims < 0 || (!out && (int64_t)w * h * n_dims > 0))
+        return fail(PSY_ERR_INVALID, "psy_pixel_samples: bad argument");
+    HIP_OK(hipSetDevice(ctx->device));
+    int rc = launch_pixel_samples((const HaltonDim*)ctx->halton_dims.p, (const uint16_t*)ctx->halton_perm.p, x0, y0, w,
+                                  h, spp, seed, dim0, n_dims, out, (hipStream_t)stream);
+    return rc ? fail(rc, "psy_pixel_samples: launch failed") : PSY_OK;
+}
+
+int psy_camera_rays(psy_ctx* ctx, const psy_scene* s, int32_t width, int32_t height, int32_t spp, uint32_t seed,
+                    float* wrays, float* wavelength, void* stream) {
+    if (!ctx || !s || width <= 0 || height <= 0 || spp <= 0 || !wrays) return fail(PSY_ERR_INVALID, "psy_camera_rays: bad argument");
+    if (width > 32767 || height > 32767) return fail(PSY_ERR_INVALID, "psy_camera_rays: resolution above 32767");
+    HIP_OK(hipSetDevice(ctx->device));
+    int rc = launch_camera_rays(s->cam, (const HaltonDim*)ctx->halton_dims.p, (const uint16_t*)ctx->halton_perm.p, width,
+                                height, spp, seed, wrays, wavelength, (hipStream_t)stream);
+    return rc ? fail(rc, "psy_camera_rays: launch failed") : PSY_OK;
+}
+
+int psy_bounce_rays(psy_ctx* ctx, const psy_scene* s, const float* in_wrays, const psy_hits* in_hits, int64_t n,
+                    int32_t width, int32_t spp, uint32_t seed, uint32_t dim_u, uint32_t dim_v, float* out_wrays,
+                    uint32_t* valid, void* stream) {
+    if (!ctx || !s || n < 0 || width <= 0 || spp <= 0 || ((!in_wrays || !out_wrays || !valid) && n > 0))
+        return fail(PSY_ERR_INVALID, "psy_bounce_rays: bad argument");
+    if (int rc = check_hits(in_hits, n)) return rc;
+    HIP_OK(hipSetDevice(ctx->device));
+    int rc = launch_bounce_rays(s->dev, (const HaltonDim*)ctx->halton_dims.p, (const uint16_t*)ctx->halton_perm.p,
+                                in_wrays, to_dev(in_hits), n, width, spp, seed, dim_u, dim_v, out_wrays, valid,
+                                (hipStream_t)stream);
+    return rc ? fail(rc, "psy_bounce_rays: launch failed") : PSY_OK;
+}
+
+}  // extern "C"

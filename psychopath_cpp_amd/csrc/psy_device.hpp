@@ -1,0 +1,112 @@
+// psy_device.hpp -- HBM layout of an uploaded scene and the launcher
+// interface between the host library and the HIP kernels.
+//
+// Layout (all arrays in device memory, read-only after psy_scene_create):
+//   bvh    : BVH4 over flattened leaf instances, 128-byte nodes; child
+//            boxes stored per axis as 4-wide lanes (the reference BBox4
+//            layout, basics/bbox.hpp:461-476), 16 bytes of child refs,
+//            16 bytes of padding (one node = one 128-byte L2 line).
+//   inst   : 32-byte leaf-instance records (object, merged transforms,
+//            effective shader) -- the assembly hierarchy flattened.
+//   obj    : 32-byte object records; control points / centres in `odata`.
+//   xf     : merged per-leaf transform lists (world -> object), 64 B each.
+//   dice   : the patch dice cache: for every patch a complete heap-ordered
+//            split tree of `dice_levels`+1 levels, 32 bytes per node:
+//            float4(min.xyz, u_center), float4(max.xyz, v_center).
+#pragma once
+
+#include <stdint.h>
+#include "psy_math.hpp"
+
+namespace psy {
+
+enum : int32_t {
+    K_SPHERE = 1, K_BILINEAR = 2, K_BICUBIC = 3, K_SPHERE_LIGHT = 4, K_RECT_LIGHT = 5
+};
+
+constexpr int32_t CHILD_EMPTY = INT32_MIN;  // unused BVH4 lane
+// child ref: >= 0 inner node index; < 0 and != CHILD_EMPTY: leaf ~ref
+
+struct alignas(16) BvhNode {
+    float lo_x[4], hi_x[4], lo_y[4], hi_y[4], lo_z[4], hi_z[4];
+    int32_t child[4];
+    int32_t pad[4];  // keeps nodes 128-byte aligned
+};
+static_assert(sizeof(BvhNode) == 128, "BVH node must be 128 bytes");
+
+struct alignas(16) DevInstance {
+    int32_t object;
+    int32_t kind;
+    int32_t xf_offset;
+    int32_t xf_count;
+    int32_t shader;     // effective shader index, -1 = none
+    int32_t pad[3];
+};
+static_assert(sizeof(DevInstance) == 32, "instance record must be 32 bytes");
+
+struct alignas(16) DevObject {
+    int32_t kind;
+    int32_t ts;
+    int32_t data;         // float offset into odata
+    int32_t dice_levels;  // cached split depth (patches)
+    int64_t dice;         // first node (in 32-byte units) in the dice pool
+    int32_t pad[2];
+};
+static_assert(sizeof(DevObject) == 32, "object record must be 32 bytes");
+
+struct DevShader { int32_t kind; float color[3]; };
+
+constexpr int MAX_CAM_SAMPLES = 8;
+struct DevCamera {
+    M44 xf[MAX_CAM_SAMPLES]; int32_t n_xf;
+    float tfov[MAX_CAM_SAMPLES]; int32_t n_tfov;
+    float aperture[MAX_CAM_SAMPLES]; int32_t n_ap;
+    float focus[MAX_CAM_SAMPLES]; int32_t n_fd;
+};
+
+// Everything a kernel needs to trace against a scene (passed by value).
+struct DevScene {
+    const BvhNode* bvh;
+    int32_t n_bvh;
+    int32_t root;             // child ref of the root (node 0, or a leaf)
+    const DevInstance* inst;
+    const DevObject* obj;
+    const float* odata;
+    const M44* xf;
+    const float4* dice;
+    const DevShader* shaders;
+    float dice_rate;
+    float min_upoly;
+    int32_t n_inst;
+};
+
+// Halton tables (dimension d >= 1): perm table offset, pow_base, terms.
+struct HaltonDim {
+    uint32_t perm_offset, pow_base, n_terms, pad;
+    uint32_t div[8];
+    uint32_t power[8];
+    double scale;
+    double pad2;
+};
+
+struct DevHits { float* t; uint32_t* inst; float* u; float* v; float* aux; };
+
+// ---- launchers (psy_kernels.hip) -------------------------------------------
+int launch_build_dice(const float* odata, const DevObject* objs, const int32_t* patch_ids,
+                      int n_patches, int levels, float4* dice, hipStream_t st);
+int launch_trace(const DevScene& s, bool world, bool occl_only, int stack_need, const float* rays,
+                 int64_t n, const DevHits& h, hipStream_t st);
+int launch_world_to_rays(const float* wrays, int64_t n, float* rays, hipStream_t st);
+int launch_halton(const HaltonDim* dims, const uint16_t* perm, uint32_t dim, const uint32_t* idx,
+                  int64_t n, float* out, hipStream_t st);
+int launch_pixel_samples(const HaltonDim* dims, const uint16_t* perm, int x0, int y0, int w, int h,
+                         int spp, uint32_t seed, uint32_t dim0, int n_dims, float* out, hipStream_t st);
+int launch_camera_rays(const DevCamera& cam, const HaltonDim* dims, const uint16_t* perm, int width,
+                       int height, int spp, uint32_t seed, float* wrays, float* wavelength,
+                       hipStream_t st);
+int launch_bounce_rays(const DevScene& s, const HaltonDim* dims, const uint16_t* perm,
+                       const float* in_wrays, const DevHits& hits, int64_t n, int width, int spp,
+                       uint32_t seed, uint32_t dim_u, uint32_t dim_v, float* out_wrays,
+                       uint32_t* valid, hipStream_t st);
+
+}  // namespace psy

@@ -1,0 +1,814 @@
+// psy_kernels.hip -- HIP kernels for gfx950 (MI355X).
+//
+// The reference traces breadth-first: it partitions a ray stream at every
+// BVH4 node and every patch split (tracer/tracer.cpp:38-313,
+// accel/bvh4.cpp:163-236, object/patch_utils.hpp:19-200).  On CDNA4 the
+// same closest hit is found with one lane per ray: the lane walks the BVH4
+// nearest-first with a short stack in LDS, and walks each patch's split
+// tree through the precomputed dice cache without any per-ray stack (heap
+// order: the next node is found from the current index alone).  A patch
+// hit's t is the midpoint of a micro-bbox slab interval, which does not
+// depend on the ray's max_t, so the result equals the reference's for any
+// visit order except exact ties.
+//
+// Arithmetic follows the reference's operation order; this file must be
+// compiled with -ffp-contract=off and IEEE division / square root.
+#include <hip/hip_runtime.h>
+#include "psy_device.hpp"
+#include "psy_patch.hpp"
+#include "../../include/psy_gpu.h"
+
+namespace psy {
+
+// ---------------------------------------------------------------------------
+// Halton + Sampler (sampling/halton.cpp, sampling/image_sampler.hpp:16-38,
+// utils/hash.hpp:6-16)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float halton_sample(const HaltonDim* dims, const uint16_t* perm, uint32_t dim,
+                                               uint32_t index) {
+    if (dim == 0) {
+        const uint32_t r = __builtin_bitreverse32(index);
+        return (float)r * (1.0f / 4294967296.0f);
+    }
+    if (dim >= 256) return 0.0f;
+    const HaltonDim& h = dims[dim];
+    uint32_t sum = 0;
+    for (uint32_t t = 0; t < h.n_terms; ++t)
+        sum += (uint32_t)perm[h.perm_offset + (index / h.div[t]) % h.pow_base] * h.power[t];
+    return (float)((double)sum * h.scale);
+}
+
+__device__ __forceinline__ uint32_t hash_u32(uint32_t n, uint32_t seed) {
+    uint32_t hash = n;
+    for (int i = 0; i < 3; ++i) {
+        hash *= 1936502639u;
+        hash ^= hash >> 16;
+        hash += seed;
+    }
+    return hash;
+}
+
+__device__ __forceinline__ uint32_t sampler_offset(uint32_t x, uint32_t y, uint32_t n, uint32_t seed) {
+    return hash_u32(x ^ ((y >> 16) | (y << 16)), seed) + n;
+}
+
+__device__ __forceinline__ float sampler_get(const HaltonDim* dims, const uint16_t* perm, uint32_t off,
+                                             uint32_t dim) {
+    // dimension reorder (sampling/image_sampler.hpp:26)
+    uint32_t d = dim;
+    if (dim < 11) {
+        const uint8_t tbl[11] = {10, 7, 6, 5, 4, 2, 9, 8, 3, 1, 0};
+        d = tbl[dim];
+    }
+    return halton_sample(dims, perm, d, off);
+}
+
+__global__ void k_halton(const HaltonDim* dims, const uint16_t* perm, uint32_t dim, const uint32_t* idx,
+                         int64_t n, float* out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = halton_sample(dims, perm, dim, idx[i]);
+}
+
+__global__ void k_pixel_samples(const HaltonDim* dims, const uint16_t* perm, int x0, int y0, int w, int h,
+                                int spp, uint32_t seed, uint32_t dim0, int n_dims, float* out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t total = (int64_t)w * h * spp;
+    if (i >= total) return;
+    const int s = (int)(i % spp);
+    const int64_t pix = i / spp;
+    const int px = x0 + (int)(pix % w);
+    const int py = y0 + (int)(pix / w);
+    const uint32_t off = sampler_offset((uint32_t)px, (uint32_t)py, (uint32_t)s, seed);
+    for (int k = 0; k < n_dims; ++k) out[i * n_dims + k] = sampler_get(dims, perm, off, dim0 + (uint32_t)k);
+}
+
+// ---------------------------------------------------------------------------
+// Dice cache builder: one lane per split-tree node, replaying the splits
+// from the patch root (object/patch_utils.hpp:38-46, :146-192).
+// ---------------------------------------------------------------------------
+template <int N>
+__device__ __forceinline__ void load_patch(const float* odata, int32_t off, Patch<N>& p) {
+    for (int i = 0; i < N; ++i) p.p[i] = mk(odata[off + 3 * i], odata[off + 3 * i + 1], odata[off + 3 * i + 2]);
+}
+
+template <int N>
+__device__ void build_node(const float* odata, const DevObject& ob, uint32_t node, float4* out) {
+    Patch<N> p;
+    load_patch<N>(odata, ob.data, p);
+    float uv[4];
+    patch_at_node<N>(p, node, uv);
+    V3 lo, hi;
+    patch_bound<N>(p, &lo, &hi);
+    out[0] = make_float4(lo.x, lo.y, lo.z, (uv[0] + uv[1]) * 0.5f);
+    out[1] = make_float4(hi.x, hi.y, hi.z, (uv[2] + uv[3]) * 0.5f);
+}
+
+__global__ void k_build_dice(const float* odata, const DevObject* objs, const int32_t* patch_ids, int n_patches,
+                             int levels, float4* dice) {
+    const int64_t per = ((int64_t)1 << (levels + 1)) - 1;
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= per * n_patches) return;
+    const int pi = (int)(gid / per);
+    const uint32_t node = (uint32_t)(gid % per);
+    const DevObject ob = objs[patch_ids[pi]];
+    float4* out = dice + 2 * (ob.dice + node);
+    if (ob.kind == K_BILINEAR) build_node<4>(odata, ob, node, out);
+    else build_node<16>(odata, ob, node, out);
+}
+
+// ---------------------------------------------------------------------------
+// Traversal
+// ---------------------------------------------------------------------------
+struct Best {
+    float t;
+    uint32_t inst;
+    float u, v, aux;
+};
+
+// Split-dice below the cached depth: depth-first over the subtree of `node`
+// (already found to need splitting), replaying splits from that node's
+// patch for every visited node.  Same leaf rule as the cached walk.
+template <int N>
+__device__ __noinline__ bool dice_deep(const DevScene& s, const DevObject& ob, uint32_t node, int sidx0,
+                                       uint32_t inst, V3 o, V3 dinv, RayW w, bool occl, Best& b) {
+    Patch<N> base;
+    load_patch<N>(s.odata, ob.data, base);
+    float uvb[4];
+    patch_at_node<N>(base, node, uvb);
+    int k = 1;
+    uint64_t pb = 1;
+    int sidx = sidx0 + 1;
+    for (;;) {
+        Patch<N> p = base;
+        float uv[4] = {uvb[0], uvb[1], uvb[2], uvb[3]};
+        for (int l = k - 1; l >= 0; --l) patch_step<N>(p, ((pb >> l) & 1ull) != 0, uv);
+        V3 lo, hi;
+        patch_bound<N>(p, &lo, &hi);
+        float h0, h1;
+        bool down = false;
+        if (bbox_ray(lo, hi, o, dinv, b.t, &h0, &h1)) {
+            const float width = smax(min_width(w, h0, h1) * s.dice_rate, s.min_upoly);
+            const float max_dim = longest_axis(sub(hi, lo));
+            if (max_dim <= width || sidx == 63 || k == 62) {
+                const float tt = (h0 + h1) * 0.5f;
+                if (tt > 0.0f && tt < b.t) {
+                    b.t = tt;
+                    b.inst = inst;
+                    b.u = (uv[0] + uv[1]) * 0.5f;
+                    b.v = (uv[2] + uv[3]) * 0.5f;
+                    b.aux = max_dim;
+                    if (occl) return true;
+                }
+            } else {
+                down = true;
+            }
+        }
+        if (down) { ++k; pb = (pb << 1) | 1ull; ++sidx; continue; }
+        for (;;) {
+            if (k == 0) return false;
+            if (pb & 1ull) { pb ^= 1ull; --sidx; break; }
+            pb >>= 1; --k;
+        }
+        if (k == 0) return false;
+    }
+}
+
+// Walk one patch's cached split tree (object/patch_utils.hpp:36-197 per ray).
+__device__ __forceinline__ bool dice_walk(const DevScene& s, const DevObject& ob, uint32_t inst, V3 o, V3 dinv,
+                                          const RayW& w, bool occl, Best& b) {
+    const float4* nodes = s.dice + 2 * ob.dice;
+    const int levels = ob.dice_levels;
+    uint32_t node = 0;
+    int depth = 0;
+    int sidx = 0;
+    for (;;) {
+        const float4 a = nodes[2 * node];
+        const float4 c = nodes[2 * node + 1];
+        const V3 lo = mk(a.x, a.y, a.z), hi = mk(c.x, c.y, c.z);
+        float h0, h1;
+        bool down = false;
+        if (bbox_ray(lo, hi, o, dinv, b.t, &h0, &h1)) {
+            const float width = smax(min_width(w, h0, h1) * s.dice_rate, s.min_upoly);
+            const float max_dim = longest_axis(sub(hi, lo));
+            if (max_dim <= width || sidx == 63) {
+                const float tt = (h0 + h1) * 0.5f;
+                if (tt > 0.0f && tt < b.t) {
+                    b.t = tt;
+                    b.inst = inst;
+                    b.u = a.w;
+                    b.v = c.w;
+                    b.aux = max_dim;
+                    if (occl) return true;
+                }
+            } else if (depth < levels) {
+                down = true;
+            } else {
+                const bool done = ob.kind == K_BILINEAR
+                                      ? dice_deep<4>(s, ob, node, sidx, inst, o, dinv, w, occl, b)
+                                      : dice_deep<16>(s, ob, node, sidx, inst, o, dinv, w, occl, b);
+                if (done) return true;
+            }
+        }
+        if (down) { node = 2 * node + 2; ++depth; ++sidx; continue; }
+        for (;;) {
+            if (node == 0) return false;
+            if ((node & 1u) == 0) { node -= 1; --sidx; break; }
+            node = (node - 1) >> 1;
+            --depth;
+        }
+    }
+}
+
+// Sphere::intersect_ray / SphereLight::intersect_ray quadratic
+// (object/sphere.cpp:67-140)
+__device__ __forceinline__ bool sphere_t(V3 ro, V3 rd, float max_t, V3 cent, float radi, float* t_out) {
+    const V3 o = sub(ro, cent);
+    const float a = dot(rd, rd);
+    const float bq = 2.0f * dot(rd, o);
+    const float c = dot(o, o) - radi * radi;
+    float disc = bq * bq - 4.0f * a * c;
+    if (disc < 0.0f) return false;
+    disc = sqrtf(disc);
+    float q;
+    if (bq < 0.0f) q = -0.5f * (bq - disc);
+    else q = -0.5f * (bq + disc);
+    float t0 = q / a, t1;
+    if (q != 0.0f) t1 = c / q; else t1 = max_t;
+    if (t0 > t1) { const float tmp = t1; t1 = t0; t0 = tmp; }
+    if (t0 >= max_t || t1 < 0.0001f) return false;
+    if (t0 >= 0.0001f) *t_out = t0;
+    else if (t1 < max_t) *t_out = t1;
+    else return false;
+    return true;
+}
+
+__device__ __forceinline__ void sphere_at(const float* d, int ts, int stride, float time, V3* cent, float* radi) {
+    float na = 0.0f;
+    const int i = lerp_seq_index(time, ts, &na);
+    if (i == -1) { *cent = mk(d[0], d[1], d[2]); *radi = d[3]; return; }
+    if (i == -2) {
+        const float* e = d + (ts - 1) * stride;
+        *cent = mk(e[0], e[1], e[2]); *radi = e[3]; return;
+    }
+    const float* a = d + i * stride;
+    const float* b = d + (i + 1) * stride;
+    *cent = lerpv(na, mk(a[0], a[1], a[2]), mk(b[0], b[1], b[2]));
+    *radi = lerpf(na, a[3], b[3]);
+}
+
+// RectangleLight::intersect_ray (lights/rectangle_light.hpp:110-147)
+__device__ __forceinline__ bool rect_t(const float* d, int ts, float time, V3 o, V3 dir, V3 dinv, float max_t,
+                                       float* t_out) {
+    if (dir.z == 0.0f) return false;
+    float wd, ht;
+    float na = 0.0f;
+    const int i = lerp_seq_index(time, ts, &na);
+    if (i == -1 || i == -2) {
+        const float* e = d + (i == -1 ? 0 : (ts - 1) * 5);
+        wd = e[0]; ht = e[1];
+    } else {
+        const float f = lerp_fac(na);
+        wd = (d[i * 5] * f) + (d[(i + 1) * 5] * na);
+        ht = (d[i * 5 + 1] * f) + (d[(i + 1) * 5 + 1] * na);
+    }
+    const float t = -o.z * dinv.z;
+    if (t <= 0.0f || t > max_t) return false;
+    const float x = o.x + (dir.x * t);
+    const float y = o.y + (dir.y * t);
+    // the y test compares against the width, as the reference does
+    if (x >= (wd * -0.5f) && x <= (wd * 0.5f) && y >= (ht * -0.5f) && y <= (wd * 0.5f)) {
+        *t_out = t;
+        return true;
+    }
+    return false;
+}
+
+struct TRay {
+    V3 o, d, dinv;
+    V3 odx, ody, ddx, ddy;  // world differentials (world-ray form only)
+    RayW w;                 // footprint in world space
+    float time;
+    bool occl;
+};
+
+template <bool WORLD>
+__device__ __forceinline__ bool intersect_leaf(const DevScene& s, uint32_t li, const TRay& r, Best& b) {
+    const DevInstance I = s.inst[li];
+    const DevObject ob = s.obj[I.object];
+    V3 o = r.o, d = r.d, dinv = r.dinv;
+    RayW w = r.w;
+    if (I.xf_count > 0) {
+        const M44 x = m_lerp_seq(r.time, s.xf + I.xf_offset, I.xf_count);
+        o = m_pos(x, r.o);
+        d = m_dir(x, r.d);
+        dinv = divv(splat(1.0f), d);
+        if (WORLD && (ob.kind == K_BILINEAR || ob.kind == K_BICUBIC))
+            w = ray_widths(o, d, m_dir(x, r.odx), m_dir(x, r.ody), m_dir(x, r.ddx), m_dir(x, r.ddy));
+    }
+    const float* od = s.odata + ob.data;
+    float t;
+    switch (ob.kind) {
+        case K_BILINEAR:
+        case K_BICUBIC:
+            return dice_walk(s, ob, li, o, dinv, w, r.occl, b);
+        case K_SPHERE: {
+            V3 c; float rad;
+            sphere_at(od, ob.ts, 4, r.time, &c, &rad);
+            if (sphere_t(o, d, b.t, c, rad, &t)) {
+                b.t = t; b.inst = li; b.aux = 0.0f;
+                b.u = 0.0f; b.v = 0.0f;  // resolved later from the hit point
+                return r.occl;
+            }
+            return false;
+        }
+        case K_SPHERE_LIGHT: {
+            V3 c; float rad;
+            sphere_at(od, ob.ts, 7, r.time, &c, &rad);
+            if (sphere_t(o, d, b.t, c, rad, &t)) {
+                b.t = t; b.inst = li; b.aux = 0.0f; b.u = 0.0f; b.v = 0.0f;
+                return r.occl;
+            }
+            return false;
+        }
+        case K_RECT_LIGHT:
+            if (rect_t(od, ob.ts, r.time, o, d, dinv, b.t, &t)) {
+                b.t = t; b.inst = li; b.aux = 0.0f; b.u = 0.0f; b.v = 0.0f;
+                return r.occl;
+            }
+            return false;
+        default:
+            return false;
+    }
+}
+
+// Latitude / longitude parameters of a sphere hit, computed once for the
+// final hit (object/sphere.cpp:131-150).
+__device__ __noinline__ void sphere_uv(const DevScene& s, Best& b, const TRay& r) {
+    const DevInstance I = s.inst[b.inst];
+    if (I.kind != K_SPHERE) return;
+    const DevObject ob = s.obj[I.object];
+    V3 o = r.o, d = r.d;
+    if (I.xf_count > 0) {
+        const M44 x = m_lerp_seq(r.time, s.xf + I.xf_offset, I.xf_count);
+        o = m_pos(x, r.o);
+        d = m_dir(x, r.d);
+    }
+    V3 c; float rad;
+    sphere_at(s.odata + ob.data, ob.ts, 4, r.time, &c, &rad);
+    const V3 p = add(o, scale(d, b.t));
+    const V3 unit_p = normalized(sub(p, c));
+    const float lat_sin = sqrtf((unit_p.x * unit_p.x) + (unit_p.y * unit_p.y));
+    const float long_cos = unit_p.x / lat_sin;
+    const float latitude = acosf(unit_p.z);
+    float longitude = 0.0f;
+    if (unit_p.x != 0.0f || unit_p.y != 0.0f) {
+        longitude = acosf(long_cos);
+        if (unit_p.y < 0.0f) longitude = (float)((2.0 * 3.1415926535897932384626433) - (double)longitude);
+    }
+    b.u = longitude / (float)(3.1415926535897932384626433 * 2);
+    b.v = (float)((double)latitude / 3.1415926535897932384626433);
+}
+
+constexpr int TRACE_BLOCK = 128;
+
+template <bool WORLD, int STACK>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DevScene s, const float* __restrict__ rays, int64_t n,
+                                                        DevHits h, uint32_t flags) {
+    __shared__ int32_t stk_ref[STACK][TRACE_BLOCK];
+    __shared__ float stk_near[STACK][TRACE_BLOCK];
+    const int tid = threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * TRACE_BLOCK + tid;
+    if (i >= n) return;
+
+    TRay r;
+    float max_t;
+    if (WORLD) {
+        r.o = mk(rays[0 * n + i], rays[1 * n + i], rays[2 * n + i]);
+        r.d = mk(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]);
+        r.odx = mk(rays[6 * n + i], rays[7 * n + i], rays[8 * n + i]);
+        r.ody = mk(rays[9 * n + i], rays[10 * n + i], rays[11 * n + i]);
+        r.ddx = mk(rays[12 * n + i], rays[13 * n + i], rays[14 * n + i]);
+        r.ddy = mk(rays[15 * n + i], rays[16 * n + i], rays[17 * n + i]);
+        r.time = rays[18 * n + i];
+        const uint32_t type = __float_as_uint(rays[19 * n + i]);
+        r.occl = type == PSY_RAY_OCCLUSION;
+        max_t = r.occl ? 1.0f : INFINITY;
+        r.w = ray_widths(r.o, r.d, r.odx, r.ody, r.ddx, r.ddy);
+    } else {
+        r.o = mk(rays[0 * n + i], rays[1 * n + i], rays[2 * n + i]);
+        r.d = mk(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]);
+        max_t = rays[6 * n + i];
+        r.time = rays[7 * n + i];
+        r.w.owx = rays[8 * n + i];
+        r.w.owy = rays[9 * n + i];
+        r.w.dwx = rays[10 * n + i];
+        r.w.dwy = rays[11 * n + i];
+        r.w.fwx = rays[12 * n + i];
+        r.w.fwy = rays[13 * n + i];
+        r.occl = (__float_as_uint(rays[14 * n + i]) & (1u << 30)) != 0;
+        r.odx = r.ody = r.ddx = r.ddy = splat(0.0f);
+    }
+    if (flags & PSY_TRACE_OCCLUSION_ONLY) r.occl = true;
+    r.dinv = divv(splat(1.0f), r.d);
+
+    Best b;
+    b.t = max_t;
+    b.inst = 0xffffffffu;
+    b.u = b.v = b.aux = 0.0f;
+
+    int sp = 0;
+    if (s.root != CHILD_EMPTY) {
+        stk_ref[0][tid] = s.root;
+        stk_near[0][tid] = 0.0f;
+        sp = 1;
+    }
+    const int ds0 = r.dinv.x < 0.0f, ds1 = r.dinv.y < 0.0f, ds2 = r.dinv.z < 0.0f;
+    while (sp > 0) {
+        --sp;
+        const int32_t ref = stk_ref[sp][tid];
+        const float nearv = stk_near[sp][tid];
+        if (!(nearv < b.t)) continue;
+        if (ref >= 0) {
+            const BvhNode* nd = s.bvh + ref;
+            const float4* q = reinterpret_cast<const float4*>(nd);
+            const float4 lx = q[0], hx = q[1], ly = q[2], hy = q[3], lz = q[4], hz = q[5];
+            const int4 ch = reinterpret_cast<const int4*>(nd)[6];
+            const float lxs[4] = {lx.x, lx.y, lx.z, lx.w}, hxs[4] = {hx.x, hx.y, hx.z, hx.w};
+            const float lys[4] = {ly.x, ly.y, ly.z, ly.w}, hys[4] = {hy.x, hy.y, hy.z, hy.w};
+            const float lzs[4] = {lz.x, lz.y, lz.z, lz.w}, hzs[4] = {hz.x, hz.y, hz.z, hz.w};
+            const int32_t chs[4] = {ch.x, ch.y, ch.z, ch.w};
+            float nr[4];
+            int32_t cr[4];
+            int cnt = 0;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                if (chs[c] == CHILD_EMPTY) continue;
+                // BBox4::intersect_ray lane (basics/bbox.hpp:566-595)
+                const float xlo = ((ds0 ? hxs[c] : lxs[c]) - r.o.x) * r.dinv.x;
+                const float xhi = ((ds0 ? lxs[c] : hxs[c]) - r.o.x) * r.dinv.x;
+                const float ylo = ((ds1 ? hys[c] : lys[c]) - r.o.y) * r.dinv.y;
+                const float yhi = ((ds1 ? lys[c] : hys[c]) - r.o.y) * r.dinv.y;
+                const float zlo = ((ds2 ? hzs[c] : lzs[c]) - r.o.z) * r.dinv.z;
+                const float zhi = ((ds2 ? lzs[c] : hzs[c]) - r.o.z) * r.dinv.z;
+                const float mins = fmaxf(fmaxf(xlo, ylo), fmaxf(zlo, 0.0f));
+                float maxs = fminf(fminf(xhi, yhi), zhi) * PSY_BBOX_MAXT_ADJUST;
+                if ((mins < b.t) && (mins <= maxs)) {
+                    // insertion sort, farthest first
+                    int k = cnt++;
+                    while (k > 0 && nr[k - 1] < mins) { nr[k] = nr[k - 1]; cr[k] = cr[k - 1]; --k; }
+                    nr[k] = mins;
+                    cr[k] = chs[c];
+                }
+            }
+            for (int k = 0; k < cnt; ++k) {
+                stk_ref[sp][tid] = cr[k];
+                stk_near[sp][tid] = nr[k];
+                ++sp;
+            }
+        } else {
+            if (intersect_leaf<WORLD>(s, (uint32_t)(~ref), r, b)) break;
+        }
+    }
+    const bool hit = b.inst != 0xffffffffu;
+    if (hit && !r.occl) sphere_uv(s, b, r);
+    h.t[i] = hit ? b.t : INFINITY;
+    h.inst[i] = b.inst;
+    h.u[i] = b.u;
+    h.v[i] = b.v;
+    if (h.aux) h.aux[i] = b.aux;
+}
+
+// ---------------------------------------------------------------------------
+// WorldRay::to_ray over a stream (basics/ray.hpp:207-252, :310-323)
+// ---------------------------------------------------------------------------
+__global__ void k_world_to_rays(const float* __restrict__ w, int64_t n, float* __restrict__ r) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const V3 o = mk(w[0 * n + i], w[1 * n + i], w[2 * n + i]);
+    const V3 d = mk(w[3 * n + i], w[4 * n + i], w[5 * n + i]);
+    const V3 odx = mk(w[6 * n + i], w[7 * n + i], w[8 * n + i]);
+    const V3 ody = mk(w[9 * n + i], w[10 * n + i], w[11 * n + i]);
+    const V3 ddx = mk(w[12 * n + i], w[13 * n + i], w[14 * n + i]);
+    const V3 ddy = mk(w[15 * n + i], w[16 * n + i], w[17 * n + i]);
+    const float time = w[18 * n + i];
+    const uint32_t type = __float_as_uint(w[19 * n + i]);
+    const bool occl = type == PSY_RAY_OCCLUSION;
+    const RayW rw = ray_widths(o, d, odx, ody, ddx, ddy);
+    r[0 * n + i] = o.x; r[1 * n + i] = o.y; r[2 * n + i] = o.z;
+    r[3 * n + i] = d.x; r[4 * n + i] = d.y; r[5 * n + i] = d.z;
+    r[6 * n + i] = occl ? 1.0f : INFINITY;
+    r[7 * n + i] = time;
+    r[8 * n + i] = rw.owx; r[9 * n + i] = rw.owy;
+    r[10 * n + i] = rw.dwx; r[11 * n + i] = rw.dwy;
+    r[12 * n + i] = rw.fwx; r[13 * n + i] = rw.fwy;
+    r[14 * n + i] = __uint_as_float(occl ? (1u << 30) : 0u);
+}
+
+// ---------------------------------------------------------------------------
+// Camera rays: init_path + next_ray_for_path step 0
+// (integrator/path_trace_integrator.cpp:91-125, basics/camera.hpp:72-104,
+// utils/monte_carlo.hpp:17-53, utils/utils.hpp:335-343)
+// ---------------------------------------------------------------------------
+#define PSY_QPI (3.1415926535897932384626433 / 4)
+
+__device__ __forceinline__ float faster_ln(float x) {
+    float y = (float)__float_as_uint(x);
+    y *= 8.2629582881927490e-8f;
+    return y - 87.989971088f;
+}
+__device__ __forceinline__ float fast_logit(float p, float width) {
+    const float n = 0.001f + (p * 0.998f);
+    return faster_ln((n / (1.0f - n))) * width * (0.6266f / 4.0f);
+}
+__device__ __forceinline__ void square_to_circle(float* x, float* y) {
+    if (*x == 0.0f && *y == 0.0f) return;
+    float radius, angle;
+    if (*x > fabsf(*y)) {
+        radius = *x;
+        angle = (float)(PSY_QPI * (double)(*y / *x));
+    } else if (*y > fabsf(*x)) {
+        radius = *y;
+        angle = (float)(PSY_QPI * (double)(2 - (*x / *y)));
+    } else if (*x < -fabsf(*y)) {
+        radius = -*x;
+        angle = (float)(PSY_QPI * (double)(4 + (*y / *x)));
+    } else {
+        radius = -*y;
+        angle = (float)(PSY_QPI * (double)(6 - (*x / *y)));
+    }
+    *x = radius * cosf(angle);
+    *y = radius * sinf(angle);
+}
+
+__device__ __forceinline__ float lerp_seq_f(float alpha, const float* seq, int n) {
+    float na = 0.0f;
+    const int i = lerp_seq_index(alpha, n, &na);
+    if (i == -1) return seq[0];
+    if (i == -2) return seq[n - 1];
+    return lerpf(na, seq[i], seq[i + 1]);
+}
+
+__device__ __forceinline__ void store_wray(float* w, int64_t n, int64_t i, V3 o, V3 d, V3 odx, V3 ody, V3 ddx,
+                                           V3 ddy, float time, uint32_t type) {
+    w[0 * n + i] = o.x; w[1 * n + i] = o.y; w[2 * n + i] = o.z;
+    w[3 * n + i] = d.x; w[4 * n + i] = d.y; w[5 * n + i] = d.z;
+    w[6 * n + i] = odx.x; w[7 * n + i] = odx.y; w[8 * n + i] = odx.z;
+    w[9 * n + i] = ody.x; w[10 * n + i] = ody.y; w[11 * n + i] = ody.z;
+    w[12 * n + i] = ddx.x; w[13 * n + i] = ddx.y; w[14 * n + i] = ddx.z;
+    w[15 * n + i] = ddy.x; w[16 * n + i] = ddy.y; w[17 * n + i] = ddy.z;
+    w[18 * n + i] = time;
+    w[19 * n + i] = __uint_as_float(type);
+}
+
+__global__ void k_camera_rays(DevCamera cam, const HaltonDim* dims, const uint16_t* perm, int width, int height,
+                              int spp, uint32_t seed, float* wrays, float* wavelength) {
+    const int64_t n = (int64_t)width * height * spp;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int s = (int)(i % spp);
+    const int64_t pix = i / spp;
+    const int px = (int)(pix % width);
+    const int py = (int)(pix / width);
+    const uint32_t off = sampler_offset((uint32_t)px, (uint32_t)py, (uint32_t)s, seed);
+    const float min_x = -1.0f, max_x = 1.0f;
+    const float max_y = ((float)height) / (float)width;
+    const float min_y = -max_y;
+    uint32_t dim = 0;
+    const float wl = (sampler_get(dims, perm, off, dim++) * (700.0f - 380.0f)) + 380.0f;
+    const float samp_x = (fast_logit(sampler_get(dims, perm, off, dim++), 1.5f) + 0.5f + (float)(short)px) / (float)width;
+    const float samp_y = (fast_logit(sampler_get(dims, perm, off, dim++), 1.5f) + 0.5f + (float)(short)py) / (float)height;
+    const float rx = (float)(((double)samp_x - 0.5) * (double)(max_x - min_x));
+    const float ry = (float)((0.5 - (double)samp_y) * (double)(max_y - min_y));
+    const float dx = (max_x - min_x) / (float)width;
+    const float dy = (max_y - min_y) / (float)height;
+    const float sa = sampler_get(dims, perm, off, dim++);
+    const float sb = sampler_get(dims, perm, off, dim++);
+    const float time = sampler_get(dims, perm, off, dim++);
+    // Camera::generate_ray(rx, ry, dx, dy, time, sa, sb)
+    const M44 xf = m_lerp_seq(time, cam.xf, cam.n_xf);
+    const float tfov = lerp_seq_f(time, cam.tfov, cam.n_tfov);
+    const float ap = lerp_seq_f(time, cam.aperture, cam.n_ap);
+    const float fd = lerp_seq_f(time, cam.focus, cam.n_fd);
+    float ox = ap * ((sa * 2) - 1);
+    float oy = ap * ((sb * 2) - 1);
+    square_to_circle(&ox, &oy);
+    const V3 o = mk(ox, oy, 0.0f);
+    V3 d = mk((rx * tfov) - (o.x / fd), (ry * tfov) - (o.y / fd), 1.0f);
+    d = normalized(d);
+    const V3 ddx = mk(dx * tfov, 0.0f, 0.0f);
+    const V3 ddy = mk(0.0f, dy * tfov, 0.0f);
+    store_wray(wrays, n, i, m_pos(xf, o), m_dir(xf, d), m_dir(xf, splat(0.0f)), m_dir(xf, splat(0.0f)),
+               m_dir(xf, ddx), m_dir(xf, ddy), time, 1u /* WorldRay::CAMERA */);
+    if (wavelength) wavelength[i] = wl;
+}
+
+// ---------------------------------------------------------------------------
+// Lambert bounce rays from closest hits
+// (integrator/path_trace_integrator.cpp:183-209,
+//  shading/surface_closure.hpp:240-326, utils/utils.hpp:214-248,
+//  basics/differential_geometry.hpp:20-40, :83-98)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void coordinate_system_from_vec3(V3 v1, V3* v2, V3* v3o) {
+    if (fabsf(v1.x) > fabsf(v1.y)) {
+        const float invlen = 1.f / sqrtf(v1.x * v1.x + v1.z * v1.z);
+        *v2 = mk(-v1.z * invlen, 0.f, v1.x * invlen);
+    } else {
+        const float invlen = 1.f / sqrtf(v1.y * v1.y + v1.z * v1.z);
+        *v2 = mk(0.f, v1.z * invlen, -v1.y * invlen);
+    }
+    *v3o = cross(v1, *v2);
+}
+__device__ __forceinline__ V3 zup_to_vec(V3 from, V3 toz) {
+    toz = normalized(toz);
+    V3 tox, toy;
+    coordinate_system_from_vec3(toz, &tox, &toy);
+    return add(add(scale(tox, from.x), scale(toy, from.y)), scale(toz, from.z));
+}
+__device__ __forceinline__ V3 transfer_origin_diff(float t, V3 normal, V3 d, V3 od, V3 dd) {
+    const V3 temp = add(od, scale(dd, t));
+    const float td = -dot(temp, normal) / dot(d, normal);
+    const V3 real_projected = add(temp, scale(d, td));
+    return scale(normalized(real_projected), length(temp));
+}
+
+// Object-space hit geometry (n, offset) of a closest hit; false for lights
+// and misses.
+__device__ bool hit_geometry(const DevScene& s, uint32_t li, float t, float u, float v, float aux, V3 wo, V3 wd,
+                             float time, M44* space, V3* p_local, V3* n_local, V3* offset, bool* backfacing) {
+    const DevInstance I = s.inst[li];
+    const DevObject ob = s.obj[I.object];
+    V3 o = wo, d = wd;
+    *space = m_identity();
+    if (I.xf_count > 0) {
+        *space = m_lerp_seq(time, s.xf + I.xf_offset, I.xf_count);
+        o = m_pos(*space, wo);
+        d = m_dir(*space, wd);
+    }
+    const float* od = s.odata + ob.data;
+    const V3 p = add(o, scale(d, t));
+    V3 n;
+    if (ob.kind == K_BILINEAR || ob.kind == K_BICUBIC) {
+        V3 dpdu, dpdv, dndu, dndv;
+        if (ob.kind == K_BILINEAR) {
+            Patch<4> pp;
+            load_patch<4>(s.odata, ob.data, pp);
+            patch_dg<4>(pp, u, v, &n, &dpdu, &dpdv, &dndu, &dndv);
+        } else {
+            Patch<16> pp;
+            load_patch<16>(s.odata, ob.data, pp);
+            patch_dg<16>(pp, u, v, &n, &dpdu, &dpdv, &dndu, &dndv);
+        }
+        *offset = scale(n, aux * 1.74f);
+    } else if (ob.kind == K_SPHERE) {
+        V3 c; float rad;
+        sphere_at(od, ob.ts, 4, time, &c, &rad);
+        n = normalized(sub(p, c));
+        *offset = scale(n, 0.000001f);
+    } else {
+        return false;
+    }
+    *backfacing = dot(n, normalized(d)) > 0.0f;
+    *p_local = p;
+    *n_local = n;
+    return true;
+}
+
+__global__ void k_bounce_rays(DevScene s, const HaltonDim* dims, const uint16_t* perm, const float* __restrict__ in,
+                              DevHits hits, int64_t n, int width, int spp, uint32_t seed, uint32_t dim_u,
+                              uint32_t dim_v, float* out, uint32_t* valid) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t li = hits.inst[i];
+    const V3 zero = splat(0.0f);
+    bool ok = li != 0xffffffffu;
+    int shader = -1;
+    if (ok) {
+        shader = s.inst[li].shader;
+        ok = shader >= 0 && s.shaders[shader].kind == PSY_SHADER_LAMBERT;
+    }
+    const V3 wo = mk(in[0 * n + i], in[1 * n + i], in[2 * n + i]);
+    const V3 wd = mk(in[3 * n + i], in[4 * n + i], in[5 * n + i]);
+    const float time = in[18 * n + i];
+    M44 space;
+    V3 pl, nl, offl;
+    bool backfacing = false;
+    if (ok) ok = hit_geometry(s, li, hits.t[i], hits.u[i], hits.v[i], hits.aux ? hits.aux[i] : 0.0f, wo, wd,
+                              time, &space, &pl, &nl, &offl, &backfacing);
+    if (!ok) {
+        valid[i] = 0;
+        store_wray(out, n, i, zero, zero, zero, zero, zero, zero, time, 2u);
+        return;
+    }
+    valid[i] = 1;
+    // DifferentialGeometry::transformed_from(space)
+    const M44 inv = m_inverse(space);
+    const V3 gp = m_pos(inv, pl);
+    V3 gn = m_dir_transpose(space, nl);
+    const float il = 1.0f / length(gn);
+    gn = scale(gn, il);
+    V3 pos_offset = m_dir(inv, offl);
+    if (backfacing) pos_offset = scale(pos_offset, -1.0f);
+    // LambertClosure::sample
+    V3 nn = normalized(gn);
+    if (dot(nn, wd) > 0.0f) nn = scale(nn, -1.0f);
+    const int sidx = (int)(i % spp);
+    const int64_t pix = i / spp;
+    const uint32_t off = sampler_offset((uint32_t)(pix % width), (uint32_t)(pix / width), (uint32_t)sidx, seed);
+    float su = sampler_get(dims, perm, off, dim_u);
+    float sv = sampler_get(dims, perm, off, dim_v);
+    su = (su * 2) - 1;
+    sv = (sv * 2) - 1;
+    square_to_circle(&su, &sv);
+    const double zz = 1.0 - (double)((su * su) + (sv * sv));
+    const float z = (float)sqrt(zz > 0.0 ? zz : 0.0);
+    const V3 outd = zup_to_vec(mk(su, sv, z), nn);
+    // LambertClosure::propagate_differentials
+    const float len = length(outd);
+    const V3 gnn = normalized(gn);
+    V3 x, y;
+    coordinate_system_from_vec3(outd, &x, &y);
+    x = normalized(x);
+    y = normalized(y);
+    const V3 podx = mk(in[6 * n + i], in[7 * n + i], in[8 * n + i]);
+    const V3 pody = mk(in[9 * n + i], in[10 * n + i], in[11 * n + i]);
+    const V3 pddx = mk(in[12 * n + i], in[13 * n + i], in[14 * n + i]);
+    const V3 pddy = mk(in[15 * n + i], in[16 * n + i], in[17 * n + i]);
+    const float t = hits.t[i];
+    store_wray(out, n, i, add(gp, pos_offset), outd, transfer_origin_diff(t, gnn, wd, podx, pddx),
+               transfer_origin_diff(t, gnn, wd, pody, pddy), sdiv(scale(x, 0.15f), len),
+               sdiv(scale(y, 0.15f), len), time, 2u /* WorldRay::R_DIFFUSE */);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
+
+static int check_launch() { return hipGetLastError() == hipSuccess ? 0 : PSY_ERR_HIP; }
+
+int launch_build_dice(const float* odata, const DevObject* objs, const int32_t* patch_ids, int n_patches,
+                      int levels, float4* dice, hipStream_t st) {
+    const int64_t per = ((int64_t)1 << (levels + 1)) - 1;
+    const int64_t total = per * n_patches;
+    if (total <= 0) return 0;
+    hipLaunchKernelGGL(k_build_dice, dim3(grid_for(total, 256)), dim3(256), 0, st, odata, objs, patch_ids,
+                       n_patches, levels, dice);
+    return check_launch();
+}
+
+int launch_trace(const DevScene& s, bool world, bool occl_only, int stack_need, const float* rays, int64_t n,
+                 const DevHits& h, hipStream_t st) {
+    if (n <= 0) return 0;
+    const uint32_t flags = occl_only ? PSY_TRACE_OCCLUSION_ONLY : 0u;
+    const dim3 grid(grid_for(n, TRACE_BLOCK)), block(TRACE_BLOCK);
+#define PSY_TRACE_CASE(W, S) hipLaunchKernelGGL((k_trace<W, S>), grid, block, 0, st, s, rays, n, h, flags)
+    if (stack_need <= 16) { if (world) PSY_TRACE_CASE(true, 16); else PSY_TRACE_CASE(false, 16); }
+    else if (stack_need <= 32) { if (world) PSY_TRACE_CASE(true, 32); else PSY_TRACE_CASE(false, 32); }
+    else if (stack_need <= 64) { if (world) PSY_TRACE_CASE(true, 64); else PSY_TRACE_CASE(false, 64); }
+    else return PSY_ERR_UNSUPPORTED;
+#undef PSY_TRACE_CASE
+    return check_launch();
+}
+
+int launch_world_to_rays(const float* wrays, int64_t n, float* rays, hipStream_t st) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_world_to_rays, dim3(grid_for(n, 256)), dim3(256), 0, st, wrays, n, rays);
+    return check_launch();
+}
+
+int launch_halton(const HaltonDim* dims, const uint16_t* perm, uint32_t dim, const uint32_t* idx, int64_t n,
+                  float* out, hipStream_t st) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_halton, dim3(grid_for(n, 256)), dim3(256), 0, st, dims, perm, dim, idx, n, out);
+    return check_launch();
+}
+
+int launch_pixel_samples(const HaltonDim* dims, const uint16_t* perm, int x0, int y0, int w, int h, int spp,
+                         uint32_t seed, uint32_t dim0, int n_dims, float* out, hipStream_t st) {
+    const int64_t total = (int64_t)w * h * spp;
+    if (total <= 0) return 0;
+    hipLaunchKernelGGL(k_pixel_samples, dim3(grid_for(total, 256)), dim3(256), 0, st, dims, perm, x0, y0, w, h,
+                       spp, seed, dim0, n_dims, out);
+    return check_launch();
+}
+
+int launch_camera_rays(const DevCamera& cam, const HaltonDim* dims, const uint16_t* perm, int width, int height,
+                       int spp, uint32_t seed, float* wrays, float* wavelength, hipStream_t st) {
+    const int64_t n = (int64_t)width * height * spp;
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_camera_rays, dim3(grid_for(n, 256)), dim3(256), 0, st, cam, dims, perm, width, height,
+                       spp, seed, wrays, wavelength);
+    return check_launch();
+}
+
+int launch_bounce_rays(const DevScene& s, const HaltonDim* dims, const uint16_t* perm, const float* in_wrays,
+                       const DevHits& hits, int64_t n, int width, int spp, uint32_t seed, uint32_t dim_u,
+                       uint32_t dim_v, float* out_wrays, uint32_t* valid, hipStream_t st) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_bounce_rays, dim3(grid_for(n, 256)), dim3(256), 0, st, s, dims, perm, in_wrays, hits, n,
+                       width, spp, seed, dim_u, dim_v, out_wrays, valid);
+    return check_launch();
+}
+
+}  // namespace psy

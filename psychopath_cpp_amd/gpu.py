@@ -1,0 +1,208 @@
+"""Python bindings of libpsy_gpu.so over ctypes, with torch for device memory.
+
+Ray and hit streams are torch tensors on the GPU: world rays as a
+(20, n) float32 tensor, object rays as (15, n) (plane layout of
+include/psy_gpu.h).  The library is the only compute path; importing this
+module on a machine without the built library raises immediately.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+import torch
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpsy_gpu.so")
+_lib = None
+
+
+class PsyError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libpsy_gpu.so (built in-tree by psychopath_cpp_amd.build)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise PsyError(f"{LIB_PATH} is missing: run `python -m psychopath_cpp_amd.build` first")
+    L = C.CDLL(LIB_PATH)
+    vp, i64, u32, i32 = C.c_void_p, C.c_int64, C.c_uint32, C.c_int32
+    L.psy_last_error.restype = C.c_char_p
+    L.psy_abi_version.restype = C.c_int
+    L.psy_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
+    L.psy_ctx_destroy.argtypes = [vp]
+    L.psy_scene_create.argtypes = [vp, C.POINTER(abi.SceneDesc), C.c_int, C.POINTER(vp)]
+    L.psy_scene_destroy.argtypes = [vp]
+    L.psy_scene_get_info.argtypes = [vp, C.POINTER(abi.SceneInfo)]
+    L.psy_scene_instance_ids.argtypes = [vp, vp, i64]
+    L.psy_trace_world.argtypes = [vp, vp, vp, i64, C.POINTER(abi.Hits), u32, vp]
+    L.psy_trace_rays.argtypes = [vp, vp, vp, i64, C.POINTER(abi.Hits), u32, vp]
+    L.psy_world_to_rays.argtypes = [vp, vp, i64, vp, vp]
+    L.psy_halton.argtypes = [vp, u32, vp, i64, vp, vp]
+    L.psy_pixel_samples.argtypes = [vp, i32, i32, i32, i32, i32, u32, u32, i32, vp, vp]
+    L.psy_camera_rays.argtypes = [vp, vp, i32, i32, i32, u32, vp, vp, vp]
+    L.psy_bounce_rays.argtypes = [vp, vp, vp, C.POINTER(abi.Hits), i64, i32, i32, u32, u32, u32, vp, vp, vp]
+    if L.psy_abi_version() != 1:
+        raise PsyError("libpsy_gpu.so ABI version mismatch")
+    _lib = L
+    return L
+
+
+def check(rc, what):
+    if rc != abi.PSY_OK:
+        msg = lib().psy_last_error().decode(errors="replace")
+        raise PsyError(f"{what} failed (status {rc}): {msg}")
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else C.c_void_p(0)
+
+
+class Context:
+    """psy_ctx_create / psy_ctx_destroy."""
+
+    def __init__(self, device=0):
+        self.device = device
+        h = C.c_void_p()
+        check(lib().psy_ctx_create(device, C.byref(h)), "psy_ctx_create")
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().psy_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GpuScene:
+    """psy_scene_create: flatten + BVH4 + dice cache resident in HBM."""
+
+    def __init__(self, ctx, scene, dice_depth=-1):
+        self.ctx = ctx
+        self._desc = scene.to_desc() if hasattr(scene, "to_desc") else scene
+        h = C.c_void_p()
+        check(lib().psy_scene_create(ctx.handle, self._desc.ptr(), int(dice_depth), C.byref(h)),
+              "psy_scene_create")
+        self.handle = h
+        info = abi.SceneInfo()
+        check(lib().psy_scene_get_info(h, C.byref(info)), "psy_scene_get_info")
+        self.info = info
+
+    def instance_ids(self):
+        n = int(self.info.n_leaf_instances)
+        out = np.zeros(max(1, n), np.uint64)
+        check(lib().psy_scene_instance_ids(self.handle, out.ctypes.data_as(C.c_void_p), out.size),
+              "psy_scene_instance_ids")
+        return out[:n]
+
+    def close(self):
+        if self.handle:
+            lib().psy_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class HitBuffers:
+    def __init__(self, n, device, aux=True):
+        self.t = torch.empty(n, dtype=torch.float32, device=device)
+        self.inst = torch.empty(n, dtype=torch.int32, device=device)
+        self.u = torch.empty(n, dtype=torch.float32, device=device)
+        self.v = torch.empty(n, dtype=torch.float32, device=device)
+        self.aux = torch.empty(n, dtype=torch.float32, device=device) if aux else None
+
+    def struct(self):
+        return abi.Hits(self.t.data_ptr(), self.inst.data_ptr(), self.u.data_ptr(), self.v.data_ptr(),
+                        self.aux.data_ptr() if self.aux is not None else None)
+
+    def inst_u32(self):
+        return self.inst.view(torch.int32).cpu().numpy().view(np.uint32)
+
+
+def _check_planes(t, planes, name):
+    if not (t.is_cuda and t.dtype == torch.float32 and t.dim() == 2 and t.shape[0] == planes and t.is_contiguous()):
+        raise PsyError(f"{name} must be a contiguous CUDA float32 tensor of shape ({planes}, n)")
+
+
+def trace_world(ctx, scene, wrays, occlusion_only=False, hits=None):
+    """Tracer::trace over world rays (20, n)."""
+    _check_planes(wrays, abi.PSY_WRAY_PLANES, "world rays")
+    n = wrays.shape[1]
+    hits = hits or HitBuffers(n, wrays.device)
+    hs = hits.struct()
+    flags = abi.PSY_TRACE_OCCLUSION_ONLY if occlusion_only else 0
+    check(lib().psy_trace_world(ctx.handle, scene.handle, _ptr(wrays), n, C.byref(hs), flags, _stream()),
+          "psy_trace_world")
+    return hits
+
+
+def trace_rays(ctx, scene, rays, occlusion_only=False, hits=None):
+    """Tracer::trace over rays with derived footprints (15, n)."""
+    _check_planes(rays, abi.PSY_RAY_PLANES, "rays")
+    n = rays.shape[1]
+    hits = hits or HitBuffers(n, rays.device)
+    hs = hits.struct()
+    flags = abi.PSY_TRACE_OCCLUSION_ONLY if occlusion_only else 0
+    check(lib().psy_trace_rays(ctx.handle, scene.handle, _ptr(rays), n, C.byref(hs), flags, _stream()),
+          "psy_trace_rays")
+    return hits
+
+
+def world_to_rays(ctx, wrays):
+    _check_planes(wrays, abi.PSY_WRAY_PLANES, "world rays")
+    n = wrays.shape[1]
+    out = torch.empty((abi.PSY_RAY_PLANES, n), dtype=torch.float32, device=wrays.device)
+    check(lib().psy_world_to_rays(ctx.handle, _ptr(wrays), n, _ptr(out), _stream()), "psy_world_to_rays")
+    return out
+
+
+def halton(ctx, dim, index):
+    if not (index.is_cuda and index.dtype == torch.int32 and index.is_contiguous()):
+        raise PsyError("index must be a contiguous CUDA int32 tensor (uint32 bit patterns)")
+    out = torch.empty(index.shape, dtype=torch.float32, device=index.device)
+    check(lib().psy_halton(ctx.handle, dim, _ptr(index), index.numel(), _ptr(out), _stream()), "psy_halton")
+    return out
+
+
+def pixel_samples(ctx, x0, y0, w, h, spp, seed, dim0, n_dims, device="cuda"):
+    out = torch.empty((w * h * spp, n_dims), dtype=torch.float32, device=device)
+    check(lib().psy_pixel_samples(ctx.handle, x0, y0, w, h, spp, seed, dim0, n_dims, _ptr(out), _stream()),
+          "psy_pixel_samples")
+    return out
+
+
+def camera_rays(ctx, scene, width, height, spp, seed, device="cuda"):
+    n = width * height * spp
+    w = torch.empty((abi.PSY_WRAY_PLANES, n), dtype=torch.float32, device=device)
+    wl = torch.empty(n, dtype=torch.float32, device=device)
+    check(lib().psy_camera_rays(ctx.handle, scene.handle, width, height, spp, seed, _ptr(w), _ptr(wl), _stream()),
+          "psy_camera_rays")
+    return w, wl
+
+
+def bounce_rays(ctx, scene, wrays, hits, width, spp, seed, dim_u=9, dim_v=10):
+    _check_planes(wrays, abi.PSY_WRAY_PLANES, "world rays")
+    n = wrays.shape[1]
+    out = torch.empty_like(wrays)
+    valid = torch.empty(n, dtype=torch.int32, device=wrays.device)
+    hs = hits.struct()
+    check(lib().psy_bounce_rays(ctx.handle, scene.handle, _ptr(wrays), C.byref(hs), n, width, spp, seed, dim_u,
+                                dim_v, _ptr(out), _ptr(valid), _stream()), "psy_bounce_rays")
+    return out, valid

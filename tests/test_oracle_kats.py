@@ -1,0 +1,161 @@
+"""The oracle pinned against the reference's own known-answer tests
+(tests/golden/reference_kats.json) and against vectors recorded from the
+reference's self-contained sources (tests/golden/ref_vectors.npz)."""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+import pytest
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def kats():
+    with open(os.path.join(GOLD, "reference_kats.json")) as f:
+        return json.load(f)
+
+
+def refvec():
+    return np.load(os.path.join(GOLD, "ref_vectors.npz"))
+
+
+def _f3(v):
+    return np.array(v, np.float32)
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def approx(a, x, eps):
+    """Catch v1 Approx: |a - x| < eps * (1 + max(|a|, |x|))."""
+    return abs(a - x) < eps * (1.0 + max(abs(a), abs(x)))
+
+
+def test_bbox_ray_kats(oracle):
+    L = oracle.lib()
+    for c in kats()["bbox_ray"]["cases"]:
+        h0 = C.c_float()
+        h1 = C.c_float()
+        hit = L.ora_bbox_ray(_p(_f3(c["bmin"])), _p(_f3(c["bmax"])), _p(_f3(c["o"])), _p(_f3(c["d"])),
+                             float("inf"), C.byref(h0), C.byref(h1))
+        assert bool(hit) == c["hit"], c["name"]
+        if c["hit"]:
+            assert h0.value == c["h0"], c["name"]
+            assert c["h1_min"] <= h1.value <= c["h1_max"], c["name"]
+
+
+def test_bbox2_ray_kats_as_bbox4(oracle):
+    L = oracle.lib()
+    for c in kats()["bbox2_ray"]["cases"]:
+        b = np.zeros((6, 4), np.float32)
+        b[0::2, :] = np.inf
+        b[1::2, :] = -np.inf
+        for lane, (lo, hi) in enumerate(c["boxes"]):
+            for ax in range(3):
+                b[2 * ax, lane] = lo[ax]
+                b[2 * ax + 1, lane] = hi[ax]
+        near = np.zeros(4, np.float32)
+        mask = L.ora_bbox4_ray(_p(np.ascontiguousarray(b)), _p(_f3(c["o"])), _p(_f3(c["d"])), float("inf"), _p(near))
+        assert mask == c["mask"], c["name"]
+        for lane, v in c["near"].items():
+            assert near[int(lane)] == v, c["name"]
+
+
+def test_matrix_inverse_kat(oracle):
+    k = kats()["matrix_inverse"]
+    inv = oracle.matrix_inverse(np.array(k["m"], np.float32)).reshape(16)
+    for a, x in zip(inv, k["inverse"]):
+        assert approx(float(a), x, k["epsilon"])
+    sing = oracle.matrix_inverse(np.array(k["singular"], np.float32))
+    assert np.isnan(sing).all()
+
+
+def test_matrix_inverse_matches_reference_bits(oracle):
+    rv = refvec()
+    for m, want in zip(rv["inv_in"], rv["inv_out"]):
+        got = oracle.matrix_inverse(m)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_vector_kats(oracle):
+    L = oracle.lib()
+    k = kats()["vector"]
+    v1, v2 = _f3(k["v1"]), _f3(k["v2"])
+    assert approx(L.ora_vec3_length(_p(v1)), k["length"], k["length_eps"])
+    n = np.zeros(3, np.float32)
+    L.ora_vec3_normalize(_p(v1), _p(n))
+    for a, x in zip(n, k["normalized"]):
+        assert approx(float(a), x, k["normalize_eps"])
+    assert approx(L.ora_vec3_dot(_p(v1), _p(v2)), k["dot"], k["dot_eps"])
+    cr = np.zeros(3, np.float32)
+    L.ora_vec3_cross(_p(v1), _p(v2), _p(cr))
+    for a, x in zip(cr, k["cross"]):
+        assert approx(float(a), x, k["cross_eps"])
+
+
+def test_vector_length_tiny_branch(oracle):
+    # Imath takes the scaled path below 2*FLT_MIN; a plain sqrt of the
+    # underflowed square would return 0 here.
+    L = oracle.lib()
+    v = np.array([3e-30, 4e-30, 0.0], np.float32)
+    assert L.ora_vec3_length(_p(v)) == pytest.approx(5e-30, rel=1e-6)
+
+
+def test_mutable_partition_kat(oracle):
+    k = kats()["mutable_partition"]
+    v = np.array(k["input"], np.int32)
+    calls = C.c_int()
+    split = oracle.lib().ora_partition_ints(_p(v), v.size, k["key"], C.byref(calls))
+    assert calls.value == k["calls"]
+    assert split == k["split"]
+    assert v.tolist() == k["output"]
+
+
+def test_halton_matches_reference_bits(oracle):
+    rv = refvec()
+    got = np.array([oracle.lib().ora_halton(int(d), int(i)) for d, i in zip(rv["halton_dim"], rv["halton_index"])],
+                   np.float32)
+    assert np.array_equal(got.view(np.uint32), rv["halton_value"].view(np.uint32))
+
+
+def test_halton_radical_inverse_closed_form(oracle):
+    # Faure's digit permutations: base 3 -> (0,1,2) (plain radical inverse),
+    # base 5 -> (0,3,2,1,4).  Sample i of a dimension is the permuted
+    # radical inverse of i scaled by (1 - 2^-24); checks digit order and the
+    # permutation recurrence (sampling/halton.py:46-87).
+    def radinv(i, b, perm, ndig=6):
+        r, f = 0.0, 1.0 / b
+        for _ in range(ndig):
+            r += perm[i % b] * f
+            i //= b
+            f /= b
+        return r
+    for i in range(200):
+        assert oracle.lib().ora_halton(1, i) == pytest.approx(radinv(i, 3, (0, 1, 2)) * (1 - 2**-24), abs=2e-7)
+        assert oracle.lib().ora_halton(2, i) == pytest.approx(radinv(i, 5, (0, 3, 2, 1, 4)) * (1 - 2**-24), abs=2e-7)
+    # base 2 is the plain bit reversal
+    for i in range(64):
+        r = int("{:032b}".format(i)[::-1], 2)
+        assert oracle.lib().ora_halton(0, i) == np.float32(r) * np.float32(2**-32)
+
+
+def test_hash_matches_reference(oracle):
+    rv = refvec()
+    L = oracle.lib()
+    got = np.array([L.ora_hash_u32(int(a), int(b)) for a, b in zip(rv["hash_n"], rv["hash_seed"])], np.uint32)
+    assert np.array_equal(got, rv["hash_value"])
+
+
+def test_oracle_against_ref_build_if_present(oracle):
+    R = oracle.ref()
+    if R is None:
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    rng = np.random.default_rng(5)
+    for _ in range(2000):
+        d = int(rng.integers(0, 256))
+        i = int(rng.integers(0, 2**32))
+        a = np.float32(oracle.lib().ora_halton(d, i))
+        b = np.float32(R.psyref_halton(d, i))
+        assert a.view(np.uint32) == b.view(np.uint32)

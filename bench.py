@@ -71,7 +71,21 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1 << 21)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--dice-depth", type=int, default=-1)
+    ap.add_argument("--split", action="store_true",
+                    help="also time the primary and the bounce half of the stream separately (stderr)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device-to-device copy of BYTES (calibrates rocprofv3 byte counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        src = torch.empty(args.copy_probe // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: 3 copies of {args.copy_probe} bytes")
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box has one GPU; "
               "the path shards by image tiles or ray ranges, see DESIGN.md)", file=sys.stderr)
@@ -115,6 +129,21 @@ def main():
     achieved = BYTES_PER_RAY * STREAM_RAYS / (ms / 1e3) / 1e9
     hit_frac = float((hits.inst != -1).float().mean().item())
     log(f"trace: {ms:.3f} ms/step (HIP events), wall {wall / args.steps * 1e3:.3f} ms/step, hit fraction {hit_frac:.4f}")
+
+    if args.split:
+        half = STREAM_RAYS // 2
+        for name, sl in (("primary", slice(0, half)), ("bounce", slice(half, STREAM_RAYS))):
+            sub = rays[:, sl].contiguous()
+            hs = gpu.HitBuffers(half, rays.device, aux=False)
+            gpu.trace_rays(ctx, scene, sub, hits=hs)
+            torch.cuda.synchronize()
+            ev0.record(stream)
+            for _ in range(args.steps):
+                gpu.trace_rays(ctx, scene, sub, hits=hs)
+            ev1.record(stream)
+            torch.cuda.synchronize()
+            pms = ev0.elapsed_time(ev1) / args.steps
+            log(f"split {name}: {pms:.3f} ms for {half} rays = {half / (pms / 1e3) / 1e6:.1f} Mrays/s")
 
     cpu = None
     if not args.no_cpu:

@@ -80,6 +80,8 @@ static v3 zup_to_vec(v3 from, v3 toz) {
 }
 
 /* Camera::generate_ray */
+void ora_camera_ray_from_dims(const ora_scene* s, float x, float y, float dx, float dy, float time,
+                              float u, float v, ora_wray* w);
 static void generate_ray(const ora_scene* s, float x, float y, float dx, float dy, float time,
                          float u, float v, ora_wray* w) {
     memset(w, 0, sizeof *w);
@@ -103,6 +105,11 @@ static void generate_ray(const ora_scene* s, float x, float y, float dx, float d
     set3(w->ody, mat4_dir(&xf, v3s(0.0f)));
     set3(w->ddx, mat4_dir(&xf, ddx));
     set3(w->ddy, mat4_dir(&xf, ddy));
+}
+
+void ora_camera_ray_from_dims(const ora_scene* s, float x, float y, float dx, float dy, float time,
+                              float u, float v, ora_wray* w) {
+    generate_ray(s, x, y, dx, dy, time, u, v, w);
 }
 
 void ora_camera_rays(const ora_scene* s, int width, int height, int spp, uint32_t seed,
@@ -151,6 +158,46 @@ static v3 transfer_origin_diff(float t, v3 normal, v3 d, v3 od, v3 dd) {
     return v3scale(v3norm(real_projected), v3len(temp));
 }
 
+/* LambertClosure::sample + the bounce-ray set-up of next_ray_for_path for
+ * one stored hit: writes o, d and differentials of `w`, the pdf, and the
+ * world-space hit point / normal. */
+void ora_bounce_dir(const ora_inter* h, const ora_wray* prev, float su, float sv, ora_wray* w, float* pdf,
+                    v3* world_p, v3* world_n) {
+    v3 gp, gn;
+    dg_to_world(h, &gp, &gn);
+    mat4 sp;
+    memcpy(&sp.m[0][0], h->space, sizeof(float) * 16);
+    const mat4 inv = ora_mat4_inverse(&sp);
+    v3 pos_offset = mat4_dir(&inv, va(h->offset));
+    if (h->backfacing) pos_offset = v3scale(pos_offset, -1.0f);
+    const v3 pd = va(prev->d);
+    v3 nn = v3norm(gn);
+    if (v3dot(nn, pd) > 0.0f) nn = v3scale(nn, -1.0f);
+    su = (su * 2) - 1;
+    sv = (sv * 2) - 1;
+    square_to_circle(&su, &sv);
+    const double zz = 1.0 - (double)((su * su) + (sv * sv));
+    const float z = (float)sqrt(zz > 0.0 ? zz : 0.0);
+    const v3 dir = v3m(su, sv, z);
+    *pdf = dir.z * (float)INV_PI;
+    const v3 outd = zup_to_vec(dir, nn);
+    set3(w->o, v3add(gp, pos_offset));
+    set3(w->d, outd);
+    /* LambertClosure::propagate_differentials */
+    const float len = v3len(outd);
+    const v3 gnn = v3norm(gn);
+    v3 x, y;
+    coordinate_system_from_vec3(outd, &x, &y);
+    x = v3norm(x);
+    y = v3norm(y);
+    set3(w->odx, transfer_origin_diff(h->t, gnn, pd, va(prev->odx), va(prev->ddx)));
+    set3(w->ody, transfer_origin_diff(h->t, gnn, pd, va(prev->ody), va(prev->ddy)));
+    set3(w->ddx, v3sdiv(v3scale(x, 0.15f), len));
+    set3(w->ddy, v3sdiv(v3scale(y, 0.15f), len));
+    *world_p = gp;
+    *world_n = gn;
+}
+
 void ora_bounce_rays(const ora_scene* s, const ora_wray* in, const ora_inter* hits,
                      int64_t first, int64_t count, int width, int spp, uint32_t seed,
                      uint32_t dim_u, uint32_t dim_v, ora_wray* out, uint32_t* valid) {
@@ -158,7 +205,6 @@ void ora_bounce_rays(const ora_scene* s, const ora_wray* in, const ora_inter* hi
     for (int64_t k = 0; k < count; ++k) {
         const int64_t i = first + k;
         const ora_inter* h = &hits[k];
-        const ora_wray* prev = &in[k];
         ora_wray* w = &out[k];
         memset(w, 0, sizeof *w);
         if (!h->hit || h->closure != PSY_SHADER_LAMBERT) { valid[k] = 0; continue; }
@@ -166,40 +212,10 @@ void ora_bounce_rays(const ora_scene* s, const ora_wray* in, const ora_inter* hi
         const int sidx = (int)(i % spp);
         const int64_t pix = i / spp;
         const uint32_t off = ora_sampler_offset((uint32_t)(pix % width), (uint32_t)(pix / width), (uint32_t)sidx, seed);
+        float pdf;
         v3 gp, gn;
-        dg_to_world(h, &gp, &gn);
-        mat4 sp;
-        memcpy(&sp.m[0][0], h->space, sizeof(float) * 16);
-        const mat4 inv = ora_mat4_inverse(&sp);
-        v3 pos_offset = mat4_dir(&inv, va(h->offset));
-        if (h->backfacing) pos_offset = v3scale(pos_offset, -1.0f);
-        /* LambertClosure::sample */
-        const v3 pd = va(prev->d);
-        v3 nn = v3norm(gn);
-        if (v3dot(nn, pd) > 0.0f) nn = v3scale(nn, -1.0f);
-        float su = ora_sampler_get(off, dim_u);
-        float sv = ora_sampler_get(off, dim_v);
-        su = (su * 2) - 1;
-        sv = (sv * 2) - 1;
-        square_to_circle(&su, &sv);
-        const double zz = 1.0 - (double)((su * su) + (sv * sv));
-        const float z = (float)sqrt(zz > 0.0 ? zz : 0.0);
-        const v3 dir = v3m(su, sv, z);
-        const v3 outd = zup_to_vec(dir, nn);
-        set3(w->o, v3add(gp, pos_offset));
-        set3(w->d, outd);
-        w->time = prev->time;
+        ora_bounce_dir(h, &in[k], ora_sampler_get(off, dim_u), ora_sampler_get(off, dim_v), w, &pdf, &gp, &gn);
+        w->time = in[k].time;
         w->type = 2;  /* WorldRay::R_DIFFUSE */
-        /* LambertClosure::propagate_differentials */
-        const float len = v3len(outd);
-        const v3 gnn = v3norm(gn);
-        v3 x, y;
-        coordinate_system_from_vec3(outd, &x, &y);
-        x = v3norm(x);
-        y = v3norm(y);
-        set3(w->odx, transfer_origin_diff(h->t, gnn, pd, va(prev->odx), va(prev->ddx)));
-        set3(w->ody, transfer_origin_diff(h->t, gnn, pd, va(prev->ody), va(prev->ddy)));
-        set3(w->ddx, v3sdiv(v3scale(x, 0.15f), len));
-        set3(w->ddy, v3sdiv(v3scale(y, 0.15f), len));
     }
 }

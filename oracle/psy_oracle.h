@@ -100,6 +100,16 @@ void ora_bounce_rays(const ora_scene* s, const ora_wray* in, const ora_inter* hi
                      uint32_t seed, uint32_t dim_u, uint32_t dim_v,
                      ora_wray* out, uint32_t* valid);
 
+/* PathTraceIntegrator::integrate (integrator/path_trace_integrator.cpp:28-384)
+ * for every pixel sample; path_xyz (width*height*spp*3) receives each
+ * sample's film contribution, ordered (y, x, s).  Returns rays traced, or
+ * -1 if a chromatic colour was met (colour upsampling is restated for
+ * achromatic colours only). */
+int64_t ora_render(const ora_scene* s, int width, int height, int spp, uint32_t seed,
+                   int threads, float* path_xyz);
+/* Color_to_SpectralSample value for an achromatic colour */
+float ora_color_to_spectrum(const float c[3]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -149,7 +149,29 @@ HaltonTables make_halton_tables() {
 struct psy_ctx {
     int device = 0;
     DBuf halton_dims, halton_perm;
+    // trace scratch: queue of rays for the below-cache pass + its counter
+    DBuf queue, qcount;
+    int64_t queue_cap = 0;
 };
+
+namespace {
+// Grows the per-context trace scratch to n entries.  Calls on one context
+// must not overlap in time (they share this scratch).
+int ensure_queue(psy_ctx* ctx, int64_t n) {
+    if (!ctx->qcount.p) {
+        int rc = ctx->qcount.alloc(sizeof(uint32_t) * 4);
+        if (rc) return rc;
+    }
+    if (n <= ctx->queue_cap) return PSY_OK;
+    HIP_OK(hipDeviceSynchronize());
+    if (ctx->queue.p) { (void)hipFree(ctx->queue.p); ctx->queue.p = nullptr; }
+    const int64_t cap = std::max<int64_t>(n, 1 << 16);
+    int rc = ctx->queue.alloc((size_t)cap * sizeof(uint32_t));
+    if (rc) { ctx->queue_cap = 0; return rc; }
+    ctx->queue_cap = cap;
+    return PSY_OK;
+}
+}  // namespace
 
 struct psy_scene {
     psy_ctx* ctx = nullptr;
@@ -610,10 +632,10 @@ int psy_scene_create(psy_ctx* ctx, const psy_scene_desc* d, int dice_depth, psy_
     if (levels < 0) {
         const double budget = 1024.0 * 1024.0 * 1024.0;  // 1 GiB of split-tree nodes
         levels = 14;
-        while (levels > 0 && (double)(((int64_t)1 << (levels + 1)) - 1) * 32.0 * (double)patch_ids.size() > budget) --levels;
+        while (levels > 0 && (double)((int64_t)1 << (levels + 1)) * 32.0 * (double)patch_ids.size() > budget) --levels;
     }
     if (levels > 20) return fail(PSY_ERR_INVALID, "psy_scene_create: dice_depth above 20");
-    const int64_t per = ((int64_t)1 << (levels + 1)) - 1;
+    const int64_t per = (int64_t)1 << (levels + 1);  // slot: 1-based heap, entry 0 unused
     for (size_t k = 0; k < patch_ids.size(); ++k) {
         objs[patch_ids[k]].dice_levels = levels;
         objs[patch_ids[k]].dice = (int64_t)k * per;
@@ -733,8 +755,9 @@ int psy_trace_world(psy_ctx* ctx, const psy_scene* s, const float* rays, int64_t
     if (n < 0) return fail(PSY_ERR_INVALID, "psy_trace_world: negative ray count");
     if (int rc = check_hits(hits, n)) return rc;
     HIP_OK(hipSetDevice(ctx->device));
+    if (int rc = ensure_queue(ctx, n)) return rc;
     int rc = launch_trace(s->dev, true, (flags & PSY_TRACE_OCCLUSION_ONLY) != 0, s->stack_need, rays, n, to_dev(hits),
-                          (hipStream_t)stream);
+                          (uint32_t*)ctx->queue.p, (uint32_t*)ctx->qcount.p, (hipStream_t)stream);
     return rc ? fail(rc, "psy_trace_world: launch failed") : PSY_OK;
 }
 
@@ -746,8 +769,9 @@ int psy_trace_rays(psy_ctx* ctx, const psy_scene* s, const float* rays, int64_t 
         return fail(PSY_ERR_UNSUPPORTED, "psy_trace_rays: scene has transformed patch instances; their footprint needs world rays (psy_trace_world)");
     if (int rc = check_hits(hits, n)) return rc;
     HIP_OK(hipSetDevice(ctx->device));
+    if (int rc = ensure_queue(ctx, n)) return rc;
     int rc = launch_trace(s->dev, false, (flags & PSY_TRACE_OCCLUSION_ONLY) != 0, s->stack_need, rays, n,
-                          to_dev(hits), (hipStream_t)stream);
+                          to_dev(hits), (uint32_t*)ctx->queue.p, (uint32_t*)ctx->qcount.p, (hipStream_t)stream);
     return rc ? fail(rc, "psy_trace_rays: launch failed") : PSY_OK;
 }
 

@@ -10,8 +10,9 @@
 //            effective shader) -- the assembly hierarchy flattened.
 //   obj    : 32-byte object records; control points / centres in `odata`.
 //   xf     : merged per-leaf transform lists (world -> object), 64 B each.
-//   dice   : the patch dice cache: for every patch a complete heap-ordered
-//            split tree of `dice_levels`+1 levels, 32 bytes per node:
+//   dice   : the patch dice cache: for every patch a complete 1-based
+//            heap-ordered split tree of `dice_levels`+1 levels in a slot of
+//            2^(levels+1) entries (entry 0 unused), 32 bytes per node:
 //            float4(min.xyz, u_center), float4(max.xyz, v_center).
 #pragma once
 
@@ -94,8 +95,9 @@ struct DevHits { float* t; uint32_t* inst; float* u; float* v; float* aux; };
 // ---- launchers (psy_kernels.hip) -------------------------------------------
 int launch_build_dice(const float* odata, const DevObject* objs, const int32_t* patch_ids,
                       int n_patches, int levels, float4* dice, hipStream_t st);
+// Two stream-ordered passes; queue (n entries) and qcount are scratch.
 int launch_trace(const DevScene& s, bool world, bool occl_only, int stack_need, const float* rays,
-                 int64_t n, const DevHits& h, hipStream_t st);
+                 int64_t n, const DevHits& h, uint32_t* queue, uint32_t* qcount, hipStream_t st);
 int launch_world_to_rays(const float* wrays, int64_t n, float* rays, hipStream_t st);
 int launch_halton(const HaltonDim* dims, const uint16_t* perm, uint32_t dim, const uint32_t* idx,
                   int64_t n, float* out, hipStream_t st);

@@ -14,6 +14,9 @@
 // Arithmetic follows the reference's operation order; this file must be
 // compiled with -ffp-contract=off and IEEE division / square root.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
+
 #include "psy_device.hpp"
 #include "psy_patch.hpp"
 #include "../../include/psy_gpu.h"
@@ -85,7 +88,9 @@ __global__ void k_pixel_samples(const HaltonDim* dims, const uint16_t* perm, int
 
 // ---------------------------------------------------------------------------
 // Dice cache builder: one lane per split-tree node, replaying the splits
-// from the patch root (object/patch_utils.hpp:38-46, :146-192).
+// from the patch root (object/patch_utils.hpp:38-46, :146-192).  Node k of a
+// patch (1-based heap) lives at entries 2k, 2k+1 of the patch's slot, so the
+// two children of a node form one aligned 64-byte pair.
 // ---------------------------------------------------------------------------
 template <int N>
 __device__ __forceinline__ void load_patch(const float* odata, int32_t off, Patch<N>& p) {
@@ -93,11 +98,11 @@ __device__ __forceinline__ void load_patch(const float* odata, int32_t off, Patc
 }
 
 template <int N>
-__device__ void build_node(const float* odata, const DevObject& ob, uint32_t node, float4* out) {
+__device__ void build_node(const float* odata, const DevObject& ob, uint32_t k, float4* out) {
     Patch<N> p;
     load_patch<N>(odata, ob.data, p);
     float uv[4];
-    patch_at_node<N>(p, node, uv);
+    patch_at_node<N>(p, k, uv);
     V3 lo, hi;
     patch_bound<N>(p, &lo, &hi);
     out[0] = make_float4(lo.x, lo.y, lo.z, (uv[0] + uv[1]) * 0.5f);
@@ -106,15 +111,20 @@ __device__ void build_node(const float* odata, const DevObject& ob, uint32_t nod
 
 __global__ void k_build_dice(const float* odata, const DevObject* objs, const int32_t* patch_ids, int n_patches,
                              int levels, float4* dice) {
-    const int64_t per = ((int64_t)1 << (levels + 1)) - 1;
+    const int64_t per = (int64_t)1 << (levels + 1);
     const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= per * n_patches) return;
     const int pi = (int)(gid / per);
-    const uint32_t node = (uint32_t)(gid % per);
+    const uint32_t k = (uint32_t)(gid % per);
     const DevObject ob = objs[patch_ids[pi]];
-    float4* out = dice + 2 * (ob.dice + node);
-    if (ob.kind == K_BILINEAR) build_node<4>(odata, ob, node, out);
-    else build_node<16>(odata, ob, node, out);
+    float4* out = dice + 2 * (ob.dice + k);
+    if (k == 0) {  // unused slot of the 1-based heap
+        out[0] = make_float4(0.f, 0.f, 0.f, 0.f);
+        out[1] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return;
+    }
+    if (ob.kind == K_BILINEAR) build_node<4>(odata, ob, k, out);
+    else build_node<16>(odata, ob, k, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -126,16 +136,36 @@ struct Best {
     float u, v, aux;
 };
 
-// Split-dice below the cached depth: depth-first over the subtree of `node`
-// (already found to need splitting), replaying splits from that node's
-// patch for every visited node.  Same leaf rule as the cached walk.
+__device__ __forceinline__ V3 xyz(float4 a) { return mk(a.x, a.y, a.z); }
+
+// The leaf rule of intersect_rays_with_patch (object/patch_utils.hpp:76-133):
+// a hit micro-bbox no larger than the ray footprint (times dice_rate) ends
+// the descent; its hit distance is the middle of the slab interval.
+__device__ __forceinline__ bool accept_leaf(float h0, float h1, float u, float v, float maxd, uint32_t inst, Best& b) {
+    const float tt = (h0 + h1) * 0.5f;
+    if (tt > 0.0f && tt < b.t) {
+        b.t = tt;
+        b.inst = inst;
+        b.u = u;
+        b.v = v;
+        b.aux = maxd;
+        return true;
+    }
+    return false;
+}
+
+// Split-dice below the cached depth: depth-first over the subtree of heap
+// node `k` (already found to need splitting), replaying the splits from the
+// patch root for every visited node.  `sidx` is the reference's split-stack
+// index of k (object/patch_utils.hpp:80: a node at stack index 63 is a leaf).
 template <int N>
-__device__ __noinline__ bool dice_deep(const DevScene& s, const DevObject& ob, uint32_t node, int sidx0,
-                                       uint32_t inst, V3 o, V3 dinv, RayW w, bool occl, Best& b) {
+__device__ __noinline__ bool dice_deep(const DevScene& s, const DevObject& ob, uint32_t k0, uint32_t inst, V3 o,
+                                       V3 dinv, RayW w, bool occl, Best& b) {
     Patch<N> base;
     load_patch<N>(s.odata, ob.data, base);
     float uvb[4];
-    patch_at_node<N>(base, node, uvb);
+    patch_at_node<N>(base, k0, uvb);
+    const int sidx0 = __builtin_popcount(k0) - 1;
     int k = 1;
     uint64_t pb = 1;
     int sidx = sidx0 + 1;
@@ -151,15 +181,8 @@ __device__ __noinline__ bool dice_deep(const DevScene& s, const DevObject& ob, u
             const float width = smax(min_width(w, h0, h1) * s.dice_rate, s.min_upoly);
             const float max_dim = longest_axis(sub(hi, lo));
             if (max_dim <= width || sidx == 63 || k == 62) {
-                const float tt = (h0 + h1) * 0.5f;
-                if (tt > 0.0f && tt < b.t) {
-                    b.t = tt;
-                    b.inst = inst;
-                    b.u = (uv[0] + uv[1]) * 0.5f;
-                    b.v = (uv[2] + uv[3]) * 0.5f;
-                    b.aux = max_dim;
-                    if (occl) return true;
-                }
+                if (accept_leaf(h0, h1, (uv[0] + uv[1]) * 0.5f, (uv[2] + uv[3]) * 0.5f, max_dim, inst, b) && occl)
+                    return true;
             } else {
                 down = true;
             }
@@ -174,48 +197,100 @@ __device__ __noinline__ bool dice_deep(const DevScene& s, const DevObject& ob, u
     }
 }
 
-// Walk one patch's cached split tree (object/patch_utils.hpp:36-197 per ray).
+template <bool DEEP>
+__device__ __forceinline__ bool deep_or_flag(const DevScene& s, const DevObject& ob, uint32_t k, uint32_t inst, V3 o,
+                                             V3 dinv, const RayW& w, bool occl, Best& b, bool& need_deep) {
+    if (!DEEP) {
+        need_deep = true;
+        return false;
+    }
+    return ob.kind == K_BILINEAR ? dice_deep<4>(s, ob, k, inst, o, dinv, w, occl, b)
+                                 : dice_deep<16>(s, ob, k, inst, o, dinv, w, occl, b);
+}
+
+// Walk one patch's cached split tree, nearest child first, without a
+// per-ray stack: `trail` keeps one bit per depth whose other child is still
+// pending, and a node's sibling and ancestors follow from its heap index.
+// Any visit order finds the reference's closest hit (the leaf t does not
+// depend on max_t); within a node the second half is taken on ties, as the
+// reference visits it first.  Returns true when an occlusion ray is done.
+template <bool DEEP>
 __device__ __forceinline__ bool dice_walk(const DevScene& s, const DevObject& ob, uint32_t inst, V3 o, V3 dinv,
-                                          const RayW& w, bool occl, Best& b) {
+                                          const RayW& w, bool occl, Best& b, bool& need_deep) {
     const float4* nodes = s.dice + 2 * ob.dice;
     const int levels = ob.dice_levels;
-    uint32_t node = 0;
-    int depth = 0;
-    int sidx = 0;
-    for (;;) {
-        const float4 a = nodes[2 * node];
-        const float4 c = nodes[2 * node + 1];
-        const V3 lo = mk(a.x, a.y, a.z), hi = mk(c.x, c.y, c.z);
+    {
+        const float4 a = nodes[2], c = nodes[3];
         float h0, h1;
-        bool down = false;
-        if (bbox_ray(lo, hi, o, dinv, b.t, &h0, &h1)) {
-            const float width = smax(min_width(w, h0, h1) * s.dice_rate, s.min_upoly);
-            const float max_dim = longest_axis(sub(hi, lo));
-            if (max_dim <= width || sidx == 63) {
-                const float tt = (h0 + h1) * 0.5f;
-                if (tt > 0.0f && tt < b.t) {
-                    b.t = tt;
-                    b.inst = inst;
-                    b.u = a.w;
-                    b.v = c.w;
-                    b.aux = max_dim;
-                    if (occl) return true;
-                }
-            } else if (depth < levels) {
-                down = true;
+        if (!bbox_ray(xyz(a), xyz(c), o, dinv, b.t, &h0, &h1)) return false;
+        const float maxd = longest_axis(sub(xyz(c), xyz(a)));
+        if (maxd <= smax(min_width(w, h0, h1) * s.dice_rate, s.min_upoly))
+            return accept_leaf(h0, h1, a.w, c.w, maxd, inst, b) && occl;
+        if (levels == 0) return deep_or_flag<DEEP>(s, ob, 1, inst, o, dinv, w, occl, b, need_deep);
+    }
+    uint32_t k = 1;
+    int depth = 0;
+    uint32_t trail = 0;
+    for (;;) {
+        // expand k: its children 2k, 2k+1 are at depth+1 <= levels
+        const uint32_t c0 = 2 * k;
+        const float4* q = nodes + 2 * c0;
+        const float4 a0 = q[0], e0 = q[1], a1 = q[2], e1 = q[3];
+        float h00, h01, h10, h11;
+        const bool hit0 = bbox_ray(xyz(a0), xyz(e0), o, dinv, b.t, &h00, &h01);
+        const bool hit1 = bbox_ray(xyz(a1), xyz(e1), o, dinv, b.t, &h10, &h11);
+        bool split0 = false, split1 = false;
+        if (hit1) {
+            const float maxd = longest_axis(sub(xyz(e1), xyz(a1)));
+            if (maxd <= smax(min_width(w, h10, h11) * s.dice_rate, s.min_upoly)) {
+                if (accept_leaf(h10, h11, a1.w, e1.w, maxd, inst, b) && occl) return true;
             } else {
-                const bool done = ob.kind == K_BILINEAR
-                                      ? dice_deep<4>(s, ob, node, sidx, inst, o, dinv, w, occl, b)
-                                      : dice_deep<16>(s, ob, node, sidx, inst, o, dinv, w, occl, b);
-                if (done) return true;
+                split1 = true;
             }
         }
-        if (down) { node = 2 * node + 2; ++depth; ++sidx; continue; }
+        if (hit0) {
+            const float maxd = longest_axis(sub(xyz(e0), xyz(a0)));
+            if (maxd <= smax(min_width(w, h00, h01) * s.dice_rate, s.min_upoly)) {
+                if (accept_leaf(h00, h01, a0.w, e0.w, maxd, inst, b) && occl) return true;
+            } else {
+                split0 = true;
+            }
+        }
+        split0 = split0 && h00 <= b.t;
+        split1 = split1 && h10 <= b.t;
+        const int cd = depth + 1;
+        if (cd == levels) {
+            if (split1 && deep_or_flag<DEEP>(s, ob, c0 + 1, inst, o, dinv, w, occl, b, need_deep)) return true;
+            if (split0 && deep_or_flag<DEEP>(s, ob, c0, inst, o, dinv, w, occl, b, need_deep)) return true;
+            split0 = split1 = false;
+        }
+        if (split0 && split1) {
+            k = (h00 < h10) ? c0 : c0 + 1;
+            trail |= 1u << cd;
+            depth = cd;
+            continue;
+        }
+        if (split0 || split1) {
+            k = split0 ? c0 : c0 + 1;
+            depth = cd;
+            continue;
+        }
+        // resume at the deepest pending sibling
         for (;;) {
-            if (node == 0) return false;
-            if ((node & 1u) == 0) { node -= 1; --sidx; break; }
-            node = (node - 1) >> 1;
-            --depth;
+            if (trail == 0) return false;
+            const int dd = 31 - __builtin_clz(trail);
+            k = (k >> (depth - dd)) ^ 1u;
+            depth = dd;
+            trail &= ~(1u << dd);
+            const float4 a = nodes[2 * k], c = nodes[2 * k + 1];
+            float h0, h1;
+            if (!bbox_ray(xyz(a), xyz(c), o, dinv, b.t, &h0, &h1)) continue;
+            const float maxd = longest_axis(sub(xyz(c), xyz(a)));
+            if (maxd <= smax(min_width(w, h0, h1) * s.dice_rate, s.min_upoly)) {
+                if (accept_leaf(h0, h1, a.w, c.w, maxd, inst, b) && occl) return true;
+                continue;
+            }
+            break;
         }
     }
 }
@@ -292,8 +367,9 @@ struct TRay {
     bool occl;
 };
 
-template <bool WORLD>
-__device__ __forceinline__ bool intersect_leaf(const DevScene& s, uint32_t li, const TRay& r, Best& b) {
+template <bool WORLD, bool DEEP>
+__device__ __forceinline__ bool intersect_leaf(const DevScene& s, uint32_t li, const TRay& r, Best& b,
+                                               bool& need_deep) {
     const DevInstance I = s.inst[li];
     const DevObject ob = s.obj[I.object];
     V3 o = r.o, d = r.d, dinv = r.dinv;
@@ -311,13 +387,12 @@ __device__ __forceinline__ bool intersect_leaf(const DevScene& s, uint32_t li, c
     switch (ob.kind) {
         case K_BILINEAR:
         case K_BICUBIC:
-            return dice_walk(s, ob, li, o, dinv, w, r.occl, b);
+            return dice_walk<DEEP>(s, ob, li, o, dinv, w, r.occl, b, need_deep);
         case K_SPHERE: {
             V3 c; float rad;
             sphere_at(od, ob.ts, 4, r.time, &c, &rad);
             if (sphere_t(o, d, b.t, c, rad, &t)) {
-                b.t = t; b.inst = li; b.aux = 0.0f;
-                b.u = 0.0f; b.v = 0.0f;  // resolved later from the hit point
+                b.t = t; b.inst = li; b.aux = 0.0f; b.u = 0.0f; b.v = 0.0f;
                 return r.occl;
             }
             return false;
@@ -344,7 +419,7 @@ __device__ __forceinline__ bool intersect_leaf(const DevScene& s, uint32_t li, c
 
 // Latitude / longitude parameters of a sphere hit, computed once for the
 // final hit (object/sphere.cpp:131-150).
-__device__ __noinline__ void sphere_uv(const DevScene& s, Best& b, const TRay& r) {
+__device__ __forceinline__ void sphere_uv(const DevScene& s, Best& b, const TRay& r) {
     const DevInstance I = s.inst[b.inst];
     if (I.kind != K_SPHERE) return;
     const DevObject ob = s.obj[I.object];
@@ -370,17 +445,21 @@ __device__ __noinline__ void sphere_uv(const DevScene& s, Best& b, const TRay& r
     b.v = (float)((double)latitude / 3.1415926535897932384626433);
 }
 
+__device__ __forceinline__ void cmpswap(float& ka, int32_t& ra, float& kb, int32_t& rb) {
+    if (kb < ka) {
+        const float tk = ka; ka = kb; kb = tk;
+        const int32_t tr = ra; ra = rb; rb = tr;
+    }
+}
+
 constexpr int TRACE_BLOCK = 128;
 
-template <bool WORLD, int STACK>
-__global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DevScene s, const float* __restrict__ rays, int64_t n,
-                                                        DevHits h, uint32_t flags) {
-    __shared__ int32_t stk_ref[STACK][TRACE_BLOCK];
-    __shared__ float stk_near[STACK][TRACE_BLOCK];
-    const int tid = threadIdx.x;
-    const int64_t i = (int64_t)blockIdx.x * TRACE_BLOCK + tid;
-    if (i >= n) return;
-
+// One ray: BVH4 nearest-first with an LDS stack, leaves through
+// intersect_leaf.  Returns true if the ray needs the below-cache pass.
+template <bool WORLD, int STACK, bool DEEP>
+__device__ __forceinline__ bool trace_one(const DevScene& s, const float* __restrict__ rays, int64_t n, int64_t i,
+                                          const DevHits& h, uint32_t flags, int32_t (*stk_ref)[TRACE_BLOCK],
+                                          float (*stk_near)[TRACE_BLOCK], int tid) {
     TRay r;
     float max_t;
     if (WORLD) {
@@ -416,6 +495,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DevScene s, const float* 
     b.t = max_t;
     b.inst = 0xffffffffu;
     b.u = b.v = b.aux = 0.0f;
+    bool need_deep = false;
 
     int sp = 0;
     if (s.root != CHILD_EMPTY) {
@@ -423,51 +503,50 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DevScene s, const float* 
         stk_near[0][tid] = 0.0f;
         sp = 1;
     }
-    const int ds0 = r.dinv.x < 0.0f, ds1 = r.dinv.y < 0.0f, ds2 = r.dinv.z < 0.0f;
+    const bool ds0 = r.dinv.x < 0.0f, ds1 = r.dinv.y < 0.0f, ds2 = r.dinv.z < 0.0f;
     while (sp > 0) {
         --sp;
         const int32_t ref = stk_ref[sp][tid];
         const float nearv = stk_near[sp][tid];
         if (!(nearv < b.t)) continue;
         if (ref >= 0) {
-            const BvhNode* nd = s.bvh + ref;
-            const float4* q = reinterpret_cast<const float4*>(nd);
+            const float4* q = reinterpret_cast<const float4*>(s.bvh + ref);
             const float4 lx = q[0], hx = q[1], ly = q[2], hy = q[3], lz = q[4], hz = q[5];
-            const int4 ch = reinterpret_cast<const int4*>(nd)[6];
-            const float lxs[4] = {lx.x, lx.y, lx.z, lx.w}, hxs[4] = {hx.x, hx.y, hx.z, hx.w};
-            const float lys[4] = {ly.x, ly.y, ly.z, ly.w}, hys[4] = {hy.x, hy.y, hy.z, hy.w};
-            const float lzs[4] = {lz.x, lz.y, lz.z, lz.w}, hzs[4] = {hz.x, hz.y, hz.z, hz.w};
-            const int32_t chs[4] = {ch.x, ch.y, ch.z, ch.w};
-            float nr[4];
-            int32_t cr[4];
-            int cnt = 0;
-#pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                if (chs[c] == CHILD_EMPTY) continue;
-                // BBox4::intersect_ray lane (basics/bbox.hpp:566-595)
-                const float xlo = ((ds0 ? hxs[c] : lxs[c]) - r.o.x) * r.dinv.x;
-                const float xhi = ((ds0 ? lxs[c] : hxs[c]) - r.o.x) * r.dinv.x;
-                const float ylo = ((ds1 ? hys[c] : lys[c]) - r.o.y) * r.dinv.y;
-                const float yhi = ((ds1 ? lys[c] : hys[c]) - r.o.y) * r.dinv.y;
-                const float zlo = ((ds2 ? hzs[c] : lzs[c]) - r.o.z) * r.dinv.z;
-                const float zhi = ((ds2 ? lzs[c] : hzs[c]) - r.o.z) * r.dinv.z;
-                const float mins = fmaxf(fmaxf(xlo, ylo), fmaxf(zlo, 0.0f));
-                float maxs = fminf(fminf(xhi, yhi), zhi) * PSY_BBOX_MAXT_ADJUST;
-                if ((mins < b.t) && (mins <= maxs)) {
-                    // insertion sort, farthest first
-                    int k = cnt++;
-                    while (k > 0 && nr[k - 1] < mins) { nr[k] = nr[k - 1]; cr[k] = cr[k - 1]; --k; }
-                    nr[k] = mins;
-                    cr[k] = chs[c];
-                }
-            }
-            for (int k = 0; k < cnt; ++k) {
-                stk_ref[sp][tid] = cr[k];
-                stk_near[sp][tid] = nr[k];
-                ++sp;
-            }
+            const int4 ch = reinterpret_cast<const int4*>(s.bvh + ref)[6];
+            // BBox4::intersect_ray lanes (basics/bbox.hpp:566-595); keys are
+            // the near distances of hit children, +inf otherwise
+#define PSY_LANE(L, KEY)                                                                             \
+    float KEY;                                                                                       \
+    {                                                                                                \
+        const float xlo = ((ds0 ? hx.L : lx.L) - r.o.x) * r.dinv.x;                                  \
+        const float xhi = ((ds0 ? lx.L : hx.L) - r.o.x) * r.dinv.x;                                  \
+        const float ylo = ((ds1 ? hy.L : ly.L) - r.o.y) * r.dinv.y;                                  \
+        const float yhi = ((ds1 ? ly.L : hy.L) - r.o.y) * r.dinv.y;                                  \
+        const float zlo = ((ds2 ? hz.L : lz.L) - r.o.z) * r.dinv.z;                                  \
+        const float zhi = ((ds2 ? lz.L : hz.L) - r.o.z) * r.dinv.z;                                  \
+        const float mins = fmaxf(fmaxf(xlo, ylo), fmaxf(zlo, 0.0f));                                 \
+        const float maxs = fminf(fminf(xhi, yhi), zhi) * PSY_BBOX_MAXT_ADJUST;                       \
+        KEY = ((mins < b.t) && (mins <= maxs)) ? mins : INFINITY;                                    \
+    }
+            PSY_LANE(x, k0)
+            PSY_LANE(y, k1)
+            PSY_LANE(z, k2)
+            PSY_LANE(w, k3)
+#undef PSY_LANE
+            int32_t r0 = ch.x, r1 = ch.y, r2 = ch.z, r3 = ch.w;
+            // sorting network, ascending
+            cmpswap(k0, r0, k1, r1);
+            cmpswap(k2, r2, k3, r3);
+            cmpswap(k0, r0, k2, r2);
+            cmpswap(k1, r1, k3, r3);
+            cmpswap(k1, r1, k2, r2);
+            // push farthest first so the nearest is popped next
+            if (k3 < INFINITY) { stk_ref[sp][tid] = r3; stk_near[sp][tid] = k3; ++sp; }
+            if (k2 < INFINITY) { stk_ref[sp][tid] = r2; stk_near[sp][tid] = k2; ++sp; }
+            if (k1 < INFINITY) { stk_ref[sp][tid] = r1; stk_near[sp][tid] = k1; ++sp; }
+            if (k0 < INFINITY) { stk_ref[sp][tid] = r0; stk_near[sp][tid] = k0; ++sp; }
         } else {
-            if (intersect_leaf<WORLD>(s, (uint32_t)(~ref), r, b)) break;
+            if (intersect_leaf<WORLD, DEEP>(s, (uint32_t)(~ref), r, b, need_deep)) break;
         }
     }
     const bool hit = b.inst != 0xffffffffu;
@@ -477,6 +556,258 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DevScene s, const float* 
     h.u[i] = b.u;
     h.v[i] = b.v;
     if (h.aux) h.aux[i] = b.aux;
+    return need_deep && !(r.occl && hit);
+}
+
+// Main-pass form of trace_one.  One loop iteration does one unit of work --
+// one BVH4 node, or one step of a patch's split-tree walk (expand a node's
+// two children, or resume at a pending sibling) -- so the 64 lanes of a wave
+// advance independently instead of waiting at every BVH leaf for the
+// longest split-tree walk in the wave.  Rays that reach the bottom of the
+// dice cache with a node still to split are reported for the second pass.
+template <bool WORLD, int STACK>
+__device__ __forceinline__ bool trace_one_fast(const DevScene& s, const float* __restrict__ rays, int64_t n,
+                                               int64_t i, const DevHits& h, uint32_t flags,
+                                               int32_t (*stk_ref)[TRACE_BLOCK], float (*stk_near)[TRACE_BLOCK],
+                                               int tid) {
+    TRay r;
+    float max_t;
+    if (WORLD) {
+        r.o = mk(rays[0 * n + i], rays[1 * n + i], rays[2 * n + i]);
+        r.d = mk(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]);
+        r.odx = mk(rays[6 * n + i], rays[7 * n + i], rays[8 * n + i]);
+        r.ody = mk(rays[9 * n + i], rays[10 * n + i], rays[11 * n + i]);
+        r.ddx = mk(rays[12 * n + i], rays[13 * n + i], rays[14 * n + i]);
+        r.ddy = mk(rays[15 * n + i], rays[16 * n + i], rays[17 * n + i]);
+        r.time = rays[18 * n + i];
+        const uint32_t type = __float_as_uint(rays[19 * n + i]);
+        r.occl = type == PSY_RAY_OCCLUSION;
+        max_t = r.occl ? 1.0f : INFINITY;
+        r.w = ray_widths(r.o, r.d, r.odx, r.ody, r.ddx, r.ddy);
+    } else {
+        r.o = mk(rays[0 * n + i], rays[1 * n + i], rays[2 * n + i]);
+        r.d = mk(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]);
+        max_t = rays[6 * n + i];
+        r.time = rays[7 * n + i];
+        r.w.owx = rays[8 * n + i];
+        r.w.owy = rays[9 * n + i];
+        r.w.dwx = rays[10 * n + i];
+        r.w.dwy = rays[11 * n + i];
+        r.w.fwx = rays[12 * n + i];
+        r.w.fwy = rays[13 * n + i];
+        r.occl = (__float_as_uint(rays[14 * n + i]) & (1u << 30)) != 0;
+        r.odx = r.ody = r.ddx = r.ddy = splat(0.0f);
+    }
+    if (flags & PSY_TRACE_OCCLUSION_ONLY) r.occl = true;
+    r.dinv = divv(splat(1.0f), r.d);
+
+    Best b;
+    b.t = max_t;
+    b.inst = 0xffffffffu;
+    b.u = b.v = b.aux = 0.0f;
+    bool need_deep = false;
+
+    // split-tree walk state (valid while `walking`)
+    bool walking = false;
+    bool expand = true;      // true: expand k; false: resume at a pending sibling
+    const float4* nodes = nullptr;
+    int levels = 0;
+    uint32_t winst = 0, k = 1, trail = 0;
+    int depth = 0;
+    V3 wo = r.o, wdinv = r.dinv;
+    RayW ww = r.w;
+
+    int sp = 0;
+    if (s.root != CHILD_EMPTY) {
+        stk_ref[0][tid] = s.root;
+        stk_near[0][tid] = 0.0f;
+        sp = 1;
+    }
+    const bool ds0 = r.dinv.x < 0.0f, ds1 = r.dinv.y < 0.0f, ds2 = r.dinv.z < 0.0f;
+    bool done = false;
+    for (;;) {
+        if (walking) {
+            if (expand) {
+                const uint32_t c0 = 2 * k;
+                const float4* q = nodes + 2 * c0;
+                const float4 a0 = q[0], e0 = q[1], a1 = q[2], e1 = q[3];
+                float h00, h01, h10, h11;
+                const bool hit0 = bbox_ray(xyz(a0), xyz(e0), wo, wdinv, b.t, &h00, &h01);
+                const bool hit1 = bbox_ray(xyz(a1), xyz(e1), wo, wdinv, b.t, &h10, &h11);
+                bool split0 = false, split1 = false;
+                if (hit1) {
+                    const float maxd = longest_axis(sub(xyz(e1), xyz(a1)));
+                    if (maxd <= smax(min_width(ww, h10, h11) * s.dice_rate, s.min_upoly)) {
+                        if (accept_leaf(h10, h11, a1.w, e1.w, maxd, winst, b) && r.occl) { done = true; break; }
+                    } else {
+                        split1 = true;
+                    }
+                }
+                if (hit0) {
+                    const float maxd = longest_axis(sub(xyz(e0), xyz(a0)));
+                    if (maxd <= smax(min_width(ww, h00, h01) * s.dice_rate, s.min_upoly)) {
+                        if (accept_leaf(h00, h01, a0.w, e0.w, maxd, winst, b) && r.occl) { done = true; break; }
+                    } else {
+                        split0 = true;
+                    }
+                }
+                split0 = split0 && h00 <= b.t;
+                split1 = split1 && h10 <= b.t;
+                const int cd = depth + 1;
+                if (cd == levels) {
+                    if (split0 || split1) need_deep = true;
+                    split0 = split1 = false;
+                }
+                if (split0 && split1) {
+                    k = (h00 < h10) ? c0 : c0 + 1;
+                    trail |= 1u << cd;
+                    depth = cd;
+                } else if (split0 || split1) {
+                    k = split0 ? c0 : c0 + 1;
+                    depth = cd;
+                } else {
+                    expand = false;
+                }
+            } else {
+                if (trail == 0) {
+                    walking = false;
+                } else {
+                    const int dd = 31 - __builtin_clz(trail);
+                    k = (k >> (depth - dd)) ^ 1u;
+                    depth = dd;
+                    trail &= ~(1u << dd);
+                    const float4 a = nodes[2 * k], c = nodes[2 * k + 1];
+                    float h0, h1;
+                    if (bbox_ray(xyz(a), xyz(c), wo, wdinv, b.t, &h0, &h1)) {
+                        const float maxd = longest_axis(sub(xyz(c), xyz(a)));
+                        if (maxd <= smax(min_width(ww, h0, h1) * s.dice_rate, s.min_upoly)) {
+                            if (accept_leaf(h0, h1, a.w, c.w, maxd, winst, b) && r.occl) { done = true; break; }
+                        } else {
+                            expand = true;
+                        }
+                    }
+                }
+            }
+            continue;
+        }
+        if (sp == 0) break;
+        --sp;
+        const int32_t ref = stk_ref[sp][tid];
+        const float nearv = stk_near[sp][tid];
+        if (!(nearv < b.t)) continue;
+        if (ref >= 0) {
+            const float4* q = reinterpret_cast<const float4*>(s.bvh + ref);
+            const float4 lx = q[0], hx = q[1], ly = q[2], hy = q[3], lz = q[4], hz = q[5];
+            const int4 ch = reinterpret_cast<const int4*>(s.bvh + ref)[6];
+#define PSY_LANE(L, KEY)                                                                             \
+    float KEY;                                                                                       \
+    {                                                                                                \
+        const float xlo = ((ds0 ? hx.L : lx.L) - r.o.x) * r.dinv.x;                                  \
+        const float xhi = ((ds0 ? lx.L : hx.L) - r.o.x) * r.dinv.x;                                  \
+        const float ylo = ((ds1 ? hy.L : ly.L) - r.o.y) * r.dinv.y;                                  \
+        const float yhi = ((ds1 ? ly.L : hy.L) - r.o.y) * r.dinv.y;                                  \
+        const float zlo = ((ds2 ? hz.L : lz.L) - r.o.z) * r.dinv.z;                                  \
+        const float zhi = ((ds2 ? lz.L : hz.L) - r.o.z) * r.dinv.z;                                  \
+        const float mins = fmaxf(fmaxf(xlo, ylo), fmaxf(zlo, 0.0f));                                 \
+        const float maxs = fminf(fminf(xhi, yhi), zhi) * PSY_BBOX_MAXT_ADJUST;                       \
+        KEY = ((mins < b.t) && (mins <= maxs)) ? mins : INFINITY;                                    \
+    }
+            PSY_LANE(x, k0)
+            PSY_LANE(y, k1)
+            PSY_LANE(z, k2)
+            PSY_LANE(w, k3)
+#undef PSY_LANE
+            int32_t r0 = ch.x, r1 = ch.y, r2 = ch.z, r3 = ch.w;
+            cmpswap(k0, r0, k1, r1);
+            cmpswap(k2, r2, k3, r3);
+            cmpswap(k0, r0, k2, r2);
+            cmpswap(k1, r1, k3, r3);
+            cmpswap(k1, r1, k2, r2);
+            if (k3 < INFINITY) { stk_ref[sp][tid] = r3; stk_near[sp][tid] = k3; ++sp; }
+            if (k2 < INFINITY) { stk_ref[sp][tid] = r2; stk_near[sp][tid] = k2; ++sp; }
+            if (k1 < INFINITY) { stk_ref[sp][tid] = r1; stk_near[sp][tid] = k1; ++sp; }
+            if (k0 < INFINITY) { stk_ref[sp][tid] = r0; stk_near[sp][tid] = k0; ++sp; }
+            continue;
+        }
+        // leaf instance
+        const uint32_t li = (uint32_t)(~ref);
+        const DevInstance I = s.inst[li];
+        const DevObject ob = s.obj[I.object];
+        if (ob.kind == K_BILINEAR || ob.kind == K_BICUBIC) {
+            wo = r.o; wdinv = r.dinv; ww = r.w;
+            if (I.xf_count > 0) {
+                const M44 x = m_lerp_seq(r.time, s.xf + I.xf_offset, I.xf_count);
+                wo = m_pos(x, r.o);
+                const V3 d = m_dir(x, r.d);
+                wdinv = divv(splat(1.0f), d);
+                if (WORLD) ww = ray_widths(wo, d, m_dir(x, r.odx), m_dir(x, r.ody), m_dir(x, r.ddx), m_dir(x, r.ddy));
+            }
+            nodes = s.dice + 2 * ob.dice;
+            levels = ob.dice_levels;
+            const float4 a = nodes[2], c = nodes[3];
+            float h0, h1;
+            if (!bbox_ray(xyz(a), xyz(c), wo, wdinv, b.t, &h0, &h1)) continue;
+            const float maxd = longest_axis(sub(xyz(c), xyz(a)));
+            if (maxd <= smax(min_width(ww, h0, h1) * s.dice_rate, s.min_upoly)) {
+                if (accept_leaf(h0, h1, a.w, c.w, maxd, li, b) && r.occl) { done = true; break; }
+                continue;
+            }
+            if (levels == 0) { need_deep = true; continue; }
+            walking = true;
+            expand = true;
+            winst = li;
+            k = 1;
+            depth = 0;
+            trail = 0;
+            continue;
+        }
+        bool unused = false;
+        if (intersect_leaf<WORLD, false>(s, li, r, b, unused)) { done = true; break; }
+    }
+    (void)done;
+    const bool hit = b.inst != 0xffffffffu;
+    if (hit && !r.occl) sphere_uv(s, b, r);
+    h.t[i] = hit ? b.t : INFINITY;
+    h.inst[i] = b.inst;
+    h.u[i] = b.u;
+    h.v[i] = b.v;
+    if (h.aux) h.aux[i] = b.aux;
+    return need_deep && !(r.occl && hit);
+}
+
+// Main pass: every ray; rays whose walk reaches the bottom of the dice cache
+// with a node still to split are queued (once each) for the second pass.
+template <bool WORLD, int STACK>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DevScene s, const float* __restrict__ rays, int64_t n,
+                                                        DevHits h, uint32_t flags, uint32_t* __restrict__ queue,
+                                                        uint32_t* __restrict__ qcount) {
+    __shared__ int32_t stk_ref[STACK][TRACE_BLOCK];
+    __shared__ float stk_near[STACK][TRACE_BLOCK];
+    const int tid = threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * TRACE_BLOCK + tid;
+    if (i >= n) return;
+    if (trace_one_fast<WORLD, STACK>(s, rays, n, i, h, flags, stk_ref, stk_near, tid)) {
+        const uint32_t slot = atomicAdd(qcount, 1u);
+        queue[slot] = (uint32_t)i;
+    }
+}
+
+// Second pass: retrace the queued rays with subdivision below the cache.
+template <bool WORLD, int STACK>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_deep(DevScene s, const float* __restrict__ rays, int64_t n,
+                                                             DevHits h, uint32_t flags,
+                                                             const uint32_t* __restrict__ queue,
+                                                             const uint32_t* __restrict__ qcount) {
+    __shared__ int32_t stk_ref[STACK][TRACE_BLOCK];
+    __shared__ float stk_near[STACK][TRACE_BLOCK];
+    const int tid = threadIdx.x;
+    const uint32_t count = *qcount;
+    for (uint32_t k = blockIdx.x * TRACE_BLOCK + tid; k < count; k += gridDim.x * TRACE_BLOCK)
+        (void)trace_one<WORLD, STACK, true>(s, rays, n, (int64_t)queue[k], h, flags, stk_ref, stk_near, tid);
+}
+
+__global__ void k_zero_u32(uint32_t* p) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -749,7 +1080,7 @@ static int check_launch() { return hipGetLastError() == hipSuccess ? 0 : PSY_ERR
 
 int launch_build_dice(const float* odata, const DevObject* objs, const int32_t* patch_ids, int n_patches,
                       int levels, float4* dice, hipStream_t st) {
-    const int64_t per = ((int64_t)1 << (levels + 1)) - 1;
+    const int64_t per = (int64_t)1 << (levels + 1);
     const int64_t total = per * n_patches;
     if (total <= 0) return 0;
     hipLaunchKernelGGL(k_build_dice, dim3(grid_for(total, 256)), dim3(256), 0, st, odata, objs, patch_ids,
@@ -757,12 +1088,22 @@ int launch_build_dice(const float* odata, const DevObject* objs, const int32_t* 
     return check_launch();
 }
 
+template <bool W, int S>
+static void launch_trace_pair(const DevScene& s, const float* rays, int64_t n, const DevHits& h, uint32_t flags,
+                              uint32_t* queue, uint32_t* qcount, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, qcount);
+    hipLaunchKernelGGL((k_trace<W, S>), dim3(grid_for(n, TRACE_BLOCK)), dim3(TRACE_BLOCK), 0, st, s, rays, n, h,
+                       flags, queue, qcount);
+    const unsigned deep_blocks = (unsigned)std::min<int64_t>(grid_for(n, TRACE_BLOCK), 1024);
+    hipLaunchKernelGGL((k_trace_deep<W, S>), dim3(deep_blocks), dim3(TRACE_BLOCK), 0, st, s, rays, n, h, flags,
+                       (const uint32_t*)queue, (const uint32_t*)qcount);
+}
+
 int launch_trace(const DevScene& s, bool world, bool occl_only, int stack_need, const float* rays, int64_t n,
-                 const DevHits& h, hipStream_t st) {
+                 const DevHits& h, uint32_t* queue, uint32_t* qcount, hipStream_t st) {
     if (n <= 0) return 0;
     const uint32_t flags = occl_only ? PSY_TRACE_OCCLUSION_ONLY : 0u;
-    const dim3 grid(grid_for(n, TRACE_BLOCK)), block(TRACE_BLOCK);
-#define PSY_TRACE_CASE(W, S) hipLaunchKernelGGL((k_trace<W, S>), grid, block, 0, st, s, rays, n, h, flags)
+#define PSY_TRACE_CASE(W, S) launch_trace_pair<W, S>(s, rays, n, h, flags, queue, qcount, st)
     if (stack_need <= 16) { if (world) PSY_TRACE_CASE(true, 16); else PSY_TRACE_CASE(false, 16); }
     else if (stack_need <= 32) { if (world) PSY_TRACE_CASE(true, 32); else PSY_TRACE_CASE(false, 32); }
     else if (stack_need <= 64) { if (world) PSY_TRACE_CASE(true, 64); else PSY_TRACE_CASE(false, 64); }

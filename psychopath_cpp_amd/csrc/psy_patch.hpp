@@ -3,8 +3,7 @@
 // (object/patch_utils.hpp:146-192: split U when ulen > vlen of time sample
 // 0, the halves by midpoint subdivision), so every node of that tree can be
 // produced independently by replaying the splits along its path from the
-// root.  Node numbering is heap order: node n has children 2n+1 (first half,
-// lower u or v) and 2n+2 (second half).
+// root.
 //   Bilinear ops: object/bilinear.hpp:44-141; Bicubic: object/bicubic.hpp:55-189.
 #pragma once
 
@@ -91,16 +90,17 @@ PSY_HD void patch_step(Patch<N>& p, bool second, float uv[4]) {
     }
 }
 
-// Heap index -> (depth, path bits MSB-first; bit 1 = second half).
-PSY_HD int heap_depth(uint32_t node) { return 31 - __builtin_clz(node + 1); }
+// Heap numbering, 1-based: node k has children 2k (first half) and 2k+1
+// (second half); its depth is floor(log2 k) and its path from the root is
+// the bits of k below the leading one, MSB first (1 = second half).
+PSY_HD int heap_depth(uint32_t k) { return 31 - __builtin_clz(k); }
 
-// Replay the splits from `root` down to heap node `node`.
+// Replay the splits from `root` down to heap node k.
 template <int N>
-PSY_HD void patch_at_node(Patch<N>& p, uint32_t node, float uv[4]) {
+PSY_HD void patch_at_node(Patch<N>& p, uint32_t k, float uv[4]) {
     uv[0] = 0.0f; uv[1] = 1.0f; uv[2] = 0.0f; uv[3] = 1.0f;
-    const int depth = heap_depth(node);
-    const uint32_t path = (node + 1) - (1u << depth);
-    for (int l = depth - 1; l >= 0; --l) patch_step(p, ((path >> l) & 1u) != 0, uv);
+    const int depth = heap_depth(k);
+    for (int l = depth - 1; l >= 0; --l) patch_step(p, ((k >> l) & 1u) != 0, uv);
 }
 
 // Differential geometry at (u, v) of the un-split patch.

@@ -71,6 +71,10 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1 << 21)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--dice-depth", type=int, default=-1)
+    ap.add_argument("--trace-mode", choices=("persistent", "plain"), default="persistent",
+                    help="main trace pass: persistent waves with lane refill, or one thread per ray")
+    ap.add_argument("--refill", type=int, default=16, help="persistent mode: refill a wave once this many lanes idle")
+    ap.add_argument("--blocks-per-cu", type=int, default=0, help="persistent mode: blocks per CU (0: occupancy)")
     ap.add_argument("--split", action="store_true",
                     help="also time the primary and the bounce half of the stream separately (stderr)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -96,6 +100,8 @@ def main():
 
     log("setting up heightfield scene")
     ctx = gpu.Context(0)
+    ctx.set_trace_mode(persistent=args.trace_mode == "persistent", refill_min=args.refill,
+                       blocks_per_cu=args.blocks_per_cu)
     scene_py = configs.heightfield_scene()
     t0 = time.perf_counter()
     scene = gpu.GpuScene(ctx, scene_py, dice_depth=args.dice_depth)
@@ -128,7 +134,8 @@ def main():
     mrays = STREAM_RAYS / (ms / 1e3) / 1e6
     achieved = BYTES_PER_RAY * STREAM_RAYS / (ms / 1e3) / 1e9
     hit_frac = float((hits.inst != -1).float().mean().item())
-    log(f"trace: {ms:.3f} ms/step (HIP events), wall {wall / args.steps * 1e3:.3f} ms/step, hit fraction {hit_frac:.4f}")
+    log(f"trace ({args.trace_mode}): {ms:.3f} ms/step (HIP events), wall {wall / args.steps * 1e3:.3f} ms/step, "
+        f"hit fraction {hit_frac:.4f}, second pass {ctx.retrace_count()} rays")
 
     if args.split:
         half = STREAM_RAYS // 2
@@ -143,7 +150,8 @@ def main():
             ev1.record(stream)
             torch.cuda.synchronize()
             pms = ev0.elapsed_time(ev1) / args.steps
-            log(f"split {name}: {pms:.3f} ms for {half} rays = {half / (pms / 1e3) / 1e6:.1f} Mrays/s")
+            log(f"split {name}: {pms:.3f} ms for {half} rays = {half / (pms / 1e3) / 1e6:.1f} Mrays/s, "
+                f"second pass {ctx.retrace_count()} rays")
 
     cpu = None
     if not args.no_cpu:
@@ -187,6 +195,7 @@ def main():
             "dice_depth": int(info.dice_depth),
             "leaf_instances": int(info.n_leaf_instances),
             "bytes_per_ray": BYTES_PER_RAY,
+            "trace_mode": args.trace_mode,
         },
         "roofline": {
             "bound": "hbm",

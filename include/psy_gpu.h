@@ -153,6 +153,19 @@ typedef struct psy_scene_info {
  * the GPU context owns device memory, scene uploads and kernels. */
 int psy_ctx_create(int device, psy_ctx** out);
 int psy_ctx_destroy(psy_ctx* ctx);
+/* Launch form of the trace's main pass (no counterpart in the reference,
+ * whose tracer is a per-thread CPU loop).  PSY_TRACE_MODE_PERSISTENT
+ * (default): one grid that fills the GPU; each wave refills its finished
+ * lanes from a global ray counter once refill_min (1..64, default 16) of its
+ * 64 lanes are idle; blocks_per_cu <= 0 takes the kernel's occupancy.
+ * PSY_TRACE_MODE_PLAIN: one thread per ray.  Results are identical. */
+#define PSY_TRACE_MODE_PLAIN 0
+#define PSY_TRACE_MODE_PERSISTENT 1
+int psy_ctx_set_trace_mode(psy_ctx* ctx, int mode, int refill_min, int blocks_per_cu);
+/* Rays of the latest trace call on this context that the main pass handed
+ * to the second pass (split below the dice cache, or a BVH stack deeper
+ * than the main pass keeps).  Synchronizes the device; for diagnostics. */
+int psy_ctx_retrace_count(psy_ctx* ctx, int64_t* out);
 const char* psy_last_error(void);
 int psy_abi_version(void);
 

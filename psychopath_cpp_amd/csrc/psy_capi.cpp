@@ -149,17 +149,30 @@ HaltonTables make_halton_tables() {
 struct psy_ctx {
     int device = 0;
     DBuf halton_dims, halton_perm;
-    // trace scratch: queue of rays for the below-cache pass + its counter
-    DBuf queue, qcount;
+    // trace scratch: queue of rays for the second pass + its counters
+    DBuf queue, counters;
     int64_t queue_cap = 0;
+    // main-pass launch form, per ray form (0: object rays, 1: world rays)
+    int n_cu = 0;
+    int occupancy[2] = {0, 0};
+    int mode = PSY_TRACE_MODE_PERSISTENT;
+    int refill_min = 16;
+    int blocks_per_cu = 0;  // <= 0: from occupancy
+    TraceTuning tuning(bool world) const {
+        TraceTuning t;
+        t.persistent = mode == PSY_TRACE_MODE_PERSISTENT;
+        t.persist_blocks = n_cu * (blocks_per_cu > 0 ? blocks_per_cu : occupancy[world ? 1 : 0]);
+        t.refill_min = refill_min;
+        return t;
+    }
 };
 
 namespace {
 // Grows the per-context trace scratch to n entries.  Calls on one context
 // must not overlap in time (they share this scratch).
 int ensure_queue(psy_ctx* ctx, int64_t n) {
-    if (!ctx->qcount.p) {
-        int rc = ctx->qcount.alloc(sizeof(uint32_t) * 4);
+    if (!ctx->counters.p) {
+        int rc = ctx->counters.alloc(sizeof(uint32_t) * 4);
         if (rc) return rc;
     }
     if (n <= ctx->queue_cap) return PSY_OK;
@@ -204,7 +217,34 @@ int psy_ctx_create(int device, psy_ctx** out) {
     if (rc) return rc;
     rc = c->halton_perm.upload(t.perm.data(), t.perm.size() * sizeof(uint16_t));
     if (rc) return rc;
+    HIP_OK(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device));
+    if (trace_persist_occupancy(false, &c->occupancy[0]) || trace_persist_occupancy(true, &c->occupancy[1]))
+        return fail(PSY_ERR_HIP, "psy_ctx_create: occupancy query failed");
     *out = c.release();
+    return PSY_OK;
+}
+
+int psy_ctx_set_trace_mode(psy_ctx* ctx, int mode, int refill_min, int blocks_per_cu) {
+    if (!ctx) return fail(PSY_ERR_INVALID, "psy_ctx_set_trace_mode: NULL context");
+    if (mode != PSY_TRACE_MODE_PLAIN && mode != PSY_TRACE_MODE_PERSISTENT)
+        return fail(PSY_ERR_INVALID, "psy_ctx_set_trace_mode: unknown mode");
+    if (refill_min < 1 || refill_min > 64) return fail(PSY_ERR_INVALID, "psy_ctx_set_trace_mode: refill_min not in 1..64");
+    if (blocks_per_cu > 64) return fail(PSY_ERR_INVALID, "psy_ctx_set_trace_mode: blocks_per_cu above 64");
+    ctx->mode = mode;
+    ctx->refill_min = refill_min;
+    ctx->blocks_per_cu = blocks_per_cu;
+    return PSY_OK;
+}
+
+int psy_ctx_retrace_count(psy_ctx* ctx, int64_t* out) {
+    if (!ctx || !out) return fail(PSY_ERR_INVALID, "psy_ctx_retrace_count: NULL argument");
+    *out = 0;
+    if (!ctx->counters.p) return PSY_OK;
+    HIP_OK(hipSetDevice(ctx->device));
+    HIP_OK(hipDeviceSynchronize());
+    uint32_t c[4];
+    HIP_OK(hipMemcpy(c, ctx->counters.p, sizeof c, hipMemcpyDeviceToHost));
+    *out = (int64_t)c[0];
     return PSY_OK;
 }
 
@@ -630,8 +670,8 @@ int psy_scene_create(psy_ctx* ctx, const psy_scene_desc* d, int dice_depth, psy_
         if (objs[i].kind == PSY_OBJ_BILINEAR || objs[i].kind == PSY_OBJ_BICUBIC) patch_ids.push_back((int32_t)i);
     int levels = dice_depth;
     if (levels < 0) {
-        const double budget = 1024.0 * 1024.0 * 1024.0;  // 1 GiB of split-tree nodes
-        levels = 14;
+        const double budget = 4.0 * 1024.0 * 1024.0 * 1024.0;  // 4 GiB of split-tree nodes
+        levels = 16;
         while (levels > 0 && (double)((int64_t)1 << (levels + 1)) * 32.0 * (double)patch_ids.size() > budget) --levels;
     }
     if (levels > 20) return fail(PSY_ERR_INVALID, "psy_scene_create: dice_depth above 20");
@@ -757,7 +797,7 @@ int psy_trace_world(psy_ctx* ctx, const psy_scene* s, const float* rays, int64_t
     HIP_OK(hipSetDevice(ctx->device));
     if (int rc = ensure_queue(ctx, n)) return rc;
     int rc = launch_trace(s->dev, true, (flags & PSY_TRACE_OCCLUSION_ONLY) != 0, s->stack_need, rays, n, to_dev(hits),
-                          (uint32_t*)ctx->queue.p, (uint32_t*)ctx->qcount.p, (hipStream_t)stream);
+                          (uint32_t*)ctx->queue.p, (uint32_t*)ctx->counters.p, ctx->tuning(true), (hipStream_t)stream);
     return rc ? fail(rc, "psy_trace_world: launch failed") : PSY_OK;
 }
 
@@ -771,7 +811,8 @@ int psy_trace_rays(psy_ctx* ctx, const psy_scene* s, const float* rays, int64_t 
     HIP_OK(hipSetDevice(ctx->device));
     if (int rc = ensure_queue(ctx, n)) return rc;
     int rc = launch_trace(s->dev, false, (flags & PSY_TRACE_OCCLUSION_ONLY) != 0, s->stack_need, rays, n,
-                          to_dev(hits), (uint32_t*)ctx->queue.p, (uint32_t*)ctx->qcount.p, (hipStream_t)stream);
+                          to_dev(hits), (uint32_t*)ctx->queue.p, (uint32_t*)ctx->counters.p, ctx->tuning(false),
+                          (hipStream_t)stream);
     return rc ? fail(rc, "psy_trace_rays: launch failed") : PSY_OK;
 }
 

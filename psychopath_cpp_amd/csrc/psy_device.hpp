@@ -95,9 +95,20 @@ struct DevHits { float* t; uint32_t* inst; float* u; float* v; float* aux; };
 // ---- launchers (psy_kernels.hip) -------------------------------------------
 int launch_build_dice(const float* odata, const DevObject* objs, const int32_t* patch_ids,
                       int n_patches, int levels, float4* dice, hipStream_t st);
-// Two stream-ordered passes; queue (n entries) and qcount are scratch.
+// How the main trace pass is launched: persistent (persist_blocks blocks,
+// lanes refilled when refill_min of a wave's 64 are idle) or one thread per
+// ray.
+struct TraceTuning {
+    bool persistent;
+    int persist_blocks;
+    int refill_min;
+};
+// Two stream-ordered passes; queue (n entries) and counters (4 x u32) are
+// scratch.
 int launch_trace(const DevScene& s, bool world, bool occl_only, int stack_need, const float* rays,
-                 int64_t n, const DevHits& h, uint32_t* queue, uint32_t* qcount, hipStream_t st);
+                 int64_t n, const DevHits& h, uint32_t* queue, uint32_t* counters,
+                 const TraceTuning& tune, hipStream_t st);
+int trace_persist_occupancy(bool world, int* blocks_per_cu);
 int launch_world_to_rays(const float* wrays, int64_t n, float* rays, hipStream_t st);
 int launch_halton(const HaltonDim* dims, const uint16_t* perm, uint32_t dim, const uint32_t* idx,
                   int64_t n, float* out, hipStream_t st);

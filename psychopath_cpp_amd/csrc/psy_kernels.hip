@@ -454,14 +454,11 @@ __device__ __forceinline__ void cmpswap(float& ka, int32_t& ra, float& kb, int32
 
 constexpr int TRACE_BLOCK = 128;
 
-// One ray: BVH4 nearest-first with an LDS stack, leaves through
-// intersect_leaf.  Returns true if the ray needs the below-cache pass.
-template <bool WORLD, int STACK, bool DEEP>
-__device__ __forceinline__ bool trace_one(const DevScene& s, const float* __restrict__ rays, int64_t n, int64_t i,
-                                          const DevHits& h, uint32_t flags, int32_t (*stk_ref)[TRACE_BLOCK],
-                                          float (*stk_near)[TRACE_BLOCK], int tid) {
-    TRay r;
-    float max_t;
+// A ray of the stream: world form (PSY_WRAY_PLANES, footprint derived here,
+// basics/ray.hpp:310-323) or object form (PSY_RAY_PLANES).
+template <bool WORLD>
+__device__ __forceinline__ void load_ray(const float* __restrict__ rays, int64_t n, int64_t i, uint32_t flags,
+                                         TRay& r, float& max_t) {
     if (WORLD) {
         r.o = mk(rays[0 * n + i], rays[1 * n + i], rays[2 * n + i]);
         r.d = mk(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]);
@@ -490,6 +487,18 @@ __device__ __forceinline__ bool trace_one(const DevScene& s, const float* __rest
     }
     if (flags & PSY_TRACE_OCCLUSION_ONLY) r.occl = true;
     r.dinv = divv(splat(1.0f), r.d);
+}
+
+
+// One ray: BVH4 nearest-first with an LDS stack, leaves through
+// intersect_leaf.  Returns true if the ray needs the below-cache pass.
+template <bool WORLD, int STACK, bool DEEP>
+__device__ __forceinline__ bool trace_one(const DevScene& s, const float* __restrict__ rays, int64_t n, int64_t i,
+                                          const DevHits& h, uint32_t flags, int32_t (*stk_ref)[TRACE_BLOCK],
+                                          float (*stk_near)[TRACE_BLOCK], int tid) {
+    TRay r;
+    float max_t;
+    load_ray<WORLD>(rays, n, i, flags, r, max_t);
 
     Best b;
     b.t = max_t;
@@ -559,255 +568,324 @@ __device__ __forceinline__ bool trace_one(const DevScene& s, const float* __rest
     return need_deep && !(r.occl && hit);
 }
 
-// Main-pass form of trace_one.  One loop iteration does one unit of work --
-// one BVH4 node, or one step of a patch's split-tree walk (expand a node's
-// two children, or resume at a pending sibling) -- so the 64 lanes of a wave
-// advance independently instead of waiting at every BVH leaf for the
-// longest split-tree walk in the wave.  Rays that reach the bottom of the
-// dice cache with a node still to split are reported for the second pass.
-template <bool WORLD, int STACK>
-__device__ __forceinline__ bool trace_one_fast(const DevScene& s, const float* __restrict__ rays, int64_t n,
-                                               int64_t i, const DevHits& h, uint32_t flags,
-                                               int32_t (*stk_ref)[TRACE_BLOCK], float (*stk_near)[TRACE_BLOCK],
-                                               int tid) {
+// ---------------------------------------------------------------------------
+// Main pass.  A lane's traversal state lives in registers (plus its LDS
+// stack) and one lane_step does one unit of work -- one BVH4 node, or one
+// step of a patch's split-tree walk (expand a node's two children, or
+// resume at a pending sibling) -- so the 64 lanes of a wave advance
+// independently instead of waiting at every BVH leaf for the longest
+// split-tree walk in the wave.  Rays that reach the bottom of the dice cache
+// with a node still to split, or whose BVH stack would outgrow the main
+// pass's MAIN_STACK entries, are handed to the second pass, which retraces
+// them with a stack sized for the scene's worst case.
+// ---------------------------------------------------------------------------
+constexpr int MAIN_STACK = 16;
+
+struct Lane {
     TRay r;
-    float max_t;
-    if (WORLD) {
-        r.o = mk(rays[0 * n + i], rays[1 * n + i], rays[2 * n + i]);
-        r.d = mk(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]);
-        r.odx = mk(rays[6 * n + i], rays[7 * n + i], rays[8 * n + i]);
-        r.ody = mk(rays[9 * n + i], rays[10 * n + i], rays[11 * n + i]);
-        r.ddx = mk(rays[12 * n + i], rays[13 * n + i], rays[14 * n + i]);
-        r.ddy = mk(rays[15 * n + i], rays[16 * n + i], rays[17 * n + i]);
-        r.time = rays[18 * n + i];
-        const uint32_t type = __float_as_uint(rays[19 * n + i]);
-        r.occl = type == PSY_RAY_OCCLUSION;
-        max_t = r.occl ? 1.0f : INFINITY;
-        r.w = ray_widths(r.o, r.d, r.odx, r.ody, r.ddx, r.ddy);
-    } else {
-        r.o = mk(rays[0 * n + i], rays[1 * n + i], rays[2 * n + i]);
-        r.d = mk(rays[3 * n + i], rays[4 * n + i], rays[5 * n + i]);
-        max_t = rays[6 * n + i];
-        r.time = rays[7 * n + i];
-        r.w.owx = rays[8 * n + i];
-        r.w.owy = rays[9 * n + i];
-        r.w.dwx = rays[10 * n + i];
-        r.w.dwy = rays[11 * n + i];
-        r.w.fwx = rays[12 * n + i];
-        r.w.fwy = rays[13 * n + i];
-        r.occl = (__float_as_uint(rays[14 * n + i]) & (1u << 30)) != 0;
-        r.odx = r.ody = r.ddx = r.ddy = splat(0.0f);
-    }
-    if (flags & PSY_TRACE_OCCLUSION_ONLY) r.occl = true;
-    r.dinv = divv(splat(1.0f), r.d);
-
     Best b;
-    b.t = max_t;
-    b.inst = 0xffffffffu;
-    b.u = b.v = b.aux = 0.0f;
-    bool need_deep = false;
+    int64_t idx;
+    // split-tree walk (valid while `walking`): patch-space ray, heap node k
+    // at `depth`, one `trail` bit per depth whose other child is pending
+    const float4* nodes;
+    V3 wo, wdinv;
+    RayW ww;
+    uint32_t winst, k, trail;
+    int levels, depth;
+    bool walking, expand;
+    // BVH: the nearest hit child is visited next without a stack round trip
+    int32_t next;
+    bool has_next;
+    int sp;
+    bool need_deep, retrace;
+};
 
-    // split-tree walk state (valid while `walking`)
-    bool walking = false;
-    bool expand = true;      // true: expand k; false: resume at a pending sibling
-    const float4* nodes = nullptr;
-    int levels = 0;
-    uint32_t winst = 0, k = 1, trail = 0;
-    int depth = 0;
-    V3 wo = r.o, wdinv = r.dinv;
-    RayW ww = r.w;
-
-    int sp = 0;
-    if (s.root != CHILD_EMPTY) {
-        stk_ref[0][tid] = s.root;
-        stk_near[0][tid] = 0.0f;
-        sp = 1;
-    }
-    const bool ds0 = r.dinv.x < 0.0f, ds1 = r.dinv.y < 0.0f, ds2 = r.dinv.z < 0.0f;
-    bool done = false;
-    for (;;) {
-        if (walking) {
-            if (expand) {
-                const uint32_t c0 = 2 * k;
-                const float4* q = nodes + 2 * c0;
-                const float4 a0 = q[0], e0 = q[1], a1 = q[2], e1 = q[3];
-                float h00, h01, h10, h11;
-                const bool hit0 = bbox_ray(xyz(a0), xyz(e0), wo, wdinv, b.t, &h00, &h01);
-                const bool hit1 = bbox_ray(xyz(a1), xyz(e1), wo, wdinv, b.t, &h10, &h11);
-                bool split0 = false, split1 = false;
-                if (hit1) {
-                    const float maxd = longest_axis(sub(xyz(e1), xyz(a1)));
-                    if (maxd <= smax(min_width(ww, h10, h11) * s.dice_rate, s.min_upoly)) {
-                        if (accept_leaf(h10, h11, a1.w, e1.w, maxd, winst, b) && r.occl) { done = true; break; }
-                    } else {
-                        split1 = true;
-                    }
-                }
-                if (hit0) {
-                    const float maxd = longest_axis(sub(xyz(e0), xyz(a0)));
-                    if (maxd <= smax(min_width(ww, h00, h01) * s.dice_rate, s.min_upoly)) {
-                        if (accept_leaf(h00, h01, a0.w, e0.w, maxd, winst, b) && r.occl) { done = true; break; }
-                    } else {
-                        split0 = true;
-                    }
-                }
-                split0 = split0 && h00 <= b.t;
-                split1 = split1 && h10 <= b.t;
-                const int cd = depth + 1;
-                if (cd == levels) {
-                    if (split0 || split1) need_deep = true;
-                    split0 = split1 = false;
-                }
-                if (split0 && split1) {
-                    k = (h00 < h10) ? c0 : c0 + 1;
-                    trail |= 1u << cd;
-                    depth = cd;
-                } else if (split0 || split1) {
-                    k = split0 ? c0 : c0 + 1;
-                    depth = cd;
-                } else {
-                    expand = false;
-                }
-            } else {
-                if (trail == 0) {
-                    walking = false;
-                } else {
-                    const int dd = 31 - __builtin_clz(trail);
-                    k = (k >> (depth - dd)) ^ 1u;
-                    depth = dd;
-                    trail &= ~(1u << dd);
-                    const float4 a = nodes[2 * k], c = nodes[2 * k + 1];
-                    float h0, h1;
-                    if (bbox_ray(xyz(a), xyz(c), wo, wdinv, b.t, &h0, &h1)) {
-                        const float maxd = longest_axis(sub(xyz(c), xyz(a)));
-                        if (maxd <= smax(min_width(ww, h0, h1) * s.dice_rate, s.min_upoly)) {
-                            if (accept_leaf(h0, h1, a.w, c.w, maxd, winst, b) && r.occl) { done = true; break; }
-                        } else {
-                            expand = true;
-                        }
-                    }
-                }
-            }
-            continue;
-        }
-        if (sp == 0) break;
-        --sp;
-        const int32_t ref = stk_ref[sp][tid];
-        const float nearv = stk_near[sp][tid];
-        if (!(nearv < b.t)) continue;
-        if (ref >= 0) {
-            const float4* q = reinterpret_cast<const float4*>(s.bvh + ref);
-            const float4 lx = q[0], hx = q[1], ly = q[2], hy = q[3], lz = q[4], hz = q[5];
-            const int4 ch = reinterpret_cast<const int4*>(s.bvh + ref)[6];
-#define PSY_LANE(L, KEY)                                                                             \
-    float KEY;                                                                                       \
-    {                                                                                                \
-        const float xlo = ((ds0 ? hx.L : lx.L) - r.o.x) * r.dinv.x;                                  \
-        const float xhi = ((ds0 ? lx.L : hx.L) - r.o.x) * r.dinv.x;                                  \
-        const float ylo = ((ds1 ? hy.L : ly.L) - r.o.y) * r.dinv.y;                                  \
-        const float yhi = ((ds1 ? ly.L : hy.L) - r.o.y) * r.dinv.y;                                  \
-        const float zlo = ((ds2 ? hz.L : lz.L) - r.o.z) * r.dinv.z;                                  \
-        const float zhi = ((ds2 ? lz.L : hz.L) - r.o.z) * r.dinv.z;                                  \
-        const float mins = fmaxf(fmaxf(xlo, ylo), fmaxf(zlo, 0.0f));                                 \
-        const float maxs = fminf(fminf(xhi, yhi), zhi) * PSY_BBOX_MAXT_ADJUST;                       \
-        KEY = ((mins < b.t) && (mins <= maxs)) ? mins : INFINITY;                                    \
-    }
-            PSY_LANE(x, k0)
-            PSY_LANE(y, k1)
-            PSY_LANE(z, k2)
-            PSY_LANE(w, k3)
-#undef PSY_LANE
-            int32_t r0 = ch.x, r1 = ch.y, r2 = ch.z, r3 = ch.w;
-            cmpswap(k0, r0, k1, r1);
-            cmpswap(k2, r2, k3, r3);
-            cmpswap(k0, r0, k2, r2);
-            cmpswap(k1, r1, k3, r3);
-            cmpswap(k1, r1, k2, r2);
-            if (k3 < INFINITY) { stk_ref[sp][tid] = r3; stk_near[sp][tid] = k3; ++sp; }
-            if (k2 < INFINITY) { stk_ref[sp][tid] = r2; stk_near[sp][tid] = k2; ++sp; }
-            if (k1 < INFINITY) { stk_ref[sp][tid] = r1; stk_near[sp][tid] = k1; ++sp; }
-            if (k0 < INFINITY) { stk_ref[sp][tid] = r0; stk_near[sp][tid] = k0; ++sp; }
-            continue;
-        }
-        // leaf instance
-        const uint32_t li = (uint32_t)(~ref);
-        const DevInstance I = s.inst[li];
-        const DevObject ob = s.obj[I.object];
-        if (ob.kind == K_BILINEAR || ob.kind == K_BICUBIC) {
-            wo = r.o; wdinv = r.dinv; ww = r.w;
-            if (I.xf_count > 0) {
-                const M44 x = m_lerp_seq(r.time, s.xf + I.xf_offset, I.xf_count);
-                wo = m_pos(x, r.o);
-                const V3 d = m_dir(x, r.d);
-                wdinv = divv(splat(1.0f), d);
-                if (WORLD) ww = ray_widths(wo, d, m_dir(x, r.odx), m_dir(x, r.ody), m_dir(x, r.ddx), m_dir(x, r.ddy));
-            }
-            nodes = s.dice + 2 * ob.dice;
-            levels = ob.dice_levels;
-            const float4 a = nodes[2], c = nodes[3];
-            float h0, h1;
-            if (!bbox_ray(xyz(a), xyz(c), wo, wdinv, b.t, &h0, &h1)) continue;
-            const float maxd = longest_axis(sub(xyz(c), xyz(a)));
-            if (maxd <= smax(min_width(ww, h0, h1) * s.dice_rate, s.min_upoly)) {
-                if (accept_leaf(h0, h1, a.w, c.w, maxd, li, b) && r.occl) { done = true; break; }
-                continue;
-            }
-            if (levels == 0) { need_deep = true; continue; }
-            walking = true;
-            expand = true;
-            winst = li;
-            k = 1;
-            depth = 0;
-            trail = 0;
-            continue;
-        }
-        bool unused = false;
-        if (intersect_leaf<WORLD, false>(s, li, r, b, unused)) { done = true; break; }
-    }
-    (void)done;
-    const bool hit = b.inst != 0xffffffffu;
-    if (hit && !r.occl) sphere_uv(s, b, r);
-    h.t[i] = hit ? b.t : INFINITY;
-    h.inst[i] = b.inst;
-    h.u[i] = b.u;
-    h.v[i] = b.v;
-    if (h.aux) h.aux[i] = b.aux;
-    return need_deep && !(r.occl && hit);
+template <bool WORLD>
+__device__ __forceinline__ void lane_init(const DevScene& s, const float* __restrict__ rays, int64_t n, int64_t i,
+                                          uint32_t flags, Lane& L) {
+    float max_t;
+    load_ray<WORLD>(rays, n, i, flags, L.r, max_t);
+    L.b.t = max_t;
+    L.b.inst = 0xffffffffu;
+    L.b.u = L.b.v = L.b.aux = 0.0f;
+    L.idx = i;
+    L.walking = false;
+    L.expand = true;
+    L.nodes = nullptr;
+    L.levels = 0;
+    L.winst = 0;
+    L.k = 1;
+    L.trail = 0;
+    L.depth = 0;
+    L.next = s.root;
+    L.has_next = s.root != CHILD_EMPTY;
+    L.sp = 0;
+    L.need_deep = false;
+    L.retrace = false;
 }
 
-// Main pass: every ray; rays whose walk reaches the bottom of the dice cache
-// with a node still to split are queued (once each) for the second pass.
-template <bool WORLD, int STACK>
+// One unit of work; true when the ray is finished (or must be retraced).
+template <bool WORLD>
+__device__ __forceinline__ bool lane_step(const DevScene& s, Lane& L, int32_t (*stk_ref)[TRACE_BLOCK],
+                                          float (*stk_near)[TRACE_BLOCK], int tid) {
+    if (L.walking) {
+        if (L.expand) {
+            const uint32_t c0 = 2 * L.k;
+            const float4* q = L.nodes + 2 * c0;
+            const float4 a0 = q[0], e0 = q[1], a1 = q[2], e1 = q[3];
+            float h00, h01, h10, h11;
+            const bool hit0 = bbox_ray(xyz(a0), xyz(e0), L.wo, L.wdinv, L.b.t, &h00, &h01);
+            const bool hit1 = bbox_ray(xyz(a1), xyz(e1), L.wo, L.wdinv, L.b.t, &h10, &h11);
+            bool split0 = false, split1 = false;
+            // the second half first: the reference visits it first on ties
+            if (hit1) {
+                const float maxd = longest_axis(sub(xyz(e1), xyz(a1)));
+                if (maxd <= smax(min_width(L.ww, h10, h11) * s.dice_rate, s.min_upoly)) {
+                    if (accept_leaf(h10, h11, a1.w, e1.w, maxd, L.winst, L.b) && L.r.occl) return true;
+                } else {
+                    split1 = true;
+                }
+            }
+            if (hit0) {
+                const float maxd = longest_axis(sub(xyz(e0), xyz(a0)));
+                if (maxd <= smax(min_width(L.ww, h00, h01) * s.dice_rate, s.min_upoly)) {
+                    if (accept_leaf(h00, h01, a0.w, e0.w, maxd, L.winst, L.b) && L.r.occl) return true;
+                } else {
+                    split0 = true;
+                }
+            }
+            split0 = split0 && h00 <= L.b.t;
+            split1 = split1 && h10 <= L.b.t;
+            const int cd = L.depth + 1;
+            if (cd == L.levels) {
+                if (split0 || split1) L.need_deep = true;
+                split0 = split1 = false;
+            }
+            if (split0 && split1) {
+                L.k = (h00 < h10) ? c0 : c0 + 1;
+                L.trail |= 1u << cd;
+                L.depth = cd;
+            } else if (split0 || split1) {
+                L.k = split0 ? c0 : c0 + 1;
+                L.depth = cd;
+            } else {
+                L.expand = false;
+            }
+            return false;
+        }
+        if (L.trail == 0) {
+            L.walking = false;
+            return false;
+        }
+        // resume at the deepest pending sibling
+        const int dd = 31 - __builtin_clz(L.trail);
+        L.k = (L.k >> (L.depth - dd)) ^ 1u;
+        L.depth = dd;
+        L.trail &= ~(1u << dd);
+        const float4 a = L.nodes[2 * L.k], c = L.nodes[2 * L.k + 1];
+        float h0, h1;
+        if (bbox_ray(xyz(a), xyz(c), L.wo, L.wdinv, L.b.t, &h0, &h1)) {
+            const float maxd = longest_axis(sub(xyz(c), xyz(a)));
+            if (maxd <= smax(min_width(L.ww, h0, h1) * s.dice_rate, s.min_upoly)) {
+                if (accept_leaf(h0, h1, a.w, c.w, maxd, L.winst, L.b) && L.r.occl) return true;
+            } else {
+                L.expand = true;
+            }
+        }
+        return false;
+    }
+
+    int32_t ref;
+    if (L.has_next) {
+        ref = L.next;
+        L.has_next = false;
+    } else {
+        if (L.sp == 0) return true;
+        --L.sp;
+        ref = stk_ref[L.sp][tid];
+        if (!(stk_near[L.sp][tid] < L.b.t)) return false;
+    }
+    if (ref >= 0) {
+        const float4* q = reinterpret_cast<const float4*>(s.bvh + ref);
+        const float4 lx = q[0], hx = q[1], ly = q[2], hy = q[3], lz = q[4], hz = q[5];
+        const int4 ch = reinterpret_cast<const int4*>(s.bvh + ref)[6];
+        const TRay& r = L.r;
+        const bool ds0 = r.dinv.x < 0.0f, ds1 = r.dinv.y < 0.0f, ds2 = r.dinv.z < 0.0f;
+        // BBox4::intersect_ray lanes (basics/bbox.hpp:566-595); keys are the
+        // near distances of hit children, +inf otherwise
+#define PSY_LANE(LN, KEY)                                                                            \
+    float KEY;                                                                                       \
+    {                                                                                                \
+        const float xlo = ((ds0 ? hx.LN : lx.LN) - r.o.x) * r.dinv.x;                                \
+        const float xhi = ((ds0 ? lx.LN : hx.LN) - r.o.x) * r.dinv.x;                                \
+        const float ylo = ((ds1 ? hy.LN : ly.LN) - r.o.y) * r.dinv.y;                                \
+        const float yhi = ((ds1 ? ly.LN : hy.LN) - r.o.y) * r.dinv.y;                                \
+        const float zlo = ((ds2 ? hz.LN : lz.LN) - r.o.z) * r.dinv.z;                                \
+        const float zhi = ((ds2 ? lz.LN : hz.LN) - r.o.z) * r.dinv.z;                                \
+        const float mins = fmaxf(fmaxf(xlo, ylo), fmaxf(zlo, 0.0f));                                 \
+        const float maxs = fminf(fminf(xhi, yhi), zhi) * PSY_BBOX_MAXT_ADJUST;                       \
+        KEY = ((mins < L.b.t) && (mins <= maxs)) ? mins : INFINITY;                                  \
+    }
+        PSY_LANE(x, k0)
+        PSY_LANE(y, k1)
+        PSY_LANE(z, k2)
+        PSY_LANE(w, k3)
+#undef PSY_LANE
+        int32_t r0 = ch.x, r1 = ch.y, r2 = ch.z, r3 = ch.w;
+        cmpswap(k0, r0, k1, r1);
+        cmpswap(k2, r2, k3, r3);
+        cmpswap(k0, r0, k2, r2);
+        cmpswap(k1, r1, k3, r3);
+        cmpswap(k1, r1, k2, r2);
+        const int npush = (k1 < INFINITY ? 1 : 0) + (k2 < INFINITY ? 1 : 0) + (k3 < INFINITY ? 1 : 0);
+        if (L.sp + npush > MAIN_STACK) {
+            L.retrace = true;
+            return true;
+        }
+        if (k3 < INFINITY) { stk_ref[L.sp][tid] = r3; stk_near[L.sp][tid] = k3; ++L.sp; }
+        if (k2 < INFINITY) { stk_ref[L.sp][tid] = r2; stk_near[L.sp][tid] = k2; ++L.sp; }
+        if (k1 < INFINITY) { stk_ref[L.sp][tid] = r1; stk_near[L.sp][tid] = k1; ++L.sp; }
+        if (k0 < INFINITY) { L.next = r0; L.has_next = true; }
+        return false;
+    }
+    // leaf instance
+    const uint32_t li = (uint32_t)(~ref);
+    const DevInstance I = s.inst[li];
+    const DevObject ob = s.obj[I.object];
+    if (ob.kind == K_BILINEAR || ob.kind == K_BICUBIC) {
+        L.wo = L.r.o;
+        L.wdinv = L.r.dinv;
+        L.ww = L.r.w;
+        if (I.xf_count > 0) {
+            const M44 x = m_lerp_seq(L.r.time, s.xf + I.xf_offset, I.xf_count);
+            L.wo = m_pos(x, L.r.o);
+            const V3 d = m_dir(x, L.r.d);
+            L.wdinv = divv(splat(1.0f), d);
+            if (WORLD)
+                L.ww = ray_widths(L.wo, d, m_dir(x, L.r.odx), m_dir(x, L.r.ody), m_dir(x, L.r.ddx),
+                                  m_dir(x, L.r.ddy));
+        }
+        L.nodes = s.dice + 2 * ob.dice;
+        L.levels = ob.dice_levels;
+        const float4 a = L.nodes[2], c = L.nodes[3];
+        float h0, h1;
+        if (!bbox_ray(xyz(a), xyz(c), L.wo, L.wdinv, L.b.t, &h0, &h1)) return false;
+        const float maxd = longest_axis(sub(xyz(c), xyz(a)));
+        if (maxd <= smax(min_width(L.ww, h0, h1) * s.dice_rate, s.min_upoly))
+            return accept_leaf(h0, h1, a.w, c.w, maxd, li, L.b) && L.r.occl;
+        if (L.levels == 0) {
+            L.need_deep = true;
+            return false;
+        }
+        L.walking = true;
+        L.expand = true;
+        L.winst = li;
+        L.k = 1;
+        L.depth = 0;
+        L.trail = 0;
+        return false;
+    }
+    bool unused = false;
+    return intersect_leaf<WORLD, false>(s, li, L.r, L.b, unused);
+}
+
+// Writes the lane's hit; true if the ray goes to the second pass.
+__device__ __forceinline__ bool lane_finish(const DevScene& s, Lane& L, const DevHits& h) {
+    const bool hit = L.b.inst != 0xffffffffu;
+    if (hit && !L.r.occl) sphere_uv(s, L.b, L.r);
+    const int64_t i = L.idx;
+    h.t[i] = hit ? L.b.t : INFINITY;
+    h.inst[i] = L.b.inst;
+    h.u[i] = L.b.u;
+    h.v[i] = L.b.v;
+    if (h.aux) h.aux[i] = L.b.aux;
+    return (L.need_deep || L.retrace) && !(L.r.occl && hit);
+}
+
+__device__ __forceinline__ void enqueue_retrace(uint32_t* __restrict__ queue, uint32_t* __restrict__ counters,
+                                                int64_t i) {
+    const uint32_t slot = atomicAdd(&counters[0], 1u);
+    queue[slot] = (uint32_t)i;
+}
+
+// Main pass, one thread per ray.
+template <bool WORLD>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_trace(DevScene s, const float* __restrict__ rays, int64_t n,
                                                         DevHits h, uint32_t flags, uint32_t* __restrict__ queue,
-                                                        uint32_t* __restrict__ qcount) {
-    __shared__ int32_t stk_ref[STACK][TRACE_BLOCK];
-    __shared__ float stk_near[STACK][TRACE_BLOCK];
+                                                        uint32_t* __restrict__ counters) {
+    __shared__ int32_t stk_ref[MAIN_STACK][TRACE_BLOCK];
+    __shared__ float stk_near[MAIN_STACK][TRACE_BLOCK];
     const int tid = threadIdx.x;
     const int64_t i = (int64_t)blockIdx.x * TRACE_BLOCK + tid;
     if (i >= n) return;
-    if (trace_one_fast<WORLD, STACK>(s, rays, n, i, h, flags, stk_ref, stk_near, tid)) {
-        const uint32_t slot = atomicAdd(qcount, 1u);
-        queue[slot] = (uint32_t)i;
+    Lane L;
+    lane_init<WORLD>(s, rays, n, i, flags, L);
+    while (!lane_step<WORLD>(s, L, stk_ref, stk_near, tid)) {
+    }
+    if (lane_finish(s, L, h)) enqueue_retrace(queue, counters, i);
+}
+
+// Main pass, persistent: a grid that fills the GPU once; every wave takes
+// rays from a global counter, and refills its finished lanes (in one
+// atomic per wave) whenever at least `refill_min` of them are idle, so the
+// wave's lanes stay busy through the long tail of incoherent rays.  Every
+// wave leaves once the counter has passed n and its lanes are idle.
+template <bool WORLD>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_persist(DevScene s, const float* __restrict__ rays,
+                                                                int64_t n, DevHits h, uint32_t flags,
+                                                                uint32_t* __restrict__ queue,
+                                                                uint32_t* __restrict__ counters, int refill_min) {
+    __shared__ int32_t stk_ref[MAIN_STACK][TRACE_BLOCK];
+    __shared__ float stk_near[MAIN_STACK][TRACE_BLOCK];
+    const int tid = threadIdx.x;
+    const uint32_t lane = __lane_id();
+    unsigned long long* next_ray = reinterpret_cast<unsigned long long*>(counters + 2);
+    Lane L;
+    bool active = false;
+    bool exhausted = false;  // wave-uniform
+    for (;;) {
+        const unsigned long long idle = __ballot(!active);
+        const int nidle = __popcll(idle);
+        if (!exhausted && nidle >= refill_min) {
+            unsigned long long base = 0;
+            if (lane == 0) base = atomicAdd(next_ray, (unsigned long long)nidle);
+            base = ((unsigned long long)(uint32_t)__shfl((int)(uint32_t)(base >> 32), 0) << 32) |
+                   (unsigned long long)(uint32_t)__shfl((int)(uint32_t)base, 0);
+            if ((int64_t)base >= n) exhausted = true;
+            if (!active) {
+                const int64_t i = (int64_t)base + __popcll(idle & ((1ull << lane) - 1ull));
+                if (i < n) {
+                    lane_init<WORLD>(s, rays, n, i, flags, L);
+                    active = true;
+                }
+            }
+        }
+        if (exhausted && __ballot(active) == 0ull) break;
+        if (active && lane_step<WORLD>(s, L, stk_ref, stk_near, tid)) {
+            active = false;
+            if (lane_finish(s, L, h)) enqueue_retrace(queue, counters, L.idx);
+        }
     }
 }
 
-// Second pass: retrace the queued rays with subdivision below the cache.
+// Second pass: retrace the queued rays with subdivision below the cache and
+// a stack of STACK >= the scene's worst case.
 template <bool WORLD, int STACK>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_deep(DevScene s, const float* __restrict__ rays, int64_t n,
                                                              DevHits h, uint32_t flags,
                                                              const uint32_t* __restrict__ queue,
-                                                             const uint32_t* __restrict__ qcount) {
+                                                             const uint32_t* __restrict__ counters) {
     __shared__ int32_t stk_ref[STACK][TRACE_BLOCK];
     __shared__ float stk_near[STACK][TRACE_BLOCK];
     const int tid = threadIdx.x;
-    const uint32_t count = *qcount;
+    const uint32_t count = counters[0];
     for (uint32_t k = blockIdx.x * TRACE_BLOCK + tid; k < count; k += gridDim.x * TRACE_BLOCK)
         (void)trace_one<WORLD, STACK, true>(s, rays, n, (int64_t)queue[k], h, flags, stk_ref, stk_near, tid);
 }
 
-__global__ void k_zero_u32(uint32_t* p) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0u;
+// counters[0]: retrace queue length; counters[2..3]: next ray (u64)
+__global__ void k_zero_counters(uint32_t* p) {
+    if (threadIdx.x < 4 && blockIdx.x == 0) p[threadIdx.x] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -1089,27 +1167,49 @@ int launch_build_dice(const float* odata, const DevObject* objs, const int32_t* 
 }
 
 template <bool W, int S>
-static void launch_trace_pair(const DevScene& s, const float* rays, int64_t n, const DevHits& h, uint32_t flags,
-                              uint32_t* queue, uint32_t* qcount, hipStream_t st) {
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, st, qcount);
-    hipLaunchKernelGGL((k_trace<W, S>), dim3(grid_for(n, TRACE_BLOCK)), dim3(TRACE_BLOCK), 0, st, s, rays, n, h,
-                       flags, queue, qcount);
-    const unsigned deep_blocks = (unsigned)std::min<int64_t>(grid_for(n, TRACE_BLOCK), 1024);
-    hipLaunchKernelGGL((k_trace_deep<W, S>), dim3(deep_blocks), dim3(TRACE_BLOCK), 0, st, s, rays, n, h, flags,
-                       (const uint32_t*)queue, (const uint32_t*)qcount);
+static void launch_deep(const DevScene& s, const float* rays, int64_t n, const DevHits& h, uint32_t flags,
+                        const uint32_t* queue, const uint32_t* counters, hipStream_t st) {
+    const unsigned blocks = (unsigned)std::min<int64_t>(grid_for(n, TRACE_BLOCK), 1024);
+    hipLaunchKernelGGL((k_trace_deep<W, S>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, s, rays, n, h, flags, queue,
+                       counters);
+}
+
+template <bool W>
+static int launch_trace_w(const DevScene& s, int stack_need, const float* rays, int64_t n, const DevHits& h,
+                          uint32_t flags, uint32_t* queue, uint32_t* counters, const TraceTuning& tune,
+                          hipStream_t st) {
+    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, counters);
+    const int64_t plain_blocks = grid_for(n, TRACE_BLOCK);
+    const int64_t persist_blocks = (int64_t)tune.persist_blocks;
+    if (tune.persistent && persist_blocks > 0 && persist_blocks < plain_blocks) {
+        hipLaunchKernelGGL((k_trace_persist<W>), dim3((unsigned)persist_blocks), dim3(TRACE_BLOCK), 0, st, s, rays,
+                           n, h, flags, queue, counters, std::max(1, std::min(64, tune.refill_min)));
+    } else {
+        hipLaunchKernelGGL((k_trace<W>), dim3((unsigned)plain_blocks), dim3(TRACE_BLOCK), 0, st, s, rays, n, h,
+                           flags, queue, counters);
+    }
+    if (stack_need <= 16) launch_deep<W, 16>(s, rays, n, h, flags, queue, counters, st);
+    else if (stack_need <= 32) launch_deep<W, 32>(s, rays, n, h, flags, queue, counters, st);
+    else if (stack_need <= 64) launch_deep<W, 64>(s, rays, n, h, flags, queue, counters, st);
+    else return PSY_ERR_UNSUPPORTED;
+    return check_launch();
 }
 
 int launch_trace(const DevScene& s, bool world, bool occl_only, int stack_need, const float* rays, int64_t n,
-                 const DevHits& h, uint32_t* queue, uint32_t* qcount, hipStream_t st) {
+                 const DevHits& h, uint32_t* queue, uint32_t* counters, const TraceTuning& tune, hipStream_t st) {
     if (n <= 0) return 0;
+    if (stack_need > 64) return PSY_ERR_UNSUPPORTED;
     const uint32_t flags = occl_only ? PSY_TRACE_OCCLUSION_ONLY : 0u;
-#define PSY_TRACE_CASE(W, S) launch_trace_pair<W, S>(s, rays, n, h, flags, queue, qcount, st)
-    if (stack_need <= 16) { if (world) PSY_TRACE_CASE(true, 16); else PSY_TRACE_CASE(false, 16); }
-    else if (stack_need <= 32) { if (world) PSY_TRACE_CASE(true, 32); else PSY_TRACE_CASE(false, 32); }
-    else if (stack_need <= 64) { if (world) PSY_TRACE_CASE(true, 64); else PSY_TRACE_CASE(false, 64); }
-    else return PSY_ERR_UNSUPPORTED;
-#undef PSY_TRACE_CASE
-    return check_launch();
+    return world ? launch_trace_w<true>(s, stack_need, rays, n, h, flags, queue, counters, tune, st)
+                 : launch_trace_w<false>(s, stack_need, rays, n, h, flags, queue, counters, tune, st);
+}
+
+int trace_persist_occupancy(bool world, int* blocks_per_cu) {
+    int nb = 0;
+    hipError_t e = world ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace_persist<true>, TRACE_BLOCK, 0)
+                         : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace_persist<false>, TRACE_BLOCK, 0);
+    *blocks_per_cu = nb;
+    return e == hipSuccess ? 0 : PSY_ERR_HIP;
 }
 
 int launch_world_to_rays(const float* wrays, int64_t n, float* rays, hipStream_t st) {

@@ -35,6 +35,8 @@ def lib():
     L.psy_abi_version.restype = C.c_int
     L.psy_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
     L.psy_ctx_destroy.argtypes = [vp]
+    L.psy_ctx_set_trace_mode.argtypes = [vp, C.c_int, C.c_int, C.c_int]
+    L.psy_ctx_retrace_count.argtypes = [vp, C.POINTER(i64)]
     L.psy_scene_create.argtypes = [vp, C.POINTER(abi.SceneDesc), C.c_int, C.POINTER(vp)]
     L.psy_scene_destroy.argtypes = [vp]
     L.psy_scene_get_info.argtypes = [vp, C.POINTER(abi.SceneInfo)]
@@ -74,6 +76,17 @@ class Context:
         h = C.c_void_p()
         check(lib().psy_ctx_create(device, C.byref(h)), "psy_ctx_create")
         self.handle = h
+
+    def set_trace_mode(self, persistent=True, refill_min=16, blocks_per_cu=0):
+        """psy_ctx_set_trace_mode: persistent main pass with lane refill, or one thread per ray."""
+        mode = abi.PSY_TRACE_MODE_PERSISTENT if persistent else abi.PSY_TRACE_MODE_PLAIN
+        check(lib().psy_ctx_set_trace_mode(self.handle, mode, refill_min, blocks_per_cu), "psy_ctx_set_trace_mode")
+
+    def retrace_count(self):
+        """psy_ctx_retrace_count: rays of the latest trace that took the second pass."""
+        v = C.c_int64()
+        check(lib().psy_ctx_retrace_count(self.handle, C.byref(v)), "psy_ctx_retrace_count")
+        return v.value
 
     def close(self):
         if self.handle:

@@ -290,3 +290,73 @@ def test_gpu_empty_and_single_instance_scenes(gpu_ctx, oracle):
     # zero-length stream is a no-op
     z = torch.empty((20, 0), dtype=torch.float32, device="cuda")
     gpu.trace_world(gpu_ctx, g, z)
+
+
+def _hits_equal(a, b):
+    return (torch.equal(a.inst, b.inst) and torch.equal(a.t, b.t) and torch.equal(a.u, b.u)
+            and torch.equal(a.v, b.v) and (a.aux is None or torch.equal(a.aux, b.aux)))
+
+
+def test_gpu_trace_modes_identical(gpu_ctx, c1):
+    """The persistent main pass (any refill threshold, any grid) gives the
+    same bits as one thread per ray."""
+    sc, g, o = c1
+    W, H, spp, seed = 256, 256, 2, 3
+    w, _ = gpu.camera_rays(gpu_ctx, g, W, H, spp, seed)
+    ctx = gpu.Context(0)
+    try:
+        ctx.set_trace_mode(persistent=False)
+        hp = gpu.trace_world(ctx, g, w)
+        b, valid = gpu.bounce_rays(ctx, g, w, hp, W, spp, seed, 9, 10)
+        stream = torch.cat([w, b[:, valid.bool()]], dim=1).contiguous()
+        rays = gpu.world_to_rays(ctx, stream)
+        ref = gpu.trace_rays(ctx, g, rays)
+        for refill, bpc in ((16, 0), (1, 0), (64, 0), (8, 1), (33, 2)):
+            say("persistent mode refill", refill, "blocks/CU", bpc)
+            ctx.set_trace_mode(persistent=True, refill_min=refill, blocks_per_cu=bpc)
+            got = gpu.trace_rays(ctx, g, rays)
+            assert _hits_equal(got, ref), (refill, bpc)
+            gw = gpu.trace_world(ctx, g, w)
+            assert _hits_equal(gw, hp), (refill, bpc)
+    finally:
+        ctx.close()
+
+
+def _nested_spheres(n=4096, seed=5):
+    """Spheres around one centre: a ray through it hits every BVH box, so
+    every node pushes three children and the main pass's stack overflows."""
+    rng = np.random.default_rng(seed)
+    root = Assembly()
+    sc = Scene(Camera([look_at((0.0, -6.0, 0.3), (0.0, 0.0, 0.0))], [np.radians(30)]), root)
+    sc.add_surface_shader("s", abi.PSY_SHADER_LAMBERT, (0.5, 0.5, 0.5))
+    c = rng.uniform(-0.05, 0.05, size=(n, 3)).astype(np.float32)
+    r = rng.uniform(0.8, 1.2, size=n).astype(np.float32)
+    for i in range(n):
+        root.add_object(f"s{i}", Sphere(c[i].tolist(), float(r[i])))
+        root.create_object_instance(f"s{i}", [], "s")
+    return sc
+
+
+def test_gpu_stack_overflow_retrace(gpu_ctx, oracle):
+    sc = _nested_spheres()
+    g = gpu.GpuScene(gpu_ctx, sc)
+    o = oracle.OracleScene(sc)
+    assert g.info.bvh_stack_need > 16
+    w, _ = gpu.camera_rays(gpu_ctx, g, 48, 48, 1, 4)
+    ctx = gpu.Context(0)
+    try:
+        for persistent in (True, False):
+            ctx.set_trace_mode(persistent=persistent)
+            hits = gpu.trace_world(ctx, g, w)
+            retraced = ctx.retrace_count()
+            say("nested spheres persistent", persistent, "retraced", retraced)
+            assert retraced > 0
+            inter = o.trace(oracle.wrays_from_planes(w.cpu().numpy()), threads=8, chunk=8192)
+            st = compare_hits(hits, g.instance_ids(), inter)
+            say(st)
+            assert st["hits"] > 0 and st["match"] >= 0.9999 and st["t_rel_max"] <= 1e-5
+            # occlusion form: any hit ends the ray
+            ho = gpu.trace_world(ctx, g, w, occlusion_only=True)
+            assert torch.equal(ho.inst != -1, hits.inst != -1)
+    finally:
+        ctx.close()

@@ -71,7 +71,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1 << 21)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--dice-depth", type=int, default=-1)
-    ap.add_argument("--trace-mode", choices=("persistent", "plain"), default="persistent",
+    ap.add_argument("--trace-mode", choices=("persistent", "plain"), default="plain",
                     help="main trace pass: persistent waves with lane refill, or one thread per ray")
     ap.add_argument("--refill", type=int, default=16, help="persistent mode: refill a wave once this many lanes idle")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="persistent mode: blocks per CU (0: occupancy)")

@@ -154,14 +154,18 @@ typedef struct psy_scene_info {
 int psy_ctx_create(int device, psy_ctx** out);
 int psy_ctx_destroy(psy_ctx* ctx);
 /* Launch form of the trace's main pass (no counterpart in the reference,
- * whose tracer is a per-thread CPU loop).  PSY_TRACE_MODE_PERSISTENT
- * (default): one grid that fills the GPU; each wave refills its finished
- * lanes from a global ray counter once refill_min (1..64, default 16) of its
- * 64 lanes are idle; blocks_per_cu <= 0 takes the kernel's occupancy.
- * PSY_TRACE_MODE_PLAIN: one thread per ray.  Results are identical. */
+ * whose tracer is a per-thread CPU loop).  PSY_TRACE_MODE_PLAIN (default):
+ * one thread per ray.  PSY_TRACE_MODE_PERSISTENT: one grid that fills the
+ * GPU; each wave refills its finished lanes from a global ray counter once
+ * refill_min (1..64) of its 64 lanes are idle; blocks_per_cu <= 0 takes the
+ * kernel's occupancy.  Results are identical. */
 #define PSY_TRACE_MODE_PLAIN 0
 #define PSY_TRACE_MODE_PERSISTENT 1
 int psy_ctx_set_trace_mode(psy_ctx* ctx, int mode, int refill_min, int blocks_per_cu);
+/* Paths per wavefront chunk of psy_render (no counterpart in the reference,
+ * whose buckets are sqrt(65536/spp) pixels square): whole pixels, at least
+ * one; the default 2^21 paths.  Results do not depend on it. */
+int psy_ctx_set_render_chunk(psy_ctx* ctx, int64_t paths);
 /* Rays of the latest trace call on this context that the main pass handed
  * to the second pass (split below the dice cache, or a BVH stack deeper
  * than the main pass keeps).  Synchronizes the device; for diagnostics. */
@@ -200,6 +204,34 @@ int psy_trace_rays(psy_ctx* ctx, const psy_scene* scene,
  * PSY_RAY_PLANES).  Replaces tracer/tracer.cpp:47-51. */
 int psy_world_to_rays(psy_ctx* ctx, const float* world_rays, int64_t n,
                       float* rays, void* stream);
+
+/* ---- rendering ---------------------------------------------------------- */
+typedef struct psy_render_params {
+    int32_t width, height;  /* image size (Film, film/film.hpp:40-60)            */
+    int32_t spp;            /* samples per pixel                                */
+    uint32_t seed;          /* ImageSampler seed (parser default 1)             */
+    int32_t path_length;    /* bounces; the reference integrator uses 4         */
+    int32_t reserved;
+} psy_render_params;
+
+/* Replaces: PathTraceIntegrator(scene, film, spp, spp, 0, seed, threads)
+ * + integrate() (integrator/path_trace_integrator.hpp:103-120,
+ * integrator/path_trace_integrator.cpp:282-384): every pixel sample of the
+ * image is one path (init_path / next_ray_for_path / update_path with light
+ * tree + MIS next-event estimation, Lambert and emit closures, hero-
+ * wavelength spectral sampling) traced as a wavefront on the device.
+ *   film_xyz   : device, width*height*3 floats, or NULL: per pixel the mean
+ *                of its finite sample values (Film::add_sample's sum over
+ *                count, film/film.hpp:135-153), CIE XYZ.
+ *   sample_xyz : device, width*height*spp*3 floats, or NULL: every sample's
+ *                film contribution Color_XYZ(col) * XYZ_NORM_FAC, ordered
+ *                (y, x, sample).
+ *   rays_traced: host, or NULL: camera + bounce + shadow rays traced
+ *                (Global::Stats::rays_shot).
+ * Grey colours only (PSY_ERR_UNSUPPORTED otherwise).  Synchronizes
+ * `stream` once per wavefront step. */
+int psy_render(psy_ctx* ctx, const psy_scene* scene, const psy_render_params* params,
+               float* film_xyz, float* sample_xyz, int64_t* rays_traced, void* stream);
 
 /* ---- sampling ----------------------------------------------------------- */
 /* Replaces: Halton::sample(dim, index) (sampling/halton.cpp:2154-2671). */

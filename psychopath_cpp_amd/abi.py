@@ -82,6 +82,11 @@ class Hits(C.Structure):
                 ("aux", C.c_void_p)]
 
 
+class RenderParams(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32), ("seed", C.c_uint32),
+                ("path_length", C.c_int32), ("reserved", C.c_int32)]
+
+
 class SceneInfo(C.Structure):
     _fields_ = [("n_leaf_instances", C.c_int64), ("n_objects", C.c_int64), ("n_bvh_nodes", C.c_int64),
                 ("n_patches", C.c_int64), ("dice_depth", C.c_int32), ("bvh_stack_need", C.c_int32),
@@ -90,8 +95,8 @@ class SceneInfo(C.Structure):
 
 # Every symbol the C ABI exports (include/psy_gpu.h).
 EXPORTED_SYMBOLS = [
-    "psy_ctx_create", "psy_ctx_destroy", "psy_ctx_set_trace_mode", "psy_ctx_retrace_count", "psy_last_error", "psy_abi_version",
+    "psy_ctx_create", "psy_ctx_destroy", "psy_ctx_set_trace_mode", "psy_ctx_retrace_count", "psy_ctx_set_render_chunk", "psy_last_error", "psy_abi_version",
     "psy_scene_create", "psy_scene_destroy", "psy_scene_get_info", "psy_scene_instance_ids",
     "psy_trace_world", "psy_trace_rays", "psy_world_to_rays",
-    "psy_halton", "psy_pixel_samples", "psy_camera_rays", "psy_bounce_rays",
+    "psy_halton", "psy_pixel_samples", "psy_camera_rays", "psy_bounce_rays", "psy_render",
 ]

@@ -21,25 +21,17 @@
 #include <vector>
 
 #include "../../include/psy_gpu.h"
-#include "psy_device.hpp"
+#include "psy_internal.hpp"
 
 using namespace psy;
 
-namespace {
-
+namespace psy {
 thread_local std::string g_err;
 
 int fail(int code, const std::string& msg) {
     g_err = msg;
     return code;
 }
-
-#define HIP_OK(expr)                                                                   \
-    do {                                                                               \
-        hipError_t e_ = (expr);                                                        \
-        if (e_ != hipSuccess)                                                          \
-            return fail(PSY_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-    } while (0)
 
 int floats_per_kind(int kind) {
     switch (kind) {
@@ -51,35 +43,17 @@ int floats_per_kind(int kind) {
         default: return 0;
     }
 }
+}  // namespace psy
 
-// ---- device buffer ---------------------------------------------------------
-struct DBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    ~DBuf() { if (p) (void)hipFree(p); }
-    int alloc(size_t b) {
-        bytes = b;
-        if (b == 0) return 0;
-        hipError_t e = hipMalloc(&p, b);
-        if (e != hipSuccess) { p = nullptr; return fail(PSY_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e)); }
-        return 0;
-    }
-    int upload(const void* src, size_t b) {
-        int rc = alloc(b);
-        if (rc) return rc;
-        if (b) HIP_OK(hipMemcpy(p, src, b, hipMemcpyHostToDevice));
-        return 0;
-    }
-};
+namespace {
 
 // ---- Halton tables (sampling/halton.py:27-87 generation rule) --------------
-struct HaltonTables {
-    std::vector<HaltonDim> dims;
-    std::vector<uint16_t> perm;
-};
+}  // namespace
 
-HaltonTables make_halton_tables() {
-    HaltonTables t;
+namespace psy {
+void make_halton_tables(std::vector<HaltonDim>& t_dims, std::vector<uint16_t>& t_perm) {
+    t_dims.clear();
+    t_perm.clear();
     const int D = 256;
     std::vector<uint32_t> primes;
     uint32_t cand = 1;
@@ -109,16 +83,16 @@ HaltonTables make_halton_tables() {
                 faure[b][i] = (uint16_t)(i < c ? 2 * faure[c][i] : 2 * faure[c][i - c] + 1);
         }
     }
-    t.dims.assign(D, HaltonDim{});
+    t_dims.assign(D, HaltonDim{});
     for (int d = 1; d < D; ++d) {
-        HaltonDim& h = t.dims[d];
+        HaltonDim& h = t_dims[d];
         const uint64_t base = primes[d];
         uint32_t digits = 1;
         uint64_t pow_base = base;
         while (pow_base * base <= 500) { pow_base *= base; ++digits; }
         uint64_t max_power = pow_base;
         while (max_power * pow_base < (1ull << 32)) max_power *= pow_base;
-        h.perm_offset = (uint32_t)t.perm.size();
+        h.perm_offset = (uint32_t)t_perm.size();
         h.pow_base = (uint32_t)pow_base;
         for (uint64_t j = 0; j < pow_base; ++j) {
             uint64_t idx = j, result = 0;
@@ -127,7 +101,7 @@ HaltonTables make_halton_tables() {
                 idx /= base;
                 result = result * base + faure[base][rem];
             }
-            t.perm.push_back((uint16_t)result);
+            t_perm.push_back((uint16_t)result);
         }
         uint64_t power = max_power / pow_base, div = 1;
         uint32_t n = 0;
@@ -140,34 +114,14 @@ HaltonTables make_halton_tables() {
         h.n_terms = n;
         h.scale = 0x1.fffffep-1 / (double)max_power;
     }
-    return t;
 }
+}  // namespace psy
+
+namespace {
 
 }  // namespace
 
-// ---------------------------------------------------------------------------
-struct psy_ctx {
-    int device = 0;
-    DBuf halton_dims, halton_perm;
-    // trace scratch: queue of rays for the second pass + its counters
-    DBuf queue, counters;
-    int64_t queue_cap = 0;
-    // main-pass launch form, per ray form (0: object rays, 1: world rays)
-    int n_cu = 0;
-    int occupancy[2] = {0, 0};
-    int mode = PSY_TRACE_MODE_PERSISTENT;
-    int refill_min = 16;
-    int blocks_per_cu = 0;  // <= 0: from occupancy
-    TraceTuning tuning(bool world) const {
-        TraceTuning t;
-        t.persistent = mode == PSY_TRACE_MODE_PERSISTENT;
-        t.persist_blocks = n_cu * (blocks_per_cu > 0 ? blocks_per_cu : occupancy[world ? 1 : 0]);
-        t.refill_min = refill_min;
-        return t;
-    }
-};
-
-namespace {
+namespace psy {
 // Grows the per-context trace scratch to n entries.  Calls on one context
 // must not overlap in time (they share this scratch).
 int ensure_queue(psy_ctx* ctx, int64_t n) {
@@ -177,25 +131,14 @@ int ensure_queue(psy_ctx* ctx, int64_t n) {
     }
     if (n <= ctx->queue_cap) return PSY_OK;
     HIP_OK(hipDeviceSynchronize());
-    if (ctx->queue.p) { (void)hipFree(ctx->queue.p); ctx->queue.p = nullptr; }
+    ctx->queue.release();
     const int64_t cap = std::max<int64_t>(n, 1 << 16);
     int rc = ctx->queue.alloc((size_t)cap * sizeof(uint32_t));
     if (rc) { ctx->queue_cap = 0; return rc; }
     ctx->queue_cap = cap;
     return PSY_OK;
 }
-}  // namespace
-
-struct psy_scene {
-    psy_ctx* ctx = nullptr;
-    DevScene dev{};
-    DevCamera cam{};
-    DBuf bvh, inst, obj, odata, xf, dice, shaders;
-    int stack_need = 1;
-    bool transformed_patches = false;
-    std::vector<uint64_t> ids;
-    psy_scene_info info{};
-};
+}  // namespace psy
 
 extern "C" {
 
@@ -212,10 +155,12 @@ int psy_ctx_create(int device, psy_ctx** out) {
     HIP_OK(hipSetDevice(device));
     std::unique_ptr<psy_ctx> c(new psy_ctx());
     c->device = device;
-    HaltonTables t = make_halton_tables();
-    int rc = c->halton_dims.upload(t.dims.data(), t.dims.size() * sizeof(HaltonDim));
+    std::vector<HaltonDim> hdims;
+    std::vector<uint16_t> hperm;
+    make_halton_tables(hdims, hperm);
+    int rc = c->halton_dims.upload(hdims);
     if (rc) return rc;
-    rc = c->halton_perm.upload(t.perm.data(), t.perm.size() * sizeof(uint16_t));
+    rc = c->halton_perm.upload(hperm);
     if (rc) return rc;
     HIP_OK(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device));
     if (trace_persist_occupancy(false, &c->occupancy[0]) || trace_persist_occupancy(true, &c->occupancy[1]))
@@ -233,6 +178,13 @@ int psy_ctx_set_trace_mode(psy_ctx* ctx, int mode, int refill_min, int blocks_pe
     ctx->mode = mode;
     ctx->refill_min = refill_min;
     ctx->blocks_per_cu = blocks_per_cu;
+    return PSY_OK;
+}
+
+int psy_ctx_set_render_chunk(psy_ctx* ctx, int64_t paths) {
+    if (!ctx) return fail(PSY_ERR_INVALID, "psy_ctx_set_render_chunk: NULL context");
+    if (paths < 1 || paths > (int64_t)INT32_MAX) return fail(PSY_ERR_INVALID, "psy_ctx_set_render_chunk: paths not in 1..2^31-1");
+    ctx->render_chunk = paths;
     return PSY_OK;
 }
 
@@ -529,22 +481,20 @@ bool invert_d(const M44& m, double inv[16]) {
 
 }  // namespace
 
-extern "C" {
+namespace psy {
 
-int psy_scene_create(psy_ctx* ctx, const psy_scene_desc* d, int dice_depth, psy_scene** out) {
-    if (!ctx || !d || !out) return fail(PSY_ERR_INVALID, "psy_scene_create: NULL argument");
-    *out = nullptr;
+int prepare_scene(const psy_scene_desc* d, int dice_depth, HostScene& H) {
+    if (!d) return fail(PSY_ERR_INVALID, "psy_scene_create: NULL argument");
     if (d->n_assemblies < 1 || !d->assemblies) return fail(PSY_ERR_INVALID, "psy_scene_create: no root assembly");
     if (d->n_shaders < 0 || (d->n_shaders > 0 && !d->shaders)) return fail(PSY_ERR_INVALID, "psy_scene_create: bad shader table");
-    HIP_OK(hipSetDevice(ctx->device));
-    std::unique_ptr<psy_scene> sc(new psy_scene());
-    sc->ctx = ctx;
+    H = HostScene{};
+    std::vector<DevObject>& objs = H.objs;
+    std::vector<float>& odata = H.odata;
+    std::vector<M44>& xf_pool = H.xf;
 
     // ---- objects (global numbering: assemblies in order) -------------------
     Flattener fl;
     fl.d = d;
-    std::vector<DevObject> objs;
-    std::vector<float> odata;
     std::vector<Box> obox;  // union over time samples
     for (int a = 0; a < d->n_assemblies; ++a) {
         const psy_assembly_desc& ad = d->assemblies[a];
@@ -592,9 +542,14 @@ int psy_scene_create(psy_ctx* ctx, const psy_scene_desc* d, int dice_depth, psy_
         }
     }
     if (!fl.run(0, -1, 0, -1)) return fail(PSY_ERR_INVALID, "psy_scene_create: " + fl.err);
+    if (int rc = prepare_lights(d, fl.obj_base, H)) return rc;
+    for (const FlatLeaf& L : fl.leaves) {
+        const int k = objs[L.object].kind;
+        // a surface without a shader gets the reference's magenta emitter
+        if (L.shader < 0 && k != PSY_OBJ_SPHERE_LIGHT && k != PSY_OBJ_RECTANGLE_LIGHT) H.chromatic = true;
+    }
 
     // ---- leaf instances, transform pool, world bounds ----------------------
-    std::vector<M44> xf_pool;
     std::vector<int32_t> frame_off(fl.frames.size());
     for (size_t f = 0; f < fl.frames.size(); ++f) {
         frame_off[f] = (int32_t)xf_pool.size();
@@ -602,7 +557,8 @@ int psy_scene_create(psy_ctx* ctx, const psy_scene_desc* d, int dice_depth, psy_
     }
     const size_t nl = fl.leaves.size();
     if (nl > (size_t)INT32_MAX / 2) return fail(PSY_ERR_UNSUPPORTED, "psy_scene_create: too many instances");
-    std::vector<DevInstance> insts(nl);
+    std::vector<DevInstance>& insts = H.insts;
+    insts.resize(nl);
     Builder2 b2;
     b2.boxes.resize(nl);
     b2.cent.resize(3 * nl);
@@ -610,12 +566,13 @@ int psy_scene_create(psy_ctx* ctx, const psy_scene_desc* d, int dice_depth, psy_
     for (size_t i = 0; i < nl; ++i) {
         const FlatLeaf& L = fl.leaves[i];
         DevInstance& I = insts[i];
+        I = DevInstance{};
         I.object = L.object;
         I.kind = objs[L.object].kind;
         I.shader = L.shader;
         I.xf_offset = L.frame >= 0 ? frame_off[L.frame] : 0;
         I.xf_count = L.frame >= 0 ? (int32_t)fl.frames[L.frame].size() : 0;
-        if (I.xf_count > 0 && (I.kind == PSY_OBJ_BILINEAR || I.kind == PSY_OBJ_BICUBIC)) sc->transformed_patches = true;
+        if (I.xf_count > 0 && (I.kind == PSY_OBJ_BILINEAR || I.kind == PSY_OBJ_BICUBIC)) H.transformed_patches = true;
         const Box& ob = obox[L.object];
         Box wb;
         if (I.xf_count == 0) {
@@ -645,9 +602,9 @@ int psy_scene_create(psy_ctx* ctx, const psy_scene_desc* d, int dice_depth, psy_
         b2.boxes[i] = wb;
         for (int k = 0; k < 3; ++k) b2.cent[3 * i + k] = 0.5f * (wb.lo[k] + wb.hi[k]);
         b2.idx[i] = (int)i;
-        sc->ids.push_back(L.id);
+        H.ids.push_back(L.id);
     }
-    std::vector<BvhNode> bvh;
+    std::vector<BvhNode>& bvh = H.bvh;
     int32_t root = CHILD_EMPTY;
     int stack_need = 1;
     if (nl == 1) {
@@ -662,10 +619,11 @@ int psy_scene_create(psy_ctx* ctx, const psy_scene_desc* d, int dice_depth, psy_
         stack_need = std::max(1, need) + 1;
         bvh.swap(cl.out);
     }
-    sc->stack_need = stack_need;
+    H.root = root;
+    H.stack_need = stack_need;
 
     // ---- dice cache --------------------------------------------------------
-    std::vector<int32_t> patch_ids;
+    std::vector<int32_t>& patch_ids = H.patch_ids;
     for (size_t i = 0; i < objs.size(); ++i)
         if (objs[i].kind == PSY_OBJ_BILINEAR || objs[i].kind == PSY_OBJ_BICUBIC) patch_ids.push_back((int32_t)i);
     int levels = dice_depth;
@@ -676,52 +634,25 @@ int psy_scene_create(psy_ctx* ctx, const psy_scene_desc* d, int dice_depth, psy_
     }
     if (levels > 20) return fail(PSY_ERR_INVALID, "psy_scene_create: dice_depth above 20");
     const int64_t per = (int64_t)1 << (levels + 1);  // slot: 1-based heap, entry 0 unused
+    H.levels = levels;
+    H.per = per;
     for (size_t k = 0; k < patch_ids.size(); ++k) {
         objs[patch_ids[k]].dice_levels = levels;
         objs[patch_ids[k]].dice = (int64_t)k * per;
     }
+    for (DevInstance& I : insts) {
+        I.dice_levels = objs[I.object].dice_levels;
+        I.dice = objs[I.object].dice;
+    }
 
-    // ---- upload ------------------------------------------------------------
-    int rc;
-    if ((rc = sc->bvh.upload(bvh.data(), bvh.size() * sizeof(BvhNode)))) return rc;
-    if ((rc = sc->inst.upload(insts.data(), insts.size() * sizeof(DevInstance)))) return rc;
-    if ((rc = sc->obj.upload(objs.data(), objs.size() * sizeof(DevObject)))) return rc;
-    if ((rc = sc->odata.upload(odata.data(), odata.size() * sizeof(float)))) return rc;
-    if ((rc = sc->xf.upload(xf_pool.data(), xf_pool.size() * sizeof(M44)))) return rc;
-    std::vector<DevShader> sh(d->n_shaders);
+    // ---- shaders, camera (basics/camera.hpp:23-69) -------------------------
+    H.shaders.resize(d->n_shaders);
     for (int i = 0; i < d->n_shaders; ++i) {
-        sh[i].kind = d->shaders[i].kind;
-        for (int k = 0; k < 3; ++k) sh[i].color[k] = d->shaders[i].color[k];
+        H.shaders[i].kind = d->shaders[i].kind;
+        for (int k = 0; k < 3; ++k) H.shaders[i].color[k] = d->shaders[i].color[k];
     }
-    if ((rc = sc->shaders.upload(sh.data(), sh.size() * sizeof(DevShader)))) return rc;
-    const size_t dice_bytes = (size_t)per * patch_ids.size() * 32;
-    if ((rc = sc->dice.alloc(dice_bytes))) return rc;
-    if (!patch_ids.empty()) {
-        DBuf pid;
-        if ((rc = pid.upload(patch_ids.data(), patch_ids.size() * sizeof(int32_t)))) return rc;
-        rc = launch_build_dice((const float*)sc->odata.p, (const DevObject*)sc->obj.p, (const int32_t*)pid.p,
-                               (int)patch_ids.size(), levels, (float4*)sc->dice.p, nullptr);
-        if (rc) return fail(rc, "psy_scene_create: dice cache build launch failed");
-        HIP_OK(hipDeviceSynchronize());
-    }
-
-    DevScene& s = sc->dev;
-    s.bvh = (const BvhNode*)sc->bvh.p;
-    s.n_bvh = (int32_t)bvh.size();
-    s.root = root;
-    s.inst = (const DevInstance*)sc->inst.p;
-    s.obj = (const DevObject*)sc->obj.p;
-    s.odata = (const float*)sc->odata.p;
-    s.xf = (const M44*)sc->xf.p;
-    s.dice = (const float4*)sc->dice.p;
-    s.shaders = (const DevShader*)sc->shaders.p;
-    s.dice_rate = d->dice_rate;
-    s.min_upoly = d->min_upoly_size;
-    s.n_inst = (int32_t)nl;
-
-    // ---- camera (basics/camera.hpp:23-69) ----------------------------------
     const psy_camera_desc& c = d->camera;
-    DevCamera& cam = sc->cam;
+    DevCamera& cam = H.cam;
     if (c.n_transforms > MAX_CAM_SAMPLES || c.n_fovs > MAX_CAM_SAMPLES || c.n_aperture_radii > MAX_CAM_SAMPLES ||
         c.n_focus_distances > MAX_CAM_SAMPLES)
         return fail(PSY_ERR_UNSUPPORTED, "psy_scene_create: more than 8 camera motion samples");
@@ -745,14 +676,76 @@ int psy_scene_create(psy_ctx* ctx, const psy_scene_desc* d, int dice_depth, psy_
         cam.n_ap = 1; cam.aperture[0] = 0.0f;
         cam.n_fd = 1; cam.focus[0] = 1.0f;
     }
+    H.dice_rate = d->dice_rate;
+    H.min_upoly = d->min_upoly_size;
+    for (int k = 0; k < 3; ++k) H.background[k] = d->background[k];
+    return PSY_OK;
+}
+
+}  // namespace psy
+
+extern "C" {
+
+int psy_scene_create(psy_ctx* ctx, const psy_scene_desc* d, int dice_depth, psy_scene** out) {
+    if (!ctx || !d || !out) return fail(PSY_ERR_INVALID, "psy_scene_create: NULL argument");
+    *out = nullptr;
+    HIP_OK(hipSetDevice(ctx->device));
+    std::unique_ptr<psy_scene> sc(new psy_scene());
+    sc->ctx = ctx;
+    HostScene H;
+    if (int rc = prepare_scene(d, dice_depth, H)) return rc;
+    sc->stack_need = H.stack_need;
+    sc->transformed_patches = H.transformed_patches;
+    sc->chromatic = H.chromatic;
+    sc->ids = H.ids;
+    for (int k = 0; k < 3; ++k) sc->background[k] = H.background[k];
+    sc->cam = H.cam;
+    const std::vector<int32_t>& patch_ids = H.patch_ids;
+    const int levels = H.levels;
+    const int64_t per = H.per;
+    const size_t nl = H.insts.size();
+
+    // ---- upload ------------------------------------------------------------
+    int rc;
+    if ((rc = sc->bvh.upload(H.bvh))) return rc;
+    if ((rc = sc->inst.upload(H.insts))) return rc;
+    if ((rc = sc->obj.upload(H.objs))) return rc;
+    if ((rc = sc->odata.upload(H.odata))) return rc;
+    if ((rc = sc->xf.upload(H.xf))) return rc;
+    if ((rc = sc->shaders.upload(H.shaders))) return rc;
+    if ((rc = upload_lights(H, sc.get()))) return rc;
+    const size_t dice_bytes = (size_t)per * patch_ids.size() * 32;
+    if ((rc = sc->dice.alloc(dice_bytes))) return rc;
+    if (!patch_ids.empty()) {
+        DBuf pid;
+        if ((rc = pid.upload(patch_ids.data(), patch_ids.size() * sizeof(int32_t)))) return rc;
+        rc = launch_build_dice((const float*)sc->odata.p, (const DevObject*)sc->obj.p, (const int32_t*)pid.p,
+                               (int)patch_ids.size(), levels, (float4*)sc->dice.p, nullptr);
+        if (rc) return fail(rc, "psy_scene_create: dice cache build launch failed");
+        HIP_OK(hipDeviceSynchronize());
+    }
+
+    DevScene& s = sc->dev;
+    s.bvh = (const BvhNode*)sc->bvh.p;
+    s.n_bvh = (int32_t)H.bvh.size();
+    s.root = H.root;
+    s.inst = (const DevInstance*)sc->inst.p;
+    s.obj = (const DevObject*)sc->obj.p;
+    s.odata = (const float*)sc->odata.p;
+    s.xf = (const M44*)sc->xf.p;
+    s.dice = (const float4*)sc->dice.p;
+    s.shaders = (const DevShader*)sc->shaders.p;
+    s.dice_rate = H.dice_rate;
+    s.min_upoly = H.min_upoly;
+    s.n_inst = (int32_t)nl;
 
     psy_scene_info& info = sc->info;
     info.n_leaf_instances = (int64_t)nl;
-    info.n_objects = (int64_t)objs.size();
-    info.n_bvh_nodes = (int64_t)bvh.size();
+    info.n_objects = (int64_t)H.objs.size();
+    info.n_bvh_nodes = (int64_t)H.bvh.size();
     info.n_patches = (int64_t)patch_ids.size();
     info.dice_depth = levels;
-    info.bvh_stack_need = stack_need;
+    info.bvh_stack_need = H.stack_need;
     info.dice_cache_bytes = (int64_t)dice_bytes;
     info.device_bytes = (int64_t)(sc->bvh.bytes + sc->inst.bytes + sc->obj.bytes + sc->odata.bytes + sc->xf.bytes +
                                   sc->shaders.bytes + dice_bytes);
@@ -793,10 +786,11 @@ int psy_trace_world(psy_ctx* ctx, const psy_scene* s, const float* rays, int64_t
                     uint32_t flags, void* stream) {
     if (!ctx || !s || (!rays && n > 0)) return fail(PSY_ERR_INVALID, "psy_trace_world: NULL argument");
     if (n < 0) return fail(PSY_ERR_INVALID, "psy_trace_world: negative ray count");
+    if (n > (int64_t)INT32_MAX) return fail(PSY_ERR_INVALID, "psy_trace_world: more than 2^31-1 rays in one call");
     if (int rc = check_hits(hits, n)) return rc;
     HIP_OK(hipSetDevice(ctx->device));
     if (int rc = ensure_queue(ctx, n)) return rc;
-    int rc = launch_trace(s->dev, true, (flags & PSY_TRACE_OCCLUSION_ONLY) != 0, s->stack_need, rays, n, to_dev(hits),
+    int rc = launch_trace(s->dev, true, (flags & PSY_TRACE_OCCLUSION_ONLY) != 0, s->stack_need, rays, n, n, to_dev(hits),
                           (uint32_t*)ctx->queue.p, (uint32_t*)ctx->counters.p, ctx->tuning(true), (hipStream_t)stream);
     return rc ? fail(rc, "psy_trace_world: launch failed") : PSY_OK;
 }
@@ -805,12 +799,13 @@ int psy_trace_rays(psy_ctx* ctx, const psy_scene* s, const float* rays, int64_t 
                    uint32_t flags, void* stream) {
     if (!ctx || !s || (!rays && n > 0)) return fail(PSY_ERR_INVALID, "psy_trace_rays: NULL argument");
     if (n < 0) return fail(PSY_ERR_INVALID, "psy_trace_rays: negative ray count");
+    if (n > (int64_t)INT32_MAX) return fail(PSY_ERR_INVALID, "psy_trace_rays: more than 2^31-1 rays in one call");
     if (s->transformed_patches)
         return fail(PSY_ERR_UNSUPPORTED, "psy_trace_rays: scene has transformed patch instances; their footprint needs world rays (psy_trace_world)");
     if (int rc = check_hits(hits, n)) return rc;
     HIP_OK(hipSetDevice(ctx->device));
     if (int rc = ensure_queue(ctx, n)) return rc;
-    int rc = launch_trace(s->dev, false, (flags & PSY_TRACE_OCCLUSION_ONLY) != 0, s->stack_need, rays, n,
+    int rc = launch_trace(s->dev, false, (flags & PSY_TRACE_OCCLUSION_ONLY) != 0, s->stack_need, rays, n, n,
                           to_dev(hits), (uint32_t*)ctx->queue.p, (uint32_t*)ctx->counters.p, ctx->tuning(false),
                           (hipStream_t)stream);
     return rc ? fail(rc, "psy_trace_rays: launch failed") : PSY_OK;

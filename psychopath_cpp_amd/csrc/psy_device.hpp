@@ -7,7 +7,9 @@
 //            layout, basics/bbox.hpp:461-476), 16 bytes of child refs,
 //            16 bytes of padding (one node = one 128-byte L2 line).
 //   inst   : 32-byte leaf-instance records (object, merged transforms,
-//            effective shader) -- the assembly hierarchy flattened.
+//            effective shader, and for patches the dice-cache slot, so a
+//            traversal reaching a patch leaf reads one record) -- the
+//            assembly hierarchy flattened.
 //   obj    : 32-byte object records; control points / centres in `odata`.
 //   xf     : merged per-leaf transform lists (world -> object), 64 B each.
 //   dice   : the patch dice cache: for every patch a complete 1-based
@@ -40,8 +42,9 @@ struct alignas(16) DevInstance {
     int32_t kind;
     int32_t xf_offset;
     int32_t xf_count;
-    int32_t shader;     // effective shader index, -1 = none
-    int32_t pad[3];
+    int32_t shader;       // effective shader index, -1 = none
+    int32_t dice_levels;  // patches: the object's cached split depth
+    int64_t dice;         // patches: the object's first dice-cache node
 };
 static_assert(sizeof(DevInstance) == 32, "instance record must be 32 bytes");
 
@@ -92,6 +95,38 @@ struct HaltonDim {
 
 struct DevHits { float* t; uint32_t* inst; float* u; float* v; float* aux; };
 
+// ---- light trees and the assembly hierarchy (for light sampling) -----------
+// One binary light tree per assembly (accel/light_tree.hpp:7-66); node
+// indices are local to the assembly's node range.
+struct alignas(16) DevLNode {
+    int32_t leaf;
+    int32_t index1, index2;  // children (inner nodes)
+    int32_t instance;        // the assembly's instance (leaves)
+    float energy;
+    int32_t bounds, n_bounds;  // time-sampled bounds in DevLights::bounds
+    int32_t pad;
+};
+static_assert(sizeof(DevLNode) == 32, "light tree node must be 32 bytes");
+struct alignas(16) DevAsm {
+    int32_t node0, n_nodes;  // light tree nodes
+    int32_t inst0;           // first DevAsmInst
+    int32_t xf0;             // first transform of the assembly's list
+    int32_t sub0;            // first child-assembly scene index in DevLights::sub
+    int32_t obj0;            // first global object
+    int32_t total_lights;
+    int32_t pad;
+};
+struct alignas(16) DevAsmInst { int32_t type, data_index, xf_index, xf_count; };
+struct DevLights {
+    const DevLNode* nodes;
+    const float4* bounds;  // 2 per box: (min.xyz, 0), (max.xyz, 0)
+    const DevAsm* asms;
+    const DevAsmInst* inst;
+    const M44* xf;
+    const int32_t* sub;
+    int32_t n_asms;
+};
+
 // ---- launchers (psy_kernels.hip) -------------------------------------------
 int launch_build_dice(const float* odata, const DevObject* objs, const int32_t* patch_ids,
                       int n_patches, int levels, float4* dice, hipStream_t st);
@@ -105,11 +140,42 @@ struct TraceTuning {
 };
 // Two stream-ordered passes; queue (n entries) and counters (4 x u32) are
 // scratch.
+// Ray plane p of ray i is at rays[p * rs + i] (rs >= n).
 int launch_trace(const DevScene& s, bool world, bool occl_only, int stack_need, const float* rays,
-                 int64_t n, const DevHits& h, uint32_t* queue, uint32_t* counters,
+                 int64_t n, int64_t rs, const DevHits& h, uint32_t* queue, uint32_t* counters,
                  const TraceTuning& tune, hipStream_t st);
 int trace_persist_occupancy(bool world, int* blocks_per_cu);
 int launch_world_to_rays(const float* wrays, int64_t n, float* rays, hipStream_t st);
+
+// ---- integrator (psy_render_dev.hpp) ----------------------------------------
+struct RenderLaunch {
+    DevScene s;
+    DevLights L;
+    DevCamera cam;
+    const HaltonDim* dims;
+    const uint16_t* perm;
+    int32_t width, height, spp, path_length;
+    uint32_t seed;
+    float bg[3];
+};
+constexpr int PATH_PLANES = 45;  // float planes of a path state
+// Path buffers of one chunk of `cap` paths: states (PATH_PLANES), two world
+// ray streams, hits, two active lists, counters (u32 x 4).
+struct RenderBuffers {
+    float* paths;
+    float* rays[2];
+    DevHits hits;
+    uint32_t* act[2];
+    uint32_t* count;  // [0], [1]: next active counts (ping-pong)
+    int64_t cap;
+};
+int launch_render_init(const RenderLaunch& R, int64_t first, int64_t n, const RenderBuffers& B, hipStream_t st);
+// Finished paths write their film XYZ to sample_xyz[3 * (sample - sbase)].
+int launch_render_update(const RenderLaunch& R, int64_t first, const RenderBuffers& B, int cur, int64_t n_in,
+                         float* sample_xyz, int64_t sbase, hipStream_t st);
+int launch_render_resolve(const float* sample_xyz, int64_t sbase, int64_t first_pixel, int64_t n_pixels, int spp,
+                          float* film, hipStream_t st);
+int launch_zero_u32(uint32_t* p, int n, hipStream_t st);
 int launch_halton(const HaltonDim* dims, const uint16_t* perm, uint32_t dim, const uint32_t* idx,
                   int64_t n, float* out, hipStream_t st);
 int launch_pixel_samples(const HaltonDim* dims, const uint16_t* perm, int x0, int y0, int w, int h,

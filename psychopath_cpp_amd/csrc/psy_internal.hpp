@@ -1,0 +1,149 @@
+// psy_internal.hpp -- host-side state shared by the translation units of
+// libpsy_gpu.so (psy_capi.cpp: context, scene upload, tracing;
+// psy_render.cpp: light trees and the path-tracing integrator).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/psy_gpu.h"
+#include "psy_device.hpp"
+
+namespace psy {
+
+// Records `msg` for psy_last_error() and returns `code`.
+int fail(int code, const std::string& msg);
+int floats_per_kind(int kind);
+
+#define HIP_OK(expr)                                                                       \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return ::psy::fail(PSY_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Owning device allocation.
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    int alloc(size_t b) {
+        release();
+        bytes = b;
+        if (b == 0) return 0;
+        hipError_t e = hipMalloc(&p, b);
+        if (e != hipSuccess) {
+            p = nullptr;
+            bytes = 0;
+            return fail(PSY_ERR_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+        }
+        return 0;
+    }
+    // grows (never shrinks) to at least b bytes; contents are not kept
+    int reserve(size_t b) { return b <= bytes && p ? 0 : alloc(b); }
+    int upload(const void* src, size_t b) {
+        int rc = alloc(b);
+        if (rc) return rc;
+        if (b) HIP_OK(hipMemcpy(p, src, b, hipMemcpyHostToDevice));
+        return 0;
+    }
+    template <class T>
+    int upload(const std::vector<T>& v) { return upload(v.data(), v.size() * sizeof(T)); }
+};
+
+}  // namespace psy
+
+struct psy_ctx {
+    int device = 0;
+    psy::DBuf halton_dims, halton_perm;
+    // trace scratch: queue of rays for the second pass + its counters
+    psy::DBuf queue, counters;
+    int64_t queue_cap = 0;
+    // main-pass launch form, per ray form (0: object rays, 1: world rays)
+    int n_cu = 0;
+    int occupancy[2] = {0, 0};
+    int mode = PSY_TRACE_MODE_PLAIN;
+    int refill_min = 16;
+    int blocks_per_cu = 0;  // <= 0: from occupancy
+    psy::TraceTuning tuning(bool world) const {
+        psy::TraceTuning t;
+        t.persistent = mode == PSY_TRACE_MODE_PERSISTENT;
+        t.persist_blocks = n_cu * (blocks_per_cu > 0 ? blocks_per_cu : occupancy[world ? 1 : 0]);
+        t.refill_min = refill_min;
+        return t;
+    }
+    // integrator scratch (path states, ray / hit streams, active lists)
+    psy::DBuf render_scratch;
+    int64_t render_chunk = 1 << 21;  // paths per wavefront chunk
+};
+
+struct psy_scene {
+    psy_ctx* ctx = nullptr;
+    psy::DevScene dev{};
+    psy::DevCamera cam{};
+    psy::DBuf bvh, inst, obj, odata, xf, dice, shaders;
+    int stack_need = 1;
+    bool transformed_patches = false;
+    std::vector<uint64_t> ids;
+    psy_scene_info info{};
+    // integrator data: per-assembly light trees and the assembly hierarchy
+    psy::DevLights lights{};
+    psy::DBuf l_nodes, l_bounds, l_asms, l_inst, l_xf, l_sub;
+    float background[3] = {0.0f, 0.0f, 0.0f};
+    bool chromatic = false;  // some colour is not grey (no spectral table)
+};
+
+namespace psy {
+
+// Everything psy_scene_create derives from a scene description, in host
+// memory, before any device call: the flattened leaves, the BVH4, objects,
+// transforms, shaders, camera, the dice-cache layout (the cache itself is
+// built on the device) and the light trees.
+struct HostScene {
+    std::vector<BvhNode> bvh;
+    int32_t root = CHILD_EMPTY;
+    int stack_need = 1;
+    std::vector<DevInstance> insts;
+    std::vector<DevObject> objs;
+    std::vector<float> odata;
+    std::vector<M44> xf;
+    std::vector<DevShader> shaders;
+    std::vector<int32_t> patch_ids;  // objects with a dice-cache slot, in slot order
+    int levels = 0;                  // cached split depth
+    int64_t per = 2;                 // nodes per slot (1-based heap, entry 0 unused)
+    std::vector<uint64_t> ids;       // InstanceID per leaf
+    bool transformed_patches = false;
+    bool chromatic = false;          // some colour is not grey (or a surface has no shader)
+    float dice_rate = 0.25f, min_upoly = 1e-5f;
+    DevCamera cam{};
+    float background[3] = {0.0f, 0.0f, 0.0f};
+    // light trees of every assembly and the assembly hierarchy
+    std::vector<DevLNode> l_nodes;
+    std::vector<float4> l_bounds;
+    std::vector<DevAsm> l_asms;
+    std::vector<DevAsmInst> l_inst;
+    std::vector<M44> l_xf;
+    std::vector<int32_t> l_sub;
+};
+
+// Host part of psy_scene_create (psy_capi.cpp); no device calls.
+int prepare_scene(const psy_scene_desc* d, int dice_depth, HostScene& H);
+// Light trees of every assembly (psy_render.cpp); obj_base: first global
+// object of each assembly.
+int prepare_lights(const psy_scene_desc* d, const std::vector<int32_t>& obj_base, HostScene& H);
+int upload_lights(const HostScene& H, psy_scene* sc);
+// Faure-permuted Halton tables, 256 dimensions (sampling/halton.py:27-87).
+void make_halton_tables(std::vector<HaltonDim>& dims, std::vector<uint16_t>& perm);
+// Grows the context's trace scratch (second-pass queue) to n rays.
+int ensure_queue(psy_ctx* ctx, int64_t n);
+}  // namespace psy

@@ -17,6 +17,10 @@
 
 namespace psy {
 
+// bit patterns of u32 fields stored in float planes
+PSY_HD uint32_t f2u(float f) { uint32_t u; __builtin_memcpy(&u, &f, 4); return u; }
+PSY_HD float u2f(uint32_t u) { float f; __builtin_memcpy(&f, &u, 4); return f; }
+
 struct V3 { float x, y, z; };
 
 PSY_HD V3 mk(float x, float y, float z) { V3 r; r.x = x; r.y = y; r.z = z; return r; }
@@ -44,6 +48,7 @@ PSY_HD float length_tiny(V3 a) {
     ax /= mx; ay /= mx; az /= mx;
     return mx * sqrtf(ax * ax + ay * ay + az * az);
 }
+PSY_HD float length2(V3 a) { return dot(a, a); }
 PSY_HD float length(V3 a) {
     const float l2 = dot(a, a);
     if (l2 < 2.0f * FLT_MIN) return length_tiny(a);
@@ -59,6 +64,10 @@ PSY_HD V3 vmin(V3 a, V3 b) { return mk(a.x < b.x ? a.x : b.x, a.y < b.y ? a.y : 
 PSY_HD V3 vmax(V3 a, V3 b) { return mk(a.x > b.x ? a.x : b.x, a.y > b.y ? a.y : b.y, a.z > b.z ? a.z : b.z); }
 PSY_HD float smin(float a, float b) { return (b < a) ? b : a; }
 PSY_HD float smax(float a, float b) { return (a < b) ? b : a; }
+// _mm_min_ps / _mm_max_ps lane semantics (the second operand when either
+// is NaN), as BBox4::intersect_ray uses them (basics/bbox.hpp:566-595)
+PSY_HD float sse_min(float a, float b) { return a < b ? a : b; }
+PSY_HD float sse_max(float a, float b) { return a > b ? a : b; }
 PSY_HD float longest_axis(V3 v) { return smax(smax(fabsf(v.x), fabsf(v.y)), fabsf(v.z)); }
 
 // ---- lerp ------------------------------------------------------------------
@@ -226,6 +235,25 @@ PSY_HD bool bbox_ray(V3 bmin, V3 bmax, V3 o, V3 dinv, float t, float* h0, float*
     return a0 <= smin(t, a1);
 }
 
+// bbox_ray for a ray whose inverse direction is finite in every component.
+// Then no slab distance is NaN, and IEEE min/max give the values of the
+// std::min/std::max sequence above (up to the sign of a zero, which no
+// result depends on), in fewer instructions.
+PSY_HD bool bbox_ray_fast(V3 bmin, V3 bmax, V3 o, V3 dinv, float t, float* h0, float* h1) {
+    const float tx1 = (bmin.x - o.x) * dinv.x;
+    const float tx2 = (bmax.x - o.x) * dinv.x;
+    const float ty1 = (bmin.y - o.y) * dinv.y;
+    const float ty2 = (bmax.y - o.y) * dinv.y;
+    const float tz1 = (bmin.z - o.z) * dinv.z;
+    const float tz2 = (bmax.z - o.z) * dinv.z;
+    const float a0 = fmaxf(fmaxf(fminf(tx1, tx2), fminf(ty1, ty2)), fmaxf(fminf(tz1, tz2), 0.0f));
+    const float a1 = fminf(fminf(fmaxf(tx1, tx2), fmaxf(ty1, ty2)), fmaxf(tz1, tz2)) * PSY_BBOX_MAXT_ADJUST;
+    *h0 = a0;
+    *h1 = a1;
+    return a0 <= fminf(t, a1);
+}
+PSY_HD bool finite3(V3 v) { return fabsf(v.x) < INFINITY && fabsf(v.y) < INFINITY && fabsf(v.z) < INFINITY; }
+
 // ---- ray footprint (basics/ray.hpp:122-153, :207-305; utils/utils.hpp:443-456)
 struct RayW { float owx, owy, dwx, dwy, fwx, fwy; };
 
@@ -250,6 +278,21 @@ PSY_HD RayW ray_widths(V3 o, V3 d, V3 odx, V3 ody, V3 ddx, V3 ddy) {
     if (tdy <= 0.0f) { w.dwy = length(ddy); w.fwy = 0.0f; }
     else { w.dwy = (ly - w.owy) / tdy; w.fwy = ly; }
     return w;
+}
+// Ray::min_width with its two flip distances precomputed (they depend on
+// the footprint only): min_width(r, a, b) == min_width_flip(r, flip_x(r),
+// flip_y(r), a, b) bit for bit.
+PSY_HD float flip_x(const RayW& r) { return (r.owx - r.fwx) / r.dwx; }
+PSY_HD float flip_y(const RayW& r) { return (r.owy - r.fwy) / r.dwy; }
+PSY_HD float min_width_flip(const RayW& r, float tflipx, float tflipy, float tnear, float tfar) {
+    float minx, miny;
+    if (tnear < tflipx && tfar > tflipx) minx = r.fwx;
+    else minx = smin(fabsf((r.owx - r.fwx) + (r.dwx * tnear)) + r.fwx,
+                     fabsf((r.owx - r.fwx) + (r.dwx * tfar)) + r.fwx);
+    if (tnear < tflipy && tfar > tflipy) miny = r.fwy;
+    else miny = smin(fabsf((r.owy - r.fwy) + (r.dwy * tnear)) + r.fwy,
+                     fabsf((r.owy - r.fwy) + (r.dwy * tfar)) + r.fwy);
+    return smin(minx, miny);
 }
 PSY_HD float min_width(const RayW& r, float tnear, float tfar) {
     const float tflipx = (r.owx - r.fwx) / r.dwx;

@@ -1,0 +1,465 @@
+// psy_render.cpp -- host side of the path-tracing integrator: per-assembly
+// light trees (LightTree::build, accel/light_tree.cpp:99-238) and the
+// wavefront loop of PathTraceIntegrator (integrator/path_trace_integrator.cpp:
+// 282-384) over the device kernels of psy_render_dev.hpp.
+//
+// The light trees keep the reference's structure exactly (midpoint splits
+// with std::partition, time-sampled node bounds, energies scaled by the
+// instance's inverse scale) because the tree decides which light each
+// shadow sample picks: the same tree gives the same light per sample.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "psy_internal.hpp"
+
+using namespace psy;
+
+namespace {
+
+// ---- BBox with the reference's operations (basics/bbox.hpp) ----------------
+struct LBox {
+    V3 lo = {INFINITY, INFINITY, INFINITY};
+    V3 hi = {-INFINITY, -INFINITY, -INFINITY};
+};
+
+LBox box_union(const LBox& a, const LBox& b) {
+    LBox r;
+    r.lo = mk(a.lo.x < b.lo.x ? a.lo.x : b.lo.x, a.lo.y < b.lo.y ? a.lo.y : b.lo.y, a.lo.z < b.lo.z ? a.lo.z : b.lo.z);
+    r.hi = mk(a.hi.x > b.hi.x ? a.hi.x : b.hi.x, a.hi.y > b.hi.y ? a.hi.y : b.hi.y, a.hi.z > b.hi.z ? a.hi.z : b.hi.z);
+    return r;
+}
+
+LBox box_lerp_seq(float alpha, const std::vector<LBox>& seq) {
+    float na = 0.0f;
+    const int i = lerp_seq_index(alpha, (int)seq.size(), &na);
+    if (i == -1) return seq[0];
+    if (i == -2) return seq.back();
+    LBox r;
+    r.lo = lerpv(na, seq[i].lo, seq[i + 1].lo);
+    r.hi = lerpv(na, seq[i].hi, seq[i + 1].hi);
+    return r;
+}
+
+// BBox::inverse_transformed: the box corners through Transform::pos_from
+LBox inverse_transformed(const LBox& b, const M44& xf) {
+    const M44 inv = m_inverse(xf);
+    LBox r;
+    for (int c = 0; c < 8; ++c) {
+        const V3 v = m_pos(inv, mk((c & 4) ? b.hi.x : b.lo.x, (c & 2) ? b.hi.y : b.lo.y, (c & 1) ? b.hi.z : b.lo.z));
+        r.lo = mk(v.x < r.lo.x ? v.x : r.lo.x, v.y < r.lo.y ? v.y : r.lo.y, v.z < r.lo.z ? v.z : r.lo.z);
+        r.hi = mk(v.x > r.hi.x ? v.x : r.hi.x, v.y > r.hi.y ? v.y : r.hi.y, v.z > r.hi.z ? v.z : r.hi.z);
+    }
+    return r;
+}
+
+// transform_from(bbs, xforms) (basics/bbox.hpp:644-668)
+std::vector<LBox> transform_from(const std::vector<LBox>& in, const M44* xs, int tc) {
+    std::vector<LBox> bbs = in;
+    const int n = (int)bbs.size();
+    if (tc == 0) return bbs;
+    if (n == tc) {
+        for (int i = 0; i < n; ++i) bbs[i] = inverse_transformed(bbs[i], xs[i]);
+    } else if (n > tc) {
+        const float s = (float)(n - 1);
+        for (int i = 0; i < n; ++i) bbs[i] = inverse_transformed(bbs[i], m_lerp_seq((float)i / s, xs, tc));
+    } else {
+        const float s = (float)(tc - 1);
+        std::vector<LBox> t;
+        for (int i = 0; i < tc; ++i) t.push_back(inverse_transformed(box_lerp_seq((float)i / s, bbs), xs[i]));
+        bbs = t;
+    }
+    return bbs;
+}
+
+// merge(vector<BBox>, vector<BBox>) (basics/bbox.hpp:620-641)
+std::vector<LBox> merge(const std::vector<LBox>& a, const std::vector<LBox>& b) {
+    std::vector<LBox> c;
+    if (a.empty()) c = b;
+    else if (b.empty()) c = a;
+    else if (a.size() == b.size()) {
+        for (size_t i = 0; i < a.size(); ++i) c.push_back(box_union(a[i], b[i]));
+    } else if (a.size() > b.size()) {
+        const float s = (float)(a.size() - 1);
+        for (size_t i = 0; i < a.size(); ++i) c.push_back(box_union(a[i], box_lerp_seq((float)i / s, b)));
+    } else {
+        const float s = (float)(b.size() - 1);
+        for (size_t i = 0; i < b.size(); ++i) c.push_back(box_union(b[i], box_lerp_seq((float)i / s, a)));
+    }
+    return c;
+}
+
+V3 box_center(const LBox& b) { return scale(add(b.lo, b.hi), 0.5f); }
+
+// Color::energy of Color(e, e, e) / of an RGB colour (color/color.hpp:369-375)
+float energy3(float r, float g, float b) { return ((r + g) + b) / 3.0f; }
+
+bool grey(const float* c) { return c[0] == c[1] && c[1] == c[2]; }
+
+// Light objects: bounds per time sample, emitted colour at sample 0.
+std::vector<LBox> light_object_bounds(const psy_object_desc& od) {
+    std::vector<LBox> bs;
+    const int nf = floats_per_kind(od.kind);
+    for (int t = 0; t < od.time_samples; ++t) {
+        const float* p = od.data + (size_t)t * nf;
+        LBox b;
+        if (od.kind == PSY_OBJ_SPHERE_LIGHT) {
+            b.lo = sub(mk(p[0], p[1], p[2]), splat(p[3]));
+            b.hi = add(mk(p[0], p[1], p[2]), splat(p[3]));
+        } else {
+            const V3 h = mk(p[0] * 0.5f, p[1] * 0.5f, 0.0f);
+            b.lo = sub(splat(0.0f), h);
+            b.hi = add(splat(0.0f), h);
+        }
+        bs.push_back(b);
+    }
+    return bs;
+}
+float light_object_energy(const psy_object_desc& od) {
+    if (od.kind == PSY_OBJ_SPHERE_LIGHT) return energy3(od.data[4], od.data[5], od.data[6]);
+    if (od.kind == PSY_OBJ_RECTANGLE_LIGHT) return energy3(od.data[2], od.data[3], od.data[4]);
+    return 0.0f;
+}
+
+struct HNode {
+    std::vector<LBox> bounds;
+    float energy = 0.0f;
+    int index1 = 0, index2 = 0;
+    bool leaf = false;
+    int instance = 0;
+};
+
+struct HTree {
+    bool built = false;
+    std::vector<HNode> nodes;
+    std::vector<LBox> bounds;  // LightAccel::bounds()
+    float total_energy = 0.0f;
+    int total_lights = 0;
+};
+
+struct BuildNode {
+    int instance;
+    V3 center;
+    LBox bbox;
+    float energy;
+};
+
+struct TreeBuilder {
+    const psy_scene_desc* d;
+    std::vector<HTree> trees;
+    std::string err;
+
+    std::vector<M44> inst_xforms(const psy_assembly_desc& ad, const psy_instance_desc& in) const {
+        std::vector<M44> xs;
+        for (int k = 0; k < in.transform_count; ++k) {
+            M44 m;
+            std::memcpy(m.m, ad.transforms + 16 * (in.transform_index + k), sizeof(float) * 16);
+            xs.push_back(m);
+        }
+        return xs;
+    }
+
+    // split_lights (accel/light_tree.cpp:159-188)
+    static std::vector<BuildNode>::iterator split(std::vector<BuildNode>::iterator start,
+                                                  std::vector<BuildNode>::iterator end) {
+        V3 mn = start->center, mx = start->center;
+        for (auto it = start + 1; it < end; ++it) {
+            for (int dd = 0; dd < 3; ++dd) {
+                const float c = (&it->center.x)[dd];
+                float& lo = (&mn.x)[dd];
+                float& hi = (&mx.x)[dd];
+                lo = lo < c ? lo : c;
+                hi = hi > c ? hi : c;
+            }
+        }
+        int axis = 0;
+        if ((mx.y - mn.y) > (mx.x - mn.x)) axis = 1;
+        if ((mx.z - mn.z) > (mx.y - mn.y)) axis = 2;
+        const float pmid = .5f * ((&mn.x)[axis] + (&mx.x)[axis]);
+        auto mid = std::partition(start, end, [axis, pmid](const BuildNode& bn) { return (&bn.center.x)[axis] < pmid; });
+        if (mid == start) ++mid;
+        return mid;
+    }
+
+    int recurse(int a, HTree& T, std::vector<BuildNode>::iterator start, std::vector<BuildNode>::iterator end) {
+        const psy_assembly_desc& ad = d->assemblies[a];
+        const int me = (int)T.nodes.size();
+        T.nodes.emplace_back();
+        if (start + 1 == end) {
+            const psy_instance_desc& in = ad.instances[start->instance];
+            const std::vector<M44> xs = inst_xforms(ad, in);
+            std::vector<LBox> bs;
+            if (in.type == PSY_INSTANCE_OBJECT) {
+                bs = transform_from(light_object_bounds(ad.objects[in.data_index]), xs.data(), (int)xs.size());
+            } else {
+                const HTree& sub = trees[ad.assemblies[in.data_index]];
+                bs = transform_from(sub.bounds, xs.data(), (int)xs.size());
+            }
+            T.nodes[me].leaf = true;
+            T.nodes[me].instance = start->instance;
+            T.nodes[me].bounds = bs;
+            T.nodes[me].energy = start->energy;
+        } else {
+            auto mid = split(start, end);
+            const int c1 = recurse(a, T, start, mid);
+            const int c2 = recurse(a, T, mid, end);
+            T.nodes[me].leaf = false;
+            T.nodes[me].index1 = c1;
+            T.nodes[me].index2 = c2;
+            T.nodes[me].bounds = merge(T.nodes[c1].bounds, T.nodes[c2].bounds);
+            T.nodes[me].energy = T.nodes[c1].energy + T.nodes[c2].energy;
+        }
+        return me;
+    }
+
+    static float surface_scale(const std::vector<M44>& xs) {
+        const M44 x = xs.empty() ? m_identity() : m_lerp_seq(0.5f, xs.data(), (int)xs.size());
+        // Transform::get_inv_scale (math/transform.hpp:104-111)
+        float s[3];
+        for (int r = 0; r < 3; ++r)
+            s[r] = std::sqrt((x.m[4 * r + 0] * x.m[4 * r + 0]) + (x.m[4 * r + 1] * x.m[4 * r + 1]) +
+                             (x.m[4 * r + 2] * x.m[4 * r + 2]));
+        return ((s[0] * s[1]) + (s[0] * s[2]) + (s[1] * s[2])) * 0.33333333f;
+    }
+
+    bool build(int a, int depth) {
+        if (depth > 64) { err = "assembly nesting deeper than 64 (cycle?)"; return false; }
+        if (trees[a].built) return true;
+        const psy_assembly_desc& ad = d->assemblies[a];
+        for (int i = 0; i < ad.n_instances; ++i) {
+            const psy_instance_desc& in = ad.instances[i];
+            if (in.type == PSY_INSTANCE_ASSEMBLY && !build(ad.assemblies[in.data_index], depth + 1)) return false;
+        }
+        HTree& T = trees[a];
+        std::vector<BuildNode> bn;
+        for (int i = 0; i < ad.n_instances; ++i) {
+            const psy_instance_desc& in = ad.instances[i];
+            const std::vector<M44> xs = inst_xforms(ad, in);
+            if (in.type == PSY_INSTANCE_OBJECT) {
+                const psy_object_desc& od = ad.objects[in.data_index];
+                const float e = light_object_energy(od);
+                if (!(e > 0.0f)) continue;
+                // Assembly::instance_bounds_at(0.5, i)
+                LBox b = box_lerp_seq(0.5f, light_object_bounds(od));
+                if (!xs.empty()) b = inverse_transformed(b, m_lerp_seq(0.5f, xs.data(), (int)xs.size()));
+                bn.push_back(BuildNode{i, box_center(b), b, e / surface_scale(xs)});
+                T.total_lights += 1;
+            } else {
+                const HTree& sub = trees[ad.assemblies[in.data_index]];
+                const float e = energy3(sub.total_energy, sub.total_energy, sub.total_energy);
+                if (!(sub.total_lights > 0 && e > 0.0f)) continue;
+                LBox b = box_lerp_seq(0.5f, sub.bounds);
+                if (!xs.empty()) b = inverse_transformed(b, m_lerp_seq(0.5f, xs.data(), (int)xs.size()));
+                bn.push_back(BuildNode{i, box_center(b), b, e / surface_scale(xs)});
+                T.total_lights += sub.total_lights;
+            }
+        }
+        if (!bn.empty()) {
+            recurse(a, T, bn.begin(), bn.end());
+            T.bounds = T.nodes[0].bounds;
+            T.total_energy = T.nodes[0].energy;
+        } else {
+            T.bounds.assign(1, LBox());
+        }
+        T.built = true;
+        return true;
+    }
+};
+
+}  // namespace
+
+namespace psy {
+
+int prepare_lights(const psy_scene_desc* d, const std::vector<int32_t>& obj_base, HostScene& H) {
+    TreeBuilder tb;
+    tb.d = d;
+    tb.trees.resize(d->n_assemblies);
+    // colours the device path can upsample: grey only
+    for (int i = 0; i < d->n_shaders; ++i)
+        if (!grey(d->shaders[i].color)) H.chromatic = true;
+    if (!grey(d->background)) H.chromatic = true;
+    for (int a = 0; a < d->n_assemblies; ++a) {
+        const psy_assembly_desc& ad = d->assemblies[a];
+        for (int i = 0; i < ad.n_objects; ++i) {
+            const psy_object_desc& od = ad.objects[i];
+            const int nf = floats_per_kind(od.kind);
+            const int c0 = od.kind == PSY_OBJ_SPHERE_LIGHT ? 4 : (od.kind == PSY_OBJ_RECTANGLE_LIGHT ? 2 : -1);
+            if (c0 < 0) continue;
+            for (int t = 0; t < od.time_samples; ++t)
+                if (!grey(od.data + (size_t)t * nf + c0)) H.chromatic = true;
+        }
+    }
+    if (!tb.build(0, 0)) return fail(PSY_ERR_INVALID, "psy_scene_create: " + tb.err);
+
+    std::vector<DevLNode>& nodes = H.l_nodes;
+    std::vector<float4>& bounds = H.l_bounds;
+    std::vector<DevAsm>& asms = H.l_asms;
+    std::vector<DevAsmInst>& ainst = H.l_inst;
+    std::vector<M44>& xf = H.l_xf;
+    std::vector<int32_t>& subs = H.l_sub;
+    asms.assign(d->n_assemblies, DevAsm{});
+    for (int a = 0; a < d->n_assemblies; ++a) {
+        const psy_assembly_desc& ad = d->assemblies[a];
+        const HTree& T = tb.trees[a];
+        DevAsm& A = asms[a];
+        A = DevAsm{};
+        A.node0 = (int32_t)nodes.size();
+        A.n_nodes = T.built ? (int32_t)T.nodes.size() : 0;
+        A.inst0 = (int32_t)ainst.size();
+        A.xf0 = (int32_t)xf.size();
+        A.sub0 = (int32_t)subs.size();
+        A.obj0 = obj_base[a];
+        A.total_lights = T.total_lights;
+        if (T.built) {
+            for (const HNode& h : T.nodes) {
+                DevLNode n{};
+                n.leaf = h.leaf ? 1 : 0;
+                n.index1 = h.index1;
+                n.index2 = h.index2;
+                n.instance = h.instance;
+                n.energy = h.energy;
+                n.bounds = (int32_t)(bounds.size() / 2);
+                n.n_bounds = (int32_t)h.bounds.size();
+                for (const LBox& b : h.bounds) {
+                    bounds.push_back(make_float4(b.lo.x, b.lo.y, b.lo.z, 0.0f));
+                    bounds.push_back(make_float4(b.hi.x, b.hi.y, b.hi.z, 0.0f));
+                }
+                nodes.push_back(n);
+            }
+        }
+        for (int i = 0; i < ad.n_instances; ++i) {
+            const psy_instance_desc& in = ad.instances[i];
+            ainst.push_back(DevAsmInst{in.type, in.data_index, in.transform_index, in.transform_count});
+        }
+        for (int k = 0; k < ad.n_transforms; ++k) {
+            M44 m;
+            std::memcpy(m.m, ad.transforms + 16 * k, sizeof(float) * 16);
+            xf.push_back(m);
+        }
+        for (int k = 0; k < ad.n_assemblies; ++k) subs.push_back(ad.assemblies[k]);
+    }
+    return PSY_OK;
+}
+
+int upload_lights(const HostScene& H, psy_scene* sc) {
+    int rc;
+    if ((rc = sc->l_nodes.upload(H.l_nodes))) return rc;
+    if ((rc = sc->l_bounds.upload(H.l_bounds))) return rc;
+    if ((rc = sc->l_asms.upload(H.l_asms))) return rc;
+    if ((rc = sc->l_inst.upload(H.l_inst))) return rc;
+    if ((rc = sc->l_xf.upload(H.l_xf))) return rc;
+    if ((rc = sc->l_sub.upload(H.l_sub))) return rc;
+    DevLights& L = sc->lights;
+    L.nodes = (const DevLNode*)sc->l_nodes.p;
+    L.bounds = (const float4*)sc->l_bounds.p;
+    L.asms = (const DevAsm*)sc->l_asms.p;
+    L.inst = (const DevAsmInst*)sc->l_inst.p;
+    L.xf = (const M44*)sc->l_xf.p;
+    L.sub = (const int32_t*)sc->l_sub.p;
+    L.n_asms = (int32_t)H.l_asms.size();
+    return PSY_OK;
+}
+
+}  // namespace psy
+
+// ---------------------------------------------------------------------------
+// psy_render: PathTraceIntegrator::integrate over the whole image
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int psy_render(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, float* film_xyz,
+               float* sample_xyz, int64_t* rays_traced, void* stream) {
+    if (!ctx || !s || !prm) return fail(PSY_ERR_INVALID, "psy_render: NULL argument");
+    if (prm->width <= 0 || prm->height <= 0 || prm->spp <= 0)
+        return fail(PSY_ERR_INVALID, "psy_render: width, height and spp must be positive");
+    if (prm->width > 32767 || prm->height > 32767)
+        return fail(PSY_ERR_INVALID, "psy_render: image sides are limited to 32767 pixels (short pixel coordinates)");
+    if (prm->path_length < 1 || prm->path_length > 32)
+        return fail(PSY_ERR_INVALID, "psy_render: path_length must be in 1..32");
+    if (s->chromatic)
+        return fail(PSY_ERR_UNSUPPORTED,
+                    "psy_render: the scene has non-grey colours (or surfaces without a shader, which emit magenta); "
+                    "the device integrator upsamples grey colours only");
+    if (rays_traced) *rays_traced = 0;
+    HIP_OK(hipSetDevice(ctx->device));
+    hipStream_t st = (hipStream_t)stream;
+
+    const int64_t spp = prm->spp;
+    const int64_t n_pix = (int64_t)prm->width * prm->height;
+    const int64_t total = n_pix * spp;
+    // chunk: whole pixels, at most ctx->render_chunk paths (at least one pixel)
+    int64_t chunk_pix = std::max<int64_t>(1, ctx->render_chunk / spp);
+    chunk_pix = std::min(chunk_pix, n_pix);
+    const int64_t cap = chunk_pix * spp;
+    if (cap > (int64_t)INT32_MAX) return fail(PSY_ERR_UNSUPPORTED, "psy_render: more than 2^31 paths per chunk");
+
+    // scratch layout (floats): paths, 2 ray streams, hits, active lists, counters, chunk samples
+    const int64_t f_paths = (int64_t)PATH_PLANES * cap;
+    const int64_t f_rays = 20 * cap;
+    const int64_t f_hits = 5 * cap;
+    const int64_t f_act = cap;
+    const int64_t f_samp = sample_xyz ? 0 : 3 * cap;
+    const int64_t f_total = f_paths + 2 * f_rays + f_hits + 2 * f_act + 64 + f_samp;
+    if (int rc = ctx->render_scratch.reserve((size_t)f_total * sizeof(float))) return rc;
+    if (int rc = ensure_queue(ctx, cap)) return rc;
+    float* base = (float*)ctx->render_scratch.p;
+    RenderBuffers B;
+    B.cap = cap;
+    B.paths = base;
+    B.rays[0] = base + f_paths;
+    B.rays[1] = B.rays[0] + f_rays;
+    float* h = B.rays[1] + f_rays;
+    B.hits = DevHits{h, (uint32_t*)(h + cap), h + 2 * cap, h + 3 * cap, h + 4 * cap};
+    B.act[0] = (uint32_t*)(h + f_hits);
+    B.act[1] = B.act[0] + cap;
+    B.count = B.act[1] + cap;
+    float* chunk_samples = (float*)(B.count + 64);
+
+    RenderLaunch R;
+    R.s = s->dev;
+    R.L = s->lights;
+    R.cam = s->cam;
+    R.dims = (const HaltonDim*)ctx->halton_dims.p;
+    R.perm = (const uint16_t*)ctx->halton_perm.p;
+    R.width = prm->width;
+    R.height = prm->height;
+    R.spp = prm->spp;
+    R.path_length = prm->path_length;
+    R.seed = prm->seed;
+    for (int k = 0; k < 3; ++k) R.bg[k] = s->background[k];
+
+    int64_t traced = 0;
+    for (int64_t first = 0; first < total; first += cap) {
+        const int64_t n = std::min(cap, total - first);
+        float* samp = sample_xyz ? sample_xyz : chunk_samples;
+        const int64_t sbase = sample_xyz ? 0 : first;
+        int rc = launch_render_init(R, first, n, B, st);
+        if (rc) return fail(rc, "psy_render: init launch failed");
+        int64_t n_active = n;
+        int cur = 0;
+        for (int step = 0; step < 2 * prm->path_length && n_active > 0; ++step) {
+            traced += n_active;
+            rc = launch_trace(s->dev, true, false, s->stack_need, B.rays[cur], n_active, cap, B.hits,
+                              (uint32_t*)ctx->queue.p, (uint32_t*)ctx->counters.p, ctx->tuning(true), st);
+            if (rc) return fail(rc, "psy_render: trace launch failed");
+            if ((rc = launch_zero_u32(B.count + (cur ^ 1), 1, st))) return fail(rc, "psy_render: launch failed");
+            rc = launch_render_update(R, first, B, cur, n_active, samp, sbase, st);
+            if (rc) return fail(rc, "psy_render: update launch failed");
+            uint32_t next = 0;
+            HIP_OK(hipMemcpyAsync(&next, B.count + (cur ^ 1), sizeof next, hipMemcpyDeviceToHost, st));
+            HIP_OK(hipStreamSynchronize(st));
+            n_active = next;
+            cur ^= 1;
+        }
+        if (film_xyz) {
+            rc = launch_render_resolve(samp, sbase, first / spp, n / spp, (int)spp, film_xyz, st);
+            if (rc) return fail(rc, "psy_render: resolve launch failed");
+        }
+    }
+    if (rays_traced) *rays_traced = traced;
+    return PSY_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,729 @@
+// psy_render_dev.hpp -- per-path device functions of the wavefront path
+// tracer; psy_kernels.hip wraps them in kernels.  Included after
+// psy_trace.hpp, inside namespace psy.
+//
+// Restates, per path and per wavefront step:
+//   init_path / next_ray_for_path / update_path
+//                       integrator/path_trace_integrator.cpp:91-279
+//   LightTree::sample, node_prob          accel/light_tree.cpp:6-96
+//   SphereLight::sample / sample_pdf      lights/sphere_light.hpp:58-141
+//   RectangleLight::sample / sample_pdf   lights/rectangle_light.hpp:40-93
+//   uniform_sample_spherical_triangle, uniform_sample_sphere,
+//   spherical_triangle_solid_angle, sphere_lambert
+//                                         utils/monte_carlo.hpp:82-233
+//   LambertClosure evaluate / sample_pdf / estimate_eval_over_solid_angle
+//                                         shading/surface_closure.hpp:233-331
+//   power_heuristic                       utils/mis.hpp:11-16
+//   SpectralSample, Color_XYZ, XYZ_NORM_FAC
+//                                         color/color.hpp:15-188
+// Colours are upsampled for grey (r == g == b) values only: there the
+// Hanika grid's spectrum is flat, (X+Y+Z)/3 at every wavelength
+// (color/color.hpp:490-511).  The host refuses scenes with other colours.
+#pragma once
+
+#include "psy_trace.hpp"
+
+namespace psy {
+
+constexpr float WL_MIN = 380.0f;
+constexpr float WL_MAX = 700.0f;
+constexpr int NSPEC = 4;  // hero wavelength + 3 (color/color.hpp:15)
+constexpr double PSY_PI_D = 3.1415926535897932384626433;
+
+// ---- path state planes ------------------------------------------------------
+// One float plane per field, `cap` entries each; u32 fields are bit patterns.
+enum : int {
+    P_OFF = 0,       // sampler offset (hash of pixel and seed + sample index)
+    P_DIM = 1,       // next sampler dimension
+    P_PIX = 2,       // x | y << 16
+    P_STEP = 3,      // step (bits 0-7), bit 8: stored hit is backfacing
+    P_TIME = 4,
+    P_WL = 5,        // hero wavelength
+    P_LASTPDF = 6,
+    P_COL = 7,       // 4 spectral samples each
+    P_FCOL = 11,
+    P_LCOL = 15,
+    P_GP = 19,       // stored hit: world position, normal, origin offset
+    P_GN = 22,
+    P_POFF = 25,
+    P_T = 28,
+    P_SHADER = 29,   // u32 shader index of the stored hit
+    P_PD = 30,       // the ray that found it: direction and differentials
+    P_PODX = 33,
+    P_PODY = 36,
+    P_PDDX = 39,
+    P_PDDY = 42,
+    P_COUNT = 45
+};
+static_assert(P_COUNT == PATH_PLANES, "path state layout");
+constexpr uint32_t STEP_BACKFACING = 1u << 8;
+
+struct PathBuf {
+    float* p;     // P_COUNT planes
+    int64_t cap;  // entries per plane
+    PSY_DEV float& f(int plane, int64_t i) const { return p[plane * cap + i]; }
+    PSY_DEV uint32_t u(int plane, int64_t i) const { return f2u(p[plane * cap + i]); }
+    PSY_DEV void setu(int plane, int64_t i, uint32_t v) const { p[plane * cap + i] = u2f(v); }
+    PSY_DEV V3 v3(int plane, int64_t i) const { return mk(f(plane, i), f(plane + 1, i), f(plane + 2, i)); }
+    PSY_DEV void set3(int plane, int64_t i, V3 v) const {
+        f(plane, i) = v.x; f(plane + 1, i) = v.y; f(plane + 2, i) = v.z;
+    }
+};
+
+// ---- spectral ---------------------------------------------------------------
+// Color_to_SpectralSample for a grey colour: Color -> XYZ (E white), the
+// flat spectrum of the white point scaled by X+Y+Z.
+PSY_DEV float color_spec(float r, float g, float b) {
+    const float x = ((r * 0.4339f) + (g * 0.3762f)) + (b * 0.1899f);
+    const float y = ((r * 0.2126f) + (g * 0.7152f)) + (b * 0.0722f);
+    const float z = ((r * 0.0177f) + (g * 0.1095f)) + (b * 0.8728f);
+    const float norm = (float)(1.0 / (double)(x + y + z));
+    if (!(norm < 3.402823466e+38f)) return 0.0f;
+    return (1.0f / 3.0f) / norm;
+}
+
+PSY_DEV float wavelength_n(float hero, int n) {
+    hero += (float)n * ((WL_MAX - WL_MIN) / NSPEC);
+    if (hero > WL_MAX) hero -= (WL_MAX - WL_MIN);
+    return hero;
+}
+
+// Wyman et al. CIE 1931 fits (color/color.hpp:150-165)
+PSY_DEV float x_1931(float w) {
+    const float t1 = (w - 442.0f) * ((w < 442.0f) ? 0.0624f : 0.0374f);
+    const float t2 = (w - 599.8f) * ((w < 599.8f) ? 0.0264f : 0.0323f);
+    const float t3 = (w - 501.1f) * ((w < 501.1f) ? 0.0490f : 0.0382f);
+    return (0.362f * expf(-0.5f * t1 * t1)) + (1.056f * expf(-0.5f * t2 * t2)) - (0.065f * expf(-0.5f * t3 * t3));
+}
+PSY_DEV float y_1931(float w) {
+    const float t1 = (w - 568.8f) * ((w < 568.8f) ? 0.0213f : 0.0247f);
+    const float t2 = (w - 530.9f) * ((w < 530.9f) ? 0.0613f : 0.0322f);
+    return (0.821f * expf(-0.5f * t1 * t1)) + (0.286f * expf(-0.5f * t2 * t2));
+}
+PSY_DEV float z_1931(float w) {
+    const float t1 = (w - 437.0f) * ((w < 437.0f) ? 0.0845f : 0.0278f);
+    const float t2 = (w - 459.0f) * ((w < 459.0f) ? 0.0385f : 0.0725f);
+    return (1.217f * expf(-0.5f * t1 * t1)) + (0.681f * expf(-0.5f * t2 * t2));
+}
+
+// Color_XYZ(SpectralSample) * XYZ_NORM_FAC: a path's film contribution
+PSY_DEV void path_film_xyz(float hero, const float col[NSPEC], float out[3]) {
+    const float norm_fac = 0.009358239977091027f * (700.0f - 380.0f);
+    float x = 0.0f, y = 0.0f, z = 0.0f;
+    for (int i = 0; i < NSPEC; ++i) {
+        const float w = wavelength_n(hero, i);
+        x += x_1931(w) * col[i];
+        y += y_1931(w) * col[i];
+        z += z_1931(w) * col[i];
+    }
+    x *= 1.0f / NSPEC;
+    y *= 1.0f / NSPEC;
+    z *= 1.0f / NSPEC;
+    out[0] = x * norm_fac;
+    out[1] = y * norm_fac;
+    out[2] = z * norm_fac;
+}
+
+// ---- sampling helpers -------------------------------------------------------
+PSY_DEV double dclamp(double x, double a, double b) { return x < a ? a : (x > b ? b : x); }
+
+PSY_DEV V3 uniform_sample_sphere(float u, float v) {
+    const float z = 1.f - (2.f * u);
+    const float r = sqrtf(smax(0.f, 1.f - z * z));
+    const float phi = (float)(2.f * PSY_PI_D * (double)v);
+    return mk(r * cosf(phi), r * sinf(phi), z);
+}
+
+// Solid angle of a spherical triangle (unit vectors A, B, C).
+PSY_DEV_FN float sph_tri_area(V3 A, V3 B, V3 C) {
+    const double cos_a = dclamp((double)dot(B, C), -1.0, 1.0);
+    const double cos_b = dclamp((double)dot(C, A), -1.0, 1.0);
+    const double cos_c = dclamp((double)dot(A, B), -1.0, 1.0);
+    const double sin_a = sqrt(1.0 - (cos_a * cos_a));
+    const double sin_b = sqrt(1.0 - (cos_b * cos_b));
+    const double sin_c = sqrt(1.0 - (cos_c * cos_c));
+    if (cos_a == 1.0 || cos_b == 1.0 || cos_c == 1.0) return 0.0f;
+    const double cos_A = dclamp((cos_a - (cos_b * cos_c)) / (sin_b * sin_c), -1.0, 1.0);
+    const double cos_B = dclamp((cos_b - (cos_c * cos_a)) / (sin_c * sin_a), -1.0, 1.0);
+    const double cos_C = dclamp((cos_c - (cos_a * cos_b)) / (sin_a * sin_b), -1.0, 1.0);
+    return (float)(acos(cos_A) + acos(cos_B) + acos(cos_C) - PSY_PI_D);
+}
+
+PSY_DEV_FN V3 uniform_sample_spherical_triangle(V3 A, V3 B, V3 C, float i, float j) {
+    const double cos_a = dclamp((double)dot(B, C), -1.0, 1.0);
+    const double cos_b = dclamp((double)dot(C, A), -1.0, 1.0);
+    const double cos_c = dclamp((double)dot(A, B), -1.0, 1.0);
+    const double sin_a = sqrt(1.0 - (cos_a * cos_a));
+    const double sin_b = sqrt(1.0 - (cos_b * cos_b));
+    const double sin_c = sqrt(1.0 - (cos_c * cos_c));
+    if (cos_a == 1.0 || cos_b == 1.0 || cos_c == 1.0) return A;
+    const double cos_A = dclamp((cos_a - (cos_b * cos_c)) / (sin_b * sin_c), -1.0, 1.0);
+    const double cos_B = dclamp((cos_b - (cos_c * cos_a)) / (sin_c * sin_a), -1.0, 1.0);
+    const double cos_C = dclamp((cos_c - (cos_a * cos_b)) / (sin_a * sin_b), -1.0, 1.0);
+    const double sin_A = sqrt(1.0 - (cos_A * cos_A));
+    const double ang_A = acos(cos_A), ang_B = acos(cos_B), ang_C = acos(cos_C);
+    const double area = ang_A + ang_B + ang_C - PSY_PI_D;
+    const double area_2 = area * i;
+    const double s = sin(area_2 - ang_A);
+    const double t = cos(area_2 - ang_A);
+    const double u = t - cos_A;
+    const double v = s + (sin_A * cos_c);
+    const double q_top = (((v * t) - (u * s)) * cos_A) - v;
+    const double q_bottom = ((v * s) + (u * t)) * sin_A;
+    const double q = q_top / q_bottom;
+    // Vec3<float> * double: the factor is rounded to float first
+    const V3 C_2 = add(scale(A, (float)q), scale(normalized(sub(C, scale(A, dot(C, A)))), (float)sqrt(1.0 - (q * q))));
+    const double z = 1.0 - (j * (1.0 - (double)dot(C_2, B)));
+    return add(scale(B, (float)z), scale(normalized(sub(C_2, scale(B, dot(C_2, B)))), (float)sqrt(1.0 - (z * z))));
+}
+
+PSY_DEV_FN float sphere_lambert(float nlcos, float rcos) {
+    const double HPI = PSY_PI_D / 2;
+    const float nlsin = (float)sqrt(1.0 - (double)(nlcos * nlcos));
+    const float rsin2 = (float)(1.0 - (double)(rcos * rcos));
+    const float rsin = sqrtf(rsin2);
+    const float ysin = rcos / nlsin;
+    const float ycos2 = (float)(1.0 - (double)(ysin * ysin));
+    const float ycos = sqrtf(ycos2);
+    const float g = (float)((-2.0 * (double)nlsin * (double)rcos * (double)ycos) + HPI - (double)asinf(ysin) +
+                            (double)(ysin * ycos));
+    const float h = nlcos * ((ycos * sqrtf(rsin2 - ycos2)) + (rsin2 * asinf(ycos / rsin)));
+    const float nl = acosf(nlcos);
+    const float r = acosf(rcos);
+    if (nl < (float)(HPI - (double)r)) return nlcos * rsin2;
+    else if (nl < (float)HPI) return (nlcos * rsin2) + g - h;
+    else if (nl < (float)(HPI + (double)r)) return (float)((double)(g + h) * (1.0 / PSY_PI_D));
+    return 0.0f;
+}
+
+// LambertClosure::estimate_eval_over_solid_angle
+PSY_DEV_FN float lambert_estimate(V3 in, V3 out, float cos_theta, V3 nor) {
+    if (cos_theta < 0.0f) return 1.0f;
+    V3 nn = normalized(nor);
+    const V3 v = normalized(out);
+    if (dot(nn, in) > 0.0f) nn = scale(nn, -1.0f);
+    const float cos_nv = dot(nn, v);
+    return sphere_lambert(cos_nv, cos_theta);
+}
+
+// LambertClosure::evaluate: the grey albedo * max(0, n.v)/pi, upsampled
+PSY_DEV float lambert_eval(const float albedo[3], V3 in, V3 out, V3 gn) {
+    V3 nn = normalized(gn);
+    const V3 v = normalized(out);
+    if (dot(nn, in) > 0.0f) nn = scale(nn, -1.0f);
+    const float fac = smax(dot(nn, v), 0.0f) * (float)(1.0 / PSY_PI_D);
+    return color_spec(albedo[0] * fac, albedo[1] * fac, albedo[2] * fac);
+}
+
+// LambertClosure::sample_pdf
+PSY_DEV float lambert_pdf(V3 in, V3 out, V3 gn) {
+    V3 nn = normalized(gn);
+    const V3 v = normalized(out);
+    if (dot(nn, normalized(in)) > 0.0f) nn = scale(nn, -1.0f);
+    return smax(dot(nn, v) * (float)(1.0 / PSY_PI_D), 0.0f);
+}
+
+// ---- lights -----------------------------------------------------------------
+PSY_DEV void rect_light_at(const float* d, int ts, float time, float* w, float* h, float col[3]) {
+    float na = 0.0f;
+    const int i = lerp_seq_index(time, ts, &na);
+    if (i == -1 || i == -2) {
+        const float* e = d + (i == -1 ? 0 : (ts - 1) * 5);
+        *w = e[0]; *h = e[1];
+        col[0] = e[2]; col[1] = e[3]; col[2] = e[4];
+        return;
+    }
+    const float f = lerp_fac(na);
+    const float* a = d + i * 5;
+    const float* b = d + (i + 1) * 5;
+    *w = (a[0] * f) + (b[0] * na);
+    *h = (a[1] * f) + (b[1] * na);
+    for (int k = 0; k < 3; ++k) col[k] = (a[2 + k] * f) + (b[2 + k] * na);
+}
+
+PSY_DEV void sphere_light_at(const float* d, int ts, float time, V3* pos, float* rad, float col[3]) {
+    float na = 0.0f;
+    const int i = lerp_seq_index(time, ts, &na);
+    if (i == -1 || i == -2) {
+        const float* e = d + (i == -1 ? 0 : (ts - 1) * 7);
+        *pos = mk(e[0], e[1], e[2]); *rad = e[3];
+        col[0] = e[4]; col[1] = e[5]; col[2] = e[6];
+        return;
+    }
+    const float* a = d + i * 7;
+    const float* b = d + (i + 1) * 7;
+    *pos = lerpv(na, mk(a[0], a[1], a[2]), mk(b[0], b[1], b[2]));
+    *rad = lerpf(na, a[3], b[3]);
+    for (int k = 0; k < 3; ++k) col[k] = (a[4 + k] * lerp_fac(na)) + (b[4 + k] * na);
+}
+
+// RectangleLight::sample_pdf
+PSY_DEV_FN float rect_light_pdf(float w, float h, V3 arr) {
+    const V3 p1 = normalized(sub(mk(w * 0.5f, h * 0.5f, 0.0f), arr));
+    const V3 p2 = normalized(sub(mk(w * -0.5f, h * 0.5f, 0.0f), arr));
+    const V3 p3 = normalized(sub(mk(w * -0.5f, h * -0.5f, 0.0f), arr));
+    const V3 p4 = normalized(sub(mk(w * 0.5f, h * -0.5f, 0.0f), arr));
+    return 1.0f / (sph_tri_area(p2, p1, p3) + sph_tri_area(p4, p1, p3));
+}
+
+// SphereLight::sample_pdf
+PSY_DEV_FN float sphere_light_pdf(V3 pos, double radius, V3 arr) {
+    const double d2 = (double)length2(sub(pos, arr));
+    const double d = sqrt(d2);
+    if (d > radius) {
+        const double s2 = (radius * radius) / d2 < 1.0 ? (radius * radius) / d2 : 1.0;
+        const double cos_theta_max = sqrt(1.0 - s2);
+        return (float)(1.0 / (2.0 * PSY_PI_D * (1.0 - cos_theta_max)));
+    }
+    return (float)(1.0f / (4.0f * PSY_PI_D));
+}
+
+struct LQuery {
+    float n, u, v;
+    V3 pos, nor, d;
+    float time;
+    M44 xform;
+    V3 to_light;
+    float spec;
+    float selection_pdf, light_sample_pdf;
+};
+
+PSY_DEV_FN void sphere_light_sample(const float* od, int ts, LQuery& q, float* pdf) {
+    V3 pos; float radf; float col[3];
+    sphere_light_at(od, ts, q.time, &pos, &radf, col);
+    const V3 arr = q.pos;
+    const double radius = radf;
+    const double surface_area_inv = 1.0 / (4.0 * PSY_PI_D * radius * radius);
+    V3 x, y, z = sub(pos, arr);
+    const double d2 = (double)length2(z);
+    const double d = sqrt(d2);
+    coordinate_system_from_vec3(z, &x, &y);
+    x = normalized(x); y = normalized(y); z = normalized(z);
+    q.spec = color_spec(col[0] * (float)surface_area_inv, col[1] * (float)surface_area_inv,
+                        col[2] * (float)surface_area_inv);
+    if (d > radius) {
+        const double s2 = (radius * radius) / d2 < 1.0 ? (radius * radius) / d2 : 1.0;
+        const double c2 = 1.0 - s2;
+        const double sin_theta_max = sqrt(s2);
+        const double cos_theta_max = sqrt(c2);
+        const double du = (double)q.u, dv = (double)q.v;
+        // uniform_sample_cone
+        const double ct = (1.0 - du) + (du * cos_theta_max);
+        const double st = sqrt(1.0 - (ct * ct));
+        const double phi0 = dv * 2.0 * PSY_PI_D;
+        V3 sample = normalized(mk((float)(cos(phi0) * st), (float)(sin(phi0) * st), (float)ct));
+        const double cos_theta = sample.z;
+        const double cos_theta2 = cos_theta * cos_theta;
+        const double sin_theta2 = 1.0 - cos_theta2 > 0.0 ? 1.0 - cos_theta2 : 0.0;
+        const double sin_theta = sqrt(sin_theta2);
+        const double D = 1.0 - (d2 * sin_theta * sin_theta / (radius * radius));
+        double cos_a;
+        if (D <= 0.0) cos_a = sin_theta_max;
+        else cos_a = ((d / radius) * sin_theta2) + (cos_theta * sqrt(D));
+        const double sa2 = 1.0 - (cos_a * cos_a);
+        const double sin_a = sqrt(sa2 > 0.0 ? sa2 : 0.0);
+        const double phi = dv * 2.0 * PSY_PI_D;
+        sample.x = (float)(cos(phi) * sin_a * radius);
+        sample.y = (float)(sin(phi) * sin_a * radius);
+        sample.z = (float)(d - (cos_a * radius));
+        q.to_light = add(add(scale(x, sample.x), scale(y, sample.y)), scale(z, sample.z));
+        *pdf = (float)(1.0 / (2.0 * PSY_PI_D * (1.0 - cos_theta_max)));
+    } else {
+        q.to_light = uniform_sample_sphere(q.u, q.v);
+        *pdf = (float)(1.0f / (4.0f * PSY_PI_D));
+    }
+}
+
+PSY_DEV_FN void rect_light_sample(const float* od, int ts, LQuery& q, float* pdf) {
+    float w, h, col[3];
+    rect_light_at(od, ts, q.time, &w, &h, col);
+    const V3 arr = q.pos;
+    const double inv_area = 1.0 / (double)(w * h);
+    const V3 p1 = normalized(sub(mk(w * 0.5f, h * 0.5f, 0.0f), arr));
+    const V3 p2 = normalized(sub(mk(w * -0.5f, h * 0.5f, 0.0f), arr));
+    const V3 p3 = normalized(sub(mk(w * -0.5f, h * -0.5f, 0.0f), arr));
+    const V3 p4 = normalized(sub(mk(w * 0.5f, h * -0.5f, 0.0f), arr));
+    const float area_1 = sph_tri_area(p2, p1, p3);
+    const float area_2 = sph_tri_area(p4, p1, p3);
+    const float prob_1 = area_1 / (area_1 + area_2);
+    const float prob_2 = 1.0f - prob_1;
+    V3 sv;
+    if (q.u < prob_1) sv = uniform_sample_spherical_triangle(p2, p1, p3, q.v, q.u / prob_1);
+    else sv = uniform_sample_spherical_triangle(p4, p1, p3, q.v, 1.0f - ((q.u - prob_1) / prob_2));
+    sv = scale(sv, -arr.z / sv.z);
+    q.to_light = sv;
+    *pdf = 1.0f / (area_1 + area_2);
+    q.spec = color_spec((col[0] * (float)inv_area) * 0.5f, (col[1] * (float)inv_area) * 0.5f,
+                        (col[2] * (float)inv_area) * 0.5f);
+}
+
+PSY_DEV void lbox(const DevLights& L, int b, V3* lo, V3* hi) {
+    const float4 a = L.bounds[2 * b], c = L.bounds[2 * b + 1];
+    *lo = mk(a.x, a.y, a.z);
+    *hi = mk(c.x, c.y, c.z);
+}
+
+PSY_DEV_FN float node_prob(const DevLights& L, const LQuery& q, int index) {
+    const DevLNode nd = L.nodes[index];
+    V3 lo, hi;
+    float na = 0.0f;
+    const int i = lerp_seq_index(q.time, nd.n_bounds, &na);
+    if (i == -1) lbox(L, nd.bounds, &lo, &hi);
+    else if (i == -2) lbox(L, nd.bounds + nd.n_bounds - 1, &lo, &hi);
+    else {
+        V3 alo, ahi, blo, bhi;
+        lbox(L, nd.bounds + i, &alo, &ahi);
+        lbox(L, nd.bounds + i + 1, &blo, &bhi);
+        lo = lerpv(na, alo, blo);
+        hi = lerpv(na, ahi, bhi);
+    }
+    const V3 c = scale(add(lo, hi), 0.5f);
+    const V3 d = sub(c, q.pos);
+    const float dist2 = length2(d);
+    const float r = length(sub(hi, lo)) * 0.5f;
+    const float r2 = r * r;
+    const float inv_surface_area = 1.0f / r2;
+    float cos_theta_max;
+    if (dist2 <= r2) {
+        cos_theta_max = -1.0f;
+    } else {
+        const float s2 = smin(1.0f, r2 / dist2);
+        cos_theta_max = sqrtf(1.0f - s2);
+    }
+    const float approx = smax(0.0f, lambert_estimate(q.d, d, cos_theta_max, q.nor));
+    return nd.energy * inv_surface_area * approx;
+}
+
+// LightTree::sample, descending through sub-assembly trees.  False when the
+// scene has no light.
+PSY_DEV_FN bool light_tree_sample(const DevScene& s, const DevLights& L, LQuery& q) {
+    int a = 0;
+    for (int guard = 0; guard < 64; ++guard) {
+        const DevAsm A = L.asms[a];
+        if (A.n_nodes == 0) return false;
+        int ni = A.node0;
+        float tot_prob = 1.0f;
+        for (;;) {
+            const DevLNode nd = L.nodes[ni];
+            if (nd.leaf) break;
+            float p1 = node_prob(L, q, A.node0 + nd.index1);
+            float p2 = node_prob(L, q, A.node0 + nd.index2);
+            const float total = p1 + p2;
+            if (total <= 0.0f) {
+                p1 = 0.5f;
+                p2 = 0.5f;
+            } else {
+                p1 /= total;
+                p2 /= total;
+            }
+            if (q.n <= p1) {
+                tot_prob *= p1;
+                ni = A.node0 + nd.index1;
+                q.n /= p1;
+            } else {
+                tot_prob *= p2;
+                ni = A.node0 + nd.index2;
+                q.n = (q.n - p1) / p2;
+            }
+        }
+        const DevAsmInst in = L.inst[A.inst0 + L.nodes[ni].instance];
+        if (in.xf_count > 0) {
+            const M44 xf = m_lerp_seq(q.time, L.xf + A.xf0 + in.xf_index, in.xf_count);
+            q.pos = m_pos(xf, q.pos);
+            const M44 inv = m_inverse(xf);
+            q.nor = normalized(m_dir_transpose(inv, q.nor));
+            q.xform = m_mul(q.xform, xf);
+        }
+        if (in.type == PSY_INSTANCE_OBJECT) {
+            const DevObject ob = s.obj[A.obj0 + in.data_index];
+            const float* od = s.odata + ob.data;
+            float p = 1.0f;
+            if (ob.kind == K_SPHERE_LIGHT) sphere_light_sample(od, ob.ts, q, &p);
+            else if (ob.kind == K_RECT_LIGHT) rect_light_sample(od, ob.ts, q, &p);
+            else return false;
+            q.to_light = m_dir(m_inverse(q.xform), q.to_light);
+            q.selection_pdf *= (tot_prob * (float)A.total_lights);
+            q.light_sample_pdf = p;
+            return true;
+        }
+        const int sub_a = L.sub[A.sub0 + in.data_index];
+        q.selection_pdf *= (tot_prob * (float)A.total_lights) / (float)L.asms[sub_a].total_lights;
+        a = sub_a;
+    }
+    return false;
+}
+
+// ---- path kernels -------------------------------------------------------------
+
+PSY_DEV float next_dim(const RenderLaunch& A, uint32_t off, uint32_t& dim) {
+    return sampler_get(A.dims, A.perm, off, dim++);
+}
+
+// init_path + the camera ray: path slot i of the chunk, sample `first + i`
+// of the image in (y, x, s) order.
+PSY_DEV void render_init_one(const RenderLaunch& A, int64_t first, int64_t i, const PathBuf& P, float* rays,
+                             int64_t rcap, uint32_t* act) {
+    const int64_t gid = first + i;
+    const int s = (int)(gid % A.spp);
+    const int64_t pix = gid / A.spp;
+    const int px = (int)(pix % A.width);
+    const int py = (int)(pix / A.width);
+    const uint32_t off = sampler_offset((uint32_t)px, (uint32_t)py, (uint32_t)s, A.seed);
+    uint32_t dim = 0;
+    const float wl = (next_dim(A, off, dim) * (WL_MAX - WL_MIN)) + WL_MIN;
+    const float sx = next_dim(A, off, dim);
+    const float sy = next_dim(A, off, dim);
+    const float sa = next_dim(A, off, dim);
+    const float sb = next_dim(A, off, dim);
+    const float time = next_dim(A, off, dim);
+    V3 o, d, ddx, ddy, z;
+    camera_ray(A.cam, A.width, A.height, px, py, sx, sy, sa, sb, time, &o, &d, &ddx, &ddy, &z);
+    store_wray(rays, rcap, i, o, d, z, z, ddx, ddy, time, 1u /* WorldRay::CAMERA */);
+    P.setu(P_OFF, i, off);
+    P.setu(P_DIM, i, dim);
+    P.setu(P_PIX, i, (uint32_t)px | ((uint32_t)py << 16));
+    P.setu(P_STEP, i, 0u);
+    P.f(P_TIME, i) = time;
+    P.f(P_WL, i) = wl;
+    P.f(P_LASTPDF, i) = 0.0f;
+    for (int k = 0; k < NSPEC; ++k) {
+        P.f(P_COL + k, i) = 0.0f;
+        P.f(P_FCOL + k, i) = 1.0f;
+        P.f(P_LCOL + k, i) = 0.0f;
+    }
+    act[i] = (uint32_t)i;
+}
+
+// Emission and light pdf of a camera / bounce ray that ended on a light
+// object (SphereLight / RectangleLight::intersect_ray closures).
+PSY_DEV_FN void light_hit(const DevScene& s, uint32_t li, V3 wo, float time, float* spec, float* light_pdf) {
+    const DevInstance I = s.inst[li];
+    const DevObject ob = s.obj[I.object];
+    V3 o = wo;
+    if (I.xf_count > 0) o = m_pos(m_lerp_seq(time, s.xf + I.xf_offset, I.xf_count), wo);
+    const float* od = s.odata + ob.data;
+    if (ob.kind == K_RECT_LIGHT) {
+        float w, h, col[3];
+        rect_light_at(od, ob.ts, time, &w, &h, col);
+        *light_pdf = rect_light_pdf(w, h, o);
+        const double area = (double)(w * h);
+        *spec = color_spec((col[0] * 0.5f) / (float)area, (col[1] * 0.5f) / (float)area,
+                           (col[2] * 0.5f) / (float)area);
+    } else {
+        V3 c; float r, col[3];
+        sphere_light_at(od, ob.ts, time, &c, &r, col);
+        const double area = 4.0 * PSY_PI_D * r * r;
+        *light_pdf = sphere_light_pdf(c, (double)r, o);
+        *spec = color_spec(col[0] / (float)area, col[1] / (float)area, col[2] / (float)area);
+    }
+}
+
+// The next ray of a path that goes on.
+struct NextRay {
+    V3 o, d, odx, ody, ddx, ddy;
+    float time;
+    uint32_t type;
+};
+
+// update_path for the ray just traced (slot j of the stream) of path i, then
+// next_ray_for_path if the path goes on (returns true, the ray in `nr`);
+// a finished path writes its film contribution to sample_xyz.
+PSY_DEV bool render_update_one(const RenderLaunch& A, int64_t first, const PathBuf& P, uint32_t i, int64_t j,
+                               const float* rays_in, const DevHits& hits, int64_t rcap, float* sample_xyz,
+                               int64_t sbase, NextRay& nr) {
+    bool alive = false;
+    V3 no = splat(0.0f), nd = splat(0.0f), nodx = splat(0.0f), nody = splat(0.0f), nddx = splat(0.0f),
+       nddy = splat(0.0f);
+    float ntime = 0.0f;
+    uint32_t ntype = 0;
+    {
+        const DevScene& s = A.s;
+        uint32_t stepw = P.u(P_STEP, i);
+        const int step = (int)(stepw & 0xffu);
+        bool done = false;
+        const uint32_t hinst = hits.inst[j];
+        const bool hit = hinst != 0xffffffffu;
+        const V3 rd = mk(rays_in[3 * rcap + j], rays_in[4 * rcap + j], rays_in[5 * rcap + j]);
+        float col[NSPEC], fcol[NSPEC];
+        for (int k = 0; k < NSPEC; ++k) { col[k] = P.f(P_COL + k, i); fcol[k] = P.f(P_FCOL + k, i); }
+        if (step & 1) {
+            // result of a shadow ray: unoccluded -> add the light sample
+            if (!hit) {
+                const uint32_t sh = P.u(P_SHADER, i);
+                const float* alb = s.shaders[sh].color;
+                const float fac = lambert_eval(alb, P.v3(P_PD, i), rd, P.v3(P_GN, i));
+                for (int k = 0; k < NSPEC; ++k) col[k] += fcol[k] * P.f(P_LCOL + k, i) * fac;
+            }
+        } else if (hit) {
+            const DevInstance I = s.inst[hinst];
+            const V3 ro = mk(rays_in[0 * rcap + j], rays_in[1 * rcap + j], rays_in[2 * rcap + j]);
+            const float time = rays_in[18 * rcap + j];
+            bool emit = false;
+            float em = 0.0f, light_pdf = 9999.0f;
+            if (I.kind == K_SPHERE_LIGHT || I.kind == K_RECT_LIGHT) {
+                emit = true;
+                light_hit(s, hinst, ro, time, &em, &light_pdf);
+            } else if (I.shader >= 0 && s.shaders[I.shader].kind == PSY_SHADER_EMIT) {
+                emit = true;
+                const float* c = s.shaders[I.shader].color;
+                em = color_spec(c[0], c[1], c[2]);
+            }
+            if (emit) {
+                done = true;
+                const float last_pdf = P.f(P_LASTPDF, i);
+                if (last_pdf != 0.0f) {
+                    const float a2 = last_pdf * last_pdf;
+                    const float b2 = light_pdf * light_pdf;
+                    const float mis_inv_pdf = (a2 / (a2 + b2)) / last_pdf;
+                    for (int k = 0; k < NSPEC; ++k) col[k] += ((fcol[k] * last_pdf) * em) * mis_inv_pdf;
+                } else {
+                    for (int k = 0; k < NSPEC; ++k) col[k] += fcol[k] * em;
+                }
+            } else {
+                // store the hit in world space (DifferentialGeometry::transformed_from)
+                M44 space;
+                V3 pl, nl, offl;
+                bool backfacing = false;
+                const float t = hits.t[j];
+                hit_geometry(s, hinst, t, hits.u[j], hits.v[j], hits.aux ? hits.aux[j] : 0.0f, ro, rd, time, &space,
+                             &pl, &nl, &offl, &backfacing);
+                const M44 inv = m_inverse(space);
+                const V3 gp = m_pos(inv, pl);
+                V3 gn = m_dir_transpose(space, nl);
+                const float il = 1.0f / length(gn);
+                gn = scale(gn, il);
+                V3 po = m_dir(inv, offl);
+                if (backfacing) po = scale(po, -1.0f);
+                P.set3(P_GP, i, gp);
+                P.set3(P_GN, i, gn);
+                P.set3(P_POFF, i, po);
+                P.f(P_T, i) = t;
+                P.setu(P_SHADER, i, (uint32_t)I.shader);
+                P.set3(P_PD, i, rd);
+                P.set3(P_PODX, i, mk(rays_in[6 * rcap + j], rays_in[7 * rcap + j], rays_in[8 * rcap + j]));
+                P.set3(P_PODY, i, mk(rays_in[9 * rcap + j], rays_in[10 * rcap + j], rays_in[11 * rcap + j]));
+                P.set3(P_PDDX, i, mk(rays_in[12 * rcap + j], rays_in[13 * rcap + j], rays_in[14 * rcap + j]));
+                P.set3(P_PDDY, i, mk(rays_in[15 * rcap + j], rays_in[16 * rcap + j], rays_in[17 * rcap + j]));
+                stepw = backfacing ? (stepw | STEP_BACKFACING) : (stepw & ~STEP_BACKFACING);
+            }
+        } else {
+            done = true;
+            const float bg = color_spec(A.bg[0], A.bg[1], A.bg[2]);
+            for (int k = 0; k < NSPEC; ++k) col[k] += fcol[k] * bg;
+        }
+        const int nstep = step + 1;
+        if (nstep == A.path_length * 2) done = true;
+
+        if (!done) {
+            // next_ray_for_path for step nstep
+            const uint32_t off = P.u(P_OFF, i);
+            uint32_t dim = P.u(P_DIM, i);
+            const V3 gp = P.v3(P_GP, i), gn = P.v3(P_GN, i), po = P.v3(P_POFF, i), pd = P.v3(P_PD, i);
+            const float ht = P.f(P_T, i);
+            const float time = P.f(P_TIME, i);
+            const float* alb = s.shaders[P.u(P_SHADER, i)].color;
+            if (nstep & 1) {
+                // shadow ray towards a light sample
+                V3 lq_nor = gn;
+                if (stepw & STEP_BACKFACING) lq_nor = scale(lq_nor, -1.0f);
+                LQuery q;
+                q.n = next_dim(A, off, dim);
+                q.u = next_dim(A, off, dim);
+                q.v = next_dim(A, off, dim);
+                q.pos = gp;
+                q.nor = lq_nor;
+                q.d = pd;
+                q.time = time;
+                q.xform = m_identity();
+                q.to_light = splat(0.0f);
+                q.spec = 0.0f;
+                q.selection_pdf = 1.0f;
+                q.light_sample_pdf = 1.0f;
+                float lcol = 0.0f;
+                if (!light_tree_sample(s, A.L, q)) {
+                    q.to_light = mk(0.0f, 0.0f, 1.0f);
+                } else {
+                    const float bsdf_pdf = lambert_pdf(pd, q.to_light, gn);
+                    const float a2 = q.light_sample_pdf * q.light_sample_pdf;
+                    const float b2 = bsdf_pdf * bsdf_pdf;
+                    const float mis_inv_pdf = (a2 / (a2 + b2)) / q.light_sample_pdf;
+                    lcol = ((q.spec * mis_inv_pdf) * (float)A.L.asms[0].total_lights) / q.selection_pdf;
+                }
+                for (int k = 0; k < NSPEC; ++k) P.f(P_LCOL + k, i) = lcol;
+                no = add(gp, po);
+                nd = scale(sub(q.to_light, po), 0.9995f);
+                ntype = PSY_RAY_OCCLUSION;
+            } else {
+                // bounce ray: LambertClosure::sample
+                float su = next_dim(A, off, dim);
+                float sv = next_dim(A, off, dim);
+                V3 nn = normalized(gn);
+                if (dot(nn, pd) > 0.0f) nn = scale(nn, -1.0f);
+                su = (su * 2) - 1;
+                sv = (sv * 2) - 1;
+                square_to_circle(&su, &sv);
+                const double zz = 1.0 - (double)((su * su) + (sv * sv));
+                const float zc = (float)sqrt(zz > 0.0 ? zz : 0.0);
+                const V3 dir = mk(su, sv, zc);
+                float pdf = dir.z * (float)(1.0 / PSY_PI_D);
+                nd = zup_to_vec(dir, nn);
+                no = add(gp, po);
+                ntype = 2u;  // WorldRay::R_DIFFUSE
+                const float filt = lambert_eval(alb, pd, nd, gn);
+                if (pdf < 0.00001f) pdf = 0.00001f;
+                for (int k = 0; k < NSPEC; ++k) fcol[k] *= filt / pdf;
+                P.f(P_LASTPDF, i) = pdf;
+            }
+            // LambertClosure::propagate_differentials
+            const float len = length(nd);
+            const V3 gnn = normalized(gn);
+            V3 x, y;
+            coordinate_system_from_vec3(nd, &x, &y);
+            x = normalized(x);
+            y = normalized(y);
+            nodx = transfer_origin_diff(ht, gnn, pd, P.v3(P_PODX, i), P.v3(P_PDDX, i));
+            nody = transfer_origin_diff(ht, gnn, pd, P.v3(P_PODY, i), P.v3(P_PDDY, i));
+            nddx = sdiv(scale(x, 0.15f), len);
+            nddy = sdiv(scale(y, 0.15f), len);
+            ntime = time;
+            P.setu(P_DIM, i, dim);
+            alive = true;
+        }
+        for (int k = 0; k < NSPEC; ++k) { P.f(P_COL + k, i) = col[k]; P.f(P_FCOL + k, i) = fcol[k]; }
+        P.setu(P_STEP, i, (stepw & ~0xffu) | (uint32_t)nstep);
+        if (!alive && sample_xyz) {
+            float xyz[3];
+            path_film_xyz(P.f(P_WL, i), col, xyz);
+            const int64_t g = first + i - sbase;
+            sample_xyz[3 * g + 0] = xyz[0];
+            sample_xyz[3 * g + 1] = xyz[1];
+            sample_xyz[3 * g + 2] = xyz[2];
+        }
+    }
+    nr.o = no; nr.d = nd; nr.odx = nodx; nr.ody = nody; nr.ddx = nddx; nr.ddy = nddy;
+    nr.time = ntime;
+    nr.type = ntype;
+    return alive;
+}
+
+// Film::add_sample over a pixel's samples, in sample order: the mean of the
+// finite samples (NaN / inf samples are skipped as the reference does).
+PSY_DEV void render_resolve_one(const float* sample_xyz, int64_t sbase, int64_t first_pixel, int64_t p, int spp,
+                                float* film) {
+    const int64_t g = first_pixel + p;
+    float sx = 0.0f, sy = 0.0f, sz = 0.0f;
+    int cnt = 0;
+    for (int s = 0; s < spp; ++s) {
+        const float* v = sample_xyz + 3 * (g * spp + s - sbase);
+        const float a = v[0], b = v[1], c = v[2];
+        if (!(fabsf(a) < INFINITY) || !(fabsf(b) < INFINITY) || !(fabsf(c) < INFINITY)) continue;
+        sx += a; sy += b; sz += c;
+        ++cnt;
+    }
+    const float inv = cnt > 0 ? 1.0f / (float)cnt : 0.0f;
+    film[3 * g + 0] = sx * inv;
+    film[3 * g + 1] = sy * inv;
+    film[3 * g + 2] = sz * inv;
+}
+
+}  // namespace psy

@@ -37,6 +37,7 @@ def lib():
     L.psy_ctx_destroy.argtypes = [vp]
     L.psy_ctx_set_trace_mode.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.psy_ctx_retrace_count.argtypes = [vp, C.POINTER(i64)]
+    L.psy_ctx_set_render_chunk.argtypes = [vp, i64]
     L.psy_scene_create.argtypes = [vp, C.POINTER(abi.SceneDesc), C.c_int, C.POINTER(vp)]
     L.psy_scene_destroy.argtypes = [vp]
     L.psy_scene_get_info.argtypes = [vp, C.POINTER(abi.SceneInfo)]
@@ -48,6 +49,7 @@ def lib():
     L.psy_pixel_samples.argtypes = [vp, i32, i32, i32, i32, i32, u32, u32, i32, vp, vp]
     L.psy_camera_rays.argtypes = [vp, vp, i32, i32, i32, u32, vp, vp, vp]
     L.psy_bounce_rays.argtypes = [vp, vp, vp, C.POINTER(abi.Hits), i64, i32, i32, u32, u32, u32, vp, vp, vp]
+    L.psy_render.argtypes = [vp, vp, C.POINTER(abi.RenderParams), vp, vp, C.POINTER(i64), vp]
     if L.psy_abi_version() != 1:
         raise PsyError("libpsy_gpu.so ABI version mismatch")
     _lib = L
@@ -219,3 +221,24 @@ def bounce_rays(ctx, scene, wrays, hits, width, spp, seed, dim_u=9, dim_v=10):
     check(lib().psy_bounce_rays(ctx.handle, scene.handle, _ptr(wrays), C.byref(hs), n, width, spp, seed, dim_u,
                                 dim_v, _ptr(out), _ptr(valid), _stream()), "psy_bounce_rays")
     return out, valid
+
+
+def render(ctx, scene, width, height, spp, seed=1, path_length=4, film=True, samples=False):
+    """psy_render: the path-traced image (PathTraceIntegrator::integrate).
+
+    Returns (film, samples, rays_traced): film is a (height, width, 3) CIE XYZ
+    tensor (per-pixel mean), samples a (height, width, spp, 3) tensor of every
+    sample's film contribution; either is None when not requested."""
+    dev = torch.device("cuda", ctx.device)
+    f = torch.empty((height, width, 3), dtype=torch.float32, device=dev) if film else None
+    s = torch.empty((height, width, spp, 3), dtype=torch.float32, device=dev) if samples else None
+    prm = abi.RenderParams(width, height, spp, seed, path_length, 0)
+    rays = C.c_int64()
+    check(lib().psy_render(ctx.handle, scene.handle, C.byref(prm), _ptr(f), _ptr(s), C.byref(rays), _stream()),
+          "psy_render")
+    return f, s, rays.value
+
+
+def set_render_chunk(ctx, paths):
+    """psy_ctx_set_render_chunk: paths per wavefront chunk of psy_render."""
+    check(lib().psy_ctx_set_render_chunk(ctx.handle, int(paths)), "psy_ctx_set_render_chunk")

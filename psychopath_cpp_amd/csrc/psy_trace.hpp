@@ -742,31 +742,38 @@ PSY_DEV bool lane_step(const DevScene& s, Lane& L, int32_t (*stk_ref)[TRACE_BLOC
     const uint32_t li = (uint32_t)(~ref);
     const DevInstance I = s.inst[li];
     if (I.kind == K_BILINEAR || I.kind == K_BICUBIC) {
-        if (WORLD) {
-            L.wo = L.r.o;
-            L.wdinv = L.r.dinv;
-            L.ww = L.r.w;
-        }
-        if (WORLD && I.xf_count > 0) {
-            const M44 x = m_lerp_seq(L.r.time, s.xf + I.xf_offset, I.xf_count);
-            L.wo = m_pos(x, L.r.o);
-            const V3 d = m_dir(x, L.r.d);
-            L.wdinv = divv(splat(1.0f), d);
-            if (!finite3(L.wdinv)) {
-                L.state |= LANE_RETRACE;
-                return true;
-            }
-            L.ww = ray_widths(L.wo, d, m_dir(x, L.r.odx), m_dir(x, L.r.ody), m_dir(x, L.r.ddx), m_dir(x, L.r.ddy));
-        }
-        L.wfx = flip_x(walk_w<WORLD>(L));
-        L.wfy = flip_y(walk_w<WORLD>(L));
+        // Walk state first, then the patch-space ray.  The transform block
+        // has no exit of its own: with a `return` inside it, hipcc (ROCm
+        // 7.2, gfx950) reused the register holding the dice slot inside the
+        // block for lanes that then walked with slot 0.
         L.slot = (uint32_t)I.dice;
         L.levels = I.dice_levels;
-        L.state |= LANE_WALKING;
         L.winst = li;
         L.k = 0;
         L.depth = -1;
         L.trail = 0;
+        bool finite = true;
+        if (WORLD) {
+            L.wo = L.r.o;
+            L.wdinv = L.r.dinv;
+            L.ww = L.r.w;
+            if (I.xf_count > 0) {
+                const M44 x = m_lerp_seq(L.r.time, s.xf + I.xf_offset, I.xf_count);
+                L.wo = m_pos(x, L.r.o);
+                const V3 d = m_dir(x, L.r.d);
+                L.wdinv = divv(splat(1.0f), d);
+                finite = finite3(L.wdinv);
+                L.ww = ray_widths(L.wo, d, m_dir(x, L.r.odx), m_dir(x, L.r.ody), m_dir(x, L.r.ddx),
+                                  m_dir(x, L.r.ddy));
+            }
+        }
+        if (!finite) {
+            L.state |= LANE_RETRACE;
+            return true;
+        }
+        L.wfx = flip_x(walk_w<WORLD>(L));
+        L.wfy = flip_y(walk_w<WORLD>(L));
+        L.state |= LANE_WALKING;
         return false;
     }
     bool unused = false;

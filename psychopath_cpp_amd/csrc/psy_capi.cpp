@@ -628,8 +628,10 @@ int prepare_scene(const psy_scene_desc* d, int dice_depth, HostScene& H) {
         if (objs[i].kind == PSY_OBJ_BILINEAR || objs[i].kind == PSY_OBJ_BICUBIC) patch_ids.push_back((int32_t)i);
     int levels = dice_depth;
     if (levels < 0) {
-        const double budget = 4.0 * 1024.0 * 1024.0 * 1024.0;  // 4 GiB of split-tree nodes
-        levels = 16;
+        // 16 GiB of split-tree nodes (under 6% of an MI355X's HBM), at most
+        // 18 levels; rays that need deeper splits take the second pass
+        const double budget = 16.0 * 1024.0 * 1024.0 * 1024.0;
+        levels = 18;
         while (levels > 0 && (double)((int64_t)1 << (levels + 1)) * 32.0 * (double)patch_ids.size() > budget) --levels;
     }
     if (levels > 20) return fail(PSY_ERR_INVALID, "psy_scene_create: dice_depth above 20");

@@ -166,6 +166,17 @@ int psy_ctx_set_trace_mode(psy_ctx* ctx, int mode, int refill_min, int blocks_pe
  * whose buckets are sqrt(65536/spp) pixels square): whole pixels, at least
  * one; the default 2^21 paths.  Results do not depend on it. */
 int psy_ctx_set_render_chunk(psy_ctx* ctx, int64_t paths);
+/* Form of psy_render (no counterpart in the reference).
+ * PSY_RENDER_WAVEFRONT (default): one path per thread per step; each step
+ * traces the whole active stream, then shades it and compacts the paths
+ * that go on (the reference's loop, integrator/path_trace_integrator.cpp:
+ * 321-349).  PSY_RENDER_FUSED: persistent waves in which each lane runs one
+ * path through trace, shading and its next ray, refilled from the chunk's
+ * paths when refill_min (1..64) of a wave's lanes are idle; blocks_per_cu
+ * <= 0 takes the kernel's occupancy.  Both give the same samples. */
+#define PSY_RENDER_WAVEFRONT 0
+#define PSY_RENDER_FUSED 1
+int psy_ctx_set_render_mode(psy_ctx* ctx, int mode, int refill_min, int blocks_per_cu);
 /* Rays of the latest trace call on this context that the main pass handed
  * to the second pass (split below the dice cache, or a BVH stack deeper
  * than the main pass keeps).  Synchronizes the device; for diagnostics. */

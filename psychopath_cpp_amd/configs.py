@@ -13,13 +13,25 @@ restated here, so a seed fully determines a scene.
                     (20,20,20), grey Lambert 0.5, pinhole camera; split-diced
                     at dice_rate 1.0 (about one micropolygon per pixel at
                     1024x1024).
+  bicubic_sphere_scene
+                    4,096 bicubic patches tiling a unit sphere (latitude /
+                    longitude grid, control points jittered by N(0, 0.02)),
+                    instanced 16 times through a two-level hierarchy of
+                    random rotation / scale / translation placements with
+                    two transform samples each (the motion between samples
+                    is a translation and scale change under a fixed
+                    rotation), two sphere lights, grey Lambert, thin-lens
+                    camera with aperture radius 0.05.
+  many_light_scene  sphere_scene plus N sphere lights (radii log-uniform in
+                    [0.01, 0.2], 0.5-3 above the ground, grey emission scale
+                    U[0.1, 10]).
 """
 import math
 
 import numpy as np
 
 from . import abi
-from .scene import Assembly, Bilinear, Camera, RectangleLight, Scene, Sphere, SphereLight, look_at, translate
+from .scene import Assembly, Bicubic, Bilinear, Camera, RectangleLight, Scene, Sphere, SphereLight, look_at, translate
 
 
 class JKiss:
@@ -119,4 +131,107 @@ def heightfield_scene(seed=7, grid=64, extent=10.0, dice_rate=1.0, light_pos=(0.
             root.create_object_instance(name, [], "ground")
     root.add_object("light", RectangleLight(light_size, light_size, light_color))
     root.create_object_instance("light", [translate(*light_pos)], None)
+    return sc
+
+
+def _gauss(rng):
+    """Standard normal sample from two JKISS uniforms (Box-Muller)."""
+    u1 = max(rng.next_float(), 1e-12)
+    u2 = rng.next_float()
+    return math.sqrt(-2.0 * math.log(u1)) * math.cos(2.0 * math.pi * u2)
+
+
+def _rotation(rng):
+    """A random rotation (row-vector 4x4) about a random axis."""
+    ax = np.array([_gauss(rng), _gauss(rng), _gauss(rng)])
+    ax /= max(np.linalg.norm(ax), 1e-9)
+    ang = rng.uniform(0.0, 2.0 * math.pi)
+    c, s = math.cos(ang), math.sin(ang)
+    x, y, z = ax
+    r = np.array([[c + x * x * (1 - c), x * y * (1 - c) - z * s, x * z * (1 - c) + y * s],
+                  [y * x * (1 - c) + z * s, c + y * y * (1 - c), y * z * (1 - c) - x * s],
+                  [z * x * (1 - c) - y * s, z * y * (1 - c) + x * s, c + z * z * (1 - c)]])
+    m = np.eye(4)
+    m[:3, :3] = r.T  # row-vector convention
+    return m
+
+
+def placement(rot, scl, pos):
+    """World -> local transform of an instance scaled by `scl`, rotated by
+    `rot` and placed at `pos` (local -> world is S * R * T, row vectors)."""
+    S = np.diag([scl, scl, scl, 1.0])
+    T = np.eye(4)
+    T[3, :3] = pos
+    return np.linalg.inv(S @ rot @ T).astype(np.float32)
+
+
+def sphere_patches(rng, grid=64, jitter=0.02):
+    """Bicubic patches (16 control points each) over a latitude/longitude
+    grid of the unit sphere, control points jittered by N(0, jitter)."""
+    patches = []
+    for j in range(grid):
+        for i in range(grid):
+            pts = []
+            for l in range(4):
+                v = math.pi * (j + l / 3.0) / grid
+                for k in range(4):
+                    u = 2.0 * math.pi * (i + k / 3.0) / grid
+                    p = [math.sin(v) * math.cos(u), math.sin(v) * math.sin(u), math.cos(v)]
+                    pts.append([p[0] + jitter * _gauss(rng), p[1] + jitter * _gauss(rng), p[2] + jitter * _gauss(rng)])
+            patches.append(np.array(pts, np.float32))
+    return patches
+
+
+def bicubic_sphere_scene(seed=3, grid=64, jitter=0.02, n_outer=4, n_inner=4, motion=True, aperture=0.05,
+                         fov_deg=45.0, camera_eye=(0.0, -26.0, 9.0), camera_target=(0.0, 0.0, 0.0),
+                         dice_rate=1.0, light_color=(60.0, 60.0, 60.0)):
+    rng = JKiss(seed)
+    sphere = Assembly()
+    for k, pts in enumerate(sphere_patches(rng, grid, jitter)):
+        sphere.add_object(f"p{k}", Bicubic(pts))
+        sphere.create_object_instance(f"p{k}", [], None)
+    mid = Assembly()
+    mid.add_assembly("sphere", sphere)
+    for k in range(n_inner):
+        rot = _rotation(rng)
+        s = rng.uniform(0.5, 1.5)
+        pos = np.array([rng.uniform(-3, 3), rng.uniform(-3, 3), rng.uniform(-3, 3)])
+        xs = [placement(rot, s, pos)]
+        if motion:
+            xs.append(placement(rot, s * rng.uniform(0.95, 1.05), pos + np.array([rng.uniform(-0.3, 0.3) for _ in range(3)])))
+        mid.create_assembly_instance("sphere", xs, None)
+    root = Assembly()
+    root.add_assembly("mid", mid)
+    for k in range(n_outer):
+        rot = _rotation(rng)
+        s = rng.uniform(0.8, 1.2)
+        pos = np.array([rng.uniform(-8, 8), rng.uniform(-8, 8), rng.uniform(-3, 3)])
+        xs = [placement(rot, s, pos)]
+        if motion:
+            xs.append(placement(rot, s, pos + np.array([rng.uniform(-0.5, 0.5) for _ in range(3)])))
+        root.create_assembly_instance("mid", xs, "grey")
+    root.add_object("lamp0", SphereLight((6.0, -6.0, 12.0), 1.5, light_color))
+    root.create_object_instance("lamp0", [], None)
+    root.add_object("lamp1", SphereLight((-8.0, 4.0, 10.0), 1.0, light_color))
+    root.create_object_instance("lamp1", [], None)
+    eye = np.asarray(camera_eye, np.float64)
+    focus = float(np.linalg.norm(eye - np.asarray(camera_target, np.float64)))
+    cam = Camera([look_at(camera_eye, camera_target)], [math.radians(fov_deg)],
+                 aperture_radii=[aperture], focus_distances=[focus])
+    sc = Scene(cam, root, background=(0.05, 0.05, 0.05), dice_rate=dice_rate)
+    sc.add_surface_shader("grey", abi.PSY_SHADER_LAMBERT, (0.6, 0.6, 0.6))
+    return sc
+
+
+def many_light_scene(seed=1, n_lights=10000, **kw):
+    """sphere_scene plus n_lights small sphere lights above the ground."""
+    sc = sphere_scene(seed=seed, **kw)
+    rng = JKiss(seed + 1000)
+    root = sc.root
+    for i in range(n_lights):
+        r = math.exp(rng.uniform(math.log(0.01), math.log(0.2)))
+        c = (rng.uniform(-10, 10), rng.uniform(-10, 10), rng.uniform(0.5, 3.0))
+        e = rng.uniform(0.1, 10.0)
+        root.add_object(f"ml{i}", SphereLight(c, r, (e, e, e)))
+        root.create_object_instance(f"ml{i}", [], None)
     return sc

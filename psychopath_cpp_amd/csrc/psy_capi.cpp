@@ -163,7 +163,8 @@ int psy_ctx_create(int device, psy_ctx** out) {
     rc = c->halton_perm.upload(hperm);
     if (rc) return rc;
     HIP_OK(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device));
-    if (trace_persist_occupancy(false, &c->occupancy[0]) || trace_persist_occupancy(true, &c->occupancy[1]))
+    if (trace_persist_occupancy(false, &c->occupancy[0]) || trace_persist_occupancy(true, &c->occupancy[1]) ||
+        render_persist_occupancy(&c->render_occupancy))
         return fail(PSY_ERR_HIP, "psy_ctx_create: occupancy query failed");
     *out = c.release();
     return PSY_OK;
@@ -178,6 +179,18 @@ int psy_ctx_set_trace_mode(psy_ctx* ctx, int mode, int refill_min, int blocks_pe
     ctx->mode = mode;
     ctx->refill_min = refill_min;
     ctx->blocks_per_cu = blocks_per_cu;
+    return PSY_OK;
+}
+
+int psy_ctx_set_render_mode(psy_ctx* ctx, int mode, int refill_min, int blocks_per_cu) {
+    if (!ctx) return fail(PSY_ERR_INVALID, "psy_ctx_set_render_mode: NULL context");
+    if (mode != PSY_RENDER_WAVEFRONT && mode != PSY_RENDER_FUSED)
+        return fail(PSY_ERR_INVALID, "psy_ctx_set_render_mode: unknown mode");
+    if (refill_min < 1 || refill_min > 64) return fail(PSY_ERR_INVALID, "psy_ctx_set_render_mode: refill_min not in 1..64");
+    if (blocks_per_cu > 64) return fail(PSY_ERR_INVALID, "psy_ctx_set_render_mode: blocks_per_cu above 64");
+    ctx->render_mode = mode;
+    ctx->render_refill = refill_min;
+    ctx->render_blocks_per_cu = blocks_per_cu;
     return PSY_OK;
 }
 

@@ -176,6 +176,27 @@ int launch_render_update(const RenderLaunch& R, int64_t first, const RenderBuffe
 int launch_render_resolve(const float* sample_xyz, int64_t sbase, int64_t first_pixel, int64_t n_pixels, int spp,
                           float* film, hipStream_t st);
 int launch_zero_u32(uint32_t* p, int n, hipStream_t st);
+// Buffers of the fused path kernel for one chunk of `cap` paths.
+struct PersistBuffers {
+    float* paths;
+    uint32_t* counters;     // u32 x 4 (see k_render_persist)
+    uint32_t* resume_ids;   // paths handed back to the kernel, and their rays
+    float* resume_rays;
+    uint32_t* defer_ids;    // paths whose ray needs the second trace pass
+    float* defer_rays;
+    float* sample_xyz;
+    int64_t sbase;
+    int64_t cap;
+};
+int launch_render_persist(const RenderLaunch& R, int64_t first, int64_t n_new, const PersistBuffers& B,
+                          int blocks, int refill_min, hipStream_t st);
+int render_persist_occupancy(int* blocks_per_cu);
+int launch_render_update_io(const RenderLaunch& R, int64_t first, float* paths, int64_t cap, const uint32_t* act_in,
+                            const float* rays_in, const DevHits& hits, int64_t n_in, uint32_t* act_out,
+                            float* rays_out, uint32_t* count_out, float* sample_xyz, int64_t sbase, hipStream_t st);
+int launch_render_count_steps(float* paths, int64_t cap, int64_t n, unsigned long long* out, hipStream_t st);
+// counters[0..7] = 0 except counters[3] = n_resume
+int launch_set_counters(uint32_t* p, uint32_t n_resume, hipStream_t st);
 int launch_halton(const HaltonDim* dims, const uint16_t* perm, uint32_t dim, const uint32_t* idx,
                   int64_t n, float* out, hipStream_t st);
 int launch_pixel_samples(const HaltonDim* dims, const uint16_t* perm, int x0, int y0, int w, int h,

@@ -84,7 +84,11 @@ struct psy_ctx {
     }
     // integrator scratch (path states, ray / hit streams, active lists)
     psy::DBuf render_scratch;
-    int64_t render_chunk = 1 << 21;  // paths per wavefront chunk
+    int64_t render_chunk = 1 << 21;  // paths per chunk
+    int render_mode = PSY_RENDER_WAVEFRONT;
+    int render_refill = 16;          // fused mode: refill a wave at this many idle lanes
+    int render_blocks_per_cu = 0;    // fused mode: <= 0 from occupancy
+    int render_occupancy = 0;
 };
 
 struct psy_scene {

@@ -259,7 +259,7 @@ __global__ __launch_bounds__(RENDER_BLOCK) void k_render_update(RenderLaunch A, 
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool alive = false;
     uint32_t i = 0;
-    NextRay nr;
+    WRay nr;
     if (j < n_in) {
         i = act_in[j];
         alive = render_update_one(A, first, P, i, j, rays_in, hits, rcap, sample_xyz, sbase, nr);
@@ -274,6 +274,103 @@ __global__ __launch_bounds__(RENDER_BLOCK) void k_render_update(RenderLaunch A, 
         act_out[k] = i;
         store_wray(rays_out, rcap, k, nr.o, nr.d, nr.odx, nr.ody, nr.ddx, nr.ddy, nr.time, nr.type);
     }
+}
+
+// Fused path kernel (the batched form of the integrator): persistent waves
+// in which every lane owns one path at a time and runs trace -> shade ->
+// next ray itself, with no stream in memory between the steps.  A wave
+// refills its idle lanes (one atomic per wave) from one index space: first
+// the `n_resume` paths handed back by the previous round (their next ray in
+// resume_rays), then the chunk's new paths.  A ray that needs the second
+// trace pass (splits below the dice cache, a deep BVH stack) parks its path
+// in the defer list; the host traces those rays with the two-pass tracer,
+// shades them and hands the survivors to the next round.
+// counters: [0..1] next index (u64), [2] defer count, [3] n_resume.
+__global__ __launch_bounds__(TRACE_BLOCK) void k_render_persist(RenderLaunch A, int64_t first, int64_t n_new,
+                                                                PathBuf P, uint32_t* __restrict__ counters,
+                                                                const uint32_t* __restrict__ resume_ids,
+                                                                const float* __restrict__ resume_rays,
+                                                                uint32_t* __restrict__ defer_ids,
+                                                                float* __restrict__ defer_rays, int64_t rcap,
+                                                                float* __restrict__ sample_xyz, int64_t sbase,
+                                                                int refill_min) {
+    __shared__ int32_t stk_ref[MAIN_STACK][TRACE_BLOCK];
+    __shared__ float stk_near[MAIN_STACK][TRACE_BLOCK];
+    const int tid = threadIdx.x;
+    const uint32_t lane = __lane_id();
+    unsigned long long* next = reinterpret_cast<unsigned long long*>(counters);
+    const int64_t n_resume = (int64_t)counters[3];
+    const int64_t n_total = n_resume + n_new;
+    const DevScene& s = A.s;
+    Lane L;
+    uint32_t pi = 0, rtype = 0;
+    bool active = false;
+    bool exhausted = false;  // wave-uniform
+    for (;;) {
+        const unsigned long long idle = __ballot(!active);
+        const int nidle = __popcll(idle);
+        if (!exhausted && nidle >= refill_min) {
+            unsigned long long base = 0;
+            if (lane == 0) base = atomicAdd(next, (unsigned long long)nidle);
+            base = ((unsigned long long)(uint32_t)__shfl((int)(uint32_t)(base >> 32), 0) << 32) |
+                   (unsigned long long)(uint32_t)__shfl((int)(uint32_t)base, 0);
+            if ((int64_t)base >= n_total) exhausted = true;
+            if (!active) {
+                const int64_t e = (int64_t)base + __popcll(idle & ((1ull << lane) - 1ull));
+                if (e < n_total) {
+                    WRay w;
+                    if (e < n_resume) {
+                        pi = resume_ids[e];
+                        load_wray(resume_rays, rcap, e, w);
+                    } else {
+                        pi = (uint32_t)(e - n_resume);
+                        render_init_path(A, first, pi, P, w);
+                    }
+                    rtype = w.type;
+                    lane_init_wray(s, w, pi, L);
+                    active = true;
+                }
+            }
+        }
+        if (exhausted && __ballot(active) == 0ull) break;
+        if (active && lane_step<true>(s, L, stk_ref, stk_near, tid)) {
+            const bool again = lane_result(s, L);
+            WRay w;
+            lane_wray(L, rtype, w);
+            if (again) {
+                const uint32_t k = atomicAdd(&counters[2], 1u);
+                defer_ids[k] = pi;
+                store_wray(defer_rays, rcap, k, w);
+                active = false;
+            } else {
+                HitRec hr;
+                hr.t = L.b.t;
+                hr.inst = L.b.inst;
+                hr.u = L.b.u;
+                hr.v = L.b.v;
+                hr.aux = L.b.aux;
+                WRay nr;
+                if (render_update_path(A, first, P, pi, w, hr, sample_xyz, sbase, nr)) {
+                    rtype = nr.type;
+                    lane_init_wray(s, nr, pi, L);
+                } else {
+                    active = false;
+                }
+            }
+        }
+    }
+}
+
+// Rays traced by a chunk's paths: every step of a path traced one ray.
+__global__ void k_render_count_steps(PathBuf P, int64_t n, unsigned long long* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t steps = i < n ? (P.u(P_STEP, i) & 0xffu) : 0u;
+    for (int o = 32; o > 0; o >>= 1) steps += (uint32_t)__shfl_xor((int)steps, o);
+    if (__lane_id() == 0 && steps) atomicAdd(out, (unsigned long long)steps);
+}
+
+__global__ void k_set_counters(uint32_t* p, uint32_t n_resume) {
+    if (threadIdx.x < 8 && blockIdx.x == 0) p[threadIdx.x] = threadIdx.x == 3 ? n_resume : 0u;
 }
 
 __global__ void k_render_resolve(const float* __restrict__ sample_xyz, int64_t sbase, int64_t first_pixel,
@@ -414,6 +511,46 @@ int launch_render_update(const RenderLaunch& R, int64_t first, const RenderBuffe
                        (const float*)B.rays[cur], B.hits, n_in, B.cap, B.act[nxt], B.rays[nxt], B.count + nxt,
                        sample_xyz, sbase);
     return hipGetLastError() == hipSuccess ? 0 : PSY_ERR_HIP;
+}
+
+int launch_render_persist(const RenderLaunch& R, int64_t first, int64_t n_new, const PersistBuffers& B,
+                          int blocks, int refill_min, hipStream_t st) {
+    PathBuf P{B.paths, B.cap};
+    hipLaunchKernelGGL(k_render_persist, dim3((unsigned)blocks), dim3(TRACE_BLOCK), 0, st, R, first, n_new, P,
+                       B.counters, (const uint32_t*)B.resume_ids, (const float*)B.resume_rays, B.defer_ids,
+                       B.defer_rays, B.cap, B.sample_xyz, B.sbase, std::max(1, std::min(64, refill_min)));
+    return hipGetLastError() == hipSuccess ? 0 : PSY_ERR_HIP;
+}
+
+int launch_render_update_io(const RenderLaunch& R, int64_t first, float* paths, int64_t cap, const uint32_t* act_in,
+                            const float* rays_in, const DevHits& hits, int64_t n_in, uint32_t* act_out,
+                            float* rays_out, uint32_t* count_out, float* sample_xyz, int64_t sbase, hipStream_t st) {
+    if (n_in <= 0) return 0;
+    PathBuf P{paths, cap};
+    hipLaunchKernelGGL(k_render_update, dim3((unsigned)((n_in + RENDER_BLOCK - 1) / RENDER_BLOCK)), dim3(RENDER_BLOCK),
+                       0, st, R, first, P, act_in, rays_in, hits, n_in, cap, act_out, rays_out, count_out, sample_xyz,
+                       sbase);
+    return hipGetLastError() == hipSuccess ? 0 : PSY_ERR_HIP;
+}
+
+int launch_render_count_steps(float* paths, int64_t cap, int64_t n, unsigned long long* out, hipStream_t st) {
+    if (n <= 0) return 0;
+    PathBuf P{paths, cap};
+    hipLaunchKernelGGL(k_render_count_steps, dim3((unsigned)((n + RENDER_BLOCK - 1) / RENDER_BLOCK)), dim3(RENDER_BLOCK),
+                       0, st, P, n, out);
+    return hipGetLastError() == hipSuccess ? 0 : PSY_ERR_HIP;
+}
+
+int launch_set_counters(uint32_t* p, uint32_t n_resume, hipStream_t st) {
+    hipLaunchKernelGGL(k_set_counters, dim3(1), dim3(64), 0, st, p, n_resume);
+    return hipGetLastError() == hipSuccess ? 0 : PSY_ERR_HIP;
+}
+
+int render_persist_occupancy(int* blocks_per_cu) {
+    int nb = 0;
+    const hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_render_persist, TRACE_BLOCK, 0);
+    *blocks_per_cu = nb;
+    return e == hipSuccess ? 0 : PSY_ERR_HIP;
 }
 
 int launch_render_resolve(const float* sample_xyz, int64_t sbase, int64_t first_pixel, int64_t n_pixels, int spp,

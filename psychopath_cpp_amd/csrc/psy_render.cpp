@@ -392,7 +392,8 @@ int psy_render(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, f
     // chunk: whole pixels, at most ctx->render_chunk paths (at least one pixel)
     int64_t chunk_pix = std::max<int64_t>(1, ctx->render_chunk / spp);
     chunk_pix = std::min(chunk_pix, n_pix);
-    const int64_t cap = chunk_pix * spp;
+    const int64_t chunk = chunk_pix * spp;              // paths per chunk (whole pixels)
+    const int64_t cap = (chunk + 63) / 64 * 64;          // storage stride, keeps every array 256-byte aligned
     if (cap > (int64_t)INT32_MAX) return fail(PSY_ERR_UNSUPPORTED, "psy_render: more than 2^31 paths per chunk");
 
     // scratch layout (floats): paths, 2 ray streams, hits, active lists, counters, chunk samples
@@ -431,8 +432,61 @@ int psy_render(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, f
     for (int k = 0; k < 3; ++k) R.bg[k] = s->background[k];
 
     int64_t traced = 0;
-    for (int64_t first = 0; first < total; first += cap) {
-        const int64_t n = std::min(cap, total - first);
+    if (ctx->render_mode == PSY_RENDER_FUSED) {
+        const int bpc = ctx->render_blocks_per_cu > 0 ? ctx->render_blocks_per_cu : std::max(1, ctx->render_occupancy);
+        const int blocks = std::max(1, ctx->n_cu * bpc);
+        uint32_t* counters = B.count + 8;  // u32 x 8 (8-byte aligned)
+        unsigned long long* steps = (unsigned long long*)(B.count + 16);
+        HIP_OK(hipMemsetAsync(steps, 0, sizeof(unsigned long long), st));
+        for (int64_t first = 0; first < total; first += chunk) {
+            const int64_t n = std::min(chunk, total - first);
+            float* samp = sample_xyz ? sample_xyz : chunk_samples;
+            const int64_t sbase = sample_xyz ? 0 : first;
+            PersistBuffers PB;
+            PB.paths = B.paths;
+            PB.counters = counters;
+            PB.defer_ids = B.act[0];
+            PB.defer_rays = B.rays[0];
+            PB.resume_ids = B.act[1];
+            PB.resume_rays = B.rays[1];
+            PB.sample_xyz = samp;
+            PB.sbase = sbase;
+            PB.cap = cap;
+            uint32_t n_resume = 0;
+            for (int round = 0;; ++round) {
+                if (round > 4 * prm->path_length + 4)
+                    return fail(PSY_ERR_HIP, "psy_render: fused rounds did not converge");
+                int rc = launch_set_counters(counters, n_resume, st);
+                if (!rc) rc = launch_render_persist(R, first, round == 0 ? n : 0, PB, blocks, ctx->render_refill, st);
+                if (rc) return fail(rc, "psy_render: fused launch failed");
+                uint32_t n_defer = 0;
+                HIP_OK(hipMemcpyAsync(&n_defer, counters + 2, sizeof n_defer, hipMemcpyDeviceToHost, st));
+                HIP_OK(hipStreamSynchronize(st));
+                if (n_defer == 0) break;
+                // the parked rays through the two-pass tracer, then shading
+                rc = launch_trace(s->dev, true, false, s->stack_need, PB.defer_rays, n_defer, cap, B.hits,
+                                  (uint32_t*)ctx->queue.p, (uint32_t*)ctx->counters.p, ctx->tuning(true), st);
+                if (!rc) rc = launch_zero_u32(counters + 4, 1, st);
+                if (!rc)
+                    rc = launch_render_update_io(R, first, B.paths, cap, PB.defer_ids, PB.defer_rays, B.hits, n_defer,
+                                                 PB.resume_ids, PB.resume_rays, counters + 4, samp, sbase, st);
+                if (rc) return fail(rc, "psy_render: second-pass launch failed");
+                HIP_OK(hipMemcpyAsync(&n_resume, counters + 4, sizeof n_resume, hipMemcpyDeviceToHost, st));
+                HIP_OK(hipStreamSynchronize(st));
+                if (n_resume == 0) break;
+            }
+            int rc = launch_render_count_steps(B.paths, cap, n, steps, st);
+            if (!rc && film_xyz) rc = launch_render_resolve(samp, sbase, first / spp, n / spp, (int)spp, film_xyz, st);
+            if (rc) return fail(rc, "psy_render: launch failed");
+        }
+        unsigned long long total_steps = 0;
+        HIP_OK(hipMemcpyAsync(&total_steps, steps, sizeof total_steps, hipMemcpyDeviceToHost, st));
+        HIP_OK(hipStreamSynchronize(st));
+        if (rays_traced) *rays_traced = (int64_t)total_steps;
+        return PSY_OK;
+    }
+    for (int64_t first = 0; first < total; first += chunk) {
+        const int64_t n = std::min(chunk, total - first);
         float* samp = sample_xyz ? sample_xyz : chunk_samples;
         const int64_t sbase = sample_xyz ? 0 : first;
         int rc = launch_render_init(R, first, n, B, st);

@@ -461,8 +461,37 @@ PSY_DEV float next_dim(const RenderLaunch& A, uint32_t off, uint32_t& dim) {
 
 // init_path + the camera ray: path slot i of the chunk, sample `first + i`
 // of the image in (y, x, s) order.
-PSY_DEV void render_init_one(const RenderLaunch& A, int64_t first, int64_t i, const PathBuf& P, float* rays,
-                             int64_t rcap, uint32_t* act) {
+// A world ray in registers (WorldRay, basics/ray.hpp:162-185).
+struct WRay {
+    V3 o, d, odx, ody, ddx, ddy;
+    float time;
+    uint32_t type;
+};
+
+// The closest-hit part of an Intersection, in registers.
+struct HitRec {
+    float t, u, v, aux;
+    uint32_t inst;
+};
+
+PSY_DEV void load_wray(const float* w, int64_t rcap, int64_t j, WRay& r) {
+    r.o = mk(w[0 * rcap + j], w[1 * rcap + j], w[2 * rcap + j]);
+    r.d = mk(w[3 * rcap + j], w[4 * rcap + j], w[5 * rcap + j]);
+    r.odx = mk(w[6 * rcap + j], w[7 * rcap + j], w[8 * rcap + j]);
+    r.ody = mk(w[9 * rcap + j], w[10 * rcap + j], w[11 * rcap + j]);
+    r.ddx = mk(w[12 * rcap + j], w[13 * rcap + j], w[14 * rcap + j]);
+    r.ddy = mk(w[15 * rcap + j], w[16 * rcap + j], w[17 * rcap + j]);
+    r.time = w[18 * rcap + j];
+    r.type = f2u(w[19 * rcap + j]);
+}
+
+PSY_DEV void store_wray(float* w, int64_t rcap, int64_t j, const WRay& r) {
+    store_wray(w, rcap, j, r.o, r.d, r.odx, r.ody, r.ddx, r.ddy, r.time, r.type);
+}
+
+// init_path + the camera ray (next_ray_for_path step 0) of path slot i of
+// the chunk, sample `first + i` of the image in (y, x, s) order.
+PSY_DEV void render_init_path(const RenderLaunch& A, int64_t first, int64_t i, const PathBuf& P, WRay& ray) {
     const int64_t gid = first + i;
     const int s = (int)(gid % A.spp);
     const int64_t pix = gid / A.spp;
@@ -476,9 +505,12 @@ PSY_DEV void render_init_one(const RenderLaunch& A, int64_t first, int64_t i, co
     const float sa = next_dim(A, off, dim);
     const float sb = next_dim(A, off, dim);
     const float time = next_dim(A, off, dim);
-    V3 o, d, ddx, ddy, z;
-    camera_ray(A.cam, A.width, A.height, px, py, sx, sy, sa, sb, time, &o, &d, &ddx, &ddy, &z);
-    store_wray(rays, rcap, i, o, d, z, z, ddx, ddy, time, 1u /* WorldRay::CAMERA */);
+    V3 z;
+    camera_ray(A.cam, A.width, A.height, px, py, sx, sy, sa, sb, time, &ray.o, &ray.d, &ray.ddx, &ray.ddy, &z);
+    ray.odx = z;
+    ray.ody = z;
+    ray.time = time;
+    ray.type = 1u;  // WorldRay::CAMERA
     P.setu(P_OFF, i, off);
     P.setu(P_DIM, i, dim);
     P.setu(P_PIX, i, (uint32_t)px | ((uint32_t)py << 16));
@@ -491,6 +523,13 @@ PSY_DEV void render_init_one(const RenderLaunch& A, int64_t first, int64_t i, co
         P.f(P_FCOL + k, i) = 1.0f;
         P.f(P_LCOL + k, i) = 0.0f;
     }
+}
+
+PSY_DEV void render_init_one(const RenderLaunch& A, int64_t first, int64_t i, const PathBuf& P, float* rays,
+                             int64_t rcap, uint32_t* act) {
+    WRay r;
+    render_init_path(A, first, i, P, r);
+    store_wray(rays, rcap, i, r);
     act[i] = (uint32_t)i;
 }
 
@@ -518,19 +557,12 @@ PSY_DEV_FN void light_hit(const DevScene& s, uint32_t li, V3 wo, float time, flo
     }
 }
 
-// The next ray of a path that goes on.
-struct NextRay {
-    V3 o, d, odx, ody, ddx, ddy;
-    float time;
-    uint32_t type;
-};
 
 // update_path for the ray just traced (slot j of the stream) of path i, then
 // next_ray_for_path if the path goes on (returns true, the ray in `nr`);
 // a finished path writes its film contribution to sample_xyz.
-PSY_DEV bool render_update_one(const RenderLaunch& A, int64_t first, const PathBuf& P, uint32_t i, int64_t j,
-                               const float* rays_in, const DevHits& hits, int64_t rcap, float* sample_xyz,
-                               int64_t sbase, NextRay& nr) {
+PSY_DEV bool render_update_path(const RenderLaunch& A, int64_t first, const PathBuf& P, uint32_t i, const WRay& ray,
+                                const HitRec& hr, float* sample_xyz, int64_t sbase, WRay& nr) {
     bool alive = false;
     V3 no = splat(0.0f), nd = splat(0.0f), nodx = splat(0.0f), nody = splat(0.0f), nddx = splat(0.0f),
        nddy = splat(0.0f);
@@ -541,9 +573,9 @@ PSY_DEV bool render_update_one(const RenderLaunch& A, int64_t first, const PathB
         uint32_t stepw = P.u(P_STEP, i);
         const int step = (int)(stepw & 0xffu);
         bool done = false;
-        const uint32_t hinst = hits.inst[j];
+        const uint32_t hinst = hr.inst;
         const bool hit = hinst != 0xffffffffu;
-        const V3 rd = mk(rays_in[3 * rcap + j], rays_in[4 * rcap + j], rays_in[5 * rcap + j]);
+        const V3 rd = ray.d;
         float col[NSPEC], fcol[NSPEC];
         for (int k = 0; k < NSPEC; ++k) { col[k] = P.f(P_COL + k, i); fcol[k] = P.f(P_FCOL + k, i); }
         if (step & 1) {
@@ -556,8 +588,8 @@ PSY_DEV bool render_update_one(const RenderLaunch& A, int64_t first, const PathB
             }
         } else if (hit) {
             const DevInstance I = s.inst[hinst];
-            const V3 ro = mk(rays_in[0 * rcap + j], rays_in[1 * rcap + j], rays_in[2 * rcap + j]);
-            const float time = rays_in[18 * rcap + j];
+            const V3 ro = ray.o;
+            const float time = ray.time;
             bool emit = false;
             float em = 0.0f, light_pdf = 9999.0f;
             if (I.kind == K_SPHERE_LIGHT || I.kind == K_RECT_LIGHT) {
@@ -584,9 +616,8 @@ PSY_DEV bool render_update_one(const RenderLaunch& A, int64_t first, const PathB
                 M44 space;
                 V3 pl, nl, offl;
                 bool backfacing = false;
-                const float t = hits.t[j];
-                hit_geometry(s, hinst, t, hits.u[j], hits.v[j], hits.aux ? hits.aux[j] : 0.0f, ro, rd, time, &space,
-                             &pl, &nl, &offl, &backfacing);
+                const float t = hr.t;
+                hit_geometry(s, hinst, t, hr.u, hr.v, hr.aux, ro, rd, time, &space, &pl, &nl, &offl, &backfacing);
                 const M44 inv = m_inverse(space);
                 const V3 gp = m_pos(inv, pl);
                 V3 gn = m_dir_transpose(space, nl);
@@ -600,10 +631,10 @@ PSY_DEV bool render_update_one(const RenderLaunch& A, int64_t first, const PathB
                 P.f(P_T, i) = t;
                 P.setu(P_SHADER, i, (uint32_t)I.shader);
                 P.set3(P_PD, i, rd);
-                P.set3(P_PODX, i, mk(rays_in[6 * rcap + j], rays_in[7 * rcap + j], rays_in[8 * rcap + j]));
-                P.set3(P_PODY, i, mk(rays_in[9 * rcap + j], rays_in[10 * rcap + j], rays_in[11 * rcap + j]));
-                P.set3(P_PDDX, i, mk(rays_in[12 * rcap + j], rays_in[13 * rcap + j], rays_in[14 * rcap + j]));
-                P.set3(P_PDDY, i, mk(rays_in[15 * rcap + j], rays_in[16 * rcap + j], rays_in[17 * rcap + j]));
+                P.set3(P_PODX, i, ray.odx);
+                P.set3(P_PODY, i, ray.ody);
+                P.set3(P_PDDX, i, ray.ddx);
+                P.set3(P_PDDY, i, ray.ddy);
                 stepw = backfacing ? (stepw | STEP_BACKFACING) : (stepw & ~STEP_BACKFACING);
             }
         } else {
@@ -704,6 +735,49 @@ PSY_DEV bool render_update_one(const RenderLaunch& A, int64_t first, const PathB
     nr.time = ntime;
     nr.type = ntype;
     return alive;
+}
+
+// A lane of the fused path kernel starts tracing world ray w (WorldRay::
+// to_ray, basics/ray.hpp:310-323, then the traversal state).
+PSY_DEV void lane_init_wray(const DevScene& s, const WRay& w, uint32_t i, Lane& L) {
+    L.r.o = w.o;
+    L.r.d = w.d;
+    L.r.odx = w.odx;
+    L.r.ody = w.ody;
+    L.r.ddx = w.ddx;
+    L.r.ddy = w.ddy;
+    L.r.time = w.time;
+    L.r.occl = w.type == PSY_RAY_OCCLUSION;
+    L.r.w = ray_widths(L.r.o, L.r.d, L.r.odx, L.r.ody, L.r.ddx, L.r.ddy);
+    L.r.dinv = divv(splat(1.0f), L.r.d);
+    lane_start(s, L, L.r.occl ? 1.0f : INFINITY, i);
+}
+
+// The ray a lane just traced, as a world ray.
+PSY_DEV void lane_wray(const Lane& L, uint32_t type, WRay& w) {
+    w.o = L.r.o;
+    w.d = L.r.d;
+    w.odx = L.r.odx;
+    w.ody = L.r.ody;
+    w.ddx = L.r.ddx;
+    w.ddy = L.r.ddy;
+    w.time = L.r.time;
+    w.type = type;
+}
+
+// render_update_path for slot j of a ray / hit stream.
+PSY_DEV bool render_update_one(const RenderLaunch& A, int64_t first, const PathBuf& P, uint32_t i, int64_t j,
+                               const float* rays_in, const DevHits& hits, int64_t rcap, float* sample_xyz,
+                               int64_t sbase, WRay& nr) {
+    WRay ray;
+    load_wray(rays_in, rcap, j, ray);
+    HitRec hr;
+    hr.t = hits.t[j];
+    hr.inst = hits.inst[j];
+    hr.u = hits.u[j];
+    hr.v = hits.v[j];
+    hr.aux = hits.aux ? hits.aux[j] : 0.0f;
+    return render_update_path(A, first, P, i, ray, hr, sample_xyz, sbase, nr);
 }
 
 // Film::add_sample over a pixel's samples, in sample order: the mean of the

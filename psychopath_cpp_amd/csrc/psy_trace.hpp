@@ -136,7 +136,7 @@ PSY_DEV bool accept_leaf(float h0, float h1, float u, float v, float maxd, uint3
 // patch root for every visited node.  `sidx` is the reference's split-stack
 // index of k (object/patch_utils.hpp:80: a node at stack index 63 is a leaf).
 template <int N>
-PSY_DEV_NOINLINE bool dice_deep(const DevScene& s, const DevObject& ob, uint32_t k0, uint32_t inst, V3 o,
+PSY_DEV bool dice_deep(const DevScene& s, const DevObject& ob, uint32_t k0, uint32_t inst, V3 o,
                                        V3 dinv, RayW w, bool occl, Best& b) {
     Patch<N> base;
     load_patch<N>(s.odata, ob.data, base);
@@ -594,11 +594,18 @@ PSY_DEV V3 walk_dinv(const Lane& L) { return WORLD ? L.wdinv : L.r.dinv; }
 template <bool WORLD>
 PSY_DEV const RayW& walk_w(const Lane& L) { return WORLD ? L.ww : L.r.w; }
 
+PSY_DEV void lane_start(const DevScene& s, Lane& L, float max_t, uint32_t i);
+
 template <bool WORLD>
 PSY_DEV void lane_init(const DevScene& s, const float* __restrict__ rays, int64_t n, uint32_t i,
                                           uint32_t flags, Lane& L) {
     float max_t;
     load_ray<WORLD>(rays, n, (int64_t)i, flags, L.r, max_t);
+    lane_start(s, L, max_t, i);
+}
+
+// Traversal state of a lane whose ray L.r is loaded.
+PSY_DEV void lane_start(const DevScene& s, Lane& L, float max_t, uint32_t i) {
     L.b.t = max_t;
     L.b.inst = 0xffffffffu;
     L.b.u = L.b.v = L.b.aux = 0.0f;
@@ -780,17 +787,25 @@ PSY_DEV bool lane_step(const DevScene& s, Lane& L, int32_t (*stk_ref)[TRACE_BLOC
     return intersect_leaf<WORLD, false>(s, li, I, L.r, L.b, unused);
 }
 
-// Writes the lane's hit; true if the ray goes to the second pass.
-PSY_DEV bool lane_finish(const DevScene& s, Lane& L, const DevHits& h) {
+// The lane's final hit (t, inst, u, v, aux) in place in L.b; true if the
+// ray goes to the second pass.
+PSY_DEV bool lane_result(const DevScene& s, Lane& L) {
     const bool hit = L.b.inst != 0xffffffffu;
     if (hit && !L.r.occl) sphere_uv(s, L.b, L.r);
+    if (!hit) L.b.t = INFINITY;
+    return (L.state & (LANE_NEED_DEEP | LANE_RETRACE)) != 0 && !(L.r.occl && hit);
+}
+
+// Writes the lane's hit; true if the ray goes to the second pass.
+PSY_DEV bool lane_finish(const DevScene& s, Lane& L, const DevHits& h) {
+    const bool again = lane_result(s, L);
     const uint32_t i = L.idx;
-    h.t[i] = hit ? L.b.t : INFINITY;
+    h.t[i] = L.b.t;
     h.inst[i] = L.b.inst;
     h.u[i] = L.b.u;
     h.v[i] = L.b.v;
     if (h.aux) h.aux[i] = L.b.aux;
-    return (L.state & (LANE_NEED_DEEP | LANE_RETRACE)) != 0 && !(L.r.occl && hit);
+    return again;
 }
 
 PSY_DEV void enqueue_retrace(uint32_t* __restrict__ queue, uint32_t* __restrict__ counters,

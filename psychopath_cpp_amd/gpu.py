@@ -38,6 +38,7 @@ def lib():
     L.psy_ctx_set_trace_mode.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.psy_ctx_retrace_count.argtypes = [vp, C.POINTER(i64)]
     L.psy_ctx_set_render_chunk.argtypes = [vp, i64]
+    L.psy_ctx_set_render_mode.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.psy_scene_create.argtypes = [vp, C.POINTER(abi.SceneDesc), C.c_int, C.POINTER(vp)]
     L.psy_scene_destroy.argtypes = [vp]
     L.psy_scene_get_info.argtypes = [vp, C.POINTER(abi.SceneInfo)]
@@ -242,3 +243,9 @@ def render(ctx, scene, width, height, spp, seed=1, path_length=4, film=True, sam
 def set_render_chunk(ctx, paths):
     """psy_ctx_set_render_chunk: paths per wavefront chunk of psy_render."""
     check(lib().psy_ctx_set_render_chunk(ctx.handle, int(paths)), "psy_ctx_set_render_chunk")
+
+
+def set_render_mode(ctx, fused=False, refill_min=16, blocks_per_cu=0):
+    """psy_ctx_set_render_mode: wavefront (default) or fused persistent paths."""
+    mode = abi.PSY_RENDER_FUSED if fused else abi.PSY_RENDER_WAVEFRONT
+    check(lib().psy_ctx_set_render_mode(ctx.handle, mode, refill_min, blocks_per_cu), "psy_ctx_set_render_mode")

@@ -158,7 +158,7 @@ int hb_render(const psy_scene_desc* d, int dice_depth, const psy_render_params* 
     for (int64_t i = 0; i < cap; ++i) render_init_one(R, 0, i, P, rays[0].data(), cap, act[0].data());
     int64_t n_active = cap, traced = 0;
     int cur = 0;
-    std::vector<NextRay> nrs((size_t)cap);
+    std::vector<WRay> nrs((size_t)cap);
     std::vector<char> alive((size_t)cap);
     for (int step = 0; step < 2 * prm->path_length && n_active > 0; ++step) {
         traced += n_active;
@@ -172,7 +172,7 @@ int hb_render(const psy_scene_desc* d, int dice_depth, const psy_render_params* 
         for (int64_t j = 0; j < n_active; ++j) {
             if (!alive[j]) continue;
             act[nxt][k] = act[cur][j];
-            const NextRay& r = nrs[j];
+            const WRay& r = nrs[j];
             store_wray(rays[nxt].data(), cap, k, r.o, r.d, r.odx, r.ody, r.ddx, r.ddy, r.time, r.type);
             ++k;
         }

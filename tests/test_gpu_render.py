@@ -125,3 +125,24 @@ def test_gpu_render_rejects_chromatic_colours(gpu_ctx):
     g = gpu.GpuScene(gpu_ctx, sc)
     with pytest.raises(gpu.PsyError, match="grey"):
         gpu.render(gpu_ctx, g, 8, 8, 1)
+
+
+@pytest.mark.parametrize("name", ["spheres", "heightfield", "instanced_lights"])
+def test_gpu_render_fused_equals_wavefront(gpu_ctx, name):
+    """The fused persistent form gives the same samples, bit for bit, and
+    the same ray count as the wavefront form."""
+    sc = {"spheres": configs.sphere_scene, "heightfield": lambda: configs.heightfield_scene(grid=16),
+          "instanced_lights": _instanced_lights_scene}[name]()
+    g = gpu.GpuScene(gpu_ctx, sc)
+    _, s_wf, r_wf = gpu.render(gpu_ctx, g, 40, 24, 5, 3, film=False, samples=True)
+    ctx = gpu.Context(0)
+    try:
+        for refill, bpc, chunk in ((16, 0, 1 << 21), (1, 1, 40 * 5), (64, 2, 7 * 5)):
+            gpu.set_render_mode(ctx, fused=True, refill_min=refill, blocks_per_cu=bpc)
+            gpu.set_render_chunk(ctx, chunk)
+            _, s_f, r_f = gpu.render(ctx, g, 40, 24, 5, 3, film=False, samples=True)
+            say(name, "fused refill", refill, "bpc", bpc, "chunk", chunk, "rays", r_f, r_wf)
+            assert r_f == r_wf
+            assert torch.equal(s_f, s_wf)
+    finally:
+        ctx.close()

@@ -248,6 +248,19 @@ int psy_render(psy_ctx* ctx, const psy_scene* scene, const psy_render_params* pa
 /* Replaces: Halton::sample(dim, index) (sampling/halton.cpp:2154-2671). */
 int psy_halton(psy_ctx* ctx, uint32_t dim, const uint32_t* index, int64_t n,
                float* out, void* stream);
+/* Sobol' generator matrices for psy_sobol: num_dims * 52 columns of 52
+ * bits, dimension-major, the layout of Sobol::Matrices::matrices
+ * (sampling/sobol.hpp:28-32, sampling/sobol.cpp:36; the reference's table
+ * has 1024 dimensions).  Host memory, copied to the device. */
+int psy_sobol_set_matrices(psy_ctx* ctx, const uint64_t* matrices, uint32_t num_dims);
+/* Replaces: Sobol::sample(dimension, index, scramble)
+ * (sampling/sobol.hpp:40-53) over a stream of indices (device memory):
+ * out[i] = (scramble_low52 ^ XOR of column (dim*52 + b) for each set bit b
+ * of index[i]) * 2^-52.  As in the reference's flat indexing, index bits
+ * 52 and up select the next dimension's columns; columns past the table
+ * read as zero. */
+int psy_sobol(psy_ctx* ctx, uint32_t dim, const uint64_t* index, int64_t n,
+              uint64_t scramble, double* out, void* stream);
 /* Replaces: Sampler(x,y,s,seed) + Sampler::get_sample(dim)
  * (sampling/image_sampler.hpp:16-38): out[i*n_dims + k] = sample of dim
  * dim0+k for pixel sample i of the tile (x0..x0+w, y0..y0+h, spp). */

@@ -49,6 +49,7 @@ def lib():
         L.ora_halton.restype = C.c_float
         L.ora_halton.argtypes = [u32, u32]
         L.ora_halton_batch.argtypes = [u32, vp, i64, vp]
+        L.ora_sobol_batch.argtypes = [vp, u32, u32, vp, i64, C.c_uint64, vp]
         L.ora_hash_u32.restype = u32
         L.ora_hash_u32.argtypes = [u32, u32]
         L.ora_sampler_offset.restype = u32
@@ -121,6 +122,76 @@ def halton(dim, index):
     out = np.empty(index.shape, np.float32)
     lib().ora_halton_batch(dim, _p(index), index.size, _p(out))
     return out
+
+
+def sobol(matrices, dim, index, scramble=0):
+    """Sobol::sample over an index array with the given generator matrices
+    (uint64, 52 columns per dimension)."""
+    m = np.ascontiguousarray(matrices, np.uint64).reshape(-1)
+    index = np.ascontiguousarray(index, np.uint64)
+    out = np.empty(index.shape, np.float64)
+    lib().ora_sobol_batch(_p(m), m.size, dim, _p(index), index.size, int(scramble), _p(out))
+    return out
+
+
+def ref_sobol_matrices():
+    """The reference's own generator table (Sobol::Matrices::matrices) read
+    from oracle/_ref, or None when that library was not built."""
+    R = ref()
+    if R is None:
+        return None
+    n = R.psyref_sobol_dims() * 52
+    return np.array([R.psyref_sobol_matrix(i) for i in range(n)], np.uint64)
+
+
+def make_sobol_matrices(n_dims, seed=1):
+    """Generator matrices of a Sobol' sequence built from its definition
+    (Bratley & Fox 1988): dimension 0 is the van der Corput sequence, each
+    further one uses a primitive polynomial over GF(2) with random odd
+    initial direction numbers.  A valid Sobol' construction, not the
+    Joe-Kuo table; for tests that need matrices without the reference."""
+    rng = np.random.default_rng(seed)
+    polys = []  # (degree, interior coefficient bits)
+    deg = 1
+    while len(polys) < n_dims:
+        for a in range(1 << (deg - 1)):
+            p = (1 << deg) | (a << 1) | 1
+            if _primitive_gf2(p, deg):
+                polys.append((deg, a))
+                if len(polys) >= n_dims:
+                    break
+        deg += 1
+    out = np.zeros((n_dims, 52), np.uint64)
+    for k in range(52):
+        out[0, k] = 1 << (51 - k)
+    for d in range(1, n_dims):
+        s, a = polys[d - 1]
+        m = [int(rng.integers(0, 1 << i)) * 2 + 1 if i else 1 for i in range(s)]
+        for i in range(s, 52):
+            v = m[i - s] ^ (m[i - s] << s)
+            for k in range(1, s):
+                if (a >> (s - 1 - k)) & 1:
+                    v ^= m[i - k] << k
+            m.append(v)
+        for i in range(52):
+            out[d, i] = m[i] << (51 - i)
+    return out.reshape(-1)
+
+
+def _primitive_gf2(p, deg):
+    order = (1 << deg) - 1
+    if deg == 1:
+        return True
+    x, e = 2, 1
+    while True:  # smallest e with x^e == 1 mod p
+        x <<= 1
+        if x >> deg:
+            x ^= p
+        e += 1
+        if x == 1:
+            return e == order
+        if e > order:
+            return False
 
 
 def matrix_inverse(m):

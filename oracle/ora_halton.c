@@ -152,3 +152,20 @@ float ora_sampler_get(uint32_t offset, uint32_t dim) {
     if (dim < 11) return ora_halton(k_dim_order[dim], offset);
     return ora_halton(dim, offset);
 }
+
+/* Sobol::sample (sampling/sobol.hpp:40-53) with the generator matrices
+ * passed in (Sobol::Matrices::matrices, sampling/sobol.cpp:36: 52 columns
+ * per dimension, dimension-major).  The loop walks the flat table from
+ * dim*52 while index bits remain, as the reference does; columns past the
+ * end of the table (n_cols) read as zero. */
+double ora_sobol(const uint64_t* m, uint32_t n_cols, uint32_t dim, uint64_t index, uint64_t scramble) {
+    uint64_t r = scramble & ~(~0ull << 52);
+    for (uint32_t i = dim * 52u; index; index >>= 1, ++i)
+        if ((index & 1ull) && i < n_cols) r ^= m[i];
+    return (double)r * (1.0 / (double)(1ull << 52));
+}
+
+void ora_sobol_batch(const uint64_t* m, uint32_t n_cols, uint32_t dim, const uint64_t* index, int64_t n,
+                     uint64_t scramble, double* out) {
+    for (int64_t i = 0; i < n; ++i) out[i] = ora_sobol(m, n_cols, dim, index[i], scramble);
+}

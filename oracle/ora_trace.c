@@ -403,7 +403,7 @@ static int sphere_light_intersect(const ora_object* ob, const ora_ray* ray, ora_
         in->light_pdf = ora_sphere_light_pdf(cent, (double)radi, ray->o);
         set3(in->offset, v3scale(n, 0.000001f));
         float col[3];
-        float na;
+        float na = 0.0f;
         int i = lerp_seq_index(ray->time, ob->ts, &na);
         for (int k = 0; k < 3; ++k) {
             if (i == -1) col[k] = ob->data[4 + k];

@@ -53,6 +53,9 @@ typedef struct {
 /* sampling */
 float ora_halton(uint32_t dim, uint32_t index);
 void ora_halton_batch(uint32_t dim, const uint32_t* index, int64_t n, float* out);
+double ora_sobol(const uint64_t* m, uint32_t n_cols, uint32_t dim, uint64_t index, uint64_t scramble);
+void ora_sobol_batch(const uint64_t* m, uint32_t n_cols, uint32_t dim, const uint64_t* index, int64_t n,
+                     uint64_t scramble, double* out);
 uint32_t ora_hash_u32(uint32_t n, uint32_t seed);
 uint32_t ora_sampler_offset(uint32_t x, uint32_t y, uint32_t n, uint32_t seed);
 float ora_sampler_get(uint32_t offset, uint32_t dim);

@@ -841,6 +841,29 @@ int psy_halton(psy_ctx* ctx, uint32_t dim, const uint32_t* index, int64_t n, flo
     return rc ? fail(rc, "psy_halton: launch failed") : PSY_OK;
 }
 
+int psy_sobol_set_matrices(psy_ctx* ctx, const uint64_t* matrices, uint32_t num_dims) {
+    if (!ctx || !matrices || num_dims == 0 || num_dims > (1u << 20))
+        return fail(PSY_ERR_INVALID, "psy_sobol_set_matrices: bad argument");
+    HIP_OK(hipSetDevice(ctx->device));
+    std::vector<uint64_t> m(matrices, matrices + (size_t)num_dims * 52u);
+    for (uint64_t v : m)
+        if (v >> 52) return fail(PSY_ERR_INVALID, "psy_sobol_set_matrices: a column has bits above bit 51");
+    if (int rc = ctx->sobol_m.upload(m)) return rc;
+    ctx->sobol_dims = num_dims;
+    return PSY_OK;
+}
+
+int psy_sobol(psy_ctx* ctx, uint32_t dim, const uint64_t* index, int64_t n, uint64_t scramble, double* out,
+              void* stream) {
+    if (!ctx || ((!index || !out) && n > 0) || n < 0) return fail(PSY_ERR_INVALID, "psy_sobol: bad argument");
+    if (ctx->sobol_dims == 0) return fail(PSY_ERR_INVALID, "psy_sobol: no matrices (psy_sobol_set_matrices)");
+    if (dim >= ctx->sobol_dims) return fail(PSY_ERR_INVALID, "psy_sobol: dimension out of range");
+    HIP_OK(hipSetDevice(ctx->device));
+    int rc = launch_sobol((const uint64_t*)ctx->sobol_m.p, ctx->sobol_dims * 52u, dim, index, n, scramble, out,
+                          (hipStream_t)stream);
+    return rc ? fail(rc, "psy_sobol: launch failed") : PSY_OK;
+}
+
 int psy_pixel_samples(psy_ctx* ctx, int32_t x0, int32_t y0, int32_t w, int32_t h, int32_t spp, uint32_t seed,
                       uint32_t dim0, int32_t n_dims, float* out, void* stream) {
     if (!ctx || w < 0 || h < 0 || spp < 1 || n_dims < 0 || (!out && (int64_t)w * h * n_dims > 0))

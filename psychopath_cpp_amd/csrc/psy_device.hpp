@@ -197,6 +197,8 @@ int launch_render_update_io(const RenderLaunch& R, int64_t first, float* paths, 
 int launch_render_count_steps(float* paths, int64_t cap, int64_t n, unsigned long long* out, hipStream_t st);
 // counters[0..7] = 0 except counters[3] = n_resume
 int launch_set_counters(uint32_t* p, uint32_t n_resume, hipStream_t st);
+int launch_sobol(const uint64_t* m, uint32_t n_cols, uint32_t dim, const uint64_t* idx, int64_t n,
+                 uint64_t scramble, double* out, hipStream_t st);
 int launch_halton(const HaltonDim* dims, const uint16_t* perm, uint32_t dim, const uint32_t* idx,
                   int64_t n, float* out, hipStream_t st);
 int launch_pixel_samples(const HaltonDim* dims, const uint16_t* perm, int x0, int y0, int w, int h,

@@ -66,6 +66,8 @@ struct DBuf {
 struct psy_ctx {
     int device = 0;
     psy::DBuf halton_dims, halton_perm;
+    psy::DBuf sobol_m;         // generator matrices, 52 columns per dimension
+    uint32_t sobol_dims = 0;
     // trace scratch: queue of rays for the second pass + its counters
     psy::DBuf queue, counters;
     int64_t queue_cap = 0;

@@ -19,6 +19,26 @@ __global__ void k_halton(const HaltonDim* dims, const uint16_t* perm, uint32_t d
     out[i] = halton_sample(dims, perm, dim, idx[i]);
 }
 
+// Sobol::sample (sampling/sobol.hpp:40-53): the XOR of the generator
+// columns of `dim` selected by the set bits of the index, plus the scramble's
+// low 52 bits, as a double in [0,1).  The reference indexes its flat table
+// at dim*52 + bit, so index bits 52 and up read the next dimension's
+// columns; columns past the end of the table read as zero here.
+__global__ void k_sobol(const uint64_t* m, uint32_t n_cols, uint32_t dim, const uint64_t* idx, int64_t n,
+                        uint64_t scramble, double* out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t index = idx[i];
+    uint64_t r = scramble & ((1ull << 52) - 1ull);
+    uint32_t c = dim * 52u;
+    while (index) {
+        if ((index & 1ull) && c < n_cols) r ^= m[c];
+        index >>= 1;
+        ++c;
+    }
+    out[i] = (double)r * (1.0 / 4503599627370496.0);
+}
+
 __global__ void k_pixel_samples(const HaltonDim* dims, const uint16_t* perm, int x0, int y0, int w, int h,
                                 int spp, uint32_t seed, uint32_t dim0, int n_dims, float* out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -463,6 +483,13 @@ int launch_halton(const HaltonDim* dims, const uint16_t* perm, uint32_t dim, con
                   float* out, hipStream_t st) {
     if (n <= 0) return 0;
     hipLaunchKernelGGL(k_halton, dim3(grid_for(n, 256)), dim3(256), 0, st, dims, perm, dim, idx, n, out);
+    return check_launch();
+}
+
+int launch_sobol(const uint64_t* m, uint32_t n_cols, uint32_t dim, const uint64_t* idx, int64_t n,
+                 uint64_t scramble, double* out, hipStream_t st) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_sobol, dim3(grid_for(n, 256)), dim3(256), 0, st, m, n_cols, dim, idx, n, scramble, out);
     return check_launch();
 }
 

@@ -47,6 +47,8 @@ def lib():
     L.psy_trace_rays.argtypes = [vp, vp, vp, i64, C.POINTER(abi.Hits), u32, vp]
     L.psy_world_to_rays.argtypes = [vp, vp, i64, vp, vp]
     L.psy_halton.argtypes = [vp, u32, vp, i64, vp, vp]
+    L.psy_sobol_set_matrices.argtypes = [vp, vp, u32]
+    L.psy_sobol.argtypes = [vp, u32, vp, i64, C.c_uint64, vp, vp]
     L.psy_pixel_samples.argtypes = [vp, i32, i32, i32, i32, i32, u32, u32, i32, vp, vp]
     L.psy_camera_rays.argtypes = [vp, vp, i32, i32, i32, u32, vp, vp, vp]
     L.psy_bounce_rays.argtypes = [vp, vp, vp, C.POINTER(abi.Hits), i64, i32, i32, u32, u32, u32, vp, vp, vp]
@@ -194,6 +196,27 @@ def halton(ctx, dim, index):
         raise PsyError("index must be a contiguous CUDA int32 tensor (uint32 bit patterns)")
     out = torch.empty(index.shape, dtype=torch.float32, device=index.device)
     check(lib().psy_halton(ctx.handle, dim, _ptr(index), index.numel(), _ptr(out), _stream()), "psy_halton")
+    return out
+
+
+def sobol_set_matrices(ctx, matrices):
+    """Generator matrices (numpy uint64, 52 columns per dimension), e.g. the
+    reference's Sobol::Matrices::matrices."""
+    m = np.ascontiguousarray(matrices, np.uint64).reshape(-1)
+    if m.size == 0 or m.size % 52:
+        raise PsyError("matrices must hold 52 columns per dimension")
+    check(lib().psy_sobol_set_matrices(ctx.handle, m.ctypes.data_as(C.c_void_p), m.size // 52),
+          "psy_sobol_set_matrices")
+
+
+def sobol(ctx, dim, index, scramble=0):
+    """Sobol::sample(dim, index[i], scramble) for a CUDA int64 tensor of
+    uint64 bit patterns; float64 results."""
+    if not (index.is_cuda and index.dtype == torch.int64 and index.is_contiguous()):
+        raise PsyError("index must be a contiguous CUDA int64 tensor (uint64 bit patterns)")
+    out = torch.empty(index.shape, dtype=torch.float64, device=index.device)
+    check(lib().psy_sobol(ctx.handle, dim, _ptr(index), index.numel(), int(scramble) & (2**64 - 1), _ptr(out),
+                          _stream()), "psy_sobol")
     return out
 
 

@@ -3,7 +3,7 @@
 Runs in a container that has the reference tree: builds oracle/_ref
 (oracle/Makefile, target `ref`) and records input/output vectors of the
 reference functions that compile on their own (Halton::sample, hash_u32,
-SIMD::invert_44_matrix, spectrum_xyz_to_p).  The vectors are data; the
+SIMD::invert_44_matrix, spectrum_xyz_to_p, Sobol::sample).  The vectors are data; the
 tests compare the oracle and the GPU library against them on any machine.
 """
 import os
@@ -51,9 +51,28 @@ def main():
     xyz = np.ones(3, np.float32)
     spec = np.array([R.psyref_spectrum_xyz_to_p(float(l), xyz.ctypes.data_as(__import__("ctypes").c_void_p))
                      for l in lam], np.float32)
+    # Sobol::sample: every 16th dimension plus the ends, edge and random
+    # 64-bit indices (bits 52+ reach into the next dimension's columns, so
+    # the last dimension only takes indices below 2^52), random scrambles
+    sdims, sidx, sscr = [], [], []
+    sedge = [0, 1, 2, 3, 2**20, 2**32 - 1, 2**32, 2**52 - 1]
+    nd = R.psyref_sobol_dims()
+    for d in sorted(set(list(range(0, nd, 16)) + [1, 2, nd - 2, nd - 1])):
+        top = 64 if d < nd - 1 else 52
+        ix = sedge + [2**52, 2**63, 2**64 - 1] if top == 64 else list(sedge)
+        ix += [int(v) for v in rng.integers(0, 2**top - 1, 24, dtype=np.uint64, endpoint=True)]
+        sdims += [d] * len(ix)
+        sidx += ix
+        sscr += [0 if k % 3 == 0 else int(v) for k, v in
+                 enumerate(rng.integers(0, 2**64 - 1, len(ix), dtype=np.uint64, endpoint=True))]
+    sdims = np.array(sdims, np.uint32)
+    sidx = np.array(sidx, np.uint64)
+    sscr = np.array(sscr, np.uint64)
+    sval = np.array([R.psyref_sobol(int(d), int(i), int(c)) for d, i, c in zip(sdims, sidx, sscr)], np.float64)
     out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ref_vectors.npz")
     np.savez_compressed(out, halton_dim=hd, halton_index=hi, halton_value=hv, hash_n=hn, hash_seed=hs,
-                        hash_value=hh, inv_in=mats, inv_out=invs, spectrum_lambda=lam, spectrum_grey=spec)
+                        hash_value=hh, inv_in=mats, inv_out=invs, spectrum_lambda=lam, spectrum_grey=spec, sobol_dim=sdims,
+                        sobol_index=sidx, sobol_scramble=sscr, sobol_value=sval)
     print("wrote", out)
 
 

@@ -360,3 +360,36 @@ def test_gpu_stack_overflow_retrace(gpu_ctx, oracle):
             assert torch.equal(ho.inst != -1, hits.inst != -1)
     finally:
         ctx.close()
+
+
+def test_gpu_sobol_bit_exact(gpu_ctx, oracle):
+    """psy_sobol against the reference's own outputs (with its generator
+    table from oracle/_ref) and against the oracle (with matrices built
+    from the definition), bit for bit."""
+    rv = np.load(os.path.join(GOLD, "ref_vectors.npz"))
+    M = oracle.ref_sobol_matrices()
+    if M is not None:
+        gpu.sobol_set_matrices(gpu_ctx, M)
+        n = 0
+        for d in np.unique(rv["sobol_dim"]):
+            sel = rv["sobol_dim"] == d
+            for c in np.unique(rv["sobol_scramble"][sel]):
+                s2 = sel & (rv["sobol_scramble"] == c)
+                t = torch.from_numpy(rv["sobol_index"][s2].view(np.int64).copy()).cuda()
+                got = gpu.sobol(gpu_ctx, int(d), t, int(c)).cpu().numpy()
+                assert np.array_equal(got.view(np.uint64), rv["sobol_value"][s2].view(np.uint64)), (d, c)
+                n += int(s2.sum())
+        say("sobol: reference vectors", n)
+    else:
+        say("sobol: oracle/_ref absent, reference vectors skipped")
+    m = oracle.make_sobol_matrices(64, seed=3)
+    gpu.sobol_set_matrices(gpu_ctx, m)
+    rng = np.random.default_rng(11)
+    idx = rng.integers(0, 2**64 - 1, 1 << 16, dtype=np.uint64, endpoint=True)
+    idx[:4] = [0, 1, 2**52, 2**64 - 1]
+    t = torch.from_numpy(idx.view(np.int64).copy()).cuda()
+    for d, c in ((0, 0), (7, 0x123456789ABCDEF), (63, 2**64 - 1)):
+        got = gpu.sobol(gpu_ctx, d, t, c).cpu().numpy()
+        assert np.array_equal(got.view(np.uint64), oracle.sobol(m, d, idx, c).view(np.uint64)), d
+    with pytest.raises(gpu.PsyError):
+        gpu.sobol(gpu_ctx, 64, t)

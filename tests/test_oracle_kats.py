@@ -159,3 +159,53 @@ def test_oracle_against_ref_build_if_present(oracle):
         a = np.float32(oracle.lib().ora_halton(d, i))
         b = np.float32(R.psyref_halton(d, i))
         assert a.view(np.uint32) == b.view(np.uint32)
+
+
+def _bitrev52(i):
+    return int("{:052b}".format(i)[::-1], 2)
+
+
+def test_sobol_closed_form_and_net_property(oracle):
+    # Generator matrices built from the definition (van der Corput first
+    # dimension, primitive polynomials after it).  Dimension 0 is the 52-bit
+    # bit reversal; the scramble XORs its low 52 bits; dimensions 0 and 1
+    # form a (0, m, 2)-net for every m: each elementary interval of area
+    # 2^-m holds exactly one of the first 2^m points (sampling/sobol.hpp:40-53).
+    m = oracle.make_sobol_matrices(4)
+    idx = np.arange(1 << 10, dtype=np.uint64)
+    d0 = oracle.sobol(m, 0, idx)
+    assert np.array_equal(d0, np.array([_bitrev52(int(i)) for i in idx], np.float64) * 2.0**-52)
+    scr = 0xFFF0_1234_5678_9ABC
+    d0s = oracle.sobol(m, 0, idx, scr)
+    want = np.array([_bitrev52(int(i)) ^ (scr & (2**52 - 1)) for i in idx], np.float64) * 2.0**-52
+    assert np.array_equal(d0s, want)
+    d1 = oracle.sobol(m, 1, idx)
+    for mm in range(1, 11):
+        n = 1 << mm
+        for a in range(mm + 1):
+            cx = np.floor(d0[:n] * (1 << a)).astype(np.int64)
+            cy = np.floor(d1[:n] * (1 << (mm - a))).astype(np.int64)
+            cells = cx * (1 << (mm - a)) + cy
+            assert np.unique(cells).size == n, (mm, a)
+    # index bits 52 and up read the next dimension's columns
+    big = np.array([1 << 52, (1 << 52) | 1], np.uint64)
+    got = oracle.sobol(m, 0, big)
+    assert got[0] == float(int(m[52])) * 2.0**-52
+    assert got[1] == float(int(m[52]) ^ int(m[0])) * 2.0**-52
+
+
+def test_sobol_matches_reference_vectors(oracle):
+    M = oracle.ref_sobol_matrices()
+    if M is None:
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    # the reference's table: dimension 0 is van der Corput, dimension 1 the
+    # direction numbers 1, 3, 5, 15, ... of x + 1
+    assert all(int(M[k]) == 1 << (51 - k) for k in range(52))
+    assert [int(M[52 + k]) >> (51 - k) for k in range(4)] == [1, 3, 5, 15]
+    rv = refvec()
+    for d in np.unique(rv["sobol_dim"]):
+        sel = rv["sobol_dim"] == d
+        for c in np.unique(rv["sobol_scramble"][sel]):
+            s2 = sel & (rv["sobol_scramble"] == c)
+            got = oracle.sobol(M, int(d), rv["sobol_index"][s2], int(c))
+            assert np.array_equal(got.view(np.uint64), rv["sobol_value"][s2].view(np.uint64)), (d, c)

@@ -181,6 +181,13 @@ int psy_ctx_set_render_mode(psy_ctx* ctx, int mode, int refill_min, int blocks_p
  * to the second pass (split below the dice cache, or a BVH stack deeper
  * than the main pass keeps).  Synchronizes the device; for diagnostics. */
 int psy_ctx_retrace_count(psy_ctx* ctx, int64_t* out);
+/* Pass timing (no counterpart in the reference): while enabled, each trace
+ * call records HIP events on its stream around the main pass and the
+ * second pass.  psy_ctx_trace_timing waits for the recorded calls and
+ * returns the summed milliseconds of each pass and the number of calls
+ * since the previous query, then forgets them. */
+int psy_ctx_set_trace_timing(psy_ctx* ctx, int enable);
+int psy_ctx_trace_timing(psy_ctx* ctx, double* main_ms, double* second_ms, int64_t* calls);
 const char* psy_last_error(void);
 int psy_abi_version(void);
 

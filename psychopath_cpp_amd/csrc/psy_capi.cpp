@@ -213,6 +213,37 @@ int psy_ctx_retrace_count(psy_ctx* ctx, int64_t* out) {
     return PSY_OK;
 }
 
+int psy_ctx_set_trace_timing(psy_ctx* ctx, int enable) {
+    if (!ctx) return fail(PSY_ERR_INVALID, "psy_ctx_set_trace_timing: NULL context");
+    ctx->timing = enable != 0;
+    return PSY_OK;
+}
+
+int psy_ctx_trace_timing(psy_ctx* ctx, double* main_ms, double* second_ms, int64_t* calls) {
+    if (!ctx) return fail(PSY_ERR_INVALID, "psy_ctx_trace_timing: NULL context");
+    HIP_OK(hipSetDevice(ctx->device));
+    double a = 0.0, b = 0.0;
+    int64_t n = 0;
+    int rc = PSY_OK;
+    for (size_t i = 0; i + 2 < ctx->timing_events.size(); i += 3) {
+        hipEvent_t e0 = (hipEvent_t)ctx->timing_events[i], e1 = (hipEvent_t)ctx->timing_events[i + 1],
+                   e2 = (hipEvent_t)ctx->timing_events[i + 2];
+        float x = 0.0f, y = 0.0f;
+        if (hipEventSynchronize(e2) != hipSuccess || hipEventElapsedTime(&x, e0, e1) != hipSuccess ||
+            hipEventElapsedTime(&y, e1, e2) != hipSuccess)
+            rc = PSY_ERR_HIP;
+        a += x;
+        b += y;
+        ++n;
+    }
+    for (void* e : ctx->timing_events) (void)hipEventDestroy((hipEvent_t)e);
+    ctx->timing_events.clear();
+    if (main_ms) *main_ms = a;
+    if (second_ms) *second_ms = b;
+    if (calls) *calls = n;
+    return rc ? fail(rc, "psy_ctx_trace_timing: an event failed") : PSY_OK;
+}
+
 int psy_ctx_destroy(psy_ctx* ctx) {
     if (!ctx) return PSY_OK;
     (void)hipSetDevice(ctx->device);

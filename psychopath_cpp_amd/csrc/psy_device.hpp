@@ -137,6 +137,9 @@ struct TraceTuning {
     bool persistent;
     int persist_blocks;
     int refill_min;
+    // hipEvent_t handles recorded around the passes when the context times
+    // them: before the main pass, after it, after the second pass.
+    void* ev[3] = {nullptr, nullptr, nullptr};
 };
 // Two stream-ordered passes; queue (n entries) and counters (4 x u32) are
 // scratch.

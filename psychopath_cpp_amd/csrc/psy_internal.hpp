@@ -77,12 +77,33 @@ struct psy_ctx {
     int mode = PSY_TRACE_MODE_PLAIN;
     int refill_min = 16;
     int blocks_per_cu = 0;  // <= 0: from occupancy
-    psy::TraceTuning tuning(bool world) const {
+    // pass timing: event triples of the trace calls since the last query
+    bool timing = false;
+    std::vector<void*> timing_events;
+    psy::TraceTuning tuning(bool world) {
         psy::TraceTuning t;
         t.persistent = mode == PSY_TRACE_MODE_PERSISTENT;
         t.persist_blocks = n_cu * (blocks_per_cu > 0 ? blocks_per_cu : occupancy[world ? 1 : 0]);
         t.refill_min = refill_min;
+        if (timing) {
+            for (int k = 0; k < 3; ++k) {
+                hipEvent_t e = nullptr;
+                if (hipEventCreate(&e) != hipSuccess) {
+                    for (int j = 0; j < k; ++j) {
+                        (void)hipEventDestroy((hipEvent_t)t.ev[j]);
+                        t.ev[j] = nullptr;
+                    }
+                    break;
+                }
+                t.ev[k] = e;
+            }
+            if (t.ev[0])
+                for (int k = 0; k < 3; ++k) timing_events.push_back(t.ev[k]);
+        }
         return t;
+    }
+    ~psy_ctx() {
+        for (void* e : timing_events) (void)hipEventDestroy((hipEvent_t)e);
     }
     // integrator scratch (path states, ray / hit streams, active lists)
     psy::DBuf render_scratch;

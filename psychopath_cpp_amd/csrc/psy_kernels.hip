@@ -429,6 +429,7 @@ static int launch_trace_w(const DevScene& s, int stack_need, const float* rays, 
                           uint32_t flags, uint32_t* queue, uint32_t* counters, const TraceTuning& tune,
                           hipStream_t st) {
     hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, counters);
+    if (tune.ev[0]) (void)hipEventRecord((hipEvent_t)tune.ev[0], st);
     const int64_t plain_blocks = grid_for(n, TRACE_BLOCK);
     const int64_t persist_blocks = (int64_t)tune.persist_blocks;
     if (tune.persistent && persist_blocks > 0 && persist_blocks < plain_blocks) {
@@ -438,10 +439,12 @@ static int launch_trace_w(const DevScene& s, int stack_need, const float* rays, 
         hipLaunchKernelGGL((k_trace<W>), dim3((unsigned)plain_blocks), dim3(TRACE_BLOCK), 0, st, s, rays, n, rs, h,
                            flags, queue, counters);
     }
+    if (tune.ev[1]) (void)hipEventRecord((hipEvent_t)tune.ev[1], st);
     if (stack_need <= 16) launch_deep<W, 16>(s, rays, n, rs, h, flags, queue, counters, st);
     else if (stack_need <= 32) launch_deep<W, 32>(s, rays, n, rs, h, flags, queue, counters, st);
     else if (stack_need <= 64) launch_deep<W, 64>(s, rays, n, rs, h, flags, queue, counters, st);
     else return PSY_ERR_UNSUPPORTED;
+    if (tune.ev[2]) (void)hipEventRecord((hipEvent_t)tune.ev[2], st);
     return check_launch();
 }
 

@@ -47,6 +47,8 @@ def lib():
     L.psy_trace_rays.argtypes = [vp, vp, vp, i64, C.POINTER(abi.Hits), u32, vp]
     L.psy_world_to_rays.argtypes = [vp, vp, i64, vp, vp]
     L.psy_halton.argtypes = [vp, u32, vp, i64, vp, vp]
+    L.psy_ctx_set_trace_timing.argtypes = [vp, C.c_int]
+    L.psy_ctx_trace_timing.argtypes = [vp, vp, vp, vp]
     L.psy_sobol_set_matrices.argtypes = [vp, vp, u32]
     L.psy_sobol.argtypes = [vp, u32, vp, i64, C.c_uint64, vp, vp]
     L.psy_pixel_samples.argtypes = [vp, i32, i32, i32, i32, i32, u32, u32, i32, vp, vp]
@@ -86,6 +88,17 @@ class Context:
         """psy_ctx_set_trace_mode: persistent main pass with lane refill, or one thread per ray."""
         mode = abi.PSY_TRACE_MODE_PERSISTENT if persistent else abi.PSY_TRACE_MODE_PLAIN
         check(lib().psy_ctx_set_trace_mode(self.handle, mode, refill_min, blocks_per_cu), "psy_ctx_set_trace_mode")
+
+    def set_trace_timing(self, enable=True):
+        """psy_ctx_set_trace_timing: HIP events around each trace call's passes."""
+        check(lib().psy_ctx_set_trace_timing(self.handle, 1 if enable else 0), "psy_ctx_set_trace_timing")
+
+    def trace_timing(self):
+        """psy_ctx_trace_timing: (main-pass ms, second-pass ms, calls) summed
+        over the trace calls since the previous query."""
+        a, b, n = C.c_double(), C.c_double(), C.c_int64()
+        check(lib().psy_ctx_trace_timing(self.handle, C.byref(a), C.byref(b), C.byref(n)), "psy_ctx_trace_timing")
+        return a.value, b.value, n.value
 
     def retrace_count(self):
         """psy_ctx_retrace_count: rays of the latest trace that took the second pass."""

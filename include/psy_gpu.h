@@ -251,6 +251,38 @@ typedef struct psy_render_params {
 int psy_render(psy_ctx* ctx, const psy_scene* scene, const psy_render_params* params,
                float* film_xyz, float* sample_xyz, int64_t* rays_traced, void* stream);
 
+/* ---- differentiation ---------------------------------------------------- */
+/* Differentiable scene parameters (no counterpart in the reference, which
+ * renders forward only):
+ *   albedo[k]         grey albedo of surface shader k (the scene's shader
+ *                     table; a Lambert shader's colour is (a, a, a); entries
+ *                     of other shaders are ignored and get zero gradient);
+ *   emission_scale[j] multiplier of the emission of the j-th light object,
+ *                     counting SphereLight / RectangleLight objects in scene
+ *                     order (assembly 0's objects first, then assembly 1's,
+ *                     ...); 1 after psy_scene_create.
+ * The light trees keep the energies of the scene as created: light
+ * selection probabilities do not change with the scales, so renders stay
+ * unbiased and every sample is a polynomial in the parameters.
+ * psy_scene_set_params copies host arrays synchronously (NULL keeps that
+ * set); do not call it while work that reads the scene is in flight. */
+int psy_scene_param_counts(const psy_scene* scene, int64_t* n_albedo, int64_t* n_emission);
+int psy_scene_get_params(const psy_scene* scene, float* albedo, float* emission_scale);
+int psy_scene_set_params(psy_scene* scene, const float* albedo, int64_t n_albedo,
+                         const float* emission_scale, int64_t n_emission);
+/* Path-replay adjoint of psy_render: the gradients of a scalar image loss
+ * with respect to the parameters above, given d_film_xyz = dLoss/dfilm
+ * (device, width*height*3 floats, the layout of psy_render's film_xyz).
+ * Every path is re-run from its sample index with the same sampler as the
+ * forward render (same seed, spp, path_length: identical paths, nothing
+ * stored between the calls), and each radiance contribution adds its
+ * derivatives.  d_albedo (n_albedo floats) and d_emission (n_emission
+ * floats) are device memory and are overwritten.  path_length <= 8.
+ * rays_traced as in psy_render.  Synchronizes `stream` once per step. */
+int psy_adjoint(psy_ctx* ctx, const psy_scene* scene, const psy_render_params* params,
+                const float* d_film_xyz, float* d_albedo, float* d_emission,
+                int64_t* rays_traced, void* stream);
+
 /* ---- sampling ----------------------------------------------------------- */
 /* Replaces: Halton::sample(dim, index) (sampling/halton.cpp:2154-2671). */
 int psy_halton(psy_ctx* ctx, uint32_t dim, const uint32_t* index, int64_t n,

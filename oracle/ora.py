@@ -49,6 +49,8 @@ def lib():
         L.ora_halton.restype = C.c_float
         L.ora_halton.argtypes = [u32, u32]
         L.ora_halton_batch.argtypes = [u32, vp, i64, vp]
+        L.ora_scene_set_params.restype = C.c_int
+        L.ora_scene_set_params.argtypes = [vp, vp, C.c_int, vp, C.c_int]
         L.ora_sobol_batch.argtypes = [vp, u32, u32, vp, i64, C.c_uint64, vp]
         L.ora_hash_u32.restype = u32
         L.ora_hash_u32.argtypes = [u32, u32]
@@ -224,6 +226,16 @@ class OracleScene:
             self.close()
         except Exception:
             pass
+
+    def set_params(self, albedo=None, emission_scale=None):
+        """Grey albedo per shader and emission scale per light object (the
+        product's psy_scene_set_params)."""
+        a = None if albedo is None else np.ascontiguousarray(albedo, np.float32)
+        e = None if emission_scale is None else np.ascontiguousarray(emission_scale, np.float32)
+        rc = lib().ora_scene_set_params(self.handle, None if a is None else _p(a), 0 if a is None else a.size,
+                                        None if e is None else _p(e), 0 if e is None else e.size)
+        if rc:
+            raise ValueError("ora_scene_set_params: parameter counts differ from the scene")
 
     def trace(self, wrays, threads=1, chunk=65536, brute=False):
         wrays = np.ascontiguousarray(wrays, WRAY_DTYPE)

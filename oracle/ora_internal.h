@@ -29,6 +29,7 @@ static inline uint32_t ray_id(const ora_ray* r) { return r->id_and_flags & RAY_I
 
 typedef struct ora_object {
     int kind, ts;
+    float escale;     /* emission scale of a light object (1); not in the light tree */
     float* data;      /* ts * floats-per-kind */
     bbox* bounds;     /* ts */
     int n_bounds;

@@ -239,6 +239,7 @@ static void sphere_light_sample(const ora_object* ob, lquery* q, v3 arr, float* 
     float ccol[3];
     for (int k = 0; k < 3; ++k) ccol[k] = col[k] * (float)surface_area_inv;
     spec_from_color(ccol, q->spec);
+    for (int k = 0; k < NSPEC; ++k) q->spec[k] *= ob->escale;
     if (d > radius) {
         const double s2 = (radius * radius) / d2 < 1.0 ? (radius * radius) / d2 : 1.0;
         const double c2 = 1.0 - s2;
@@ -305,6 +306,7 @@ static void rect_light_sample(const ora_object* ob, lquery* q, v3 arr, float* pd
     float ccol[3];
     for (int k = 0; k < 3; ++k) ccol[k] = (col[k] * (float)inv_area) * 0.5f;
     spec_from_color(ccol, q->spec);
+    for (int k = 0; k < NSPEC; ++k) q->spec[k] *= ob->escale;
 }
 
 /* LightTree::sample (accel/light_tree.cpp:30-96) */

@@ -447,6 +447,7 @@ ora_scene* ora_scene_create(const psy_scene_desc* d) {
             const psy_object_desc* od = &ad->objects[i];
             ora_object* o = &a->obj[i];
             o->kind = od->kind;
+            o->escale = 1.0f;
             o->ts = od->time_samples;
             const int nf = ora_kind_floats(od->kind) * od->time_samples;
             o->data = (float*)malloc(sizeof(float) * (nf + 1));
@@ -509,4 +510,29 @@ void ora_scene_destroy(ora_scene* s) {
 int64_t ora_scene_bvh4_nodes(const ora_scene* s, int assembly) {
     if (!s || assembly < 0 || assembly >= s->n_asm) return -1;
     return (int64_t)s->asms[assembly].n_nodes;
+}
+
+/* Differentiable parameters (the product's psy_scene_set_params): grey
+ * albedo per shader (Lambert shaders only) and an emission scale per light
+ * object in scene order.  Light trees keep the energies of the scene as
+ * created. */
+int ora_scene_set_params(ora_scene* s, const float* albedo, int n_albedo, const float* escale, int n_escale) {
+    if (albedo) {
+        if (n_albedo != s->n_shaders) return -1;
+        for (int i = 0; i < n_albedo; ++i)
+            if (s->shaders[i].kind == PSY_SHADER_LAMBERT)
+                for (int k = 0; k < 3; ++k) s->shaders[i].color[k] = albedo[i];
+    }
+    if (escale) {
+        int j = 0;
+        for (int a = 0; a < s->n_asm; ++a)
+            for (int i = 0; i < s->asms[a].n_obj; ++i) {
+                ora_object* o = &s->asms[a].obj[i];
+                if (o->kind != PSY_OBJ_SPHERE_LIGHT && o->kind != PSY_OBJ_RECTANGLE_LIGHT) continue;
+                if (j >= n_escale) return -1;
+                o->escale = escale[j++];
+            }
+        if (j != n_escale) return -1;
+    }
+    return 0;
 }

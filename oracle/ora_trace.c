@@ -368,7 +368,7 @@ static int rect_light_intersect(const ora_object* ob, const ora_ray* ray, ora_in
         set3(in->offset, v3m(0.0f, 0.0f, 0.000001f));
         const double area = (double)(w * h);
         in->closure = PSY_SHADER_EMIT;
-        for (int k = 0; k < 3; ++k) in->closure_col[k] = (col[k] * 0.5f) / (float)area;
+        for (int k = 0; k < 3; ++k) in->closure_col[k] = ((col[k] * 0.5f) / (float)area) * ob->escale;
         return 1;
     }
     return 0;
@@ -411,7 +411,7 @@ static int sphere_light_intersect(const ora_object* ob, const ora_ray* ray, ora_
             else col[k] = (ob->data[i * 7 + 4 + k] * lerp_fac(na)) + (ob->data[(i + 1) * 7 + 4 + k] * na);
         }
         in->closure = PSY_SHADER_EMIT;
-        for (int k = 0; k < 3; ++k) in->closure_col[k] = col[k] / (float)area;
+        for (int k = 0; k < 3; ++k) in->closure_col[k] = (col[k] / (float)area) * ob->escale;
     }
     return 1;
 }

@@ -75,6 +75,9 @@ int ora_partition_ints(int* v, int n, int key, int* calls);
 
 /* scene */
 typedef struct ora_scene ora_scene;
+/* Grey albedo per shader (Lambert shaders) and emission scale per light
+ * object in scene order; the light trees keep the scene's energies. */
+int ora_scene_set_params(ora_scene* s, const float* albedo, int n_albedo, const float* escale, int n_escale);
 ora_scene* ora_scene_create(const psy_scene_desc* desc);
 void ora_scene_destroy(ora_scene* s);
 int64_t ora_scene_bvh4_nodes(const ora_scene* s, int assembly);

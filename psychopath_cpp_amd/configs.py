@@ -223,9 +223,16 @@ def bicubic_sphere_scene(seed=3, grid=64, jitter=0.02, n_outer=4, n_inner=4, mot
     return sc
 
 
-def many_light_scene(seed=1, n_lights=10000, **kw):
+def adjoint_scene(seed=5, n_spheres=256, n_lights=1000):
+    """The differentiable-render scene: n_spheres spheres and the ground as
+    in sphere_scene (one Lambert shader each, albedo U[0.2,0.8]), lit only
+    by n_lights small sphere lights as in many_light_scene."""
+    return many_light_scene(seed=seed, n_lights=n_lights, n_spheres=n_spheres, big_lights=0)
+
+
+def many_light_scene(seed=1, n_lights=10000, big_lights=2, **kw):
     """sphere_scene plus n_lights small sphere lights above the ground."""
-    sc = sphere_scene(seed=seed, **kw)
+    sc = sphere_scene(seed=seed, n_lights=big_lights, **kw)
     rng = JKiss(seed + 1000)
     root = sc.root
     for i in range(n_lights):

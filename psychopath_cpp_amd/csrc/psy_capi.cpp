@@ -691,6 +691,15 @@ int prepare_scene(const psy_scene_desc* d, int dice_depth, HostScene& H) {
         I.dice = objs[I.object].dice;
     }
 
+    // ---- emission scales of the light objects ------------------------------
+    H.escale.assign(objs.size(), 1.0f);
+    H.light_param.assign(objs.size(), -1);
+    for (size_t i = 0; i < objs.size(); ++i)
+        if (objs[i].kind == PSY_OBJ_SPHERE_LIGHT || objs[i].kind == PSY_OBJ_RECTANGLE_LIGHT) {
+            H.light_param[i] = (int32_t)H.light_objects.size();
+            H.light_objects.push_back((int32_t)i);
+        }
+
     // ---- shaders, camera (basics/camera.hpp:23-69) -------------------------
     H.shaders.resize(d->n_shaders);
     for (int i = 0; i < d->n_shaders; ++i) {
@@ -759,6 +768,11 @@ int psy_scene_create(psy_ctx* ctx, const psy_scene_desc* d, int dice_depth, psy_
     if ((rc = sc->odata.upload(H.odata))) return rc;
     if ((rc = sc->xf.upload(H.xf))) return rc;
     if ((rc = sc->shaders.upload(H.shaders))) return rc;
+    if ((rc = sc->escale_dev.upload(H.escale))) return rc;
+    if ((rc = sc->lparam_dev.upload(H.light_param))) return rc;
+    sc->host_shaders = H.shaders;
+    sc->light_objects = H.light_objects;
+    sc->escale = H.escale;
     if ((rc = upload_lights(H, sc.get()))) return rc;
     const size_t dice_bytes = (size_t)per * patch_ids.size() * 32;
     if ((rc = sc->dice.alloc(dice_bytes))) return rc;
@@ -781,6 +795,7 @@ int psy_scene_create(psy_ctx* ctx, const psy_scene_desc* d, int dice_depth, psy_
     s.xf = (const M44*)sc->xf.p;
     s.dice = (const float4*)sc->dice.p;
     s.shaders = (const DevShader*)sc->shaders.p;
+    s.escale = (const float*)sc->escale_dev.p;
     s.dice_rate = H.dice_rate;
     s.min_upoly = H.min_upoly;
     s.n_inst = (int32_t)nl;
@@ -796,6 +811,55 @@ int psy_scene_create(psy_ctx* ctx, const psy_scene_desc* d, int dice_depth, psy_
     info.device_bytes = (int64_t)(sc->bvh.bytes + sc->inst.bytes + sc->obj.bytes + sc->odata.bytes + sc->xf.bytes +
                                   sc->shaders.bytes + dice_bytes);
     *out = sc.release();
+    return PSY_OK;
+}
+
+int psy_scene_param_counts(const psy_scene* s, int64_t* n_albedo, int64_t* n_emission) {
+    if (!s) return fail(PSY_ERR_INVALID, "psy_scene_param_counts: NULL scene");
+    if (n_albedo) *n_albedo = (int64_t)s->host_shaders.size();
+    if (n_emission) *n_emission = (int64_t)s->light_objects.size();
+    return PSY_OK;
+}
+
+int psy_scene_get_params(const psy_scene* s, float* albedo, float* emission_scale) {
+    if (!s) return fail(PSY_ERR_INVALID, "psy_scene_get_params: NULL scene");
+    if (albedo)
+        for (size_t i = 0; i < s->host_shaders.size(); ++i) albedo[i] = s->host_shaders[i].color[0];
+    if (emission_scale)
+        for (size_t j = 0; j < s->light_objects.size(); ++j) emission_scale[j] = s->escale[s->light_objects[j]];
+    return PSY_OK;
+}
+
+int psy_scene_set_params(psy_scene* s, const float* albedo, int64_t n_albedo, const float* emission_scale,
+                         int64_t n_emission) {
+    if (!s) return fail(PSY_ERR_INVALID, "psy_scene_set_params: NULL scene");
+    if (albedo && n_albedo != (int64_t)s->host_shaders.size())
+        return fail(PSY_ERR_INVALID, "psy_scene_set_params: albedo count differs from the scene's shader count");
+    if (emission_scale && n_emission != (int64_t)s->light_objects.size())
+        return fail(PSY_ERR_INVALID, "psy_scene_set_params: emission count differs from the scene's light objects");
+    if (albedo)
+        for (int64_t i = 0; i < n_albedo; ++i)
+            if (!(albedo[i] >= 0.0f && albedo[i] < 3.402823466e+38f))
+                return fail(PSY_ERR_INVALID, "psy_scene_set_params: albedos must be finite and >= 0");
+    if (emission_scale)
+        for (int64_t j = 0; j < n_emission; ++j)
+            if (!(emission_scale[j] >= 0.0f && emission_scale[j] < 3.402823466e+38f))
+                return fail(PSY_ERR_INVALID, "psy_scene_set_params: emission scales must be finite and >= 0");
+    HIP_OK(hipSetDevice(s->ctx->device));
+    if (albedo) {
+        for (int64_t i = 0; i < n_albedo; ++i)
+            if (s->host_shaders[i].kind == PSY_SHADER_LAMBERT)
+                for (int k = 0; k < 3; ++k) s->host_shaders[i].color[k] = albedo[i];
+        if (!s->host_shaders.empty())
+            HIP_OK(hipMemcpy(s->shaders.p, s->host_shaders.data(), s->host_shaders.size() * sizeof(DevShader),
+                             hipMemcpyHostToDevice));
+    }
+    if (emission_scale) {
+        for (int64_t j = 0; j < n_emission; ++j) s->escale[s->light_objects[j]] = emission_scale[j];
+        if (!s->escale.empty())
+            HIP_OK(hipMemcpy(s->escale_dev.p, s->escale.data(), s->escale.size() * sizeof(float),
+                             hipMemcpyHostToDevice));
+    }
     return PSY_OK;
 }
 

@@ -79,6 +79,7 @@ struct DevScene {
     const M44* xf;
     const float4* dice;
     const DevShader* shaders;
+    const float* escale;      // per object: emission scale of a light object (1 by default)
     float dice_rate;
     float min_upoly;
     int32_t n_inst;
@@ -176,6 +177,28 @@ int launch_render_init(const RenderLaunch& R, int64_t first, int64_t n, const Re
 // Finished paths write their film XYZ to sample_xyz[3 * (sample - sbase)].
 int launch_render_update(const RenderLaunch& R, int64_t first, const RenderBuffers& B, int cur, int64_t n_in,
                          float* sample_xyz, int64_t sbase, hipStream_t st);
+// Adjoint pass buffers: Q planes (adjoint path state, `cap` entries each),
+// the gradient vector (n_params: albedos by shader, then emission scales by
+// light object), per-block partial rows when the sums fit in LDS.
+constexpr int ADJ_MAX_VERTS = 8;  // scattering vertices kept per path (path_length limit)
+constexpr int ADJ_STATE_PLANES = 4 + ADJ_MAX_VERTS;
+constexpr int ADJ_LDS_MAX_PARAMS = 16384;
+struct AdjointBuffers {
+    float* q;
+    const float* d_film;
+    int32_t n_albedo, n_params;
+    const int32_t* lparam;
+    float* grad;
+    float* partials;
+    int rows;
+    bool lds;
+};
+int launch_fill_f32(float* p, int64_t n, float v, hipStream_t st);
+int launch_adjoint_init(const RenderLaunch& R, const RenderBuffers& B, const AdjointBuffers& J, int64_t n,
+                        hipStream_t st);
+int launch_render_adjoint(const RenderLaunch& R, int64_t first, const RenderBuffers& B, const AdjointBuffers& J,
+                          int cur, int64_t n_in, hipStream_t st);
+int launch_adjoint_reduce(const AdjointBuffers& J, float* d_albedo, float* d_emission, hipStream_t st);
 int launch_render_resolve(const float* sample_xyz, int64_t sbase, int64_t first_pixel, int64_t n_pixels, int spp,
                           float* film, hipStream_t st);
 int launch_zero_u32(uint32_t* p, int n, hipStream_t st);

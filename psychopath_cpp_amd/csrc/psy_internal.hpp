@@ -119,6 +119,12 @@ struct psy_scene {
     psy::DevScene dev{};
     psy::DevCamera cam{};
     psy::DBuf bvh, inst, obj, odata, xf, dice, shaders;
+    // differentiable parameters: shader albedos (host copy of the table),
+    // emission scales of the light objects
+    std::vector<psy::DevShader> host_shaders;
+    std::vector<int32_t> light_objects;
+    std::vector<float> escale;
+    psy::DBuf escale_dev, lparam_dev;
     int stack_need = 1;
     bool transformed_patches = false;
     std::vector<uint64_t> ids;
@@ -145,6 +151,9 @@ struct HostScene {
     std::vector<float> odata;
     std::vector<M44> xf;
     std::vector<DevShader> shaders;
+    std::vector<float> escale;         // per object: emission scale (1)
+    std::vector<int32_t> light_param;  // per object: index among light objects, -1 else
+    std::vector<int32_t> light_objects;
     std::vector<int32_t> patch_ids;  // objects with a dice-cache slot, in slot order
     int levels = 0;                  // cached split depth
     int64_t per = 2;                 // nodes per slot (1-based heap, entry 0 unused)

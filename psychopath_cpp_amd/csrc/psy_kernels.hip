@@ -267,7 +267,23 @@ __global__ __launch_bounds__(RENDER_BLOCK) void k_render_init(RenderLaunch A, in
 
 // Paths that go on are appended to act_out / rays_out with one atomic per
 // wave (ballot + prefix count): the stream compaction that replaces the
-// reference's std::partition of the path array.
+// reference's std::partition of the path array.  Every lane of the wave
+// calls it.
+__device__ __forceinline__ void compact_push(bool alive, uint32_t i, const WRay& nr, uint32_t* __restrict__ act_out,
+                                             float* __restrict__ rays_out, int64_t rcap,
+                                             uint32_t* __restrict__ count_out) {
+    const unsigned long long m = __ballot(alive);
+    const uint32_t lane = __lane_id();
+    uint32_t base = 0;
+    if (lane == 0 && m) base = atomicAdd(count_out, (uint32_t)__popcll(m));
+    base = (uint32_t)__shfl((int)base, 0);
+    if (alive) {
+        const uint32_t k = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        act_out[k] = i;
+        store_wray(rays_out, rcap, k, nr.o, nr.d, nr.odx, nr.ody, nr.ddx, nr.ddy, nr.time, nr.type);
+    }
+}
+
 __global__ __launch_bounds__(RENDER_BLOCK) void k_render_update(RenderLaunch A, int64_t first, PathBuf P,
                                                                 const uint32_t* __restrict__ act_in,
                                                                 const float* __restrict__ rays_in, DevHits hits,
@@ -284,16 +300,82 @@ __global__ __launch_bounds__(RENDER_BLOCK) void k_render_update(RenderLaunch A, 
         i = act_in[j];
         alive = render_update_one(A, first, P, i, j, rays_in, hits, rcap, sample_xyz, sbase, nr);
     }
-    const unsigned long long m = __ballot(alive);
-    const uint32_t lane = __lane_id();
-    uint32_t base = 0;
-    if (lane == 0 && m) base = atomicAdd(count_out, (uint32_t)__popcll(m));
-    base = (uint32_t)__shfl((int)base, 0);
-    if (alive) {
-        const uint32_t k = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-        act_out[k] = i;
-        store_wray(rays_out, rcap, k, nr.o, nr.d, nr.odx, nr.ody, nr.ddx, nr.ddy, nr.time, nr.type);
+    compact_push(alive, i, nr, act_out, rays_out, rcap, count_out);
+}
+
+// ---- adjoint ----------------------------------------------------------------
+// Gradient accumulator of the adjoint pass: float atomics into a block's LDS
+// row of n_params sums, or straight into the global gradient vector.
+struct AdjAcc {
+    static constexpr bool on = true;
+    PathBuf Q;
+    int32_t n_albedo;
+    const int32_t* lparam;  // per object: light parameter index or -1
+    float* acc;
+    __device__ int light_param(int obj) const { return lparam[obj]; }
+    __device__ void add(int p, float v) const { atomicAdd(acc + p, v); }
+};
+
+__global__ __launch_bounds__(RENDER_BLOCK) void k_adjoint_init(RenderLaunch A, PathBuf P, PathBuf Q, int64_t n,
+                                                               const float* __restrict__ d_film) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) adjoint_init_one(A, P, Q, i, d_film);
+}
+
+// The shading step of the adjoint pass: update_path + next_ray_for_path for
+// every active path (the forward step, identical decisions) plus the
+// gradients of its contributions.  A fixed grid walks the stream; with LDS
+// sums each block adds its row into partials[blockIdx.x] at the end (rows are
+// owned by block index, and launches on one stream do not overlap).
+template <bool LDS>
+__global__ __launch_bounds__(RENDER_BLOCK) void k_render_adjoint(
+    RenderLaunch A, int64_t first, PathBuf P, PathBuf Q, const uint32_t* __restrict__ act_in,
+    const float* __restrict__ rays_in, DevHits hits, int64_t n_in, int64_t rcap, uint32_t* __restrict__ act_out,
+    float* __restrict__ rays_out, uint32_t* __restrict__ count_out, int32_t n_albedo, int32_t n_params,
+    const int32_t* __restrict__ lparam, float* __restrict__ grad, float* __restrict__ partials) {
+    extern __shared__ float sacc[];
+    AdjAcc acc{Q, n_albedo, lparam, LDS ? sacc : grad};
+    if (LDS) {
+        for (int p = threadIdx.x; p < n_params; p += blockDim.x) sacc[p] = 0.0f;
+        __syncthreads();
     }
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t j0 = (int64_t)blockIdx.x * blockDim.x; j0 < n_in; j0 += stride) {
+        const int64_t j = j0 + threadIdx.x;
+        bool alive = false;
+        uint32_t i = 0;
+        WRay nr;
+        if (j < n_in) {
+            i = act_in[j];
+            alive = render_update_one(A, first, P, i, j, rays_in, hits, rcap, nullptr, 0, nr, acc);
+        }
+        compact_push(alive, i, nr, act_out, rays_out, rcap, count_out);
+    }
+    if (LDS) {
+        __syncthreads();
+        float* row = partials + (int64_t)blockIdx.x * n_params;
+        for (int p = threadIdx.x; p < n_params; p += blockDim.x) row[p] += sacc[p];
+    }
+}
+
+// d_albedo[p] / d_emission[p - n_albedo] = grad[p] + the sum of the rows.
+__global__ void k_adjoint_reduce(const float* __restrict__ grad, const float* __restrict__ partials, int rows,
+                                 int n_albedo, int n_params, float* __restrict__ d_albedo,
+                                 float* __restrict__ d_emission) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n_params) return;
+    float v = grad[p];
+    for (int r = 0; r < rows; ++r) v += partials[(int64_t)r * n_params + p];
+    if (p < n_albedo) {
+        if (d_albedo) d_albedo[p] = v;
+    } else if (d_emission) {
+        d_emission[p - n_albedo] = v;
+    }
+}
+
+__global__ void k_fill_f32(float* p, int64_t n, float v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 // Fused path kernel (the batched form of the integrator): persistent waves
@@ -370,7 +452,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_render_persist(RenderLaunch A, 
                 hr.v = L.b.v;
                 hr.aux = L.b.aux;
                 WRay nr;
-                if (render_update_path(A, first, P, pi, w, hr, sample_xyz, sbase, nr)) {
+                if (render_update_path(A, first, P, pi, w, hr, sample_xyz, sbase, nr, NoAdjoint())) {
                     rtype = nr.type;
                     lane_init_wray(s, nr, pi, L);
                 } else {
@@ -540,6 +622,47 @@ int launch_render_update(const RenderLaunch& R, int64_t first, const RenderBuffe
                        dim3(RENDER_BLOCK), 0, st, R, first, P, (const uint32_t*)B.act[cur],
                        (const float*)B.rays[cur], B.hits, n_in, B.cap, B.act[nxt], B.rays[nxt], B.count + nxt,
                        sample_xyz, sbase);
+    return hipGetLastError() == hipSuccess ? 0 : PSY_ERR_HIP;
+}
+
+int launch_fill_f32(float* p, int64_t n, float v, hipStream_t st) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(n, 256)), dim3(256), 0, st, p, n, v);
+    return hipGetLastError() == hipSuccess ? 0 : PSY_ERR_HIP;
+}
+
+int launch_adjoint_init(const RenderLaunch& R, const RenderBuffers& B, const AdjointBuffers& J, int64_t n,
+                        hipStream_t st) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_adjoint_init, dim3((unsigned)((n + RENDER_BLOCK - 1) / RENDER_BLOCK)), dim3(RENDER_BLOCK), 0,
+                       st, R, PathBuf{B.paths, B.cap}, PathBuf{J.q, B.cap}, n, J.d_film);
+    return hipGetLastError() == hipSuccess ? 0 : PSY_ERR_HIP;
+}
+
+int launch_render_adjoint(const RenderLaunch& R, int64_t first, const RenderBuffers& B, const AdjointBuffers& J,
+                          int cur, int64_t n_in, hipStream_t st) {
+    if (n_in <= 0) return 0;
+    const int nxt = cur ^ 1;
+    const int64_t want = (n_in + RENDER_BLOCK - 1) / RENDER_BLOCK;
+    const unsigned grid = (unsigned)std::min<int64_t>(want, J.rows);
+    const PathBuf P{B.paths, B.cap}, Q{J.q, B.cap};
+    if (J.lds) {
+        hipLaunchKernelGGL(k_render_adjoint<true>, dim3(grid), dim3(RENDER_BLOCK), (size_t)J.n_params * sizeof(float),
+                           st, R, first, P, Q, (const uint32_t*)B.act[cur], (const float*)B.rays[cur], B.hits, n_in,
+                           B.cap, B.act[nxt], B.rays[nxt], B.count + nxt, J.n_albedo, J.n_params, J.lparam, J.grad,
+                           J.partials);
+    } else {
+        hipLaunchKernelGGL(k_render_adjoint<false>, dim3(grid), dim3(RENDER_BLOCK), 0, st, R, first, P, Q,
+                           (const uint32_t*)B.act[cur], (const float*)B.rays[cur], B.hits, n_in, B.cap, B.act[nxt],
+                           B.rays[nxt], B.count + nxt, J.n_albedo, J.n_params, J.lparam, J.grad, J.partials);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : PSY_ERR_HIP;
+}
+
+int launch_adjoint_reduce(const AdjointBuffers& J, float* d_albedo, float* d_emission, hipStream_t st) {
+    if (J.n_params <= 0) return 0;
+    hipLaunchKernelGGL(k_adjoint_reduce, dim3((unsigned)((J.n_params + 255) / 256)), dim3(256), 0, st, J.grad,
+                       J.partials, J.lds ? J.rows : 0, J.n_albedo, J.n_params, d_albedo, d_emission);
     return hipGetLastError() == hipSuccess ? 0 : PSY_ERR_HIP;
 }
 

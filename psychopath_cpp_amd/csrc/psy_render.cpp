@@ -367,46 +367,59 @@ int upload_lights(const HostScene& H, psy_scene* sc) {
 // ---------------------------------------------------------------------------
 // psy_render: PathTraceIntegrator::integrate over the whole image
 // ---------------------------------------------------------------------------
-extern "C" {
+namespace {
 
-int psy_render(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, float* film_xyz,
-               float* sample_xyz, int64_t* rays_traced, void* stream) {
-    if (!ctx || !s || !prm) return fail(PSY_ERR_INVALID, "psy_render: NULL argument");
+// Chunking and scratch layout of one render call: chunks of whole pixels,
+// at most ctx->render_chunk paths; `extra` floats after the render scratch.
+struct RenderSetup {
+    int64_t spp, n_pix, total, chunk, cap;
+    RenderBuffers B;
+    float* chunk_samples;
+    float* extra;
+};
+
+int check_render(const psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, const char* who) {
+    if (!ctx || !s || !prm) return fail(PSY_ERR_INVALID, std::string(who) + ": NULL argument");
     if (prm->width <= 0 || prm->height <= 0 || prm->spp <= 0)
-        return fail(PSY_ERR_INVALID, "psy_render: width, height and spp must be positive");
+        return fail(PSY_ERR_INVALID, std::string(who) + ": width, height and spp must be positive");
     if (prm->width > 32767 || prm->height > 32767)
-        return fail(PSY_ERR_INVALID, "psy_render: image sides are limited to 32767 pixels (short pixel coordinates)");
+        return fail(PSY_ERR_INVALID, std::string(who) + ": image sides are limited to 32767 pixels (short pixel coordinates)");
     if (prm->path_length < 1 || prm->path_length > 32)
-        return fail(PSY_ERR_INVALID, "psy_render: path_length must be in 1..32");
+        return fail(PSY_ERR_INVALID, std::string(who) + ": path_length must be in 1..32");
     if (s->chromatic)
         return fail(PSY_ERR_UNSUPPORTED,
-                    "psy_render: the scene has non-grey colours (or surfaces without a shader, which emit magenta); "
-                    "the device integrator upsamples grey colours only");
-    if (rays_traced) *rays_traced = 0;
-    HIP_OK(hipSetDevice(ctx->device));
-    hipStream_t st = (hipStream_t)stream;
+                    std::string(who) + ": the scene has non-grey colours (or surfaces without a shader, which emit "
+                    "magenta); the device integrator upsamples grey colours only");
+    return PSY_OK;
+}
 
-    const int64_t spp = prm->spp;
-    const int64_t n_pix = (int64_t)prm->width * prm->height;
-    const int64_t total = n_pix * spp;
-    // chunk: whole pixels, at most ctx->render_chunk paths (at least one pixel)
-    int64_t chunk_pix = std::max<int64_t>(1, ctx->render_chunk / spp);
-    chunk_pix = std::min(chunk_pix, n_pix);
-    const int64_t chunk = chunk_pix * spp;              // paths per chunk (whole pixels)
-    const int64_t cap = (chunk + 63) / 64 * 64;          // storage stride, keeps every array 256-byte aligned
+void plan_render(const psy_ctx* ctx, const psy_render_params* prm, RenderSetup& U) {
+    U.spp = prm->spp;
+    U.n_pix = (int64_t)prm->width * prm->height;
+    U.total = U.n_pix * U.spp;
+    int64_t chunk_pix = std::max<int64_t>(1, ctx->render_chunk / U.spp);
+    chunk_pix = std::min(chunk_pix, U.n_pix);
+    U.chunk = chunk_pix * U.spp;            // paths per chunk (whole pixels)
+    U.cap = (U.chunk + 63) / 64 * 64;       // storage stride, keeps every array 256-byte aligned
+}
+
+// After plan_render: the scratch, with `extra_floats` after the render's own.
+int alloc_render(psy_ctx* ctx, bool own_samples, int64_t extra_floats, RenderSetup& U) {
+    const int64_t cap = U.cap;
     if (cap > (int64_t)INT32_MAX) return fail(PSY_ERR_UNSUPPORTED, "psy_render: more than 2^31 paths per chunk");
-
-    // scratch layout (floats): paths, 2 ray streams, hits, active lists, counters, chunk samples
+    // scratch (floats): paths, 2 ray streams, hits, active lists, counters,
+    // chunk samples, extra
     const int64_t f_paths = (int64_t)PATH_PLANES * cap;
     const int64_t f_rays = 20 * cap;
     const int64_t f_hits = 5 * cap;
     const int64_t f_act = cap;
-    const int64_t f_samp = sample_xyz ? 0 : 3 * cap;
-    const int64_t f_total = f_paths + 2 * f_rays + f_hits + 2 * f_act + 64 + f_samp;
+    const int64_t f_samp = own_samples ? 3 * cap : 0;
+    const int64_t f_extra = (extra_floats + 63) / 64 * 64;
+    const int64_t f_total = f_paths + 2 * f_rays + f_hits + 2 * f_act + 64 + f_samp + f_extra;
     if (int rc = ctx->render_scratch.reserve((size_t)f_total * sizeof(float))) return rc;
     if (int rc = ensure_queue(ctx, cap)) return rc;
     float* base = (float*)ctx->render_scratch.p;
-    RenderBuffers B;
+    RenderBuffers& B = U.B;
     B.cap = cap;
     B.paths = base;
     B.rays[0] = base + f_paths;
@@ -416,8 +429,12 @@ int psy_render(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, f
     B.act[0] = (uint32_t*)(h + f_hits);
     B.act[1] = B.act[0] + cap;
     B.count = B.act[1] + cap;
-    float* chunk_samples = (float*)(B.count + 64);
+    U.chunk_samples = (float*)(B.count + 64);
+    U.extra = U.chunk_samples + f_samp;
+    return PSY_OK;
+}
 
+RenderLaunch make_launch(const psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm) {
     RenderLaunch R;
     R.s = s->dev;
     R.L = s->lights;
@@ -430,76 +447,35 @@ int psy_render(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, f
     R.path_length = prm->path_length;
     R.seed = prm->seed;
     for (int k = 0; k < 3; ++k) R.bg[k] = s->background[k];
+    return R;
+}
 
+// The wavefront loop (integrator/path_trace_integrator.cpp:321-349) over
+// every chunk: init, then per step trace the active stream, shade it and
+// compact the paths that go on.  With `adj`, the shading step is the
+// adjoint pass's (same paths, plus gradients) and no samples are written.
+int run_wavefront(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, const RenderLaunch& R,
+                  const RenderSetup& U, float* film_xyz, float* sample_xyz, const AdjointBuffers* adj,
+                  int64_t* traced_out, hipStream_t st) {
+    const RenderBuffers& B = U.B;
     int64_t traced = 0;
-    if (ctx->render_mode == PSY_RENDER_FUSED) {
-        const int bpc = ctx->render_blocks_per_cu > 0 ? ctx->render_blocks_per_cu : std::max(1, ctx->render_occupancy);
-        const int blocks = std::max(1, ctx->n_cu * bpc);
-        uint32_t* counters = B.count + 8;  // u32 x 8 (8-byte aligned)
-        unsigned long long* steps = (unsigned long long*)(B.count + 16);
-        HIP_OK(hipMemsetAsync(steps, 0, sizeof(unsigned long long), st));
-        for (int64_t first = 0; first < total; first += chunk) {
-            const int64_t n = std::min(chunk, total - first);
-            float* samp = sample_xyz ? sample_xyz : chunk_samples;
-            const int64_t sbase = sample_xyz ? 0 : first;
-            PersistBuffers PB;
-            PB.paths = B.paths;
-            PB.counters = counters;
-            PB.defer_ids = B.act[0];
-            PB.defer_rays = B.rays[0];
-            PB.resume_ids = B.act[1];
-            PB.resume_rays = B.rays[1];
-            PB.sample_xyz = samp;
-            PB.sbase = sbase;
-            PB.cap = cap;
-            uint32_t n_resume = 0;
-            for (int round = 0;; ++round) {
-                if (round > 4 * prm->path_length + 4)
-                    return fail(PSY_ERR_HIP, "psy_render: fused rounds did not converge");
-                int rc = launch_set_counters(counters, n_resume, st);
-                if (!rc) rc = launch_render_persist(R, first, round == 0 ? n : 0, PB, blocks, ctx->render_refill, st);
-                if (rc) return fail(rc, "psy_render: fused launch failed");
-                uint32_t n_defer = 0;
-                HIP_OK(hipMemcpyAsync(&n_defer, counters + 2, sizeof n_defer, hipMemcpyDeviceToHost, st));
-                HIP_OK(hipStreamSynchronize(st));
-                if (n_defer == 0) break;
-                // the parked rays through the two-pass tracer, then shading
-                rc = launch_trace(s->dev, true, false, s->stack_need, PB.defer_rays, n_defer, cap, B.hits,
-                                  (uint32_t*)ctx->queue.p, (uint32_t*)ctx->counters.p, ctx->tuning(true), st);
-                if (!rc) rc = launch_zero_u32(counters + 4, 1, st);
-                if (!rc)
-                    rc = launch_render_update_io(R, first, B.paths, cap, PB.defer_ids, PB.defer_rays, B.hits, n_defer,
-                                                 PB.resume_ids, PB.resume_rays, counters + 4, samp, sbase, st);
-                if (rc) return fail(rc, "psy_render: second-pass launch failed");
-                HIP_OK(hipMemcpyAsync(&n_resume, counters + 4, sizeof n_resume, hipMemcpyDeviceToHost, st));
-                HIP_OK(hipStreamSynchronize(st));
-                if (n_resume == 0) break;
-            }
-            int rc = launch_render_count_steps(B.paths, cap, n, steps, st);
-            if (!rc && film_xyz) rc = launch_render_resolve(samp, sbase, first / spp, n / spp, (int)spp, film_xyz, st);
-            if (rc) return fail(rc, "psy_render: launch failed");
-        }
-        unsigned long long total_steps = 0;
-        HIP_OK(hipMemcpyAsync(&total_steps, steps, sizeof total_steps, hipMemcpyDeviceToHost, st));
-        HIP_OK(hipStreamSynchronize(st));
-        if (rays_traced) *rays_traced = (int64_t)total_steps;
-        return PSY_OK;
-    }
-    for (int64_t first = 0; first < total; first += chunk) {
-        const int64_t n = std::min(chunk, total - first);
-        float* samp = sample_xyz ? sample_xyz : chunk_samples;
+    for (int64_t first = 0; first < U.total; first += U.chunk) {
+        const int64_t n = std::min(U.chunk, U.total - first);
+        float* samp = sample_xyz ? sample_xyz : U.chunk_samples;
         const int64_t sbase = sample_xyz ? 0 : first;
         int rc = launch_render_init(R, first, n, B, st);
+        if (!rc && adj) rc = launch_adjoint_init(R, B, *adj, n, st);
         if (rc) return fail(rc, "psy_render: init launch failed");
         int64_t n_active = n;
         int cur = 0;
         for (int step = 0; step < 2 * prm->path_length && n_active > 0; ++step) {
             traced += n_active;
-            rc = launch_trace(s->dev, true, false, s->stack_need, B.rays[cur], n_active, cap, B.hits,
+            rc = launch_trace(s->dev, true, false, s->stack_need, B.rays[cur], n_active, B.cap, B.hits,
                               (uint32_t*)ctx->queue.p, (uint32_t*)ctx->counters.p, ctx->tuning(true), st);
             if (rc) return fail(rc, "psy_render: trace launch failed");
             if ((rc = launch_zero_u32(B.count + (cur ^ 1), 1, st))) return fail(rc, "psy_render: launch failed");
-            rc = launch_render_update(R, first, B, cur, n_active, samp, sbase, st);
+            rc = adj ? launch_render_adjoint(R, first, B, *adj, cur, n_active, st)
+                     : launch_render_update(R, first, B, cur, n_active, samp, sbase, st);
             if (rc) return fail(rc, "psy_render: update launch failed");
             uint32_t next = 0;
             HIP_OK(hipMemcpyAsync(&next, B.count + (cur ^ 1), sizeof next, hipMemcpyDeviceToHost, st));
@@ -507,12 +483,127 @@ int psy_render(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, f
             n_active = next;
             cur ^= 1;
         }
-        if (film_xyz) {
-            rc = launch_render_resolve(samp, sbase, first / spp, n / spp, (int)spp, film_xyz, st);
+        if (film_xyz && !adj) {
+            rc = launch_render_resolve(samp, sbase, first / U.spp, n / U.spp, (int)U.spp, film_xyz, st);
             if (rc) return fail(rc, "psy_render: resolve launch failed");
         }
     }
-    if (rays_traced) *rays_traced = traced;
+    if (traced_out) *traced_out = traced;
+    return PSY_OK;
+}
+
+// The fused persistent form (PSY_RENDER_FUSED).
+int run_fused(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, const RenderLaunch& R,
+              const RenderSetup& U, float* film_xyz, float* sample_xyz, int64_t* rays_traced, hipStream_t st) {
+    const RenderBuffers& B = U.B;
+    const int64_t cap = U.cap, spp = U.spp;
+    const int bpc = ctx->render_blocks_per_cu > 0 ? ctx->render_blocks_per_cu : std::max(1, ctx->render_occupancy);
+    const int blocks = std::max(1, ctx->n_cu * bpc);
+    uint32_t* counters = B.count + 8;  // u32 x 8 (8-byte aligned)
+    unsigned long long* steps = (unsigned long long*)(B.count + 16);
+    HIP_OK(hipMemsetAsync(steps, 0, sizeof(unsigned long long), st));
+    for (int64_t first = 0; first < U.total; first += U.chunk) {
+        const int64_t n = std::min(U.chunk, U.total - first);
+        float* samp = sample_xyz ? sample_xyz : U.chunk_samples;
+        const int64_t sbase = sample_xyz ? 0 : first;
+        PersistBuffers PB;
+        PB.paths = B.paths;
+        PB.counters = counters;
+        PB.defer_ids = B.act[0];
+        PB.defer_rays = B.rays[0];
+        PB.resume_ids = B.act[1];
+        PB.resume_rays = B.rays[1];
+        PB.sample_xyz = samp;
+        PB.sbase = sbase;
+        PB.cap = cap;
+        uint32_t n_resume = 0;
+        for (int round = 0;; ++round) {
+            if (round > 4 * prm->path_length + 4) return fail(PSY_ERR_HIP, "psy_render: fused rounds did not converge");
+            int rc = launch_set_counters(counters, n_resume, st);
+            if (!rc) rc = launch_render_persist(R, first, round == 0 ? n : 0, PB, blocks, ctx->render_refill, st);
+            if (rc) return fail(rc, "psy_render: fused launch failed");
+            uint32_t n_defer = 0;
+            HIP_OK(hipMemcpyAsync(&n_defer, counters + 2, sizeof n_defer, hipMemcpyDeviceToHost, st));
+            HIP_OK(hipStreamSynchronize(st));
+            if (n_defer == 0) break;
+            // the parked rays through the two-pass tracer, then shading
+            rc = launch_trace(s->dev, true, false, s->stack_need, PB.defer_rays, n_defer, cap, B.hits,
+                              (uint32_t*)ctx->queue.p, (uint32_t*)ctx->counters.p, ctx->tuning(true), st);
+            if (!rc) rc = launch_zero_u32(counters + 4, 1, st);
+            if (!rc)
+                rc = launch_render_update_io(R, first, B.paths, cap, PB.defer_ids, PB.defer_rays, B.hits, n_defer,
+                                             PB.resume_ids, PB.resume_rays, counters + 4, samp, sbase, st);
+            if (rc) return fail(rc, "psy_render: second-pass launch failed");
+            HIP_OK(hipMemcpyAsync(&n_resume, counters + 4, sizeof n_resume, hipMemcpyDeviceToHost, st));
+            HIP_OK(hipStreamSynchronize(st));
+            if (n_resume == 0) break;
+        }
+        int rc = launch_render_count_steps(B.paths, cap, n, steps, st);
+        if (!rc && film_xyz) rc = launch_render_resolve(samp, sbase, first / spp, n / spp, (int)spp, film_xyz, st);
+        if (rc) return fail(rc, "psy_render: launch failed");
+    }
+    unsigned long long total_steps = 0;
+    HIP_OK(hipMemcpyAsync(&total_steps, steps, sizeof total_steps, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    if (rays_traced) *rays_traced = (int64_t)total_steps;
+    return PSY_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int psy_render(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, float* film_xyz,
+               float* sample_xyz, int64_t* rays_traced, void* stream) {
+    if (int rc = check_render(ctx, s, prm, "psy_render")) return rc;
+    if (rays_traced) *rays_traced = 0;
+    HIP_OK(hipSetDevice(ctx->device));
+    hipStream_t st = (hipStream_t)stream;
+    RenderSetup U;
+    plan_render(ctx, prm, U);
+    if (int rc = alloc_render(ctx, sample_xyz == nullptr, 0, U)) return rc;
+    const RenderLaunch R = make_launch(ctx, s, prm);
+    if (ctx->render_mode == PSY_RENDER_FUSED) return run_fused(ctx, s, prm, R, U, film_xyz, sample_xyz, rays_traced, st);
+    return run_wavefront(ctx, s, prm, R, U, film_xyz, sample_xyz, nullptr, rays_traced, st);
+}
+
+int psy_adjoint(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, const float* d_film_xyz,
+                float* d_albedo, float* d_emission, int64_t* rays_traced, void* stream) {
+    if (int rc = check_render(ctx, s, prm, "psy_adjoint")) return rc;
+    if (!d_film_xyz) return fail(PSY_ERR_INVALID, "psy_adjoint: NULL film gradient");
+    if (prm->path_length > ADJ_MAX_VERTS)
+        return fail(PSY_ERR_UNSUPPORTED, "psy_adjoint: path_length above 8 (scattering vertices kept per path)");
+    const int64_t n_albedo = (int64_t)s->host_shaders.size();
+    const int64_t n_light = (int64_t)s->light_objects.size();
+    if ((n_albedo > 0 && !d_albedo) || (n_light > 0 && !d_emission))
+        return fail(PSY_ERR_INVALID, "psy_adjoint: NULL gradient output");
+    const int64_t n_params = n_albedo + n_light;
+    if (n_params > (int64_t)INT32_MAX / 2) return fail(PSY_ERR_UNSUPPORTED, "psy_adjoint: too many parameters");
+    if (rays_traced) *rays_traced = 0;
+    HIP_OK(hipSetDevice(ctx->device));
+    hipStream_t st = (hipStream_t)stream;
+    AdjointBuffers J{};
+    J.n_albedo = (int32_t)n_albedo;
+    J.n_params = (int32_t)n_params;
+    J.lds = n_params <= ADJ_LDS_MAX_PARAMS;
+    J.rows = std::max(1, 2 * ctx->n_cu);
+    J.d_film = d_film_xyz;
+    J.lparam = (const int32_t*)s->lparam_dev.p;
+    // extra scratch: adjoint path state, gradient vector, partial rows
+    RenderSetup U;
+    plan_render(ctx, prm, U);
+    const int64_t f_q = (int64_t)ADJ_STATE_PLANES * U.cap;
+    const int64_t f_grad = (n_params + 63) / 64 * 64;
+    const int64_t f_part = J.lds ? (int64_t)J.rows * n_params : 0;
+    if (int rc = alloc_render(ctx, true, f_q + f_grad + f_part, U)) return rc;
+    J.q = U.extra;
+    J.grad = J.q + f_q;
+    J.partials = J.grad + f_grad;
+    int rc = launch_fill_f32(J.grad, f_grad + f_part, 0.0f, st);
+    if (rc) return fail(rc, "psy_adjoint: launch failed");
+    const RenderLaunch R = make_launch(ctx, s, prm);
+    if ((rc = run_wavefront(ctx, s, prm, R, U, nullptr, nullptr, &J, rays_traced, st))) return rc;
+    if ((rc = launch_adjoint_reduce(J, d_albedo, d_emission, st))) return fail(rc, "psy_adjoint: launch failed");
     return PSY_OK;
 }
 

@@ -284,7 +284,10 @@ struct LQuery {
     float time;
     M44 xform;
     V3 to_light;
-    float spec;
+    float spec;       // the light's emission at the sample, times its emission scale
+    float spec_base;  // the emission without the scale
+    float scale;      // that emission scale
+    int32_t obj;      // the sampled light object
     float selection_pdf, light_sample_pdf;
 };
 
@@ -442,6 +445,10 @@ PSY_DEV_FN bool light_tree_sample(const DevScene& s, const DevLights& L, LQuery&
             else if (ob.kind == K_RECT_LIGHT) rect_light_sample(od, ob.ts, q, &p);
             else return false;
             q.to_light = m_dir(m_inverse(q.xform), q.to_light);
+            q.obj = A.obj0 + in.data_index;
+            q.scale = s.escale[q.obj];
+            q.spec_base = q.spec;
+            q.spec *= q.scale;
             q.selection_pdf *= (tot_prob * (float)A.total_lights);
             q.light_sample_pdf = p;
             return true;
@@ -535,7 +542,9 @@ PSY_DEV void render_init_one(const RenderLaunch& A, int64_t first, int64_t i, co
 
 // Emission and light pdf of a camera / bounce ray that ended on a light
 // object (SphereLight / RectangleLight::intersect_ray closures).
-PSY_DEV_FN void light_hit(const DevScene& s, uint32_t li, V3 wo, float time, float* spec, float* light_pdf) {
+// *spec is the emission times the object's emission scale, *base without it.
+PSY_DEV_FN void light_hit(const DevScene& s, uint32_t li, V3 wo, float time, float* spec, float* base,
+                          float* light_pdf) {
     const DevInstance I = s.inst[li];
     const DevObject ob = s.obj[I.object];
     V3 o = wo;
@@ -546,23 +555,93 @@ PSY_DEV_FN void light_hit(const DevScene& s, uint32_t li, V3 wo, float time, flo
         rect_light_at(od, ob.ts, time, &w, &h, col);
         *light_pdf = rect_light_pdf(w, h, o);
         const double area = (double)(w * h);
-        *spec = color_spec((col[0] * 0.5f) / (float)area, (col[1] * 0.5f) / (float)area,
+        *base = color_spec((col[0] * 0.5f) / (float)area, (col[1] * 0.5f) / (float)area,
                            (col[2] * 0.5f) / (float)area);
     } else {
         V3 c; float r, col[3];
         sphere_light_at(od, ob.ts, time, &c, &r, col);
         const double area = 4.0 * PSY_PI_D * r * r;
         *light_pdf = sphere_light_pdf(c, (double)r, o);
-        *spec = color_spec(col[0] / (float)area, col[1] / (float)area, col[2] / (float)area);
+        *base = color_spec(col[0] / (float)area, col[1] / (float)area, col[2] / (float)area);
     }
+    *spec = *base * s.escale[I.object];
 }
 
+
+// ---- adjoint (path replay) ---------------------------------------------------
+// The adjoint pass re-runs every path from its sample index with the same
+// sampler, scene and parameters, so it reproduces the forward paths exactly
+// without storing them.  Each radiance contribution C of a path is the
+// product of a factor B that no parameter enters, the grey albedos of the
+// path's scattering vertices a_v and, for light emission, the light's
+// emission scale e:  C = B * prod(a_v) * e.  With w = dLoss/dC of the path
+// (its pixel's film gradient through Color_XYZ), the pass adds
+//   dLoss/de   += w * B * prod(a_v)
+//   dLoss/da_v += w * B * e * prod(a_u, u != v)   for every vertex v.
+// Light selection does not depend on the parameters (the light trees keep
+// the energies of the scene as created), so every C is a polynomial in them.
+enum : int {
+    Q_W = 0,       // w: dLoss/dC for this path
+    Q_B = 1,       // B of the throughput (albedos taken as 1)
+    Q_LB = 2,      // B of the pending shadow ray's light sample
+    Q_LOBJ = 3,    // its light object (i32), -1 for none
+    Q_CHAIN = 4,   // shader of scattering vertex 0, 1, ... (u32)
+    Q_COUNT = Q_CHAIN + ADJ_MAX_VERTS
+};
+static_assert(Q_COUNT == ADJ_STATE_PLANES, "adjoint state layout");
+
+// The forward integrator: no adjoint state.
+struct NoAdjoint {
+    static constexpr bool on = false;
+    PathBuf Q{nullptr, 0};
+    int32_t n_albedo = 0;
+    PSY_DEV int light_param(int) const { return -1; }
+    PSY_DEV void add(int, float) const {}
+};
+
+// Gradients of one contribution w*B (`wb`) with `nverts` scattering vertices
+// and light object `light_obj` (-1: no scaled emitter).
+template <class Adj>
+PSY_DEV void adj_contrib(const DevScene& s, const Adj& adj, uint32_t i, int nverts, float wb, int32_t light_obj) {
+    if (wb == 0.0f) return;
+    float a[ADJ_MAX_VERTS];
+    uint32_t sh[ADJ_MAX_VERTS];
+    float all = 1.0f;
+#pragma unroll
+    for (int v = 0; v < ADJ_MAX_VERTS; ++v) {
+        sh[v] = 0u;
+        a[v] = 1.0f;
+        if (v < nverts) {
+            sh[v] = adj.Q.u(Q_CHAIN + v, i);
+            a[v] = s.shaders[sh[v]].color[0];
+            all *= a[v];
+        }
+    }
+    float e = 1.0f;
+    const int lp = light_obj >= 0 ? adj.light_param(light_obj) : -1;
+    if (lp >= 0) {
+        e = s.escale[light_obj];
+        adj.add(adj.n_albedo + lp, wb * all);
+    }
+#pragma unroll
+    for (int v = 0; v < ADJ_MAX_VERTS; ++v) {
+        if (v < nverts) {
+            float ex = e;
+#pragma unroll
+            for (int u = 0; u < ADJ_MAX_VERTS; ++u)
+                if (u != v && u < nverts) ex *= a[u];
+            adj.add((int)sh[v], wb * ex);
+        }
+    }
+}
 
 // update_path for the ray just traced (slot j of the stream) of path i, then
 // next_ray_for_path if the path goes on (returns true, the ray in `nr`);
 // a finished path writes its film contribution to sample_xyz.
+template <class Adj>
 PSY_DEV bool render_update_path(const RenderLaunch& A, int64_t first, const PathBuf& P, uint32_t i, const WRay& ray,
-                                const HitRec& hr, float* sample_xyz, int64_t sbase, WRay& nr) {
+                                const HitRec& hr, float* sample_xyz, int64_t sbase, WRay& nr, const Adj& adj) {
+    const float one[3] = {1.0f, 1.0f, 1.0f};
     bool alive = false;
     V3 no = splat(0.0f), nd = splat(0.0f), nodx = splat(0.0f), nody = splat(0.0f), nddx = splat(0.0f),
        nddy = splat(0.0f);
@@ -585,32 +664,45 @@ PSY_DEV bool render_update_path(const RenderLaunch& A, int64_t first, const Path
                 const float* alb = s.shaders[sh].color;
                 const float fac = lambert_eval(alb, P.v3(P_PD, i), rd, P.v3(P_GN, i));
                 for (int k = 0; k < NSPEC; ++k) col[k] += fcol[k] * P.f(P_LCOL + k, i) * fac;
+                if constexpr (Adj::on) {
+                    const Adj& Z = adj;
+                    const float fac1 = lambert_eval(one, P.v3(P_PD, i), rd, P.v3(P_GN, i));
+                    const float wb = Z.Q.f(Q_W, i) * ((Z.Q.f(Q_B, i) * Z.Q.f(Q_LB, i)) * fac1);
+                    adj_contrib(s, Z, i, (step >> 1) + 1, wb, (int32_t)Z.Q.u(Q_LOBJ, i));
+                }
             }
         } else if (hit) {
             const DevInstance I = s.inst[hinst];
             const V3 ro = ray.o;
             const float time = ray.time;
             bool emit = false;
-            float em = 0.0f, light_pdf = 9999.0f;
+            float em = 0.0f, em_base = 0.0f, light_pdf = 9999.0f;
+            int32_t em_obj = -1;
             if (I.kind == K_SPHERE_LIGHT || I.kind == K_RECT_LIGHT) {
                 emit = true;
-                light_hit(s, hinst, ro, time, &em, &light_pdf);
+                light_hit(s, hinst, ro, time, &em, &em_base, &light_pdf);
+                em_obj = I.object;
             } else if (I.shader >= 0 && s.shaders[I.shader].kind == PSY_SHADER_EMIT) {
                 emit = true;
                 const float* c = s.shaders[I.shader].color;
                 em = color_spec(c[0], c[1], c[2]);
+                em_base = em;
             }
             if (emit) {
                 done = true;
                 const float last_pdf = P.f(P_LASTPDF, i);
+                float wb = 0.0f;
                 if (last_pdf != 0.0f) {
                     const float a2 = last_pdf * last_pdf;
                     const float b2 = light_pdf * light_pdf;
                     const float mis_inv_pdf = (a2 / (a2 + b2)) / last_pdf;
                     for (int k = 0; k < NSPEC; ++k) col[k] += ((fcol[k] * last_pdf) * em) * mis_inv_pdf;
+                    if constexpr (Adj::on) wb = ((adj.Q.f(Q_B, i) * last_pdf) * em_base) * mis_inv_pdf;
                 } else {
                     for (int k = 0; k < NSPEC; ++k) col[k] += fcol[k] * em;
+                    if constexpr (Adj::on) wb = adj.Q.f(Q_B, i) * em_base;
                 }
+                if constexpr (Adj::on) adj_contrib(s, adj, i, step >> 1, adj.Q.f(Q_W, i) * wb, em_obj);
             } else {
                 // store the hit in world space (DifferentialGeometry::transformed_from)
                 M44 space;
@@ -630,6 +722,7 @@ PSY_DEV bool render_update_path(const RenderLaunch& A, int64_t first, const Path
                 P.set3(P_POFF, i, po);
                 P.f(P_T, i) = t;
                 P.setu(P_SHADER, i, (uint32_t)I.shader);
+                if constexpr (Adj::on) adj.Q.setu(Q_CHAIN + (step >> 1), i, (uint32_t)I.shader);
                 P.set3(P_PD, i, rd);
                 P.set3(P_PODX, i, ray.odx);
                 P.set3(P_PODY, i, ray.ody);
@@ -641,6 +734,7 @@ PSY_DEV bool render_update_path(const RenderLaunch& A, int64_t first, const Path
             done = true;
             const float bg = color_spec(A.bg[0], A.bg[1], A.bg[2]);
             for (int k = 0; k < NSPEC; ++k) col[k] += fcol[k] * bg;
+            if constexpr (Adj::on) adj_contrib(s, adj, i, step >> 1, adj.Q.f(Q_W, i) * (adj.Q.f(Q_B, i) * bg), -1);
         }
         const int nstep = step + 1;
         if (nstep == A.path_length * 2) done = true;
@@ -668,19 +762,29 @@ PSY_DEV bool render_update_path(const RenderLaunch& A, int64_t first, const Path
                 q.xform = m_identity();
                 q.to_light = splat(0.0f);
                 q.spec = 0.0f;
+                q.spec_base = 0.0f;
+                q.scale = 1.0f;
+                q.obj = -1;
                 q.selection_pdf = 1.0f;
                 q.light_sample_pdf = 1.0f;
-                float lcol = 0.0f;
+                float lcol = 0.0f, lbase = 0.0f;
                 if (!light_tree_sample(s, A.L, q)) {
                     q.to_light = mk(0.0f, 0.0f, 1.0f);
+                    q.obj = -1;
                 } else {
                     const float bsdf_pdf = lambert_pdf(pd, q.to_light, gn);
                     const float a2 = q.light_sample_pdf * q.light_sample_pdf;
                     const float b2 = bsdf_pdf * bsdf_pdf;
                     const float mis_inv_pdf = (a2 / (a2 + b2)) / q.light_sample_pdf;
                     lcol = ((q.spec * mis_inv_pdf) * (float)A.L.asms[0].total_lights) / q.selection_pdf;
+                    if constexpr (Adj::on)
+                        lbase = ((q.spec_base * mis_inv_pdf) * (float)A.L.asms[0].total_lights) / q.selection_pdf;
                 }
                 for (int k = 0; k < NSPEC; ++k) P.f(P_LCOL + k, i) = lcol;
+                if constexpr (Adj::on) {
+                    adj.Q.f(Q_LB, i) = lbase;
+                    adj.Q.setu(Q_LOBJ, i, (uint32_t)q.obj);
+                }
                 no = add(gp, po);
                 nd = scale(sub(q.to_light, po), 0.9995f);
                 ntype = PSY_RAY_OCCLUSION;
@@ -703,6 +807,7 @@ PSY_DEV bool render_update_path(const RenderLaunch& A, int64_t first, const Path
                 const float filt = lambert_eval(alb, pd, nd, gn);
                 if (pdf < 0.00001f) pdf = 0.00001f;
                 for (int k = 0; k < NSPEC; ++k) fcol[k] *= filt / pdf;
+                if constexpr (Adj::on) adj.Q.f(Q_B, i) *= lambert_eval(one, pd, nd, gn) / pdf;
                 P.f(P_LASTPDF, i) = pdf;
             }
             // LambertClosure::propagate_differentials
@@ -766,9 +871,10 @@ PSY_DEV void lane_wray(const Lane& L, uint32_t type, WRay& w) {
 }
 
 // render_update_path for slot j of a ray / hit stream.
+template <class Adj = NoAdjoint>
 PSY_DEV bool render_update_one(const RenderLaunch& A, int64_t first, const PathBuf& P, uint32_t i, int64_t j,
                                const float* rays_in, const DevHits& hits, int64_t rcap, float* sample_xyz,
-                               int64_t sbase, WRay& nr) {
+                               int64_t sbase, WRay& nr, const Adj& adj = Adj()) {
     WRay ray;
     load_wray(rays_in, rcap, j, ray);
     HitRec hr;
@@ -777,7 +883,26 @@ PSY_DEV bool render_update_one(const RenderLaunch& A, int64_t first, const PathB
     hr.u = hits.u[j];
     hr.v = hits.v[j];
     hr.aux = hits.aux ? hits.aux[j] : 0.0f;
-    return render_update_path(A, first, P, i, ray, hr, sample_xyz, sbase, nr);
+    return render_update_path(A, first, P, i, ray, hr, sample_xyz, sbase, nr, adj);
+}
+
+// The adjoint state of a new path: w = dLoss/dC from its pixel's film
+// gradient (film = mean over the pixel's spp samples of Color_XYZ(col) *
+// XYZ_NORM_FAC, and a contribution C adds to every spectral sample of col),
+// B = 1, no pending light sample.
+PSY_DEV void adjoint_init_one(const RenderLaunch& A, const PathBuf& P, const PathBuf& Q, int64_t i,
+                              const float* d_film) {
+    const uint32_t pix = P.u(P_PIX, i);
+    const int64_t p = (int64_t)(pix >> 16) * A.width + (int64_t)(pix & 0xffffu);
+    const float ones[NSPEC] = {1.0f, 1.0f, 1.0f, 1.0f};
+    float xyz[3];
+    path_film_xyz(P.f(P_WL, i), ones, xyz);
+    const float w = ((d_film[3 * p] * xyz[0]) + (d_film[3 * p + 1] * xyz[1]) + (d_film[3 * p + 2] * xyz[2])) /
+                    (float)A.spp;
+    Q.f(Q_W, i) = w;
+    Q.f(Q_B, i) = 1.0f;
+    Q.f(Q_LB, i) = 0.0f;
+    Q.setu(Q_LOBJ, i, 0xffffffffu);
 }
 
 // Film::add_sample over a pixel's samples, in sample order: the mean of the

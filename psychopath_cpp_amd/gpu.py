@@ -55,6 +55,10 @@ def lib():
     L.psy_camera_rays.argtypes = [vp, vp, i32, i32, i32, u32, vp, vp, vp]
     L.psy_bounce_rays.argtypes = [vp, vp, vp, C.POINTER(abi.Hits), i64, i32, i32, u32, u32, u32, vp, vp, vp]
     L.psy_render.argtypes = [vp, vp, C.POINTER(abi.RenderParams), vp, vp, C.POINTER(i64), vp]
+    L.psy_scene_param_counts.argtypes = [vp, C.POINTER(i64), C.POINTER(i64)]
+    L.psy_scene_get_params.argtypes = [vp, vp, vp]
+    L.psy_scene_set_params.argtypes = [vp, vp, i64, vp, i64]
+    L.psy_adjoint.argtypes = [vp, vp, C.POINTER(abi.RenderParams), vp, vp, vp, C.POINTER(i64), vp]
     if L.psy_abi_version() != 1:
         raise PsyError("libpsy_gpu.so ABI version mismatch")
     _lib = L
@@ -131,6 +135,32 @@ class GpuScene:
         info = abi.SceneInfo()
         check(lib().psy_scene_get_info(h, C.byref(info)), "psy_scene_get_info")
         self.info = info
+
+    def param_counts(self):
+        """psy_scene_param_counts: (shader albedos, light emission scales)."""
+        a, e = C.c_int64(), C.c_int64()
+        check(lib().psy_scene_param_counts(self.handle, C.byref(a), C.byref(e)), "psy_scene_param_counts")
+        return a.value, e.value
+
+    def get_params(self):
+        """psy_scene_get_params: (albedo, emission_scale) float32 arrays."""
+        na, ne = self.param_counts()
+        a = np.zeros(max(1, na), np.float32)
+        e = np.zeros(max(1, ne), np.float32)
+        check(lib().psy_scene_get_params(self.handle, a.ctypes.data_as(C.c_void_p), e.ctypes.data_as(C.c_void_p)),
+              "psy_scene_get_params")
+        return a[:na], e[:ne]
+
+    def set_params(self, albedo=None, emission_scale=None):
+        """psy_scene_set_params from host arrays (None keeps a set)."""
+        a = None if albedo is None else np.ascontiguousarray(
+            albedo.detach().cpu().numpy() if torch.is_tensor(albedo) else albedo, np.float32)
+        e = None if emission_scale is None else np.ascontiguousarray(
+            emission_scale.detach().cpu().numpy() if torch.is_tensor(emission_scale) else emission_scale, np.float32)
+        check(lib().psy_scene_set_params(self.handle, None if a is None else a.ctypes.data_as(C.c_void_p),
+                                         0 if a is None else a.size,
+                                         None if e is None else e.ctypes.data_as(C.c_void_p),
+                                         0 if e is None else e.size), "psy_scene_set_params")
 
     def instance_ids(self):
         n = int(self.info.n_leaf_instances)
@@ -274,6 +304,23 @@ def render(ctx, scene, width, height, spp, seed=1, path_length=4, film=True, sam
     check(lib().psy_render(ctx.handle, scene.handle, C.byref(prm), _ptr(f), _ptr(s), C.byref(rays), _stream()),
           "psy_render")
     return f, s, rays.value
+
+
+def adjoint(ctx, scene, d_film, spp, seed=1, path_length=4):
+    """psy_adjoint: (d_albedo, d_emission, rays_traced) for the film gradient
+    d_film, a (height, width, 3) CUDA float32 tensor."""
+    if not (d_film.is_cuda and d_film.dtype == torch.float32 and d_film.dim() == 3 and d_film.shape[2] == 3):
+        raise PsyError("d_film must be a (height, width, 3) CUDA float32 tensor")
+    d_film = d_film.contiguous()
+    height, width = int(d_film.shape[0]), int(d_film.shape[1])
+    na, ne = scene.param_counts()
+    ga = torch.zeros(na, dtype=torch.float32, device=d_film.device)
+    ge = torch.zeros(ne, dtype=torch.float32, device=d_film.device)
+    prm = abi.RenderParams(width, height, spp, seed, path_length, 0)
+    rays = C.c_int64()
+    check(lib().psy_adjoint(ctx.handle, scene.handle, C.byref(prm), _ptr(d_film), _ptr(ga) if na else None,
+                            _ptr(ge) if ne else None, C.byref(rays), _stream()), "psy_adjoint")
+    return ga, ge, rays.value
 
 
 def set_render_chunk(ctx, paths):

@@ -24,7 +24,9 @@ def build():
     cmd = ([pb.hipcc(), "-x", "hip", "--offload-host-only", "-shared"] + flags +
            [SRC, "-o", LIB + ".tmp", "-L" + os.path.dirname(prod), "-lpsy_gpu",
             "-Wl,-rpath," + os.path.dirname(prod), "-lpthread"])
-    subprocess.run(cmd, check=True, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE)
+    r = subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("host build failed:\n" + r.stderr[-4000:])
     os.replace(LIB + ".tmp", LIB)
     return LIB
 
@@ -43,6 +45,8 @@ def lib():
                            C.POINTER(i64), C.c_int]
     L.hb_render.argtypes = [C.POINTER(abi.SceneDesc), C.c_int, C.POINTER(abi.RenderParams), vp, C.POINTER(i64),
                             C.c_int]
+    L.hb_render_params.argtypes = [C.POINTER(abi.SceneDesc), C.c_int, C.POINTER(abi.RenderParams), vp, C.c_int, vp,
+                                   C.c_int, vp, vp, vp, C.POINTER(i64), C.c_int]
     _lib = L
     return L
 
@@ -80,3 +84,41 @@ def render(scene_py, width, height, spp, seed=1, path_length=4, dice_depth=10, t
     if rc != 0:
         raise RuntimeError(f"hb_render failed with status {rc}")
     return out, rays.value
+
+
+def render_params(scene_py, width, height, spp, albedo=None, escale=None, d_film=None, seed=1, path_length=4,
+                  dice_depth=10, threads=8):
+    """psy_render (d_film None: per-sample film values) or psy_adjoint
+    (d_film (height, width, 3): the gradient vector, albedos then emission
+    scales) with the given parameters, as the kernels compute them."""
+    from psychopath_cpp_amd import abi
+    desc = scene_py.to_desc()
+    prm = abi.RenderParams(width, height, spp, seed, path_length, 0)
+    a = None if albedo is None else np.ascontiguousarray(albedo, np.float32)
+    e = None if escale is None else np.ascontiguousarray(escale, np.float32)
+    n_params = param_counts(desc)
+    out = np.zeros((height, width, spp, 3), np.float32) if d_film is None else None
+    df = None if d_film is None else np.ascontiguousarray(d_film, np.float32)
+    grad = np.zeros(max(1, sum(n_params)), np.float32)
+    rays = C.c_int64()
+    rc = lib().hb_render_params(desc.ptr(), dice_depth, C.byref(prm), None if a is None else _p(a),
+                                0 if a is None else a.size, None if e is None else _p(e), 0 if e is None else e.size,
+                                None if out is None else _p(out), None if df is None else _p(df), _p(grad),
+                                C.byref(rays), threads)
+    if rc != 0:
+        raise RuntimeError(f"hb_render_params failed with status {rc}")
+    return (out if d_film is None else grad[:sum(n_params)]), rays.value
+
+
+def param_counts(desc):
+    """(shaders, light objects) of a scene description, in the order of
+    psy_scene_param_counts."""
+    from psychopath_cpp_amd import abi
+    d = desc.desc if hasattr(desc, "desc") else desc.ptr().contents
+    n_light = 0
+    for a in range(d.n_assemblies):
+        ad = d.assemblies[a]
+        for i in range(ad.n_objects):
+            if ad.objects[i].kind in (abi.PSY_OBJ_SPHERE_LIGHT, abi.PSY_OBJ_RECTANGLE_LIGHT):
+                n_light += 1
+    return d.n_shaders, n_light

@@ -54,6 +54,7 @@ int build(const psy_scene_desc* d, int dice_depth, HostBuilt& B) {
     s.xf = H.xf.data();
     s.dice = B.dice.data();
     s.shaders = H.shaders.data();
+    s.escale = H.escale.data();
     s.dice_rate = H.dice_rate;
     s.min_upoly = H.min_upoly;
     s.n_inst = (int32_t)H.insts.size();
@@ -124,13 +125,44 @@ int hb_trace(const psy_scene_desc* d, int dice_depth, int world, const float* ra
     return PSY_OK;
 }
 
-// psy_render on the CPU: one chunk holding every path, the wavefront loop
-// with in-order compaction.  sample_xyz: width*height*spp*3 floats.
-int hb_render(const psy_scene_desc* d, int dice_depth, const psy_render_params* prm, float* sample_xyz,
-              int64_t* rays_traced, int threads) {
+}  // extern "C"
+
+namespace {
+// The adjoint accumulator of the kernels, with plain adds (the update step
+// runs on one thread here).
+struct HostAdj {
+    static constexpr bool on = true;
+    PathBuf Q;
+    int32_t n_albedo;
+    const int32_t* lparam;
+    float* acc;
+    PSY_DEV int light_param(int obj) const { return lparam[obj]; }
+    PSY_DEV void add(int p, float v) const { acc[p] += v; }
+};
+}  // namespace
+
+extern "C" {
+
+// psy_render (d_film == NULL) or psy_adjoint on the CPU: one chunk holding
+// every path, the wavefront loop with in-order compaction.  albedo /
+// escale: psy_scene_set_params values or NULL.  sample_xyz:
+// width*height*spp*3 floats (render); grad: n_albedo + n_lights (adjoint).
+int hb_render_params(const psy_scene_desc* d, int dice_depth, const psy_render_params* prm, const float* albedo,
+                     int n_albedo, const float* escale, int n_escale, float* sample_xyz, const float* d_film,
+                     float* grad, int64_t* rays_traced, int threads) {
     HostBuilt B;
     if (int rc = build(d, dice_depth, B)) return rc;
     if (B.H.chromatic) return PSY_ERR_UNSUPPORTED;
+    if (albedo) {
+        if (n_albedo != (int)B.H.shaders.size()) return PSY_ERR_INVALID;
+        for (int i = 0; i < n_albedo; ++i)
+            if (B.H.shaders[i].kind == PSY_SHADER_LAMBERT)
+                for (int k = 0; k < 3; ++k) B.H.shaders[i].color[k] = albedo[i];
+    }
+    if (escale) {
+        if (n_escale != (int)B.H.light_objects.size()) return PSY_ERR_INVALID;
+        for (int j = 0; j < n_escale; ++j) B.H.escale[B.H.light_objects[j]] = escale[j];
+    }
     const int64_t cap = (int64_t)prm->width * prm->height * prm->spp;
     std::vector<float> paths((size_t)(PATH_PLANES * cap));
     std::vector<float> rays[2] = {std::vector<float>((size_t)(20 * cap)), std::vector<float>((size_t)(20 * cap))};
@@ -154,8 +186,15 @@ int hb_render(const psy_scene_desc* d, int dice_depth, const psy_render_params* 
     R.path_length = prm->path_length;
     R.seed = prm->seed;
     for (int k = 0; k < 3; ++k) R.bg[k] = B.H.background[k];
+    std::vector<float> qplanes(d_film ? (size_t)(ADJ_STATE_PLANES * cap) : 0);
+    const int n_shaders = (int)B.H.shaders.size();
+    HostAdj adj{PathBuf{qplanes.data(), cap}, n_shaders, B.H.light_param.data(), grad};
+    if (d_film)
+        for (size_t p = 0; p < (size_t)n_shaders + B.H.light_objects.size(); ++p) grad[p] = 0.0f;
     const PathBuf P{paths.data(), cap};
     for (int64_t i = 0; i < cap; ++i) render_init_one(R, 0, i, P, rays[0].data(), cap, act[0].data());
+    if (d_film)
+        for (int64_t i = 0; i < cap; ++i) adjoint_init_one(R, P, adj.Q, i, d_film);
     int64_t n_active = cap, traced = 0;
     int cur = 0;
     std::vector<WRay> nrs((size_t)cap);
@@ -164,9 +203,14 @@ int hb_render(const psy_scene_desc* d, int dice_depth, const psy_render_params* 
         traced += n_active;
         const float* rin = rays[cur].data();
         parallel_for(n_active, threads, [&](int64_t j) { trace_ray<true>(B.s, rin, cap, j, 0u, h); });
-        parallel_for(n_active, threads, [&](int64_t j) {
-            alive[j] = render_update_one(R, 0, P, act[cur][j], j, rin, h, cap, sample_xyz, 0, nrs[j]) ? 1 : 0;
-        });
+        if (d_film) {
+            for (int64_t j = 0; j < n_active; ++j)
+                alive[j] = render_update_one(R, 0, P, act[cur][j], j, rin, h, cap, nullptr, 0, nrs[j], adj) ? 1 : 0;
+        } else {
+            parallel_for(n_active, threads, [&](int64_t j) {
+                alive[j] = render_update_one(R, 0, P, act[cur][j], j, rin, h, cap, sample_xyz, 0, nrs[j]) ? 1 : 0;
+            });
+        }
         const int nxt = cur ^ 1;
         int64_t k = 0;
         for (int64_t j = 0; j < n_active; ++j) {
@@ -181,6 +225,12 @@ int hb_render(const psy_scene_desc* d, int dice_depth, const psy_render_params* 
     }
     if (rays_traced) *rays_traced = traced;
     return PSY_OK;
+}
+
+int hb_render(const psy_scene_desc* d, int dice_depth, const psy_render_params* prm, float* sample_xyz,
+              int64_t* rays_traced, int threads) {
+    return hb_render_params(d, dice_depth, prm, nullptr, 0, nullptr, 0, sample_xyz, nullptr, nullptr, rays_traced,
+                            threads);
 }
 
 }  // extern "C"

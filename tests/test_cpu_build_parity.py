@@ -72,3 +72,33 @@ def test_cpu_build_render_matches_oracle(name):
     assert np.isfinite(hs).all() and hs.mean() > 0.0
     assert close >= 0.95 and worst <= 3.0
     assert abs(hrays - orays) <= 0.01 * orays
+
+
+def test_cpu_build_adjoint_matches_finite_differences():
+    """The adjoint pass (psy_render_dev.hpp's path replay, compiled for the
+    host) against central finite differences of the oracle's renderer and of
+    the same code's forward render, every parameter of a small scene with
+    12 sphere lights."""
+    from tests import _adjoint as J
+    sc = configs.adjoint_scene(n_spheres=16, n_lights=12)
+    n_sh, n_l = hostbuild.param_counts(sc.to_desc())
+    W, H, S = 32, 24, 4
+    alb0 = np.array([c[0] for _, c in sc.shaders], np.float32)
+    alb, esc = J.perturbed(alb0, n_l, seed=0)
+    o = ora.OracleScene(sc)
+
+    def film_oracle(a, e):
+        o.set_params(a, e)
+        return o.render(W, H, S, 1, threads=8)[0].astype(np.float64).mean(axis=2)
+
+    def film_hb(a, e):
+        return hostbuild.render_params(sc, W, H, S, albedo=a, escale=e)[0].astype(np.float64).mean(axis=2)
+
+    target = film_oracle(alb0, np.ones(n_l, np.float32))
+    grad, _ = hostbuild.render_params(sc, W, H, S, albedo=alb, escale=esc, d_film=J.d_loss(film_hb(alb, esc), target))
+    params = list(range(n_sh + n_l))
+    J.check(grad, J.central_fd(film_oracle, alb, esc, target, params), "adjoint vs oracle FD")
+    # against its own forward code the agreement is far tighter than the bar
+    fd_hb = J.central_fd(film_hb, alb, esc, target, params)
+    sig = np.abs(fd_hb) >= 1e-3 * np.abs(fd_hb).max()
+    assert (np.abs(grad - fd_hb)[sig] / np.abs(fd_hb)[sig]).max() <= 1e-3

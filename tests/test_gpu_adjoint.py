@@ -1,0 +1,151 @@
+"""The path-replay adjoint (psy_adjoint) through the C ABI and the torch
+autograd wrapper: gradients of an L2 image loss with respect to every
+shader albedo and light emission scale, against central finite differences
+(h = 1e-3, common random numbers) of the oracle's renderer and of the
+device's own forward render.  Bars in tests/_adjoint.py (1e-2 relative)."""
+import time
+
+import numpy as np
+import pytest
+import torch
+
+from psychopath_cpp_amd import configs, diff, gpu
+from tests import _adjoint as J
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def _albedo0(sc):
+    return np.array([c[0] for _, c in sc.shaders], np.float32)
+
+
+def test_gpu_adjoint_matches_oracle_fd(gpu_ctx, oracle):
+    """Small scene (16 spheres, 12 sphere lights): every parameter."""
+    sc = configs.adjoint_scene(n_spheres=16, n_lights=12)
+    g = gpu.GpuScene(gpu_ctx, sc)
+    n_sh, n_l = g.param_counts()
+    assert n_sh == len(sc.shaders) and n_l == 12
+    a_get, e_get = g.get_params()
+    assert np.array_equal(a_get, _albedo0(sc)) and np.array_equal(e_get, np.ones(12, np.float32))
+    W, H, S = 32, 24, 4
+    alb, esc = J.perturbed(_albedo0(sc), n_l, seed=0)
+    o = oracle.OracleScene(sc)
+
+    def film_oracle(a, e):
+        o.set_params(a, e)
+        return o.render(W, H, S, 1, threads=8)[0].astype(np.float64).mean(axis=2)
+
+    target = film_oracle(_albedo0(sc), np.ones(n_l, np.float32))
+    g.set_params(alb, esc)
+    film, _, _ = gpu.render(gpu_ctx, g, W, H, S, 1)
+    d = torch.from_numpy(J.d_loss(film.cpu().numpy(), target)).cuda()
+    ga, ge, rays = gpu.adjoint(gpu_ctx, g, d, S, 1)
+    grad = np.concatenate([ga.cpu().numpy(), ge.cpu().numpy()])
+    say("adjoint rays", rays)
+    fd = J.central_fd(film_oracle, alb, esc, target, list(range(n_sh + n_l)))
+    J.check(grad, fd, "GPU adjoint vs oracle FD (32x24x4)", say)
+
+
+def _c4(gpu_ctx, W, H, S):
+    sc = configs.adjoint_scene()
+    g = gpu.GpuScene(gpu_ctx, sc)
+    n_sh, n_l = g.param_counts()
+    assert (n_sh, n_l) == (257, 1000)
+    a0 = _albedo0(sc)
+    target, _, _ = gpu.render(gpu_ctx, g, W, H, S, 1)
+    alb, esc = J.perturbed(a0, n_l, seed=1)
+    g.set_params(alb, esc)
+    return sc, g, a0, alb, esc, target.cpu().numpy()
+
+
+def test_gpu_adjoint_c4_matches_fd(gpu_ctx, oracle):
+    """256 spheres + ground, 1000 sphere lights, target at the true
+    parameters, parameters perturbed x U[0.5,1.5]; 32 random parameters.
+    Full size (512x512, 16 spp) against the device's own forward render;
+    128x128x4 against the oracle's."""
+    sc, g, a0, alb, esc, target = _c4(gpu_ctx, 512, 512, 16)
+    n_sh, n_l = len(alb), len(esc)
+    rng = np.random.default_rng(2)
+    params = sorted(rng.choice(n_sh + n_l, 32, replace=False).tolist())
+
+    def film_gpu(a, e, W=512, H=512, S=16):
+        g.set_params(a, e)
+        f, _, _ = gpu.render(gpu_ctx, g, W, H, S, 1)
+        return f.cpu().numpy()
+
+    d = torch.from_numpy(J.d_loss(film_gpu(alb, esc), target)).cuda()
+    g.set_params(alb, esc)
+    ga, ge, _ = gpu.adjoint(gpu_ctx, g, d, 16, 1)
+    grad = np.concatenate([ga.cpu().numpy(), ge.cpu().numpy()])
+    fd = J.central_fd(film_gpu, alb, esc, target, params)
+    J.check(grad[params], fd, "C4 512x512x16 adjoint vs device FD", say)
+
+    # reduced size against the oracle's renderer
+    W, H, S = 128, 128, 4
+    o = oracle.OracleScene(sc)
+
+    def film_oracle(a, e):
+        o.set_params(a, e)
+        return o.render(W, H, S, 1, threads=16)[0].astype(np.float64).mean(axis=2)
+
+    target_s = film_oracle(a0, np.ones(n_l, np.float32))
+    film_s = film_gpu(alb, esc, W, H, S)
+    g.set_params(alb, esc)
+    ga, ge, _ = gpu.adjoint(gpu_ctx, g, torch.from_numpy(J.d_loss(film_s, target_s)).cuda(), S, 1)
+    grad_s = np.concatenate([ga.cpu().numpy(), ge.cpu().numpy()])
+    fd_s = J.central_fd(film_oracle, alb, esc, target_s, params[:16])
+    J.check(grad_s[params[:16]], fd_s, "C4 128x128x4 adjoint vs oracle FD", say)
+
+
+def test_gpu_autograd_function_and_time(gpu_ctx):
+    """diff.render_image: backward equals psy_adjoint; the adjoint pass takes
+    at most 3x the forward render's time at 512x512x16."""
+    sc, g, a0, alb, esc, target = _c4(gpu_ctx, 512, 512, 16)
+    a = torch.tensor(alb, requires_grad=True)
+    e = torch.tensor(esc, requires_grad=True)
+    tgt = torch.from_numpy(target).cuda()
+    film = diff.render_image(gpu_ctx, g, a, e, 512, 512, 16)
+    loss = ((film.double() - tgt.double()) ** 2).sum()
+    loss.backward()
+    d = (2.0 * (film.detach().double() - tgt.double())).float()
+    g.set_params(alb, esc)
+    ga, ge, _ = gpu.adjoint(gpu_ctx, g, d, 16, 1)
+    # float atomics sum the contributions in a run-dependent order: the two
+    # calls agree to rounding, not bit for bit
+    for x, y in ((a.grad, ga.cpu()), (e.grad, ge.cpu())):
+        diff_max = float((x - y).abs().max())
+        say(f"autograd vs direct adjoint: max difference {diff_max:.3e} (largest gradient {float(y.abs().max()):.3e})")
+        torch.testing.assert_close(x, y, rtol=1e-3, atol=1e-5 * float(y.abs().max()))
+    # time: forward render vs adjoint pass, same parameters and stream
+    for _ in range(2):
+        gpu.render(gpu_ctx, g, 512, 512, 16, 1)
+        gpu.adjoint(gpu_ctx, g, d, 16, 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        gpu.render(gpu_ctx, g, 512, 512, 16, 1)
+    torch.cuda.synchronize()
+    tf = (time.perf_counter() - t0) / 3
+    t0 = time.perf_counter()
+    for _ in range(3):
+        gpu.adjoint(gpu_ctx, g, d, 16, 1)
+    torch.cuda.synchronize()
+    ta = (time.perf_counter() - t0) / 3
+    say(f"C4 512x512x16: forward {tf * 1e3:.2f} ms, adjoint {ta * 1e3:.2f} ms, ratio {ta / tf:.2f}")
+    assert ta <= 3.0 * tf
+
+
+def test_gpu_adjoint_rejects_bad_input(gpu_ctx):
+    sc = configs.adjoint_scene(n_spheres=4, n_lights=3)
+    g = gpu.GpuScene(gpu_ctx, sc)
+    d = torch.zeros((8, 8, 3), dtype=torch.float32, device="cuda")
+    with pytest.raises(gpu.PsyError, match="path_length"):
+        gpu.adjoint(gpu_ctx, g, d, 1, 1, path_length=9)
+    with pytest.raises(gpu.PsyError, match="count"):
+        g.set_params(np.ones(3, np.float32), None)
+    with pytest.raises(gpu.PsyError, match="finite"):
+        g.set_params(None, np.array([1.0, -1.0, 1.0], np.float32))

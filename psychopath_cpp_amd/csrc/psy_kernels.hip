@@ -133,7 +133,13 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_deep(DevScene s, const fl
     __shared__ float stk_near[STACK][TRACE_BLOCK];
     const int tid = threadIdx.x;
     const uint32_t count = counters[0];
-    for (uint32_t k = blockIdx.x * TRACE_BLOCK + tid; k < count; k += gridDim.x * TRACE_BLOCK)
+    // Queue entry k goes to wave k % n_waves, lane k / n_waves: a short queue
+    // puts one ray in each wave, so rays that take different paths through
+    // the split trees run side by side instead of serializing in one wave.
+    const uint32_t n_waves = gridDim.x * (TRACE_BLOCK / 64);
+    const uint32_t wave = blockIdx.x * (TRACE_BLOCK / 64) + (uint32_t)(tid >> 6);
+    const uint32_t lane = (uint32_t)(tid & 63);
+    for (uint64_t k = (uint64_t)wave + (uint64_t)lane * n_waves; k < count; k += 64ull * n_waves)
         (void)trace_one<WORLD, STACK, true>(s, rays, rs, (int64_t)queue[k], h, flags, stk_ref, stk_near, tid);
 }
 

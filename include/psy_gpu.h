@@ -162,6 +162,11 @@ int psy_ctx_destroy(psy_ctx* ctx);
 #define PSY_TRACE_MODE_PLAIN 0
 #define PSY_TRACE_MODE_PERSISTENT 1
 int psy_ctx_set_trace_mode(psy_ctx* ctx, int mode, int refill_min, int blocks_per_cu);
+/* Main-pass traversal stack in LDS (no counterpart in the reference, whose
+ * BitStack and node stack live in CPU memory): 16 entries (default, 5 waves
+ * per SIMD) or 32 (2 waves per SIMD; fewer rays handed to the second pass
+ * on deep hierarchies).  Results are identical. */
+int psy_ctx_set_trace_stack(psy_ctx* ctx, int entries);
 /* Paths per wavefront chunk of psy_render (no counterpart in the reference,
  * whose buckets are sqrt(65536/spp) pixels square): whole pixels, at least
  * one; the default 2^21 paths.  Results do not depend on it. */
@@ -176,7 +181,18 @@ int psy_ctx_set_render_chunk(psy_ctx* ctx, int64_t paths);
  * <= 0 takes the kernel's occupancy.  Both give the same samples. */
 #define PSY_RENDER_WAVEFRONT 0
 #define PSY_RENDER_FUSED 1
+/* PSY_RENDER_STREAM: the batched form.  A pool of (render chunk) path
+ * slots works through the image's samples; the shading step gives a
+ * finished path's slot to the next sample, so one compacted stream mixes
+ * paths of many image tiles at every bounce depth and stays full until the
+ * samples run out.  Stream lengths stay on the device (the trace reads
+ * them there) and psy_ctx_set_stream_steps steps are captured as one HIP
+ * graph, with one host synchronization per graph launch.  refill_min and
+ * blocks_per_cu are ignored.  Same samples as the other forms. */
+#define PSY_RENDER_STREAM 2
 int psy_ctx_set_render_mode(psy_ctx* ctx, int mode, int refill_min, int blocks_per_cu);
+/* Steps per captured graph of PSY_RENDER_STREAM: even, 2..64 (default 4). */
+int psy_ctx_set_stream_steps(psy_ctx* ctx, int steps_per_graph);
 /* Rays of the latest trace call on this context that the main pass handed
  * to the second pass (split below the dice cache, or a BVH stack deeper
  * than the main pass keeps).  Synchronizes the device; for diagnostics. */
@@ -250,6 +266,26 @@ typedef struct psy_render_params {
  * `stream` once per wavefront step. */
 int psy_render(psy_ctx* ctx, const psy_scene* scene, const psy_render_params* params,
                float* film_xyz, float* sample_xyz, int64_t* rays_traced, void* stream);
+
+/* The render autotuner (no counterpart in the reference): times psy_render
+ * in PSY_RENDER_STREAM form on every pixel at an spp that keeps to
+ * max_samples samples (<= 0: 2^23), by coordinate descent over the pool
+ * size (render chunk, 2^19..2^23), the main pass's launch form
+ * (psy_ctx_set_trace_mode), its LDS stack (psy_ctx_set_trace_stack) and
+ * the steps per graph (psy_ctx_set_stream_steps), and leaves the fastest
+ * setting on the context.  `out` (or NULL) receives it. */
+typedef struct psy_render_tuning {
+    int32_t mode;             /* PSY_RENDER_STREAM                       */
+    int32_t trace_mode;       /* PSY_TRACE_MODE_*                        */
+    int32_t refill_min;
+    int32_t stack;            /* main-pass stack entries                 */
+    int32_t steps_per_graph;
+    int32_t trials;           /* renders timed                           */
+    int64_t pool;             /* paths in flight                         */
+    double mrays_per_s;       /* of the best trial                       */
+} psy_render_tuning;
+int psy_render_autotune(psy_ctx* ctx, const psy_scene* scene, const psy_render_params* params,
+                        int64_t max_samples, psy_render_tuning* out);
 
 /* ---- differentiation ---------------------------------------------------- */
 /* Differentiable scene parameters (no counterpart in the reference, which

@@ -587,7 +587,11 @@ static int patch_pred(void* vctx, ora_ray* ray) {
         hit = bbox_intersect(&lb, ray->o, ray->d_inv, ray->max_t, &h0, &h1);
     }
     if (!hit) return 1;
-    const float width = smax(ora_ray_min_width(ray, h0, h1) * c->tr->scene->dice_rate, c->tr->scene->min_upoly);
+    /* std::max(min_width * dice_rate, min_upoly) (object/patch_utils.hpp:77);
+     * a NaN footprint, for which the reference splits until its boxes stop
+     * being hit (which need not end), takes min_upoly here */
+    const float mw = ora_ray_min_width(ray, h0, h1) * c->tr->scene->dice_rate;
+    const float width = mw > c->tr->scene->min_upoly ? mw : c->tr->scene->min_upoly;
     if (c->max_dim <= width || c->stack_i == (SPLIT_STACK_SIZE - 1)) {
         const float tt = (h0 + h1) * 0.5f;
         if (tt > 0.0f && tt < ray->max_t) {

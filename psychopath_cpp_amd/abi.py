@@ -33,6 +33,7 @@ PSY_TRACE_MODE_PLAIN = 0
 PSY_TRACE_MODE_PERSISTENT = 1
 PSY_RENDER_WAVEFRONT = 0
 PSY_RENDER_FUSED = 1
+PSY_RENDER_STREAM = 2
 
 FLOATS_PER_KIND = {
     PSY_OBJ_SPHERE: 4,
@@ -89,6 +90,12 @@ class RenderParams(C.Structure):
                 ("path_length", C.c_int32), ("reserved", C.c_int32)]
 
 
+class RenderTuning(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("trace_mode", C.c_int32), ("refill_min", C.c_int32), ("stack", C.c_int32),
+                ("steps_per_graph", C.c_int32), ("trials", C.c_int32), ("pool", C.c_int64),
+                ("mrays_per_s", C.c_double)]
+
+
 class SceneInfo(C.Structure):
     _fields_ = [("n_leaf_instances", C.c_int64), ("n_objects", C.c_int64), ("n_bvh_nodes", C.c_int64),
                 ("n_patches", C.c_int64), ("dice_depth", C.c_int32), ("bvh_stack_need", C.c_int32),
@@ -97,7 +104,7 @@ class SceneInfo(C.Structure):
 
 # Every symbol the C ABI exports (include/psy_gpu.h).
 EXPORTED_SYMBOLS = [
-    "psy_ctx_create", "psy_ctx_destroy", "psy_ctx_set_trace_mode", "psy_ctx_retrace_count", "psy_ctx_set_render_chunk", "psy_ctx_set_render_mode", "psy_ctx_set_trace_timing", "psy_ctx_trace_timing", "psy_last_error", "psy_abi_version",
+    "psy_ctx_create", "psy_ctx_destroy", "psy_ctx_set_trace_mode", "psy_ctx_retrace_count", "psy_ctx_set_render_chunk", "psy_ctx_set_render_mode", "psy_ctx_set_trace_timing", "psy_ctx_set_trace_stack", "psy_ctx_set_stream_steps", "psy_render_autotune", "psy_ctx_trace_timing", "psy_last_error", "psy_abi_version",
     "psy_scene_create", "psy_scene_destroy", "psy_scene_get_info", "psy_scene_instance_ids",
     "psy_trace_world", "psy_trace_rays", "psy_world_to_rays",
     "psy_halton", "psy_sobol_set_matrices", "psy_sobol", "psy_pixel_samples", "psy_camera_rays", "psy_bounce_rays", "psy_render",

@@ -133,7 +133,8 @@ int ensure_queue(psy_ctx* ctx, int64_t n) {
     HIP_OK(hipDeviceSynchronize());
     ctx->queue.release();
     const int64_t cap = std::max<int64_t>(n, 1 << 16);
-    int rc = ctx->queue.alloc((size_t)cap * sizeof(uint32_t));
+    // two queues of cap rays: the second pass's, then the third pass's
+    int rc = ctx->queue.alloc((size_t)(2 * cap) * sizeof(uint32_t));
     if (rc) { ctx->queue_cap = 0; return rc; }
     ctx->queue_cap = cap;
     return PSY_OK;
@@ -163,10 +164,18 @@ int psy_ctx_create(int device, psy_ctx** out) {
     rc = c->halton_perm.upload(hperm);
     if (rc) return rc;
     HIP_OK(hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device));
-    if (trace_persist_occupancy(false, &c->occupancy[0]) || trace_persist_occupancy(true, &c->occupancy[1]) ||
-        render_persist_occupancy(&c->render_occupancy))
+    if (trace_persist_occupancy(false, 16, &c->occupancy[0]) || trace_persist_occupancy(true, 16, &c->occupancy[1]) ||
+        trace_persist_occupancy(false, 32, &c->occupancy32[0]) ||
+        trace_persist_occupancy(true, 32, &c->occupancy32[1]) || render_persist_occupancy(&c->render_occupancy))
         return fail(PSY_ERR_HIP, "psy_ctx_create: occupancy query failed");
     *out = c.release();
+    return PSY_OK;
+}
+
+int psy_ctx_set_trace_stack(psy_ctx* ctx, int entries) {
+    if (!ctx) return fail(PSY_ERR_INVALID, "psy_ctx_set_trace_stack: NULL context");
+    if (entries != 16 && entries != 32) return fail(PSY_ERR_INVALID, "psy_ctx_set_trace_stack: entries must be 16 or 32");
+    ctx->trace_stack = entries;
     return PSY_OK;
 }
 
@@ -184,13 +193,21 @@ int psy_ctx_set_trace_mode(psy_ctx* ctx, int mode, int refill_min, int blocks_pe
 
 int psy_ctx_set_render_mode(psy_ctx* ctx, int mode, int refill_min, int blocks_per_cu) {
     if (!ctx) return fail(PSY_ERR_INVALID, "psy_ctx_set_render_mode: NULL context");
-    if (mode != PSY_RENDER_WAVEFRONT && mode != PSY_RENDER_FUSED)
+    if (mode != PSY_RENDER_WAVEFRONT && mode != PSY_RENDER_FUSED && mode != PSY_RENDER_STREAM)
         return fail(PSY_ERR_INVALID, "psy_ctx_set_render_mode: unknown mode");
     if (refill_min < 1 || refill_min > 64) return fail(PSY_ERR_INVALID, "psy_ctx_set_render_mode: refill_min not in 1..64");
     if (blocks_per_cu > 64) return fail(PSY_ERR_INVALID, "psy_ctx_set_render_mode: blocks_per_cu above 64");
     ctx->render_mode = mode;
     ctx->render_refill = refill_min;
     ctx->render_blocks_per_cu = blocks_per_cu;
+    return PSY_OK;
+}
+
+int psy_ctx_set_stream_steps(psy_ctx* ctx, int steps_per_graph) {
+    if (!ctx) return fail(PSY_ERR_INVALID, "psy_ctx_set_stream_steps: NULL context");
+    if (steps_per_graph < 2 || steps_per_graph > 64 || (steps_per_graph & 1))
+        return fail(PSY_ERR_INVALID, "psy_ctx_set_stream_steps: steps must be even, 2..64");
+    ctx->render_graph_iters = steps_per_graph;
     return PSY_OK;
 }
 

@@ -136,8 +136,11 @@ int launch_build_dice(const float* odata, const DevObject* objs, const int32_t* 
 // ray.
 struct TraceTuning {
     bool persistent;
-    int persist_blocks;
+    int persist_blocks;    // persistent grid with the 16-entry stack
+    int persist_blocks32;  // and with the 32-entry stack
     int refill_min;
+    int stack = 16;        // main-pass LDS stack entries: 16 or 32
+    const uint32_t* n_dev = nullptr;  // device-side ray count (<= n), or NULL
     // hipEvent_t handles recorded around the passes when the context times
     // them: before the main pass, after it, after the second pass.
     void* ev[3] = {nullptr, nullptr, nullptr};
@@ -148,7 +151,7 @@ struct TraceTuning {
 int launch_trace(const DevScene& s, bool world, bool occl_only, int stack_need, const float* rays,
                  int64_t n, int64_t rs, const DevHits& h, uint32_t* queue, uint32_t* counters,
                  const TraceTuning& tune, hipStream_t st);
-int trace_persist_occupancy(bool world, int* blocks_per_cu);
+int trace_persist_occupancy(bool world, int stack, int* blocks_per_cu);
 int launch_world_to_rays(const float* wrays, int64_t n, float* rays, hipStream_t st);
 
 // ---- integrator (psy_render_dev.hpp) ----------------------------------------
@@ -162,7 +165,7 @@ struct RenderLaunch {
     uint32_t seed;
     float bg[3];
 };
-constexpr int PATH_PLANES = 45;  // float planes of a path state
+constexpr int PATH_PLANES = 46;  // float planes of a path state
 // Path buffers of one chunk of `cap` paths: states (PATH_PLANES), two world
 // ray streams, hits, two active lists, counters (u32 x 4).
 struct RenderBuffers {
@@ -177,6 +180,20 @@ int launch_render_init(const RenderLaunch& R, int64_t first, int64_t n, const Re
 // Finished paths write their film XYZ to sample_xyz[3 * (sample - sbase)].
 int launch_render_update(const RenderLaunch& R, int64_t first, const RenderBuffers& B, int cur, int64_t n_in,
                          float* sample_xyz, int64_t sbase, hipStream_t st);
+// Stream form: device-side state of a pool of path slots.
+struct StreamState {
+    uint32_t count[2];         // ping-pong stream lengths
+    uint32_t n_wait;           // slots parked after an even step
+    uint32_t pad;
+    unsigned long long next;   // next image sample to start
+    unsigned long long end;    // end of the sample window
+    unsigned long long rays;   // rays traced
+};
+int launch_stream_init(const RenderLaunch& R, const RenderBuffers& B, int64_t first, int64_t n, int64_t end,
+                       StreamState* S, bool reset_rays, hipStream_t st);
+int launch_stream_shade(const RenderLaunch& R, const RenderBuffers& B, int cur, StreamState* S, uint32_t* wait,
+                        int blocks, float* sample_xyz, int64_t sbase, hipStream_t st);
+
 // Adjoint pass buffers: Q planes (adjoint path state, `cap` entries each),
 // the gradient vector (n_params: albedos by shader, then emission scales by
 // light object), per-block partial rows when the sums fit in LDS.

@@ -73,19 +73,23 @@ struct psy_ctx {
     int64_t queue_cap = 0;
     // main-pass launch form, per ray form (0: object rays, 1: world rays)
     int n_cu = 0;
-    int occupancy[2] = {0, 0};
+    int occupancy[2] = {0, 0};    // 16-entry stack
+    int occupancy32[2] = {0, 0};  // 32-entry stack
+    int trace_stack = 16;         // main-pass LDS stack entries
     int mode = PSY_TRACE_MODE_PLAIN;
     int refill_min = 16;
     int blocks_per_cu = 0;  // <= 0: from occupancy
     // pass timing: event triples of the trace calls since the last query
     bool timing = false;
     std::vector<void*> timing_events;
-    psy::TraceTuning tuning(bool world) {
+    psy::TraceTuning tuning(bool world, bool timed = true) {
         psy::TraceTuning t;
         t.persistent = mode == PSY_TRACE_MODE_PERSISTENT;
         t.persist_blocks = n_cu * (blocks_per_cu > 0 ? blocks_per_cu : occupancy[world ? 1 : 0]);
+        t.persist_blocks32 = n_cu * (blocks_per_cu > 0 ? blocks_per_cu : occupancy32[world ? 1 : 0]);
         t.refill_min = refill_min;
-        if (timing) {
+        t.stack = trace_stack;
+        if (timing && timed) {
             for (int k = 0; k < 3; ++k) {
                 hipEvent_t e = nullptr;
                 if (hipEventCreate(&e) != hipSuccess) {
@@ -104,6 +108,7 @@ struct psy_ctx {
     }
     ~psy_ctx() {
         for (void* e : timing_events) (void)hipEventDestroy((hipEvent_t)e);
+        if (capture_stream) (void)hipStreamDestroy(capture_stream);
     }
     // integrator scratch (path states, ray / hit streams, active lists)
     psy::DBuf render_scratch;
@@ -112,6 +117,8 @@ struct psy_ctx {
     int render_refill = 16;          // fused mode: refill a wave at this many idle lanes
     int render_blocks_per_cu = 0;    // fused mode: <= 0 from occupancy
     int render_occupancy = 0;
+    int render_graph_iters = 4;      // stream mode: steps per captured graph
+    hipStream_t capture_stream = nullptr;  // for graph capture (never runs work)
 };
 
 struct psy_scene {

@@ -70,30 +70,35 @@ __global__ void k_build_dice(const float* odata, const DevObject* objs, const in
     else build_node<16>(odata, ob, k, out);
 }
 
-template <bool WORLD>
-__global__ __launch_bounds__(TRACE_BLOCK, WORLD ? 4 : PSY_MAIN_WAVES_PER_SIMD) void k_trace(DevScene s, const float* __restrict__ rays, int64_t n,
-                                                        int64_t rs, DevHits h, uint32_t flags, uint32_t* __restrict__ queue,
-                                                        uint32_t* __restrict__ counters) {
-    __shared__ int32_t stk_ref[MAIN_STACK][TRACE_BLOCK];
-    __shared__ float stk_near[MAIN_STACK][TRACE_BLOCK];
+// Main pass, one thread per ray.  n_dev (or NULL): a device-side ray
+// count, read at launch, that lowers n (streams whose length only the
+// device knows, e.g. in a captured graph).
+template <bool WORLD, int STACK>
+__global__ __launch_bounds__(TRACE_BLOCK, main_waves(WORLD, STACK)) void k_trace(
+    DevScene s, const float* __restrict__ rays, int64_t n, int64_t rs, DevHits h, uint32_t flags,
+    uint32_t* __restrict__ queue, uint32_t* __restrict__ counters, const uint32_t* __restrict__ n_dev) {
+    __shared__ int32_t stk_ref[STACK][TRACE_BLOCK];
+    __shared__ float stk_near[STACK][TRACE_BLOCK];
     const int tid = threadIdx.x;
     const int64_t i = (int64_t)blockIdx.x * TRACE_BLOCK + tid;
+    if (n_dev) n = std::min<int64_t>(n, (int64_t)*n_dev);
     if (i >= n) return;
     Lane L;
     lane_init<WORLD>(s, rays, rs, (uint32_t)i, flags, L);
-    while (!lane_step<WORLD>(s, L, stk_ref, stk_near, tid)) {
+    while (!lane_step<WORLD, STACK>(s, L, stk_ref, stk_near, tid)) {
     }
     if (lane_finish(s, L, h)) enqueue_retrace(queue, counters, (uint32_t)i);
 }
 
-template <bool WORLD>
-__global__ __launch_bounds__(TRACE_BLOCK, WORLD ? 4 : PSY_MAIN_WAVES_PER_SIMD) void k_trace_persist(DevScene s, const float* __restrict__ rays,
-                                                                int64_t n, int64_t rs, DevHits h, uint32_t flags,
-                                                                uint32_t* __restrict__ queue,
-                                                                uint32_t* __restrict__ counters, int refill_min) {
-    __shared__ int32_t stk_ref[MAIN_STACK][TRACE_BLOCK];
-    __shared__ float stk_near[MAIN_STACK][TRACE_BLOCK];
+template <bool WORLD, int STACK>
+__global__ __launch_bounds__(TRACE_BLOCK, main_waves(WORLD, STACK)) void k_trace_persist(
+    DevScene s, const float* __restrict__ rays, int64_t n, int64_t rs, DevHits h, uint32_t flags,
+    uint32_t* __restrict__ queue, uint32_t* __restrict__ counters, int refill_min,
+    const uint32_t* __restrict__ n_dev) {
+    __shared__ int32_t stk_ref[STACK][TRACE_BLOCK];
+    __shared__ float stk_near[STACK][TRACE_BLOCK];
     const int tid = threadIdx.x;
+    if (n_dev) n = std::min<int64_t>(n, (int64_t)*n_dev);
     const uint32_t lane = __lane_id();
     unsigned long long* next_ray = reinterpret_cast<unsigned long long*>(counters + 2);
     Lane L;
@@ -117,7 +122,7 @@ __global__ __launch_bounds__(TRACE_BLOCK, WORLD ? 4 : PSY_MAIN_WAVES_PER_SIMD) v
             }
         }
         if (exhausted && __ballot(active) == 0ull) break;
-        if (active && lane_step<WORLD>(s, L, stk_ref, stk_near, tid)) {
+        if (active && lane_step<WORLD, STACK>(s, L, stk_ref, stk_near, tid)) {
             active = false;
             if (lane_finish(s, L, h)) enqueue_retrace(queue, counters, L.idx);
         }
@@ -128,7 +133,8 @@ template <bool WORLD, int STACK>
 __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_deep(DevScene s, const float* __restrict__ rays, int64_t n,
                                                              int64_t rs, DevHits h, uint32_t flags,
                                                              const uint32_t* __restrict__ queue,
-                                                             const uint32_t* __restrict__ counters) {
+                                                             uint32_t* __restrict__ counters,
+                                                             uint32_t* __restrict__ coop_queue) {
     __shared__ int32_t stk_ref[STACK][TRACE_BLOCK];
     __shared__ float stk_near[STACK][TRACE_BLOCK];
     const int tid = threadIdx.x;
@@ -139,8 +145,32 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_deep(DevScene s, const fl
     const uint32_t n_waves = gridDim.x * (TRACE_BLOCK / 64);
     const uint32_t wave = blockIdx.x * (TRACE_BLOCK / 64) + (uint32_t)(tid >> 6);
     const uint32_t lane = (uint32_t)(tid & 63);
-    for (uint64_t k = (uint64_t)wave + (uint64_t)lane * n_waves; k < count; k += 64ull * n_waves)
-        (void)trace_one<WORLD, STACK, true>(s, rays, rs, (int64_t)queue[k], h, flags, stk_ref, stk_near, tid);
+    for (uint64_t k = (uint64_t)wave + (uint64_t)lane * n_waves; k < count; k += 64ull * n_waves) {
+        const uint32_t i = queue[k];
+        // a ray that runs out of node budget goes to the third pass
+        if (trace_one<WORLD, STACK, D_SERIAL>(s, rays, rs, (int64_t)i, h, flags, stk_ref, stk_near, tid, DEEP_BUDGET))
+            coop_queue[atomicAdd(&counters[1], 1u)] = i;
+    }
+}
+
+// Third pass: the rays whose split trees outran the second pass's budget,
+// one per wave, every lane of the wave on the same ray; the split-dice
+// below the cache runs across the lanes from a node stack in LDS.
+template <bool WORLD, int STACK>
+__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_coop(DevScene s, const float* __restrict__ rays, int64_t n,
+                                                             int64_t rs, DevHits h, uint32_t flags,
+                                                             const uint32_t* __restrict__ counters,
+                                                             const uint32_t* __restrict__ coop_queue) {
+    __shared__ int32_t stk_ref[STACK][TRACE_BLOCK];
+    __shared__ float stk_near[STACK][TRACE_BLOCK];
+    __shared__ uint64_t nodes[TRACE_BLOCK / 64][COOP_CAP];
+    const int tid = threadIdx.x;
+    const uint32_t count = counters[1];
+    const uint32_t n_waves = gridDim.x * (TRACE_BLOCK / 64);
+    const uint32_t wib = (uint32_t)(tid >> 6);
+    for (uint32_t k = blockIdx.x * (TRACE_BLOCK / 64) + wib; k < count; k += n_waves)
+        (void)trace_one<WORLD, STACK, D_COOP>(s, rays, rs, (int64_t)coop_queue[k], h, flags, stk_ref, stk_near, tid,
+                                              INT32_MAX, nodes[wib]);
 }
 
 __global__ void k_zero_counters(uint32_t* p) {
@@ -309,6 +339,88 @@ __global__ __launch_bounds__(RENDER_BLOCK) void k_render_update(RenderLaunch A, 
     compact_push(alive, i, nr, act_out, rays_out, rcap, count_out);
 }
 
+// ---- stream form ---------------------------------------------------------------
+// A pool of `cap` path slots works through the samples [first, end) of the
+// image: the shading step hands a finished path's slot to the next sample
+// (one atomic per wave), whose camera ray joins the same compacted stream.
+// So one stream mixes the paths of many image tiles at every bounce depth,
+// it stays full until the samples run out, and each step's length lives
+// only on the device (the trace reads it there), which lets the host capture
+// a run of steps as one graph.
+// Paths start only on even steps (cur == 0 streams), so every path's step
+// parity is the stream's: even streams hold camera and bounce rays (closest
+// hit), odd streams shadow rays (any hit) -- no wave mixes the two kinds.  A
+// path that ends after an even step parks its slot on the wait list for one
+// step; the odd step's shading refills finished and parked slots alike.
+__global__ void k_stream_init(RenderLaunch A, PathBuf P, int64_t first, int64_t n, int64_t end,
+                              float* __restrict__ rays, int64_t rcap, uint32_t* __restrict__ act,
+                              StreamState* __restrict__ S, int reset_rays) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) render_init_one(A, first, i, P, rays, rcap, act);
+    if (i == 0) {
+        S->count[0] = (uint32_t)n;
+        S->count[1] = 0u;
+        S->n_wait = 0u;
+        S->next = (unsigned long long)(first + n);
+        S->end = (unsigned long long)end;
+        if (reset_rays) S->rays = 0ull;
+    }
+}
+
+__global__ __launch_bounds__(RENDER_BLOCK) void k_stream_shade(RenderLaunch A, PathBuf P, int cur,
+                                                               const uint32_t* __restrict__ act_in,
+                                                               const float* __restrict__ rays_in, DevHits hits,
+                                                               int64_t rcap, uint32_t* __restrict__ act_out,
+                                                               float* __restrict__ rays_out,
+                                                               uint32_t* __restrict__ wait,
+                                                               StreamState* __restrict__ S,
+                                                               float* __restrict__ sample_xyz, int64_t sbase) {
+    const int64_t n_in = (int64_t)S->count[cur];
+    // odd step: the parked slots are refilled too (entries past the stream)
+    const int64_t n_wait = cur ? (int64_t)S->n_wait : 0;
+    const int64_t n_all = n_in + n_wait;
+    const int64_t end = (int64_t)S->end;
+    if (blockIdx.x == 0 && threadIdx.x == 0) S->rays += (unsigned long long)n_in;
+    uint32_t* count_out = &S->count[cur ^ 1];
+    const uint32_t lane = __lane_id();
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t j0 = (int64_t)blockIdx.x * blockDim.x; j0 < n_all; j0 += stride) {
+        const int64_t j = j0 + threadIdx.x;
+        bool alive = false;
+        uint32_t i = 0;
+        WRay nr;
+        if (j < n_in) {
+            i = act_in[j];
+            alive = render_update_one(A, 0, P, i, j, rays_in, hits, rcap, sample_xyz, sbase, nr);
+        } else if (j < n_all) {
+            i = wait[j - n_in];
+        }
+        bool want = j < n_all && !alive;
+        if (!cur) {
+            // even step: finished slots wait for the odd step
+            const unsigned long long mw = __ballot(want);
+            uint32_t wb = 0;
+            if (lane == 0 && mw) wb = atomicAdd(&S->n_wait, (uint32_t)__popcll(mw));
+            wb = (uint32_t)__shfl((int)wb, 0);
+            if (want) wait[wb + (uint32_t)__popcll(mw & ((1ull << lane) - 1ull))] = i;
+            want = false;
+        }
+        const unsigned long long m = __ballot(want);
+        unsigned long long base = 0;
+        if (lane == 0 && m) base = atomicAdd(&S->next, (unsigned long long)__popcll(m));
+        base = ((unsigned long long)(uint32_t)__shfl((int)(uint32_t)(base >> 32), 0) << 32) |
+               (unsigned long long)(uint32_t)__shfl((int)(uint32_t)base, 0);
+        if (want) {
+            const int64_t g = (int64_t)base + __popcll(m & ((1ull << lane) - 1ull));
+            if (g < end) {
+                render_init_path(A, g, i, P, nr);
+                alive = true;
+            }
+        }
+        compact_push(alive, i, nr, act_out, rays_out, rcap, count_out);
+    }
+}
+
 // ---- adjoint ----------------------------------------------------------------
 // Gradient accumulator of the adjoint pass: float atomics into a block's LDS
 // row of n_params sums, or straight into the global gradient vector.
@@ -432,7 +544,7 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_render_persist(RenderLaunch A, 
                         load_wray(resume_rays, rcap, e, w);
                     } else {
                         pi = (uint32_t)(e - n_resume);
-                        render_init_path(A, first, pi, P, w);
+                        render_init_path(A, first + pi, pi, P, w);
                     }
                     rtype = w.type;
                     lane_init_wray(s, w, pi, L);
@@ -506,10 +618,13 @@ int launch_build_dice(const float* odata, const DevObject* objs, const int32_t* 
 
 template <bool W, int S>
 static void launch_deep(const DevScene& s, const float* rays, int64_t n, int64_t rs, const DevHits& h, uint32_t flags,
-                        const uint32_t* queue, const uint32_t* counters, hipStream_t st) {
+                        const uint32_t* queue, uint32_t* counters, hipStream_t st) {
     const unsigned blocks = (unsigned)std::min<int64_t>(grid_for(n, TRACE_BLOCK), 1024);
+    uint32_t* coop_queue = const_cast<uint32_t*>(queue) + n;
     hipLaunchKernelGGL((k_trace_deep<W, S>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, s, rays, n, rs, h, flags,
-                       queue, counters);
+                       queue, counters, coop_queue);
+    hipLaunchKernelGGL((k_trace_coop<W, S>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, s, rays, n, rs, h, flags,
+                       (const uint32_t*)counters, (const uint32_t*)coop_queue);
 }
 
 template <bool W>
@@ -519,13 +634,21 @@ static int launch_trace_w(const DevScene& s, int stack_need, const float* rays, 
     hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, counters);
     if (tune.ev[0]) (void)hipEventRecord((hipEvent_t)tune.ev[0], st);
     const int64_t plain_blocks = grid_for(n, TRACE_BLOCK);
-    const int64_t persist_blocks = (int64_t)tune.persist_blocks;
+    const int64_t persist_blocks = (int64_t)(tune.stack > 16 ? tune.persist_blocks32 : tune.persist_blocks);
+    const int refill = std::max(1, std::min(64, tune.refill_min));
     if (tune.persistent && persist_blocks > 0 && persist_blocks < plain_blocks) {
-        hipLaunchKernelGGL((k_trace_persist<W>), dim3((unsigned)persist_blocks), dim3(TRACE_BLOCK), 0, st, s, rays,
-                           n, rs, h, flags, queue, counters, std::max(1, std::min(64, tune.refill_min)));
+        if (tune.stack > 16)
+            hipLaunchKernelGGL((k_trace_persist<W, 32>), dim3((unsigned)persist_blocks), dim3(TRACE_BLOCK), 0, st, s,
+                               rays, n, rs, h, flags, queue, counters, refill, tune.n_dev);
+        else
+            hipLaunchKernelGGL((k_trace_persist<W, MAIN_STACK>), dim3((unsigned)persist_blocks), dim3(TRACE_BLOCK), 0,
+                               st, s, rays, n, rs, h, flags, queue, counters, refill, tune.n_dev);
+    } else if (tune.stack > 16) {
+        hipLaunchKernelGGL((k_trace<W, 32>), dim3((unsigned)plain_blocks), dim3(TRACE_BLOCK), 0, st, s, rays, n, rs,
+                           h, flags, queue, counters, tune.n_dev);
     } else {
-        hipLaunchKernelGGL((k_trace<W>), dim3((unsigned)plain_blocks), dim3(TRACE_BLOCK), 0, st, s, rays, n, rs, h,
-                           flags, queue, counters);
+        hipLaunchKernelGGL((k_trace<W, MAIN_STACK>), dim3((unsigned)plain_blocks), dim3(TRACE_BLOCK), 0, st, s, rays, n,
+                           rs, h, flags, queue, counters, tune.n_dev);
     }
     if (tune.ev[1]) (void)hipEventRecord((hipEvent_t)tune.ev[1], st);
     if (stack_need <= 16) launch_deep<W, 16>(s, rays, n, rs, h, flags, queue, counters, st);
@@ -545,10 +668,16 @@ int launch_trace(const DevScene& s, bool world, bool occl_only, int stack_need, 
                  : launch_trace_w<false>(s, stack_need, rays, n, rs, h, flags, queue, counters, tune, st);
 }
 
-int trace_persist_occupancy(bool world, int* blocks_per_cu) {
+int trace_persist_occupancy(bool world, int stack, int* blocks_per_cu) {
     int nb = 0;
-    hipError_t e = world ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace_persist<true>, TRACE_BLOCK, 0)
-                         : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace_persist<false>, TRACE_BLOCK, 0);
+    hipError_t e;
+    if (stack > 16)
+        e = world ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace_persist<true, 32>, TRACE_BLOCK, 0)
+                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace_persist<false, 32>, TRACE_BLOCK, 0);
+    else
+        e = world ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace_persist<true, MAIN_STACK>, TRACE_BLOCK, 0)
+                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trace_persist<false, MAIN_STACK>, TRACE_BLOCK,
+                                                                 0);
     *blocks_per_cu = nb;
     return e == hipSuccess ? 0 : PSY_ERR_HIP;
 }
@@ -634,6 +763,23 @@ int launch_render_update(const RenderLaunch& R, int64_t first, const RenderBuffe
 int launch_fill_f32(float* p, int64_t n, float v, hipStream_t st) {
     if (n <= 0) return 0;
     hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(n, 256)), dim3(256), 0, st, p, n, v);
+    return hipGetLastError() == hipSuccess ? 0 : PSY_ERR_HIP;
+}
+
+int launch_stream_init(const RenderLaunch& R, const RenderBuffers& B, int64_t first, int64_t n, int64_t end,
+                       StreamState* S, bool reset_rays, hipStream_t st) {
+    const int64_t g = std::max<int64_t>(1, (n + RENDER_BLOCK - 1) / RENDER_BLOCK);
+    hipLaunchKernelGGL(k_stream_init, dim3((unsigned)g), dim3(RENDER_BLOCK), 0, st, R, PathBuf{B.paths, B.cap}, first, n,
+                       end, B.rays[0], B.cap, B.act[0], S, reset_rays ? 1 : 0);
+    return hipGetLastError() == hipSuccess ? 0 : PSY_ERR_HIP;
+}
+
+int launch_stream_shade(const RenderLaunch& R, const RenderBuffers& B, int cur, StreamState* S, uint32_t* wait,
+                        int blocks, float* sample_xyz, int64_t sbase, hipStream_t st) {
+    const int nxt = cur ^ 1;
+    hipLaunchKernelGGL(k_stream_shade, dim3((unsigned)std::max(1, blocks)), dim3(RENDER_BLOCK), 0, st, R,
+                       PathBuf{B.paths, B.cap}, cur, (const uint32_t*)B.act[cur], (const float*)B.rays[cur], B.hits,
+                       B.cap, B.act[nxt], B.rays[nxt], wait, S, sample_xyz, sbase);
     return hipGetLastError() == hipSuccess ? 0 : PSY_ERR_HIP;
 }
 

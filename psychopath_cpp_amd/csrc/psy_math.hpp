@@ -217,6 +217,16 @@ PSY_HD M44 m_inverse(const M44& a) {
 
 // ---- BBox (basics/bbox.hpp:138-177) ----------------------------------------
 #define PSY_BBOX_MAXT_ADJUST 1.00000024f
+// The dicing width std::max(ray.min_width(..) * dice_rate, min_upoly_size)
+// of object/patch_utils.hpp:77 for a footprint that is a number.  A NaN
+// footprint (NaN ray differentials: a transfer across a zero cosine) makes
+// the reference split until its boxes stop being hit, which need not end;
+// here it takes min_upoly_size, so such a ray dices to the smallest
+// micropolygons and ends.
+PSY_HD float dice_width(float scaled_width, float min_upoly) {
+    return scaled_width > min_upoly ? scaled_width : min_upoly;
+}
+
 PSY_HD bool bbox_ray(V3 bmin, V3 bmax, V3 o, V3 dinv, float t, float* h0, float* h1) {
     const float tx1 = (bmin.x - o.x) * dinv.x;
     const float tx2 = (bmax.x - o.x) * dinv.x;

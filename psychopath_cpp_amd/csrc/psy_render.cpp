@@ -8,6 +8,7 @@
 // instance's inverse scale) because the tree decides which light each
 // shadow sample picks: the same tree gives the same light per sample.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -549,6 +550,88 @@ int run_fused(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, co
     return PSY_OK;
 }
 
+// The stream form (PSY_RENDER_STREAM): a pool of U.cap path slots over
+// windows of the image's samples, K steps per captured graph.
+int capture_steps(psy_ctx* ctx, const psy_scene* s, const RenderLaunch& R, const RenderSetup& U, StreamState* S,
+                  uint32_t* wait, float* samp, int64_t sbase, int iters, hipGraphExec_t* exec) {
+    if (!ctx->capture_stream) HIP_OK(hipStreamCreateWithFlags(&ctx->capture_stream, hipStreamNonBlocking));
+    hipStream_t cs = ctx->capture_stream;
+    const RenderBuffers& B = U.B;
+    TraceTuning tune = ctx->tuning(true, false);
+    const int blocks = std::max(1, 2 * ctx->n_cu);
+    HIP_OK(hipStreamBeginCapture(cs, hipStreamCaptureModeRelaxed));
+    int rc = 0;
+    for (int it = 0; it < iters && !rc; ++it) {
+        const int cur = it & 1;
+        tune.n_dev = &S->count[cur];
+        rc = launch_trace(s->dev, true, false, s->stack_need, B.rays[cur], U.cap, U.cap, B.hits,
+                          (uint32_t*)ctx->queue.p, (uint32_t*)ctx->counters.p, tune, cs);
+        if (!rc) rc = launch_zero_u32(&S->count[cur ^ 1], 1, cs);
+        if (!rc) rc = launch_stream_shade(R, B, cur, S, wait, blocks, samp, sbase, cs);
+        if (!rc && cur) rc = launch_zero_u32(&S->n_wait, 1, cs);  // the odd step refilled every parked slot
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(cs, &g);
+    if (rc || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(rc ? rc : PSY_ERR_HIP, "psy_render: capturing the stream steps failed");
+    }
+    const hipError_t ei = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) return fail(PSY_ERR_HIP, "psy_render: graph instantiation failed");
+    return PSY_OK;
+}
+
+int run_stream(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, const RenderLaunch& R,
+               const RenderSetup& U, int64_t window, float* film_xyz, float* sample_xyz, int64_t* rays_traced,
+               hipStream_t st) {
+    StreamState* S = (StreamState*)U.extra;
+    uint32_t* wait = (uint32_t*)(U.extra + 64);
+    float* wsamp = sample_xyz ? sample_xyz : U.extra + 64 + U.cap;
+    const int iters = ctx->render_graph_iters;
+    for (int64_t w0 = 0; w0 < U.total; w0 += window) {
+        const int64_t w1 = std::min(U.total, w0 + window);
+        const int64_t sbase = sample_xyz ? 0 : w0;
+        const int64_t n0 = std::min(U.cap, w1 - w0);
+        if (int rc = launch_stream_init(R, U.B, w0, n0, w1, S, w0 == 0, st))
+            return fail(rc, "psy_render: stream init launch failed");
+        hipGraphExec_t exec = nullptr;
+        if (int rc = capture_steps(ctx, s, R, U, S, wait, wsamp, sbase, iters, &exec)) return rc;
+        // every iteration with a live stream traces at least one ray and a
+        // path takes at most 2 * path_length of them
+        const int64_t max_iters = 2 * (int64_t)prm->path_length * ((w1 - w0 + U.cap - 1) / U.cap + 1) * 4 + 64;
+        int rc = PSY_OK;
+        for (int64_t done = 0;; done += iters) {
+            if (done > max_iters) {
+                rc = fail(PSY_ERR_HIP, "psy_render: stream steps did not converge");
+                break;
+            }
+            if (hipGraphLaunch(exec, st) != hipSuccess) {
+                rc = fail(PSY_ERR_HIP, "psy_render: graph launch failed");
+                break;
+            }
+            uint32_t n = 0;
+            if (hipMemcpyAsync(&n, &S->count[0], sizeof n, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess) {
+                rc = fail(PSY_ERR_HIP, "psy_render: stream synchronization failed");
+                break;
+            }
+            if (n == 0) break;
+        }
+        (void)hipGraphExecDestroy(exec);
+        if (rc) return rc;
+        if (film_xyz) {
+            rc = launch_render_resolve(wsamp, sbase, w0 / U.spp, (w1 - w0) / U.spp, (int)U.spp, film_xyz, st);
+            if (rc) return fail(rc, "psy_render: resolve launch failed");
+        }
+    }
+    unsigned long long total_rays = 0;
+    HIP_OK(hipMemcpyAsync(&total_rays, &S->rays, sizeof total_rays, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    if (rays_traced) *rays_traced = (int64_t)total_rays;
+    return PSY_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -561,10 +644,100 @@ int psy_render(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, f
     hipStream_t st = (hipStream_t)stream;
     RenderSetup U;
     plan_render(ctx, prm, U);
-    if (int rc = alloc_render(ctx, sample_xyz == nullptr, 0, U)) return rc;
     const RenderLaunch R = make_launch(ctx, s, prm);
+    if (ctx->render_mode == PSY_RENDER_STREAM) {
+        // sample windows of whole pixels: the whole image when the caller
+        // keeps every sample, else up to max(4 pools, 2^26) samples at a time
+        int64_t window = U.total;
+        if (!sample_xyz) {
+            window = std::min(U.total, std::max<int64_t>(4 * U.cap, (int64_t)1 << 26));
+            window = std::max<int64_t>(U.spp, window / U.spp * U.spp);
+        }
+        if (int rc = alloc_render(ctx, false, 64 + U.cap + (sample_xyz ? 0 : 3 * window), U)) return rc;
+        return run_stream(ctx, s, prm, R, U, window, film_xyz, sample_xyz, rays_traced, st);
+    }
+    if (int rc = alloc_render(ctx, sample_xyz == nullptr, 0, U)) return rc;
     if (ctx->render_mode == PSY_RENDER_FUSED) return run_fused(ctx, s, prm, R, U, film_xyz, sample_xyz, rays_traced, st);
     return run_wavefront(ctx, s, prm, R, U, film_xyz, sample_xyz, nullptr, rays_traced, st);
+}
+
+// The render autotuner: coordinate descent over the stream form's pool size,
+// the main pass's launch form and LDS stack depth, and the steps per
+// captured graph, timing psy_render on a sample budget of the image (every
+// pixel at reduced spp).  Leaves the best setting on the context.
+int psy_render_autotune(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, int64_t max_samples,
+                        psy_render_tuning* out) {
+    if (int rc = check_render(ctx, s, prm, "psy_render_autotune")) return rc;
+    HIP_OK(hipSetDevice(ctx->device));
+    psy_render_params p = *prm;
+    const int64_t n_pix = (int64_t)p.width * p.height;
+    if (max_samples <= 0) max_samples = (int64_t)1 << 23;
+    p.spp = (int32_t)std::max<int64_t>(1, std::min<int64_t>(p.spp, max_samples / std::max<int64_t>(1, n_pix)));
+    const int64_t total = n_pix * p.spp;
+    // saved so that a failed trial leaves the context as it was
+    const int s_mode = ctx->render_mode, s_tmode = ctx->mode, s_refill = ctx->refill_min, s_stack = ctx->trace_stack,
+              s_iters = ctx->render_graph_iters;
+    const int64_t s_chunk = ctx->render_chunk;
+    struct Cfg { int64_t pool; int tmode, refill, stack, iters; };
+    auto apply = [&](const Cfg& c) {
+        ctx->render_mode = PSY_RENDER_STREAM;
+        ctx->render_chunk = c.pool;
+        ctx->mode = c.tmode;
+        ctx->refill_min = c.refill;
+        ctx->trace_stack = c.stack;
+        ctx->render_graph_iters = c.iters;
+    };
+    int trials = 0;
+    auto timed = [&](const Cfg& c, double* mrays) -> int {
+        apply(c);
+        int64_t rays = 0;
+        int rc = psy_render(ctx, s, &p, nullptr, nullptr, &rays, nullptr);  // warm-up: buffers, code
+        if (rc) return rc;
+        HIP_OK(hipDeviceSynchronize());
+        const auto t0 = std::chrono::steady_clock::now();
+        rc = psy_render(ctx, s, &p, nullptr, nullptr, &rays, nullptr);
+        if (rc) return rc;
+        HIP_OK(hipDeviceSynchronize());
+        const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        *mrays = (double)rays / std::max(dt, 1e-9) / 1e6;
+        ++trials;
+        return PSY_OK;
+    };
+    Cfg best{std::min<int64_t>((int64_t)1 << 21, total), PSY_TRACE_MODE_PLAIN, 16, 16, 4};
+    double best_m = 0.0;
+    int rc = timed(best, &best_m);
+    auto consider = [&](Cfg c) -> int {
+        double m = 0.0;
+        if (int r = timed(c, &m)) return r;
+        if (m > best_m) { best_m = m; best = c; }
+        return PSY_OK;
+    };
+    // pool size: powers of two from 2^19, up to the workload
+    for (int64_t pool = (int64_t)1 << 19; !rc && pool <= ((int64_t)1 << 23); pool <<= 1) {
+        const int64_t c = std::min(pool, total);
+        if (c != best.pool) { Cfg t = best; t.pool = c; rc = consider(t); }
+        if (c == total) break;
+    }
+    if (!rc) { Cfg t = best; t.tmode = PSY_TRACE_MODE_PERSISTENT; t.refill = 32; rc = consider(t); }
+    if (!rc) { Cfg t = best; t.stack = 32; rc = consider(t); }
+    for (int it : {2, 8, 16}) if (!rc && it != best.iters) { Cfg t = best; t.iters = it; rc = consider(t); }
+    if (rc) {
+        ctx->render_mode = s_mode; ctx->mode = s_tmode; ctx->refill_min = s_refill; ctx->trace_stack = s_stack;
+        ctx->render_graph_iters = s_iters; ctx->render_chunk = s_chunk;
+        return rc;
+    }
+    apply(best);
+    if (out) {
+        out->mode = PSY_RENDER_STREAM;
+        out->trace_mode = best.tmode;
+        out->refill_min = best.refill;
+        out->stack = best.stack;
+        out->steps_per_graph = best.iters;
+        out->trials = trials;
+        out->pool = best.pool;
+        out->mrays_per_s = best_m;
+    }
+    return PSY_OK;
 }
 
 int psy_adjoint(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, const float* d_film_xyz,

@@ -53,7 +53,8 @@ enum : int {
     P_PODY = 36,
     P_PDDX = 39,
     P_PDDY = 42,
-    P_COUNT = 45
+    P_SIDX = 45,     // sample index within the pixel
+    P_COUNT = 46
 };
 static_assert(P_COUNT == PATH_PLANES, "path state layout");
 constexpr uint32_t STEP_BACKFACING = 1u << 8;
@@ -498,8 +499,7 @@ PSY_DEV void store_wray(float* w, int64_t rcap, int64_t j, const WRay& r) {
 
 // init_path + the camera ray (next_ray_for_path step 0) of path slot i of
 // the chunk, sample `first + i` of the image in (y, x, s) order.
-PSY_DEV void render_init_path(const RenderLaunch& A, int64_t first, int64_t i, const PathBuf& P, WRay& ray) {
-    const int64_t gid = first + i;
+PSY_DEV void render_init_path(const RenderLaunch& A, int64_t gid, int64_t i, const PathBuf& P, WRay& ray) {
     const int s = (int)(gid % A.spp);
     const int64_t pix = gid / A.spp;
     const int px = (int)(pix % A.width);
@@ -521,6 +521,7 @@ PSY_DEV void render_init_path(const RenderLaunch& A, int64_t first, int64_t i, c
     P.setu(P_OFF, i, off);
     P.setu(P_DIM, i, dim);
     P.setu(P_PIX, i, (uint32_t)px | ((uint32_t)py << 16));
+    P.setu(P_SIDX, i, (uint32_t)s);
     P.setu(P_STEP, i, 0u);
     P.f(P_TIME, i) = time;
     P.f(P_WL, i) = wl;
@@ -535,7 +536,7 @@ PSY_DEV void render_init_path(const RenderLaunch& A, int64_t first, int64_t i, c
 PSY_DEV void render_init_one(const RenderLaunch& A, int64_t first, int64_t i, const PathBuf& P, float* rays,
                              int64_t rcap, uint32_t* act) {
     WRay r;
-    render_init_path(A, first, i, P, r);
+    render_init_path(A, first + i, i, P, r);
     store_wray(rays, rcap, i, r);
     act[i] = (uint32_t)i;
 }
@@ -633,6 +634,12 @@ PSY_DEV void adj_contrib(const DevScene& s, const Adj& adj, uint32_t i, int nver
             adj.add((int)sh[v], wb * ex);
         }
     }
+}
+
+// The image sample a path slot holds: (y * width + x) * spp + s.
+PSY_DEV int64_t path_sample(const RenderLaunch& A, const PathBuf& P, uint32_t i) {
+    const uint32_t pix = P.u(P_PIX, i);
+    return ((int64_t)(pix >> 16) * A.width + (int64_t)(pix & 0xffffu)) * A.spp + (int64_t)P.u(P_SIDX, i);
 }
 
 // update_path for the ray just traced (slot j of the stream) of path i, then
@@ -830,7 +837,7 @@ PSY_DEV bool render_update_path(const RenderLaunch& A, int64_t first, const Path
         if (!alive && sample_xyz) {
             float xyz[3];
             path_film_xyz(P.f(P_WL, i), col, xyz);
-            const int64_t g = first + i - sbase;
+            const int64_t g = path_sample(A, P, i) - sbase;
             sample_xyz[3 * g + 0] = xyz[0];
             sample_xyz[3 * g + 1] = xyz[1];
             sample_xyz[3 * g + 2] = xyz[2];

@@ -131,58 +131,301 @@ PSY_DEV bool accept_leaf(float h0, float h1, float u, float v, float maxd, uint3
     return false;
 }
 
-// Split-dice below the cached depth: depth-first over the subtree of heap
-// node `k` (already found to need splitting), replaying the splits from the
-// patch root for every visited node.  `sidx` is the reference's split-stack
-// index of k (object/patch_utils.hpp:80: a node at stack index 63 is a leaf).
+// How a trace handles a patch node that needs splits below the dice cache:
+// D_FLAG flags the ray for the second pass (main pass), D_SERIAL splits on
+// in the lane within a node budget (second pass; a ray that runs out of
+// budget is flagged for the third pass), D_COOP splits with the 64 lanes of
+// the wave together (third pass: every lane runs the same ray).
+constexpr int D_FLAG = 0, D_SERIAL = 1, D_COOP = 2;
+struct DeepState {
+    bool need;          // flagged for the next pass
+    int32_t budget;     // D_SERIAL: node expansions left
+    uint64_t* coop;     // D_COOP: the wave's node stack in LDS
+};
+constexpr int32_t DEEP_BUDGET = 512;   // expansions per ray in the second pass
+constexpr int COOP_CAP = 2048;         // node stack entries per wave in the third pass
+
+// Split-dice below the cached depth, over the subtree of a node that is
+// already known to need splitting (`base`: its patch, `uvb`: its parameter
+// rectangle, `sidx0`: its split-stack index), nearest half first: a node's
+// two halves are bounded and tested together, the nearer one that needs
+// splitting is descended into (the second half on ties, as the reference
+// visits it first) and the other is left pending, to be re-tested against
+// the best hit found meanwhile.  Visiting near to far matters for rays that
+// graze a surface: their slab overlaps long runs of micro-bboxes, and a
+// fixed order can improve the hit at every one of them, while near-to-far
+// prunes all but the first.  The walk keeps no patch stack: a node is its
+// path of halves (`pb`, deepest half in bit 0), replayed from `base`, and
+// `trail` has one bit per depth whose sibling is pending.  The reference's
+// split-stack index (object/patch_utils.hpp:80: a node at index 63 is a
+// leaf) grows by one for every second half on the path.
 template <int N>
-PSY_DEV bool dice_deep(const DevScene& s, const DevObject& ob, uint32_t k0, uint32_t inst, V3 o,
-                                       V3 dinv, RayW w, bool occl, Best& b) {
-    Patch<N> base;
-    load_patch<N>(s.odata, ob.data, base);
-    float uvb[4];
-    patch_at_node<N>(base, k0, uvb);
-    const int sidx0 = __builtin_popcount(k0) - 1;
-    int k = 1;
-    uint64_t pb = 1;
-    int sidx = sidx0 + 1;
+PSY_DEV bool dice_deep_from(const DevScene& s, const Patch<N>& base, const float uvb[4], int sidx0, uint32_t inst,
+                            V3 o, V3 dinv, const RayW& w, bool occl, Best& b, DeepState& ds) {
+    int k = 0;           // depth below the start node
+    uint64_t pb = 0;     // path of the current node
+    uint64_t trail = 0;  // bit d: the sibling of the path's node at depth d is pending
+    bool expand = true;  // the current node is hit and needs splitting
     for (;;) {
+        if (expand) {
+            if (--ds.budget < 0) {
+                ds.need = true;
+                return true;
+            }
+            Patch<N> p = base;
+            float uv[4] = {uvb[0], uvb[1], uvb[2], uvb[3]};
+            for (int l = k - 1; l >= 0; --l) patch_step<N>(p, ((pb >> l) & 1ull) != 0, uv);
+            Patch<N> p1 = p;
+            float uv1[4] = {uv[0], uv[1], uv[2], uv[3]};
+            patch_step<N>(p1, true, uv1);
+            patch_step<N>(p, false, uv);
+            const int sidx = sidx0 + __builtin_popcountll(pb);  // of the first half; the second adds one
+            bool go0 = false, go1 = false;
+            float n0 = 0.0f, n1 = 0.0f;
+            {
+                V3 lo, hi;
+                patch_bound<N>(p1, &lo, &hi);
+                float h0, h1;
+                if (bbox_ray(lo, hi, o, dinv, b.t, &h0, &h1)) {
+                    const float width = dice_width(min_width(w, h0, h1) * s.dice_rate, s.min_upoly);
+                    const float max_dim = longest_axis(sub(hi, lo));
+                    if (max_dim <= width || sidx + 1 == 63 || k + 1 == 62) {
+                        if (accept_leaf(h0, h1, (uv1[0] + uv1[1]) * 0.5f, (uv1[2] + uv1[3]) * 0.5f, max_dim, inst,
+                                        b) && occl)
+                            return true;
+                    } else {
+                        go1 = true;
+                        n1 = h0;
+                    }
+                }
+            }
+            {
+                V3 lo, hi;
+                patch_bound<N>(p, &lo, &hi);
+                float h0, h1;
+                if (bbox_ray(lo, hi, o, dinv, b.t, &h0, &h1)) {
+                    const float width = dice_width(min_width(w, h0, h1) * s.dice_rate, s.min_upoly);
+                    const float max_dim = longest_axis(sub(hi, lo));
+                    if (max_dim <= width || sidx == 63 || k + 1 == 62) {
+                        if (accept_leaf(h0, h1, (uv[0] + uv[1]) * 0.5f, (uv[2] + uv[3]) * 0.5f, max_dim, inst, b) &&
+                            occl)
+                            return true;
+                    } else {
+                        go0 = true;
+                        n0 = h0;
+                    }
+                }
+            }
+            if (go0 || go1) {
+                const bool take1 = go1 && (!go0 || n1 <= n0);
+                if (go0 && go1) trail |= 1ull << (k + 1);
+                pb = (pb << 1) | (take1 ? 1ull : 0ull);
+                ++k;
+                continue;
+            }
+        }
+        // the deepest pending sibling, re-tested against the best hit so far
+        if (trail == 0) return false;
+        const int d = 63 - __builtin_clzll(trail);
+        trail &= ~(1ull << d);
+        pb = (pb >> (k - d)) ^ 1ull;
+        k = d;
         Patch<N> p = base;
         float uv[4] = {uvb[0], uvb[1], uvb[2], uvb[3]};
         for (int l = k - 1; l >= 0; --l) patch_step<N>(p, ((pb >> l) & 1ull) != 0, uv);
         V3 lo, hi;
         patch_bound<N>(p, &lo, &hi);
         float h0, h1;
-        bool down = false;
+        expand = false;
         if (bbox_ray(lo, hi, o, dinv, b.t, &h0, &h1)) {
-            const float width = smax(min_width(w, h0, h1) * s.dice_rate, s.min_upoly);
+            const float width = dice_width(min_width(w, h0, h1) * s.dice_rate, s.min_upoly);
             const float max_dim = longest_axis(sub(hi, lo));
-            if (max_dim <= width || sidx == 63 || k == 62) {
+            if (max_dim <= width || sidx0 + __builtin_popcountll(pb) == 63 || k == 62) {
                 if (accept_leaf(h0, h1, (uv[0] + uv[1]) * 0.5f, (uv[2] + uv[3]) * 0.5f, max_dim, inst, b) && occl)
                     return true;
             } else {
-                down = true;
+                expand = true;
             }
         }
-        if (down) { ++k; pb = (pb << 1) | 1ull; ++sidx; continue; }
-        for (;;) {
-            if (k == 0) return false;
-            if (pb & 1ull) { pb ^= 1ull; --sidx; break; }
-            pb >>= 1; --k;
-        }
-        if (k == 0) return false;
     }
 }
 
-template <bool DEEP>
-PSY_DEV bool deep_or_flag(const DevScene& s, const DevObject& ob, uint32_t k, uint32_t inst, V3 o,
-                                             V3 dinv, const RayW& w, bool occl, Best& b, bool& need_deep) {
-    if (!DEEP) {
-        need_deep = true;
-        return false;
+// dice_deep_from below heap node k0 of a patch's split tree.
+template <int N>
+PSY_DEV bool dice_deep(const DevScene& s, const DevObject& ob, uint32_t k0, uint32_t inst, V3 o, V3 dinv,
+                       const RayW& w, bool occl, Best& b, DeepState& ds) {
+    Patch<N> base;
+    load_patch<N>(s.odata, ob.data, base);
+    float uvb[4];
+    patch_at_node<N>(base, k0, uvb);
+    return dice_deep_from<N>(s, base, uvb, __builtin_popcount(k0) - 1, inst, o, dinv, w, occl, b, ds);
+}
+
+// The wave's best of the lanes' hits: least t, the lowest lane on ties.
+// (Device only, as is everything of the cooperative walk.)
+__device__ inline void wave_best(Best& b) {
+    for (int off = 32; off >= 1; off >>= 1) {
+        const float t2 = __shfl_xor(b.t, off);
+        const float u2 = __shfl_xor(b.u, off);
+        const float v2 = __shfl_xor(b.v, off);
+        const float a2 = __shfl_xor(b.aux, off);
+        const uint32_t i2 = (uint32_t)__shfl_xor((int)b.inst, off);
+        const bool other_low = (__lane_id() & (uint32_t)off) != 0;  // the partner has the lower lane
+        if (t2 < b.t || (t2 == b.t && other_low)) {
+            b.t = t2; b.u = u2; b.v = v2; b.aux = a2; b.inst = i2;
+        }
     }
-    return ob.kind == K_BILINEAR ? dice_deep<4>(s, ob, k, inst, o, dinv, w, occl, b)
-                                 : dice_deep<16>(s, ob, k, inst, o, dinv, w, occl, b);
+}
+
+// LDS order between the lanes of one wave (no other wave shares the memory).
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// A node of the cooperative walk: its path below k0 under a leading one bit.
+template <int N>
+__device__ void coop_node_patch(const Patch<N>& base, const float uvb[4], uint64_t node, Patch<N>& p, float uv[4]) {
+    const int k = 63 - __builtin_clzll(node);
+    p = base;
+    uv[0] = uvb[0]; uv[1] = uvb[1]; uv[2] = uvb[2]; uv[3] = uvb[3];
+    for (int l = k - 1; l >= 0; --l) patch_step<N>(p, ((node >> l) & 1ull) != 0, uv);
+}
+
+// dice_deep with the 64 lanes of a wave on one ray (every lane calls it with
+// the same arguments): each round the lanes take up to 64 nodes off the
+// wave's LDS stack, split them, test both halves against the wave's best
+// hit, accept leaves and push the halves that need splitting (the nearer
+// one last, so it comes off first).  The closest hit is the same in any
+// visit order.  A half that finds the stack full is walked by its lane
+// alone.  Every lane returns the same result.
+template <int N>
+__device__ bool dice_deep_coop(const DevScene& s, const DevObject& ob, uint32_t k0, uint32_t inst, V3 o, V3 dinv,
+                            const RayW& w, bool occl, Best& b, uint64_t* stk) {
+    Patch<N> base;
+    load_patch<N>(s.odata, ob.data, base);
+    float uvb[4];
+    patch_at_node<N>(base, k0, uvb);
+    const int sidx0 = __builtin_popcount(k0) - 1;
+    const uint32_t lane = __lane_id();
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    if (lane == 0) stk[0] = 1ull;  // k0 itself
+    wave_lds_sync();
+    int count = 1;
+    while (count > 0) {
+        const int take = count < 64 ? count : 64;
+        const int base_i = count - take;
+        const bool active = (int)lane < take;
+        const uint64_t node = active ? stk[base_i + (int)lane] : 1ull;
+        count = base_i;
+        wave_lds_sync();
+        bool push0 = false, push1 = false, accepted = false;
+        uint64_t c0 = 0, c1 = 0;
+        uint64_t lower = 0, upper = 0;  // with both halves: the farther below, the nearer on top
+        if (active) {
+            const int k = 63 - __builtin_clzll(node);
+            Patch<N> p;
+            float uv[4];
+            coop_node_patch<N>(base, uvb, node, p, uv);
+            Patch<N> p1 = p;
+            float uv1[4] = {uv[0], uv[1], uv[2], uv[3]};
+            patch_step<N>(p1, true, uv1);
+            patch_step<N>(p, false, uv);
+            const int sidx = sidx0 + __builtin_popcountll(node) - 1;
+            float n0 = 0.0f, n1 = 0.0f;
+            {
+                V3 lo, hi;
+                patch_bound<N>(p1, &lo, &hi);
+                float h0, h1;
+                if (bbox_ray(lo, hi, o, dinv, b.t, &h0, &h1)) {
+                    const float width = dice_width(min_width(w, h0, h1) * s.dice_rate, s.min_upoly);
+                    const float max_dim = longest_axis(sub(hi, lo));
+                    if (max_dim <= width || sidx + 1 == 63 || k + 1 == 62) {
+                        accepted |= accept_leaf(h0, h1, (uv1[0] + uv1[1]) * 0.5f, (uv1[2] + uv1[3]) * 0.5f, max_dim,
+                                                inst, b);
+                    } else {
+                        push1 = true;
+                        n1 = h0;
+                        c1 = (node << 1) | 1ull;
+                    }
+                }
+            }
+            {
+                V3 lo, hi;
+                patch_bound<N>(p, &lo, &hi);
+                float h0, h1;
+                if (bbox_ray(lo, hi, o, dinv, b.t, &h0, &h1)) {
+                    const float width = dice_width(min_width(w, h0, h1) * s.dice_rate, s.min_upoly);
+                    const float max_dim = longest_axis(sub(hi, lo));
+                    if (max_dim <= width || sidx == 63 || k + 1 == 62) {
+                        accepted |= accept_leaf(h0, h1, (uv[0] + uv[1]) * 0.5f, (uv[2] + uv[3]) * 0.5f, max_dim,
+                                                inst, b);
+                    } else {
+                        push0 = true;
+                        n0 = h0;
+                        c0 = node << 1;
+                    }
+                }
+            }
+            // the second half counts as nearer on ties
+            if (push0 && push1) {
+                const bool second_nearer = n1 <= n0;
+                lower = second_nearer ? c0 : c1;
+                upper = second_nearer ? c1 : c0;
+            } else {
+                lower = push0 ? c0 : c1;
+            }
+        }
+        const bool pa = push0 || push1;
+        const bool pd = push0 && push1;
+        const unsigned long long ma = __ballot(pa), md = __ballot(pd);
+        const int na = __popcll(ma);
+        const int ia = count + __popcll(ma & lt);
+        const int ib = count + na + __popcll(md & lt);
+        DeepState solo{false, INT32_MAX, nullptr};
+        if (pa) {
+            if (ia < COOP_CAP) {
+                stk[ia] = lower;
+            } else {
+                Patch<N> q;
+                float uq[4];
+                coop_node_patch<N>(base, uvb, lower, q, uq);
+                accepted |= dice_deep_from<N>(s, q, uq, sidx0 + __builtin_popcountll(lower) - 1, inst, o, dinv, w,
+                                              occl, b, solo);
+            }
+        }
+        if (pd) {
+            if (ib < COOP_CAP) {
+                stk[ib] = upper;
+            } else {
+                Patch<N> q;
+                float uq[4];
+                coop_node_patch<N>(base, uvb, upper, q, uq);
+                accepted |= dice_deep_from<N>(s, q, uq, sidx0 + __builtin_popcountll(upper) - 1, inst, o, dinv, w,
+                                              occl, b, solo);
+            }
+        }
+        count = min(count + na + __popcll(md), COOP_CAP);
+        wave_lds_sync();
+        wave_best(b);
+        if (occl && __ballot(accepted) != 0ull) return true;
+    }
+    return false;
+}
+
+template <int DEEP>
+PSY_DEV bool deep_or_flag(const DevScene& s, const DevObject& ob, uint32_t k, uint32_t inst, V3 o,
+                                             V3 dinv, const RayW& w, bool occl, Best& b, DeepState& ds) {
+    if constexpr (DEEP == D_FLAG) {
+        ds.need = true;
+        return false;
+    } else if constexpr (DEEP == D_COOP) {
+        return ob.kind == K_BILINEAR ? dice_deep_coop<4>(s, ob, k, inst, o, dinv, w, occl, b, ds.coop)
+                                     : dice_deep_coop<16>(s, ob, k, inst, o, dinv, w, occl, b, ds.coop);
+    } else {
+        return ob.kind == K_BILINEAR ? dice_deep<4>(s, ob, k, inst, o, dinv, w, occl, b, ds)
+                                     : dice_deep<16>(s, ob, k, inst, o, dinv, w, occl, b, ds);
+    }
 }
 
 // Walk one patch's cached split tree, nearest child first, without a
@@ -191,9 +434,9 @@ PSY_DEV bool deep_or_flag(const DevScene& s, const DevObject& ob, uint32_t k, ui
 // Any visit order finds the reference's closest hit (the leaf t does not
 // depend on max_t); within a node the second half is taken on ties, as the
 // reference visits it first.  Returns true when an occlusion ray is done.
-template <bool DEEP>
+template <int DEEP>
 PSY_DEV bool dice_walk(const DevScene& s, const DevObject& ob, uint32_t inst, V3 o, V3 dinv,
-                                          const RayW& w, bool occl, Best& b, bool& need_deep) {
+                                          const RayW& w, bool occl, Best& b, DeepState& need_deep) {
     const float4* nodes = s.dice + 2 * ob.dice;
     const int levels = ob.dice_levels;
     {
@@ -201,7 +444,7 @@ PSY_DEV bool dice_walk(const DevScene& s, const DevObject& ob, uint32_t inst, V3
         float h0, h1;
         if (!bbox_ray(xyz(a), xyz(c), o, dinv, b.t, &h0, &h1)) return false;
         const float maxd = longest_axis(sub(xyz(c), xyz(a)));
-        if (maxd <= smax(min_width(w, h0, h1) * s.dice_rate, s.min_upoly))
+        if (maxd <= dice_width(min_width(w, h0, h1) * s.dice_rate, s.min_upoly))
             return accept_leaf(h0, h1, a.w, c.w, maxd, inst, b) && occl;
         if (levels == 0) return deep_or_flag<DEEP>(s, ob, 1, inst, o, dinv, w, occl, b, need_deep);
     }
@@ -219,7 +462,7 @@ PSY_DEV bool dice_walk(const DevScene& s, const DevObject& ob, uint32_t inst, V3
         bool split0 = false, split1 = false;
         if (hit1) {
             const float maxd = longest_axis(sub(xyz(e1), xyz(a1)));
-            if (maxd <= smax(min_width(w, h10, h11) * s.dice_rate, s.min_upoly)) {
+            if (maxd <= dice_width(min_width(w, h10, h11) * s.dice_rate, s.min_upoly)) {
                 if (accept_leaf(h10, h11, a1.w, e1.w, maxd, inst, b) && occl) return true;
             } else {
                 split1 = true;
@@ -227,7 +470,7 @@ PSY_DEV bool dice_walk(const DevScene& s, const DevObject& ob, uint32_t inst, V3
         }
         if (hit0) {
             const float maxd = longest_axis(sub(xyz(e0), xyz(a0)));
-            if (maxd <= smax(min_width(w, h00, h01) * s.dice_rate, s.min_upoly)) {
+            if (maxd <= dice_width(min_width(w, h00, h01) * s.dice_rate, s.min_upoly)) {
                 if (accept_leaf(h00, h01, a0.w, e0.w, maxd, inst, b) && occl) return true;
             } else {
                 split0 = true;
@@ -263,7 +506,7 @@ PSY_DEV bool dice_walk(const DevScene& s, const DevObject& ob, uint32_t inst, V3
             float h0, h1;
             if (!bbox_ray(xyz(a), xyz(c), o, dinv, b.t, &h0, &h1)) continue;
             const float maxd = longest_axis(sub(xyz(c), xyz(a)));
-            if (maxd <= smax(min_width(w, h0, h1) * s.dice_rate, s.min_upoly)) {
+            if (maxd <= dice_width(min_width(w, h0, h1) * s.dice_rate, s.min_upoly)) {
                 if (accept_leaf(h0, h1, a.w, c.w, maxd, inst, b) && occl) return true;
                 continue;
             }
@@ -344,9 +587,9 @@ struct TRay {
     bool occl;
 };
 
-template <bool WORLD, bool DEEP>
+template <bool WORLD, int DEEP>
 PSY_DEV bool intersect_leaf(const DevScene& s, uint32_t li, const DevInstance& I, const TRay& r,
-                                               Best& b, bool& need_deep) {
+                                               Best& b, DeepState& need_deep) {
     const DevObject ob = s.obj[I.object];
     V3 o = r.o, d = r.d, dinv = r.dinv;
     RayW w = r.w;
@@ -467,11 +710,14 @@ PSY_DEV void load_ray(const float* __restrict__ rays, int64_t n, int64_t i, uint
 
 
 // One ray: BVH4 nearest-first with an LDS stack, leaves through
-// intersect_leaf.  Returns true if the ray needs the below-cache pass.
-template <bool WORLD, int STACK, bool DEEP>
+// intersect_leaf, below-cache splits per DEEP (D_SERIAL within `budget`
+// node expansions, D_COOP with the wave's stack `coop`).  Returns true if
+// the ray is flagged for the next pass.
+template <bool WORLD, int STACK, int DEEP>
 PSY_DEV bool trace_one(const DevScene& s, const float* __restrict__ rays, int64_t n, int64_t i,
                                           const DevHits& h, uint32_t flags, int32_t (*stk_ref)[TRACE_BLOCK],
-                                          float (*stk_near)[TRACE_BLOCK], int tid) {
+                                          float (*stk_near)[TRACE_BLOCK], int tid, int32_t budget = INT32_MAX,
+                                          uint64_t* coop = nullptr) {
     TRay r;
     float max_t;
     load_ray<WORLD>(rays, n, i, flags, r, max_t);
@@ -480,7 +726,7 @@ PSY_DEV bool trace_one(const DevScene& s, const float* __restrict__ rays, int64_
     b.t = max_t;
     b.inst = 0xffffffffu;
     b.u = b.v = b.aux = 0.0f;
-    bool need_deep = false;
+    DeepState need_deep{false, budget, coop};
 
     int sp = 0;
     if (s.root != CHILD_EMPTY) {
@@ -542,7 +788,7 @@ PSY_DEV bool trace_one(const DevScene& s, const float* __restrict__ rays, int64_
     h.u[i] = b.u;
     h.v[i] = b.v;
     if (h.aux) h.aux[i] = b.aux;
-    return need_deep && !(r.occl && hit);
+    return need_deep.need && !(r.occl && hit);
 }
 
 // ---------------------------------------------------------------------------
@@ -563,6 +809,10 @@ PSY_DEV bool trace_one(const DevScene& s, const float* __restrict__ rays, int64_
 #define PSY_MAIN_WAVES_PER_SIMD 5
 #endif
 constexpr int MAIN_STACK = PSY_MAIN_STACK;
+// The main pass also comes with a 32-entry stack at lower occupancy (fewer
+// rays handed to the second pass on deep hierarchies); waves per SIMD of
+// each form, as launch bounds.
+constexpr int main_waves(bool world, int stack) { return stack <= 16 ? (world ? 4 : PSY_MAIN_WAVES_PER_SIMD) : 2; }
 
 struct Lane {
     TRay r;
@@ -636,7 +886,7 @@ PSY_DEV void lane_start(const DevScene& s, Lane& L, float max_t, uint32_t i) {
 //             expanded; its children lie inside it, so expanding it without
 //             a re-test finds the same hits)
 //   else    : one BVH4 node, or one leaf instance
-template <bool WORLD>
+template <bool WORLD, int STACK = MAIN_STACK>
 PSY_DEV bool lane_step(const DevScene& s, Lane& L, int32_t (*stk_ref)[TRACE_BLOCK],
                                           float (*stk_near)[TRACE_BLOCK], int tid) {
     if (L.state & LANE_WALKING) {
@@ -652,7 +902,7 @@ PSY_DEV bool lane_step(const DevScene& s, Lane& L, int32_t (*stk_ref)[TRACE_BLOC
         // the second half first: the reference visits it first on ties
         if (hit1) {
             const float maxd = longest_axis(sub(xyz(e1), xyz(a1)));
-            if (maxd <= smax(min_width_flip(ww, L.wfx, L.wfy, h10, h11) * s.dice_rate, s.min_upoly)) {
+            if (maxd <= dice_width(min_width_flip(ww, L.wfx, L.wfy, h10, h11) * s.dice_rate, s.min_upoly)) {
                 if (accept_leaf(h10, h11, a1.w, e1.w, maxd, L.winst, L.b) && L.r.occl) return true;
             } else {
                 split1 = true;
@@ -660,7 +910,7 @@ PSY_DEV bool lane_step(const DevScene& s, Lane& L, int32_t (*stk_ref)[TRACE_BLOC
         }
         if (hit0) {
             const float maxd = longest_axis(sub(xyz(e0), xyz(a0)));
-            if (maxd <= smax(min_width_flip(ww, L.wfx, L.wfy, h00, h01) * s.dice_rate, s.min_upoly)) {
+            if (maxd <= dice_width(min_width_flip(ww, L.wfx, L.wfy, h00, h01) * s.dice_rate, s.min_upoly)) {
                 if (accept_leaf(h00, h01, a0.w, e0.w, maxd, L.winst, L.b) && L.r.occl) return true;
             } else {
                 split0 = true;
@@ -735,7 +985,7 @@ PSY_DEV bool lane_step(const DevScene& s, Lane& L, int32_t (*stk_ref)[TRACE_BLOC
         cmpswap(k1, r1, k3, r3);
         cmpswap(k1, r1, k2, r2);
         const int npush = (k1 < INFINITY ? 1 : 0) + (k2 < INFINITY ? 1 : 0) + (k3 < INFINITY ? 1 : 0);
-        if (L.sp + npush > MAIN_STACK) {
+        if (L.sp + npush > STACK) {
             L.state |= LANE_RETRACE;
             return true;
         }
@@ -783,8 +1033,8 @@ PSY_DEV bool lane_step(const DevScene& s, Lane& L, int32_t (*stk_ref)[TRACE_BLOC
         L.state |= LANE_WALKING;
         return false;
     }
-    bool unused = false;
-    return intersect_leaf<WORLD, false>(s, li, I, L.r, L.b, unused);
+    DeepState unused{false, 0, nullptr};
+    return intersect_leaf<WORLD, D_FLAG>(s, li, I, L.r, L.b, unused);
 }
 
 // The lane's final hit (t, inst, u, v, aux) in place in L.b; true if the

@@ -48,6 +48,9 @@ def lib():
     L.psy_world_to_rays.argtypes = [vp, vp, i64, vp, vp]
     L.psy_halton.argtypes = [vp, u32, vp, i64, vp, vp]
     L.psy_ctx_set_trace_timing.argtypes = [vp, C.c_int]
+    L.psy_ctx_set_trace_stack.argtypes = [vp, C.c_int]
+    L.psy_ctx_set_stream_steps.argtypes = [vp, C.c_int]
+    L.psy_render_autotune.argtypes = [vp, vp, C.POINTER(abi.RenderParams), i64, C.POINTER(abi.RenderTuning)]
     L.psy_ctx_trace_timing.argtypes = [vp, vp, vp, vp]
     L.psy_sobol_set_matrices.argtypes = [vp, vp, u32]
     L.psy_sobol.argtypes = [vp, u32, vp, i64, C.c_uint64, vp, vp]
@@ -92,6 +95,10 @@ class Context:
         """psy_ctx_set_trace_mode: persistent main pass with lane refill, or one thread per ray."""
         mode = abi.PSY_TRACE_MODE_PERSISTENT if persistent else abi.PSY_TRACE_MODE_PLAIN
         check(lib().psy_ctx_set_trace_mode(self.handle, mode, refill_min, blocks_per_cu), "psy_ctx_set_trace_mode")
+
+    def set_trace_stack(self, entries):
+        """psy_ctx_set_trace_stack: main-pass LDS stack entries (16 or 32)."""
+        check(lib().psy_ctx_set_trace_stack(self.handle, int(entries)), "psy_ctx_set_trace_stack")
 
     def set_trace_timing(self, enable=True):
         """psy_ctx_set_trace_timing: HIP events around each trace call's passes."""
@@ -328,7 +335,23 @@ def set_render_chunk(ctx, paths):
     check(lib().psy_ctx_set_render_chunk(ctx.handle, int(paths)), "psy_ctx_set_render_chunk")
 
 
-def set_render_mode(ctx, fused=False, refill_min=16, blocks_per_cu=0):
-    """psy_ctx_set_render_mode: wavefront (default) or fused persistent paths."""
-    mode = abi.PSY_RENDER_FUSED if fused else abi.PSY_RENDER_WAVEFRONT
+def set_render_mode(ctx, fused=False, refill_min=16, blocks_per_cu=0, stream=False):
+    """psy_ctx_set_render_mode: wavefront (default), fused persistent paths,
+    or the stream form (stream=True)."""
+    mode = abi.PSY_RENDER_STREAM if stream else (abi.PSY_RENDER_FUSED if fused else abi.PSY_RENDER_WAVEFRONT)
     check(lib().psy_ctx_set_render_mode(ctx.handle, mode, refill_min, blocks_per_cu), "psy_ctx_set_render_mode")
+
+
+def set_stream_steps(ctx, steps):
+    """psy_ctx_set_stream_steps: steps per captured graph (even, 2..64)."""
+    check(lib().psy_ctx_set_stream_steps(ctx.handle, int(steps)), "psy_ctx_set_stream_steps")
+
+
+def autotune_render(ctx, scene, width, height, spp, seed=1, path_length=4, max_samples=0):
+    """psy_render_autotune: leaves the fastest stream-form setting on ctx and
+    returns it as a dict."""
+    prm = abi.RenderParams(width, height, spp, seed, path_length, 0)
+    out = abi.RenderTuning()
+    check(lib().psy_render_autotune(ctx.handle, scene.handle, C.byref(prm), int(max_samples), C.byref(out)),
+          "psy_render_autotune")
+    return {f: getattr(out, f) for f, _ in abi.RenderTuning._fields_}

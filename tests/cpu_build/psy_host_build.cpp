@@ -80,7 +80,7 @@ bool trace_ray(const DevScene& s, const float* rays, int64_t rs, int64_t i, uint
     while (!lane_step<W>(s, L, stk_ref, stk_near, 0)) {
     }
     if (!lane_finish(s, L, h)) return false;
-    (void)trace_one<W, 64, true>(s, rays, rs, i, h, flags, stk_ref, stk_near, 0);
+    (void)trace_one<W, 64, D_SERIAL>(s, rays, rs, i, h, flags, stk_ref, stk_near, 0, INT32_MAX);
     return true;
 }
 

@@ -146,3 +146,51 @@ def test_gpu_render_fused_equals_wavefront(gpu_ctx, name):
             assert torch.equal(s_f, s_wf)
     finally:
         ctx.close()
+
+
+@pytest.mark.parametrize("name", ["spheres", "heightfield", "instanced_lights"])
+def test_gpu_render_stream_equals_wavefront(gpu_ctx, name):
+    """The stream form (refilled path pool, device-side stream lengths,
+    graph-captured steps) gives the wavefront's samples bit for bit, for any
+    pool size, steps per graph, main-pass launch form and stack depth."""
+    sc = {"spheres": configs.sphere_scene, "heightfield": lambda: configs.heightfield_scene(grid=16),
+          "instanced_lights": _instanced_lights_scene}[name]()
+    g = gpu.GpuScene(gpu_ctx, sc)
+    f_wf, s_wf, r_wf = gpu.render(gpu_ctx, g, 40, 24, 5, 3, film=True, samples=True)
+    ctx = gpu.Context(0)
+    try:
+        gpu.set_render_mode(ctx, stream=True)
+        for pool, steps, persistent, stack in ((1 << 21, 4, False, 16), (40 * 5, 2, True, 32), (35, 8, False, 16),
+                                               (64, 6, True, 16)):
+            gpu.set_render_chunk(ctx, pool)
+            gpu.set_stream_steps(ctx, steps)
+            ctx.set_trace_mode(persistent=persistent, refill_min=16)
+            ctx.set_trace_stack(stack)
+            f_s, s_s, r_s = gpu.render(ctx, g, 40, 24, 5, 3, film=True, samples=True)
+            say(name, "stream pool", pool, "steps", steps, "persistent", persistent, "stack", stack, "rays", r_s, r_wf)
+            assert r_s == r_wf
+            assert torch.equal(s_s, s_wf)
+            assert torch.equal(f_s, f_wf)
+        # film only: the library keeps the samples itself
+        f_s, _, r_s = gpu.render(ctx, g, 40, 24, 5, 3, film=True, samples=False)
+        assert r_s == r_wf and torch.equal(f_s, f_wf)
+    finally:
+        ctx.close()
+
+
+def test_gpu_render_autotune(gpu_ctx):
+    """psy_render_autotune leaves a stream-form setting on the context that
+    renders the same image."""
+    sc = configs.heightfield_scene(grid=16)
+    g = gpu.GpuScene(gpu_ctx, sc)
+    f_wf, _, r_wf = gpu.render(gpu_ctx, g, 64, 48, 4, 1)
+    ctx = gpu.Context(0)
+    try:
+        t = gpu.autotune_render(ctx, g, 64, 48, 4, max_samples=64 * 48 * 2)
+        say("autotune", t)
+        assert t["mode"] == abi.PSY_RENDER_STREAM and t["trials"] >= 4 and t["mrays_per_s"] > 0
+        assert t["stack"] in (16, 32) and t["steps_per_graph"] % 2 == 0
+        f_s, _, r_s = gpu.render(ctx, g, 64, 48, 4, 1)
+        assert r_s == r_wf and torch.equal(f_s, f_wf)
+    finally:
+        ctx.close()

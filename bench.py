@@ -88,6 +88,99 @@ def traffic_from_profile():
     return None
 
 
+REF_BUCKET_PATHS = 65536   # Config::samples_per_bucket (config.cpp:12; integrator/path_trace_integrator.cpp:29-34)
+
+
+def _render_rate(gpu, ctx, g, W, H, spp, pl, reps=2):
+    """(Mrays/s, film) of psy_render with the context's current settings."""
+    import torch
+    film, _, rays = gpu.render(ctx, g, W, H, spp, 1, pl)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        film, _, rays = gpu.render(ctx, g, W, H, spp, 1, pl)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    return rays / dt / 1e6, film
+
+
+def batching_forms():
+    """The batched / autotuned integrator against the plain wavefront: the
+    same images (bit-identical samples), Mrays/s of each form."""
+    import torch
+    from psychopath_cpp_amd import configs, gpu
+    out = []
+    for name, make, W, H, spp, pl in (
+            ("heightfield 64x64 bilinear patches (dice_rate 1.0), 1024x1024, 16 spp, path length 4",
+             configs.heightfield_scene, 1024, 1024, 16, 4),
+            ("4096 bicubic patches on a sphere instanced 16x (2 levels, 2-sample motion), DoF 0.05, 480x270, "
+             "4 spp, path length 6", configs.bicubic_sphere_scene, 480, 270, 4, 6)):
+        ctx = gpu.Context(0)
+        g = gpu.GpuScene(ctx, make())
+        gpu.set_render_mode(ctx)
+        gpu.set_render_chunk(ctx, REF_BUCKET_PATHS)
+        plain, f_plain = _render_rate(gpu, ctx, g, W, H, spp, pl, reps=1)
+        log(f"batching [{name[:40]}]: plain wavefront, {REF_BUCKET_PATHS}-path buckets: {plain:.1f} Mrays/s")
+        gpu.set_render_chunk(ctx, 1 << 22)
+        wide, f_wide = _render_rate(gpu, ctx, g, W, H, spp, pl)
+        log(f"batching: wavefront, 2^22-path chunks: {wide:.1f} Mrays/s")
+        t0 = time.perf_counter()
+        tune = gpu.autotune_render(ctx, g, W, H, spp, path_length=pl)
+        tune_s = time.perf_counter() - t0
+        tuned, f_tuned = _render_rate(gpu, ctx, g, W, H, spp, pl)
+        log(f"batching: autotuned {tune} in {tune_s:.2f} s: {tuned:.1f} Mrays/s")
+        same = bool(torch.equal(f_plain, f_tuned) and torch.equal(f_plain, f_wide))
+        out.append({"workload": name, "plain_mrays": round(plain, 2),
+                    "plain_form": f"wavefront, {REF_BUCKET_PATHS}-path buckets, host sync per step",
+                    "wavefront_2p22_mrays": round(wide, 2), "autotuned_mrays": round(tuned, 2),
+                    "autotuned": {"form": "stream" if tune["mode"] == 2 else "wavefront", "paths": int(tune["pool"]),
+                                  "trace": "persistent" if tune["trace_mode"] == 1 else "plain",
+                                  "stack": int(tune["stack"]), "steps_per_graph": int(tune["steps_per_graph"]),
+                                  "tune_seconds": round(tune_s, 2)},
+                    "speedup_vs_plain": round(tuned / plain, 2), "speedup_vs_2p22_wavefront": round(tuned / wide, 2),
+                    "images_identical": same})
+        g.close()
+        ctx.close()
+    return out
+
+
+def adjoint_timing():
+    """Adjoint / forward time of the differentiable render."""
+    import numpy as np
+    import torch
+    from psychopath_cpp_amd import configs, gpu
+    ctx = gpu.Context(0)
+    sc = configs.adjoint_scene()
+    g = gpu.GpuScene(ctx, sc)
+    rng = np.random.default_rng(1)
+    alb0, _ = g.get_params()
+    n_l = g.param_counts()[1]
+    target, _, _ = gpu.render(ctx, g, 512, 512, 16, 1)
+    g.set_params((alb0 * rng.uniform(0.5, 1.5, alb0.size)).astype(np.float32),
+                 rng.uniform(0.5, 1.5, n_l).astype(np.float32))
+    film, _, _ = gpu.render(ctx, g, 512, 512, 16, 1)
+    d = (2.0 * (film - target)).float().contiguous()
+    gpu.adjoint(ctx, g, d, 16, 1)
+    torch.cuda.synchronize()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        gpu.render(ctx, g, 512, 512, 16, 1)
+    torch.cuda.synchronize()
+    tf = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        gpu.adjoint(ctx, g, d, 16, 1)
+    torch.cuda.synchronize()
+    ta = (time.perf_counter() - t0) / reps
+    log(f"adjoint: forward {tf * 1e3:.2f} ms, adjoint {ta * 1e3:.2f} ms, ratio {ta / tf:.2f}")
+    g.close()
+    ctx.close()
+    return {"workload": "256 spheres + ground, 1000 sphere lights, 512x512, 16 spp, path length 4; gradients of an "
+                        "L2 image loss w.r.t. 257 albedos and 1000 emission scales",
+            "forward_ms": round(tf * 1e3, 3), "adjoint_ms": round(ta * 1e3, 3), "ratio": round(ta / tf, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +189,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1 << 21)
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="minimum CPU baseline time (repeats the sample)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-forms", action="store_true", help="skip the batching and adjoint measurements")
     ap.add_argument("--dice-depth", type=int, default=-1)
     ap.add_argument("--trace-mode", choices=("persistent", "plain"), default="plain",
                     help="main trace pass: persistent waves with lane refill, or one thread per ray")
@@ -232,7 +326,7 @@ def main():
             "trace_mode": args.trace_mode,
         },
         "roofline": {
-            "kernel": "k_trace<false> (main pass, %.3f ms/launch)" % main_avg,
+            "kernel": "k_trace<false, 16> (main pass, %.3f ms/launch)" % main_avg,
             "bound": "hbm",
             "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS,
@@ -242,6 +336,9 @@ def main():
         },
         "cpu_baseline": cpu,
     }
+    if not args.no_forms:
+        out["batching"] = batching_forms()
+        out["adjoint"] = adjoint_timing()
     print(json.dumps(out), flush=True)
     return 0
 

@@ -268,14 +268,15 @@ int psy_render(psy_ctx* ctx, const psy_scene* scene, const psy_render_params* pa
                float* film_xyz, float* sample_xyz, int64_t* rays_traced, void* stream);
 
 /* The render autotuner (no counterpart in the reference): times psy_render
- * in PSY_RENDER_STREAM form on every pixel at an spp that keeps to
- * max_samples samples (<= 0: 2^23), by coordinate descent over the pool
- * size (render chunk, 2^19..2^23), the main pass's launch form
- * (psy_ctx_set_trace_mode), its LDS stack (psy_ctx_set_trace_stack) and
- * the steps per graph (psy_ctx_set_stream_steps), and leaves the fastest
- * setting on the context.  `out` (or NULL) receives it. */
+ * on every pixel at an spp that keeps to max_samples samples (<= 0: 2^23),
+ * by coordinate descent over the render form (PSY_RENDER_STREAM or
+ * PSY_RENDER_WAVEFRONT) with its paths in flight (render chunk,
+ * 2^19..2^23), the main pass's launch form (psy_ctx_set_trace_mode), its
+ * LDS stack (psy_ctx_set_trace_stack) and the stream form's steps per graph
+ * (psy_ctx_set_stream_steps), and leaves the fastest setting on the
+ * context.  `out` (or NULL) receives it. */
 typedef struct psy_render_tuning {
-    int32_t mode;             /* PSY_RENDER_STREAM                       */
+    int32_t mode;             /* PSY_RENDER_STREAM or PSY_RENDER_WAVEFRONT */
     int32_t trace_mode;       /* PSY_TRACE_MODE_*                        */
     int32_t refill_min;
     int32_t stack;            /* main-pass stack entries                 */

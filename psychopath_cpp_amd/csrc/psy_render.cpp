@@ -661,10 +661,11 @@ int psy_render(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, f
     return run_wavefront(ctx, s, prm, R, U, film_xyz, sample_xyz, nullptr, rays_traced, st);
 }
 
-// The render autotuner: coordinate descent over the stream form's pool size,
-// the main pass's launch form and LDS stack depth, and the steps per
-// captured graph, timing psy_render on a sample budget of the image (every
-// pixel at reduced spp).  Leaves the best setting on the context.
+// The render autotuner: coordinate descent over the render form and its
+// paths in flight (wavefront chunk or stream pool), the main pass's launch
+// form and LDS stack depth, and the stream form's steps per captured graph,
+// timing psy_render on a sample budget of the image (every pixel at reduced
+// spp).  Leaves the best setting on the context.
 int psy_render_autotune(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, int64_t max_samples,
                         psy_render_tuning* out) {
     if (int rc = check_render(ctx, s, prm, "psy_render_autotune")) return rc;
@@ -678,9 +679,9 @@ int psy_render_autotune(psy_ctx* ctx, const psy_scene* s, const psy_render_param
     const int s_mode = ctx->render_mode, s_tmode = ctx->mode, s_refill = ctx->refill_min, s_stack = ctx->trace_stack,
               s_iters = ctx->render_graph_iters;
     const int64_t s_chunk = ctx->render_chunk;
-    struct Cfg { int64_t pool; int tmode, refill, stack, iters; };
+    struct Cfg { int mode; int64_t pool; int tmode, refill, stack, iters; };
     auto apply = [&](const Cfg& c) {
-        ctx->render_mode = PSY_RENDER_STREAM;
+        ctx->render_mode = c.mode;
         ctx->render_chunk = c.pool;
         ctx->mode = c.tmode;
         ctx->refill_min = c.refill;
@@ -703,7 +704,7 @@ int psy_render_autotune(psy_ctx* ctx, const psy_scene* s, const psy_render_param
         ++trials;
         return PSY_OK;
     };
-    Cfg best{std::min<int64_t>((int64_t)1 << 21, total), PSY_TRACE_MODE_PLAIN, 16, 16, 4};
+    Cfg best{PSY_RENDER_STREAM, std::min<int64_t>((int64_t)1 << 21, total), PSY_TRACE_MODE_PLAIN, 16, 16, 4};
     double best_m = 0.0;
     int rc = timed(best, &best_m);
     auto consider = [&](Cfg c) -> int {
@@ -712,15 +713,18 @@ int psy_render_autotune(psy_ctx* ctx, const psy_scene* s, const psy_render_param
         if (m > best_m) { best_m = m; best = c; }
         return PSY_OK;
     };
-    // pool size: powers of two from 2^19, up to the workload
-    for (int64_t pool = (int64_t)1 << 19; !rc && pool <= ((int64_t)1 << 23); pool <<= 1) {
-        const int64_t c = std::min(pool, total);
-        if (c != best.pool) { Cfg t = best; t.pool = c; rc = consider(t); }
-        if (c == total) break;
+    // form and paths in flight: powers of two from 2^19, up to the workload
+    for (int mode : {PSY_RENDER_STREAM, PSY_RENDER_WAVEFRONT}) {
+        for (int64_t pool = (int64_t)1 << 19; !rc && pool <= ((int64_t)1 << 23); pool <<= 1) {
+            const int64_t c = std::min(pool, total);
+            if (mode != best.mode || c != best.pool) { Cfg t = best; t.mode = mode; t.pool = c; rc = consider(t); }
+            if (c == total) break;
+        }
     }
     if (!rc) { Cfg t = best; t.tmode = PSY_TRACE_MODE_PERSISTENT; t.refill = 32; rc = consider(t); }
     if (!rc) { Cfg t = best; t.stack = 32; rc = consider(t); }
-    for (int it : {2, 8, 16}) if (!rc && it != best.iters) { Cfg t = best; t.iters = it; rc = consider(t); }
+    if (best.mode == PSY_RENDER_STREAM)
+        for (int it : {2, 8, 16}) if (!rc && it != best.iters) { Cfg t = best; t.iters = it; rc = consider(t); }
     if (rc) {
         ctx->render_mode = s_mode; ctx->mode = s_tmode; ctx->refill_min = s_refill; ctx->trace_stack = s_stack;
         ctx->render_graph_iters = s_iters; ctx->render_chunk = s_chunk;
@@ -728,7 +732,7 @@ int psy_render_autotune(psy_ctx* ctx, const psy_scene* s, const psy_render_param
     }
     apply(best);
     if (out) {
-        out->mode = PSY_RENDER_STREAM;
+        out->mode = best.mode;
         out->trace_mode = best.tmode;
         out->refill_min = best.refill;
         out->stack = best.stack;

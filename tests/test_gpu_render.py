@@ -179,8 +179,8 @@ def test_gpu_render_stream_equals_wavefront(gpu_ctx, name):
 
 
 def test_gpu_render_autotune(gpu_ctx):
-    """psy_render_autotune leaves a stream-form setting on the context that
-    renders the same image."""
+    """psy_render_autotune leaves a setting on the context that renders the
+    same image."""
     sc = configs.heightfield_scene(grid=16)
     g = gpu.GpuScene(gpu_ctx, sc)
     f_wf, _, r_wf = gpu.render(gpu_ctx, g, 64, 48, 4, 1)
@@ -188,7 +188,8 @@ def test_gpu_render_autotune(gpu_ctx):
     try:
         t = gpu.autotune_render(ctx, g, 64, 48, 4, max_samples=64 * 48 * 2)
         say("autotune", t)
-        assert t["mode"] == abi.PSY_RENDER_STREAM and t["trials"] >= 4 and t["mrays_per_s"] > 0
+        assert t["mode"] in (abi.PSY_RENDER_STREAM, abi.PSY_RENDER_WAVEFRONT)
+        assert t["trials"] >= 4 and t["mrays_per_s"] > 0
         assert t["stack"] in (16, 32) and t["steps_per_graph"] % 2 == 0
         f_s, _, r_s = gpu.render(ctx, g, 64, 48, 4, 1)
         assert r_s == r_wf and torch.equal(f_s, f_wf)

@@ -204,6 +204,13 @@ int psy_ctx_retrace_count(psy_ctx* ctx, int64_t* out);
  * since the previous query, then forgets them. */
 int psy_ctx_set_trace_timing(psy_ctx* ctx, int enable);
 int psy_ctx_trace_timing(psy_ctx* ctx, double* main_ms, double* second_ms, int64_t* calls);
+/* Device memory for callers that do not link HIP themselves (the C++ host
+ * layer, include/psychopath_gpu.hpp): plain allocation on the context's
+ * device and synchronous copies.  No counterpart in the reference. */
+int psy_malloc(psy_ctx* ctx, size_t bytes, void** out);
+int psy_free(psy_ctx* ctx, void* p);
+int psy_copy_to_device(psy_ctx* ctx, void* dst, const void* src, size_t bytes);
+int psy_copy_to_host(psy_ctx* ctx, void* dst, const void* src, size_t bytes);
 const char* psy_last_error(void);
 int psy_abi_version(void);
 

@@ -1,0 +1,587 @@
+/*
+ * psychopath_gpu.hpp -- the C++ host interface of the MI355X ray-stream
+ * library: the reference's scene / tracer / integrator classes, kept by
+ * name, argument meaning and ownership, over the C ABI of psy_gpu.h.
+ * Header only; link libpsy_gpu.so.  No HIP header is needed: device
+ * memory goes through psy_malloc / psy_copy_*.
+ *
+ *   Assembly            scene/assembly.hpp:74-330
+ *   Scene               scene/scene.hpp:33-53
+ *   Camera              basics/camera.hpp:14-105
+ *   Sphere, Bilinear, Bicubic, SphereLight, RectangleLight
+ *                       object/sphere.hpp, object/bilinear.hpp,
+ *                       object/bicubic.hpp, lights/sphere_light.hpp,
+ *                       lights/rectangle_light.hpp
+ *   LambertShader, EmitShader
+ *                       shading/surface_shader.hpp:28-53
+ *   WorldRay            basics/ray.hpp:162-185
+ *   Intersection        basics/intersection.hpp:22-49 (closest-hit part)
+ *   Tracer::trace       tracer/tracer.hpp:57-76, tracer/tracer.cpp:38-121
+ *   Film                film/film.hpp:78-153 (per-pixel CIE XYZ means)
+ *   PathTraceIntegrator integrator/path_trace_integrator.hpp:103-120
+ *
+ * Differences a caller of the reference should know:
+ *   - Scene::finalize() builds the scene on the GPU (flattened instance
+ *     BVH, dice cache, light trees) and needs a device; failures throw
+ *     std::runtime_error with psy_last_error()'s text (the reference has
+ *     no error path);
+ *   - Intersection carries the closest hit (t, InstanceID, surface u/v,
+ *     the flattened leaf instance, the patch leaf extent) rather than a
+ *     vtable'd closure: shading runs on the device inside
+ *     PathTraceIntegrator;
+ *   - colours are grey (r == g == b) for rendering; the tracer takes any.
+ */
+#ifndef PSYCHOPATH_GPU_HPP
+#define PSYCHOPATH_GPU_HPP
+
+#include <array>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <utility>
+#include <vector>
+
+#include "psy_gpu.h"
+
+namespace psy {
+
+inline void check(int rc, const char* what) {
+    if (rc != PSY_OK) throw std::runtime_error(std::string(what) + ": " + psy_last_error());
+}
+
+struct Vec3 {
+    float x = 0.0f, y = 0.0f, z = 0.0f;
+};
+
+struct Color {
+    float r = 0.0f, g = 0.0f, b = 0.0f;
+    Color() = default;
+    explicit Color(float v) : r(v), g(v), b(v) {}
+    Color(float r_, float g_, float b_) : r(r_), g(g_), b(b_) {}
+};
+
+// Row-major 4x4 in the reference's row-vector convention (p' = p * M),
+// the layout of Matrix44::data (math/matrix.hpp:18-22).
+struct Transform {
+    std::array<float, 16> m{{1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1}};
+    Transform() = default;
+    explicit Transform(const float* d) { std::memcpy(m.data(), d, sizeof(float) * 16); }
+};
+
+// ---- objects ----------------------------------------------------------------
+class Object {
+public:
+    virtual ~Object() = default;
+    virtual int kind() const = 0;
+    virtual int time_samples() const = 0;
+    virtual std::vector<float> data() const = 0;  // time_samples * floats-per-kind
+};
+
+class Sphere : public Object {
+public:
+    Sphere(Vec3 center, float radius) : c_{center}, r_{radius} {}
+    Sphere(std::vector<Vec3> centers, std::vector<float> radii) : c_(std::move(centers)), r_(std::move(radii)) {
+        if (c_.empty() || c_.size() != r_.size()) throw std::invalid_argument("Sphere: one radius per center");
+    }
+    int kind() const override { return PSY_OBJ_SPHERE; }
+    int time_samples() const override { return (int)c_.size(); }
+    std::vector<float> data() const override {
+        std::vector<float> d;
+        for (size_t i = 0; i < c_.size(); ++i) d.insert(d.end(), {c_[i].x, c_[i].y, c_[i].z, r_[i]});
+        return d;
+    }
+
+private:
+    std::vector<Vec3> c_;
+    std::vector<float> r_;
+};
+
+class Bilinear : public Object {
+public:
+    Bilinear(Vec3 v1, Vec3 v2, Vec3 v3, Vec3 v4) { add_time_sample(v1, v2, v3, v4); }
+    void add_time_sample(Vec3 v1, Vec3 v2, Vec3 v3, Vec3 v4) { v_.push_back({v1, v2, v3, v4}); }
+    int kind() const override { return PSY_OBJ_BILINEAR; }
+    int time_samples() const override { return (int)v_.size(); }
+    std::vector<float> data() const override {
+        std::vector<float> d;
+        for (const auto& s : v_)
+            for (const Vec3& p : s) d.insert(d.end(), {p.x, p.y, p.z});
+        return d;
+    }
+
+private:
+    std::vector<std::array<Vec3, 4>> v_;
+};
+
+class Bicubic : public Object {
+public:
+    explicit Bicubic(const std::array<Vec3, 16>& v) { add_time_sample(v); }
+    void add_time_sample(const std::array<Vec3, 16>& v) { v_.push_back(v); }
+    int kind() const override { return PSY_OBJ_BICUBIC; }
+    int time_samples() const override { return (int)v_.size(); }
+    std::vector<float> data() const override {
+        std::vector<float> d;
+        for (const auto& s : v_)
+            for (const Vec3& p : s) d.insert(d.end(), {p.x, p.y, p.z});
+        return d;
+    }
+
+private:
+    std::vector<std::array<Vec3, 16>> v_;
+};
+
+class SphereLight : public Object {
+public:
+    SphereLight(std::vector<Vec3> positions, std::vector<float> radii, std::vector<Color> colors)
+        : p_(std::move(positions)), r_(std::move(radii)), c_(std::move(colors)) {
+        if (p_.empty() || p_.size() != r_.size() || p_.size() != c_.size())
+            throw std::invalid_argument("SphereLight: one radius and colour per position");
+    }
+    int kind() const override { return PSY_OBJ_SPHERE_LIGHT; }
+    int time_samples() const override { return (int)p_.size(); }
+    std::vector<float> data() const override {
+        std::vector<float> d;
+        for (size_t i = 0; i < p_.size(); ++i)
+            d.insert(d.end(), {p_[i].x, p_[i].y, p_[i].z, r_[i], c_[i].r, c_[i].g, c_[i].b});
+        return d;
+    }
+
+private:
+    std::vector<Vec3> p_;
+    std::vector<float> r_;
+    std::vector<Color> c_;
+};
+
+class RectangleLight : public Object {
+public:
+    RectangleLight(std::vector<std::pair<float, float>> dimensions, std::vector<Color> colors)
+        : dim_(std::move(dimensions)), c_(std::move(colors)) {
+        if (dim_.empty() || dim_.size() != c_.size())
+            throw std::invalid_argument("RectangleLight: one colour per dimension sample");
+    }
+    int kind() const override { return PSY_OBJ_RECTANGLE_LIGHT; }
+    int time_samples() const override { return (int)dim_.size(); }
+    std::vector<float> data() const override {
+        std::vector<float> d;
+        for (size_t i = 0; i < dim_.size(); ++i)
+            d.insert(d.end(), {dim_[i].first, dim_[i].second, c_[i].r, c_[i].g, c_[i].b});
+        return d;
+    }
+
+private:
+    std::vector<std::pair<float, float>> dim_;
+    std::vector<Color> c_;
+};
+
+// ---- shaders ----------------------------------------------------------------
+class SurfaceShader {
+public:
+    virtual ~SurfaceShader() = default;
+    virtual int kind() const = 0;
+    Color col;
+
+protected:
+    explicit SurfaceShader(Color c) : col(c) {}
+};
+
+class LambertShader : public SurfaceShader {
+public:
+    explicit LambertShader(Color c) : SurfaceShader(c) {}
+    int kind() const override { return PSY_SHADER_LAMBERT; }
+};
+
+class EmitShader : public SurfaceShader {
+public:
+    explicit EmitShader(Color c) : SurfaceShader(c) {}
+    int kind() const override { return PSY_SHADER_EMIT; }
+};
+
+// ---- assembly ---------------------------------------------------------------
+class Assembly {
+public:
+    bool add_surface_shader(const std::string& name, std::unique_ptr<SurfaceShader>&& shader) {
+        if (shader_map_.count(name)) return false;
+        shader_map_[name] = shaders_.size();
+        shaders_.push_back(std::move(shader));
+        return true;
+    }
+    const SurfaceShader* get_surface_shader(const std::string& name) const {
+        auto it = shader_map_.find(name);
+        return it == shader_map_.end() ? nullptr : shaders_[it->second].get();
+    }
+    bool add_object(const std::string& name, std::unique_ptr<Object>&& object) {
+        if (object_map_.count(name)) return false;
+        object_map_[name] = objects_.size();
+        objects_.push_back(std::move(object));
+        return true;
+    }
+    bool add_assembly(const std::string& name, std::unique_ptr<Assembly>&& assembly) {
+        if (assembly_map_.count(name)) return false;
+        assembly_map_[name] = assemblies_.size();
+        assemblies_.push_back(std::move(assembly));
+        return true;
+    }
+    bool create_object_instance(const std::string& name, const std::vector<Transform>& transforms,
+                                const SurfaceShader* surface_shader = nullptr) {
+        auto it = object_map_.find(name);
+        if (it == object_map_.end()) return false;
+        instances_.push_back({PSY_INSTANCE_OBJECT, (int)it->second, transforms, surface_shader});
+        return true;
+    }
+    bool create_assembly_instance(const std::string& name, const std::vector<Transform>& transforms,
+                                  const SurfaceShader* surface_shader = nullptr) {
+        auto it = assembly_map_.find(name);
+        if (it == assembly_map_.end()) return false;
+        instances_.push_back({PSY_INSTANCE_ASSEMBLY, (int)it->second, transforms, surface_shader});
+        return true;
+    }
+    // The device build happens in Scene::finalize.
+    bool finalize() { return true; }
+
+private:
+    friend struct Scene;
+    struct Instance {
+        int type, index;
+        std::vector<Transform> transforms;
+        const SurfaceShader* shader;
+    };
+    std::vector<std::unique_ptr<SurfaceShader>> shaders_;
+    std::vector<std::unique_ptr<Object>> objects_;
+    std::vector<std::unique_ptr<Assembly>> assemblies_;
+    std::vector<Instance> instances_;
+    std::unordered_map<std::string, size_t> shader_map_, object_map_, assembly_map_;
+};
+
+// ---- camera, scene ----------------------------------------------------------
+struct Camera {
+    std::vector<Transform> transforms;
+    std::vector<float> fovs;  // radians
+    std::vector<float> aperture_radii;
+    std::vector<float> focus_distances;
+    Camera(std::vector<Transform>& transforms_, std::vector<float>& fovs_, std::vector<float>& aperture_radii_,
+           std::vector<float>& focus_distances_)
+        : transforms(transforms_), fovs(fovs_), aperture_radii(aperture_radii_), focus_distances(focus_distances_) {}
+};
+
+// The device context every Scene, Tracer and integrator of the process
+// shares (one per process, on device 0 unless set_device is called first).
+class Device {
+public:
+    static Device& get() {
+        static Device d;
+        return d;
+    }
+    static void set_device(int device) { get().device_ = device; }
+    psy_ctx* ctx() {
+        if (!ctx_) check(psy_ctx_create(device_, &ctx_), "psy_ctx_create");
+        return ctx_;
+    }
+    ~Device() {
+        if (ctx_) psy_ctx_destroy(ctx_);
+    }
+
+private:
+    int device_ = 0;
+    psy_ctx* ctx_ = nullptr;
+};
+
+struct Scene {
+    std::string name;
+    Color background_color;
+    std::unique_ptr<Camera> camera;
+    std::unique_ptr<Assembly> root;
+    float dice_rate = 0.25f;        // Config::dice_rate (config.cpp:7)
+    float min_upoly_size = 1e-5f;   // Config::min_upoly_size (config.cpp:8)
+    int dice_depth = -1;            // dice cache depth; < 0: from the scene
+
+    Scene() : root(new Assembly()) {}
+    ~Scene() {
+        if (handle) psy_scene_destroy(handle);
+    }
+
+    // The flattened description psy_scene_create reads; valid while *this
+    // is alive and unchanged.
+    struct Description {
+        psy_scene_desc desc{};
+        std::vector<psy_assembly_desc> assemblies;
+        std::vector<psy_shader_desc> shaders;
+        std::vector<std::vector<psy_object_desc>> objects;
+        std::vector<std::vector<psy_instance_desc>> instances;
+        std::vector<std::vector<float>> transforms, object_data;
+        std::vector<std::vector<int32_t>> children;
+        std::vector<float> cam_xf;
+    };
+    std::unique_ptr<Description> describe() const {
+        if (!camera) throw std::runtime_error("Scene: no camera");
+        std::unique_ptr<Description> D(new Description());
+        std::vector<const Assembly*> order;
+        std::unordered_map<const SurfaceShader*, int> shader_index;
+        // assemblies in depth-first order, root first
+        std::function<int(const Assembly*)> visit = [&](const Assembly* a) -> int {
+            const int me = (int)order.size();
+            order.push_back(a);
+            for (const auto& sh : a->shaders_) {
+                shader_index[sh.get()] = (int)D->shaders.size();
+                psy_shader_desc sd{};
+                sd.kind = sh->kind();
+                sd.color[0] = sh->col.r;
+                sd.color[1] = sh->col.g;
+                sd.color[2] = sh->col.b;
+                D->shaders.push_back(sd);
+            }
+            D->children.emplace_back();
+            for (const auto& c : a->assemblies_) {
+                const int ci = visit(c.get());
+                D->children[me].push_back(ci);
+            }
+            return me;
+        };
+        visit(root.get());
+        const size_t na = order.size();
+        D->objects.resize(na);
+        D->instances.resize(na);
+        D->transforms.resize(na);
+        D->object_data.resize(na);
+        for (size_t a = 0; a < na; ++a) {
+            const Assembly* A = order[a];
+            std::vector<size_t> data_off;
+            for (const auto& o : A->objects_) {
+                data_off.push_back(D->object_data[a].size());
+                const std::vector<float> d = o->data();
+                D->object_data[a].insert(D->object_data[a].end(), d.begin(), d.end());
+            }
+            for (size_t i = 0; i < A->objects_.size(); ++i) {
+                psy_object_desc od{};
+                od.kind = A->objects_[i]->kind();
+                od.time_samples = A->objects_[i]->time_samples();
+                od.data = D->object_data[a].data() + data_off[i];
+                D->objects[a].push_back(od);
+            }
+            for (const auto& in : A->instances_) {
+                psy_instance_desc id{};
+                id.type = in.type;
+                id.data_index = in.index;  // object index, or index into this assembly's children
+                id.transform_index = (int32_t)(D->transforms[a].size() / 16);
+                id.transform_count = (int32_t)in.transforms.size();
+                for (const Transform& t : in.transforms)
+                    D->transforms[a].insert(D->transforms[a].end(), t.m.begin(), t.m.end());
+                id.shader_index = -1;
+                if (in.shader) {
+                    auto it = shader_index.find(in.shader);
+                    if (it == shader_index.end()) throw std::runtime_error("Scene: instance shader not in the scene");
+                    id.shader_index = it->second;
+                }
+                D->instances[a].push_back(id);
+            }
+        }
+        D->assemblies.resize(na);
+        for (size_t a = 0; a < na; ++a) {
+            psy_assembly_desc& ad = D->assemblies[a];
+            ad.objects = D->objects[a].data();
+            ad.n_objects = (int32_t)D->objects[a].size();
+            ad.instances = D->instances[a].data();
+            ad.n_instances = (int32_t)D->instances[a].size();
+            ad.transforms = D->transforms[a].data();
+            ad.n_transforms = (int32_t)(D->transforms[a].size() / 16);
+            ad.assemblies = D->children[a].data();
+            ad.n_assemblies = (int32_t)D->children[a].size();
+        }
+        for (const Transform& t : camera->transforms) D->cam_xf.insert(D->cam_xf.end(), t.m.begin(), t.m.end());
+        psy_scene_desc& d = D->desc;
+        d.assemblies = D->assemblies.data();
+        d.n_assemblies = (int32_t)na;
+        d.shaders = D->shaders.data();
+        d.n_shaders = (int32_t)D->shaders.size();
+        d.camera.transforms = D->cam_xf.data();
+        d.camera.n_transforms = (int32_t)camera->transforms.size();
+        d.camera.fovs = camera->fovs.data();
+        d.camera.n_fovs = (int32_t)camera->fovs.size();
+        d.camera.aperture_radii = camera->aperture_radii.data();
+        d.camera.n_aperture_radii = (int32_t)camera->aperture_radii.size();
+        d.camera.focus_distances = camera->focus_distances.data();
+        d.camera.n_focus_distances = (int32_t)camera->focus_distances.size();
+        d.background[0] = background_color.r;
+        d.background[1] = background_color.g;
+        d.background[2] = background_color.b;
+        d.dice_rate = dice_rate;
+        d.min_upoly_size = min_upoly_size;
+        return D;
+    }
+
+    // Scene::finalize: builds the scene on the device.
+    void finalize() {
+        root->finalize();
+        std::unique_ptr<Description> D = describe();
+        if (handle) {
+            psy_scene_destroy(handle);
+            handle = nullptr;
+        }
+        check(psy_scene_create(Device::get().ctx(), &D->desc, dice_depth, &handle), "psy_scene_create");
+    }
+
+    psy_scene* handle = nullptr;
+};
+
+// ---- tracing ----------------------------------------------------------------
+struct WorldRay {
+    enum Type : uint16_t {
+        NONE = 0,
+        CAMERA = 1 << 0,
+        R_DIFFUSE = 1 << 1,
+        R_SPECULAR = 1 << 2,
+        T_DIFFUSE = 1 << 3,
+        T_SPECULAR = 1 << 4,
+        OCCLUSION = 1 << 5
+    };
+    Vec3 o, d;
+    Vec3 odx, ody;
+    Vec3 ddx, ddy;
+    float time = 0.0f;
+    Type type = NONE;
+};
+
+struct Intersection {
+    bool hit = false;
+    uint64_t id = 0;                  // InstanceID::id of the leaf hit
+    float t = 1.0f / 0.0f;            // +inf on a miss
+    float u = 0.0f, v = 0.0f;         // surface parameters (patches, spheres)
+    float offset_dim = 0.0f;          // patch leaf extent (ray offset = n * 1.74 * offset_dim)
+    uint32_t instance = 0xffffffffu;  // flattened leaf instance
+};
+
+class Tracer {
+public:
+    explicit Tracer(Scene* scene) : scene_(scene) {
+        if (!scene_ || !scene_->handle) throw std::runtime_error("Tracer: the scene is not finalized");
+        psy_scene_info info{};
+        check(psy_scene_get_info(scene_->handle, &info), "psy_scene_get_info");
+        ids_.resize((size_t)info.n_leaf_instances);
+        if (!ids_.empty())
+            check(psy_scene_instance_ids(scene_->handle, ids_.data(), (int64_t)ids_.size()),
+                  "psy_scene_instance_ids");
+    }
+    ~Tracer() { release(); }
+    Tracer(const Tracer&) = delete;
+    Tracer& operator=(const Tracer&) = delete;
+
+    // Tracer::trace: closest hit (any hit for OCCLUSION rays) of every ray.
+    uint32_t trace(const WorldRay* w_rays_begin, const WorldRay* w_rays_end, Intersection* intersections_begin,
+                   Intersection* intersections_end) {
+        const int64_t n = w_rays_end - w_rays_begin;
+        if (n != intersections_end - intersections_begin)
+            throw std::invalid_argument("Tracer::trace: one intersection per ray");
+        if (n == 0) return 0;
+        reserve(n);
+        // WorldRay records to the 20-plane stream layout
+        host_.assign((size_t)(PSY_WRAY_PLANES * n), 0.0f);
+        for (int64_t i = 0; i < n; ++i) {
+            const WorldRay& r = w_rays_begin[i];
+            const float v[18] = {r.o.x,   r.o.y,   r.o.z,   r.d.x,   r.d.y,   r.d.z,   r.odx.x, r.odx.y, r.odx.z,
+                                 r.ody.x, r.ody.y, r.ody.z, r.ddx.x, r.ddx.y, r.ddx.z, r.ddy.x, r.ddy.y, r.ddy.z};
+            for (int p = 0; p < 18; ++p) host_[(size_t)(p * n + i)] = v[p];
+            host_[(size_t)(18 * n + i)] = r.time;
+            const uint32_t ty = (uint32_t)r.type;
+            std::memcpy(&host_[(size_t)(19 * n + i)], &ty, 4);
+        }
+        psy_ctx* ctx = Device::get().ctx();
+        check(psy_copy_to_device(ctx, rays_, host_.data(), sizeof(float) * host_.size()), "Tracer::trace");
+        float* h = (float*)hits_;
+        psy_hits hits{h, (uint32_t*)(h + n), h + 2 * n, h + 3 * n, h + 4 * n};
+        check(psy_trace_world(ctx, scene_->handle, (const float*)rays_, n, &hits, 0u, nullptr), "psy_trace_world");
+        host_.resize((size_t)(5 * n));
+        check(psy_copy_to_host(ctx, host_.data(), hits_, sizeof(float) * 5 * (size_t)n), "Tracer::trace");
+        for (int64_t i = 0; i < n; ++i) {
+            Intersection& x = intersections_begin[i];
+            uint32_t inst;
+            std::memcpy(&inst, &host_[(size_t)(n + i)], 4);
+            x.hit = inst != 0xffffffffu;
+            x.instance = inst;
+            x.id = x.hit ? ids_[inst] : 0;
+            x.t = host_[(size_t)i];
+            x.u = host_[(size_t)(2 * n + i)];
+            x.v = host_[(size_t)(3 * n + i)];
+            x.offset_dim = host_[(size_t)(4 * n + i)];
+        }
+        return (uint32_t)n;
+    }
+
+private:
+    void reserve(int64_t n) {
+        if (n <= cap_) return;
+        release();
+        psy_ctx* ctx = Device::get().ctx();
+        check(psy_malloc(ctx, sizeof(float) * PSY_WRAY_PLANES * (size_t)n, &rays_), "psy_malloc");
+        check(psy_malloc(ctx, sizeof(float) * 5 * (size_t)n, &hits_), "psy_malloc");
+        cap_ = n;
+    }
+    void release() {
+        if (rays_) psy_free(Device::get().ctx(), rays_);
+        if (hits_) psy_free(Device::get().ctx(), hits_);
+        rays_ = hits_ = nullptr;
+        cap_ = 0;
+    }
+    Scene* scene_;
+    std::vector<uint64_t> ids_;
+    std::vector<float> host_;
+    void* rays_ = nullptr;
+    void* hits_ = nullptr;
+    int64_t cap_ = 0;
+};
+
+// ---- film, integrator -------------------------------------------------------
+// Film: per-pixel CIE XYZ (the mean of the pixel's samples), row-major.
+struct Film {
+    uint16_t width, height;
+    std::vector<float> xyz;  // width * height * 3
+    Film(uint16_t w, uint16_t h) : width(w), height(h), xyz((size_t)w * h * 3, 0.0f) {}
+};
+
+class PathTraceIntegrator {
+public:
+    // variance_max and spp_max (adaptive sampling) are not used: every
+    // pixel takes spp samples; thread_count is the reference's CPU threads.
+    PathTraceIntegrator(Scene* scene_, Film* image_, int spp_, int spp_max_, float variance_max_, uint32_t seed_,
+                        int thread_count_ = 1, std::function<void()> callback_ = std::function<void()>())
+        : scene(scene_), image(image_), spp(spp_), seed(seed_), callback(std::move(callback_)) {
+        (void)spp_max_;
+        (void)variance_max_;
+        (void)thread_count_;
+    }
+    int path_length = 4;  // the reference integrator's bounces
+
+    void integrate() {
+        if (!scene || !scene->handle) throw std::runtime_error("PathTraceIntegrator: the scene is not finalized");
+        psy_ctx* ctx = Device::get().ctx();
+        const size_t bytes = sizeof(float) * image->xyz.size();
+        void* film = nullptr;
+        check(psy_malloc(ctx, bytes, &film), "psy_malloc");
+        psy_render_params p{};
+        p.width = image->width;
+        p.height = image->height;
+        p.spp = spp;
+        p.seed = seed;
+        p.path_length = path_length;
+        int64_t rays = 0;
+        int rc = psy_render(ctx, scene->handle, &p, (float*)film, nullptr, &rays, nullptr);
+        if (rc == PSY_OK) rc = psy_copy_to_host(ctx, image->xyz.data(), film, bytes);
+        psy_free(ctx, film);
+        check(rc, "PathTraceIntegrator::integrate");
+        rays_shot = rays;
+        if (callback) callback();
+    }
+
+    Scene* scene;
+    Film* image;
+    int spp;
+    uint32_t seed;
+    std::function<void()> callback;
+    int64_t rays_shot = 0;  // Global::Stats::rays_shot of the last integrate()
+};
+
+}  // namespace psy
+
+#endif  // PSYCHOPATH_GPU_HPP

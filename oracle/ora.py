@@ -84,7 +84,7 @@ def lib():
         L.ora_camera_rays.argtypes = [vp, C.c_int, C.c_int, C.c_int, u32, i64, i64, vp, vp]
         L.ora_bounce_rays.argtypes = [vp, vp, vp, i64, i64, C.c_int, C.c_int, u32, u32, u32, vp, vp]
         L.ora_render.restype = i64
-        L.ora_render.argtypes = [vp, C.c_int, C.c_int, C.c_int, u32, C.c_int, vp]
+        L.ora_render.argtypes = [vp, C.c_int, C.c_int, C.c_int, u32, C.c_int, C.c_int, vp]
         L.ora_color_to_spectrum.restype = C.c_float
         L.ora_color_to_spectrum.argtypes = [vp]
         _lib = L
@@ -251,10 +251,10 @@ class OracleScene:
         lib().ora_camera_rays(self.handle, width, height, spp, seed, first, n, _p(out), _p(wl))
         return out, wl
 
-    def render(self, width, height, spp, seed, threads=8):
+    def render(self, width, height, spp, seed, threads=8, path_length=4):
         """Per-sample film contributions (height, width, spp, 3) and rays traced."""
         out = np.zeros((height, width, spp, 3), np.float32)
-        rays = lib().ora_render(self.handle, width, height, spp, seed, int(threads), _p(out))
+        rays = lib().ora_render(self.handle, width, height, spp, seed, int(path_length), int(threads), _p(out))
         if rays < 0:
             raise RuntimeError("oracle render: chromatic colours are not restated")
         return out, int(rays)

@@ -362,6 +362,7 @@ typedef struct {
     uint32_t off, dim;
     float time;
     int step;
+    int path_length;   /* bounces (the reference integrator's 4 unless set) */
     short px, py;
     ora_inter inter;
     ora_wray prev_ray;
@@ -548,7 +549,7 @@ static void update_path(const ora_scene* s, path_t* p, const ora_wray* ray, cons
         }
     }
     p->step++;
-    if (p->step == 4 * 2) p->done = 1;
+    if (p->step == p->path_length * 2) p->done = 1;
 }
 
 /* Color_XYZ(SpectralSample) * XYZ_NORM_FAC and Film::add_sample */
@@ -568,6 +569,7 @@ typedef struct {
     const ora_scene* s;
     int width, height, spp;
     uint32_t seed;
+    int path_length;
     int bucket;
     int nbx, nby;
     int next;
@@ -599,6 +601,7 @@ static void* render_worker(void* vp) {
                 for (int s = 0; s < J->spp; ++s) {
                     init_path(&paths[n], ora_sampler_offset((uint32_t)x, (uint32_t)y, (uint32_t)s, J->seed), (short)x,
                               (short)y);
+                    paths[n].path_length = J->path_length;
                     idx[n] = ((int64_t)y * J->width + x) * J->spp + s;
                     ++n;
                 }
@@ -630,11 +633,11 @@ static void* render_worker(void* vp) {
  * each sample (Color_XYZ(col) * XYZ_NORM_FAC), ordered (y, x, s).  Returns
  * the number of rays traced (camera + bounce + shadow), or -1 when a
  * chromatic colour was encountered (colour upsampling not restated). */
-int64_t ora_render(const ora_scene* s, int width, int height, int spp, uint32_t seed, int threads,
+int64_t ora_render(const ora_scene* s, int width, int height, int spp, uint32_t seed, int path_length, int threads,
                    float* path_xyz_out) {
     render_job J;
     memset(&J, 0, sizeof J);
-    J.s = s; J.width = width; J.height = height; J.spp = spp; J.seed = seed;
+    J.s = s; J.width = width; J.height = height; J.spp = spp; J.seed = seed; J.path_length = path_length;
     int bsz = (int)sqrtf(65536.0f / (float)spp);
     const int maxb = (int)sqrtf((float)(width * height) / ((float)threads * 4.0f));
     if (bsz > maxb) bsz = maxb;

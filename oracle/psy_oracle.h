@@ -112,7 +112,7 @@ void ora_bounce_rays(const ora_scene* s, const ora_wray* in, const ora_inter* hi
  * -1 if a chromatic colour was met (colour upsampling is restated for
  * achromatic colours only). */
 int64_t ora_render(const ora_scene* s, int width, int height, int spp, uint32_t seed,
-                   int threads, float* path_xyz);
+                   int path_length, int threads, float* path_xyz);
 /* Color_to_SpectralSample value for an achromatic colour */
 float ora_color_to_spectrum(const float c[3]);
 

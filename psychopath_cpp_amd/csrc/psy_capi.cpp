@@ -261,6 +261,37 @@ int psy_ctx_trace_timing(psy_ctx* ctx, double* main_ms, double* second_ms, int64
     return rc ? fail(rc, "psy_ctx_trace_timing: an event failed") : PSY_OK;
 }
 
+int psy_malloc(psy_ctx* ctx, size_t bytes, void** out) {
+    if (!ctx || !out) return fail(PSY_ERR_INVALID, "psy_malloc: NULL argument");
+    *out = nullptr;
+    HIP_OK(hipSetDevice(ctx->device));
+    if (bytes == 0) return PSY_OK;
+    HIP_OK(hipMalloc(out, bytes));
+    return PSY_OK;
+}
+
+int psy_free(psy_ctx* ctx, void* p) {
+    if (!ctx) return fail(PSY_ERR_INVALID, "psy_free: NULL context");
+    if (!p) return PSY_OK;
+    HIP_OK(hipSetDevice(ctx->device));
+    HIP_OK(hipFree(p));
+    return PSY_OK;
+}
+
+int psy_copy_to_device(psy_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || ((!dst || !src) && bytes)) return fail(PSY_ERR_INVALID, "psy_copy_to_device: NULL argument");
+    HIP_OK(hipSetDevice(ctx->device));
+    if (bytes) HIP_OK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return PSY_OK;
+}
+
+int psy_copy_to_host(psy_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx || ((!dst || !src) && bytes)) return fail(PSY_ERR_INVALID, "psy_copy_to_host: NULL argument");
+    HIP_OK(hipSetDevice(ctx->device));
+    if (bytes) HIP_OK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return PSY_OK;
+}
+
 int psy_ctx_destroy(psy_ctx* ctx) {
     if (!ctx) return PSY_OK;
     (void)hipSetDevice(ctx->device);

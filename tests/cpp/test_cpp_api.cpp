@@ -1,0 +1,147 @@
+// TEST PROGRAM for include/psychopath_gpu.hpp: builds a scene through the
+// reference-style C++ classes (Assembly, Scene, Camera, shaders, objects,
+// an instanced sub-assembly) and
+//   describe           prints the flattened psy_scene_desc, one fact a line
+//   nodevice           finalizes without a GPU and expects the library's error
+//   gpu <out>          finalizes, traces a grid of camera rays with
+//                      Tracer::trace and renders with PathTraceIntegrator,
+//                      writing the results to <out> (raw little-endian)
+// tests/test_cpp_api.py and tests/test_gpu_cpp_api.py build the same scene
+// in Python and compare.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+
+#include "psychopath_gpu.hpp"
+
+using namespace psy;
+
+static Transform translate(float x, float y, float z) {
+    Transform t;
+    t.m[12] = x;
+    t.m[13] = y;
+    t.m[14] = z;
+    return t;
+}
+
+static std::unique_ptr<Scene> make_scene() {
+    std::unique_ptr<Scene> sc(new Scene());
+    const float cam[16] = {1.0f, -0.0f, 0.0f, 0.0f, 0.0f, 0.4472135901451111f, 0.8944271802902222f, 0.0f,
+                           0.0f, 0.8944271802902222f, -0.4472135901451111f, 0.0f, 0.0f, -9.0f, 5.0f, 1.0f};
+    std::vector<Transform> cx{Transform(cam)};
+    std::vector<float> fov{0.8726646f};
+    std::vector<float> ap{0.0f};
+    std::vector<float> fd{10.0f};
+    sc->camera.reset(new Camera(cx, fov, ap, fd));
+    sc->background_color = Color(0.05f);
+    Assembly& root = *sc->root;
+    root.add_surface_shader("grey", std::unique_ptr<SurfaceShader>(new LambertShader(Color(0.6f))));
+    root.add_surface_shader("dark", std::unique_ptr<SurfaceShader>(new LambertShader(Color(0.3f))));
+    root.add_object("ground", std::unique_ptr<Object>(new Bilinear({-8, -8, 0}, {8, -8, 0}, {-8, 8, 0}, {8, 8, 0})));
+    root.create_object_instance("ground", {}, root.get_surface_shader("grey"));
+    root.add_object("ball", std::unique_ptr<Object>(new Sphere({0.0f, 0.0f, 1.0f}, 1.0f)));
+    root.create_object_instance("ball", {}, root.get_surface_shader("dark"));
+    std::array<Vec3, 16> cp;
+    for (int j = 0; j < 4; ++j)
+        for (int i = 0; i < 4; ++i)
+            cp[j * 4 + i] = Vec3{-3.0f + (float)i, 2.0f + (float)j, (i == 1 || i == 2) && (j == 1 || j == 2) ? 1.5f : 0.2f};
+    root.add_object("hill", std::unique_ptr<Object>(new Bicubic(cp)));
+    root.create_object_instance("hill", {translate(0.5f, 0.0f, 0.0f)}, root.get_surface_shader("grey"));
+    std::unique_ptr<Assembly> lamp(new Assembly());
+    lamp->add_object("bulb", std::unique_ptr<Object>(new SphereLight({{0.0f, 0.0f, 0.0f}}, {0.4f}, {Color(30.0f)})));
+    lamp->create_object_instance("bulb", {}, nullptr);
+    lamp->add_object("panel", std::unique_ptr<Object>(new RectangleLight({{1.0f, 1.0f}}, {Color(10.0f)})));
+    lamp->create_object_instance("panel", {translate(0.0f, 0.0f, 1.0f)}, nullptr);
+    root.add_assembly("lamp", std::move(lamp));
+    root.create_assembly_instance("lamp", {translate(-2.5f, 1.0f, 4.0f)}, nullptr);
+    root.create_assembly_instance("lamp", {translate(2.0f, -1.0f, 3.0f)}, nullptr);
+    return sc;
+}
+
+int main(int argc, char** argv) {
+    const std::string mode = argc > 1 ? argv[1] : "describe";
+    std::unique_ptr<Scene> sc = make_scene();
+    if (mode == "describe") {
+        auto D = sc->describe();
+        const psy_scene_desc& d = D->desc;
+        std::printf("assemblies %d\nshaders %d\n", d.n_assemblies, d.n_shaders);
+        for (int i = 0; i < d.n_shaders; ++i)
+            std::printf("shader %d %d %.7g %.7g %.7g\n", i, d.shaders[i].kind, d.shaders[i].color[0],
+                        d.shaders[i].color[1], d.shaders[i].color[2]);
+        for (int a = 0; a < d.n_assemblies; ++a) {
+            const psy_assembly_desc& A = d.assemblies[a];
+            std::printf("assembly %d objects %d instances %d transforms %d children %d\n", a, A.n_objects,
+                        A.n_instances, A.n_transforms, A.n_assemblies);
+            for (int i = 0; i < A.n_objects; ++i) {
+                const int nf = A.objects[i].kind == PSY_OBJ_SPHERE ? 4 : A.objects[i].kind == PSY_OBJ_BILINEAR ? 12
+                               : A.objects[i].kind == PSY_OBJ_BICUBIC ? 48 : A.objects[i].kind == PSY_OBJ_SPHERE_LIGHT ? 7 : 5;
+                double sum = 0.0;
+                for (int k = 0; k < nf * A.objects[i].time_samples; ++k) sum += A.objects[i].data[k];
+                std::printf("object %d %d kind %d ts %d sum %.6f\n", a, i, A.objects[i].kind,
+                            A.objects[i].time_samples, sum);
+            }
+            for (int i = 0; i < A.n_instances; ++i)
+                std::printf("instance %d %d type %d data %d xf %d+%d shader %d\n", a, i, A.instances[i].type,
+                            A.instances[i].data_index, A.instances[i].transform_index,
+                            A.instances[i].transform_count, A.instances[i].shader_index);
+        }
+        return 0;
+    }
+    if (mode == "nodevice") {
+        try {
+            sc->finalize();
+        } catch (const std::runtime_error& e) {
+            std::printf("error %s\n", e.what());
+            return 0;
+        }
+        std::printf("finalized\n");
+        return 0;
+    }
+    if (mode == "gpu" && argc > 2) {
+        sc->finalize();
+        // a 32 x 24 grid of pinhole camera rays through pixel centres
+        const int W = 32, H = 24;
+        std::vector<WorldRay> rays((size_t)W * H);
+        const float tf = std::tan(0.8726646f * 0.5f);
+        const float* m = sc->camera->transforms[0].m.data();
+        for (int y = 0; y < H; ++y)
+            for (int x = 0; x < W; ++x) {
+                const float sx = (((float)x + 0.5f) / W * 2.0f - 1.0f) * tf;
+                const float sy = -(((float)y + 0.5f) / H * 2.0f - 1.0f) * tf * H / W;
+                WorldRay& r = rays[(size_t)y * W + x];
+                // camera space (sx, sy, 1) to world: row vector times the matrix
+                r.d = Vec3{sx * m[0] + sy * m[4] + m[8], sx * m[1] + sy * m[5] + m[9], sx * m[2] + sy * m[6] + m[10]};
+                r.o = Vec3{m[12], m[13], m[14]};
+                r.ddx = Vec3{0.002f, 0.0f, 0.0f};
+                r.ddy = Vec3{0.0f, 0.002f, 0.0f};
+                r.type = WorldRay::CAMERA;
+            }
+        Tracer tracer(sc.get());
+        std::vector<Intersection> hits(rays.size());
+        const uint32_t n = tracer.trace(rays.data(), rays.data() + rays.size(), hits.data(), hits.data() + hits.size());
+        Film film(W, H);
+        PathTraceIntegrator integ(sc.get(), &film, 4, 4, 0.0f, 1u, 8);
+        integ.integrate();
+        std::ofstream f(argv[2], std::ios::binary);
+        const int32_t hdr[3] = {(int32_t)n, W, H};
+        f.write((const char*)hdr, sizeof hdr);
+        for (const WorldRay& r : rays) {
+            const float v[20] = {r.o.x,   r.o.y,   r.o.z,   r.d.x,   r.d.y,   r.d.z,   r.odx.x, r.odx.y, r.odx.z, r.ody.x,
+                                 r.ody.y, r.ody.z, r.ddx.x, r.ddx.y, r.ddx.z, r.ddy.x, r.ddy.y, r.ddy.z, r.time, 0.0f};
+            f.write((const char*)v, sizeof v);
+        }
+        for (const Intersection& h : hits) {
+            const float tv[3] = {h.t, h.u, h.v};
+            f.write((const char*)tv, sizeof tv);
+            f.write((const char*)&h.id, sizeof h.id);
+            f.write((const char*)&h.instance, sizeof h.instance);
+        }
+        f.write((const char*)film.xyz.data(), sizeof(float) * film.xyz.size());
+        f.write((const char*)&integ.rays_shot, sizeof integ.rays_shot);
+        std::printf("traced %u rays, rendered %lld rays\n", n, (long long)integ.rays_shot);
+        return 0;
+    }
+    std::fprintf(stderr, "usage: test_cpp_api describe | nodevice | gpu <out>\n");
+    return 2;
+}

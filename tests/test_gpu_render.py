@@ -195,3 +195,23 @@ def test_gpu_render_autotune(gpu_ctx):
         assert r_s == r_wf and torch.equal(f_s, f_wf)
     finally:
         ctx.close()
+
+
+def test_gpu_render_bicubic_instanced_motion_matches_oracle(gpu_ctx, oracle):
+    """The batched-form scene at a small size: bicubic patches on a sphere,
+    instanced through two levels with 2-sample transform motion, depth of
+    field, path length 6."""
+    sc = configs.bicubic_sphere_scene(grid=8, n_outer=2, n_inner=2)
+    g = gpu.GpuScene(gpu_ctx, sc)
+    o = oracle.OracleScene(sc)
+    film, samples, rays = gpu.render(gpu_ctx, g, 32, 18, 4, 1, 6, samples=True)
+    torch.cuda.synchronize()
+    os_, orays = o.render(32, 18, 4, 1, threads=8, path_length=6)
+    check_image("bicubic instanced motion 32x18x4", film.cpu().numpy(), samples.cpu().numpy(), rays, os_, orays)
+
+
+def test_gpu_render_many_lights_matches_oracle(gpu_ctx, oracle):
+    """Light-tree sampling with MIS over 2000 small sphere lights."""
+    sc = configs.many_light_scene(n_lights=2000)
+    check_image("many lights 32x32x4", *render_both(gpu_ctx, oracle, sc, 32, 32, 4)[1:])
+

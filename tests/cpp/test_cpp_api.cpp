@@ -17,11 +17,13 @@
 
 using namespace psy;
 
+// World -> local transform of an instance placed at (x, y, z) (instance
+// transforms map the parent's space to the instance's).
 static Transform translate(float x, float y, float z) {
     Transform t;
-    t.m[12] = x;
-    t.m[13] = y;
-    t.m[14] = z;
+    t.m[12] = -x;
+    t.m[13] = -y;
+    t.m[14] = -z;
     return t;
 }
 
@@ -71,8 +73,10 @@ int main(int argc, char** argv) {
                         d.shaders[i].color[1], d.shaders[i].color[2]);
         for (int a = 0; a < d.n_assemblies; ++a) {
             const psy_assembly_desc& A = d.assemblies[a];
-            std::printf("assembly %d objects %d instances %d transforms %d children %d\n", a, A.n_objects,
-                        A.n_instances, A.n_transforms, A.n_assemblies);
+            double xs = 0.0;
+            for (int k = 0; k < 16 * A.n_transforms; ++k) xs += A.transforms[k] * (double)(k % 16 + 1);
+            std::printf("assembly %d objects %d instances %d transforms %d children %d xfsum %.6f\n", a, A.n_objects,
+                        A.n_instances, A.n_transforms, A.n_assemblies, xs);
             for (int i = 0; i < A.n_objects; ++i) {
                 const int nf = A.objects[i].kind == PSY_OBJ_SPHERE ? 4 : A.objects[i].kind == PSY_OBJ_BILINEAR ? 12
                                : A.objects[i].kind == PSY_OBJ_BICUBIC ? 48 : A.objects[i].kind == PSY_OBJ_SPHERE_LIGHT ? 7 : 5;

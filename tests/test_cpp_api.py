@@ -59,8 +59,9 @@ def _describe_python(sc):
         lines.append(f"shader {i} {d.shaders[i].kind} {c[0]:.7g} {c[1]:.7g} {c[2]:.7g}")
     for a in range(d.n_assemblies):
         A = d.assemblies[a]
+        xs = sum(float(A.transforms[k]) * (k % 16 + 1) for k in range(16 * A.n_transforms))
         lines.append(f"assembly {a} objects {A.n_objects} instances {A.n_instances} transforms {A.n_transforms} "
-                     f"children {A.n_assemblies}")
+                     f"children {A.n_assemblies} xfsum {xs:.6f}")
         for i in range(A.n_objects):
             o = A.objects[i]
             n = abi.FLOATS_PER_KIND[o.kind] * o.time_samples

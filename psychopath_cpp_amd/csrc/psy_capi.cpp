@@ -670,8 +670,15 @@ int prepare_scene(const psy_scene_desc* d, int dice_depth, HostScene& H) {
         if (I.xf_count == 0) {
             wb = ob;
         } else {
+            // the object's box corners carried to world space at time
+            // samples; with motion, 64 steps per motion segment, and a pad
+            // of the largest corner displacement between neighbouring steps
+            // (the transformed corners move smoothly, so between steps they
+            // stay within that distance of the sampled boxes)
             const std::vector<M44>& fr = fl.frames[L.frame];
-            const int steps = fr.size() > 1 ? 32 : 0;
+            const int steps = fr.size() > 1 ? 64 * (int)(fr.size() - 1) : 0;
+            float prev[8][3] = {};
+            float move = 0.0f;
             for (int s = 0; s <= steps; ++s) {
                 const float t = steps ? (float)s / steps : 0.0f;
                 const M44 m = m_lerp_seq(t, fr.data(), (int)fr.size());
@@ -684,11 +691,14 @@ int prepare_scene(const psy_scene_desc* d, int dice_depth, HostScene& H) {
                     for (int j = 0; j < 4; ++j) r[j] = pc[0] * inv[0 * 4 + j] + pc[1] * inv[1 * 4 + j] + pc[2] * inv[2 * 4 + j] + inv[3 * 4 + j];
                     const float q[3] = {(float)(r[0] / r[3]), (float)(r[1] / r[3]), (float)(r[2] / r[3])};
                     wb.growp(q);
+                    if (s > 0)
+                        for (int k = 0; k < 3; ++k) move = std::max(move, std::fabs(q[k] - prev[c][k]));
+                    for (int k = 0; k < 3; ++k) prev[c][k] = q[k];
                 }
             }
             float ext = 0.0f;
             for (int k = 0; k < 3; ++k) ext = std::max(ext, std::max(std::fabs(wb.lo[k]), std::fabs(wb.hi[k])));
-            const float pad = (fr.size() > 1 ? 2e-2f * std::max(ext, 1e-3f) : 1e-5f * ext) + 1e-6f;
+            const float pad = move + 1e-5f * ext + 1e-6f;
             for (int k = 0; k < 3; ++k) { wb.lo[k] -= pad; wb.hi[k] += pad; }
         }
         b2.boxes[i] = wb;

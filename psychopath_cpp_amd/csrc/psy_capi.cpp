@@ -421,7 +421,8 @@ struct Box {
 };
 
 struct B2 {
-    Box box;
+    Box box;         // over the whole shutter (the split heuristic's box)
+    Box box0, box1;  // at time 0 and 1: their lerp bounds the subtree at any time
     int left = -1, right = -1;
     int leaf = -1;
 };
@@ -429,15 +430,23 @@ struct B2 {
 struct Builder2 {
     std::vector<B2> nodes;
     std::vector<Box> boxes;
+    std::vector<Box> boxes0, boxes1;  // per prim at time 0 and 1
     std::vector<float> cent;  // 3 per prim
     std::vector<int> idx;
 
     int build(int begin, int end) {
         const int me = (int)nodes.size();
         nodes.emplace_back();
-        Box b, cb;
-        for (int i = begin; i < end; ++i) { b.grow(boxes[idx[i]]); cb.growp(&cent[3 * idx[i]]); }
+        Box b, cb, b0, b1;
+        for (int i = begin; i < end; ++i) {
+            b.grow(boxes[idx[i]]);
+            b0.grow(boxes0[idx[i]]);
+            b1.grow(boxes1[idx[i]]);
+            cb.growp(&cent[3 * idx[i]]);
+        }
         nodes[me].box = b;
+        nodes[me].box0 = b0;
+        nodes[me].box1 = b1;
         if (end - begin == 1) { nodes[me].leaf = idx[begin]; return me; }
         int axis = 0;
         float ext[3];
@@ -489,7 +498,8 @@ struct Builder2 {
 
 struct Collapser {
     const std::vector<B2>* n2;
-    std::vector<BvhNode> out;
+    std::vector<BvhNode> out;   // child boxes at time 0
+    std::vector<BvhNode> out1;  // the same nodes: child box change from time 0 to 1
     int max_need = 0;
 
     // returns child ref, and the traversal stack need of the subtree
@@ -514,6 +524,7 @@ struct Collapser {
         }
         const int me = (int)out.size();
         out.emplace_back();
+        out1.emplace_back();
         int32_t refs[4];
         int sub_need = 0;
         for (int k = 0; k < 4; ++k) {
@@ -525,17 +536,26 @@ struct Collapser {
                 refs[k] = CHILD_EMPTY;
             }
         }
+        // out: child boxes at time 0; out1: their change up to time 1
+        // (0 for empty slots, whose +inf / -inf then stay put)
         BvhNode& o = out[me];
+        BvhNode& o1 = out1[me];
         for (int k = 0; k < 4; ++k) {
-            o.child[k] = refs[k];
+            o.child[k] = o1.child[k] = refs[k];
             if (k < (int)kids.size()) {
-                const Box& b = (*n2)[kids[k]].box;
+                const Box& b = (*n2)[kids[k]].box0;
+                const Box& e = (*n2)[kids[k]].box1;
                 o.lo_x[k] = b.lo[0]; o.hi_x[k] = b.hi[0];
                 o.lo_y[k] = b.lo[1]; o.hi_y[k] = b.hi[1];
                 o.lo_z[k] = b.lo[2]; o.hi_z[k] = b.hi[2];
+                o1.lo_x[k] = e.lo[0] - b.lo[0]; o1.hi_x[k] = e.hi[0] - b.hi[0];
+                o1.lo_y[k] = e.lo[1] - b.lo[1]; o1.hi_y[k] = e.hi[1] - b.hi[1];
+                o1.lo_z[k] = e.lo[2] - b.lo[2]; o1.hi_z[k] = e.hi[2] - b.hi[2];
             } else {
                 o.lo_x[k] = o.lo_y[k] = o.lo_z[k] = INFINITY;
                 o.hi_x[k] = o.hi_y[k] = o.hi_z[k] = -INFINITY;
+                o1.lo_x[k] = o1.lo_y[k] = o1.lo_z[k] = 0.0f;
+                o1.hi_x[k] = o1.hi_y[k] = o1.hi_z[k] = 0.0f;
             }
         }
         const int kc = (int)kids.size();
@@ -652,7 +672,10 @@ int prepare_scene(const psy_scene_desc* d, int dice_depth, HostScene& H) {
     std::vector<DevInstance>& insts = H.insts;
     insts.resize(nl);
     Builder2 b2;
+    bool moving = false;
     b2.boxes.resize(nl);
+    b2.boxes0.resize(nl);
+    b2.boxes1.resize(nl);
     b2.cent.resize(3 * nl);
     b2.idx.resize(nl);
     for (size_t i = 0; i < nl; ++i) {
@@ -667,16 +690,20 @@ int prepare_scene(const psy_scene_desc* d, int dice_depth, HostScene& H) {
         if (I.xf_count > 0 && (I.kind == PSY_OBJ_BILINEAR || I.kind == PSY_OBJ_BICUBIC)) H.transformed_patches = true;
         const Box& ob = obox[L.object];
         Box wb;
+        Box wb0, wb1;
         if (I.xf_count == 0) {
             wb = ob;
+            wb0 = wb1 = ob;
         } else {
             // the object's box corners carried to world space at time
-            // samples; with motion, 64 steps per motion segment, and a pad
-            // of the largest corner displacement between neighbouring steps
-            // (the transformed corners move smoothly, so between steps they
-            // stay within that distance of the sampled boxes)
+            // samples (64 steps per motion segment).  wb: over the whole
+            // shutter; wb0, wb1: at time 0 and 1, padded by the largest
+            // amount the sampled boxes leave their lerp, plus the largest
+            // corner displacement between neighbouring steps (the corners
+            // move smoothly, so between steps they stay within it)
             const std::vector<M44>& fr = fl.frames[L.frame];
             const int steps = fr.size() > 1 ? 64 * (int)(fr.size() - 1) : 0;
+            std::vector<Box> at((size_t)steps + 1);
             float prev[8][3] = {};
             float move = 0.0f;
             for (int s = 0; s <= steps; ++s) {
@@ -691,17 +718,38 @@ int prepare_scene(const psy_scene_desc* d, int dice_depth, HostScene& H) {
                     for (int j = 0; j < 4; ++j) r[j] = pc[0] * inv[0 * 4 + j] + pc[1] * inv[1 * 4 + j] + pc[2] * inv[2 * 4 + j] + inv[3 * 4 + j];
                     const float q[3] = {(float)(r[0] / r[3]), (float)(r[1] / r[3]), (float)(r[2] / r[3])};
                     wb.growp(q);
+                    at[(size_t)s].growp(q);
                     if (s > 0)
                         for (int k = 0; k < 3; ++k) move = std::max(move, std::fabs(q[k] - prev[c][k]));
                     for (int k = 0; k < 3; ++k) prev[c][k] = q[k];
                 }
             }
+            float dev = 0.0f;
+            for (int s = 0; s <= steps; ++s) {
+                const float t = steps ? (float)s / steps : 0.0f;
+                for (int k = 0; k < 3; ++k) {
+                    const float lo = at[0].lo[k] * (1.0f - t) + at[(size_t)steps].lo[k] * t;
+                    const float hi = at[0].hi[k] * (1.0f - t) + at[(size_t)steps].hi[k] * t;
+                    dev = std::max(dev, std::max(lo - at[(size_t)s].lo[k], at[(size_t)s].hi[k] - hi));
+                }
+            }
             float ext = 0.0f;
             for (int k = 0; k < 3; ++k) ext = std::max(ext, std::max(std::fabs(wb.lo[k]), std::fabs(wb.hi[k])));
             const float pad = move + 1e-5f * ext + 1e-6f;
-            for (int k = 0; k < 3; ++k) { wb.lo[k] -= pad; wb.hi[k] += pad; }
+            wb0 = at[0];
+            wb1 = at[(size_t)steps];
+            for (int k = 0; k < 3; ++k) {
+                wb.lo[k] -= pad; wb.hi[k] += pad;
+                // between samples the lerp moves by at most one step's
+                // displacement too, hence the second `move`
+                wb0.lo[k] -= pad + dev + move; wb0.hi[k] += pad + dev + move;
+                wb1.lo[k] -= pad + dev + move; wb1.hi[k] += pad + dev + move;
+            }
+            if (steps) moving = true;
         }
         b2.boxes[i] = wb;
+        b2.boxes0[i] = wb0;
+        b2.boxes1[i] = wb1;
         for (int k = 0; k < 3; ++k) b2.cent[3 * i + k] = 0.5f * (wb.lo[k] + wb.hi[k]);
         b2.idx[i] = (int)i;
         H.ids.push_back(L.id);
@@ -720,6 +768,7 @@ int prepare_scene(const psy_scene_desc* d, int dice_depth, HostScene& H) {
         root = cl.emit(0, &need);
         stack_need = std::max(1, need) + 1;
         bvh.swap(cl.out);
+        if (moving) H.bvh1.swap(cl.out1);
     }
     H.root = root;
     H.stack_need = stack_need;
@@ -821,6 +870,7 @@ int psy_scene_create(psy_ctx* ctx, const psy_scene_desc* d, int dice_depth, psy_
     // ---- upload ------------------------------------------------------------
     int rc;
     if ((rc = sc->bvh.upload(H.bvh))) return rc;
+    if (!H.bvh1.empty() && (rc = sc->bvh1.upload(H.bvh1))) return rc;
     if ((rc = sc->inst.upload(H.insts))) return rc;
     if ((rc = sc->obj.upload(H.objs))) return rc;
     if ((rc = sc->odata.upload(H.odata))) return rc;
@@ -845,6 +895,7 @@ int psy_scene_create(psy_ctx* ctx, const psy_scene_desc* d, int dice_depth, psy_
 
     DevScene& s = sc->dev;
     s.bvh = (const BvhNode*)sc->bvh.p;
+    s.bvh1 = H.bvh1.empty() ? nullptr : (const BvhNode*)sc->bvh1.p;
     s.n_bvh = (int32_t)H.bvh.size();
     s.root = H.root;
     s.inst = (const DevInstance*)sc->inst.p;
@@ -866,7 +917,7 @@ int psy_scene_create(psy_ctx* ctx, const psy_scene_desc* d, int dice_depth, psy_
     info.dice_depth = levels;
     info.bvh_stack_need = H.stack_need;
     info.dice_cache_bytes = (int64_t)dice_bytes;
-    info.device_bytes = (int64_t)(sc->bvh.bytes + sc->inst.bytes + sc->obj.bytes + sc->odata.bytes + sc->xf.bytes +
+    info.device_bytes = (int64_t)(sc->bvh.bytes + sc->bvh1.bytes + sc->inst.bytes + sc->obj.bytes + sc->odata.bytes + sc->xf.bytes +
                                   sc->shaders.bytes + dice_bytes);
     *out = sc.release();
     return PSY_OK;

@@ -71,6 +71,9 @@ struct DevCamera {
 // Everything a kernel needs to trace against a scene (passed by value).
 struct DevScene {
     const BvhNode* bvh;
+    const BvhNode* bvh1;      // moving scenes: each node's child box change
+                              // from time 0 (bvh) to time 1, which a ray
+                              // scales by its time; null for static scenes
     int32_t n_bvh;
     int32_t root;             // child ref of the root (node 0, or a leaf)
     const DevInstance* inst;

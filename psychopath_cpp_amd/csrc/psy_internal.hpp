@@ -125,7 +125,7 @@ struct psy_scene {
     psy_ctx* ctx = nullptr;
     psy::DevScene dev{};
     psy::DevCamera cam{};
-    psy::DBuf bvh, inst, obj, odata, xf, dice, shaders;
+    psy::DBuf bvh, bvh1, inst, obj, odata, xf, dice, shaders;
     // differentiable parameters: shader albedos (host copy of the table),
     // emission scales of the light objects
     std::vector<psy::DevShader> host_shaders;
@@ -151,6 +151,7 @@ namespace psy {
 // built on the device) and the light trees.
 struct HostScene {
     std::vector<BvhNode> bvh;
+    std::vector<BvhNode> bvh1;  // child box change to time 1 when some leaf moves
     int32_t root = CHILD_EMPTY;
     int stack_need = 1;
     std::vector<DevInstance> insts;

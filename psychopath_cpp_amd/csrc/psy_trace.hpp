@@ -587,6 +587,26 @@ struct TRay {
     bool occl;
 };
 
+// A BVH4 node's child boxes at the ray's time: the stored boxes for a
+// static scene; for a moving one, the time-0 boxes plus the ray's time
+// (clamped to the shutter, as the transforms are) times their change up to
+// time 1.  Each endpoint box is the union of its subtree's, padded on the
+// host so the interpolation holds every leaf at every time (and the pad
+// covers the rounding of the two operations here).
+PSY_DEV void node_boxes(const DevScene& s, int32_t ref, float time, float4& lx, float4& hx, float4& ly, float4& hy,
+                        float4& lz, float4& hz) {
+    const float4* q = reinterpret_cast<const float4*>(s.bvh + ref);
+    lx = q[0]; hx = q[1]; ly = q[2]; hy = q[3]; lz = q[4]; hz = q[5];
+    if (s.bvh1) {
+        const float4* q1 = reinterpret_cast<const float4*>(s.bvh1 + ref);
+        const float t = fminf(fmaxf(time, 0.0f), 1.0f);
+        const float4 a0 = q1[0], a1 = q1[1], a2 = q1[2], a3 = q1[3], a4 = q1[4], a5 = q1[5];
+#define PSY_MIX(V, W) V = make_float4(V.x + W.x * t, V.y + W.y * t, V.z + W.z * t, V.w + W.w * t)
+        PSY_MIX(lx, a0); PSY_MIX(hx, a1); PSY_MIX(ly, a2); PSY_MIX(hy, a3); PSY_MIX(lz, a4); PSY_MIX(hz, a5);
+#undef PSY_MIX
+    }
+}
+
 template <bool WORLD, int DEEP>
 PSY_DEV bool intersect_leaf(const DevScene& s, uint32_t li, const DevInstance& I, const TRay& r,
                                                Best& b, DeepState& need_deep) {
@@ -741,8 +761,8 @@ PSY_DEV bool trace_one(const DevScene& s, const float* __restrict__ rays, int64_
         const float nearv = stk_near[sp][tid];
         if (!(nearv < b.t)) continue;
         if (ref >= 0) {
-            const float4* q = reinterpret_cast<const float4*>(s.bvh + ref);
-            const float4 lx = q[0], hx = q[1], ly = q[2], hy = q[3], lz = q[4], hz = q[5];
+            float4 lx, hx, ly, hy, lz, hz;
+            node_boxes(s, ref, r.time, lx, hx, ly, hy, lz, hz);
             const int4 ch = reinterpret_cast<const int4*>(s.bvh + ref)[6];
             // BBox4::intersect_ray lanes (basics/bbox.hpp:566-595); keys are
             // the near distances of hit children, +inf otherwise
@@ -952,10 +972,10 @@ PSY_DEV bool lane_step(const DevScene& s, Lane& L, int32_t (*stk_ref)[TRACE_BLOC
         if (!(stk_near[L.sp][tid] < L.b.t)) return false;
     }
     if (ref >= 0) {
-        const float4* q = reinterpret_cast<const float4*>(s.bvh + ref);
-        const float4 lx = q[0], hx = q[1], ly = q[2], hy = q[3], lz = q[4], hz = q[5];
-        const int4 ch = reinterpret_cast<const int4*>(s.bvh + ref)[6];
         const TRay& r = L.r;
+        float4 lx, hx, ly, hy, lz, hz;
+        node_boxes(s, ref, r.time, lx, hx, ly, hy, lz, hz);
+        const int4 ch = reinterpret_cast<const int4*>(s.bvh + ref)[6];
         const bool ds0 = r.dinv.x < 0.0f, ds1 = r.dinv.y < 0.0f, ds2 = r.dinv.z < 0.0f;
         // BBox4::intersect_ray lanes (basics/bbox.hpp:566-595) for a finite
         // inverse direction (no NaN); keys are the near distances of hit

@@ -46,6 +46,7 @@ int build(const psy_scene_desc* d, int dice_depth, HostBuilt& B) {
     }
     DevScene& s = B.s;
     s.bvh = H.bvh.data();
+    s.bvh1 = H.bvh1.empty() ? nullptr : H.bvh1.data();
     s.n_bvh = (int32_t)H.bvh.size();
     s.root = H.root;
     s.inst = H.insts.data();

@@ -38,6 +38,69 @@ def test_cpu_build_instanced_hierarchy(motion, dice_depth):
         assert hb["retraced"] > 0
 
 
+def _spinning_scene():
+    """Instances that turn fast through three transform samples (up to 90
+    degrees a segment, with scale changes), nested two levels deep: the
+    interpolated node bounds of a moving scene must still hold every leaf
+    at every ray time."""
+    from psychopath_cpp_amd.scene import Assembly, Bicubic, Camera, Scene, Sphere, SphereLight, look_at
+
+    def spin(ang_deg, scl, pos):
+        a = np.radians(ang_deg)
+        c, s = np.cos(a), np.sin(a)
+        m = np.array([[c * scl, s * scl, 0, 0], [-s * scl, c * scl, 0, 0], [0, 0, scl, 0], [pos[0], pos[1], pos[2], 1]])
+        return np.linalg.inv(m).astype(np.float32)
+
+    rng = np.random.default_rng(21)
+    inner = Assembly()
+    for k in range(4):
+        base = rng.normal(size=3) * 0.6 + np.array([1.0, 0.0, 0.0])
+        pts = [base + np.array([i * 0.4, j * 0.4, rng.normal() * 0.2]) for j in range(4) for i in range(4)]
+        inner.add_object(f"bc{k}", Bicubic(np.array(pts)))
+        inner.create_object_instance(f"bc{k}", [], None)
+    inner.add_object("ball", Sphere([1.5, 0.5, 0.3], 0.4))
+    inner.create_object_instance("ball", [], None)
+    mid = Assembly()
+    mid.add_assembly("inner", inner)
+    for k in range(3):
+        pos = rng.normal(size=3) * 1.2
+        mid.create_assembly_instance("inner", [spin(0 + 40 * k, 1.0, pos), spin(70 + 40 * k, 1.2, pos + 0.3),
+                                               spin(150 + 40 * k, 0.9, pos)], None)
+    root = Assembly()
+    root.add_assembly("mid", mid)
+    for k in range(3):
+        pos = np.array([rng.uniform(-4, 4), rng.uniform(-4, 4), rng.uniform(-1, 1)])
+        root.create_assembly_instance("mid", [spin(-30 * k, 1.0, pos), spin(-30 * k + 90, 1.0, pos + 0.5),
+                                              spin(-30 * k + 120, 1.1, pos - 0.5)], "grey")
+    root.add_object("lamp", SphereLight([0, 0, 9], 1.0, (5, 5, 5)))
+    root.create_object_instance("lamp", [], None)
+    from psychopath_cpp_amd import abi
+    sc = Scene(Camera([look_at((0, -14, 7), (0, 0, 0))], [np.radians(50)]), root, dice_rate=0.5)
+    sc.add_surface_shader("grey", abi.PSY_SHADER_LAMBERT, (0.6, 0.6, 0.6))
+    return sc
+
+
+def test_cpu_build_fast_motion_node_bounds():
+    """Checked against the oracle's every-instance trace: the reference's own
+    instance culling interpolates per-time-sample boxes (bbox_lerp in its
+    BVH), which do not hold turning geometry between samples, so its BVH
+    trace misses surfaces here (the ray then misses or finds one farther
+    away) that the every-instance trace and the device's padded bounds
+    find."""
+    sc = _spinning_scene()
+    o = ora.OracleScene(sc)
+    w, _ = o.camera_rays(96, 72, 2, 5)
+    hb = hostbuild.trace(sc, ora.planes_from_wrays(w), world=True, dice_depth=12)
+    exact = o.trace(w, threads=8, chunk=8192, brute=True)
+    agree, t_ok, hits = compare(hb, exact)
+    assert hits > 0.1 * len(w)
+    assert agree >= 0.9999 and t_ok >= 0.9999
+    culled = o.trace(w, threads=8, chunk=8192)
+    ch = culled["hit"] != 0
+    assert not (ch & (exact["hit"] == 0)).any()
+    assert (culled["t"][ch] >= exact["t"][ch]).all()
+
+
 def test_cpu_build_heightfield_primary_and_object_rays():
     sc = configs.heightfield_scene(grid=16)
     o = ora.OracleScene(sc)

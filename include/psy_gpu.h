@@ -269,8 +269,10 @@ typedef struct psy_render_params {
  *                (y, x, sample).
  *   rays_traced: host, or NULL: camera + bounce + shadow rays traced
  *                (Global::Stats::rays_shot).
- * Grey colours only (PSY_ERR_UNSUPPORTED otherwise).  Synchronizes
- * `stream` once per wavefront step. */
+ * Colours other than grey (and surfaces without a shader, which emit
+ * magenta) need the spectrum grid (psy_spectrum_set_grid;
+ * PSY_ERR_UNSUPPORTED otherwise).  Synchronizes `stream` once per wavefront
+ * step. */
 int psy_render(psy_ctx* ctx, const psy_scene* scene, const psy_render_params* params,
                float* film_xyz, float* sample_xyz, int64_t* rays_traced, void* stream);
 
@@ -344,6 +346,34 @@ int psy_sobol_set_matrices(psy_ctx* ctx, const uint64_t* matrices, uint32_t num_
  * read as zero. */
 int psy_sobol(psy_ctx* ctx, uint32_t dim, const uint64_t* index, int64_t n,
               uint64_t scramble, double* out, void* stream);
+/* ---- colour ------------------------------------------------------------- */
+/* The spectrum grid of tristimulus upsampling (Hanika et al.;
+ * color/spectrum_grid.h, color/spectra_xyz_5nm_380_780_0.97.h), handed over
+ * by the caller as data (the library embeds none):
+ *   cells        : grid_width * grid_height cells, row-major, 8 ints each
+ *                  (inside, num_points, idx[6]) -- spectrum_grid_cell_t,
+ *                  i.e. (const int32_t*)spectrum_grid;
+ *   points       : n_points data points of 4 + n_samples floats (xystar[2],
+ *                  uv[2], spectrum[n_samples]) -- spectrum_data_point_t,
+ *                  i.e. (const float*)spectrum_data_points;
+ *   sample_min/max, n_samples: the spectra's wavelength range and knots
+ *                  (spectrum_sample_min/max, spectrum_num_samples);
+ *   mat_xy_to_uv : the 3x2 xy -> uv matrix (spectrum_mat_xy_to_uv).
+ * Host memory, validated and copied to the device.  With a grid set,
+ * psy_render upsamples every colour through it, as the reference does;
+ * without one (or after a call with cells == NULL) it takes grey colours
+ * only, on the grid's flat white-point spectrum. */
+int psy_spectrum_set_grid(psy_ctx* ctx, const int32_t* cells, int32_t grid_width, int32_t grid_height,
+                          const float* points, int32_t n_points, int32_t n_samples, float sample_min,
+                          float sample_max, const float* mat_xy_to_uv);
+/* Replaces: Color_to_SpectralSample(col, hero) (color/color.hpp:490-511,
+ * color/spectrum_grid.h:21-135) over a stream (device memory): rgb 3*n,
+ * hero_wavelength n (nm, in [380, 700]), out 4*n = the spectral samples at
+ * the hero wavelength and its 3 rotations.  Without a grid: the flat
+ * spectrum (exact for grey colours only). */
+int psy_color_to_spectrum(psy_ctx* ctx, const float* rgb, const float* hero_wavelength, int64_t n, float* out,
+                          void* stream);
+
 /* Replaces: Sampler(x,y,s,seed) + Sampler::get_sample(dim)
  * (sampling/image_sampler.hpp:16-38): out[i*n_dims + k] = sample of dim
  * dim0+k for pixel sample i of the tile (x0..x0+w, y0..y0+h, spp). */

@@ -280,6 +280,19 @@ public:
         if (!ctx_) check(psy_ctx_create(device_, &ctx_), "psy_ctx_create");
         return ctx_;
     }
+    // Colour upsampling for non-grey colours: the reference's spectrum grid
+    // (color/spectra_xyz_5nm_380_780_0.97.h), e.g.
+    //   Device::get().set_spectrum_grid((const int32_t*)spectrum_grid, spectrum_grid_width,
+    //       spectrum_grid_height, (const float*)spectrum_data_points,
+    //       sizeof(spectrum_data_points) / sizeof(spectrum_data_points[0]), spectrum_num_samples,
+    //       spectrum_sample_min, spectrum_sample_max, spectrum_mat_xy_to_uv);
+    void set_spectrum_grid(const int32_t* cells, int grid_width, int grid_height, const float* points,
+                           int n_points, int n_samples, float sample_min, float sample_max,
+                           const float* mat_xy_to_uv) {
+        check(psy_spectrum_set_grid(ctx(), cells, grid_width, grid_height, points, n_points, n_samples, sample_min,
+                                    sample_max, mat_xy_to_uv),
+              "psy_spectrum_set_grid");
+    }
     ~Device() {
         if (ctx_) psy_ctx_destroy(ctx_);
     }

@@ -87,6 +87,11 @@ def lib():
         L.ora_render.argtypes = [vp, C.c_int, C.c_int, C.c_int, u32, C.c_int, C.c_int, vp]
         L.ora_color_to_spectrum.restype = C.c_float
         L.ora_color_to_spectrum.argtypes = [vp]
+        L.ora_spectrum_set_grid.restype = C.c_int
+        L.ora_spectrum_set_grid.argtypes = [vp, C.c_int, C.c_int, vp, C.c_int, C.c_int, C.c_float, C.c_float, vp]
+        L.ora_spectrum_xyz_to_p.restype = C.c_float
+        L.ora_spectrum_xyz_to_p.argtypes = [C.c_float, vp]
+        L.ora_color_to_spectral_sample.argtypes = [vp, C.c_float, vp]
         _lib = L
     return _lib
 
@@ -111,6 +116,13 @@ def ref():
         R.psyref_invert44.argtypes = [C.c_void_p]
         R.psyref_spectrum_xyz_to_p.restype = C.c_float
         R.psyref_spectrum_xyz_to_p.argtypes = [C.c_float, C.c_void_p]
+        if hasattr(R, "psyref_spectrum_tables"):
+            pp = C.POINTER
+            R.psyref_spectrum_tables.argtypes = [pp(C.c_void_p), pp(C.c_int), pp(C.c_int), pp(C.c_void_p),
+                                                 pp(C.c_int), pp(C.c_int), pp(C.c_float), pp(C.c_float),
+                                                 pp(C.c_void_p)]
+            R.psyref_color_to_spectrum.restype = C.c_float
+            R.psyref_color_to_spectrum.argtypes = [C.c_void_p, C.c_float]
         _ref = R
     return _ref
 
@@ -144,6 +156,63 @@ def ref_sobol_matrices():
         return None
     n = R.psyref_sobol_dims() * 52
     return np.array([R.psyref_sobol_matrix(i) for i in range(n)], np.uint64)
+
+
+class SpectrumGrid:
+    """The spectrum grid tables (color/spectrum_grid.h) as arrays: cells
+    (w*h, 8) int32 = inside, num_points, idx[6]; points (n, 4 + n_samples)
+    float32 = xystar, uv, spectrum; the xy -> uv matrix (6,)."""
+
+    def __init__(self, cells, w, h, points, n_samples, smin, smax, mat):
+        self.cells = np.ascontiguousarray(cells, np.int32).reshape(w * h, 8)
+        self.points = np.ascontiguousarray(points, np.float32).reshape(-1, 4 + n_samples)
+        self.w, self.h, self.n_samples = int(w), int(h), int(n_samples)
+        self.smin, self.smax = float(smin), float(smax)
+        self.mat = np.ascontiguousarray(mat, np.float32).reshape(6)
+
+    @property
+    def n_points(self):
+        return self.points.shape[0]
+
+
+def ref_spectrum_grid():
+    """The reference's own spectrum grid tables read from oracle/_ref, or
+    None when that library was not built."""
+    R = ref()
+    if R is None or not hasattr(R, "psyref_spectrum_tables"):
+        return None
+    cells, pts, mat = C.c_void_p(), C.c_void_p(), C.c_void_p()
+    w, h, n, ns = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    smin, smax = C.c_float(), C.c_float()
+    R.psyref_spectrum_tables(C.byref(cells), C.byref(w), C.byref(h), C.byref(pts), C.byref(n), C.byref(ns),
+                             C.byref(smin), C.byref(smax), C.byref(mat))
+    cv = np.ctypeslib.as_array(C.cast(cells, C.POINTER(C.c_int32)), (w.value * h.value * 8,)).copy()
+    pv = np.ctypeslib.as_array(C.cast(pts, C.POINTER(C.c_float)), (n.value * (4 + ns.value),)).copy()
+    mv = np.ctypeslib.as_array(C.cast(mat, C.POINTER(C.c_float)), (6,)).copy()
+    return SpectrumGrid(cv, w.value, h.value, pv, ns.value, smin.value, smax.value, mv)
+
+
+def spectrum_set_grid(grid):
+    """Hand a SpectrumGrid to the oracle (None clears it: grey colours only)."""
+    if grid is None:
+        rc = lib().ora_spectrum_set_grid(None, 0, 0, None, 0, 0, 0.0, 0.0, None)
+    else:
+        rc = lib().ora_spectrum_set_grid(_p(grid.cells), grid.w, grid.h, _p(grid.points), grid.n_points,
+                                         grid.n_samples, grid.smin, grid.smax, _p(grid.mat))
+    if rc != 0:
+        raise ValueError("ora_spectrum_set_grid: bad table")
+
+
+def spectrum_xyz_to_p(lam, xyz):
+    xyz = np.ascontiguousarray(xyz, np.float32)
+    return np.float32(lib().ora_spectrum_xyz_to_p(float(lam), _p(xyz)))
+
+
+def color_to_spectral_sample(rgb, hero):
+    rgb = np.ascontiguousarray(rgb, np.float32)
+    out = np.zeros(4, np.float32)
+    lib().ora_color_to_spectral_sample(_p(rgb), float(hero), _p(out))
+    return out
 
 
 def make_sobol_matrices(n_dims, seed=1):

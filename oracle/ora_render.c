@@ -17,12 +17,14 @@
  *                                  color/color.hpp:15-188
  *   Film::add_sample (sum + count) film/film.hpp:135-153
  *
- * Colour -> spectrum (color/color.hpp:490-511 through the Hanika grid,
- * color/spectrum_grid.h) is restated for achromatic colours only: at the
- * equal-energy white point the grid's spectrum is flat (pinned against the
- * reference by tests/golden/ref_vectors.npz), so the upsampled value is
- * (X+Y+Z)/3 at every wavelength.  Chromatic colours are reported through
- * ora_render's return value and not rendered.
+ * Colour -> spectrum (color/color.hpp:490-511): through the Hanika grid
+ * (color/spectrum_grid.h:21-135, ora_spectrum_xyz_to_p) when the caller has
+ * set one (ora_spectrum_set_grid: the reference's own tables), else for
+ * achromatic colours only, where the grid's spectrum is flat to within
+ * 2e-6 relative (pinned against the reference by
+ * tests/golden/ref_vectors.npz): (X+Y+Z)/3 at every wavelength.  Without a
+ * grid, chromatic colours are reported through ora_render's return value
+ * and not rendered.
  */
 #include <stdlib.h>
 #include <string.h>
@@ -64,10 +66,115 @@ float ora_color_to_spectrum(const float c[3]) {
     return (1.0f / 3.0f) / norm;
 }
 
-static void spec_from_color(const float c[3], float e[NSPEC]) {
-    const float v = ora_color_to_spectrum(c);
-    for (int i = 0; i < NSPEC; ++i) e[i] = v;
+/* ---- the spectrum grid (color/spectrum_grid.h) ----------------------------
+ * The grid of Hanika et al.'s tristimulus upsampling, as data set by the
+ * caller (the reference's spectrum_grid / spectrum_data_points tables):
+ * cells are 8 ints (inside, num_points, idx[6]), data points are
+ * 4 + n_samples floats (xystar[2], uv[2], spectrum[n_samples]). */
+static struct {
+    int set;
+    int32_t* cells;
+    float* points;
+    int w, h, n_points, n_samples;
+    float smin, smax;
+    float m[6];
+} g_grid;
+
+int ora_spectrum_set_grid(const int32_t* cells, int w, int h, const float* points, int n_points, int n_samples,
+                          float smin, float smax, const float* mat_xy_to_uv) {
+    free(g_grid.cells);
+    free(g_grid.points);
+    memset(&g_grid, 0, sizeof g_grid);
+    if (!cells) return 0;  /* cleared */
+    if (w < 1 || h < 1 || n_points < 1 || n_samples < 2 || !(smin < smax) || !points || !mat_xy_to_uv) return -1;
+    g_grid.cells = (int32_t*)malloc(sizeof(int32_t) * 8 * (size_t)(w * h));
+    g_grid.points = (float*)malloc(sizeof(float) * (size_t)(4 + n_samples) * (size_t)n_points);
+    memcpy(g_grid.cells, cells, sizeof(int32_t) * 8 * (size_t)(w * h));
+    memcpy(g_grid.points, points, sizeof(float) * (size_t)(4 + n_samples) * (size_t)n_points);
+    g_grid.w = w; g_grid.h = h; g_grid.n_points = n_points; g_grid.n_samples = n_samples;
+    g_grid.smin = smin; g_grid.smax = smax;
+    memcpy(g_grid.m, mat_xy_to_uv, sizeof g_grid.m);
+    g_grid.set = 1;
+    return 0;
 }
+
+static const float* gpoint(int i) { return g_grid.points + (size_t)(4 + g_grid.n_samples) * (size_t)i; }
+
+/* spectrum_xyz_to_p (color/spectrum_grid.h:21-135): the grid cell of the
+ * colour's chromaticity, the cell's data point spectra at lambda (linear
+ * between the 5 nm knots), then bilinear weights in an inside cell or the
+ * barycentric weights of the fan triangle around idx[0] that holds the
+ * point; scaled by X+Y+Z.  Outside the grid, or a NaN chromaticity: 0. */
+float ora_spectrum_xyz_to_p(float lambda, const float* xyz) {
+    if (!g_grid.set) return 0.0f;
+    const float norm = (float)(1.0 / (double)(xyz[0] + xyz[1] + xyz[2]));
+    if (!(norm < FLT_MAX)) return 0.0f;
+    const float cx = xyz[0] * norm, cy = xyz[1] * norm;
+    float uv[2];
+    uv[0] = ((g_grid.m[0] * cx) + (g_grid.m[1] * cy)) + g_grid.m[2];
+    uv[1] = ((g_grid.m[3] * cx) + (g_grid.m[4] * cy)) + g_grid.m[5];
+    if (!(uv[0] >= 0.0f && uv[0] < (float)g_grid.w && uv[1] >= 0.0f && uv[1] < (float)g_grid.h)) return 0.0f;
+    const int ui = (int)uv[0], vi = (int)uv[1];
+    const int32_t* cell = g_grid.cells + 8 * (ui + g_grid.w * vi);
+    const int inside = cell[0], num = cell[1];
+    const int32_t* idx = cell + 2;
+    const float sb = ((lambda - g_grid.smin) / (g_grid.smax - g_grid.smin)) * (float)(g_grid.n_samples - 1);
+    const int sb0 = (int)sb;
+    const int sb1 = sb + 1.0f < (float)g_grid.n_samples ? (int)(sb + 1.0f) : g_grid.n_samples - 1;
+    const float sbf = sb - (float)sb0;
+    float p[6];
+    for (int i = 0; i < num; ++i) {
+        const float* sp = gpoint(idx[i]) + 4;
+        p[i] = (sp[sb0] * (1.0f - sbf)) + (sp[sb1] * sbf);
+    }
+    float ip = 0.0f;
+    if (inside) {
+        const float u = uv[0] - (float)ui, v = uv[1] - (float)vi;
+        ip = ((((p[0] * (1.0f - u)) * (1.0f - v)) + ((p[2] * (1.0f - u)) * v)) + ((p[3] * u) * v)) +
+             ((p[1] * u) * (1.0f - v));
+    } else if (num > 0) {
+        const float* q0 = gpoint(idx[0]);
+        const float ex = uv[0] - q0[2], ey = uv[1] - q0[3];
+        float e0x = gpoint(idx[1])[2] - q0[2], e0y = gpoint(idx[1])[3] - q0[3];
+        float uu = (e0x * ey) - (ex * e0y);
+        for (int i = 0; i < num - 1; ++i) {
+            const float* qn = gpoint(i == num - 2 ? idx[1] : idx[i + 2]);
+            const float e1x = qn[2] - q0[2], e1y = qn[3] - q0[3];
+            const float vv = (ex * e1y) - (e1x * ey);
+            const float area = (e0x * e1y) - (e1x * e0y);
+            const float u = uu / area, v = vv / area;
+            const float w = (1.0f - u) - v;
+            if (u < 0.0f || v < 0.0f || w < 0.0f) {
+                uu = -vv;
+                e0x = e1x;
+                e0y = e1y;
+                continue;
+            }
+            ip = ((p[0] * w) + (p[i + 1] * v)) + (p[i == num - 2 ? 1 : i + 2] * u);
+            break;
+        }
+    }
+    return ip / norm;
+}
+
+/* Color_to_SpectralSample (color/color.hpp:490-511) at the 4 wavelengths of
+ * hero: through the grid when one is set, else the flat white-point
+ * spectrum for grey colours (chromatic ones are reported). */
+static void spec_from_color(const float c[3], float hero, float e[NSPEC]) {
+    if (!g_grid.set) {
+        const float v = ora_color_to_spectrum(c);
+        for (int i = 0; i < NSPEC; ++i) e[i] = v;
+        return;
+    }
+    float xyz[3];
+    xyz[0] = ((c[0] * 0.4339f) + (c[1] * 0.3762f)) + (c[2] * 0.1899f);
+    xyz[1] = ((c[0] * 0.2126f) + (c[1] * 0.7152f)) + (c[2] * 0.0722f);
+    xyz[2] = ((c[0] * 0.0177f) + (c[1] * 0.1095f)) + (c[2] * 0.8728f);
+    for (int i = 0; i < NSPEC; ++i)
+        e[i] = ora_spectrum_xyz_to_p(wavelength_n(hero, i), xyz) * (1.0f / 0.009358239977091027f);
+}
+
+void ora_color_to_spectral_sample(const float c[3], float hero, float e[4]) { spec_from_color(c, hero, e); }
 
 /* Wyman et al. CIE 1931 fits (color/color.hpp:150-165) */
 static float X_1931(float w) {
@@ -238,7 +345,7 @@ static void sphere_light_sample(const ora_object* ob, lquery* q, v3 arr, float* 
     x = v3norm(x); y = v3norm(y); z = v3norm(z);
     float ccol[3];
     for (int k = 0; k < 3; ++k) ccol[k] = col[k] * (float)surface_area_inv;
-    spec_from_color(ccol, q->spec);
+    spec_from_color(ccol, q->wavelength, q->spec);
     for (int k = 0; k < NSPEC; ++k) q->spec[k] *= ob->escale;
     if (d > radius) {
         const double s2 = (radius * radius) / d2 < 1.0 ? (radius * radius) / d2 : 1.0;
@@ -305,7 +412,7 @@ static void rect_light_sample(const ora_object* ob, lquery* q, v3 arr, float* pd
     *pdf = 1.0f / (area_1 + area_2);
     float ccol[3];
     for (int k = 0; k < 3; ++k) ccol[k] = (col[k] * (float)inv_area) * 0.5f;
-    spec_from_color(ccol, q->spec);
+    spec_from_color(ccol, q->wavelength, q->spec);
     for (int k = 0; k < NSPEC; ++k) q->spec[k] *= ob->escale;
 }
 
@@ -426,13 +533,13 @@ static void lambert_differentials(float t, const ora_wray* in, v3 gn, ora_wray* 
     set3(out->ddy, v3sdiv(v3scale(y, 0.15f), len));
 }
 
-static void lambert_eval(const float albedo[3], v3 in, v3 out, v3 gn, float e[NSPEC]) {
+static void lambert_eval(const float albedo[3], v3 in, v3 out, v3 gn, float hero, float e[NSPEC]) {
     v3 nn = v3norm(gn);
     const v3 v = v3norm(out);
     if (v3dot(nn, in) > 0.0f) nn = v3scale(nn, -1.0f);
     const float fac = smax(v3dot(nn, v), 0.0f) * (float)INV_PI;
     const float c[3] = {albedo[0] * fac, albedo[1] * fac, albedo[2] * fac};
-    spec_from_color(c, e);
+    spec_from_color(c, hero, e);
 }
 
 static float lambert_pdf(v3 in, v3 out, v3 gn) {
@@ -506,7 +613,7 @@ static void next_ray(const ora_scene* s, path_t* p, int width, int height, ora_w
         ray->time = p->time;
         ray->type = 2;
         float filt[NSPEC];
-        lambert_eval(p->inter.closure_col, va(p->prev_ray.d), va(ray->d), gn, filt);
+        lambert_eval(p->inter.closure_col, va(p->prev_ray.d), va(ray->d), gn, p->wavelength, filt);
         if (pdf < 0.00001f) pdf = 0.00001f;
         for (int i = 0; i < NSPEC; ++i) p->fcol[i] *= filt[i] / pdf;
         p->last_pdf = pdf;
@@ -520,7 +627,7 @@ static void update_path(const ora_scene* s, path_t* p, const ora_wray* ray, cons
             v3 gp, gn, po;
             hit_world(&p->inter, &gp, &gn, &po);
             float fac[NSPEC];
-            lambert_eval(p->inter.closure_col, va(p->prev_ray.d), va(ray->d), gn, fac);
+            lambert_eval(p->inter.closure_col, va(p->prev_ray.d), va(ray->d), gn, p->wavelength, fac);
             for (int i = 0; i < NSPEC; ++i) p->col[i] += p->fcol[i] * p->lcol[i] * fac[i];
         }
     } else {
@@ -528,7 +635,7 @@ static void update_path(const ora_scene* s, path_t* p, const ora_wray* ray, cons
             if (in->closure == PSY_SHADER_EMIT) {
                 p->done = 1;
                 float em[NSPEC];
-                spec_from_color(in->closure_col, em);
+                spec_from_color(in->closure_col, p->wavelength, em);
                 if (p->last_pdf != 0.0f) {
                     const float a2 = p->last_pdf * p->last_pdf;
                     const float b2 = in->light_pdf * in->light_pdf;
@@ -544,7 +651,7 @@ static void update_path(const ora_scene* s, path_t* p, const ora_wray* ray, cons
         } else {
             p->done = 1;
             float bg[NSPEC];
-            spec_from_color(s->background, bg);
+            spec_from_color(s->background, p->wavelength, bg);
             for (int i = 0; i < NSPEC; ++i) p->col[i] += p->fcol[i] * bg[i];
         }
     }

@@ -115,6 +115,16 @@ int64_t ora_render(const ora_scene* s, int width, int height, int spp, uint32_t 
                    int path_length, int threads, float* path_xyz);
 /* Color_to_SpectralSample value for an achromatic colour */
 float ora_color_to_spectrum(const float c[3]);
+/* The spectrum grid (color/spectrum_grid.h) as data: cells of 8 ints
+ * (inside, num_points, idx[6]), w x h row-major; data points of
+ * 4 + n_samples floats (xystar[2], uv[2], spectrum[n_samples]); the xy -> uv
+ * 3x2 matrix.  NULL cells clears it.  Returns 0, or -1 on bad sizes. */
+int ora_spectrum_set_grid(const int32_t* cells, int w, int h, const float* points, int n_points, int n_samples,
+                          float smin, float smax, const float* mat_xy_to_uv);
+/* spectrum_xyz_to_p (color/spectrum_grid.h:21-135) over the set grid */
+float ora_spectrum_xyz_to_p(float lambda, const float* xyz);
+/* Color_to_SpectralSample (color/color.hpp:509-511) at hero's 4 wavelengths */
+void ora_color_to_spectral_sample(const float c[3], float hero, float e[4]);
 
 #ifdef __cplusplus
 }

@@ -35,4 +35,29 @@ float psyref_invert44(float* m16) {
     return r;
 }
 float psyref_spectrum_xyz_to_p(float lambda, const float* xyz) { return spectrum_xyz_to_p(lambda, xyz); }
+// The grid tables themselves, for callers that hand them to the oracle and
+// the GPU library (which embed none).
+void psyref_spectrum_tables(const int32_t** cells, int* w, int* h, const float** points, int* n_points,
+                            int* n_samples, float* smin, float* smax, const float** mat_xy_to_uv) {
+    static_assert(sizeof(spectrum_grid_cell_t) == 8 * sizeof(int32_t), "cell layout");
+    static_assert(sizeof(spectrum_data_point_t) == (4 + 81) * sizeof(float), "point layout");
+    *cells = (const int32_t*)spectrum_grid;
+    *w = spectrum_grid_width;
+    *h = spectrum_grid_height;
+    *points = (const float*)spectrum_data_points;
+    *n_points = (int)(sizeof(spectrum_data_points) / sizeof(spectrum_data_points[0]));
+    *n_samples = spectrum_num_samples;
+    *smin = spectrum_sample_min;
+    *smax = spectrum_sample_max;
+    *mat_xy_to_uv = spectrum_mat_xy_to_uv;
+}
+float psyref_color_to_spectrum(const float* rgb, float lambda) {
+    // Color_to_spectrum (color/color.hpp:494-496) without the Color type:
+    // Color_to_XYZ then XYZ_to_spectrum, in the reference's float order
+    float xyz[3];
+    xyz[0] = (rgb[0] * 0.4339f) + (rgb[1] * 0.3762f) + (rgb[2] * 0.1899f);
+    xyz[1] = (rgb[0] * 0.2126f) + (rgb[1] * 0.7152f) + (rgb[2] * 0.0722f);
+    xyz[2] = (rgb[0] * 0.0177f) + (rgb[1] * 0.1095f) + (rgb[2] * 0.8728f);
+    return spectrum_xyz_to_p(lambda, xyz) * (1.0f / equal_energy_reflectance);
+}
 }

@@ -108,6 +108,7 @@ EXPORTED_SYMBOLS = [
     "psy_malloc", "psy_free", "psy_copy_to_device", "psy_copy_to_host",
     "psy_scene_create", "psy_scene_destroy", "psy_scene_get_info", "psy_scene_instance_ids",
     "psy_trace_world", "psy_trace_rays", "psy_world_to_rays",
-    "psy_halton", "psy_sobol_set_matrices", "psy_sobol", "psy_pixel_samples", "psy_camera_rays", "psy_bounce_rays", "psy_render",
+    "psy_halton", "psy_sobol_set_matrices", "psy_sobol", "psy_spectrum_set_grid", "psy_color_to_spectrum",
+    "psy_pixel_samples", "psy_camera_rays", "psy_bounce_rays", "psy_render",
     "psy_scene_param_counts", "psy_scene_get_params", "psy_scene_set_params", "psy_adjoint",
 ]

@@ -1068,6 +1068,56 @@ int psy_sobol(psy_ctx* ctx, uint32_t dim, const uint64_t* index, int64_t n, uint
     return rc ? fail(rc, "psy_sobol: launch failed") : PSY_OK;
 }
 
+int psy_spectrum_set_grid(psy_ctx* ctx, const int32_t* cells, int32_t grid_width, int32_t grid_height,
+                          const float* points, int32_t n_points, int32_t n_samples, float sample_min,
+                          float sample_max, const float* mat_xy_to_uv) {
+    if (!ctx) return fail(PSY_ERR_INVALID, "psy_spectrum_set_grid: NULL context");
+    HIP_OK(hipSetDevice(ctx->device));
+    if (!cells) {
+        ctx->spec = DevSpectrum{};
+        return PSY_OK;
+    }
+    if (grid_width < 1 || grid_height < 1 || grid_width > 4096 || grid_height > 4096 || n_points < 1 ||
+        n_samples < 2 || n_samples > 4096 || !points || !mat_xy_to_uv || !(sample_min < sample_max) ||
+        !(sample_min <= 380.0f) || !(sample_max >= 700.0f))
+        return fail(PSY_ERR_INVALID, "psy_spectrum_set_grid: bad sizes (the grid must cover 380..700 nm)");
+    const int64_t n_cells = (int64_t)grid_width * grid_height;
+    for (int64_t c = 0; c < n_cells; ++c) {
+        const int32_t* e = cells + 8 * c;
+        const int inside = e[0], num = e[1];
+        if (inside < 0 || inside > 1 || num < 0 || num > 6 || (inside && num != 4))
+            return fail(PSY_ERR_INVALID, "psy_spectrum_set_grid: bad cell (inside, num_points)");
+        for (int j = 0; j < num; ++j)
+            if (e[2 + j] < 0 || e[2 + j] >= n_points)
+                return fail(PSY_ERR_INVALID, "psy_spectrum_set_grid: cell point index out of range");
+    }
+    for (int k = 0; k < 6; ++k)
+        if (!std::isfinite(mat_xy_to_uv[k])) return fail(PSY_ERR_INVALID, "psy_spectrum_set_grid: bad matrix");
+    if (int rc = ctx->spec_cells.upload(cells, sizeof(int32_t) * 8 * (size_t)n_cells)) return rc;
+    if (int rc = ctx->spec_points.upload(points, sizeof(float) * (size_t)(4 + n_samples) * (size_t)n_points))
+        return rc;
+    DevSpectrum& S = ctx->spec;
+    S.cells = (const int32_t*)ctx->spec_cells.p;
+    S.points = (const float*)ctx->spec_points.p;
+    S.w = grid_width;
+    S.h = grid_height;
+    S.n_points = n_points;
+    S.n_samples = n_samples;
+    S.smin = sample_min;
+    S.smax = sample_max;
+    for (int k = 0; k < 6; ++k) S.m[k] = mat_xy_to_uv[k];
+    return PSY_OK;
+}
+
+int psy_color_to_spectrum(psy_ctx* ctx, const float* rgb, const float* hero_wavelength, int64_t n, float* out,
+                          void* stream) {
+    if (!ctx || n < 0 || ((!rgb || !hero_wavelength || !out) && n > 0))
+        return fail(PSY_ERR_INVALID, "psy_color_to_spectrum: bad argument");
+    HIP_OK(hipSetDevice(ctx->device));
+    int rc = launch_color_spectrum(ctx->spec, rgb, hero_wavelength, n, out, (hipStream_t)stream);
+    return rc ? fail(rc, "psy_color_to_spectrum: launch failed") : PSY_OK;
+}
+
 int psy_pixel_samples(psy_ctx* ctx, int32_t x0, int32_t y0, int32_t w, int32_t h, int32_t spp, uint32_t seed,
                       uint32_t dim0, int32_t n_dims, float* out, void* stream) {
     if (!ctx || w < 0 || h < 0 || spp < 1 || n_dims < 0 || (!out && (int64_t)w * h * n_dims > 0))

@@ -158,6 +158,17 @@ int trace_persist_occupancy(bool world, int stack, int* blocks_per_cu);
 int launch_world_to_rays(const float* wrays, int64_t n, float* rays, hipStream_t st);
 
 // ---- integrator (psy_render_dev.hpp) ----------------------------------------
+// The spectrum grid of colour upsampling (color/spectrum_grid.h) as the
+// caller handed it over: cells of 8 ints (inside, num_points, idx[6]),
+// w x h row-major; data points of 4 + n_samples floats (xystar[2], uv[2],
+// spectrum).  cells == NULL: no grid (grey colours only, flat spectrum).
+struct DevSpectrum {
+    const int32_t* cells;
+    const float* points;
+    int32_t w, h, n_points, n_samples;
+    float smin, smax;
+    float m[6];  // xy -> uv
+};
 struct RenderLaunch {
     DevScene s;
     DevLights L;
@@ -167,6 +178,7 @@ struct RenderLaunch {
     int32_t width, height, spp, path_length;
     uint32_t seed;
     float bg[3];
+    DevSpectrum spec;
 };
 constexpr int PATH_PLANES = 46;  // float planes of a path state
 // Path buffers of one chunk of `cap` paths: states (PATH_PLANES), two world
@@ -245,6 +257,9 @@ int launch_render_count_steps(float* paths, int64_t cap, int64_t n, unsigned lon
 int launch_set_counters(uint32_t* p, uint32_t n_resume, hipStream_t st);
 int launch_sobol(const uint64_t* m, uint32_t n_cols, uint32_t dim, const uint64_t* idx, int64_t n,
                  uint64_t scramble, double* out, hipStream_t st);
+// out[4 i + k] = Color_to_SpectralSample(rgb[3 i ..], hero[i]).e[k]
+int launch_color_spectrum(const DevSpectrum& S, const float* rgb, const float* hero, int64_t n, float* out,
+                          hipStream_t st);
 int launch_halton(const HaltonDim* dims, const uint16_t* perm, uint32_t dim, const uint32_t* idx,
                   int64_t n, float* out, hipStream_t st);
 int launch_pixel_samples(const HaltonDim* dims, const uint16_t* perm, int x0, int y0, int w, int h,

@@ -68,6 +68,8 @@ struct psy_ctx {
     psy::DBuf halton_dims, halton_perm;
     psy::DBuf sobol_m;         // generator matrices, 52 columns per dimension
     uint32_t sobol_dims = 0;
+    psy::DBuf spec_cells, spec_points;  // the spectrum grid (psy_spectrum_set_grid)
+    psy::DevSpectrum spec{};            // cells == NULL: none set
     // trace scratch: queue of rays for the second pass + its counters
     psy::DBuf queue, counters;
     int64_t queue_cap = 0;

@@ -713,6 +713,21 @@ int launch_sobol(const uint64_t* m, uint32_t n_cols, uint32_t dim, const uint64_
     return check_launch();
 }
 
+__global__ void k_color_spectrum(DevSpectrum S, const float* rgb, const float* hero, int64_t n, float* out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float e[NSPEC];
+    color_spec4(S, rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2], hero[i], e);
+    for (int k = 0; k < NSPEC; ++k) out[NSPEC * i + k] = e[k];
+}
+
+int launch_color_spectrum(const DevSpectrum& S, const float* rgb, const float* hero, int64_t n, float* out,
+                          hipStream_t st) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_color_spectrum, dim3(grid_for(n, 256)), dim3(256), 0, st, S, rgb, hero, n, out);
+    return check_launch();
+}
+
 int launch_pixel_samples(const HaltonDim* dims, const uint16_t* perm, int x0, int y0, int w, int h, int spp,
                          uint32_t seed, uint32_t dim0, int n_dims, float* out, hipStream_t st) {
     const int64_t total = (int64_t)w * h * spp;

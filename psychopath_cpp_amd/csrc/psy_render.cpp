@@ -387,10 +387,10 @@ int check_render(const psy_ctx* ctx, const psy_scene* s, const psy_render_params
         return fail(PSY_ERR_INVALID, std::string(who) + ": image sides are limited to 32767 pixels (short pixel coordinates)");
     if (prm->path_length < 1 || prm->path_length > 32)
         return fail(PSY_ERR_INVALID, std::string(who) + ": path_length must be in 1..32");
-    if (s->chromatic)
+    if (s->chromatic && !ctx->spec.cells)
         return fail(PSY_ERR_UNSUPPORTED,
                     std::string(who) + ": the scene has non-grey colours (or surfaces without a shader, which emit "
-                    "magenta); the device integrator upsamples grey colours only");
+                    "magenta); set the spectrum grid (psy_spectrum_set_grid) to upsample them");
     return PSY_OK;
 }
 
@@ -448,6 +448,7 @@ RenderLaunch make_launch(const psy_ctx* ctx, const psy_scene* s, const psy_rende
     R.path_length = prm->path_length;
     R.seed = prm->seed;
     for (int k = 0; k < 3; ++k) R.bg[k] = s->background[k];
+    R.spec = ctx->spec;
     return R;
 }
 
@@ -747,6 +748,9 @@ int psy_render_autotune(psy_ctx* ctx, const psy_scene* s, const psy_render_param
 int psy_adjoint(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, const float* d_film_xyz,
                 float* d_albedo, float* d_emission, int64_t* rays_traced, void* stream) {
     if (int rc = check_render(ctx, s, prm, "psy_adjoint")) return rc;
+    if (s->chromatic)
+        return fail(PSY_ERR_UNSUPPORTED, "psy_adjoint: the scene has non-grey colours; the parameters are grey "
+                    "albedos and emission scales");
     if (!d_film_xyz) return fail(PSY_ERR_INVALID, "psy_adjoint: NULL film gradient");
     if (prm->path_length > ADJ_MAX_VERTS)
         return fail(PSY_ERR_UNSUPPORTED, "psy_adjoint: path_length above 8 (scattering vertices kept per path)");
@@ -778,7 +782,10 @@ int psy_adjoint(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, 
     J.partials = J.grad + f_grad;
     int rc = launch_fill_f32(J.grad, f_grad + f_part, 0.0f, st);
     if (rc) return fail(rc, "psy_adjoint: launch failed");
-    const RenderLaunch R = make_launch(ctx, s, prm);
+    // the replay takes the flat grey spectrum: paths do not depend on
+    // colours, and every contribution is then the same at the 4 wavelengths
+    RenderLaunch R = make_launch(ctx, s, prm);
+    R.spec = DevSpectrum{};
     if ((rc = run_wavefront(ctx, s, prm, R, U, nullptr, nullptr, &J, rays_traced, st))) return rc;
     if ((rc = launch_adjoint_reduce(J, d_albedo, d_emission, st))) return fail(rc, "psy_adjoint: launch failed");
     return PSY_OK;

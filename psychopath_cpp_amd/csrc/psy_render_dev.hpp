@@ -16,9 +16,13 @@
 //   power_heuristic                       utils/mis.hpp:11-16
 //   SpectralSample, Color_XYZ, XYZ_NORM_FAC
 //                                         color/color.hpp:15-188
-// Colours are upsampled for grey (r == g == b) values only: there the
-// Hanika grid's spectrum is flat, (X+Y+Z)/3 at every wavelength
-// (color/color.hpp:490-511).  The host refuses scenes with other colours.
+//   Color_to_SpectralSample, spectrum_xyz_to_p
+//                                         color/color.hpp:490-511,
+//                                         color/spectrum_grid.h:21-135
+// Colours are upsampled through the spectrum grid the caller set on the
+// context; without one, grey (r == g == b) colours take the grid's flat
+// white-point spectrum, (X+Y+Z)/3 at every wavelength, and the host refuses
+// scenes with other colours.
 #pragma once
 
 #include "psy_trace.hpp"
@@ -87,6 +91,94 @@ PSY_DEV float wavelength_n(float hero, int n) {
     hero += (float)n * ((WL_MAX - WL_MIN) / NSPEC);
     if (hero > WL_MAX) hero -= (WL_MAX - WL_MIN);
     return hero;
+}
+
+// Color_to_SpectralSample (color/color.hpp:490-511) at the 4 wavelengths of
+// `hero`.  With a grid, spectrum_xyz_to_p (color/spectrum_grid.h:21-135):
+// the cell of the colour's chromaticity (in the grid's uv space), then at
+// each wavelength the cell's data point spectra, linear between knots,
+// weighted bilinearly in an inside cell or by the barycentric weights of
+// the fan triangle around idx[0] that holds the point; times X+Y+Z and
+// 1 / equal_energy_reflectance.  The cell and its weights do not depend on
+// the wavelength and are found once.  Outside the grid (or a NaN
+// chromaticity) the spectrum is 0.
+PSY_DEV void color_spec4(const DevSpectrum& S, float r, float g, float b, float hero, float out[NSPEC]) {
+    if (!S.cells) {
+        const float v = color_spec(r, g, b);
+        for (int k = 0; k < NSPEC; ++k) out[k] = v;
+        return;
+    }
+    for (int k = 0; k < NSPEC; ++k) out[k] = 0.0f;
+    const float x = ((r * 0.4339f) + (g * 0.3762f)) + (b * 0.1899f);
+    const float y = ((r * 0.2126f) + (g * 0.7152f)) + (b * 0.0722f);
+    const float z = ((r * 0.0177f) + (g * 0.1095f)) + (b * 0.8728f);
+    const float norm = (float)(1.0 / (double)(x + y + z));
+    if (!(norm < 3.402823466e+38f)) return;
+    const float cx = x * norm, cy = y * norm;
+    const float u = ((S.m[0] * cx) + (S.m[1] * cy)) + S.m[2];
+    const float v = ((S.m[3] * cx) + (S.m[4] * cy)) + S.m[5];
+    if (!(u >= 0.0f && u < (float)S.w && v >= 0.0f && v < (float)S.h)) return;
+    const int ui = (int)u, vi = (int)v;
+    const int32_t* cell = S.cells + 8 * (ui + S.w * vi);
+    const int inside = cell[0], num = cell[1];
+    const int stride = 4 + S.n_samples;
+    // corner/fan weights: inside -> bilinear in (fu, fv); fan -> triangle
+    // (idx[0], idx[t1], idx[t2]) with weights (bw, bv, bu)
+    float fu = 0.0f, fv = 0.0f, bw = 0.0f, bv = 0.0f, bu = 0.0f;
+    int t1 = -1, t2 = -1;
+    if (inside) {
+        fu = u - (float)ui;
+        fv = v - (float)vi;
+    } else if (num > 0) {
+        const float* q0 = S.points + (int64_t)stride * cell[2];
+        const float* q1 = S.points + (int64_t)stride * cell[3];
+        const float ex = u - q0[2], ey = v - q0[3];
+        float e0x = q1[2] - q0[2], e0y = q1[3] - q0[3];
+        float uu = (e0x * ey) - (ex * e0y);
+        for (int i = 0; i < num - 1; ++i) {
+            const int j = i == num - 2 ? 1 : i + 2;
+            const float* qn = S.points + (int64_t)stride * cell[2 + j];
+            const float e1x = qn[2] - q0[2], e1y = qn[3] - q0[3];
+            const float vv = (ex * e1y) - (e1x * ey);
+            const float area = (e0x * e1y) - (e1x * e0y);
+            const float cu = uu / area, cv = vv / area;
+            const float cw = (1.0f - cu) - cv;
+            if (cu < 0.0f || cv < 0.0f || cw < 0.0f) {
+                uu = -vv;
+                e0x = e1x;
+                e0y = e1y;
+                continue;
+            }
+            t1 = i + 1;
+            t2 = j;
+            bw = cw;
+            bv = cv;
+            bu = cu;
+            break;
+        }
+        if (t1 < 0) return;
+    } else {
+        return;
+    }
+    for (int k = 0; k < NSPEC; ++k) {
+        const float lam = wavelength_n(hero, k);
+        const float sb = ((lam - S.smin) / (S.smax - S.smin)) * (float)(S.n_samples - 1);
+        const int sb0 = (int)sb;
+        const int sb1 = sb + 1.0f < (float)S.n_samples ? (int)(sb + 1.0f) : S.n_samples - 1;
+        const float sbf = sb - (float)sb0;
+        auto pv = [&](int c) {
+            const float* sp = S.points + (int64_t)stride * cell[2 + c] + 4;
+            return (sp[sb0] * (1.0f - sbf)) + (sp[sb1] * sbf);
+        };
+        float ip;
+        if (inside) {
+            ip = ((((pv(0) * (1.0f - fu)) * (1.0f - fv)) + ((pv(2) * (1.0f - fu)) * fv)) + ((pv(3) * fu) * fv)) +
+                 ((pv(1) * fu) * (1.0f - fv));
+        } else {
+            ip = ((pv(0) * bw) + (pv(t1) * bv)) + (pv(t2) * bu);
+        }
+        out[k] = (ip / norm) * (1.0f / 0.009358239977091027f);
+    }
 }
 
 // Wyman et al. CIE 1931 fits (color/color.hpp:150-165)
@@ -207,13 +299,24 @@ PSY_DEV_FN float lambert_estimate(V3 in, V3 out, float cos_theta, V3 nor) {
     return sphere_lambert(cos_nv, cos_theta);
 }
 
-// LambertClosure::evaluate: the grey albedo * max(0, n.v)/pi, upsampled
-PSY_DEV float lambert_eval(const float albedo[3], V3 in, V3 out, V3 gn) {
+// LambertClosure::evaluate: the albedo * max(0, n.v)/pi, upsampled at the
+// 4 wavelengths of `hero`
+PSY_DEV void lambert_eval(const DevSpectrum& S, const float albedo[3], V3 in, V3 out, V3 gn, float hero,
+                          float e[NSPEC]) {
     V3 nn = normalized(gn);
     const V3 v = normalized(out);
     if (dot(nn, in) > 0.0f) nn = scale(nn, -1.0f);
     const float fac = smax(dot(nn, v), 0.0f) * (float)(1.0 / PSY_PI_D);
-    return color_spec(albedo[0] * fac, albedo[1] * fac, albedo[2] * fac);
+    color_spec4(S, albedo[0] * fac, albedo[1] * fac, albedo[2] * fac, hero, e);
+}
+// The same for the grey albedo 1 on the flat spectrum (the adjoint's
+// parameter-free factor)
+PSY_DEV float lambert_eval1(V3 in, V3 out, V3 gn) {
+    V3 nn = normalized(gn);
+    const V3 v = normalized(out);
+    if (dot(nn, in) > 0.0f) nn = scale(nn, -1.0f);
+    const float fac = smax(dot(nn, v), 0.0f) * (float)(1.0 / PSY_PI_D);
+    return color_spec(fac, fac, fac);
 }
 
 // LambertClosure::sample_pdf
@@ -283,16 +386,17 @@ struct LQuery {
     float n, u, v;
     V3 pos, nor, d;
     float time;
+    float wl;         // hero wavelength
     M44 xform;
     V3 to_light;
-    float spec;       // the light's emission at the sample, times its emission scale
-    float spec_base;  // the emission without the scale
+    float spec[NSPEC];       // the light's emission at the sample, times its emission scale
+    float spec_base[NSPEC];  // the emission without the scale
     float scale;      // that emission scale
     int32_t obj;      // the sampled light object
     float selection_pdf, light_sample_pdf;
 };
 
-PSY_DEV_FN void sphere_light_sample(const float* od, int ts, LQuery& q, float* pdf) {
+PSY_DEV_FN void sphere_light_sample(const DevSpectrum& SP, const float* od, int ts, LQuery& q, float* pdf) {
     V3 pos; float radf; float col[3];
     sphere_light_at(od, ts, q.time, &pos, &radf, col);
     const V3 arr = q.pos;
@@ -303,8 +407,8 @@ PSY_DEV_FN void sphere_light_sample(const float* od, int ts, LQuery& q, float* p
     const double d = sqrt(d2);
     coordinate_system_from_vec3(z, &x, &y);
     x = normalized(x); y = normalized(y); z = normalized(z);
-    q.spec = color_spec(col[0] * (float)surface_area_inv, col[1] * (float)surface_area_inv,
-                        col[2] * (float)surface_area_inv);
+    color_spec4(SP, col[0] * (float)surface_area_inv, col[1] * (float)surface_area_inv,
+                col[2] * (float)surface_area_inv, q.wl, q.spec);
     if (d > radius) {
         const double s2 = (radius * radius) / d2 < 1.0 ? (radius * radius) / d2 : 1.0;
         const double c2 = 1.0 - s2;
@@ -338,7 +442,7 @@ PSY_DEV_FN void sphere_light_sample(const float* od, int ts, LQuery& q, float* p
     }
 }
 
-PSY_DEV_FN void rect_light_sample(const float* od, int ts, LQuery& q, float* pdf) {
+PSY_DEV_FN void rect_light_sample(const DevSpectrum& SP, const float* od, int ts, LQuery& q, float* pdf) {
     float w, h, col[3];
     rect_light_at(od, ts, q.time, &w, &h, col);
     const V3 arr = q.pos;
@@ -357,8 +461,8 @@ PSY_DEV_FN void rect_light_sample(const float* od, int ts, LQuery& q, float* pdf
     sv = scale(sv, -arr.z / sv.z);
     q.to_light = sv;
     *pdf = 1.0f / (area_1 + area_2);
-    q.spec = color_spec((col[0] * (float)inv_area) * 0.5f, (col[1] * (float)inv_area) * 0.5f,
-                        (col[2] * (float)inv_area) * 0.5f);
+    color_spec4(SP, (col[0] * (float)inv_area) * 0.5f, (col[1] * (float)inv_area) * 0.5f,
+                (col[2] * (float)inv_area) * 0.5f, q.wl, q.spec);
 }
 
 PSY_DEV void lbox(const DevLights& L, int b, V3* lo, V3* hi) {
@@ -400,7 +504,7 @@ PSY_DEV_FN float node_prob(const DevLights& L, const LQuery& q, int index) {
 
 // LightTree::sample, descending through sub-assembly trees.  False when the
 // scene has no light.
-PSY_DEV_FN bool light_tree_sample(const DevScene& s, const DevLights& L, LQuery& q) {
+PSY_DEV_FN bool light_tree_sample(const DevScene& s, const DevLights& L, const DevSpectrum& SP, LQuery& q) {
     int a = 0;
     for (int guard = 0; guard < 64; ++guard) {
         const DevAsm A = L.asms[a];
@@ -442,14 +546,16 @@ PSY_DEV_FN bool light_tree_sample(const DevScene& s, const DevLights& L, LQuery&
             const DevObject ob = s.obj[A.obj0 + in.data_index];
             const float* od = s.odata + ob.data;
             float p = 1.0f;
-            if (ob.kind == K_SPHERE_LIGHT) sphere_light_sample(od, ob.ts, q, &p);
-            else if (ob.kind == K_RECT_LIGHT) rect_light_sample(od, ob.ts, q, &p);
+            if (ob.kind == K_SPHERE_LIGHT) sphere_light_sample(SP, od, ob.ts, q, &p);
+            else if (ob.kind == K_RECT_LIGHT) rect_light_sample(SP, od, ob.ts, q, &p);
             else return false;
             q.to_light = m_dir(m_inverse(q.xform), q.to_light);
             q.obj = A.obj0 + in.data_index;
             q.scale = s.escale[q.obj];
-            q.spec_base = q.spec;
-            q.spec *= q.scale;
+            for (int k = 0; k < NSPEC; ++k) {
+                q.spec_base[k] = q.spec[k];
+                q.spec[k] *= q.scale;
+            }
             q.selection_pdf *= (tot_prob * (float)A.total_lights);
             q.light_sample_pdf = p;
             return true;
@@ -544,8 +650,8 @@ PSY_DEV void render_init_one(const RenderLaunch& A, int64_t first, int64_t i, co
 // Emission and light pdf of a camera / bounce ray that ended on a light
 // object (SphereLight / RectangleLight::intersect_ray closures).
 // *spec is the emission times the object's emission scale, *base without it.
-PSY_DEV_FN void light_hit(const DevScene& s, uint32_t li, V3 wo, float time, float* spec, float* base,
-                          float* light_pdf) {
+PSY_DEV_FN void light_hit(const DevScene& s, const DevSpectrum& SP, uint32_t li, V3 wo, float time, float wl,
+                          float spec[NSPEC], float base[NSPEC], float* light_pdf) {
     const DevInstance I = s.inst[li];
     const DevObject ob = s.obj[I.object];
     V3 o = wo;
@@ -556,16 +662,16 @@ PSY_DEV_FN void light_hit(const DevScene& s, uint32_t li, V3 wo, float time, flo
         rect_light_at(od, ob.ts, time, &w, &h, col);
         *light_pdf = rect_light_pdf(w, h, o);
         const double area = (double)(w * h);
-        *base = color_spec((col[0] * 0.5f) / (float)area, (col[1] * 0.5f) / (float)area,
-                           (col[2] * 0.5f) / (float)area);
+        color_spec4(SP, (col[0] * 0.5f) / (float)area, (col[1] * 0.5f) / (float)area,
+                    (col[2] * 0.5f) / (float)area, wl, base);
     } else {
         V3 c; float r, col[3];
         sphere_light_at(od, ob.ts, time, &c, &r, col);
         const double area = 4.0 * PSY_PI_D * r * r;
         *light_pdf = sphere_light_pdf(c, (double)r, o);
-        *base = color_spec(col[0] / (float)area, col[1] / (float)area, col[2] / (float)area);
+        color_spec4(SP, col[0] / (float)area, col[1] / (float)area, col[2] / (float)area, wl, base);
     }
-    *spec = *base * s.escale[I.object];
+    for (int k = 0; k < NSPEC; ++k) spec[k] = base[k] * s.escale[I.object];
 }
 
 
@@ -648,7 +754,6 @@ PSY_DEV int64_t path_sample(const RenderLaunch& A, const PathBuf& P, uint32_t i)
 template <class Adj>
 PSY_DEV bool render_update_path(const RenderLaunch& A, int64_t first, const PathBuf& P, uint32_t i, const WRay& ray,
                                 const HitRec& hr, float* sample_xyz, int64_t sbase, WRay& nr, const Adj& adj) {
-    const float one[3] = {1.0f, 1.0f, 1.0f};
     bool alive = false;
     V3 no = splat(0.0f), nd = splat(0.0f), nodx = splat(0.0f), nody = splat(0.0f), nddx = splat(0.0f),
        nddy = splat(0.0f);
@@ -669,11 +774,12 @@ PSY_DEV bool render_update_path(const RenderLaunch& A, int64_t first, const Path
             if (!hit) {
                 const uint32_t sh = P.u(P_SHADER, i);
                 const float* alb = s.shaders[sh].color;
-                const float fac = lambert_eval(alb, P.v3(P_PD, i), rd, P.v3(P_GN, i));
-                for (int k = 0; k < NSPEC; ++k) col[k] += fcol[k] * P.f(P_LCOL + k, i) * fac;
+                float fac[NSPEC];
+                lambert_eval(A.spec, alb, P.v3(P_PD, i), rd, P.v3(P_GN, i), P.f(P_WL, i), fac);
+                for (int k = 0; k < NSPEC; ++k) col[k] += fcol[k] * P.f(P_LCOL + k, i) * fac[k];
                 if constexpr (Adj::on) {
                     const Adj& Z = adj;
-                    const float fac1 = lambert_eval(one, P.v3(P_PD, i), rd, P.v3(P_GN, i));
+                    const float fac1 = lambert_eval1(P.v3(P_PD, i), rd, P.v3(P_GN, i));
                     const float wb = Z.Q.f(Q_W, i) * ((Z.Q.f(Q_B, i) * Z.Q.f(Q_LB, i)) * fac1);
                     adj_contrib(s, Z, i, (step >> 1) + 1, wb, (int32_t)Z.Q.u(Q_LOBJ, i));
                 }
@@ -683,17 +789,20 @@ PSY_DEV bool render_update_path(const RenderLaunch& A, int64_t first, const Path
             const V3 ro = ray.o;
             const float time = ray.time;
             bool emit = false;
-            float em = 0.0f, em_base = 0.0f, light_pdf = 9999.0f;
+            float em[NSPEC], em_base[NSPEC], light_pdf = 9999.0f;
             int32_t em_obj = -1;
             if (I.kind == K_SPHERE_LIGHT || I.kind == K_RECT_LIGHT) {
                 emit = true;
-                light_hit(s, hinst, ro, time, &em, &em_base, &light_pdf);
+                light_hit(s, A.spec, hinst, ro, time, P.f(P_WL, i), em, em_base, &light_pdf);
                 em_obj = I.object;
-            } else if (I.shader >= 0 && s.shaders[I.shader].kind == PSY_SHADER_EMIT) {
+            } else if (I.shader < 0 || s.shaders[I.shader].kind == PSY_SHADER_EMIT) {
+                // an EmitShader, or no shader at all: the reference's
+                // magenta emitter (Color(1, 0, 1), no MIS light pdf)
                 emit = true;
-                const float* c = s.shaders[I.shader].color;
-                em = color_spec(c[0], c[1], c[2]);
-                em_base = em;
+                const float magenta[3] = {1.0f, 0.0f, 1.0f};
+                const float* c = I.shader < 0 ? magenta : s.shaders[I.shader].color;
+                color_spec4(A.spec, c[0], c[1], c[2], P.f(P_WL, i), em);
+                for (int k = 0; k < NSPEC; ++k) em_base[k] = em[k];
             }
             if (emit) {
                 done = true;
@@ -703,11 +812,11 @@ PSY_DEV bool render_update_path(const RenderLaunch& A, int64_t first, const Path
                     const float a2 = last_pdf * last_pdf;
                     const float b2 = light_pdf * light_pdf;
                     const float mis_inv_pdf = (a2 / (a2 + b2)) / last_pdf;
-                    for (int k = 0; k < NSPEC; ++k) col[k] += ((fcol[k] * last_pdf) * em) * mis_inv_pdf;
-                    if constexpr (Adj::on) wb = ((adj.Q.f(Q_B, i) * last_pdf) * em_base) * mis_inv_pdf;
+                    for (int k = 0; k < NSPEC; ++k) col[k] += ((fcol[k] * last_pdf) * em[k]) * mis_inv_pdf;
+                    if constexpr (Adj::on) wb = ((adj.Q.f(Q_B, i) * last_pdf) * em_base[0]) * mis_inv_pdf;
                 } else {
-                    for (int k = 0; k < NSPEC; ++k) col[k] += fcol[k] * em;
-                    if constexpr (Adj::on) wb = adj.Q.f(Q_B, i) * em_base;
+                    for (int k = 0; k < NSPEC; ++k) col[k] += fcol[k] * em[k];
+                    if constexpr (Adj::on) wb = adj.Q.f(Q_B, i) * em_base[0];
                 }
                 if constexpr (Adj::on) adj_contrib(s, adj, i, step >> 1, adj.Q.f(Q_W, i) * wb, em_obj);
             } else {
@@ -739,9 +848,11 @@ PSY_DEV bool render_update_path(const RenderLaunch& A, int64_t first, const Path
             }
         } else {
             done = true;
-            const float bg = color_spec(A.bg[0], A.bg[1], A.bg[2]);
-            for (int k = 0; k < NSPEC; ++k) col[k] += fcol[k] * bg;
-            if constexpr (Adj::on) adj_contrib(s, adj, i, step >> 1, adj.Q.f(Q_W, i) * (adj.Q.f(Q_B, i) * bg), -1);
+            float bg[NSPEC];
+            color_spec4(A.spec, A.bg[0], A.bg[1], A.bg[2], P.f(P_WL, i), bg);
+            for (int k = 0; k < NSPEC; ++k) col[k] += fcol[k] * bg[k];
+            if constexpr (Adj::on)
+                adj_contrib(s, adj, i, step >> 1, adj.Q.f(Q_W, i) * (adj.Q.f(Q_B, i) * bg[0]), -1);
         }
         const int nstep = step + 1;
         if (nstep == A.path_length * 2) done = true;
@@ -766,16 +877,19 @@ PSY_DEV bool render_update_path(const RenderLaunch& A, int64_t first, const Path
                 q.nor = lq_nor;
                 q.d = pd;
                 q.time = time;
+                q.wl = P.f(P_WL, i);
                 q.xform = m_identity();
                 q.to_light = splat(0.0f);
-                q.spec = 0.0f;
-                q.spec_base = 0.0f;
+                for (int k = 0; k < NSPEC; ++k) {
+                    q.spec[k] = 0.0f;
+                    q.spec_base[k] = 0.0f;
+                }
                 q.scale = 1.0f;
                 q.obj = -1;
                 q.selection_pdf = 1.0f;
                 q.light_sample_pdf = 1.0f;
-                float lcol = 0.0f, lbase = 0.0f;
-                if (!light_tree_sample(s, A.L, q)) {
+                float lcol[NSPEC] = {0.0f, 0.0f, 0.0f, 0.0f}, lbase = 0.0f;
+                if (!light_tree_sample(s, A.L, A.spec, q)) {
                     q.to_light = mk(0.0f, 0.0f, 1.0f);
                     q.obj = -1;
                 } else {
@@ -783,11 +897,12 @@ PSY_DEV bool render_update_path(const RenderLaunch& A, int64_t first, const Path
                     const float a2 = q.light_sample_pdf * q.light_sample_pdf;
                     const float b2 = bsdf_pdf * bsdf_pdf;
                     const float mis_inv_pdf = (a2 / (a2 + b2)) / q.light_sample_pdf;
-                    lcol = ((q.spec * mis_inv_pdf) * (float)A.L.asms[0].total_lights) / q.selection_pdf;
+                    for (int k = 0; k < NSPEC; ++k)
+                        lcol[k] = ((q.spec[k] * mis_inv_pdf) * (float)A.L.asms[0].total_lights) / q.selection_pdf;
                     if constexpr (Adj::on)
-                        lbase = ((q.spec_base * mis_inv_pdf) * (float)A.L.asms[0].total_lights) / q.selection_pdf;
+                        lbase = ((q.spec_base[0] * mis_inv_pdf) * (float)A.L.asms[0].total_lights) / q.selection_pdf;
                 }
-                for (int k = 0; k < NSPEC; ++k) P.f(P_LCOL + k, i) = lcol;
+                for (int k = 0; k < NSPEC; ++k) P.f(P_LCOL + k, i) = lcol[k];
                 if constexpr (Adj::on) {
                     adj.Q.f(Q_LB, i) = lbase;
                     adj.Q.setu(Q_LOBJ, i, (uint32_t)q.obj);
@@ -811,10 +926,11 @@ PSY_DEV bool render_update_path(const RenderLaunch& A, int64_t first, const Path
                 nd = zup_to_vec(dir, nn);
                 no = add(gp, po);
                 ntype = 2u;  // WorldRay::R_DIFFUSE
-                const float filt = lambert_eval(alb, pd, nd, gn);
+                float filt[NSPEC];
+                lambert_eval(A.spec, alb, pd, nd, gn, P.f(P_WL, i), filt);
                 if (pdf < 0.00001f) pdf = 0.00001f;
-                for (int k = 0; k < NSPEC; ++k) fcol[k] *= filt / pdf;
-                if constexpr (Adj::on) adj.Q.f(Q_B, i) *= lambert_eval(one, pd, nd, gn) / pdf;
+                for (int k = 0; k < NSPEC; ++k) fcol[k] *= filt[k] / pdf;
+                if constexpr (Adj::on) adj.Q.f(Q_B, i) *= lambert_eval1(pd, nd, gn) / pdf;
                 P.f(P_LASTPDF, i) = pdf;
             }
             // LambertClosure::propagate_differentials

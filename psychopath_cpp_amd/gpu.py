@@ -54,6 +54,9 @@ def lib():
     L.psy_ctx_trace_timing.argtypes = [vp, vp, vp, vp]
     L.psy_sobol_set_matrices.argtypes = [vp, vp, u32]
     L.psy_sobol.argtypes = [vp, u32, vp, i64, C.c_uint64, vp, vp]
+    L.psy_spectrum_set_grid.argtypes = [vp, vp, C.c_int32, C.c_int32, vp, C.c_int32, C.c_int32, C.c_float,
+                                        C.c_float, vp]
+    L.psy_color_to_spectrum.argtypes = [vp, vp, vp, i64, vp, vp]
     L.psy_pixel_samples.argtypes = [vp, i32, i32, i32, i32, i32, u32, u32, i32, vp, vp]
     L.psy_camera_rays.argtypes = [vp, vp, i32, i32, i32, u32, vp, vp, vp]
     L.psy_bounce_rays.argtypes = [vp, vp, vp, C.POINTER(abi.Hits), i64, i32, i32, u32, u32, u32, vp, vp, vp]
@@ -267,6 +270,40 @@ def sobol(ctx, dim, index, scramble=0):
     out = torch.empty(index.shape, dtype=torch.float64, device=index.device)
     check(lib().psy_sobol(ctx.handle, dim, _ptr(index), index.numel(), int(scramble) & (2**64 - 1), _ptr(out),
                           _stream()), "psy_sobol")
+    return out
+
+
+def spectrum_set_grid(ctx, cells, w, h, points, n_samples, sample_min, sample_max, mat_xy_to_uv):
+    """The spectrum grid of colour upsampling (color/spectrum_grid.h) as
+    arrays: cells (w*h, 8) int32 = inside, num_points, idx[6]; points
+    (n, 4 + n_samples) float32 = xystar, uv, spectrum; the xy -> uv matrix
+    (6 floats).  cells None clears it (grey colours only)."""
+    if cells is None:
+        check(lib().psy_spectrum_set_grid(ctx.handle, None, 0, 0, None, 0, 0, 0.0, 0.0, None),
+              "psy_spectrum_set_grid")
+        return
+    c = np.ascontiguousarray(cells, np.int32).reshape(-1)
+    p = np.ascontiguousarray(points, np.float32).reshape(-1)
+    m = np.ascontiguousarray(mat_xy_to_uv, np.float32).reshape(-1)
+    if c.size != 8 * w * h or p.size % (4 + n_samples) or m.size != 6:
+        raise PsyError("spectrum grid arrays do not match their sizes")
+    check(lib().psy_spectrum_set_grid(ctx.handle, c.ctypes.data_as(C.c_void_p), w, h, p.ctypes.data_as(C.c_void_p),
+                                      p.size // (4 + n_samples), n_samples, sample_min, sample_max,
+                                      m.ctypes.data_as(C.c_void_p)), "psy_spectrum_set_grid")
+
+
+def color_to_spectrum(ctx, rgb, hero):
+    """Color_to_SpectralSample for CUDA float32 tensors rgb (n, 3) and hero
+    wavelengths (n,): (n, 4) spectral samples."""
+    for t in (rgb, hero):
+        if not (t.is_cuda and t.dtype == torch.float32 and t.is_contiguous()):
+            raise PsyError("rgb and hero must be contiguous CUDA float32 tensors")
+    n = hero.numel()
+    if rgb.numel() != 3 * n:
+        raise PsyError("rgb must hold 3 floats per wavelength")
+    out = torch.empty((n, 4), dtype=torch.float32, device=hero.device)
+    check(lib().psy_color_to_spectrum(ctx.handle, _ptr(rgb), _ptr(hero), n, _ptr(out), _stream()),
+          "psy_color_to_spectrum")
     return out
 
 

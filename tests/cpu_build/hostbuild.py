@@ -47,6 +47,7 @@ def lib():
                             C.c_int]
     L.hb_render_params.argtypes = [C.POINTER(abi.SceneDesc), C.c_int, C.POINTER(abi.RenderParams), vp, C.c_int, vp,
                                    C.c_int, vp, vp, vp, C.POINTER(i64), C.c_int]
+    L.hb_set_spectrum_grid.argtypes = [vp, C.c_int, C.c_int, vp, C.c_int, C.c_int, C.c_float, C.c_float, vp]
     _lib = L
     return L
 
@@ -108,6 +109,17 @@ def render_params(scene_py, width, height, spp, albedo=None, escale=None, d_film
     if rc != 0:
         raise RuntimeError(f"hb_render_params failed with status {rc}")
     return (out if d_film is None else grad[:sum(n_params)]), rays.value
+
+
+def set_spectrum_grid(grid):
+    """Hand the host-built integrator a spectrum grid (an object with the
+    arrays of oracle.ora.SpectrumGrid), or None to clear it."""
+    L = lib()
+    if grid is None:
+        L.hb_set_spectrum_grid(None, 0, 0, None, 0, 0, 0.0, 0.0, None)
+    else:
+        L.hb_set_spectrum_grid(_p(grid.cells), grid.w, grid.h, _p(grid.points), grid.n_points, grid.n_samples,
+                               grid.smin, grid.smax, _p(grid.mat))
 
 
 def param_counts(desc):

@@ -147,13 +147,36 @@ extern "C" {
 // psy_render (d_film == NULL) or psy_adjoint on the CPU: one chunk holding
 // every path, the wavefront loop with in-order compaction.  albedo /
 // escale: psy_scene_set_params values or NULL.  sample_xyz:
+// The spectrum grid the host-built integrator upsamples colours with (as
+// psy_spectrum_set_grid; cells == NULL clears it).
+static std::vector<int32_t> g_spec_cells;
+static std::vector<float> g_spec_points;
+static DevSpectrum g_spec{};
+
+void hb_set_spectrum_grid(const int32_t* cells, int w, int h, const float* points, int n_points, int n_samples,
+                          float smin, float smax, const float* mat) {
+    g_spec = DevSpectrum{};
+    if (!cells) return;
+    g_spec_cells.assign(cells, cells + 8 * (size_t)w * h);
+    g_spec_points.assign(points, points + (size_t)(4 + n_samples) * n_points);
+    g_spec.cells = g_spec_cells.data();
+    g_spec.points = g_spec_points.data();
+    g_spec.w = w;
+    g_spec.h = h;
+    g_spec.n_points = n_points;
+    g_spec.n_samples = n_samples;
+    g_spec.smin = smin;
+    g_spec.smax = smax;
+    for (int k = 0; k < 6; ++k) g_spec.m[k] = mat[k];
+}
+
 // width*height*spp*3 floats (render); grad: n_albedo + n_lights (adjoint).
 int hb_render_params(const psy_scene_desc* d, int dice_depth, const psy_render_params* prm, const float* albedo,
                      int n_albedo, const float* escale, int n_escale, float* sample_xyz, const float* d_film,
                      float* grad, int64_t* rays_traced, int threads) {
     HostBuilt B;
     if (int rc = build(d, dice_depth, B)) return rc;
-    if (B.H.chromatic) return PSY_ERR_UNSUPPORTED;
+    if (B.H.chromatic && (d_film || !g_spec.cells)) return PSY_ERR_UNSUPPORTED;
     if (albedo) {
         if (n_albedo != (int)B.H.shaders.size()) return PSY_ERR_INVALID;
         for (int i = 0; i < n_albedo; ++i)
@@ -187,6 +210,7 @@ int hb_render_params(const psy_scene_desc* d, int dice_depth, const psy_render_p
     R.path_length = prm->path_length;
     R.seed = prm->seed;
     for (int k = 0; k < 3; ++k) R.bg[k] = B.H.background[k];
+    R.spec = d_film ? DevSpectrum{} : g_spec;
     std::vector<float> qplanes(d_film ? (size_t)(ADJ_STATE_PLANES * cap) : 0);
     const int n_shaders = (int)B.H.shaders.size();
     HostAdj adj{PathBuf{qplanes.data(), cap}, n_shaders, B.H.light_param.data(), grad};

@@ -137,6 +137,34 @@ def test_cpu_build_render_matches_oracle(name):
     assert abs(hrays - orays) <= 0.01 * orays
 
 
+def test_cpu_build_chromatic_render_matches_oracle():
+    """Coloured albedos, lights, emitters, background and the no-shader
+    magenta emitter through the reference's spectrum grid: the integrator's
+    per-wavelength colour code against the oracle's."""
+    G = ora.ref_spectrum_grid()
+    if G is None:
+        pytest.skip("oracle/_ref not built: no spectrum grid")
+    sc = R._chromatic_scene()
+    W, H, spp = 24, 16, 4
+    with pytest.raises(RuntimeError):
+        ora.OracleScene(sc).render(W, H, spp, 1, threads=8)  # no grid: chromatic refused
+    ora.spectrum_set_grid(G)
+    hostbuild.set_spectrum_grid(G)
+    try:
+        hs, hrays = hostbuild.render(sc, W, H, spp, seed=1)
+        os_, orays = ora.OracleScene(sc).render(W, H, spp, 1, threads=8)
+    finally:
+        ora.spectrum_set_grid(None)
+        hostbuild.set_spectrum_grid(None)
+    worst, close = R.tile_check(hs, os_)
+    print("chromatic rays", hrays, orays, "close", close, "worst tile z", worst)
+    assert np.isfinite(hs).all() and hs.mean() > 0.0
+    assert close >= 0.95 and worst <= 3.0
+    assert abs(hrays - orays) <= 0.01 * orays
+    m = hs.reshape(-1, 3).mean(axis=0)
+    assert m.max() > 1.2 * m.min()
+
+
 def test_cpu_build_adjoint_matches_finite_differences():
     """The adjoint pass (psy_render_dev.hpp's path replay, compiled for the
     host) against central finite differences of the oracle's renderer and of

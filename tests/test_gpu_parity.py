@@ -362,6 +362,60 @@ def test_gpu_stack_overflow_retrace(gpu_ctx, oracle):
         ctx.close()
 
 
+def test_gpu_color_to_spectrum_bit_exact(oracle):
+    """psy_color_to_spectrum over the reference's spectrum grid (from
+    oracle/_ref) against the reference's own Color_to_spectrum and the
+    oracle, bit for bit; without a grid, the flat spectrum for grey."""
+    G = oracle.ref_spectrum_grid()
+    if G is None:
+        pytest.skip("oracle/_ref not built: no spectrum grid")
+    R = oracle.ref()
+    rng = np.random.default_rng(17)
+    n = 1 << 14
+    rgb = (rng.uniform(0, 1, (n, 3)) ** 2 * rng.uniform(0.001, 100, (n, 1))).astype(np.float32)
+    rgb[:64] = rng.uniform(-0.5, 3.0, (64, 3))  # out of gamut, negative
+    rgb[64] = 0.0
+    rgb[65:70] = rgb[65:70, :1]  # grey
+    hero = rng.uniform(380, 700, n).astype(np.float32)
+    hero[:2] = (380.0, 700.0)
+    ctx = gpu.Context(0)
+    try:
+        grey = torch.from_numpy(np.repeat(rgb[:, :1], 3, axis=1)).cuda()
+        flat = gpu.color_to_spectrum(ctx, grey, torch.from_numpy(hero).cuda()).cpu().numpy()
+        want_flat = np.array([oracle.lib().ora_color_to_spectrum(np.ascontiguousarray(c).ctypes.data_as(
+            __import__("ctypes").c_void_p)) for c in grey.cpu().numpy()[:512]], np.float32)
+        assert np.array_equal(flat[:512, 0].view(np.uint32), want_flat.view(np.uint32))
+        assert (flat == flat[:, :1]).all()
+        gpu.spectrum_set_grid(ctx, G.cells, G.w, G.h, G.points, G.n_samples, G.smin, G.smax, G.mat)
+        got = gpu.color_to_spectrum(ctx, torch.from_numpy(rgb).cuda(), torch.from_numpy(hero).cuda()).cpu().numpy()
+    finally:
+        ctx.close()
+    oracle.spectrum_set_grid(G)
+    try:
+        want = np.stack([oracle.color_to_spectral_sample(rgb[i], hero[i]) for i in range(n)])
+    finally:
+        oracle.spectrum_set_grid(None)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    for i in range(0, n, 7):
+        for k in range(4):
+            lam = np.float32(hero[i] + np.float32(k) * np.float32(80.0))
+            if lam > 700.0:
+                lam = np.float32(lam - np.float32(320.0))
+            r = np.float32(R.psyref_color_to_spectrum(rgb[i].ctypes.data_as(__import__("ctypes").c_void_p),
+                                                      float(lam)))
+            assert got[i, k].view(np.uint32) == r.view(np.uint32), (i, k, rgb[i], hero[i])
+    say("color_to_spectrum: bit-exact on", n, "colours x 4 wavelengths")
+    # a bad table is refused
+    ctx = gpu.Context(0)
+    try:
+        bad = G.cells.copy()
+        bad[0, 1] = 7
+        with pytest.raises(gpu.PsyError):
+            gpu.spectrum_set_grid(ctx, bad, G.w, G.h, G.points, G.n_samples, G.smin, G.smax, G.mat)
+    finally:
+        ctx.close()
+
+
 def test_gpu_sobol_bit_exact(gpu_ctx, oracle):
     """psy_sobol against the reference's own outputs (with its generator
     table from oracle/_ref) and against the oracle (with matrices built

@@ -95,6 +95,89 @@ def _instanced_lights_scene():
     return sc
 
 
+def _chromatic_scene():
+    """Coloured Lambert albedos, coloured sphere and rectangle lights, an
+    EmitShader surface, a surface with no shader (the reference's magenta
+    emitter) and a coloured background: every colour site of the integrator
+    upsampled through the spectrum grid."""
+    root = Assembly()
+    sc = Scene(Camera([look_at((0.0, -9.0, 5.0), (0.0, 0.0, 0.5))], [np.radians(55)]), root,
+               background=(0.02, 0.05, 0.12))
+    sc.add_surface_shader("red", abi.PSY_SHADER_LAMBERT, (0.8, 0.15, 0.1))
+    sc.add_surface_shader("green", abi.PSY_SHADER_LAMBERT, (0.1, 0.7, 0.2))
+    sc.add_surface_shader("blue", abi.PSY_SHADER_LAMBERT, (0.15, 0.2, 0.9))
+    sc.add_surface_shader("glow", abi.PSY_SHADER_EMIT, (2.0, 1.2, 0.1))
+    root.add_object("ground", Bilinear([(-8, -8, 0), (8, -8, 0), (-8, 8, 0), (8, 8, 0)]))
+    root.create_object_instance("ground", [], "green")
+    for k, (x, sh) in enumerate(((-2.2, "red"), (0.0, "blue"), (2.2, "glow"))):
+        root.add_object(f"ball{k}", Sphere((x, 0.5, 1.0), 0.9))
+        root.create_object_instance(f"ball{k}", [], sh)
+    root.add_object("bare", Sphere((0.0, 3.0, 0.6), 0.6))
+    root.create_object_instance("bare", [], None)
+    root.add_object("bulb", SphereLight((-2.0, -1.5, 4.5), 0.5, (40.0, 25.0, 8.0)))
+    root.create_object_instance("bulb", [], None)
+    root.add_object("panel", RectangleLight(1.5, 1.5, (4.0, 8.0, 16.0)))
+    root.create_object_instance("panel", [translate(2.5, -1.0, 4.0)], None)
+    return sc
+
+
+def set_reference_grid(set_fn, oracle):
+    """Hand the reference's spectrum grid (from oracle/_ref) to `set_fn`;
+    skips the test when that library was not built."""
+    G = oracle.ref_spectrum_grid()
+    if G is None:
+        pytest.skip("oracle/_ref not built: no spectrum grid")
+    set_fn(G)
+    return G
+
+
+def test_gpu_render_chromatic_matches_oracle(oracle):
+    sc = _chromatic_scene()
+    ctx = gpu.Context(0)
+    try:
+        g = gpu.GpuScene(ctx, sc)
+        with pytest.raises(gpu.PsyError, match="psy_spectrum_set_grid"):
+            gpu.render(ctx, g, 8, 8, 1)
+        set_reference_grid(lambda G: gpu.spectrum_set_grid(ctx, G.cells, G.w, G.h, G.points, G.n_samples, G.smin,
+                                                           G.smax, G.mat), oracle)
+        oracle.spectrum_set_grid(oracle.ref_spectrum_grid())
+        try:
+            film, samples, rays = gpu.render(ctx, g, 32, 24, 8, 1, samples=True)
+            torch.cuda.synchronize()
+            os_, orays = oracle.OracleScene(sc).render(32, 24, 8, 1, threads=8)
+        finally:
+            oracle.spectrum_set_grid(None)
+        gs = samples.cpu().numpy()
+        check_image("chromatic 32x24x8", film.cpu().numpy(), gs, rays, os_, orays)
+        # the image is coloured: X, Y and Z means differ
+        m = gs.reshape(-1, 3).mean(axis=0)
+        assert m.max() > 1.2 * m.min()
+        # the adjoint's parameters are grey albedos
+        with pytest.raises(gpu.PsyError, match="non-grey"):
+            gpu.adjoint(ctx, g, torch.zeros((8, 8, 3), dtype=torch.float32, device="cuda"), 1)
+        g.close()
+    finally:
+        ctx.close()
+
+
+def test_gpu_render_grid_and_flat_grey_agree(oracle):
+    """A grey scene renders the same through the grid (the reference's
+    arithmetic) as on the flat white-point spectrum, to 1e-5."""
+    sc = configs.sphere_scene()
+    ctx = gpu.Context(0)
+    try:
+        g = gpu.GpuScene(ctx, sc)
+        _, flat, r0 = gpu.render(ctx, g, 32, 32, 4, 1, film=False, samples=True)
+        set_reference_grid(lambda G: gpu.spectrum_set_grid(ctx, G.cells, G.w, G.h, G.points, G.n_samples, G.smin,
+                                                           G.smax, G.mat), oracle)
+        _, grid, r1 = gpu.render(ctx, g, 32, 32, 4, 1, film=False, samples=True)
+        assert r0 == r1
+        np.testing.assert_allclose(grid.cpu().numpy(), flat.cpu().numpy(), rtol=1e-5, atol=1e-9)
+        g.close()
+    finally:
+        ctx.close()
+
+
 def test_gpu_render_instanced_lights_matches_oracle(gpu_ctx, oracle):
     sc = _instanced_lights_scene()
     check_image("instanced lights 24x24x8", *render_both(gpu_ctx, oracle, sc, 24, 24, 8)[1:])

@@ -209,3 +209,79 @@ def test_sobol_matches_reference_vectors(oracle):
             s2 = sel & (rv["sobol_scramble"] == c)
             got = oracle.sobol(M, int(d), rv["sobol_index"][s2], int(c))
             assert np.array_equal(got.view(np.uint64), rv["sobol_value"][s2].view(np.uint64)), (d, c)
+
+
+def test_spectrum_white_point_is_flat(oracle):
+    """spectrum_xyz_to_p at the equal-energy white point, recorded from the
+    reference (color/spectrum_grid.h), is flat: times 1/equal_energy_
+    reflectance it is 1 = (X+Y+Z)/3 at every wavelength, to 2e-6.  That is
+    the value the grey-only (no grid) path uses."""
+    rv = refvec()
+    lam, spec = rv["spectrum_lambda"], rv["spectrum_grey"]
+    assert lam.min() == 380.0 and lam.max() == 700.0
+    val = spec.astype(np.float64) * (1.0 / np.float64(np.float32(0.009358239977091027)))
+    assert np.abs(val - 1.0).max() <= 2e-6
+    c = np.ones(3, np.float32)
+    assert abs(oracle.lib().ora_color_to_spectrum(_p(c)) - 1.0) <= 1e-6
+
+
+def _random_xyz(rng, n):
+    """Chromaticities over the whole grid and beyond it: XYZ of random
+    linear RGB (in gamut), plus arbitrary triples (out of gamut, negative
+    sums, zero)."""
+    out = []
+    M = np.array([[0.4339, 0.3762, 0.1899], [0.2126, 0.7152, 0.0722], [0.0177, 0.1095, 0.8728]], np.float32)
+    for k in range(n):
+        if k % 4 == 0:
+            out.append(rng.uniform(-0.2, 2.0, 3).astype(np.float32))
+        else:
+            rgb = (rng.uniform(0, 1, 3) ** 3 * rng.uniform(0.01, 100)).astype(np.float32)
+            out.append((M @ rgb).astype(np.float32))
+    out[0] = np.zeros(3, np.float32)
+    return out
+
+
+def test_spectrum_grid_matches_reference_bits(oracle):
+    """The oracle's spectrum_xyz_to_p and Color_to_SpectralSample over the
+    reference's own grid tables against the reference's own functions
+    (oracle/_ref), bit for bit: bilinear cells, fan cells, outside the
+    grid, zero and negative brightness."""
+    G = oracle.ref_spectrum_grid()
+    if G is None:
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    R = oracle.ref()
+    assert G.w * G.h == G.cells.shape[0] and (G.cells[:, 0] == 1).any() and (G.cells[:, 0] == 0).any()
+    oracle.spectrum_set_grid(G)
+    try:
+        rng = np.random.default_rng(3)
+        cells_seen = set()
+        for xyz in _random_xyz(rng, 6000):
+            lam = np.float32(rng.uniform(380, 700))
+            a = oracle.spectrum_xyz_to_p(lam, xyz)
+            b = np.float32(R.psyref_spectrum_xyz_to_p(float(lam), _p(xyz)))
+            assert a.view(np.uint32) == b.view(np.uint32), (lam, xyz, a, b)
+            s = float(xyz.sum())
+            if s > 0:
+                u = G.mat[0] * xyz[0] / s + G.mat[1] * xyz[1] / s + G.mat[2]
+                v = G.mat[3] * xyz[0] / s + G.mat[4] * xyz[1] / s + G.mat[5]
+                if 0 <= u < G.w and 0 <= v < G.h:
+                    cells_seen.add(int(G.cells[int(u) + G.w * int(v), 0]))
+        assert cells_seen == {0, 1}  # both interpolation paths ran
+        for k in range(500):
+            rgb = (rng.uniform(0, 1, 3) * rng.uniform(0.1, 50)).astype(np.float32)
+            hero = np.float32(rng.uniform(380, 700))
+            got = oracle.color_to_spectral_sample(rgb, hero)
+            for i in range(4):
+                lam = np.float32(hero + np.float32(i) * np.float32(80.0))
+                if lam > 700.0:
+                    lam = np.float32(lam - np.float32(320.0))
+                want = np.float32(R.psyref_color_to_spectrum(_p(rgb), float(lam)))
+                assert got[i].view(np.uint32) == want.view(np.uint32)
+        # brightness scales the spectrum (spectrum_xyz_to_p divides by 1/(X+Y+Z))
+        for xyz in _random_xyz(rng, 200)[1:]:
+            lam = np.float32(rng.uniform(380, 700))
+            a = float(oracle.spectrum_xyz_to_p(lam, xyz))
+            b = float(oracle.spectrum_xyz_to_p(lam, (xyz * np.float32(4.0)).astype(np.float32)))
+            assert abs(b - 4.0 * a) <= 1e-5 * max(abs(b), 1e-20)
+    finally:
+        oracle.spectrum_set_grid(None)

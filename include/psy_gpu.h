@@ -101,6 +101,43 @@ typedef struct psy_scene_desc {
     float min_upoly_size;  /* Config::min_upoly_size (config.cpp:8), default 1e-5       */
 } psy_scene_desc;
 
+/* ---- scene files (.psy) ------------------------------------------------ */
+/* Host-only parsing of the reference's scene format into the description
+ * above.  Replaces: DataTree::build_from_file (parser/data_tree.cpp:
+ * 333-380, here over text already in memory) and Parser::parse_next_frame
+ * (parser/parser.cpp:37-183) with the node parsers it calls
+ * (parser/parser.cpp:186-662).  What the reference's parser reads is read
+ * the same way (Fov in degrees; numbers as -?[0-9]+[.]?[0-9]*; names with
+ * their '$'; object and assembly names resolved in their own assembly,
+ * shader names up the parent chain); what it skips with a message (unknown
+ * node types, instances of unknown data, unknown shader types) is skipped
+ * and reported by psy_parsed_scene_warnings.  Scenes the device path cannot
+ * render fail with PSY_ERR_UNSUPPORTED: an instance bound to a GTR shader,
+ * lights whose motion sample lists differ in length (other than 1). */
+typedef struct psy_parsed_scene psy_parsed_scene;
+typedef struct psy_render_settings {
+    int32_t width, height;   /* RenderSettings Resolution (default 1 1)      */
+    int32_t spp;             /* SamplesPerPixel (default 1)                  */
+    int32_t seed;            /* Seed (default 1)                             */
+    float dice_rate;         /* DicingRate (default 0.25; also in the desc)  */
+    char name[256];          /* the Scene's name without its '$'             */
+    char output_path[1024];  /* Output Path, between its quotes              */
+} psy_render_settings;
+/* Number of Scene nodes (frames) in the text. */
+int psy_parse_scene_count(const char* text, int64_t len, int32_t* n_scenes);
+/* Parses Scene node `scene_index` (0-based) of the text. */
+int psy_parse_scene(const char* text, int64_t len, int32_t scene_index, psy_parsed_scene** out);
+/* The scene description (valid until psy_parsed_scene_destroy). */
+const psy_scene_desc* psy_parsed_scene_desc(const psy_parsed_scene* scene);
+int psy_parsed_scene_settings(const psy_parsed_scene* scene, psy_render_settings* out);
+/* What the parser skipped, one line each ("" when nothing). */
+const char* psy_parsed_scene_warnings(const psy_parsed_scene* scene);
+void psy_parsed_scene_destroy(psy_parsed_scene* scene);
+/* DataTree::print_tree (parser/data_tree.cpp:383-398) of every top-level
+ * node: writes at most cap bytes (NUL-terminated) and the full size
+ * (with the NUL) to *needed. */
+int psy_parse_dump_tree(const char* text, int64_t len, char* out, int64_t cap, int64_t* needed);
+
 /* ---- ray / hit streams (device memory) -------------------------------- */
 
 /* World rays, WorldRay (basics/ray.hpp:162-185): 20 planes of n entries.
@@ -158,9 +195,13 @@ int psy_ctx_destroy(psy_ctx* ctx);
  * one thread per ray.  PSY_TRACE_MODE_PERSISTENT: one grid that fills the
  * GPU; each wave refills its finished lanes from a global ray counter once
  * refill_min (1..64) of its 64 lanes are idle; blocks_per_cu <= 0 takes the
- * kernel's occupancy.  Results are identical. */
+ * kernel's occupancy.  PSY_TRACE_MODE_COHERENT: persistent, and each step
+ * of a wave advances only the lanes of its majority kind of work
+ * (split-tree walk, or BVH node / leaf), so the wave issues one kind of
+ * step at a time.  Results are identical. */
 #define PSY_TRACE_MODE_PLAIN 0
 #define PSY_TRACE_MODE_PERSISTENT 1
+#define PSY_TRACE_MODE_COHERENT 2
 int psy_ctx_set_trace_mode(psy_ctx* ctx, int mode, int refill_min, int blocks_per_cu);
 /* Main-pass traversal stack in LDS (no counterpart in the reference, whose
  * BitStack and node stack live in CPU memory): 16 entries (default, 5 waves

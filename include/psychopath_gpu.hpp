@@ -37,7 +37,9 @@
 #include <array>
 #include <cstdint>
 #include <cstring>
+#include <fstream>
 #include <functional>
+#include <sstream>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -593,6 +595,82 @@ public:
     uint32_t seed;
     std::function<void()> callback;
     int64_t rays_shot = 0;  // Global::Stats::rays_shot of the last integrate()
+};
+
+// One parsed frame (renderer/renderer.hpp): its scene, already built on the
+// device, and its render settings.  render() fills a Film; writing the
+// image file is left to the caller.
+struct Renderer {
+    std::unique_ptr<Scene> scene;
+    int res_x = 1, res_y = 1, spp = 1;
+    uint32_t seed = 1;
+    std::string output_path;
+    std::string warnings;  // what the parser skipped, one line each
+
+    void render(Film& film, int path_length = 4) {
+        PathTraceIntegrator integ(scene.get(), &film, spp, spp, 0.0f, seed);
+        integ.path_length = path_length;
+        integ.integrate();
+    }
+};
+
+// Parser (parser/parser.hpp:16-91) over a .psy file: parse_next_frame()
+// returns the next Scene node as a Renderer, nullptr after the last.
+class Parser {
+public:
+    explicit Parser(const std::string& input_path) {
+        std::ifstream in(input_path, std::ios::in | std::ios::binary);
+        if (!in) throw std::runtime_error("Parser: cannot read " + input_path);
+        std::ostringstream ss;
+        ss << in.rdbuf();
+        text_ = ss.str();
+        int32_t n = 0;
+        check(psy_parse_scene_count(text_.data(), (int64_t)text_.size(), &n), "psy_parse_scene_count");
+        count_ = n;
+    }
+    ~Parser() { release(); }
+    Parser(const Parser&) = delete;
+    Parser& operator=(const Parser&) = delete;
+
+    int frame_count() const { return count_; }
+
+    // The next frame's description and settings, without building it on
+    // the device (valid until the next call or the Parser's end).
+    const psy_scene_desc* parse_next_description(psy_render_settings* settings) {
+        release();
+        if (next_ >= count_) return nullptr;
+        check(psy_parse_scene(text_.data(), (int64_t)text_.size(), next_++, &parsed_), "psy_parse_scene");
+        if (settings) check(psy_parsed_scene_settings(parsed_, settings), "psy_parsed_scene_settings");
+        return psy_parsed_scene_desc(parsed_);
+    }
+
+    std::unique_ptr<Renderer> parse_next_frame(int dice_depth = -1) {
+        psy_render_settings st{};
+        const psy_scene_desc* d = parse_next_description(&st);
+        if (!d) return nullptr;
+        std::unique_ptr<Renderer> r(new Renderer());
+        r->scene.reset(new Scene());
+        r->scene->name = st.name;
+        r->scene->dice_rate = d->dice_rate;
+        r->scene->background_color = Color(d->background[0], d->background[1], d->background[2]);
+        check(psy_scene_create(Device::get().ctx(), d, dice_depth, &r->scene->handle), "psy_scene_create");
+        r->res_x = st.width;
+        r->res_y = st.height;
+        r->spp = st.spp;
+        r->seed = (uint32_t)st.seed;
+        r->output_path = st.output_path;
+        r->warnings = psy_parsed_scene_warnings(parsed_);
+        return r;
+    }
+
+private:
+    void release() {
+        if (parsed_) psy_parsed_scene_destroy(parsed_);
+        parsed_ = nullptr;
+    }
+    std::string text_;
+    int count_ = 0, next_ = 0;
+    psy_parsed_scene* parsed_ = nullptr;
 };
 
 }  // namespace psy

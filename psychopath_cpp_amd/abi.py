@@ -31,6 +31,7 @@ PSY_RAY_R_DIFFUSE = 2
 PSY_TRACE_OCCLUSION_ONLY = 1
 PSY_TRACE_MODE_PLAIN = 0
 PSY_TRACE_MODE_PERSISTENT = 1
+PSY_TRACE_MODE_COHERENT = 2
 PSY_RENDER_WAVEFRONT = 0
 PSY_RENDER_FUSED = 1
 PSY_RENDER_STREAM = 2
@@ -73,6 +74,11 @@ class CameraDesc(C.Structure):
                 ("focus_distances", C.POINTER(C.c_float)), ("n_focus_distances", C.c_int32)]
 
 
+class RenderSettings(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32), ("seed", C.c_int32),
+                ("dice_rate", C.c_float), ("name", C.c_char * 256), ("output_path", C.c_char * 1024)]
+
+
 class SceneDesc(C.Structure):
     _fields_ = [("assemblies", C.POINTER(AssemblyDesc)), ("n_assemblies", C.c_int32),
                 ("shaders", C.POINTER(ShaderDesc)), ("n_shaders", C.c_int32),
@@ -108,6 +114,8 @@ EXPORTED_SYMBOLS = [
     "psy_malloc", "psy_free", "psy_copy_to_device", "psy_copy_to_host",
     "psy_scene_create", "psy_scene_destroy", "psy_scene_get_info", "psy_scene_instance_ids",
     "psy_trace_world", "psy_trace_rays", "psy_world_to_rays",
+    "psy_parse_scene_count", "psy_parse_scene", "psy_parsed_scene_desc", "psy_parsed_scene_settings",
+    "psy_parsed_scene_warnings", "psy_parsed_scene_destroy", "psy_parse_dump_tree",
     "psy_halton", "psy_sobol_set_matrices", "psy_sobol", "psy_spectrum_set_grid", "psy_color_to_spectrum",
     "psy_pixel_samples", "psy_camera_rays", "psy_bounce_rays", "psy_render",
     "psy_scene_param_counts", "psy_scene_get_params", "psy_scene_set_params", "psy_adjoint",

@@ -139,6 +139,7 @@ int launch_build_dice(const float* odata, const DevObject* objs, const int32_t* 
 // ray.
 struct TraceTuning {
     bool persistent;
+    bool coherent = false;  // persistent: step one kind of work per wave step
     int persist_blocks;    // persistent grid with the 16-entry stack
     int persist_blocks32;  // and with the 32-entry stack
     int refill_min;

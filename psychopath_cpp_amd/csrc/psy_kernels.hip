@@ -90,11 +90,13 @@ __global__ __launch_bounds__(TRACE_BLOCK, main_waves(WORLD, STACK)) void k_trace
     if (lane_finish(s, L, h)) enqueue_retrace(queue, counters, (uint32_t)i);
 }
 
+constexpr int PERSIST_CHUNK = 256;
+
 template <bool WORLD, int STACK>
-__global__ __launch_bounds__(TRACE_BLOCK, main_waves(WORLD, STACK)) void k_trace_persist(
+__global__ __launch_bounds__(TRACE_BLOCK, persist_waves(WORLD, STACK)) void k_trace_persist(
     DevScene s, const float* __restrict__ rays, int64_t n, int64_t rs, DevHits h, uint32_t flags,
     uint32_t* __restrict__ queue, uint32_t* __restrict__ counters, int refill_min,
-    const uint32_t* __restrict__ n_dev) {
+    const uint32_t* __restrict__ n_dev, int coherent) {
     __shared__ int32_t stk_ref[STACK][TRACE_BLOCK];
     __shared__ float stk_near[STACK][TRACE_BLOCK];
     const int tid = threadIdx.x;
@@ -104,25 +106,42 @@ __global__ __launch_bounds__(TRACE_BLOCK, main_waves(WORLD, STACK)) void k_trace
     Lane L;
     bool active = false;
     bool exhausted = false;  // wave-uniform
+    // the wave's claimed rays [cbase, cend): one atomic per PERSIST_CHUNK
+    // rays, so the waves do not queue on the counter at every refill
+    int64_t cbase = 0, cend = 0;
     for (;;) {
         const unsigned long long idle = __ballot(!active);
         const int nidle = __popcll(idle);
         if (!exhausted && nidle >= refill_min) {
-            unsigned long long base = 0;
-            if (lane == 0) base = atomicAdd(next_ray, (unsigned long long)nidle);
-            base = ((unsigned long long)(uint32_t)__shfl((int)(uint32_t)(base >> 32), 0) << 32) |
-                   (unsigned long long)(uint32_t)__shfl((int)(uint32_t)base, 0);
-            if ((int64_t)base >= n) exhausted = true;
-            if (!active) {
-                const int64_t i = (int64_t)base + __popcll(idle & ((1ull << lane) - 1ull));
-                if (i < n) {
-                    lane_init<WORLD>(s, rays, rs, (uint32_t)i, flags, L);
+            if (cbase >= cend) {
+                unsigned long long base = 0;
+                if (lane == 0) base = atomicAdd(next_ray, (unsigned long long)PERSIST_CHUNK);
+                base = ((unsigned long long)(uint32_t)__shfl((int)(uint32_t)(base >> 32), 0) << 32) |
+                       (unsigned long long)(uint32_t)__shfl((int)(uint32_t)base, 0);
+                if ((int64_t)base >= n) exhausted = true;
+                cbase = (int64_t)base;
+                cend = std::min<int64_t>(cbase + PERSIST_CHUNK, n);
+            }
+            if (!exhausted) {
+                const int64_t take = std::min<int64_t>(nidle, cend - cbase);
+                const int rank = __popcll(idle & ((1ull << lane) - 1ull));
+                if (!active && rank < take) {
+                    lane_init<WORLD>(s, rays, rs, (uint32_t)(cbase + rank), flags, L);
                     active = true;
                 }
+                cbase += take;
             }
         }
-        if (exhausted && __ballot(active) == 0ull) break;
-        if (active && lane_step<WORLD, STACK>(s, L, stk_ref, stk_near, tid)) {
+        const unsigned long long busy = __ballot(active);
+        if (exhausted && busy == 0ull) break;
+        bool go = active;
+        if (coherent) {
+            // the majority kind of step this round (wave-uniform)
+            const bool walk = active && (L.state & LANE_WALKING) != 0;
+            const bool walk_round = 2 * __popcll(__ballot(walk)) >= __popcll(busy);
+            go = active && walk == walk_round;
+        }
+        if (go && lane_step<WORLD, STACK>(s, L, stk_ref, stk_near, tid)) {
             active = false;
             if (lane_finish(s, L, h)) enqueue_retrace(queue, counters, L.idx);
         }
@@ -639,10 +658,11 @@ static int launch_trace_w(const DevScene& s, int stack_need, const float* rays, 
     if (tune.persistent && persist_blocks > 0 && persist_blocks < plain_blocks) {
         if (tune.stack > 16)
             hipLaunchKernelGGL((k_trace_persist<W, 32>), dim3((unsigned)persist_blocks), dim3(TRACE_BLOCK), 0, st, s,
-                               rays, n, rs, h, flags, queue, counters, refill, tune.n_dev);
+                               rays, n, rs, h, flags, queue, counters, refill, tune.n_dev, tune.coherent ? 1 : 0);
         else
             hipLaunchKernelGGL((k_trace_persist<W, MAIN_STACK>), dim3((unsigned)persist_blocks), dim3(TRACE_BLOCK), 0,
-                               st, s, rays, n, rs, h, flags, queue, counters, refill, tune.n_dev);
+                               st, s, rays, n, rs, h, flags, queue, counters, refill, tune.n_dev,
+                               tune.coherent ? 1 : 0);
     } else if (tune.stack > 16) {
         hipLaunchKernelGGL((k_trace<W, 32>), dim3((unsigned)plain_blocks), dim3(TRACE_BLOCK), 0, st, s, rays, n, rs,
                            h, flags, queue, counters, tune.n_dev);

@@ -833,6 +833,12 @@ constexpr int MAIN_STACK = PSY_MAIN_STACK;
 // rays handed to the second pass on deep hierarchies); waves per SIMD of
 // each form, as launch bounds.
 constexpr int main_waves(bool world, int stack) { return stack <= 16 ? (world ? 4 : PSY_MAIN_WAVES_PER_SIMD) : 2; }
+// The persistent form keeps its refill state beside the lane's and needs a
+// few more registers: at 5 waves per SIMD it spilled to scratch.
+#ifndef PSY_PERSIST_WAVES_PER_SIMD
+#define PSY_PERSIST_WAVES_PER_SIMD 4
+#endif
+constexpr int persist_waves(bool world, int stack) { return stack <= 16 ? PSY_PERSIST_WAVES_PER_SIMD : 2; }
 
 struct Lane {
     TRay r;
@@ -1086,11 +1092,12 @@ PSY_DEV void enqueue_retrace(uint32_t* __restrict__ queue, uint32_t* __restrict_
 
 // Main pass, one thread per ray.
 
-// Main pass, persistent: a grid that fills the GPU once; every wave takes
-// rays from a global counter, and refills its finished lanes (in one
-// atomic per wave) whenever at least `refill_min` of them are idle, so the
-// wave's lanes stay busy through the long tail of incoherent rays.  Every
-// wave leaves once the counter has passed n and its lanes are idle.
+// Main pass, persistent: a grid that fills the GPU once; every wave claims
+// rays from a global counter 256 at a time (one atomic per claim) and
+// refills its finished lanes from its claim whenever at least `refill_min`
+// of them are idle, so the wave's lanes stay busy through the long tail of
+// incoherent rays.  Every wave leaves once the counter has passed n and its
+// lanes are idle.
 
 // Second pass: retrace the queued rays with subdivision below the cache and
 // a stack of STACK >= the scene's worst case.

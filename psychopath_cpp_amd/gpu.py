@@ -54,6 +54,15 @@ def lib():
     L.psy_ctx_trace_timing.argtypes = [vp, vp, vp, vp]
     L.psy_sobol_set_matrices.argtypes = [vp, vp, u32]
     L.psy_sobol.argtypes = [vp, u32, vp, i64, C.c_uint64, vp, vp]
+    L.psy_parse_scene_count.argtypes = [C.c_char_p, i64, C.POINTER(i32)]
+    L.psy_parse_scene.argtypes = [C.c_char_p, i64, i32, C.POINTER(vp)]
+    L.psy_parsed_scene_desc.restype = C.POINTER(abi.SceneDesc)
+    L.psy_parsed_scene_desc.argtypes = [vp]
+    L.psy_parsed_scene_settings.argtypes = [vp, C.POINTER(abi.RenderSettings)]
+    L.psy_parsed_scene_warnings.restype = C.c_char_p
+    L.psy_parsed_scene_warnings.argtypes = [vp]
+    L.psy_parsed_scene_destroy.argtypes = [vp]
+    L.psy_parse_dump_tree.argtypes = [C.c_char_p, i64, C.c_char_p, i64, C.POINTER(i64)]
     L.psy_spectrum_set_grid.argtypes = [vp, vp, C.c_int32, C.c_int32, vp, C.c_int32, C.c_int32, C.c_float,
                                         C.c_float, vp]
     L.psy_color_to_spectrum.argtypes = [vp, vp, vp, i64, vp, vp]
@@ -94,9 +103,12 @@ class Context:
         check(lib().psy_ctx_create(device, C.byref(h)), "psy_ctx_create")
         self.handle = h
 
-    def set_trace_mode(self, persistent=True, refill_min=16, blocks_per_cu=0):
-        """psy_ctx_set_trace_mode: persistent main pass with lane refill, or one thread per ray."""
-        mode = abi.PSY_TRACE_MODE_PERSISTENT if persistent else abi.PSY_TRACE_MODE_PLAIN
+    def set_trace_mode(self, persistent=True, refill_min=16, blocks_per_cu=0, coherent=False):
+        """psy_ctx_set_trace_mode: persistent main pass with lane refill (coherent: one kind of step per wave
+        step), or one thread per ray."""
+        mode = abi.PSY_TRACE_MODE_PLAIN
+        if persistent:
+            mode = abi.PSY_TRACE_MODE_COHERENT if coherent else abi.PSY_TRACE_MODE_PERSISTENT
         check(lib().psy_ctx_set_trace_mode(self.handle, mode, refill_min, blocks_per_cu), "psy_ctx_set_trace_mode")
 
     def set_trace_stack(self, entries):

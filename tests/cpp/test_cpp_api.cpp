@@ -6,6 +6,11 @@
 //   gpu <out>          finalizes, traces a grid of camera rays with
 //                      Tracer::trace and renders with PathTraceIntegrator,
 //                      writing the results to <out> (raw little-endian)
+//   psyparse <file>    reads a .psy file with Parser and prints every
+//                      frame's settings and description (no GPU)
+//   psyrender <file> <out>
+//                      renders the file's first frame with Parser /
+//                      Renderer and writes its film to <out>
 // tests/test_cpp_api.py and tests/test_gpu_cpp_api.py build the same scene
 // in Python and compare.
 #include <cmath>
@@ -25,6 +30,32 @@ static Transform translate(float x, float y, float z) {
     t.m[13] = -y;
     t.m[14] = -z;
     return t;
+}
+
+static void print_desc(const psy_scene_desc& d) {
+    std::printf("assemblies %d\nshaders %d\n", d.n_assemblies, d.n_shaders);
+    for (int i = 0; i < d.n_shaders; ++i)
+        std::printf("shader %d %d %.7g %.7g %.7g\n", i, d.shaders[i].kind, d.shaders[i].color[0],
+                    d.shaders[i].color[1], d.shaders[i].color[2]);
+    for (int a = 0; a < d.n_assemblies; ++a) {
+        const psy_assembly_desc& A = d.assemblies[a];
+        double xs = 0.0;
+        for (int k = 0; k < 16 * A.n_transforms; ++k) xs += A.transforms[k] * (double)(k % 16 + 1);
+        std::printf("assembly %d objects %d instances %d transforms %d children %d xfsum %.6f\n", a, A.n_objects,
+                    A.n_instances, A.n_transforms, A.n_assemblies, xs);
+        for (int i = 0; i < A.n_objects; ++i) {
+            const int nf = A.objects[i].kind == PSY_OBJ_SPHERE ? 4 : A.objects[i].kind == PSY_OBJ_BILINEAR ? 12
+                           : A.objects[i].kind == PSY_OBJ_BICUBIC ? 48 : A.objects[i].kind == PSY_OBJ_SPHERE_LIGHT ? 7 : 5;
+            double sum = 0.0;
+            for (int k = 0; k < nf * A.objects[i].time_samples; ++k) sum += A.objects[i].data[k];
+            std::printf("object %d %d kind %d ts %d sum %.6f\n", a, i, A.objects[i].kind,
+                        A.objects[i].time_samples, sum);
+        }
+        for (int i = 0; i < A.n_instances; ++i)
+            std::printf("instance %d %d type %d data %d xf %d+%d shader %d\n", a, i, A.instances[i].type,
+                        A.instances[i].data_index, A.instances[i].transform_index,
+                        A.instances[i].transform_count, A.instances[i].shader_index);
+    }
 }
 
 static std::unique_ptr<Scene> make_scene() {
@@ -66,30 +97,30 @@ int main(int argc, char** argv) {
     std::unique_ptr<Scene> sc = make_scene();
     if (mode == "describe") {
         auto D = sc->describe();
-        const psy_scene_desc& d = D->desc;
-        std::printf("assemblies %d\nshaders %d\n", d.n_assemblies, d.n_shaders);
-        for (int i = 0; i < d.n_shaders; ++i)
-            std::printf("shader %d %d %.7g %.7g %.7g\n", i, d.shaders[i].kind, d.shaders[i].color[0],
-                        d.shaders[i].color[1], d.shaders[i].color[2]);
-        for (int a = 0; a < d.n_assemblies; ++a) {
-            const psy_assembly_desc& A = d.assemblies[a];
-            double xs = 0.0;
-            for (int k = 0; k < 16 * A.n_transforms; ++k) xs += A.transforms[k] * (double)(k % 16 + 1);
-            std::printf("assembly %d objects %d instances %d transforms %d children %d xfsum %.6f\n", a, A.n_objects,
-                        A.n_instances, A.n_transforms, A.n_assemblies, xs);
-            for (int i = 0; i < A.n_objects; ++i) {
-                const int nf = A.objects[i].kind == PSY_OBJ_SPHERE ? 4 : A.objects[i].kind == PSY_OBJ_BILINEAR ? 12
-                               : A.objects[i].kind == PSY_OBJ_BICUBIC ? 48 : A.objects[i].kind == PSY_OBJ_SPHERE_LIGHT ? 7 : 5;
-                double sum = 0.0;
-                for (int k = 0; k < nf * A.objects[i].time_samples; ++k) sum += A.objects[i].data[k];
-                std::printf("object %d %d kind %d ts %d sum %.6f\n", a, i, A.objects[i].kind,
-                            A.objects[i].time_samples, sum);
-            }
-            for (int i = 0; i < A.n_instances; ++i)
-                std::printf("instance %d %d type %d data %d xf %d+%d shader %d\n", a, i, A.instances[i].type,
-                            A.instances[i].data_index, A.instances[i].transform_index,
-                            A.instances[i].transform_count, A.instances[i].shader_index);
+        print_desc(D->desc);
+        return 0;
+    }
+    if (mode == "psyparse" && argc > 2) {
+        Parser parser(argv[2]);
+        psy_render_settings st{};
+        for (const psy_scene_desc* d; (d = parser.parse_next_description(&st));) {
+            std::printf("frame %s %dx%d spp %d seed %d dice %.7g output %s\n", st.name, st.width, st.height, st.spp,
+                        st.seed, st.dice_rate, st.output_path);
+            print_desc(*d);
         }
+        return 0;
+    }
+    if (mode == "psyrender" && argc > 3) {
+        Parser parser(argv[2]);
+        std::unique_ptr<Renderer> r = parser.parse_next_frame();
+        if (!r) return 3;
+        Film film((uint16_t)r->res_x, (uint16_t)r->res_y);
+        r->render(film);
+        std::ofstream f(argv[3], std::ios::binary);
+        const int32_t hdr[3] = {r->res_x, r->res_y, r->spp};
+        f.write((const char*)hdr, sizeof hdr);
+        f.write((const char*)film.xyz.data(), sizeof(float) * film.xyz.size());
+        std::printf("rendered %s %dx%d spp %d\n", r->scene->name.c_str(), r->res_x, r->res_y, r->spp);
         return 0;
     }
     if (mode == "nodevice") {
@@ -146,6 +177,7 @@ int main(int argc, char** argv) {
         std::printf("traced %u rays, rendered %lld rays\n", n, (long long)integ.rays_shot);
         return 0;
     }
-    std::fprintf(stderr, "usage: test_cpp_api describe | nodevice | gpu <out>\n");
+    std::fprintf(stderr, "usage: test_cpp_api describe | nodevice | gpu <out> | psyparse <file> | "
+                         "psyrender <file> <out>\n");
     return 2;
 }

@@ -52,7 +52,8 @@ def cpp_binary():
 
 
 def _describe_python(sc):
-    d = sc.to_desc().desc
+    handle = sc.to_desc()  # keeps the buffers the description points into
+    d = handle.desc
     lines = [f"assemblies {d.n_assemblies}", f"shaders {d.n_shaders}"]
     for i in range(d.n_shaders):
         c = d.shaders[i].color
@@ -77,6 +78,26 @@ def _describe_python(sc):
 def test_cpp_interface_flattens_like_python():
     out = subprocess.run([cpp_binary(), "describe"], check=True, stdout=subprocess.PIPE, text=True).stdout
     assert out.strip().splitlines() == _describe_python(cpp_equivalent_scene())
+
+
+def test_cpp_parser_reads_psy_like_python(tmp_path):
+    """psy::Parser (the reference's Parser over psy_parse_scene) reads a .psy
+    file of the same scene into the same description the scene API builds,
+    for each of its frames."""
+    from tests import _psywriter
+    sc = cpp_equivalent_scene()
+    text = _psywriter.write_psy(sc, 32, 24, 4, 1, "one") + _psywriter.write_psy(sc, 8, 6, 2, 9, "two")
+    path = tmp_path / "frames.psy"
+    path.write_text(text)
+    out = subprocess.run([cpp_binary(), "psyparse", str(path)], check=True, stdout=subprocess.PIPE,
+                         text=True).stdout.strip().splitlines()
+    want = _describe_python(sc)
+    frames = [i for i, l in enumerate(out) if l.startswith("frame ")]
+    assert len(frames) == 2
+    assert out[frames[0]] == "frame one 32x24 spp 4 seed 1 dice 0.25 output "
+    assert out[frames[1]] == "frame two 8x6 spp 2 seed 9 dice 0.25 output "
+    assert out[frames[0] + 1:frames[1]] == want
+    assert out[frames[1] + 1:] == want
 
 
 def test_cpp_interface_reports_missing_device():

@@ -16,6 +16,27 @@ from tests.test_cpp_api import cpp_binary, cpp_equivalent_scene
 pytestmark = pytest.mark.gpu
 
 
+def test_gpu_cpp_parser_renderer(gpu_ctx, tmp_path):
+    """psy::Parser::parse_next_frame + Renderer::render on a .psy file give
+    the same film, bit for bit, as the scene built through the scene API and
+    rendered through the C ABI."""
+    from tests import _psywriter
+    sc = cpp_equivalent_scene()
+    path = tmp_path / "frame.psy"
+    path.write_text(_psywriter.write_psy(sc, 24, 16, 3, 5, "frame"))
+    out = tmp_path / "film.bin"
+    r = subprocess.run([cpp_binary(), "psyrender", str(path), str(out)], stdout=subprocess.PIPE,
+                       stderr=subprocess.PIPE, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0
+    raw = out.read_bytes()
+    W, H, spp = np.frombuffer(raw[:12], np.int32)
+    film_cpp = np.frombuffer(raw[12:12 + 4 * W * H * 3], np.float32).reshape(H, W, 3)
+    g = gpu.GpuScene(gpu_ctx, sc)
+    film, _, _ = gpu.render(gpu_ctx, g, int(W), int(H), int(spp), 5)
+    assert np.array_equal(film.cpu().numpy().view(np.uint32), film_cpp.view(np.uint32))
+
+
 def test_gpu_cpp_tracer_and_integrator(gpu_ctx):
     binary = cpp_binary()
     with tempfile.TemporaryDirectory() as d:

@@ -298,8 +298,9 @@ def _hits_equal(a, b):
 
 
 def test_gpu_trace_modes_identical(gpu_ctx, c1):
-    """The persistent main pass (any refill threshold, any grid) gives the
-    same bits as one thread per ray."""
+    """The persistent main pass (any refill threshold, any grid, with or
+    without one kind of step per wave step) gives the same bits as one
+    thread per ray."""
     sc, g, o = c1
     W, H, spp, seed = 256, 256, 2, 3
     w, _ = gpu.camera_rays(gpu_ctx, g, W, H, spp, seed)
@@ -311,13 +312,14 @@ def test_gpu_trace_modes_identical(gpu_ctx, c1):
         stream = torch.cat([w, b[:, valid.bool()]], dim=1).contiguous()
         rays = gpu.world_to_rays(ctx, stream)
         ref = gpu.trace_rays(ctx, g, rays)
-        for refill, bpc in ((16, 0), (1, 0), (64, 0), (8, 1), (33, 2)):
-            say("persistent mode refill", refill, "blocks/CU", bpc)
-            ctx.set_trace_mode(persistent=True, refill_min=refill, blocks_per_cu=bpc)
+        for refill, bpc, coherent in ((16, 0, False), (1, 0, False), (64, 0, False), (8, 1, False),
+                                      (33, 2, False), (16, 0, True), (1, 1, True)):
+            say("persistent mode refill", refill, "blocks/CU", bpc, "coherent", coherent)
+            ctx.set_trace_mode(persistent=True, refill_min=refill, blocks_per_cu=bpc, coherent=coherent)
             got = gpu.trace_rays(ctx, g, rays)
-            assert _hits_equal(got, ref), (refill, bpc)
+            assert _hits_equal(got, ref), (refill, bpc, coherent)
             gw = gpu.trace_world(ctx, g, w)
-            assert _hits_equal(gw, hp), (refill, bpc)
+            assert _hits_equal(gw, hp), (refill, bpc, coherent)
     finally:
         ctx.close()
 

@@ -178,6 +178,28 @@ def test_gpu_render_grid_and_flat_grey_agree(oracle):
         ctx.close()
 
 
+def test_gpu_render_parsed_psy_scene(gpu_ctx):
+    """A scene written as .psy text and parsed by psy_parse_scene renders the
+    same samples, bit for bit, as the same scene built through the scene
+    API."""
+    from psychopath_cpp_amd import psyfile
+    from tests import _psywriter
+    for sc in (_instanced_lights_scene(), configs.bicubic_sphere_scene(grid=8, n_outer=2, n_inner=2)):
+        parsed = psyfile.ParsedScene(_psywriter.write_psy(sc, 40, 30, 4, 3))
+        assert parsed.warnings == ""
+        s = parsed.settings
+        runs = []
+        for src in (sc, parsed):
+            g = gpu.GpuScene(gpu_ctx, src)
+            _, smp, rays = gpu.render(gpu_ctx, g, s["width"], s["height"], s["spp"], s["seed"], film=False,
+                                      samples=True)
+            runs.append((smp.cpu(), rays))
+            g.close()
+        say("parsed .psy render: rays", runs[0][1], runs[1][1])
+        assert runs[0][1] == runs[1][1] and torch.equal(runs[0][0], runs[1][0])
+        parsed.close()
+
+
 def test_gpu_render_instanced_lights_matches_oracle(gpu_ctx, oracle):
     sc = _instanced_lights_scene()
     check_image("instanced lights 24x24x8", *render_both(gpu_ctx, oracle, sc, 24, 24, 8)[1:])

@@ -138,6 +138,23 @@ void psy_parsed_scene_destroy(psy_parsed_scene* scene);
  * (with the NUL) to *needed. */
 int psy_parse_dump_tree(const char* text, int64_t len, char* out, int64_t cap, int64_t* needed);
 
+/* ---- film output (host memory) ------------------------------------------ */
+/* Replaces: Film::scanline_image_8bbc (film/film.hpp:173-225): the per-pixel
+ * mean XYZ film (as psy_render writes it, copied to the host) as 8-bit
+ * sRGB with white point E (XYZ_to_sRGB_E, color/color.hpp:425-439), times
+ * 255, dithered, clamped and truncated; scanline order, 3 bytes a pixel.
+ * sample_counts (or NULL: every pixel sampled): pixels with 0 samples show
+ * the reference's grey checkerboard.  The dither of each channel is a hash
+ * of (dither_seed, pixel, channel) in [-0.5, 0.5); dither_seed 0: none. */
+int psy_film_to_srgb8(const float* film_xyz, const int32_t* sample_counts, int32_t width, int32_t height,
+                      uint32_t dither_seed, uint8_t* rgb8);
+/* write_png_from_film's file write (renderer/renderer.cpp:24-42): an 8-bit
+ * RGB PNG (uncompressed deflate). */
+int psy_write_png(const char* path, const uint8_t* rgb8, int32_t width, int32_t height);
+/* A float dump the reference does not have: the film as linear sRGB with
+ * white point E (unclamped), PFM, little-endian. */
+int psy_write_pfm(const char* path, const float* film_xyz, int32_t width, int32_t height);
+
 /* ---- ray / hit streams (device memory) -------------------------------- */
 
 /* World rays, WorldRay (basics/ray.hpp:162-185): 20 planes of n entries.

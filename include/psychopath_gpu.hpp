@@ -597,9 +597,18 @@ public:
     int64_t rays_shot = 0;  // Global::Stats::rays_shot of the last integrate()
 };
 
+// write_png_from_film (renderer/renderer.cpp:24-42): the film as dithered
+// 8-bit sRGB (Film::scanline_image_8bbc) in a PNG file.
+inline void write_png(const Film& film, const std::string& path, uint32_t dither_seed = 1) {
+    std::vector<uint8_t> rgb((size_t)film.width * film.height * 3);
+    check(psy_film_to_srgb8(film.xyz.data(), nullptr, film.width, film.height, dither_seed, rgb.data()),
+          "psy_film_to_srgb8");
+    check(psy_write_png(path.c_str(), rgb.data(), film.width, film.height), "psy_write_png");
+}
+
 // One parsed frame (renderer/renderer.hpp): its scene, already built on the
-// device, and its render settings.  render() fills a Film; writing the
-// image file is left to the caller.
+// device, and its render settings.  render() fills a Film and, with an
+// output path, writes it as a PNG there.
 struct Renderer {
     std::unique_ptr<Scene> scene;
     int res_x = 1, res_y = 1, spp = 1;
@@ -607,10 +616,11 @@ struct Renderer {
     std::string output_path;
     std::string warnings;  // what the parser skipped, one line each
 
-    void render(Film& film, int path_length = 4) {
+    void render(Film& film, int path_length = 4, bool write_output = false) {
         PathTraceIntegrator integ(scene.get(), &film, spp, spp, 0.0f, seed);
         integ.path_length = path_length;
         integ.integrate();
+        if (write_output && !output_path.empty()) write_png(film, output_path);
     }
 };
 

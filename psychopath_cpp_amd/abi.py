@@ -116,6 +116,7 @@ EXPORTED_SYMBOLS = [
     "psy_trace_world", "psy_trace_rays", "psy_world_to_rays",
     "psy_parse_scene_count", "psy_parse_scene", "psy_parsed_scene_desc", "psy_parsed_scene_settings",
     "psy_parsed_scene_warnings", "psy_parsed_scene_destroy", "psy_parse_dump_tree",
+    "psy_film_to_srgb8", "psy_write_png", "psy_write_pfm",
     "psy_halton", "psy_sobol_set_matrices", "psy_sobol", "psy_spectrum_set_grid", "psy_color_to_spectrum",
     "psy_pixel_samples", "psy_camera_rays", "psy_bounce_rays", "psy_render",
     "psy_scene_param_counts", "psy_scene_get_params", "psy_scene_set_params", "psy_adjoint",

@@ -63,6 +63,9 @@ def lib():
     L.psy_parsed_scene_warnings.argtypes = [vp]
     L.psy_parsed_scene_destroy.argtypes = [vp]
     L.psy_parse_dump_tree.argtypes = [C.c_char_p, i64, C.c_char_p, i64, C.POINTER(i64)]
+    L.psy_film_to_srgb8.argtypes = [vp, vp, i32, i32, u32, vp]
+    L.psy_write_png.argtypes = [C.c_char_p, vp, i32, i32]
+    L.psy_write_pfm.argtypes = [C.c_char_p, vp, i32, i32]
     L.psy_spectrum_set_grid.argtypes = [vp, vp, C.c_int32, C.c_int32, vp, C.c_int32, C.c_int32, C.c_float,
                                         C.c_float, vp]
     L.psy_color_to_spectrum.argtypes = [vp, vp, vp, i64, vp, vp]

@@ -215,10 +215,13 @@ int psy_ctx_destroy(psy_ctx* ctx);
  * kernel's occupancy.  PSY_TRACE_MODE_COHERENT: persistent, and each step
  * of a wave advances only the lanes of its majority kind of work
  * (split-tree walk, or BVH node / leaf), so the wave issues one kind of
- * step at a time.  Results are identical. */
+ * step at a time.  PSY_TRACE_MODE_PLAIN_COHERENT: one thread per ray, with
+ * the same majority stepping (faster on instanced, moving and bicubic
+ * scenes; the autotuner tries it).  Results are identical. */
 #define PSY_TRACE_MODE_PLAIN 0
 #define PSY_TRACE_MODE_PERSISTENT 1
 #define PSY_TRACE_MODE_COHERENT 2
+#define PSY_TRACE_MODE_PLAIN_COHERENT 3
 int psy_ctx_set_trace_mode(psy_ctx* ctx, int mode, int refill_min, int blocks_per_cu);
 /* Main-pass traversal stack in LDS (no counterpart in the reference, whose
  * BitStack and node stack live in CPU memory): 16 entries (default, 5 waves

@@ -32,6 +32,7 @@ PSY_TRACE_OCCLUSION_ONLY = 1
 PSY_TRACE_MODE_PLAIN = 0
 PSY_TRACE_MODE_PERSISTENT = 1
 PSY_TRACE_MODE_COHERENT = 2
+PSY_TRACE_MODE_PLAIN_COHERENT = 3
 PSY_RENDER_WAVEFRONT = 0
 PSY_RENDER_FUSED = 1
 PSY_RENDER_STREAM = 2

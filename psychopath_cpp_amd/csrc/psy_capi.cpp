@@ -181,7 +181,8 @@ int psy_ctx_set_trace_stack(psy_ctx* ctx, int entries) {
 
 int psy_ctx_set_trace_mode(psy_ctx* ctx, int mode, int refill_min, int blocks_per_cu) {
     if (!ctx) return fail(PSY_ERR_INVALID, "psy_ctx_set_trace_mode: NULL context");
-    if (mode != PSY_TRACE_MODE_PLAIN && mode != PSY_TRACE_MODE_PERSISTENT && mode != PSY_TRACE_MODE_COHERENT)
+    if (mode != PSY_TRACE_MODE_PLAIN && mode != PSY_TRACE_MODE_PERSISTENT && mode != PSY_TRACE_MODE_COHERENT &&
+        mode != PSY_TRACE_MODE_PLAIN_COHERENT)
         return fail(PSY_ERR_INVALID, "psy_ctx_set_trace_mode: unknown mode");
     if (refill_min < 1 || refill_min > 64) return fail(PSY_ERR_INVALID, "psy_ctx_set_trace_mode: refill_min not in 1..64");
     if (blocks_per_cu > 64) return fail(PSY_ERR_INVALID, "psy_ctx_set_trace_mode: blocks_per_cu above 64");

@@ -140,6 +140,7 @@ int launch_build_dice(const float* odata, const DevObject* objs, const int32_t* 
 struct TraceTuning {
     bool persistent;
     bool coherent = false;  // persistent: step one kind of work per wave step
+    bool plain_coherent = false;  // one thread per ray, stepping one kind of work per wave step
     int persist_blocks;    // persistent grid with the 16-entry stack
     int persist_blocks32;  // and with the 32-entry stack
     int refill_min;

@@ -88,6 +88,7 @@ struct psy_ctx {
         psy::TraceTuning t;
         t.persistent = mode == PSY_TRACE_MODE_PERSISTENT || mode == PSY_TRACE_MODE_COHERENT;
         t.coherent = mode == PSY_TRACE_MODE_COHERENT;
+        t.plain_coherent = mode == PSY_TRACE_MODE_PLAIN_COHERENT;
         t.persist_blocks = n_cu * (blocks_per_cu > 0 ? blocks_per_cu : occupancy[world ? 1 : 0]);
         t.persist_blocks32 = n_cu * (blocks_per_cu > 0 ? blocks_per_cu : occupancy32[world ? 1 : 0]);
         t.refill_min = refill_min;

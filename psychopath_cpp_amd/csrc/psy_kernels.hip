@@ -72,9 +72,11 @@ __global__ void k_build_dice(const float* odata, const DevObject* objs, const in
 
 // Main pass, one thread per ray.  n_dev (or NULL): a device-side ray
 // count, read at launch, that lowers n (streams whose length only the
-// device knows, e.g. in a captured graph).
-template <bool WORLD, int STACK>
-__global__ __launch_bounds__(TRACE_BLOCK, main_waves(WORLD, STACK)) void k_trace(
+// device knows, e.g. in a captured graph).  COHERENT: each wave step
+// advances only the lanes of the wave's majority kind of step (split-tree
+// walk, or BVH node / leaf), so a step issues one kind of work.
+template <bool WORLD, int STACK, bool COHERENT>
+__global__ __launch_bounds__(TRACE_BLOCK, main_waves(WORLD, STACK, COHERENT)) void k_trace(
     DevScene s, const float* __restrict__ rays, int64_t n, int64_t rs, DevHits h, uint32_t flags,
     uint32_t* __restrict__ queue, uint32_t* __restrict__ counters, const uint32_t* __restrict__ n_dev) {
     __shared__ int32_t stk_ref[STACK][TRACE_BLOCK];
@@ -85,7 +87,18 @@ __global__ __launch_bounds__(TRACE_BLOCK, main_waves(WORLD, STACK)) void k_trace
     if (i >= n) return;
     Lane L;
     lane_init<WORLD>(s, rays, rs, (uint32_t)i, flags, L);
-    while (!lane_step<WORLD, STACK>(s, L, stk_ref, stk_near, tid)) {
+    if constexpr (!COHERENT) {
+        while (!lane_step<WORLD, STACK>(s, L, stk_ref, stk_near, tid)) {
+        }
+    } else {
+        bool done = false;
+        for (;;) {
+            const unsigned long long busy = __ballot(!done);
+            if (busy == 0ull) break;
+            const bool walk = !done && (L.state & LANE_WALKING) != 0;
+            const bool walk_round = 2 * __popcll(__ballot(walk)) >= __popcll(busy);  // wave-uniform
+            if (!done && walk == walk_round && lane_step<WORLD, STACK>(s, L, stk_ref, stk_near, tid)) done = true;
+        }
     }
     if (lane_finish(s, L, h)) enqueue_retrace(queue, counters, (uint32_t)i);
 }
@@ -664,11 +677,19 @@ static int launch_trace_w(const DevScene& s, int stack_need, const float* rays, 
                                st, s, rays, n, rs, h, flags, queue, counters, refill, tune.n_dev,
                                tune.coherent ? 1 : 0);
     } else if (tune.stack > 16) {
-        hipLaunchKernelGGL((k_trace<W, 32>), dim3((unsigned)plain_blocks), dim3(TRACE_BLOCK), 0, st, s, rays, n, rs,
-                           h, flags, queue, counters, tune.n_dev);
+        if (tune.plain_coherent)
+            hipLaunchKernelGGL((k_trace<W, 32, true>), dim3((unsigned)plain_blocks), dim3(TRACE_BLOCK), 0, st, s, rays,
+                               n, rs, h, flags, queue, counters, tune.n_dev);
+        else
+            hipLaunchKernelGGL((k_trace<W, 32, false>), dim3((unsigned)plain_blocks), dim3(TRACE_BLOCK), 0, st, s, rays,
+                               n, rs, h, flags, queue, counters, tune.n_dev);
     } else {
-        hipLaunchKernelGGL((k_trace<W, MAIN_STACK>), dim3((unsigned)plain_blocks), dim3(TRACE_BLOCK), 0, st, s, rays, n,
-                           rs, h, flags, queue, counters, tune.n_dev);
+        if (tune.plain_coherent)
+            hipLaunchKernelGGL((k_trace<W, MAIN_STACK, true>), dim3((unsigned)plain_blocks), dim3(TRACE_BLOCK), 0, st, s,
+                               rays, n, rs, h, flags, queue, counters, tune.n_dev);
+        else
+            hipLaunchKernelGGL((k_trace<W, MAIN_STACK, false>), dim3((unsigned)plain_blocks), dim3(TRACE_BLOCK), 0, st,
+                               s, rays, n, rs, h, flags, queue, counters, tune.n_dev);
     }
     if (tune.ev[1]) (void)hipEventRecord((hipEvent_t)tune.ev[1], st);
     if (stack_need <= 16) launch_deep<W, 16>(s, rays, n, rs, h, flags, queue, counters, st);

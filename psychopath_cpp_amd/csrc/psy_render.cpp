@@ -724,6 +724,7 @@ int psy_render_autotune(psy_ctx* ctx, const psy_scene* s, const psy_render_param
     }
     if (!rc) { Cfg t = best; t.tmode = PSY_TRACE_MODE_PERSISTENT; t.refill = 32; rc = consider(t); }
     if (!rc) { Cfg t = best; t.stack = 32; rc = consider(t); }
+    if (!rc && best.tmode == PSY_TRACE_MODE_PLAIN) { Cfg t = best; t.tmode = PSY_TRACE_MODE_PLAIN_COHERENT; rc = consider(t); }
     if (best.mode == PSY_RENDER_STREAM)
         for (int it : {2, 8, 16}) if (!rc && it != best.iters) { Cfg t = best; t.iters = it; rc = consider(t); }
     if (rc) {

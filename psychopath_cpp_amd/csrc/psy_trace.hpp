@@ -832,7 +832,9 @@ constexpr int MAIN_STACK = PSY_MAIN_STACK;
 // The main pass also comes with a 32-entry stack at lower occupancy (fewer
 // rays handed to the second pass on deep hierarchies); waves per SIMD of
 // each form, as launch bounds.
-constexpr int main_waves(bool world, int stack) { return stack <= 16 ? (world ? 4 : PSY_MAIN_WAVES_PER_SIMD) : 2; }
+constexpr int main_waves(bool world, int stack, bool coherent = false) {
+    return stack <= 16 ? ((world || coherent) ? 4 : PSY_MAIN_WAVES_PER_SIMD) : 2;
+}
 // The persistent form keeps its refill state beside the lane's and needs a
 // few more registers: at 5 waves per SIMD it spilled to scratch.
 #ifndef PSY_PERSIST_WAVES_PER_SIMD

@@ -109,7 +109,7 @@ class Context:
     def set_trace_mode(self, persistent=True, refill_min=16, blocks_per_cu=0, coherent=False):
         """psy_ctx_set_trace_mode: persistent main pass with lane refill (coherent: one kind of step per wave
         step), or one thread per ray."""
-        mode = abi.PSY_TRACE_MODE_PLAIN
+        mode = abi.PSY_TRACE_MODE_PLAIN_COHERENT if coherent else abi.PSY_TRACE_MODE_PLAIN
         if persistent:
             mode = abi.PSY_TRACE_MODE_COHERENT if coherent else abi.PSY_TRACE_MODE_PERSISTENT
         check(lib().psy_ctx_set_trace_mode(self.handle, mode, refill_min, blocks_per_cu), "psy_ctx_set_trace_mode")

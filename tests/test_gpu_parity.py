@@ -320,6 +320,10 @@ def test_gpu_trace_modes_identical(gpu_ctx, c1):
             assert _hits_equal(got, ref), (refill, bpc, coherent)
             gw = gpu.trace_world(ctx, g, w)
             assert _hits_equal(gw, hp), (refill, bpc, coherent)
+        say("one thread per ray, one kind of step per wave step")
+        ctx.set_trace_mode(persistent=False, coherent=True)
+        assert _hits_equal(gpu.trace_rays(ctx, g, rays), ref)
+        assert _hits_equal(gpu.trace_world(ctx, g, w), hp)
     finally:
         ctx.close()
 

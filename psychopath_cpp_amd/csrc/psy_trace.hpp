@@ -593,16 +593,28 @@ struct TRay {
 // time 1.  Each endpoint box is the union of its subtree's, padded on the
 // host so the interpolation holds every leaf at every time (and the pad
 // covers the rounding of the two operations here).
-PSY_DEV void node_boxes(const DevScene& s, int32_t ref, float time, float4& lx, float4& hx, float4& ly, float4& hy,
-                        float4& lz, float4& hz) {
-    const float4* q = reinterpret_cast<const float4*>(s.bvh + ref);
-    lx = q[0]; hx = q[1]; ly = q[2]; hy = q[3]; lz = q[4]; hz = q[5];
+//
+// The planes come back as near / far per axis for the ray's direction
+// signs (ds*: that component of the inverse direction is negative, so the
+// max plane is met first): each lane loads them from its own offsets
+// instead of loading min and max and selecting per child.
+PSY_DEV void node_boxes(const DevScene& s, int32_t ref, float time, bool ds0, bool ds1, bool ds2, float4& nx,
+                        float4& fx, float4& ny, float4& fy, float4& nz, float4& fz) {
+    // 32-bit byte offsets from the (wave-uniform) array base
+    const uint32_t o = (uint32_t)ref * (uint32_t)sizeof(BvhNode);
+    const uint32_t ix = ds0 ? 16u : 0u, iy = ds1 ? 48u : 32u, iz = ds2 ? 80u : 64u;
+    const char* b = reinterpret_cast<const char*>(s.bvh);
+#define PSY_AT(B, OFF) (*reinterpret_cast<const float4*>((B) + (OFF)))
+    nx = PSY_AT(b, o + ix); fx = PSY_AT(b, o + (ix ^ 16u)); ny = PSY_AT(b, o + iy); fy = PSY_AT(b, o + (iy ^ 16u));
+    nz = PSY_AT(b, o + iz); fz = PSY_AT(b, o + (iz ^ 16u));
     if (s.bvh1) {
-        const float4* q1 = reinterpret_cast<const float4*>(s.bvh1 + ref);
+        const char* b1 = reinterpret_cast<const char*>(s.bvh1);
         const float t = fminf(fmaxf(time, 0.0f), 1.0f);
-        const float4 a0 = q1[0], a1 = q1[1], a2 = q1[2], a3 = q1[3], a4 = q1[4], a5 = q1[5];
+        const float4 a0 = PSY_AT(b1, o + ix), a1 = PSY_AT(b1, o + (ix ^ 16u)), a2 = PSY_AT(b1, o + iy),
+                     a3 = PSY_AT(b1, o + (iy ^ 16u)), a4 = PSY_AT(b1, o + iz), a5 = PSY_AT(b1, o + (iz ^ 16u));
+#undef PSY_AT
 #define PSY_MIX(V, W) V = make_float4(V.x + W.x * t, V.y + W.y * t, V.z + W.z * t, V.w + W.w * t)
-        PSY_MIX(lx, a0); PSY_MIX(hx, a1); PSY_MIX(ly, a2); PSY_MIX(hy, a3); PSY_MIX(lz, a4); PSY_MIX(hz, a5);
+        PSY_MIX(nx, a0); PSY_MIX(fx, a1); PSY_MIX(ny, a2); PSY_MIX(fy, a3); PSY_MIX(nz, a4); PSY_MIX(fz, a5);
 #undef PSY_MIX
     }
 }
@@ -761,20 +773,20 @@ PSY_DEV bool trace_one(const DevScene& s, const float* __restrict__ rays, int64_
         const float nearv = stk_near[sp][tid];
         if (!(nearv < b.t)) continue;
         if (ref >= 0) {
-            float4 lx, hx, ly, hy, lz, hz;
-            node_boxes(s, ref, r.time, lx, hx, ly, hy, lz, hz);
+            float4 nx, fx, ny, fy, nz, fz;
+            node_boxes(s, ref, r.time, ds0, ds1, ds2, nx, fx, ny, fy, nz, fz);
             const int4 ch = reinterpret_cast<const int4*>(s.bvh + ref)[6];
             // BBox4::intersect_ray lanes (basics/bbox.hpp:566-595); keys are
             // the near distances of hit children, +inf otherwise
 #define PSY_LANE(L, KEY)                                                                             \
     float KEY;                                                                                       \
     {                                                                                                \
-        const float xlo = ((ds0 ? hx.L : lx.L) - r.o.x) * r.dinv.x;                                  \
-        const float xhi = ((ds0 ? lx.L : hx.L) - r.o.x) * r.dinv.x;                                  \
-        const float ylo = ((ds1 ? hy.L : ly.L) - r.o.y) * r.dinv.y;                                  \
-        const float yhi = ((ds1 ? ly.L : hy.L) - r.o.y) * r.dinv.y;                                  \
-        const float zlo = ((ds2 ? hz.L : lz.L) - r.o.z) * r.dinv.z;                                  \
-        const float zhi = ((ds2 ? lz.L : hz.L) - r.o.z) * r.dinv.z;                                  \
+        const float xlo = (nx.L - r.o.x) * r.dinv.x;                                                 \
+        const float xhi = (fx.L - r.o.x) * r.dinv.x;                                                 \
+        const float ylo = (ny.L - r.o.y) * r.dinv.y;                                                 \
+        const float yhi = (fy.L - r.o.y) * r.dinv.y;                                                 \
+        const float zlo = (nz.L - r.o.z) * r.dinv.z;                                                 \
+        const float zhi = (fz.L - r.o.z) * r.dinv.z;                                                 \
         const float mins = sse_max(sse_max(xlo, ylo), sse_max(zlo, 0.0f));                           \
         const float maxs = sse_max(sse_min(sse_min(xhi, yhi), zhi), -INFINITY) * PSY_BBOX_MAXT_ADJUST; \
         KEY = ((mins < b.t) && (mins <= maxs)) ? mins : INFINITY;                                    \
@@ -981,22 +993,22 @@ PSY_DEV bool lane_step(const DevScene& s, Lane& L, int32_t (*stk_ref)[TRACE_BLOC
     }
     if (ref >= 0) {
         const TRay& r = L.r;
-        float4 lx, hx, ly, hy, lz, hz;
-        node_boxes(s, ref, r.time, lx, hx, ly, hy, lz, hz);
-        const int4 ch = reinterpret_cast<const int4*>(s.bvh + ref)[6];
         const bool ds0 = r.dinv.x < 0.0f, ds1 = r.dinv.y < 0.0f, ds2 = r.dinv.z < 0.0f;
+        float4 nx, fx, ny, fy, nz, fz;
+        node_boxes(s, ref, r.time, ds0, ds1, ds2, nx, fx, ny, fy, nz, fz);
+        const int4 ch = reinterpret_cast<const int4*>(s.bvh + ref)[6];
         // BBox4::intersect_ray lanes (basics/bbox.hpp:566-595) for a finite
         // inverse direction (no NaN); keys are the near distances of hit
         // children, +inf otherwise
 #define PSY_LANE(LN, KEY)                                                                            \
     float KEY;                                                                                       \
     {                                                                                                \
-        const float xlo = ((ds0 ? hx.LN : lx.LN) - r.o.x) * r.dinv.x;                                \
-        const float xhi = ((ds0 ? lx.LN : hx.LN) - r.o.x) * r.dinv.x;                                \
-        const float ylo = ((ds1 ? hy.LN : ly.LN) - r.o.y) * r.dinv.y;                                \
-        const float yhi = ((ds1 ? ly.LN : hy.LN) - r.o.y) * r.dinv.y;                                \
-        const float zlo = ((ds2 ? hz.LN : lz.LN) - r.o.z) * r.dinv.z;                                \
-        const float zhi = ((ds2 ? lz.LN : hz.LN) - r.o.z) * r.dinv.z;                                \
+        const float xlo = (nx.LN - r.o.x) * r.dinv.x;                                                \
+        const float xhi = (fx.LN - r.o.x) * r.dinv.x;                                                \
+        const float ylo = (ny.LN - r.o.y) * r.dinv.y;                                                \
+        const float yhi = (fy.LN - r.o.y) * r.dinv.y;                                                \
+        const float zlo = (nz.LN - r.o.z) * r.dinv.z;                                                \
+        const float zhi = (fz.LN - r.o.z) * r.dinv.z;                                                \
         const float mins = fmaxf(fmaxf(xlo, ylo), fmaxf(zlo, 0.0f));                                 \
         const float maxs = fminf(fminf(xhi, yhi), zhi) * PSY_BBOX_MAXT_ADJUST;                       \
         KEY = ((mins < L.b.t) && (mins <= maxs)) ? mins : INFINITY;                                  \

@@ -326,7 +326,7 @@ def main():
             "trace_mode": args.trace_mode,
         },
         "roofline": {
-            "kernel": "k_trace<false, 16> (main pass, %.3f ms/launch)" % main_avg,
+            "kernel": "k_trace<false, 16, false> (main pass, %.3f ms/launch)" % main_avg,
             "bound": "hbm",
             "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS,

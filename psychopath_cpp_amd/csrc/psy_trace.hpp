@@ -703,7 +703,10 @@ PSY_DEV void cmpswap(float& ka, int32_t& ra, float& kb, int32_t& rb) {
     }
 }
 
-constexpr int TRACE_BLOCK = 128;
+#ifndef PSY_TRACE_BLOCK
+#define PSY_TRACE_BLOCK 64
+#endif
+constexpr int TRACE_BLOCK = PSY_TRACE_BLOCK;  // threads per block of the trace kernels
 
 // A ray of the stream: world form (PSY_WRAY_PLANES, footprint derived here,
 // basics/ray.hpp:310-323) or object form (PSY_RAY_PLANES).

@@ -328,6 +328,44 @@ def test_gpu_trace_modes_identical(gpu_ctx, c1):
         ctx.close()
 
 
+def test_gpu_c1_full_stream(c1, oracle):
+    """The benchmark's full C1 stream (2^22 rays: 2^21 primary at 1024 x
+    1024 x 2 spp plus 2^21 first-bounce rays): every launch form of the main
+    pass gives the same bits at full size, every primary ray hits the field
+    (the camera sees only the field), and a random sample of 8192 rays
+    drawn from the whole stream matches the oracle."""
+    import bench
+    sc, g, o = c1
+    ctx = gpu.Context(0)
+    try:
+        wrays, _ = bench.make_stream(ctx, g)
+        rays = gpu.world_to_rays(ctx, wrays)
+        ref = gpu.trace_rays(ctx, g, rays)
+        n = rays.shape[1]
+        assert n == 1 << 22
+        inst = ref.inst.cpu().numpy().view(np.uint32)
+        assert (inst[: n // 2] != 0xFFFFFFFF).all()
+        for mode in ((True, 16, False), (True, 8, True), (False, 16, True)):
+            ctx.set_trace_mode(persistent=mode[0], refill_min=mode[1], coherent=mode[2])
+            assert _hits_equal(gpu.trace_rays(ctx, g, rays), ref), mode
+        rng = np.random.default_rng(29)
+        pick = np.sort(rng.choice(n, 8192, replace=False))
+        w = oracle.wrays_from_planes(wrays[:, torch.from_numpy(pick).cuda()].cpu().numpy())
+        inter = o.trace(w, threads=8, chunk=8192)
+        hit_o = inter["hit"] != 0
+        hit_g = inst[pick] != 0xFFFFFFFF
+        ids = g.instance_ids()
+        same_id = hit_o == hit_g
+        same_id[hit_g & hit_o] = ids[inst[pick][hit_g & hit_o]] == inter["id"][hit_g & hit_o]
+        t_g = ref.t.cpu().numpy()[pick]
+        both = hit_g & hit_o
+        t_ok = np.abs(t_g[both] - inter["t"][both]) <= 1e-5 * np.abs(inter["t"][both])
+        say("full stream sample: ids agree", same_id.mean(), "t within 1e-5", t_ok.mean())
+        assert same_id.mean() >= 0.9999 and t_ok.mean() >= 0.9999
+    finally:
+        ctx.close()
+
+
 def _nested_spheres(n=4096, seed=5):
     """Spheres around one centre: a ray through it hits every BVH box, so
     every node pushes three children and the main pass's stack overflows."""

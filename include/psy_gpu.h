@@ -258,6 +258,11 @@ int psy_ctx_set_stream_steps(psy_ctx* ctx, int steps_per_graph);
  * to the second pass (split below the dice cache, or a BVH stack deeper
  * than the main pass keeps).  Synchronizes the device; for diagnostics. */
 int psy_ctx_retrace_count(psy_ctx* ctx, int64_t* out);
+/* Rays of the latest trace call (psy_trace_*, or the last step of a
+ * psy_render) that took the second pass, and those of them that ran out of
+ * its split budget and took the third, wave-cooperative pass.  Synchronizes
+ * the device. */
+int psy_ctx_trace_pass_counts(psy_ctx* ctx, int64_t* second, int64_t* third);
 /* Pass timing (no counterpart in the reference): while enabled, each trace
  * call records HIP events on its stream around the main pass and the
  * second pass.  psy_ctx_trace_timing waits for the recorded calls and

@@ -231,6 +231,19 @@ int psy_ctx_retrace_count(psy_ctx* ctx, int64_t* out) {
     return PSY_OK;
 }
 
+int psy_ctx_trace_pass_counts(psy_ctx* ctx, int64_t* second, int64_t* third) {
+    if (!ctx || !second || !third) return fail(PSY_ERR_INVALID, "psy_ctx_trace_pass_counts: NULL argument");
+    *second = *third = 0;
+    if (!ctx->counters.p) return PSY_OK;
+    HIP_OK(hipSetDevice(ctx->device));
+    HIP_OK(hipDeviceSynchronize());
+    uint32_t c[4];
+    HIP_OK(hipMemcpy(c, ctx->counters.p, sizeof c, hipMemcpyDeviceToHost));
+    *second = (int64_t)c[0];
+    *third = (int64_t)c[1];
+    return PSY_OK;
+}
+
 int psy_ctx_set_trace_timing(psy_ctx* ctx, int enable) {
     if (!ctx) return fail(PSY_ERR_INVALID, "psy_ctx_set_trace_timing: NULL context");
     ctx->timing = enable != 0;

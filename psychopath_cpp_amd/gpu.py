@@ -37,6 +37,7 @@ def lib():
     L.psy_ctx_destroy.argtypes = [vp]
     L.psy_ctx_set_trace_mode.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.psy_ctx_retrace_count.argtypes = [vp, C.POINTER(i64)]
+    L.psy_ctx_trace_pass_counts.argtypes = [vp, C.POINTER(i64), C.POINTER(i64)]
     L.psy_ctx_set_render_chunk.argtypes = [vp, i64]
     L.psy_ctx_set_render_mode.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.psy_scene_create.argtypes = [vp, C.POINTER(abi.SceneDesc), C.c_int, C.POINTER(vp)]
@@ -131,6 +132,15 @@ class Context:
 
     def retrace_count(self):
         """psy_ctx_retrace_count: rays of the latest trace that took the second pass."""
+        return self._retrace_count()
+
+    def pass_counts(self):
+        """psy_ctx_trace_pass_counts: (second-pass rays, third-pass rays) of the latest trace."""
+        a, b = C.c_int64(), C.c_int64()
+        check(lib().psy_ctx_trace_pass_counts(self.handle, C.byref(a), C.byref(b)), "psy_ctx_trace_pass_counts")
+        return a.value, b.value
+
+    def _retrace_count(self):
         v = C.c_int64()
         check(lib().psy_ctx_retrace_count(self.handle, C.byref(v)), "psy_ctx_retrace_count")
         return v.value

@@ -131,6 +131,23 @@ PSY_DEV bool accept_leaf(float h0, float h1, float u, float v, float maxd, uint3
     return false;
 }
 
+// bbox_ray for a box of a patch's split tree, taking a NaN far distance as a
+// miss.  Under bbox_ray's min/max order the far distance is NaN only when
+// the x slab gives 0 * inf: the ray's direction has x = +-0 and the box's
+// low x plane holds the ray origin (y and z slabs never make it NaN).  Such
+// a box counts as hit at near distance 0 whatever its y and z extent, and it
+// never takes a hit itself, as (near + far) / 2 is NaN.  Halving at
+// midpoints never moves a half's low x plane below its parent's
+// ((a + b) * 0.5 >= min(a, b) in floating point), so each box below has its
+// low x plane at the origin (NaN again) or beyond it, where it misses: with
+// d.x = +0 its near distance is +inf while y or z (one of d.y, d.z is
+// nonzero) gives a finite far one; with d.x = -0 its far distance is -inf.
+// Skipping the subtree therefore changes no result, while walking it dices
+// the whole column of the patch along the plane to micropolygons.
+PSY_DEV bool patch_box_ray(V3 lo, V3 hi, V3 o, V3 dinv, float t, float* h0, float* h1) {
+    return bbox_ray(lo, hi, o, dinv, t, h0, h1) && *h1 == *h1;
+}
+
 // How a trace handles a patch node that needs splits below the dice cache:
 // D_FLAG flags the ray for the second pass (main pass), D_SERIAL splits on
 // in the lane within a node budget (second pass; a ray that runs out of
@@ -186,7 +203,7 @@ PSY_DEV bool dice_deep_from(const DevScene& s, const Patch<N>& base, const float
                 V3 lo, hi;
                 patch_bound<N>(p1, &lo, &hi);
                 float h0, h1;
-                if (bbox_ray(lo, hi, o, dinv, b.t, &h0, &h1)) {
+                if (patch_box_ray(lo, hi, o, dinv, b.t, &h0, &h1)) {
                     const float width = dice_width(min_width(w, h0, h1) * s.dice_rate, s.min_upoly);
                     const float max_dim = longest_axis(sub(hi, lo));
                     if (max_dim <= width || sidx + 1 == 63 || k + 1 == 62) {
@@ -203,7 +220,7 @@ PSY_DEV bool dice_deep_from(const DevScene& s, const Patch<N>& base, const float
                 V3 lo, hi;
                 patch_bound<N>(p, &lo, &hi);
                 float h0, h1;
-                if (bbox_ray(lo, hi, o, dinv, b.t, &h0, &h1)) {
+                if (patch_box_ray(lo, hi, o, dinv, b.t, &h0, &h1)) {
                     const float width = dice_width(min_width(w, h0, h1) * s.dice_rate, s.min_upoly);
                     const float max_dim = longest_axis(sub(hi, lo));
                     if (max_dim <= width || sidx == 63 || k + 1 == 62) {
@@ -237,7 +254,7 @@ PSY_DEV bool dice_deep_from(const DevScene& s, const Patch<N>& base, const float
         patch_bound<N>(p, &lo, &hi);
         float h0, h1;
         expand = false;
-        if (bbox_ray(lo, hi, o, dinv, b.t, &h0, &h1)) {
+        if (patch_box_ray(lo, hi, o, dinv, b.t, &h0, &h1)) {
             const float width = dice_width(min_width(w, h0, h1) * s.dice_rate, s.min_upoly);
             const float max_dim = longest_axis(sub(hi, lo));
             if (max_dim <= width || sidx0 + __builtin_popcountll(pb) == 63 || k == 62) {
@@ -337,7 +354,7 @@ __device__ bool dice_deep_coop(const DevScene& s, const DevObject& ob, uint32_t 
                 V3 lo, hi;
                 patch_bound<N>(p1, &lo, &hi);
                 float h0, h1;
-                if (bbox_ray(lo, hi, o, dinv, b.t, &h0, &h1)) {
+                if (patch_box_ray(lo, hi, o, dinv, b.t, &h0, &h1)) {
                     const float width = dice_width(min_width(w, h0, h1) * s.dice_rate, s.min_upoly);
                     const float max_dim = longest_axis(sub(hi, lo));
                     if (max_dim <= width || sidx + 1 == 63 || k + 1 == 62) {
@@ -354,7 +371,7 @@ __device__ bool dice_deep_coop(const DevScene& s, const DevObject& ob, uint32_t 
                 V3 lo, hi;
                 patch_bound<N>(p, &lo, &hi);
                 float h0, h1;
-                if (bbox_ray(lo, hi, o, dinv, b.t, &h0, &h1)) {
+                if (patch_box_ray(lo, hi, o, dinv, b.t, &h0, &h1)) {
                     const float width = dice_width(min_width(w, h0, h1) * s.dice_rate, s.min_upoly);
                     const float max_dim = longest_axis(sub(hi, lo));
                     if (max_dim <= width || sidx == 63 || k + 1 == 62) {
@@ -442,7 +459,7 @@ PSY_DEV bool dice_walk(const DevScene& s, const DevObject& ob, uint32_t inst, V3
     {
         const float4 a = nodes[2], c = nodes[3];
         float h0, h1;
-        if (!bbox_ray(xyz(a), xyz(c), o, dinv, b.t, &h0, &h1)) return false;
+        if (!patch_box_ray(xyz(a), xyz(c), o, dinv, b.t, &h0, &h1)) return false;
         const float maxd = longest_axis(sub(xyz(c), xyz(a)));
         if (maxd <= dice_width(min_width(w, h0, h1) * s.dice_rate, s.min_upoly))
             return accept_leaf(h0, h1, a.w, c.w, maxd, inst, b) && occl;
@@ -457,8 +474,8 @@ PSY_DEV bool dice_walk(const DevScene& s, const DevObject& ob, uint32_t inst, V3
         const float4* q = nodes + 2 * c0;
         const float4 a0 = q[0], e0 = q[1], a1 = q[2], e1 = q[3];
         float h00, h01, h10, h11;
-        const bool hit0 = bbox_ray(xyz(a0), xyz(e0), o, dinv, b.t, &h00, &h01);
-        const bool hit1 = bbox_ray(xyz(a1), xyz(e1), o, dinv, b.t, &h10, &h11);
+        const bool hit0 = patch_box_ray(xyz(a0), xyz(e0), o, dinv, b.t, &h00, &h01);
+        const bool hit1 = patch_box_ray(xyz(a1), xyz(e1), o, dinv, b.t, &h10, &h11);
         bool split0 = false, split1 = false;
         if (hit1) {
             const float maxd = longest_axis(sub(xyz(e1), xyz(a1)));
@@ -504,7 +521,7 @@ PSY_DEV bool dice_walk(const DevScene& s, const DevObject& ob, uint32_t inst, V3
             trail &= ~(1u << dd);
             const float4 a = nodes[2 * k], c = nodes[2 * k + 1];
             float h0, h1;
-            if (!bbox_ray(xyz(a), xyz(c), o, dinv, b.t, &h0, &h1)) continue;
+            if (!patch_box_ray(xyz(a), xyz(c), o, dinv, b.t, &h0, &h1)) continue;
             const float maxd = longest_axis(sub(xyz(c), xyz(a)));
             if (maxd <= dice_width(min_width(w, h0, h1) * s.dice_rate, s.min_upoly)) {
                 if (accept_leaf(h0, h1, a.w, c.w, maxd, inst, b) && occl) return true;

@@ -193,3 +193,21 @@ def test_cpu_build_adjoint_matches_finite_differences():
     fd_hb = J.central_fd(film_hb, alb, esc, target, params)
     sig = np.abs(fd_hb) >= 1e-3 * np.abs(fd_hb).max()
     assert (np.abs(grad - fd_hb)[sig] / np.abs(fd_hb)[sig]).max() <= 1e-3
+
+
+def test_cpu_build_zero_x_direction_rays_on_split_plane():
+    """Rays whose direction has x = +0 or -0, from an origin at x = 0: the
+    ground patch's first split puts a half's low x plane exactly at the
+    origin, where the box test's far distance is NaN.  The kernels skip such
+    subtrees (no box below them can take a hit) and the oracle walks them in
+    full; hits must agree exactly."""
+    sc = configs.sphere_scene()
+    o = ora.OracleScene(sc)
+    w, _ = o.camera_rays(64, 64, 1, 3)
+    sel = w[np.arange(0, len(w), len(w) // 32)][:32].copy()
+    sel["d"][:, 0] = np.where(np.arange(len(sel)) % 2 == 0, 0.0, -0.0).astype(np.float32)
+    assert (sel["o"][:, 0] == 0.0).all()
+    inter = o.trace(sel, threads=8, chunk=64)
+    hb = hostbuild.trace(sc, ora.planes_from_wrays(sel), world=True, dice_depth=8)
+    agree, t_ok, hits = compare(hb, inter)
+    assert hits > 0 and agree == 1.0 and t_ok == 1.0

@@ -284,8 +284,9 @@ int psy_abi_version(void);
 /* Replaces: Scene::finalize -> Assembly::finalize -> BVH4::build /
  * LightTree::build (scene/scene.hpp:50-52, scene/assembly.hpp:206-238,
  * accel/bvh4.cpp:17-159).  Flattens the assembly hierarchy, builds the
- * instance BVH and the patch dice cache in HBM.  dice_depth < 0 picks it
- * from the scene. */
+ * instance BVH and the patch dice cache in HBM.  dice_depth (0..24) is the
+ * cached split depth; < 0 picks it from the patch count: up to 24 levels
+ * within 2 GiB of cache, else up to 18 within 16 GiB. */
 int psy_scene_create(psy_ctx* ctx, const psy_scene_desc* desc, int dice_depth,
                      psy_scene** out);
 int psy_scene_destroy(psy_scene* scene);

@@ -793,13 +793,17 @@ int prepare_scene(const psy_scene_desc* d, int dice_depth, HostScene& H) {
         if (objs[i].kind == PSY_OBJ_BILINEAR || objs[i].kind == PSY_OBJ_BICUBIC) patch_ids.push_back((int32_t)i);
     int levels = dice_depth;
     if (levels < 0) {
-        // 16 GiB of split-tree nodes (under 6% of an MI355X's HBM), at most
-        // 18 levels; rays that need deeper splits take the second pass
-        const double budget = 16.0 * 1024.0 * 1024.0 * 1024.0;
-        levels = 18;
-        while (levels > 0 && (double)((int64_t)1 << (levels + 1)) * 32.0 * (double)patch_ids.size() > budget) --levels;
+        // Up to 24 levels while the cache stays within 2 GiB (scenes of a few
+        // large patches, e.g. a ground plane diced to pixel size), else up to
+        // 18 levels within 16 GiB (under 6% of an MI355X's HBM); rays that
+        // need deeper splits take the second pass
+        const double gib = 1024.0 * 1024.0 * 1024.0;
+        auto bytes = [&](int l) { return (double)((int64_t)1 << (l + 1)) * 32.0 * (double)patch_ids.size(); };
+        levels = 24;
+        while (levels > 18 && bytes(levels) > 2.0 * gib) --levels;
+        while (levels > 0 && bytes(levels) > 16.0 * gib) --levels;
     }
-    if (levels > 20) return fail(PSY_ERR_INVALID, "psy_scene_create: dice_depth above 20");
+    if (levels > 24) return fail(PSY_ERR_INVALID, "psy_scene_create: dice_depth above 24");
     const int64_t per = (int64_t)1 << (levels + 1);  // slot: 1-based heap, entry 0 unused
     H.levels = levels;
     H.per = per;

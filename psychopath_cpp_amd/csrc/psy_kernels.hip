@@ -162,7 +162,7 @@ __global__ __launch_bounds__(TRACE_BLOCK, persist_waves(WORLD, STACK)) void k_tr
 }
 
 template <bool WORLD, int STACK>
-__global__ __launch_bounds__(TRACE_BLOCK) void k_trace_deep(DevScene s, const float* __restrict__ rays, int64_t n,
+__global__ __launch_bounds__(TRACE_BLOCK, PSY_DEEP_WAVES) void k_trace_deep(DevScene s, const float* __restrict__ rays, int64_t n,
                                                              int64_t rs, DevHits h, uint32_t flags,
                                                              const uint32_t* __restrict__ queue,
                                                              uint32_t* __restrict__ counters,
@@ -651,7 +651,7 @@ int launch_build_dice(const float* odata, const DevObject* objs, const int32_t* 
 template <bool W, int S>
 static void launch_deep(const DevScene& s, const float* rays, int64_t n, int64_t rs, const DevHits& h, uint32_t flags,
                         const uint32_t* queue, uint32_t* counters, hipStream_t st) {
-    const unsigned blocks = (unsigned)std::min<int64_t>(grid_for(n, TRACE_BLOCK), 1024);
+    const unsigned blocks = (unsigned)std::min<int64_t>(grid_for(n, TRACE_BLOCK), PSY_DEEP_BLOCKS);
     uint32_t* coop_queue = const_cast<uint32_t*>(queue) + n;
     hipLaunchKernelGGL((k_trace_deep<W, S>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, s, rays, n, rs, h, flags,
                        queue, counters, coop_queue);

@@ -805,6 +805,9 @@ int prepare_scene(const psy_scene_desc* d, int dice_depth, HostScene& H) {
     }
     if (levels > 24) return fail(PSY_ERR_INVALID, "psy_scene_create: dice_depth above 24");
     const int64_t per = (int64_t)1 << (levels + 1);  // slot: 1-based heap, entry 0 unused
+    // the main pass addresses the cache with 32-bit node indices
+    if ((double)per * (double)patch_ids.size() > 4294967295.0)
+        return fail(PSY_ERR_INVALID, "psy_scene_create: dice cache above 2^32 nodes; lower dice_depth");
     H.levels = levels;
     H.per = per;
     for (size_t k = 0; k < patch_ids.size(); ++k) {

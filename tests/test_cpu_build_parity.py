@@ -211,3 +211,17 @@ def test_cpu_build_zero_x_direction_rays_on_split_plane():
     hb = hostbuild.trace(sc, ora.planes_from_wrays(sel), world=True, dice_depth=8)
     agree, t_ok, hits = compare(hb, inter)
     assert hits > 0 and agree == 1.0 and t_ok == 1.0
+
+
+def test_cpu_build_rejects_oversized_dice_cache():
+    """Scene set-up refuses a dice depth above 24, and a cache of more than
+    2^32 nodes (the main pass's node indices are 32-bit): 256 patches at
+    24 levels hold 2^33."""
+    sc = configs.heightfield_scene(grid=16)
+    w, _ = ora.OracleScene(sc).camera_rays(4, 4, 1, 1)
+    planes = ora.planes_from_wrays(w)
+    for depth in (25, 24):
+        with pytest.raises(RuntimeError):
+            hostbuild.trace(sc, planes, world=True, dice_depth=depth)
+    hb = hostbuild.trace(sc, planes, world=True, dice_depth=8)
+    assert (hb["inst"] != 0xFFFFFFFF).all()

@@ -161,16 +161,16 @@ __global__ __launch_bounds__(TRACE_BLOCK, persist_waves(WORLD, STACK)) void k_tr
     }
 }
 
-template <bool WORLD, int STACK>
-__global__ __launch_bounds__(TRACE_BLOCK, PSY_DEEP_WAVES) void k_trace_deep(DevScene s, const float* __restrict__ rays, int64_t n,
-                                                             int64_t rs, DevHits h, uint32_t flags,
-                                                             const uint32_t* __restrict__ queue,
-                                                             uint32_t* __restrict__ counters,
-                                                             uint32_t* __restrict__ coop_queue) {
+template <bool WORLD, int STACK, bool WIDE>
+__global__ __launch_bounds__(TRACE_BLOCK, WIDE ? DEEP_WIDE_WAVES : 1)
+void k_trace_deep(DevScene s, const float* __restrict__ rays, int64_t n, int64_t rs, DevHits h, uint32_t flags,
+                  const uint32_t* __restrict__ queue, uint32_t* __restrict__ counters,
+                  uint32_t* __restrict__ coop_queue) {
     __shared__ int32_t stk_ref[STACK][TRACE_BLOCK];
     __shared__ float stk_near[STACK][TRACE_BLOCK];
     const int tid = threadIdx.x;
     const uint32_t count = counters[0];
+    if (WIDE != (count >= DEEP_WIDE_MIN)) return;  // the other build walks this queue
     // Queue entry k goes to wave k % n_waves, lane k / n_waves: a short queue
     // puts one ray in each wave, so rays that take different paths through
     // the split trees run side by side instead of serializing in one wave.
@@ -651,10 +651,15 @@ int launch_build_dice(const float* odata, const DevObject* objs, const int32_t* 
 template <bool W, int S>
 static void launch_deep(const DevScene& s, const float* rays, int64_t n, int64_t rs, const DevHits& h, uint32_t flags,
                         const uint32_t* queue, uint32_t* counters, hipStream_t st) {
-    const unsigned blocks = (unsigned)std::min<int64_t>(grid_for(n, TRACE_BLOCK), PSY_DEEP_BLOCKS);
+    const unsigned blocks = (unsigned)std::min<int64_t>(grid_for(n, TRACE_BLOCK), DEEP_ROUND);
     uint32_t* coop_queue = const_cast<uint32_t*>(queue) + n;
-    hipLaunchKernelGGL((k_trace_deep<W, S>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, s, rays, n, rs, h, flags,
+    hipLaunchKernelGGL((k_trace_deep<W, S, false>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, s, rays, n, rs, h, flags,
                        queue, counters, coop_queue);
+    if (n >= (int64_t)DEEP_WIDE_MIN) {
+        const unsigned wide = (unsigned)std::min<int64_t>(grid_for(n, TRACE_BLOCK), DEEP_ROUND * DEEP_WIDE_WAVES);
+        hipLaunchKernelGGL((k_trace_deep<W, S, true>), dim3(wide), dim3(TRACE_BLOCK), 0, st, s, rays, n, rs, h,
+                           flags, queue, counters, coop_queue);
+    }
     hipLaunchKernelGGL((k_trace_coop<W, S>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, s, rays, n, rs, h, flags,
                        (const uint32_t*)counters, (const uint32_t*)coop_queue);
 }

@@ -724,18 +724,17 @@ PSY_DEV void cmpswap(float& ka, int32_t& ra, float& kb, int32_t& rb) {
 #define PSY_TRACE_BLOCK 64
 #endif
 constexpr int TRACE_BLOCK = PSY_TRACE_BLOCK;  // threads per block of the trace kernels
-// The second pass is compiled for 2 waves per SIMD (128 registers less
-// than it would take, paid for with some spills: 1.4x faster than 1 wave
-// on a ground plane diced below the cache), and the second and third passes
-// launch one resident round of 64-thread blocks at that occupancy over
-// 256 CUs; their rays come from a device-side queue whose length the host
-// does not know.
-#ifndef PSY_DEEP_WAVES
-#define PSY_DEEP_WAVES 2
-#endif
-#ifndef PSY_DEEP_BLOCKS
-#define PSY_DEEP_BLOCKS 2048
-#endif
+// The second pass has two builds, both launched on every trace; each reads
+// the queue length and only one of them walks the queue.  The narrow one
+// runs 1 wave per SIMD (256 VGPRs, no spills): the lowest latency per ray,
+// for short queues (a primary stream hands on a few hundred rays).  The wide
+// one runs 2 waves per SIMD (128 VGPRs, paid for with spills): 1.4x the
+// throughput on long queues (a ground plane diced below the cache hands on
+// half the camera rays).  Each launches one resident round of 64-thread
+// blocks over 256 CUs, as does the third pass (1 wave per SIMD).
+constexpr uint32_t DEEP_WIDE_MIN = 4096;  // queue length from which the wide build walks it
+constexpr int DEEP_WIDE_WAVES = 2;
+constexpr int DEEP_ROUND = 1024;          // blocks per wave per SIMD over 256 CUs
 
 // A ray of the stream: world form (PSY_WRAY_PLANES, footprint derived here,
 // basics/ray.hpp:310-323) or object form (PSY_RAY_PLANES).

@@ -366,6 +366,45 @@ def test_gpu_c1_full_stream(c1, oracle):
         ctx.close()
 
 
+def test_gpu_long_second_pass_queue(oracle):
+    """Camera rays at 1024 x 1024 over the sphere scene: the ground patch
+    dices below its 24 cached levels for a large share of them, so the
+    second pass walks a queue of thousands of rays (its wide build, 2 waves
+    per SIMD, takes queues of 4096 or more); a random sample of 4096 rays
+    matches the oracle."""
+    sc = configs.sphere_scene()
+    o = oracle.OracleScene(sc)
+    ctx = gpu.Context(0)
+    try:
+        g = gpu.GpuScene(ctx, sc)
+        assert g.info.dice_depth == 24
+        w, _ = gpu.camera_rays(ctx, g, 1024, 1024, 1, 3)
+        hits = gpu.trace_world(ctx, g, w)
+        second, third = ctx.pass_counts()
+        say("second-pass queue", second, "third", third)
+        assert second >= 4096
+        n = w.shape[1]
+        inst = hits.inst.cpu().numpy().view(np.uint32)
+        rng = np.random.default_rng(31)
+        pick = np.sort(rng.choice(n, 4096, replace=False))
+        wo = oracle.wrays_from_planes(w[:, torch.from_numpy(pick).cuda()].cpu().numpy())
+        inter = o.trace(wo, threads=8, chunk=4096)
+        hit_o = inter["hit"] != 0
+        hit_g = inst[pick] != 0xFFFFFFFF
+        ids = g.instance_ids()
+        same_id = hit_o == hit_g
+        both = hit_g & hit_o
+        same_id[both] = ids[inst[pick][both]] == inter["id"][both]
+        t_g = hits.t.cpu().numpy()[pick]
+        t_ok = np.abs(t_g[both] - inter["t"][both]) <= 1e-5 * np.abs(inter["t"][both])
+        say("long queue sample: ids agree", same_id.mean(), "t within 1e-5", t_ok.mean())
+        assert hit_o.mean() > 0.3
+        assert same_id.mean() >= 0.9999 and t_ok.mean() >= 0.9999
+        g.close()
+    finally:
+        ctx.close()
+
+
 def _nested_spheres(n=4096, seed=5):
     """Spheres around one centre: a ray through it hits every BVH box, so
     every node pushes three children and the main pass's stack overflows."""

@@ -54,12 +54,44 @@ def make_stream(ctx, scene):
     return wrays, int(valid.sum().item())
 
 
-def cpu_baseline(scene_py, wrays, sample, min_seconds):
-    """The oracle's breadth-first tracer on the first `sample` rays of the
-    stream, repeated until at least `min_seconds` of CPU work are timed."""
+def host_threads():
+    """Host cores this process may use: its CPU affinity (what `nproc`
+    reports), capped by OMP_NUM_THREADS where the host sets it (a shared
+    GPU host sets it to the CPU share of one GPU)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def mixed_sample(n_total, sample):
+    """Indices of an equal mix of the stream's two halves: the first
+    sample/2 primary rays and the first sample/2 bounce rays."""
+    import numpy as np
+    half = sample // 2
+    return np.concatenate([np.arange(half), n_total // 2 + np.arange(sample - half)])
+
+
+def cpu_baseline(scene_py, wrays, pick, min_seconds):
+    """The oracle's breadth-first tracer on the rays `pick` of the stream,
+    repeated until at least `min_seconds` of CPU work are timed."""
+    import torch
     from oracle import ora
-    threads = min(16, os.cpu_count() or 1)
-    sub = wrays[:, :sample].cpu().numpy()
+    threads = host_threads()
+    sample = len(pick)
+    sub = wrays[:, torch.from_numpy(pick).to(wrays.device)].cpu().numpy()
     w = ora.wrays_from_planes(sub)
     o = ora.OracleScene(scene_py)
     inter = None
@@ -87,6 +119,10 @@ def traffic_from_profile():
                 return float(line.split(":", 1)[1].strip())
     return None
 
+
+# names of PSY_RENDER_* and PSY_TRACE_MODE_* (include/psy_gpu.h)
+RENDER_FORMS = {0: "wavefront", 1: "fused", 2: "stream"}
+TRACE_MODES = {0: "plain", 1: "persistent", 2: "persistent-coherent", 3: "plain-coherent"}
 
 REF_BUCKET_PATHS = 65536   # Config::samples_per_bucket (config.cpp:12; integrator/path_trace_integrator.cpp:29-34)
 
@@ -121,6 +157,9 @@ def batching_forms():
         gpu.set_render_chunk(ctx, REF_BUCKET_PATHS)
         plain, f_plain = _render_rate(gpu, ctx, g, W, H, spp, pl, reps=1)
         log(f"batching [{name[:40]}]: plain wavefront, {REF_BUCKET_PATHS}-path buckets: {plain:.1f} Mrays/s")
+        gpu.set_render_chunk(ctx, 1 << 21)
+        dflt, f_dflt = _render_rate(gpu, ctx, g, W, H, spp, pl)
+        log(f"batching: wavefront, library default 2^21-path chunks: {dflt:.1f} Mrays/s")
         gpu.set_render_chunk(ctx, 1 << 22)
         wide, f_wide = _render_rate(gpu, ctx, g, W, H, spp, pl)
         log(f"batching: wavefront, 2^22-path chunks: {wide:.1f} Mrays/s")
@@ -129,15 +168,20 @@ def batching_forms():
         tune_s = time.perf_counter() - t0
         tuned, f_tuned = _render_rate(gpu, ctx, g, W, H, spp, pl)
         log(f"batching: autotuned {tune} in {tune_s:.2f} s: {tuned:.1f} Mrays/s")
-        same = bool(torch.equal(f_plain, f_tuned) and torch.equal(f_plain, f_wide))
+        same = bool(torch.equal(f_plain, f_tuned) and torch.equal(f_plain, f_wide) and torch.equal(f_plain, f_dflt))
         out.append({"workload": name, "plain_mrays": round(plain, 2),
                     "plain_form": f"wavefront, {REF_BUCKET_PATHS}-path buckets, host sync per step",
+                    "default_wavefront_mrays": round(dflt, 2),
+                    "default_form": "wavefront, the library's default 2^21-path chunks",
                     "wavefront_2p22_mrays": round(wide, 2), "autotuned_mrays": round(tuned, 2),
-                    "autotuned": {"form": "stream" if tune["mode"] == 2 else "wavefront", "paths": int(tune["pool"]),
-                                  "trace": "persistent" if tune["trace_mode"] == 1 else "plain",
+                    "autotuned": {"form": RENDER_FORMS.get(int(tune["mode"]), str(tune["mode"])),
+                                  "paths": int(tune["pool"]),
+                                  "trace": TRACE_MODES.get(int(tune["trace_mode"]), str(tune["trace_mode"])),
                                   "stack": int(tune["stack"]), "steps_per_graph": int(tune["steps_per_graph"]),
                                   "tune_seconds": round(tune_s, 2)},
-                    "speedup_vs_plain": round(tuned / plain, 2), "speedup_vs_2p22_wavefront": round(tuned / wide, 2),
+                    "speedup_vs_plain": round(tuned / plain, 2),
+                    "speedup_vs_default_wavefront": round(tuned / dflt, 2),
+                    "speedup_vs_2p22_wavefront": round(tuned / wide, 2),
                     "images_identical": same})
         g.close()
         ctx.close()
@@ -284,14 +328,17 @@ def main():
     cpu = None
     if not args.no_cpu:
         sample = min(args.cpu_sample, STREAM_RAYS)
-        log(f"cpu baseline: oracle breadth-first tracer on {sample} rays")
-        cpu_mrays, threads, inter, dt, reps = cpu_baseline(scene_py, wrays, sample, args.cpu_seconds)
+        pick = mixed_sample(STREAM_RAYS, sample)
+        model = cpu_model()
+        log(f"cpu baseline: oracle breadth-first tracer on {sample} rays (half primary, half bounce), {model}")
+        cpu_mrays, threads, inter, dt, reps = cpu_baseline(scene_py, wrays, pick, args.cpu_seconds)
         log(f"cpu baseline: {cpu_mrays:.3f} Mrays/s on {threads} threads ({reps} passes, {dt:.1f}s)")
         # spot parity on the same sample
         import numpy as np
         ids = scene.instance_ids()
-        gi = hits.inst[:sample].cpu().numpy().view(np.uint32)
-        gt = hits.t[:sample].cpu().numpy()
+        tp = torch.from_numpy(pick).to(hits.inst.device)
+        gi = hits.inst[tp].cpu().numpy().view(np.uint32)
+        gt = hits.t[tp].cpu().numpy()
         ghit = gi != 0xFFFFFFFF
         ok = (ghit == (inter["hit"] != 0))
         both = ghit & (inter["hit"] != 0)
@@ -299,9 +346,13 @@ def main():
         ok[both] &= np.abs(gt[both] - inter["t"][both]) <= 1e-5 * np.abs(inter["t"][both])
         log(f"parity on the cpu sample: {ok.mean():.6f} of rays match")
         cpu = {"value": round(cpu_mrays, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
-               "sample": f"first {sample} rays of the same 2^22 stream (primary rays, 1024x1024x2 spp), "
-                         f"traced {reps} times ({dt:.1f} s) by the oracle restatement of the reference "
-                         f"breadth-first BVH4 + split-dice tracer, 65536-ray chunks, {threads} threads"}
+               "cpu_model": model,
+               "host_cpus": os.cpu_count(),
+               "sample": f"{sample} rays of the same 2^22 stream, an equal mix of its two halves (the first "
+                         f"{sample // 2} primary rays, 1024x1024x2 spp, and the first {sample - sample // 2} "
+                         f"cosine first-bounce rays), traced {reps} times ({dt:.1f} s) by the oracle restatement "
+                         f"of the reference breadth-first BVH4 + split-dice tracer, 65536-ray chunks, "
+                         f"{threads} threads (this process's CPU affinity, capped by OMP_NUM_THREADS)"}
 
     out = {
         "metric": METRIC,

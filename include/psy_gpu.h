@@ -228,6 +228,15 @@ int psy_ctx_set_trace_mode(psy_ctx* ctx, int mode, int refill_min, int blocks_pe
  * per SIMD) or 32 (2 waves per SIMD; fewer rays handed to the second pass
  * on deep hierarchies).  Results are identical. */
 int psy_ctx_set_trace_stack(psy_ctx* ctx, int entries);
+/* The trace's second pass (no counterpart in the reference, which splits
+ * patches per ray batch without a cache): rays that the main pass hands on
+ * are retraced one lane each, by a narrow build (1 wave per SIMD) for short
+ * queues or a wide build (2 waves per SIMD) for queues of wide_min rays or
+ * more (default 4096; 0: always the wide build; UINT32_MAX: always the
+ * narrow one); a ray that takes more than `budget` (>= 1, default 512)
+ * split-tree node expansions there goes on to the third, wave-cooperative
+ * pass.  Results do not depend on either setting. */
+int psy_ctx_set_second_pass(psy_ctx* ctx, uint32_t wide_min, int32_t budget);
 /* Paths per wavefront chunk of psy_render (no counterpart in the reference,
  * whose buckets are sqrt(65536/spp) pixels square): whole pixels, at least
  * one; the default 2^21 paths.  Results do not depend on it. */
@@ -299,7 +308,15 @@ int psy_scene_instance_ids(const psy_scene* scene, uint64_t* out, int64_t n);
 /* Replaces: uint32_t Tracer::trace(const WorldRay*, const WorldRay*,
  *           Intersection*, Intersection*)  (tracer/tracer.hpp:76,
  *           tracer/tracer.cpp:38-121).  Closest hit per ray (or any hit
- *           with PSY_TRACE_OCCLUSION_ONLY / per-ray occlusion type). */
+ *           with PSY_TRACE_OCCLUSION_ONLY / per-ray occlusion type).
+ * Work is queued on `stream` and the call returns without waiting.  The
+ * trace, render and adjoint calls of one context share its scratch
+ * memory (the second- and third-pass queues): a call on a stream other
+ * than the previous call's is made to wait on the device for that call,
+ * so calls on different streams of one context run one after the other.
+ * (Calls queued on a stream under graph capture are not ordered this
+ * way.)  The caller orders its own ray and hit buffers.  One context is
+ * not to be used from several host threads at once. */
 int psy_trace_world(psy_ctx* ctx, const psy_scene* scene,
                     const float* world_rays, int64_t n,
                     const psy_hits* hits, uint32_t flags, void* stream);

@@ -139,6 +139,31 @@ int ensure_queue(psy_ctx* ctx, int64_t n) {
     ctx->queue_cap = cap;
     return PSY_OK;
 }
+
+static int capturing(hipStream_t st, bool* yes) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIP_OK(hipStreamIsCapturing(st, &cs));
+    *yes = cs != hipStreamCaptureStatusNone;
+    return PSY_OK;
+}
+
+int scratch_order(psy_ctx* ctx, hipStream_t st) {
+    bool cap = false;
+    if (int rc = capturing(st, &cap)) return rc;
+    if (cap || !ctx->scratch_recorded || ctx->scratch_stream == st) return PSY_OK;
+    HIP_OK(hipStreamWaitEvent(st, ctx->scratch_done, 0));
+    return PSY_OK;
+}
+
+int scratch_release(psy_ctx* ctx, hipStream_t st) {
+    bool cap = false;
+    if (int rc = capturing(st, &cap)) return rc;
+    if (cap) return PSY_OK;
+    HIP_OK(hipEventRecord(ctx->scratch_done, st));
+    ctx->scratch_stream = st;
+    ctx->scratch_recorded = true;
+    return PSY_OK;
+}
 }  // namespace psy
 
 extern "C" {
@@ -168,7 +193,16 @@ int psy_ctx_create(int device, psy_ctx** out) {
         trace_persist_occupancy(false, 32, &c->occupancy32[0]) ||
         trace_persist_occupancy(true, 32, &c->occupancy32[1]) || render_persist_occupancy(&c->render_occupancy))
         return fail(PSY_ERR_HIP, "psy_ctx_create: occupancy query failed");
+    HIP_OK(hipEventCreateWithFlags(&c->scratch_done, hipEventDisableTiming));
     *out = c.release();
+    return PSY_OK;
+}
+
+int psy_ctx_set_second_pass(psy_ctx* ctx, uint32_t wide_min, int32_t budget) {
+    if (!ctx) return fail(PSY_ERR_INVALID, "psy_ctx_set_second_pass: NULL context");
+    if (budget < 1) return fail(PSY_ERR_INVALID, "psy_ctx_set_second_pass: budget must be at least 1");
+    ctx->deep_wide_min = wide_min;
+    ctx->deep_budget = budget;
     return PSY_OK;
 }
 
@@ -439,6 +473,7 @@ struct B2 {
     Box box0, box1;  // at time 0 and 1: their lerp bounds the subtree at any time
     int left = -1, right = -1;
     int leaf = -1;
+    int count = 0;   // leaves below
 };
 
 struct Builder2 {
@@ -448,7 +483,14 @@ struct Builder2 {
     std::vector<float> cent;  // 3 per prim
     std::vector<int> idx;
 
-    int build(int begin, int end) {
+    // Binned SAH down to depth SAH_DEPTH, median splits below it, so a
+    // skewed layout (e.g. instances spaced geometrically, which SAH peels
+    // off one at a time) stays within SAH_DEPTH + log2(n) levels.  With
+    // median_only every split is a median one (a balanced tree).
+    static constexpr int SAH_DEPTH = 20;
+    bool median_only = false;
+
+    int build(int begin, int end, int depth = 0) {
         const int me = (int)nodes.size();
         nodes.emplace_back();
         Box b, cb, b0, b1;
@@ -461,6 +503,7 @@ struct Builder2 {
         nodes[me].box = b;
         nodes[me].box0 = b0;
         nodes[me].box1 = b1;
+        nodes[me].count = end - begin;
         if (end - begin == 1) { nodes[me].leaf = idx[begin]; return me; }
         int axis = 0;
         float ext[3];
@@ -468,7 +511,10 @@ struct Builder2 {
         if (ext[1] > ext[axis]) axis = 1;
         if (ext[2] > ext[axis]) axis = 2;
         int mid = (begin + end) / 2;
-        if (ext[axis] > 0.0f) {
+        if (ext[axis] > 0.0f && (median_only || depth >= SAH_DEPTH)) {
+            std::nth_element(idx.begin() + begin, idx.begin() + mid, idx.begin() + end,
+                             [&](int a, int c) { return cent[3 * a + axis] < cent[3 * c + axis]; });
+        } else if (ext[axis] > 0.0f) {
             const int NB = 16;
             Box bb[NB];
             int bc[NB] = {0};
@@ -502,8 +548,8 @@ struct Builder2 {
                                  [&](int a, int c) { return cent[3 * a + axis] < cent[3 * c + axis]; });
             }
         }
-        const int l = build(begin, mid);
-        const int r = build(mid, end);
+        const int l = build(begin, mid, depth + 1);
+        const int r = build(mid, end, depth + 1);
         nodes[me].left = l;
         nodes[me].right = r;
         return me;
@@ -515,6 +561,10 @@ struct Collapser {
     std::vector<BvhNode> out;   // child boxes at time 0
     std::vector<BvhNode> out1;  // the same nodes: child box change from time 0 to 1
     int max_need = 0;
+    // which binary child a 4-wide node opens up next: the largest surface
+    // area (the SAH's choice), or with by_count the most leaves (halves the
+    // depth of a balanced tree)
+    bool by_count = false;
 
     // returns child ref, and the traversal stack need of the subtree
     int32_t emit(int b2, int* need) {
@@ -527,7 +577,7 @@ struct Collapser {
             for (int k = 0; k < (int)kids.size(); ++k) {
                 const B2& c = (*n2)[kids[k]];
                 if (c.leaf >= 0) continue;
-                const float a = c.box.area();
+                const float a = by_count ? (float)c.count : c.box.area();
                 if (a > ba) { ba = a; best = k; }
             }
             if (best < 0) break;
@@ -781,6 +831,23 @@ int prepare_scene(const psy_scene_desc* d, int dice_depth, HostScene& H) {
         int need = 0;
         root = cl.emit(0, &need);
         stack_need = std::max(1, need) + 1;
+        if (stack_need > 64) {
+            // A layout the SAH tree nests too deeply for the traversal
+            // stacks: a balanced tree (median splits, 4-wide nodes opened up
+            // by leaf count) needs at most 3 entries per 4-wide level, 49
+            // for 2^32 leaves.
+            b2.nodes.clear();
+            b2.median_only = true;
+            b2.build(0, (int)nl);
+            cl = Collapser{};
+            cl.n2 = &b2.nodes;
+            cl.by_count = true;
+            root = cl.emit(0, &need);
+            stack_need = std::max(1, need) + 1;
+        }
+        if (stack_need > 64)
+            return fail(PSY_ERR_UNSUPPORTED, "psy_scene_create: the instance BVH needs a traversal stack of " +
+                                                 std::to_string(stack_need) + " entries; at most 64 are supported");
         bvh.swap(cl.out);
         if (moving) H.bvh1.swap(cl.out1);
     }
@@ -1030,9 +1097,12 @@ int psy_trace_world(psy_ctx* ctx, const psy_scene* s, const float* rays, int64_t
     if (int rc = check_hits(hits, n)) return rc;
     HIP_OK(hipSetDevice(ctx->device));
     if (int rc = ensure_queue(ctx, n)) return rc;
+    const hipStream_t st = (hipStream_t)stream;
+    if (int rc = scratch_order(ctx, st)) return rc;
     int rc = launch_trace(s->dev, true, (flags & PSY_TRACE_OCCLUSION_ONLY) != 0, s->stack_need, rays, n, n, to_dev(hits),
-                          (uint32_t*)ctx->queue.p, (uint32_t*)ctx->counters.p, ctx->tuning(true), (hipStream_t)stream);
-    return rc ? fail(rc, "psy_trace_world: launch failed") : PSY_OK;
+                          (uint32_t*)ctx->queue.p, (uint32_t*)ctx->counters.p, ctx->tuning(true), st);
+    if (rc) return fail(rc, "psy_trace_world: launch failed");
+    return scratch_release(ctx, st);
 }
 
 int psy_trace_rays(psy_ctx* ctx, const psy_scene* s, const float* rays, int64_t n, const psy_hits* hits,
@@ -1045,10 +1115,12 @@ int psy_trace_rays(psy_ctx* ctx, const psy_scene* s, const float* rays, int64_t 
     if (int rc = check_hits(hits, n)) return rc;
     HIP_OK(hipSetDevice(ctx->device));
     if (int rc = ensure_queue(ctx, n)) return rc;
+    const hipStream_t st = (hipStream_t)stream;
+    if (int rc = scratch_order(ctx, st)) return rc;
     int rc = launch_trace(s->dev, false, (flags & PSY_TRACE_OCCLUSION_ONLY) != 0, s->stack_need, rays, n, n,
-                          to_dev(hits), (uint32_t*)ctx->queue.p, (uint32_t*)ctx->counters.p, ctx->tuning(false),
-                          (hipStream_t)stream);
-    return rc ? fail(rc, "psy_trace_rays: launch failed") : PSY_OK;
+                          to_dev(hits), (uint32_t*)ctx->queue.p, (uint32_t*)ctx->counters.p, ctx->tuning(false), st);
+    if (rc) return fail(rc, "psy_trace_rays: launch failed");
+    return scratch_release(ctx, st);
 }
 
 int psy_world_to_rays(psy_ctx* ctx, const float* wrays, int64_t n, float* rays, void* stream) {

@@ -134,6 +134,11 @@ struct DevLights {
 // ---- launchers (psy_kernels.hip) -------------------------------------------
 int launch_build_dice(const float* odata, const DevObject* objs, const int32_t* patch_ids,
                       int n_patches, int levels, float4* dice, hipStream_t st);
+// Second-pass defaults: the queue length from which its wide build walks
+// the queue, and the split-tree node expansions per ray before the third
+// pass takes the ray.
+constexpr uint32_t DEEP_WIDE_MIN = 4096;
+constexpr int32_t DEEP_BUDGET = 512;
 // How the main trace pass is launched: persistent (persist_blocks blocks,
 // lanes refilled when refill_min of a wave's 64 are idle) or one thread per
 // ray.
@@ -146,6 +151,12 @@ struct TraceTuning {
     int refill_min;
     int stack = 16;        // main-pass LDS stack entries: 16 or 32
     const uint32_t* n_dev = nullptr;  // device-side ray count (<= n), or NULL
+    // second pass: queue length from which its wide build walks the queue
+    // (0: always the wide build, UINT32_MAX: always the narrow one), and
+    // the split-tree node expansions a ray may take there before it is
+    // handed to the third pass
+    uint32_t wide_min = DEEP_WIDE_MIN;
+    int32_t deep_budget = DEEP_BUDGET;
     // hipEvent_t handles recorded around the passes when the context times
     // them: before the main pass, after it, after the second pass.
     void* ev[3] = {nullptr, nullptr, nullptr};

@@ -73,6 +73,17 @@ struct psy_ctx {
     // trace scratch: queue of rays for the second pass + its counters
     psy::DBuf queue, counters;
     int64_t queue_cap = 0;
+    // Calls on one context share that scratch.  Each call records
+    // scratch_done on its stream when its work is queued; a call on another
+    // stream first makes its stream wait for that event, so calls made on
+    // different streams run one after the other (scratch_order,
+    // scratch_release in psy_capi.cpp).
+    hipEvent_t scratch_done = nullptr;
+    hipStream_t scratch_stream = nullptr;
+    bool scratch_recorded = false;
+    // second pass: queue length from which the wide build walks, node budget
+    uint32_t deep_wide_min = psy::DEEP_WIDE_MIN;
+    int32_t deep_budget = psy::DEEP_BUDGET;
     // main-pass launch form, per ray form (0: object rays, 1: world rays)
     int n_cu = 0;
     int occupancy[2] = {0, 0};    // 16-entry stack
@@ -93,6 +104,8 @@ struct psy_ctx {
         t.persist_blocks32 = n_cu * (blocks_per_cu > 0 ? blocks_per_cu : occupancy32[world ? 1 : 0]);
         t.refill_min = refill_min;
         t.stack = trace_stack;
+        t.wide_min = deep_wide_min;
+        t.deep_budget = deep_budget;
         if (timing && timed) {
             for (int k = 0; k < 3; ++k) {
                 hipEvent_t e = nullptr;
@@ -112,6 +125,7 @@ struct psy_ctx {
     }
     ~psy_ctx() {
         for (void* e : timing_events) (void)hipEventDestroy((hipEvent_t)e);
+        if (scratch_done) (void)hipEventDestroy(scratch_done);
         if (capture_stream) (void)hipStreamDestroy(capture_stream);
     }
     // integrator scratch (path states, ray / hit streams, active lists)
@@ -194,4 +208,10 @@ int upload_lights(const HostScene& H, psy_scene* sc);
 void make_halton_tables(std::vector<HaltonDim>& dims, std::vector<uint16_t>& perm);
 // Grows the context's trace scratch (second-pass queue) to n rays.
 int ensure_queue(psy_ctx* ctx, int64_t n);
+// Orders a call that uses the context's trace scratch on stream st after
+// the previous such call (made on any stream): scratch_order before the
+// call's first launch, scratch_release after its last.  Streams under graph
+// capture are left alone.
+int scratch_order(psy_ctx* ctx, hipStream_t st);
+int scratch_release(psy_ctx* ctx, hipStream_t st);
 }  // namespace psy

@@ -165,12 +165,12 @@ template <bool WORLD, int STACK, bool WIDE>
 __global__ __launch_bounds__(TRACE_BLOCK, WIDE ? DEEP_WIDE_WAVES : 1)
 void k_trace_deep(DevScene s, const float* __restrict__ rays, int64_t n, int64_t rs, DevHits h, uint32_t flags,
                   const uint32_t* __restrict__ queue, uint32_t* __restrict__ counters,
-                  uint32_t* __restrict__ coop_queue) {
+                  uint32_t* __restrict__ coop_queue, uint32_t wide_min, int32_t budget) {
     __shared__ int32_t stk_ref[STACK][TRACE_BLOCK];
     __shared__ float stk_near[STACK][TRACE_BLOCK];
     const int tid = threadIdx.x;
     const uint32_t count = counters[0];
-    if (WIDE != (count >= DEEP_WIDE_MIN)) return;  // the other build walks this queue
+    if (WIDE != (count >= wide_min)) return;  // the other build walks this queue
     // Queue entry k goes to wave k % n_waves, lane k / n_waves: a short queue
     // puts one ray in each wave, so rays that take different paths through
     // the split trees run side by side instead of serializing in one wave.
@@ -179,8 +179,9 @@ void k_trace_deep(DevScene s, const float* __restrict__ rays, int64_t n, int64_t
     const uint32_t lane = (uint32_t)(tid & 63);
     for (uint64_t k = (uint64_t)wave + (uint64_t)lane * n_waves; k < count; k += 64ull * n_waves) {
         const uint32_t i = queue[k];
+        if ((int64_t)i >= n) continue;  // entries are ray indices of this call
         // a ray that runs out of node budget goes to the third pass
-        if (trace_one<WORLD, STACK, D_SERIAL>(s, rays, rs, (int64_t)i, h, flags, stk_ref, stk_near, tid, DEEP_BUDGET))
+        if (trace_one<WORLD, STACK, D_SERIAL>(s, rays, rs, (int64_t)i, h, flags, stk_ref, stk_near, tid, budget))
             coop_queue[atomicAdd(&counters[1], 1u)] = i;
     }
 }
@@ -200,9 +201,12 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_coop(DevScene s, const fl
     const uint32_t count = counters[1];
     const uint32_t n_waves = gridDim.x * (TRACE_BLOCK / 64);
     const uint32_t wib = (uint32_t)(tid >> 6);
-    for (uint32_t k = blockIdx.x * (TRACE_BLOCK / 64) + wib; k < count; k += n_waves)
-        (void)trace_one<WORLD, STACK, D_COOP>(s, rays, rs, (int64_t)coop_queue[k], h, flags, stk_ref, stk_near, tid,
-                                              INT32_MAX, nodes[wib]);
+    for (uint32_t k = blockIdx.x * (TRACE_BLOCK / 64) + wib; k < count; k += n_waves) {
+        const uint32_t i = coop_queue[k];  // wave-uniform
+        if ((int64_t)i >= n) continue;
+        (void)trace_one<WORLD, STACK, D_COOP>(s, rays, rs, (int64_t)i, h, flags, stk_ref, stk_near, tid, INT32_MAX,
+                                              nodes[wib]);
+    }
 }
 
 __global__ void k_zero_counters(uint32_t* p) {
@@ -650,15 +654,17 @@ int launch_build_dice(const float* odata, const DevObject* objs, const int32_t* 
 
 template <bool W, int S>
 static void launch_deep(const DevScene& s, const float* rays, int64_t n, int64_t rs, const DevHits& h, uint32_t flags,
-                        const uint32_t* queue, uint32_t* counters, hipStream_t st) {
+                        const uint32_t* queue, uint32_t* counters, const TraceTuning& tune, hipStream_t st) {
     const unsigned blocks = (unsigned)std::min<int64_t>(grid_for(n, TRACE_BLOCK), DEEP_ROUND);
     uint32_t* coop_queue = const_cast<uint32_t*>(queue) + n;
-    hipLaunchKernelGGL((k_trace_deep<W, S, false>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, s, rays, n, rs, h, flags,
-                       queue, counters, coop_queue);
-    if (n >= (int64_t)DEEP_WIDE_MIN) {
+    const int32_t budget = std::max<int32_t>(1, tune.deep_budget);
+    if (tune.wide_min > 0)
+        hipLaunchKernelGGL((k_trace_deep<W, S, false>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, s, rays, n, rs, h,
+                           flags, queue, counters, coop_queue, tune.wide_min, budget);
+    if (n >= (int64_t)tune.wide_min) {
         const unsigned wide = (unsigned)std::min<int64_t>(grid_for(n, TRACE_BLOCK), DEEP_ROUND * DEEP_WIDE_WAVES);
         hipLaunchKernelGGL((k_trace_deep<W, S, true>), dim3(wide), dim3(TRACE_BLOCK), 0, st, s, rays, n, rs, h,
-                           flags, queue, counters, coop_queue);
+                           flags, queue, counters, coop_queue, tune.wide_min, budget);
     }
     hipLaunchKernelGGL((k_trace_coop<W, S>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, s, rays, n, rs, h, flags,
                        (const uint32_t*)counters, (const uint32_t*)coop_queue);
@@ -697,9 +703,9 @@ static int launch_trace_w(const DevScene& s, int stack_need, const float* rays, 
                                s, rays, n, rs, h, flags, queue, counters, tune.n_dev);
     }
     if (tune.ev[1]) (void)hipEventRecord((hipEvent_t)tune.ev[1], st);
-    if (stack_need <= 16) launch_deep<W, 16>(s, rays, n, rs, h, flags, queue, counters, st);
-    else if (stack_need <= 32) launch_deep<W, 32>(s, rays, n, rs, h, flags, queue, counters, st);
-    else if (stack_need <= 64) launch_deep<W, 64>(s, rays, n, rs, h, flags, queue, counters, st);
+    if (stack_need <= 16) launch_deep<W, 16>(s, rays, n, rs, h, flags, queue, counters, tune, st);
+    else if (stack_need <= 32) launch_deep<W, 32>(s, rays, n, rs, h, flags, queue, counters, tune, st);
+    else if (stack_need <= 64) launch_deep<W, 64>(s, rays, n, rs, h, flags, queue, counters, tune, st);
     else return PSY_ERR_UNSUPPORTED;
     if (tune.ev[2]) (void)hipEventRecord((hipEvent_t)tune.ev[2], st);
     return check_launch();

@@ -633,16 +633,8 @@ int run_stream(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, c
     return PSY_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-int psy_render(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, float* film_xyz,
-               float* sample_xyz, int64_t* rays_traced, void* stream) {
-    if (int rc = check_render(ctx, s, prm, "psy_render")) return rc;
-    if (rays_traced) *rays_traced = 0;
-    HIP_OK(hipSetDevice(ctx->device));
-    hipStream_t st = (hipStream_t)stream;
+int render_forms(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, float* film_xyz, float* sample_xyz,
+                 int64_t* rays_traced, hipStream_t st) {
     RenderSetup U;
     plan_render(ctx, prm, U);
     const RenderLaunch R = make_launch(ctx, s, prm);
@@ -660,6 +652,22 @@ int psy_render(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, f
     if (int rc = alloc_render(ctx, sample_xyz == nullptr, 0, U)) return rc;
     if (ctx->render_mode == PSY_RENDER_FUSED) return run_fused(ctx, s, prm, R, U, film_xyz, sample_xyz, rays_traced, st);
     return run_wavefront(ctx, s, prm, R, U, film_xyz, sample_xyz, nullptr, rays_traced, st);
+}
+
+}  // namespace
+
+extern "C" {
+
+int psy_render(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, float* film_xyz,
+               float* sample_xyz, int64_t* rays_traced, void* stream) {
+    if (int rc = check_render(ctx, s, prm, "psy_render")) return rc;
+    if (rays_traced) *rays_traced = 0;
+    HIP_OK(hipSetDevice(ctx->device));
+    hipStream_t st = (hipStream_t)stream;
+    if (int rc = scratch_order(ctx, st)) return rc;
+    const int rc = render_forms(ctx, s, prm, film_xyz, sample_xyz, rays_traced, st);
+    const int rr = scratch_release(ctx, st);
+    return rc ? rc : rr;
 }
 
 // The render autotuner: coordinate descent over the render form and its
@@ -781,6 +789,7 @@ int psy_adjoint(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, 
     J.q = U.extra;
     J.grad = J.q + f_q;
     J.partials = J.grad + f_grad;
+    if (int rc = scratch_order(ctx, st)) return rc;
     int rc = launch_fill_f32(J.grad, f_grad + f_part, 0.0f, st);
     if (rc) return fail(rc, "psy_adjoint: launch failed");
     // the replay takes the flat grey spectrum: paths do not depend on
@@ -789,7 +798,7 @@ int psy_adjoint(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, 
     R.spec = DevSpectrum{};
     if ((rc = run_wavefront(ctx, s, prm, R, U, nullptr, nullptr, &J, rays_traced, st))) return rc;
     if ((rc = launch_adjoint_reduce(J, d_albedo, d_emission, st))) return fail(rc, "psy_adjoint: launch failed");
-    return PSY_OK;
+    return scratch_release(ctx, st);
 }
 
 }  // extern "C"

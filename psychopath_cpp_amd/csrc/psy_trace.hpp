@@ -159,7 +159,6 @@ struct DeepState {
     int32_t budget;     // D_SERIAL: node expansions left
     uint64_t* coop;     // D_COOP: the wave's node stack in LDS
 };
-constexpr int32_t DEEP_BUDGET = 512;   // expansions per ray in the second pass
 constexpr int COOP_CAP = 2048;         // node stack entries per wave in the third pass
 
 // Split-dice below the cached depth, over the subtree of a node that is
@@ -731,8 +730,9 @@ constexpr int TRACE_BLOCK = PSY_TRACE_BLOCK;  // threads per block of the trace 
 // one runs 2 waves per SIMD (128 VGPRs, paid for with spills): 1.4x the
 // throughput on long queues (a ground plane diced below the cache hands on
 // half the camera rays).  Each launches one resident round of 64-thread
-// blocks over 256 CUs, as does the third pass (1 wave per SIMD).
-constexpr uint32_t DEEP_WIDE_MIN = 4096;  // queue length from which the wide build walks it
+// blocks over 256 CUs, as does the third pass (1 wave per SIMD).  The
+// queue length that picks the wide build (DEEP_WIDE_MIN by default) and
+// the second pass's node budget (DEEP_BUDGET) are launch parameters.
 constexpr int DEEP_WIDE_WAVES = 2;
 constexpr int DEEP_ROUND = 1024;          // blocks per wave per SIMD over 256 CUs
 

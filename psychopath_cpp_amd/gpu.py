@@ -50,6 +50,7 @@ def lib():
     L.psy_halton.argtypes = [vp, u32, vp, i64, vp, vp]
     L.psy_ctx_set_trace_timing.argtypes = [vp, C.c_int]
     L.psy_ctx_set_trace_stack.argtypes = [vp, C.c_int]
+    L.psy_ctx_set_second_pass.argtypes = [vp, u32, i32]
     L.psy_ctx_set_stream_steps.argtypes = [vp, C.c_int]
     L.psy_render_autotune.argtypes = [vp, vp, C.POINTER(abi.RenderParams), i64, C.POINTER(abi.RenderTuning)]
     L.psy_ctx_trace_timing.argtypes = [vp, vp, vp, vp]
@@ -118,6 +119,12 @@ class Context:
     def set_trace_stack(self, entries):
         """psy_ctx_set_trace_stack: main-pass LDS stack entries (16 or 32)."""
         check(lib().psy_ctx_set_trace_stack(self.handle, int(entries)), "psy_ctx_set_trace_stack")
+
+    def set_second_pass(self, wide_min=4096, budget=512):
+        """psy_ctx_set_second_pass: queue length from which the second pass's wide build walks the queue
+        (0: always wide, 0xffffffff: always narrow) and its node budget per ray before the third pass."""
+        check(lib().psy_ctx_set_second_pass(self.handle, int(wide_min) & 0xFFFFFFFF, int(budget)),
+              "psy_ctx_set_second_pass")
 
     def set_trace_timing(self, enable=True):
         """psy_ctx_set_trace_timing: HIP events around each trace call's passes."""

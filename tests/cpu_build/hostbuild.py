@@ -48,6 +48,7 @@ def lib():
     L.hb_render_params.argtypes = [C.POINTER(abi.SceneDesc), C.c_int, C.POINTER(abi.RenderParams), vp, C.c_int, vp,
                                    C.c_int, vp, vp, vp, C.POINTER(i64), C.c_int]
     L.hb_set_spectrum_grid.argtypes = [vp, C.c_int, C.c_int, vp, C.c_int, C.c_int, C.c_float, C.c_float, vp]
+    L.hb_scene_stack_need.argtypes = [C.POINTER(abi.SceneDesc), C.c_int, C.POINTER(C.c_int)]
     _lib = L
     return L
 
@@ -72,6 +73,14 @@ def trace(scene_py, rays, world=True, dice_depth=10, occlusion_only=False, threa
     if rc != 0:
         raise RuntimeError(f"hb_trace failed with status {rc}")
     return {"t": t, "inst": inst, "u": u, "v": v, "aux": aux, "retraced": rt.value}
+
+
+def scene_stack_need(scene_py, dice_depth=0):
+    """(status, traversal stack need) of psy_scene_create's host preparation."""
+    desc = scene_py.to_desc()
+    need = C.c_int()
+    rc = lib().hb_scene_stack_need(desc.ptr(), dice_depth, C.byref(need))
+    return rc, need.value
 
 
 def render(scene_py, width, height, spp, seed=1, path_length=4, dice_depth=10, threads=8):

@@ -109,6 +109,15 @@ void parallel_for(int64_t n, int threads, F f) {
 
 extern "C" {
 
+// psy_scene_create's host preparation alone: its status and the instance
+// BVH's traversal stack need.
+int hb_scene_stack_need(const psy_scene_desc* d, int dice_depth, int* need) {
+    HostScene H;
+    const int rc = prepare_scene(d, dice_depth, H);
+    *need = rc ? 0 : H.stack_need;
+    return rc;
+}
+
 // psy_trace_world / psy_trace_rays on the CPU.  Returns a psy_status;
 // *retraced = rays that took the second pass.
 int hb_trace(const psy_scene_desc* d, int dice_depth, int world, const float* rays, int64_t n, uint32_t flags,

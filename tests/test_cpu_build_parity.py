@@ -225,3 +225,36 @@ def test_cpu_build_rejects_oversized_dice_cache():
             hostbuild.trace(sc, planes, world=True, dice_depth=depth)
     hb = hostbuild.trace(sc, planes, world=True, dice_depth=8)
     assert (hb["inst"] != 0xFFFFFFFF).all()
+
+
+def _geometric_row_scene(n=1000):
+    """Spheres along x at geometrically growing spacing: a binned SAH build
+    peels them off one at a time, so an unbounded build would nest as deep
+    as there are spheres."""
+    from psychopath_cpp_amd.scene import Assembly, Camera, Scene, Sphere, look_at, translate
+    root = Assembly()
+    root.add_object("s", Sphere((0.0, 0.0, 0.0), 0.4))
+    x = 0.0
+    for k in range(n):
+        root.create_object_instance("s", [translate(x, 0.0, 0.0)], "grey")
+        x += 1.0 * (1.07 ** k)
+    cam = Camera([look_at((-5.0, 3.0, 1.0), (40.0, 0.0, 0.0))], [0.9])
+    sc = Scene(cam, root)
+    sc.add_surface_shader("grey", 1, (0.5, 0.5, 0.5))
+    return sc
+
+
+def test_cpu_build_skewed_layout_bvh_depth():
+    """A skewed instance layout still builds a BVH whose traversal stack
+    fits the trace kernels (at most 64 entries), and traces like the
+    oracle's exhaustive instance loop."""
+    sc = _geometric_row_scene()
+    rc, need = hostbuild.scene_stack_need(sc)
+    assert rc == 0 and 1 <= need <= 64, (rc, need)
+    o = ora.OracleScene(sc)
+    w, _ = o.camera_rays(96, 32, 1, 1)
+    hb = hostbuild.trace(sc, ora.planes_from_wrays(w), world=True, dice_depth=0)
+    inter = o.trace(w, threads=8, chunk=4096, brute=True)
+    agree, t_ok, hits = compare(hb, inter)
+    assert hits > 50
+    assert agree >= 0.9999 and t_ok >= 0.9999

@@ -400,6 +400,72 @@ def test_gpu_long_second_pass_queue(oracle):
         say("long queue sample: ids agree", same_id.mean(), "t within 1e-5", t_ok.mean())
         assert hit_o.mean() > 0.3
         assert same_id.mean() >= 0.9999 and t_ok.mean() >= 0.9999
+        # the same queue through the narrow build alone and the wide build
+        # alone: the same hits bit for bit; with a node budget of 1 and 4
+        # expansions most second-pass rays finish in the third,
+        # wave-cooperative pass, whose lanes merge hits in another order
+        # (exact ties may resolve differently): the same closest hits
+        t_ref = hits.t.cpu().numpy()
+        for wide_min, budget in ((0xFFFFFFFF, 512), (0, 512), (4096, 1), (0, 4)):
+            ctx.set_second_pass(wide_min, budget)
+            again = gpu.trace_world(ctx, g, w)
+            s2, s3 = ctx.pass_counts()
+            say(f"second pass wide_min {wide_min} budget {budget}: second {s2}, third {s3}")
+            assert s2 == second
+            if budget == 512:
+                assert _hits_equal(again, hits), (wide_min, budget)
+                continue
+            if budget == 1:
+                assert s3 > 0
+            ia = again.inst.cpu().numpy().view(np.uint32)
+            ta = again.t.cpu().numpy()
+            diff_i = ia != inst
+            hit = inst != 0xFFFFFFFF
+            rel = np.abs(ta[hit] - t_ref[hit]) / np.abs(t_ref[hit])
+            say(f"  vs default: {int(diff_i.sum())} ids differ, {int((rel > 1e-5).sum())} t beyond 1e-5, "
+                f"max rel {float(rel.max()) if rel.size else 0.0:.3g}, "
+                f"{int((~np.isclose(again.u.cpu().numpy(), hits.u.cpu().numpy(), rtol=0, atol=0)).sum())} u differ")
+            ok = ~diff_i
+            ok[hit] &= rel <= 1e-5
+            assert ok.mean() >= 0.9999, (wide_min, budget, ok.mean())
+        ctx.set_second_pass()
+        g.close()
+    finally:
+        ctx.close()
+
+
+def test_gpu_trace_calls_on_two_streams(oracle):
+    """Two trace calls of one context queued on two torch streams without a
+    host synchronization between them share the context's second-pass
+    queue; the library orders them, so each gives the hits it gives alone.
+    The rays need the second pass (the sphere scene's ground dices below
+    its cache), whose queue a racing call would overwrite."""
+    sc = configs.sphere_scene()
+    ctx = gpu.Context(0)
+    try:
+        g = gpu.GpuScene(ctx, sc)
+        wa, _ = gpu.camera_rays(ctx, g, 512, 512, 1, 5)
+        wb, _ = gpu.camera_rays(ctx, g, 384, 384, 2, 6)
+        ra = gpu.trace_world(ctx, g, wa)
+        na = ctx.pass_counts()[0]
+        rb = gpu.trace_world(ctx, g, wb)
+        nb = ctx.pass_counts()[0]
+        say("second-pass rays alone:", na, nb)
+        assert na > 1000 and nb > 1000
+        s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+        for rep in range(3):
+            ha = gpu.HitBuffers(wa.shape[1], wa.device)
+            hb = gpu.HitBuffers(wb.shape[1], wb.device)
+            torch.cuda.synchronize()
+            with torch.cuda.stream(s1):
+                gpu.trace_world(ctx, g, wa, hits=ha)
+            with torch.cuda.stream(s2):
+                gpu.trace_world(ctx, g, wb, hits=hb)
+            with torch.cuda.stream(s1):
+                gpu.trace_world(ctx, g, wa, hits=ha)
+            torch.cuda.synchronize()
+            assert _hits_equal(ha, ra), rep
+            assert _hits_equal(hb, rb), rep
         g.close()
     finally:
         ctx.close()

@@ -298,6 +298,7 @@ int psy_abi_version(void);
  * within 2 GiB of cache, else up to 18 within 16 GiB. */
 int psy_scene_create(psy_ctx* ctx, const psy_scene_desc* desc, int dice_depth,
                      psy_scene** out);
+/* A scene may be destroyed before or after the context it was created on. */
 int psy_scene_destroy(psy_scene* scene);
 int psy_scene_get_info(const psy_scene* scene, psy_scene_info* info);
 /* InstanceID (basics/instance_id.hpp:9-40) of every flattened leaf, as the

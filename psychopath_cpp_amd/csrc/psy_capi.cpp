@@ -342,7 +342,7 @@ int psy_copy_to_host(psy_ctx* ctx, void* dst, const void* src, size_t bytes) {
 
 int psy_ctx_destroy(psy_ctx* ctx) {
     if (!ctx) return PSY_OK;
-    (void)hipSetDevice(ctx->device);
+    if (hipSetDevice(ctx->device) != hipSuccess) (void)hipGetLastError();
     delete ctx;
     return PSY_OK;
 }
@@ -942,6 +942,7 @@ int psy_scene_create(psy_ctx* ctx, const psy_scene_desc* d, int dice_depth, psy_
     HIP_OK(hipSetDevice(ctx->device));
     std::unique_ptr<psy_scene> sc(new psy_scene());
     sc->ctx = ctx;
+    sc->device = ctx->device;
     HostScene H;
     if (int rc = prepare_scene(d, dice_depth, H)) return rc;
     sc->stack_need = H.stack_need;
@@ -1042,7 +1043,7 @@ int psy_scene_set_params(psy_scene* s, const float* albedo, int64_t n_albedo, co
         for (int64_t j = 0; j < n_emission; ++j)
             if (!(emission_scale[j] >= 0.0f && emission_scale[j] < 3.402823466e+38f))
                 return fail(PSY_ERR_INVALID, "psy_scene_set_params: emission scales must be finite and >= 0");
-    HIP_OK(hipSetDevice(s->ctx->device));
+    HIP_OK(hipSetDevice(s->device));
     if (albedo) {
         for (int64_t i = 0; i < n_albedo; ++i)
             if (s->host_shaders[i].kind == PSY_SHADER_LAMBERT)
@@ -1062,7 +1063,8 @@ int psy_scene_set_params(psy_scene* s, const float* albedo, int64_t n_albedo, co
 
 int psy_scene_destroy(psy_scene* s) {
     if (!s) return PSY_OK;
-    (void)hipSetDevice(s->ctx->device);
+    // a failure here must not linger as the thread's last HIP error
+    if (hipSetDevice(s->device) != hipSuccess) (void)hipGetLastError();
     delete s;
     return PSY_OK;
 }

@@ -141,6 +141,7 @@ struct psy_ctx {
 
 struct psy_scene {
     psy_ctx* ctx = nullptr;
+    int device = 0;  // the context's device (the scene may outlive its context)
     psy::DevScene dev{};
     psy::DevCamera cam{};
     psy::DBuf bvh, bvh1, inst, obj, odata, xf, dice, shaders;

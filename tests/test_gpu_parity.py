@@ -442,10 +442,11 @@ def test_gpu_trace_calls_on_two_streams(oracle):
     its cache), whose queue a racing call would overwrite."""
     sc = configs.sphere_scene()
     ctx = gpu.Context(0)
+    g = None
     try:
         g = gpu.GpuScene(ctx, sc)
-        wa, _ = gpu.camera_rays(ctx, g, 512, 512, 1, 5)
-        wb, _ = gpu.camera_rays(ctx, g, 384, 384, 2, 6)
+        wa, _ = gpu.camera_rays(ctx, g, 1024, 768, 1, 5)
+        wb, _ = gpu.camera_rays(ctx, g, 800, 600, 1, 6)
         ra = gpu.trace_world(ctx, g, wa)
         na = ctx.pass_counts()[0]
         rb = gpu.trace_world(ctx, g, wb)
@@ -466,8 +467,9 @@ def test_gpu_trace_calls_on_two_streams(oracle):
             torch.cuda.synchronize()
             assert _hits_equal(ha, ra), rep
             assert _hits_equal(hb, rb), rep
-        g.close()
     finally:
+        if g is not None:
+            g.close()
         ctx.close()
 
 

@@ -194,6 +194,8 @@ int psy_ctx_create(int device, psy_ctx** out) {
         trace_persist_occupancy(true, 32, &c->occupancy32[1]) || render_persist_occupancy(&c->render_occupancy))
         return fail(PSY_ERR_HIP, "psy_ctx_create: occupancy query failed");
     HIP_OK(hipEventCreateWithFlags(&c->scratch_done, hipEventDisableTiming));
+    if (static_scratch_address(device, &c->static_scratch))
+        return fail(PSY_ERR_HIP, "psy_ctx_create: device address of the static trace scratch not found");
     *out = c.release();
     return PSY_OK;
 }
@@ -1123,6 +1125,29 @@ int psy_trace_rays(psy_ctx* ctx, const psy_scene* s, const float* rays, int64_t 
                           to_dev(hits), (uint32_t*)ctx->queue.p, (uint32_t*)ctx->counters.p, ctx->tuning(false), st);
     if (rc) return fail(rc, "psy_trace_rays: launch failed");
     return scratch_release(ctx, st);
+}
+
+int psy_trace_rays_static(psy_ctx* ctx, const psy_scene* s, const float* rays, int64_t n, const psy_hits* hits,
+                          uint32_t flags, void* stream) {
+    if (!ctx || !s || (!rays && n > 0)) return fail(PSY_ERR_INVALID, "psy_trace_rays_static: NULL argument");
+    if (n < 0) return fail(PSY_ERR_INVALID, "psy_trace_rays_static: negative ray count");
+    if (n > (int64_t)PSY_STATIC_MAX_RAYS)
+        return fail(PSY_ERR_INVALID, "psy_trace_rays_static: " + std::to_string(n) + " rays; at most " +
+                                         std::to_string((int64_t)PSY_STATIC_MAX_RAYS) +
+                                         " (PSY_STATIC_MAX_RAYS) per call");
+    if (s->transformed_patches)
+        return fail(PSY_ERR_UNSUPPORTED, "psy_trace_rays_static: scene has transformed patch instances; their footprint needs world rays (psy_trace_world)");
+    if (int rc = check_hits(hits, n)) return rc;
+    if (!ctx->static_scratch) return fail(PSY_ERR_HIP, "psy_trace_rays_static: no static scratch on this device");
+    HIP_OK(hipSetDevice(ctx->device));
+    // queues of up to 2 * PSY_STATIC_MAX_RAYS entries (second pass, then the
+    // third pass's at queue + n), then the counters, which the trace's first
+    // kernel clears
+    uint32_t* queue = ctx->static_scratch;
+    uint32_t* counters = ctx->static_scratch + 2 * (int64_t)PSY_STATIC_MAX_RAYS;
+    int rc = launch_trace(s->dev, false, (flags & PSY_TRACE_OCCLUSION_ONLY) != 0, s->stack_need, rays, n, n,
+                          to_dev(hits), queue, counters, ctx->tuning(false, false), (hipStream_t)stream);
+    return rc ? fail(rc, "psy_trace_rays_static: launch failed") : PSY_OK;
 }
 
 int psy_world_to_rays(psy_ctx* ctx, const float* wrays, int64_t n, float* rays, void* stream) {

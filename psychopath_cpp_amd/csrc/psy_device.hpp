@@ -168,6 +168,10 @@ int launch_trace(const DevScene& s, bool world, bool occl_only, int stack_need, 
                  int64_t n, int64_t rs, const DevHits& h, uint32_t* queue, uint32_t* counters,
                  const TraceTuning& tune, hipStream_t st);
 int trace_persist_occupancy(bool world, int stack, int* blocks_per_cu);
+// Device address of the allocation-free trace's scratch array on `device`,
+// which must be the current device (looked up on the first call per device).
+// Layout: queues of 2 * PSY_STATIC_MAX_RAYS entries, then the counters.
+int static_scratch_address(int device, uint32_t** out);
 int launch_world_to_rays(const float* wrays, int64_t n, float* rays, hipStream_t st);
 
 // ---- integrator (psy_render_dev.hpp) ----------------------------------------

@@ -81,6 +81,8 @@ struct psy_ctx {
     hipEvent_t scratch_done = nullptr;
     hipStream_t scratch_stream = nullptr;
     bool scratch_recorded = false;
+    // the allocation-free trace's scratch on this device (static_scratch_address)
+    uint32_t* static_scratch = nullptr;
     // second pass: queue length from which the wide build walks, node budget
     uint32_t deep_wide_min = psy::DEEP_WIDE_MIN;
     int32_t deep_budget = psy::DEEP_BUDGET;

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "psy_trace.hpp"
 #include "psy_render_dev.hpp"
@@ -931,6 +932,36 @@ int launch_render_resolve(const float* sample_xyz, int64_t sbase, int64_t first_
     hipLaunchKernelGGL(k_render_resolve, dim3((unsigned)((n_pixels + RENDER_BLOCK - 1) / RENDER_BLOCK)),
                        dim3(RENDER_BLOCK), 0, st, sample_xyz, sbase, first_pixel, n_pixels, spp, film);
     return hipGetLastError() == hipSuccess ? 0 : PSY_ERR_HIP;
+}
+
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free trace (psy_trace_rays_static): the second-
+// and third-pass queues of up to PSY_STATIC_MAX_RAYS rays, then the pass
+// counters, in one file-scope device array.  Each GPU the library runs on
+// holds its own copy (32 MiB + 256 B) until the process ends.  In host code
+// `scratch` names a host-side stand-in, never the device memory: the device
+// address comes from hipGetSymbolAddress, once per device, below.
+// ---------------------------------------------------------------------------
+constexpr int64_t STATIC_COUNTER_WORDS = 64;
+static __device__ __align__(256) uint32_t scratch[2 * (int64_t)PSY_STATIC_MAX_RAYS + STATIC_COUNTER_WORDS];
+
+namespace {
+constexpr int STATIC_MAX_DEVICES = 64;
+std::mutex g_static_mu;
+uint32_t* g_static_ptr[STATIC_MAX_DEVICES] = {};
+}  // namespace
+
+int static_scratch_address(int device, uint32_t** out) {
+    *out = nullptr;
+    if (device < 0 || device >= STATIC_MAX_DEVICES) return PSY_ERR_UNSUPPORTED;
+    std::lock_guard<std::mutex> lock(g_static_mu);
+    if (!g_static_ptr[device]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return PSY_ERR_HIP;
+        g_static_ptr[device] = static_cast<uint32_t*>(p);
+    }
+    *out = g_static_ptr[device];
+    return 0;
 }
 
 }  // namespace psy

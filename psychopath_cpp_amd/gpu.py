@@ -46,6 +46,7 @@ def lib():
     L.psy_scene_instance_ids.argtypes = [vp, vp, i64]
     L.psy_trace_world.argtypes = [vp, vp, vp, i64, C.POINTER(abi.Hits), u32, vp]
     L.psy_trace_rays.argtypes = [vp, vp, vp, i64, C.POINTER(abi.Hits), u32, vp]
+    L.psy_trace_rays_static.argtypes = [vp, vp, vp, i64, C.POINTER(abi.Hits), u32, vp]
     L.psy_world_to_rays.argtypes = [vp, vp, i64, vp, vp]
     L.psy_halton.argtypes = [vp, u32, vp, i64, vp, vp]
     L.psy_ctx_set_trace_timing.argtypes = [vp, C.c_int]
@@ -265,6 +266,19 @@ def trace_rays(ctx, scene, rays, occlusion_only=False, hits=None):
     flags = abi.PSY_TRACE_OCCLUSION_ONLY if occlusion_only else 0
     check(lib().psy_trace_rays(ctx.handle, scene.handle, _ptr(rays), n, C.byref(hs), flags, _stream()),
           "psy_trace_rays")
+    return hits
+
+
+def trace_rays_static(ctx, scene, rays, occlusion_only=False, hits=None):
+    """psy_trace_rays_static: trace_rays through the library's fixed scratch (no device allocation in the
+    call); at most abi.PSY_STATIC_MAX_RAYS rays."""
+    _check_planes(rays, abi.PSY_RAY_PLANES, "rays")
+    n = rays.shape[1]
+    hits = hits or HitBuffers(n, rays.device)
+    hs = hits.struct()
+    flags = abi.PSY_TRACE_OCCLUSION_ONLY if occlusion_only else 0
+    check(lib().psy_trace_rays_static(ctx.handle, scene.handle, _ptr(rays), n, C.byref(hs), flags, _stream()),
+          "psy_trace_rays_static")
     return hits
 
 

@@ -92,6 +92,37 @@ def lib():
         L.ora_spectrum_xyz_to_p.restype = C.c_float
         L.ora_spectrum_xyz_to_p.argtypes = [C.c_float, vp]
         L.ora_color_to_spectral_sample.argtypes = [vp, C.c_float, vp]
+        f = C.c_float
+        L.ora_kat_sphere_t.restype = C.c_int
+        L.ora_kat_sphere_t.argtypes = [vp, vp, vp, f, f, vp]
+        L.ora_kat_closest_ray_t2.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.ora_kat_min_width.restype = f
+        L.ora_kat_min_width.argtypes = [vp, f, f]
+        L.ora_kat_patch_split.argtypes = [C.c_int, vp, C.c_int, vp, vp]
+        L.ora_kat_patch_dg.argtypes = [C.c_int, vp, f, f, vp, vp, vp, vp, vp]
+        L.ora_kat_instance_id.restype = C.c_uint64
+        L.ora_kat_instance_id.argtypes = [vp, vp, C.c_int]
+        L.ora_kat_sphere_light_pdf.restype = f
+        L.ora_kat_sphere_light_pdf.argtypes = [vp, f, vp]
+        L.ora_kat_rect_light_pdf.restype = f
+        L.ora_kat_rect_light_pdf.argtypes = [f, f, vp]
+        L.ora_kat_sphere_light_sample.argtypes = [vp, f, vp, f, f, vp, vp]
+        L.ora_kat_rect_light_sample.argtypes = [f, f, vp, f, f, vp, vp]
+        L.ora_kat_faster_ln.restype = f
+        L.ora_kat_faster_ln.argtypes = [f]
+        L.ora_kat_fast_logit.restype = f
+        L.ora_kat_fast_logit.argtypes = [f, f]
+        L.ora_kat_square_to_circle.argtypes = [vp, vp]
+        L.ora_kat_cie_1931.argtypes = [f, vp]
+        L.ora_kat_sphere_lambert.restype = f
+        L.ora_kat_sphere_lambert.argtypes = [f, f]
+        L.ora_kat_lambert.argtypes = [f, vp, vp, vp, vp, vp]
+        L.ora_kat_light_select.restype = C.c_int
+        L.ora_kat_light_select.argtypes = [vp, vp, vp, vp, f, f, f, f, vp, vp, vp, vp]
+        L.ora_kat_total_lights.restype = i64
+        L.ora_kat_total_lights.argtypes = [vp]
+        L.ora_bounce_dir.argtypes = [vp, vp, f, f, vp, vp, vp, vp]
+        L.ora_camera_ray_from_dims.argtypes = [vp, f, f, f, f, f, f, f, vp]
         _lib = L
     return _lib
 

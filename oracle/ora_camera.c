@@ -219,3 +219,8 @@ void ora_bounce_rays(const ora_scene* s, const ora_wray* in, const ora_inter* hi
         w->type = 2;  /* WorldRay::R_DIFFUSE */
     }
 }
+
+/* ---- entry points of the oracle's own checks (tests/test_oracle_pins.py) */
+float ora_kat_faster_ln(float x) { return faster_ln(x); }
+float ora_kat_fast_logit(float p, float width) { return fast_logit(p, width); }
+void ora_kat_square_to_circle(float* x, float* y) { square_to_circle(x, y); }

@@ -764,3 +764,85 @@ int64_t ora_render(const ora_scene* s, int width, int height, int spp, uint32_t 
     if (g_chromatic_seen) return -1;
     return J.rays;
 }
+
+/* ---- entry points of the oracle's own checks (tests/test_oracle_pins.py) */
+void ora_kat_cie_1931(float lambda, float xyz[3]) {
+    xyz[0] = X_1931(lambda);
+    xyz[1] = Y_1931(lambda);
+    xyz[2] = Z_1931(lambda);
+}
+
+float ora_kat_sphere_lambert(float nlcos, float rcos) { return sphere_lambert(nlcos, rcos); }
+
+/* LambertClosure eval (flat spectrum of a grey albedo, first wavelength)
+ * and sample_pdf for incoming direction `in`, outgoing `out`, normal n */
+void ora_kat_lambert(float albedo, const float in[3], const float out[3], const float n[3], float* eval,
+                     float* pdf) {
+    const float a[3] = {albedo, albedo, albedo};
+    float e[NSPEC];
+    lambert_eval(a, va(in), va(out), va(n), 500.0f, e);
+    *eval = e[0];
+    *pdf = lambert_pdf(va(in), va(out), va(n));
+}
+
+static ora_object kat_light(int kind, const float* data, int nf) {
+    ora_object ob;
+    memset(&ob, 0, sizeof ob);
+    ob.kind = kind;
+    ob.ts = 1;
+    ob.escale = 1.0f;
+    ob.data = (float*)data;
+    (void)nf;
+    return ob;
+}
+
+/* SphereLight::sample from point arr: the vector to the sampled point and
+ * the sample's pdf (grey colour 1) */
+void ora_kat_sphere_light_sample(const float c[3], float radius, const float arr[3], float u, float v,
+                                 float to_light[3], float* pdf) {
+    const float data[7] = {c[0], c[1], c[2], radius, 1.0f, 1.0f, 1.0f};
+    ora_object ob = kat_light(PSY_OBJ_SPHERE_LIGHT, data, 7);
+    lquery q;
+    memset(&q, 0, sizeof q);
+    q.u = u; q.v = v; q.wavelength = 500.0f;
+    sphere_light_sample(&ob, &q, va(arr), pdf);
+    set3(to_light, q.to_light);
+}
+
+/* RectangleLight::sample (the light in its own space, centred at the
+ * origin in the z = 0 plane) */
+void ora_kat_rect_light_sample(float w, float h, const float arr[3], float u, float v, float to_light[3],
+                               float* pdf) {
+    const float data[5] = {w, h, 1.0f, 1.0f, 1.0f};
+    ora_object ob = kat_light(PSY_OBJ_RECTANGLE_LIGHT, data, 5);
+    lquery q;
+    memset(&q, 0, sizeof q);
+    q.u = u; q.v = v; q.wavelength = 500.0f;
+    rect_light_sample(&ob, &q, va(arr), pdf);
+    set3(to_light, q.to_light);
+}
+
+/* LightTree::sample on the scene's root tree from a shading point: the
+ * selected light's InstanceID, the selection pdf (times the root's light
+ * count, as the reference keeps it), the light-sample pdf and the vector
+ * to the light.  Returns 0 when the scene has no light. */
+int ora_kat_light_select(const ora_scene* s, const float pos[3], const float nor[3], const float d[3], float time,
+                         float n, float u, float v, uint64_t* id, float* selection_pdf, float* light_sample_pdf,
+                         float to_light[3]) {
+    lquery q;
+    memset(&q, 0, sizeof q);
+    q.n = n; q.u = u; q.v = v;
+    q.pos = va(pos); q.nor = va(nor); q.d = va(d);
+    q.wavelength = 500.0f; q.time = time;
+    q.xform = mat4_identity();
+    q.selection_pdf = 1.0f;
+    q.light_sample_pdf = 1.0f;
+    const int ok = light_tree_sample(&s->asms[0], &q);
+    *id = q.id;
+    *selection_pdf = q.selection_pdf;
+    *light_sample_pdf = q.light_sample_pdf;
+    set3(to_light, q.to_light);
+    return ok;
+}
+
+int64_t ora_kat_total_lights(const ora_scene* s) { return (int64_t)s->asms[0].total_lights; }

@@ -1075,3 +1075,58 @@ int64_t ora_trace_brute(const ora_scene* s, const ora_wray* rays, int64_t n, ora
                         int threads, int64_t chunk) {
     return trace_impl(s, rays, n, out, threads, chunk, 1);
 }
+
+/* ---- entry points of the oracle's own checks (tests/test_oracle_pins.py):
+ * the functions above, one at a time, on plain arrays ------------------- */
+int ora_kat_sphere_t(const float o[3], const float d[3], const float c[3], float r, float max_t, float* t) {
+    ora_ray ray;
+    memset(&ray, 0, sizeof ray);
+    ray.o = va(o);
+    ray.d = va(d);
+    ray.max_t = max_t;
+    ray.d_inv = v3div(v3s(1.0f), ray.d);
+    return sphere_hit_t(&ray, va(c), r, t);
+}
+
+void ora_kat_closest_ray_t2(const float o1[3], const float d1[3], const float o2[3], const float d2[3],
+                            float* t, float* dist) {
+    closest_ray_t2(va(o1), va(d1), va(o2), va(d2), t, dist);
+}
+
+float ora_kat_min_width(const float w6[6], float tnear, float tfar) {
+    ora_ray ray;
+    memset(&ray, 0, sizeof ray);
+    ray.owx = w6[0]; ray.owy = w6[1]; ray.dwx = w6[2]; ray.dwy = w6[3]; ray.fwx = w6[4]; ray.fwy = w6[5];
+    return ora_ray_min_width(&ray, tnear, tfar);
+}
+
+/* kind PSY_OBJ_BILINEAR (4 points) or PSY_OBJ_BICUBIC (16); dir 0: split_u */
+void ora_kat_patch_split(int kind, const float* p, int dir, float* p1, float* p2) {
+    if (kind == PSY_OBJ_BILINEAR) {
+        if (dir == 0) bilinear_split_u(p, p1, p2); else bilinear_split_v(p, p1, p2);
+    } else {
+        if (dir == 0) bicubic_split_u(p, p1, p2); else bicubic_split_v(p, p1, p2);
+    }
+}
+
+void ora_kat_patch_dg(int kind, const float* p, float u, float v, float n[3], float dpdu[3], float dpdv[3],
+                      float dndu[3], float dndv[3]) {
+    v3 vn, vu, vv, nu, nv;
+    if (kind == PSY_OBJ_BILINEAR) bilinear_dg(p, u, v, &vn, &vu, &vv, &nu, &nv);
+    else bicubic_dg(p, u, v, &vn, &vu, &vv, &nu, &nv);
+    set3(n, vn); set3(dpdu, vu); set3(dpdv, vv); set3(dndu, nu); set3(dndv, nv);
+}
+
+/* InstanceID push_back sequence as the tracer builds element ids */
+uint64_t ora_kat_instance_id(const uint64_t* sub, const int* bits, int n) {
+    tracer tr;
+    memset(&tr, 0, sizeof tr);
+    for (int i = 0; i < n; ++i) id_push(&tr, sub[i], bits[i]);
+    return tr.element_id;
+}
+
+float ora_kat_sphere_light_pdf(const float pos[3], float radius, const float arr[3]) {
+    return ora_sphere_light_pdf(va(pos), (double)radius, va(arr));
+}
+
+float ora_kat_rect_light_pdf(float w, float h, const float arr[3]) { return ora_rect_light_pdf(w, h, va(arr)); }

@@ -126,6 +126,47 @@ float ora_spectrum_xyz_to_p(float lambda, const float* xyz);
 /* Color_to_SpectralSample (color/color.hpp:509-511) at hero's 4 wavelengths */
 void ora_color_to_spectral_sample(const float c[3], float hero, float e[4]);
 
+/* ---- single functions, for the oracle's own checks ------------------- */
+/* Sphere::intersect_ray's quadratic (object/sphere.cpp:67-140) */
+int ora_kat_sphere_t(const float o[3], const float d[3], const float c[3], float r, float max_t, float* t);
+/* closest_ray_t2 (utils/utils.hpp:443-456) */
+void ora_kat_closest_ray_t2(const float o1[3], const float d1[3], const float o2[3], const float d2[3],
+                            float* t, float* dist);
+/* Ray::min_width (basics/ray.hpp:132-153); w6 = owx owy dwx dwy fwx fwy */
+float ora_kat_min_width(const float w6[6], float tnear, float tfar);
+/* Bilinear / Bicubic split_u (dir 0) / split_v (object/bilinear.hpp:63-73,
+ * object/bicubic.hpp:74-110); kind PSY_OBJ_BILINEAR or PSY_OBJ_BICUBIC */
+void ora_kat_patch_split(int kind, const float* p, int dir, float* p1, float* p2);
+/* differential_geometry (object/bilinear.hpp:100-141, bicubic.hpp:140-189) */
+void ora_kat_patch_dg(int kind, const float* p, float u, float v, float n[3], float dpdu[3], float dpdv[3],
+                      float dndu[3], float dndv[3]);
+/* InstanceID::push_back sequence (basics/instance_id.hpp:18-23) */
+uint64_t ora_kat_instance_id(const uint64_t* sub, const int* bits, int n);
+/* SphereLight / RectangleLight sample_pdf and sample (lights/) */
+float ora_kat_sphere_light_pdf(const float pos[3], float radius, const float arr[3]);
+float ora_kat_rect_light_pdf(float w, float h, const float arr[3]);
+void ora_kat_sphere_light_sample(const float c[3], float radius, const float arr[3], float u, float v,
+                                 float to_light[3], float* pdf);
+void ora_kat_rect_light_sample(float w, float h, const float arr[3], float u, float v, float to_light[3],
+                               float* pdf);
+/* faster_ln, fast_logit (utils/utils.hpp:335-343, utils/monte_carlo.hpp:17-21),
+ * square_to_circle (utils/monte_carlo.hpp:29-53) */
+float ora_kat_faster_ln(float x);
+float ora_kat_fast_logit(float p, float width);
+void ora_kat_square_to_circle(float* x, float* y);
+/* X_1931 / Y_1931 / Z_1931 (color/color.hpp:150-165) */
+void ora_kat_cie_1931(float lambda, float xyz[3]);
+/* sphere_lambert (utils/monte_carlo.hpp:207-233) */
+float ora_kat_sphere_lambert(float nlcos, float rcos);
+/* LambertClosure evaluate / sample_pdf (shading/surface_closure.hpp:233-331) */
+void ora_kat_lambert(float albedo, const float in[3], const float out[3], const float n[3], float* eval,
+                     float* pdf);
+/* LightTree::sample on the root tree (accel/light_tree.cpp:30-96) */
+int ora_kat_light_select(const ora_scene* s, const float pos[3], const float nor[3], const float d[3], float time,
+                         float n, float u, float v, uint64_t* id, float* selection_pdf, float* light_sample_pdf,
+                         float to_light[3]);
+int64_t ora_kat_total_lights(const ora_scene* s);
+
 #ifdef __cplusplus
 }
 #endif

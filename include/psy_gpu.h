@@ -330,7 +330,7 @@ int psy_trace_rays(psy_ctx* ctx, const psy_scene* scene,
  * allocate device memory (code under HIP graph capture, tight serving
  * loops, harnesses that forbid hipMalloc).  Same arguments, kernels and
  * results as psy_trace_rays, but its second- and third-pass queues and
- * counters live in a fixed array of the library's own on each GPU (32 MiB
+ * counters live in a fixed array of the library's own on each GPU (48 MiB
  * + 256 B, held until the process ends) instead of context memory, and it
  * calls no hipMalloc, hipFree or other allocating routine.  At most
  * PSY_STATIC_MAX_RAYS rays per call (PSY_ERR_INVALID above).  Every call on
@@ -338,7 +338,7 @@ int psy_trace_rays(psy_ctx* ctx, const psy_scene* scene,
  * on one GPU race, and the caller must order them (one stream, or events).
  * It does not record pass timing, and psy_ctx_trace_pass_counts does not
  * report its calls. */
-#define PSY_STATIC_MAX_RAYS 4194304
+#define PSY_STATIC_MAX_RAYS 2097152
 int psy_trace_rays_static(psy_ctx* ctx, const psy_scene* scene,
                           const float* rays, int64_t n,
                           const psy_hits* hits, uint32_t flags, void* stream);

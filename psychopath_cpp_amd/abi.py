@@ -46,7 +46,7 @@ FLOATS_PER_KIND = {
 }
 
 MISS = 0xFFFFFFFF
-PSY_STATIC_MAX_RAYS = 4194304
+PSY_STATIC_MAX_RAYS = 2097152
 
 
 class ObjectDesc(C.Structure):

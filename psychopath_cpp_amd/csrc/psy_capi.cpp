@@ -126,15 +126,17 @@ namespace psy {
 // must not overlap in time (they share this scratch).
 int ensure_queue(psy_ctx* ctx, int64_t n) {
     if (!ctx->counters.p) {
-        int rc = ctx->counters.alloc(sizeof(uint32_t) * 4);
+        int rc = ctx->counters.alloc(sizeof(uint32_t) * 8);
         if (rc) return rc;
     }
     if (n <= ctx->queue_cap) return PSY_OK;
     HIP_OK(hipDeviceSynchronize());
     ctx->queue.release();
     const int64_t cap = std::max<int64_t>(n, 1 << 16);
-    // two queues of cap rays: the second pass's, then the third pass's
-    int rc = ctx->queue.alloc((size_t)(2 * cap) * sizeof(uint32_t));
+    // for up to cap rays: the second pass's queue, the third pass's queue,
+    // the second-pass rays' deep-node lists and the deep-node pool (3 words
+    // an entry)
+    int rc = ctx->queue.alloc((size_t)(6 * cap) * sizeof(uint32_t));
     if (rc) { ctx->queue_cap = 0; return rc; }
     ctx->queue_cap = cap;
     return PSY_OK;
@@ -1140,11 +1142,11 @@ int psy_trace_rays_static(psy_ctx* ctx, const psy_scene* s, const float* rays, i
     if (int rc = check_hits(hits, n)) return rc;
     if (!ctx->static_scratch) return fail(PSY_ERR_HIP, "psy_trace_rays_static: no static scratch on this device");
     HIP_OK(hipSetDevice(ctx->device));
-    // queues of up to 2 * PSY_STATIC_MAX_RAYS entries (second pass, then the
-    // third pass's at queue + n), then the counters, which the trace's first
-    // kernel clears
+    // 6 * PSY_STATIC_MAX_RAYS words for an n-ray call's queues, deep-node
+    // lists and pool (laid out from queue by 6 n words), then the counters,
+    // which the trace's first kernel clears
     uint32_t* queue = ctx->static_scratch;
-    uint32_t* counters = ctx->static_scratch + 2 * (int64_t)PSY_STATIC_MAX_RAYS;
+    uint32_t* counters = ctx->static_scratch + 6 * (int64_t)PSY_STATIC_MAX_RAYS;
     int rc = launch_trace(s->dev, false, (flags & PSY_TRACE_OCCLUSION_ONLY) != 0, s->stack_need, rays, n, n,
                           to_dev(hits), queue, counters, ctx->tuning(false, false), (hipStream_t)stream);
     return rc ? fail(rc, "psy_trace_rays_static: launch failed") : PSY_OK;

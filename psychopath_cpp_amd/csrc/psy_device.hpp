@@ -161,7 +161,8 @@ struct TraceTuning {
     // them: before the main pass, after it, after the second pass.
     void* ev[3] = {nullptr, nullptr, nullptr};
 };
-// Two stream-ordered passes; queue (n entries) and counters (4 x u32) are
+// Three stream-ordered passes; queue (6 n words: second- and third-pass
+// queues, second-pass deep-node lists and pool) and counters (8 x u32) are
 // scratch.
 // Ray plane p of ray i is at rays[p * rs + i] (rs >= n).
 int launch_trace(const DevScene& s, bool world, bool occl_only, int stack_need, const float* rays,
@@ -170,7 +171,8 @@ int launch_trace(const DevScene& s, bool world, bool occl_only, int stack_need, 
 int trace_persist_occupancy(bool world, int stack, int* blocks_per_cu);
 // Device address of the allocation-free trace's scratch array on `device`,
 // which must be the current device (looked up on the first call per device).
-// Layout: queues of 2 * PSY_STATIC_MAX_RAYS entries, then the counters.
+// Layout: 6 * PSY_STATIC_MAX_RAYS words of queues, deep-node lists and
+// pool, then the counters.
 int static_scratch_address(int device, uint32_t** out);
 int launch_world_to_rays(const float* wrays, int64_t n, float* rays, hipStream_t st);
 

@@ -79,7 +79,8 @@ __global__ void k_build_dice(const float* odata, const DevObject* objs, const in
 template <bool WORLD, int STACK, bool COHERENT>
 __global__ __launch_bounds__(TRACE_BLOCK, main_waves(WORLD, STACK, COHERENT)) void k_trace(
     DevScene s, const float* __restrict__ rays, int64_t n, int64_t rs, DevHits h, uint32_t flags,
-    uint32_t* __restrict__ queue, uint32_t* __restrict__ counters, const uint32_t* __restrict__ n_dev) {
+    uint32_t* __restrict__ queue, uint32_t* __restrict__ heads, uint32_t* __restrict__ counters, DeepPool pool,
+    const uint32_t* __restrict__ n_dev) {
     __shared__ int32_t stk_ref[STACK][TRACE_BLOCK];
     __shared__ float stk_near[STACK][TRACE_BLOCK];
     const int tid = threadIdx.x;
@@ -89,7 +90,7 @@ __global__ __launch_bounds__(TRACE_BLOCK, main_waves(WORLD, STACK, COHERENT)) vo
     Lane L;
     lane_init<WORLD>(s, rays, rs, (uint32_t)i, flags, L);
     if constexpr (!COHERENT) {
-        while (!lane_step<WORLD, STACK>(s, L, stk_ref, stk_near, tid)) {
+        while (!lane_step<WORLD, STACK>(s, L, stk_ref, stk_near, tid, pool)) {
         }
     } else {
         bool done = false;
@@ -98,10 +99,11 @@ __global__ __launch_bounds__(TRACE_BLOCK, main_waves(WORLD, STACK, COHERENT)) vo
             if (busy == 0ull) break;
             const bool walk = !done && (L.state & LANE_WALKING) != 0;
             const bool walk_round = 2 * __popcll(__ballot(walk)) >= __popcll(busy);  // wave-uniform
-            if (!done && walk == walk_round && lane_step<WORLD, STACK>(s, L, stk_ref, stk_near, tid)) done = true;
+            if (!done && walk == walk_round && lane_step<WORLD, STACK>(s, L, stk_ref, stk_near, tid, pool))
+                done = true;
         }
     }
-    if (lane_finish(s, L, h)) enqueue_retrace(queue, counters, (uint32_t)i);
+    if (lane_finish(s, L, h)) enqueue_second(queue, heads, counters, L);
 }
 
 constexpr int PERSIST_CHUNK = 256;
@@ -109,8 +111,8 @@ constexpr int PERSIST_CHUNK = 256;
 template <bool WORLD, int STACK>
 __global__ __launch_bounds__(TRACE_BLOCK, persist_waves(WORLD, STACK)) void k_trace_persist(
     DevScene s, const float* __restrict__ rays, int64_t n, int64_t rs, DevHits h, uint32_t flags,
-    uint32_t* __restrict__ queue, uint32_t* __restrict__ counters, int refill_min,
-    const uint32_t* __restrict__ n_dev, int coherent) {
+    uint32_t* __restrict__ queue, uint32_t* __restrict__ heads, uint32_t* __restrict__ counters, DeepPool pool,
+    int refill_min, const uint32_t* __restrict__ n_dev, int coherent) {
     __shared__ int32_t stk_ref[STACK][TRACE_BLOCK];
     __shared__ float stk_near[STACK][TRACE_BLOCK];
     const int tid = threadIdx.x;
@@ -155,9 +157,9 @@ __global__ __launch_bounds__(TRACE_BLOCK, persist_waves(WORLD, STACK)) void k_tr
             const bool walk_round = 2 * __popcll(__ballot(walk)) >= __popcll(busy);
             go = active && walk == walk_round;
         }
-        if (go && lane_step<WORLD, STACK>(s, L, stk_ref, stk_near, tid)) {
+        if (go && lane_step<WORLD, STACK>(s, L, stk_ref, stk_near, tid, pool)) {
             active = false;
-            if (lane_finish(s, L, h)) enqueue_retrace(queue, counters, L.idx);
+            if (lane_finish(s, L, h)) enqueue_second(queue, heads, counters, L);
         }
     }
 }
@@ -165,7 +167,8 @@ __global__ __launch_bounds__(TRACE_BLOCK, persist_waves(WORLD, STACK)) void k_tr
 template <bool WORLD, int STACK, bool WIDE>
 __global__ __launch_bounds__(TRACE_BLOCK, WIDE ? DEEP_WIDE_WAVES : 1)
 void k_trace_deep(DevScene s, const float* __restrict__ rays, int64_t n, int64_t rs, DevHits h, uint32_t flags,
-                  const uint32_t* __restrict__ queue, uint32_t* __restrict__ counters,
+                  const uint32_t* __restrict__ queue, const uint32_t* __restrict__ heads,
+                  const uint32_t* __restrict__ pool, uint32_t* __restrict__ counters,
                   uint32_t* __restrict__ coop_queue, uint32_t wide_min, int32_t budget) {
     __shared__ int32_t stk_ref[STACK][TRACE_BLOCK];
     __shared__ float stk_near[STACK][TRACE_BLOCK];
@@ -179,11 +182,17 @@ void k_trace_deep(DevScene s, const float* __restrict__ rays, int64_t n, int64_t
     const uint32_t wave = blockIdx.x * (TRACE_BLOCK / 64) + (uint32_t)(tid >> 6);
     const uint32_t lane = (uint32_t)(tid & 63);
     for (uint64_t k = (uint64_t)wave + (uint64_t)lane * n_waves; k < count; k += 64ull * n_waves) {
-        const uint32_t i = queue[k];
+        const uint32_t e = queue[k];
+        const uint32_t i = e & ~QUEUE_RESUME;
         if ((int64_t)i >= n) continue;  // entries are ray indices of this call
         // a ray that runs out of node budget goes to the third pass
-        if (trace_one<WORLD, STACK, D_SERIAL>(s, rays, rs, (int64_t)i, h, flags, stk_ref, stk_near, tid, budget))
-            coop_queue[atomicAdd(&counters[1], 1u)] = i;
+        bool more;
+        if (e & QUEUE_RESUME) {
+            more = resume_list<WORLD>(s, rays, rs, (int64_t)i, heads[k], pool, h, flags, budget);
+        } else {
+            more = trace_one<WORLD, STACK, D_SERIAL>(s, rays, rs, (int64_t)i, h, flags, stk_ref, stk_near, tid, budget);
+        }
+        if (more) coop_queue[atomicAdd(&counters[1], 1u)] = i;
     }
 }
 
@@ -210,8 +219,10 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_coop(DevScene s, const fl
     }
 }
 
+// counters of a trace call: [0] second-pass queue, [1] third-pass queue,
+// [2..3] next ray of the persistent form (u64), [4] deep-node pool
 __global__ void k_zero_counters(uint32_t* p) {
-    if (threadIdx.x < 4 && blockIdx.x == 0) p[threadIdx.x] = 0u;
+    if (threadIdx.x < 8 && blockIdx.x == 0) p[threadIdx.x] = 0u;
 }
 
 __global__ void k_world_to_rays(const float* __restrict__ w, int64_t n, float* __restrict__ r) {
@@ -590,7 +601,9 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_render_persist(RenderLaunch A, 
             }
         }
         if (exhausted && __ballot(active) == 0ull) break;
-        if (active && lane_step<true>(s, L, stk_ref, stk_near, tid)) {
+        // no deep-node pool: a ray that needs splits below the cache is
+        // handed back whole and traced by the second pass
+        if (active && lane_step<true>(s, L, stk_ref, stk_near, tid, DeepPool{nullptr, nullptr, 0u})) {
             const bool again = lane_result(s, L);
             WRay w;
             lane_wray(L, rtype, w);
@@ -655,17 +668,18 @@ int launch_build_dice(const float* odata, const DevObject* objs, const int32_t* 
 
 template <bool W, int S>
 static void launch_deep(const DevScene& s, const float* rays, int64_t n, int64_t rs, const DevHits& h, uint32_t flags,
-                        const uint32_t* queue, uint32_t* counters, const TraceTuning& tune, hipStream_t st) {
+                        const uint32_t* queue, const uint32_t* heads, const uint32_t* pool, uint32_t* counters,
+                        const TraceTuning& tune, hipStream_t st) {
     const unsigned blocks = (unsigned)std::min<int64_t>(grid_for(n, TRACE_BLOCK), DEEP_ROUND);
     uint32_t* coop_queue = const_cast<uint32_t*>(queue) + n;
     const int32_t budget = std::max<int32_t>(1, tune.deep_budget);
     if (tune.wide_min > 0)
         hipLaunchKernelGGL((k_trace_deep<W, S, false>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, s, rays, n, rs, h,
-                           flags, queue, counters, coop_queue, tune.wide_min, budget);
+                           flags, queue, heads, pool, counters, coop_queue, tune.wide_min, budget);
     if (n >= (int64_t)tune.wide_min) {
         const unsigned wide = (unsigned)std::min<int64_t>(grid_for(n, TRACE_BLOCK), DEEP_ROUND * DEEP_WIDE_WAVES);
         hipLaunchKernelGGL((k_trace_deep<W, S, true>), dim3(wide), dim3(TRACE_BLOCK), 0, st, s, rays, n, rs, h,
-                           flags, queue, counters, coop_queue, tune.wide_min, budget);
+                           flags, queue, heads, pool, counters, coop_queue, tune.wide_min, budget);
     }
     hipLaunchKernelGGL((k_trace_coop<W, S>), dim3(blocks), dim3(TRACE_BLOCK), 0, st, s, rays, n, rs, h, flags,
                        (const uint32_t*)counters, (const uint32_t*)coop_queue);
@@ -675,6 +689,11 @@ template <bool W>
 static int launch_trace_w(const DevScene& s, int stack_need, const float* rays, int64_t n, int64_t rs, const DevHits& h,
                           uint32_t flags, uint32_t* queue, uint32_t* counters, const TraceTuning& tune,
                           hipStream_t st) {
+    // scratch layout for an n-ray call (6 n words): second-pass queue (n),
+    // third-pass queue (n), the second-pass rays' deep-node lists (n), the
+    // deep-node pool (n entries of 3 words)
+    uint32_t* heads = queue + 2 * n;
+    const DeepPool pool{queue + 3 * n, counters + 4, (uint32_t)std::min<int64_t>(n, (int64_t)UINT32_MAX - 2)};
     hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, counters);
     if (tune.ev[0]) (void)hipEventRecord((hipEvent_t)tune.ev[0], st);
     const int64_t plain_blocks = grid_for(n, TRACE_BLOCK);
@@ -683,30 +702,30 @@ static int launch_trace_w(const DevScene& s, int stack_need, const float* rays, 
     if (tune.persistent && persist_blocks > 0 && persist_blocks < plain_blocks) {
         if (tune.stack > 16)
             hipLaunchKernelGGL((k_trace_persist<W, 32>), dim3((unsigned)persist_blocks), dim3(TRACE_BLOCK), 0, st, s,
-                               rays, n, rs, h, flags, queue, counters, refill, tune.n_dev, tune.coherent ? 1 : 0);
+                               rays, n, rs, h, flags, queue, heads, counters, pool, refill, tune.n_dev, tune.coherent ? 1 : 0);
         else
             hipLaunchKernelGGL((k_trace_persist<W, MAIN_STACK>), dim3((unsigned)persist_blocks), dim3(TRACE_BLOCK), 0,
-                               st, s, rays, n, rs, h, flags, queue, counters, refill, tune.n_dev,
+                               st, s, rays, n, rs, h, flags, queue, heads, counters, pool, refill, tune.n_dev,
                                tune.coherent ? 1 : 0);
     } else if (tune.stack > 16) {
         if (tune.plain_coherent)
             hipLaunchKernelGGL((k_trace<W, 32, true>), dim3((unsigned)plain_blocks), dim3(TRACE_BLOCK), 0, st, s, rays,
-                               n, rs, h, flags, queue, counters, tune.n_dev);
+                               n, rs, h, flags, queue, heads, counters, pool, tune.n_dev);
         else
             hipLaunchKernelGGL((k_trace<W, 32, false>), dim3((unsigned)plain_blocks), dim3(TRACE_BLOCK), 0, st, s, rays,
-                               n, rs, h, flags, queue, counters, tune.n_dev);
+                               n, rs, h, flags, queue, heads, counters, pool, tune.n_dev);
     } else {
         if (tune.plain_coherent)
             hipLaunchKernelGGL((k_trace<W, MAIN_STACK, true>), dim3((unsigned)plain_blocks), dim3(TRACE_BLOCK), 0, st, s,
-                               rays, n, rs, h, flags, queue, counters, tune.n_dev);
+                               rays, n, rs, h, flags, queue, heads, counters, pool, tune.n_dev);
         else
             hipLaunchKernelGGL((k_trace<W, MAIN_STACK, false>), dim3((unsigned)plain_blocks), dim3(TRACE_BLOCK), 0, st,
-                               s, rays, n, rs, h, flags, queue, counters, tune.n_dev);
+                               s, rays, n, rs, h, flags, queue, heads, counters, pool, tune.n_dev);
     }
     if (tune.ev[1]) (void)hipEventRecord((hipEvent_t)tune.ev[1], st);
-    if (stack_need <= 16) launch_deep<W, 16>(s, rays, n, rs, h, flags, queue, counters, tune, st);
-    else if (stack_need <= 32) launch_deep<W, 32>(s, rays, n, rs, h, flags, queue, counters, tune, st);
-    else if (stack_need <= 64) launch_deep<W, 64>(s, rays, n, rs, h, flags, queue, counters, tune, st);
+    if (stack_need <= 16) launch_deep<W, 16>(s, rays, n, rs, h, flags, queue, heads, pool.e, counters, tune, st);
+    else if (stack_need <= 32) launch_deep<W, 32>(s, rays, n, rs, h, flags, queue, heads, pool.e, counters, tune, st);
+    else if (stack_need <= 64) launch_deep<W, 64>(s, rays, n, rs, h, flags, queue, heads, pool.e, counters, tune, st);
     else return PSY_ERR_UNSUPPORTED;
     if (tune.ev[2]) (void)hipEventRecord((hipEvent_t)tune.ev[2], st);
     return check_launch();
@@ -936,14 +955,16 @@ int launch_render_resolve(const float* sample_xyz, int64_t sbase, int64_t first_
 
 // ---------------------------------------------------------------------------
 // Scratch of the allocation-free trace (psy_trace_rays_static): the second-
-// and third-pass queues of up to PSY_STATIC_MAX_RAYS rays, then the pass
-// counters, in one file-scope device array.  Each GPU the library runs on
-// holds its own copy (32 MiB + 256 B) until the process ends.  In host code
+// and third-pass queues, the second-pass rays' deep-node lists and the
+// deep-node pool for up to PSY_STATIC_MAX_RAYS rays (6 words a ray), then
+// the pass counters, in one file-scope device array.  Each GPU the library
+// runs on holds its own copy (48 MiB + 256 B) until the process ends.  In
+// host code
 // `scratch` names a host-side stand-in, never the device memory: the device
 // address comes from hipGetSymbolAddress, once per device, below.
 // ---------------------------------------------------------------------------
 constexpr int64_t STATIC_COUNTER_WORDS = 64;
-static __device__ __align__(256) uint32_t scratch[2 * (int64_t)PSY_STATIC_MAX_RAYS + STATIC_COUNTER_WORDS];
+static __device__ __align__(256) uint32_t scratch[6 * (int64_t)PSY_STATIC_MAX_RAYS + STATIC_COUNTER_WORDS];
 
 namespace {
 constexpr int STATIC_MAX_DEVICES = 64;

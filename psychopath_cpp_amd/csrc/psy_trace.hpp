@@ -905,8 +905,50 @@ struct Lane {
     int32_t next;
     int sp;
     uint32_t state;  // LANE_* bits
+    // the split-tree nodes at the bottom of the dice cache that still need
+    // splitting, as a list in the trace's deep-node pool (DEEP_NONE: empty);
+    // LANE_DEEP_MANY once the pool had no room for one
+    uint32_t deep_head;
 };
-enum : uint32_t { LANE_WALKING = 1u, LANE_NEED_DEEP = 2u, LANE_RETRACE = 4u };
+enum : uint32_t { LANE_WALKING = 1u, LANE_NEED_DEEP = 2u, LANE_RETRACE = 4u, LANE_DEEP_MANY = 8u };
+constexpr uint32_t DEEP_NONE = 0xffffffffu;
+
+// The deep-node pool of one trace call: entry e is 3 words (leaf instance,
+// heap node, next entry of the same ray's list or DEEP_NONE); `count` hands
+// out entries, `cap` bounds them (0: no pool, every such ray is retraced).
+struct DeepPool {
+    uint32_t* e;
+    uint32_t* count;
+    uint32_t cap;
+};
+
+PSY_DEV uint32_t atomic_add_u32(uint32_t* p, uint32_t v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return atomicAdd(p, v);
+#else
+    return __atomic_fetch_add(p, v, __ATOMIC_RELAXED);
+#endif
+}
+
+// `want` consecutive entries of a counter for each calling lane: one atomic
+// per wave for the lanes of the wave that call it together.
+PSY_DEV uint32_t wave_alloc(uint32_t* counter, uint32_t want) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const unsigned long long b1 = __ballot(want >= 1u), b2 = __ballot(want >= 2u);
+    const unsigned long long act = __ballot(1);
+    const uint32_t lane = __lane_id();
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    const uint32_t off = (uint32_t)(__popcll(b1 & lt) + __popcll(b2 & lt));
+    const uint32_t total = (uint32_t)(__popcll(b1) + __popcll(b2));
+    const int leader = __ffsll((long long)act) - 1;
+    uint32_t base = 0;
+    if ((int)lane == leader && total) base = atomicAdd(counter, total);
+    base = (uint32_t)__shfl((int)base, leader);
+    return base + off;
+#else
+    return __atomic_fetch_add(counter, want, __ATOMIC_RELAXED);
+#endif
+}
 
 template <bool WORLD>
 PSY_DEV V3 walk_o(const Lane& L) { return WORLD ? L.wo : L.r.o; }
@@ -940,6 +982,7 @@ PSY_DEV void lane_start(const DevScene& s, Lane& L, float max_t, uint32_t i) {
     L.next = s.root;
     L.sp = 0;
     L.state = 0;
+    L.deep_head = DEEP_NONE;
     // the main pass's box tests assume a finite inverse direction; the
     // second pass follows the reference's NaN behaviour exactly
     if (!finite3(L.r.dinv)) {
@@ -959,7 +1002,7 @@ PSY_DEV void lane_start(const DevScene& s, Lane& L, float max_t, uint32_t i) {
 //   else    : one BVH4 node, or one leaf instance
 template <bool WORLD, int STACK = MAIN_STACK>
 PSY_DEV bool lane_step(const DevScene& s, Lane& L, int32_t (*stk_ref)[TRACE_BLOCK],
-                                          float (*stk_near)[TRACE_BLOCK], int tid) {
+                                          float (*stk_near)[TRACE_BLOCK], int tid, const DeepPool& P) {
     if (L.state & LANE_WALKING) {
         const uint32_t c0 = 2 * L.k;
         const float4* q = s.dice + 2 * ((size_t)L.slot + c0);
@@ -991,7 +1034,28 @@ PSY_DEV bool lane_step(const DevScene& s, Lane& L, int32_t (*stk_ref)[TRACE_BLOC
         split1 = split1 && h10 <= L.b.t;
         const int cd = L.depth + 1;
         if (cd == L.levels) {
-            if (split0 || split1) L.state |= LANE_NEED_DEEP;
+            // children that need splits below the cache go onto the lane's
+            // list for the second pass to resume from
+            const uint32_t nd = (split0 ? 1u : 0u) + (split1 ? 1u : 0u);
+            if (nd) {
+                L.state |= LANE_NEED_DEEP;
+                // rare, so one atomic per lane (a wave-aggregated one costs the main
+                // pass registers)
+                const uint32_t e0 = P.cap ? atomic_add_u32(P.count, nd) : 0u;
+                if (e0 + nd > P.cap) {
+                    L.state |= LANE_DEEP_MANY;
+                } else {
+                    uint32_t e = e0;
+                    if (split1) {
+                        P.e[3 * e] = L.winst; P.e[3 * e + 1] = c0 + 1; P.e[3 * e + 2] = L.deep_head;
+                        L.deep_head = e++;
+                    }
+                    if (split0) {
+                        P.e[3 * e] = L.winst; P.e[3 * e + 1] = c0; P.e[3 * e + 2] = L.deep_head;
+                        L.deep_head = e;
+                    }
+                }
+            }
             split0 = split1 = false;
         }
         if (split0 && split1) {
@@ -1129,10 +1193,69 @@ PSY_DEV bool lane_finish(const DevScene& s, Lane& L, const DevHits& h) {
     return again;
 }
 
-PSY_DEV void enqueue_retrace(uint32_t* __restrict__ queue, uint32_t* __restrict__ counters,
-                                                uint32_t i) {
-    const uint32_t slot = atomicAdd(&counters[0], 1u);
-    queue[slot] = i;
+// Hands a finished lane's ray to the second pass: queue entry = ray index,
+// with bit 31 set when the second pass can resume it from the cache nodes
+// it listed (heads[slot] = its list) instead of retracing it from the BVH
+// root.
+constexpr uint32_t QUEUE_RESUME = 0x80000000u;
+PSY_DEV void enqueue_second(uint32_t* __restrict__ queue, uint32_t* __restrict__ heads,
+                            uint32_t* __restrict__ counters, const Lane& L) {
+    const uint32_t slot = wave_alloc(&counters[0], 1u);
+    const bool listed = (L.state & (LANE_RETRACE | LANE_DEEP_MANY)) == 0;
+    queue[slot] = L.idx | (listed ? QUEUE_RESUME : 0u);
+    heads[slot] = L.deep_head;
+}
+
+// Second pass for a ray the main pass left with a list of cache nodes to
+// split further: its hit so far (as the main pass wrote it), then the
+// split-dice below each listed node within a shared budget of node
+// expansions.  The main pass explored everything else with the same or a
+// looser bound on t, so the result is the full trace's.  Returns true when
+// the budget ran out (the ray then takes the third pass, which retraces it
+// whole).
+template <bool WORLD>
+PSY_DEV bool resume_list(const DevScene& s, const float* __restrict__ rays, int64_t n, int64_t i, uint32_t head,
+                         const uint32_t* __restrict__ pool, const DevHits& h, uint32_t flags, int32_t budget) {
+    TRay r;
+    float max_t;
+    load_ray<WORLD>(rays, n, i, flags, r, max_t);
+    Best b;
+    b.inst = h.inst[i];
+    const bool hit = b.inst != 0xffffffffu;
+    b.t = hit ? h.t[i] : max_t;
+    b.u = h.u[i];
+    b.v = h.v[i];
+    b.aux = h.aux ? h.aux[i] : 0.0f;
+    const uint32_t before = b.inst;
+    const float t_before = b.t;
+    DeepState ds{false, budget, nullptr};
+    for (uint32_t e = head; e != DEEP_NONE; e = pool[3 * e + 2]) {
+        const uint32_t li = pool[3 * e], k = pool[3 * e + 1];
+        const DevInstance I = s.inst[li];
+        const DevObject ob = s.obj[I.object];
+        V3 o = r.o, dinv = r.dinv;
+        RayW w = r.w;
+        if (I.xf_count > 0) {
+            const M44 x = m_lerp_seq(r.time, s.xf + I.xf_offset, I.xf_count);
+            o = m_pos(x, r.o);
+            const V3 d = m_dir(x, r.d);
+            dinv = divv(splat(1.0f), d);
+            if (WORLD) w = ray_widths(o, d, m_dir(x, r.odx), m_dir(x, r.ody), m_dir(x, r.ddx), m_dir(x, r.ddy));
+        }
+        bool done;
+        if (ob.kind == K_BILINEAR) done = dice_deep<4>(s, ob, k, li, o, dinv, w, r.occl, b, ds);
+        else done = dice_deep<16>(s, ob, k, li, o, dinv, w, r.occl, b, ds);
+        if (ds.need) return true;
+        if (done) break;  // an occlusion ray found its hit
+    }
+    if (b.inst != before || b.t != t_before) {
+        h.t[i] = b.t;
+        h.inst[i] = b.inst;
+        h.u[i] = b.u;
+        h.v[i] = b.v;
+        if (h.aux) h.aux[i] = b.aux;
+    }
+    return false;
 }
 
 // Main pass, one thread per ray.

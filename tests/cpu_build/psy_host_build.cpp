@@ -76,11 +76,21 @@ template <bool W>
 bool trace_ray(const DevScene& s, const float* rays, int64_t rs, int64_t i, uint32_t flags, const DevHits& h) {
     static thread_local int32_t stk_ref[64][TRACE_BLOCK];
     static thread_local float stk_near[64][TRACE_BLOCK];
+    // the ray's own deep-node pool (the kernels share one per call)
+    static thread_local std::vector<uint32_t> pool_e(3 * 4096);
+    uint32_t pool_n = 0;
+    const DeepPool pool{pool_e.data(), &pool_n, 4096u};
     Lane L;
     lane_init<W>(s, rays, rs, (uint32_t)i, flags, L);
-    while (!lane_step<W>(s, L, stk_ref, stk_near, 0)) {
+    while (!lane_step<W>(s, L, stk_ref, stk_near, 0, pool)) {
     }
     if (!lane_finish(s, L, h)) return false;
+    // the second pass resumes a ray from its listed cache nodes, or
+    // retraces it; a resumed ray that runs out of budget is retraced whole
+    // (the third pass)
+    if ((L.state & (LANE_RETRACE | LANE_DEEP_MANY)) == 0 &&
+        !resume_list<W>(s, rays, rs, i, L.deep_head, pool_e.data(), h, flags, DEEP_BUDGET))
+        return true;
     (void)trace_one<W, 64, D_SERIAL>(s, rays, rs, i, h, flags, stk_ref, stk_near, 0, INT32_MAX);
     return true;
 }

@@ -120,6 +120,11 @@ def traffic_from_profile():
     return None
 
 
+# Hero wavelengths of every render: uniform over the reference's
+# [380, 700] nm with 3 more at +80 nm steps wrapped into it
+# (color/color.hpp:15-16, 31-38), not the [380, 780] some configurations name.
+HERO_WAVELENGTH_NM = [380.0, 700.0]
+
 # names of PSY_RENDER_* and PSY_TRACE_MODE_* (include/psy_gpu.h)
 RENDER_FORMS = {0: "wavefront", 1: "fused", 2: "stream"}
 TRACE_MODES = {0: "plain", 1: "persistent", 2: "persistent-coherent", 3: "plain-coherent"}
@@ -182,7 +187,7 @@ def batching_forms():
                     "speedup_vs_plain": round(tuned / plain, 2),
                     "speedup_vs_default_wavefront": round(tuned / dflt, 2),
                     "speedup_vs_2p22_wavefront": round(tuned / wide, 2),
-                    "images_identical": same})
+                    "images_identical": same, "hero_wavelength_nm": HERO_WAVELENGTH_NM})
         g.close()
         ctx.close()
     return out
@@ -222,7 +227,8 @@ def adjoint_timing():
     ctx.close()
     return {"workload": "256 spheres + ground, 1000 sphere lights, 512x512, 16 spp, path length 4; gradients of an "
                         "L2 image loss w.r.t. 257 albedos and 1000 emission scales",
-            "forward_ms": round(tf * 1e3, 3), "adjoint_ms": round(ta * 1e3, 3), "ratio": round(ta / tf, 3)}
+            "forward_ms": round(tf * 1e3, 3), "adjoint_ms": round(ta * 1e3, 3), "ratio": round(ta / tf, 3),
+            "hero_wavelength_nm": HERO_WAVELENGTH_NM}
 
 
 def main():

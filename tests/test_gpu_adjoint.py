@@ -64,9 +64,10 @@ def _c4(gpu_ctx, W, H, S):
 
 def test_gpu_adjoint_c4_matches_fd(gpu_ctx, oracle):
     """256 spheres + ground, 1000 sphere lights, target at the true
-    parameters, parameters perturbed x U[0.5,1.5]; 32 random parameters.
-    Full size (512x512, 16 spp) against the device's own forward render;
-    128x128x4 against the oracle's."""
+    parameters, parameters perturbed x U[0.5,1.5]; 32 random parameters,
+    at full size (512x512, 16 spp): against central finite differences of
+    the oracle's renderer (the reference's integrator restated) and of the
+    device's own forward render."""
     sc, g, a0, alb, esc, target = _c4(gpu_ctx, 512, 512, 16)
     n_sh, n_l = len(alb), len(esc)
     rng = np.random.default_rng(2)
@@ -84,8 +85,8 @@ def test_gpu_adjoint_c4_matches_fd(gpu_ctx, oracle):
     fd = J.central_fd(film_gpu, alb, esc, target, params)
     J.check(grad[params], fd, "C4 512x512x16 adjoint vs device FD", say)
 
-    # reduced size against the oracle's renderer
-    W, H, S = 128, 128, 4
+    # against the oracle's renderer, same size, all 32 parameters
+    W, H, S = 512, 512, 16
     o = oracle.OracleScene(sc)
 
     def film_oracle(a, e):
@@ -97,8 +98,8 @@ def test_gpu_adjoint_c4_matches_fd(gpu_ctx, oracle):
     g.set_params(alb, esc)
     ga, ge, _ = gpu.adjoint(gpu_ctx, g, torch.from_numpy(J.d_loss(film_s, target_s)).cuda(), S, 1)
     grad_s = np.concatenate([ga.cpu().numpy(), ge.cpu().numpy()])
-    fd_s = J.central_fd(film_oracle, alb, esc, target_s, params[:16])
-    J.check(grad_s[params[:16]], fd_s, "C4 128x128x4 adjoint vs oracle FD", say)
+    fd_s = J.central_fd(film_oracle, alb, esc, target_s, params)
+    J.check(grad_s[params], fd_s, "C4 512x512x16 adjoint vs oracle FD (32 parameters)", say)
 
 
 def test_gpu_autograd_function_and_time(gpu_ctx):

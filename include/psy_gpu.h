@@ -342,6 +342,28 @@ int psy_trace_rays(psy_ctx* ctx, const psy_scene* scene,
 int psy_trace_rays_static(psy_ctx* ctx, const psy_scene* scene,
                           const float* rays, int64_t n,
                           const psy_hits* hits, uint32_t flags, void* stream);
+/* The rest of the reference's Intersection (basics/intersection.hpp:22-49)
+ * for the closest hits of a psy_trace_world call, as its tracer fills it
+ * at the accepted hit (tracer/tracer.cpp:241-250 with
+ * object/sphere.cpp:120-189, object/patch_utils.hpp:94-131,
+ * lights/sphere_light.hpp:159-246, lights/rectangle_light.hpp:110-147 and
+ * SurfaceShader::shade, shading/surface_shader.hpp:28-53), for a caller that
+ * shades the hits: `out` gets PSY_ISECT_PLANES planes of n floats (device):
+ *   0-2 p, 3-5 n, 6-8 dpdu, 9-11 dpdv, 12-14 dndu, 15-17 dndv  (geo, in
+ *        the hit object's space; 0 for lights' derivatives),
+ *   18-20 offset, 21-36 space (Transform: world -> object, row-major
+ *        matrix at the ray's time; identity when untransformed),
+ *   37 light_pdf (light hits; 9999, the reference's default, elsewhere),
+ *   38 backfacing (uint32 0/1),
+ *   39 closure kind (uint32: PSY_SHADER_LAMBERT or PSY_SHADER_EMIT, 0 on
+ *        a miss, whose other planes hold Intersection's defaults), 40-42 the closure's colour (LambertClosure / EmitClosure
+ *        colour, linear RGB; a light's is its emitted colour per area
+ *        times its emission scale), 43 shader index (int32, -1 for lights
+ *        and for the magenta emitter of a surface without a shader).
+ * world_rays and hits are those of the trace (hits.aux must be set). */
+#define PSY_ISECT_PLANES 44
+int psy_intersections(psy_ctx* ctx, const psy_scene* scene, const float* world_rays, int64_t n,
+                      const psy_hits* hits, float* out, void* stream);
 /* WorldRay::to_ray over a stream: world rays -> object rays (planes as
  * PSY_RAY_PLANES).  Replaces tracer/tracer.cpp:47-51. */
 int psy_world_to_rays(psy_ctx* ctx, const float* world_rays, int64_t n,

@@ -15,7 +15,9 @@
  *   LambertShader, EmitShader
  *                       shading/surface_shader.hpp:28-53
  *   WorldRay            basics/ray.hpp:162-185
- *   Intersection        basics/intersection.hpp:22-49 (closest-hit part)
+ *   Intersection        basics/intersection.hpp:22-49, with the closures
+ *                       LambertClosure / EmitClosure / SurfaceClosureUnion
+ *                       (shading/surface_closure.hpp, shading/closure_union.hpp)
  *   Tracer::trace       tracer/tracer.hpp:57-76, tracer/tracer.cpp:38-121
  *   Film                film/film.hpp:78-153 (per-pixel CIE XYZ means)
  *   PathTraceIntegrator integrator/path_trace_integrator.hpp:103-120
@@ -25,16 +27,19 @@
  *     BVH, dice cache, light trees) and needs a device; failures throw
  *     std::runtime_error with psy_last_error()'s text (the reference has
  *     no error path);
- *   - Intersection carries the closest hit (t, InstanceID, surface u/v,
- *     the flattened leaf instance, the patch leaf extent) rather than a
- *     vtable'd closure: shading runs on the device inside
- *     PathTraceIntegrator;
+ *   - Intersection carries the closest hit (t, InstanceID, differential
+ *     geometry, offset, space, backfacing, light pdf) and the closure the
+ *     shader binds, re-materialised on the host from the device's record;
+ *     the closures keep linear RGB colours and give RGB factors (the
+ *     reference upsamples them to spectral samples; PathTraceIntegrator
+ *     does that on the device);
  *   - colours are grey (r == g == b) for rendering; the tracer takes any.
  */
 #ifndef PSYCHOPATH_GPU_HPP
 #define PSYCHOPATH_GPU_HPP
 
 #include <array>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <fstream>
@@ -459,12 +464,149 @@ struct WorldRay {
     Type type = NONE;
 };
 
+// DifferentialGeometry (basics/differential_geometry.hpp:8-71): a hit in
+// the space of the object hit.
+struct DifferentialGeometry {
+    Vec3 p, n;
+    float u = 0.0f, v = 0.0f;
+    Vec3 dpdu, dpdv, dndu, dndv;
+};
+
+namespace detail {
+inline Vec3 v3(float x, float y, float z) { return Vec3{x, y, z}; }
+inline float dot(const Vec3& a, const Vec3& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline Vec3 scale(const Vec3& a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
+inline Vec3 add(const Vec3& a, const Vec3& b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline Vec3 cross(const Vec3& a, const Vec3& b) {
+    return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+inline Vec3 normalized(const Vec3& a) {
+    const float l = std::sqrt(dot(a, a));
+    return l > 0.0f ? scale(a, 1.0f / l) : a;
+}
+constexpr double PI = 3.1415926535897932384626433;
+}  // namespace detail
+
+// The closures a hit's shader binds (shading/surface_closure.hpp:23-331).
+// The reference evaluates them to SpectralSamples at a hero wavelength;
+// here they keep their linear RGB colour and give the factor that
+// multiplies it, for the caller's own colour handling.
+class SurfaceClosure {
+public:
+    virtual ~SurfaceClosure() = default;
+    virtual bool is_delta() const = 0;
+    virtual bool is_emitter() const = 0;
+    const Color& color() const { return col_; }
+
+protected:
+    explicit SurfaceClosure(Color c) : col_(c) {}
+    Color col_;
+};
+
+// LambertClosure (shading/surface_closure.hpp:233-331).
+class LambertClosure final : public SurfaceClosure {
+public:
+    explicit LambertClosure(Color c = Color(1.0f)) : SurfaceClosure(c) {}
+    bool is_delta() const override { return false; }
+    bool is_emitter() const override { return false; }
+    // evaluate(): col * max(0, n.out) / pi, n turned to the side of `in`
+    Color evaluate(const Vec3& in, const Vec3& out, const DifferentialGeometry& geo) const {
+        const float f = factor(in, out, geo);
+        return Color(col_.r * f, col_.g * f, col_.b * f);
+    }
+    float sample_pdf(const Vec3& in, const Vec3& out, const DifferentialGeometry& geo) const {
+        Vec3 nn = detail::normalized(geo.n);
+        if (detail::dot(nn, detail::normalized(in)) > 0.0f) nn = detail::scale(nn, -1.0f);
+        const float c = detail::dot(nn, detail::normalized(out)) * (float)(1.0 / detail::PI);
+        return c > 0.0f ? c : 0.0f;
+    }
+    // sample(): a cosine-distributed direction about the normal (concentric
+    // disc mapping of si, sj), its filter colour and pdf
+    void sample(const Vec3& in, const DifferentialGeometry& geo, float si, float sj, Vec3* out, Color* filter,
+                float* pdf) const {
+        Vec3 nn = detail::normalized(geo.n);
+        if (detail::dot(nn, in) > 0.0f) nn = detail::scale(nn, -1.0f);
+        float x = (si * 2) - 1, y = (sj * 2) - 1;
+        square_to_circle(&x, &y);
+        const double zz = 1.0 - (double)((x * x) + (y * y));
+        const float z = (float)std::sqrt(zz > 0.0 ? zz : 0.0);
+        *pdf = z * (float)(1.0 / detail::PI);
+        Vec3 tx, ty;
+        if (std::fabs(nn.x) > std::fabs(nn.y)) {
+            const float il = 1.f / std::sqrt(nn.x * nn.x + nn.z * nn.z);
+            tx = detail::v3(-nn.z * il, 0.f, nn.x * il);
+        } else {
+            const float il = 1.f / std::sqrt(nn.y * nn.y + nn.z * nn.z);
+            tx = detail::v3(0.f, nn.z * il, -nn.y * il);
+        }
+        ty = detail::cross(nn, tx);
+        *out = detail::add(detail::add(detail::scale(tx, x), detail::scale(ty, y)), detail::scale(nn, z));
+        *filter = evaluate(in, *out, geo);
+    }
+
+private:
+    float factor(const Vec3& in, const Vec3& out, const DifferentialGeometry& geo) const {
+        Vec3 nn = detail::normalized(geo.n);
+        if (detail::dot(nn, in) > 0.0f) nn = detail::scale(nn, -1.0f);
+        const float c = detail::dot(nn, detail::normalized(out));
+        return (c > 0.0f ? c : 0.0f) * (float)(1.0 / detail::PI);
+    }
+    // square_to_circle (utils/monte_carlo.hpp:29-53)
+    static void square_to_circle(float* x, float* y) {
+        if (*x == 0.0f && *y == 0.0f) return;
+        float radius, angle;
+        const double q = detail::PI / 4;
+        if (*x > std::fabs(*y)) { radius = *x; angle = (float)(q * (double)(*y / *x)); }
+        else if (*y > std::fabs(*x)) { radius = *y; angle = (float)(q * (double)(2 - (*x / *y))); }
+        else if (*x < -std::fabs(*y)) { radius = -*x; angle = (float)(q * (double)(4 + (*y / *x))); }
+        else { radius = -*y; angle = (float)(q * (double)(6 - (*x / *y))); }
+        *x = radius * std::cos(angle);
+        *y = radius * std::sin(angle);
+    }
+};
+
+// EmitClosure (shading/surface_closure.hpp:186-228): emits its colour.
+class EmitClosure final : public SurfaceClosure {
+public:
+    explicit EmitClosure(Color c = Color(1.0f)) : SurfaceClosure(c) {}
+    bool is_delta() const override { return false; }
+    bool is_emitter() const override { return true; }
+    const Color& emitted_color() const { return col_; }
+};
+
+// SurfaceClosureUnion (shading/closure_union.hpp:16-43): storage for one
+// closure of either kind; get() is null until init().
+class SurfaceClosureUnion {
+public:
+    void init(const LambertClosure& c) { lambert_ = c; kind_ = 1; }
+    void init(const EmitClosure& c) { emit_ = c; kind_ = 2; }
+    SurfaceClosure* get() { return kind_ == 1 ? (SurfaceClosure*)&lambert_ : kind_ == 2 ? (SurfaceClosure*)&emit_ : nullptr; }
+    const SurfaceClosure* get() const {
+        return kind_ == 1 ? (const SurfaceClosure*)&lambert_ : kind_ == 2 ? (const SurfaceClosure*)&emit_ : nullptr;
+    }
+
+private:
+    LambertClosure lambert_;
+    EmitClosure emit_;
+    int kind_ = 0;
+};
+
+// Intersection (basics/intersection.hpp:22-49), as the reference's tracer
+// fills it at the accepted hit (tracer/tracer.cpp:241-250).
 struct Intersection {
     bool hit = false;
     uint64_t id = 0;                  // InstanceID::id of the leaf hit
     float t = 1.0f / 0.0f;            // +inf on a miss
-    float u = 0.0f, v = 0.0f;         // surface parameters (patches, spheres)
-    float offset_dim = 0.0f;          // patch leaf extent (ray offset = n * 1.74 * offset_dim)
+    float u = 0.0f, v = 0.0f;         // surface parameters (patches, spheres; geo.u, geo.v)
+    DifferentialGeometry geo;         // in the space of the object hit
+    Vec3 offset;                      // ray origin offset off the surface (object space)
+    float light_pdf = 9999.0f;        // light hits: pdf of sampling this direction
+    bool backfacing = false;
+    Transform space;                  // world -> object at the ray's time
+    SurfaceClosureUnion surface_closure;
+    float closure_prob = 1.0f;        // one closure per shader: always selected
+    int32_t shader = -1;              // the scene's shader index, -1 for lights / none
+    float offset_dim = 0.0f;          // patch leaf extent (offset = n * 1.74 * offset_dim)
     uint32_t instance = 0xffffffffu;  // flattened leaf instance
 };
 
@@ -507,19 +649,42 @@ public:
         float* h = (float*)hits_;
         psy_hits hits{h, (uint32_t*)(h + n), h + 2 * n, h + 3 * n, h + 4 * n};
         check(psy_trace_world(ctx, scene_->handle, (const float*)rays_, n, &hits, 0u, nullptr), "psy_trace_world");
+        check(psy_intersections(ctx, scene_->handle, (const float*)rays_, n, &hits, (float*)isect_, nullptr),
+              "psy_intersections");
         host_.resize((size_t)(5 * n));
         check(psy_copy_to_host(ctx, host_.data(), hits_, sizeof(float) * 5 * (size_t)n), "Tracer::trace");
+        rec_.resize((size_t)(PSY_ISECT_PLANES * n));
+        check(psy_copy_to_host(ctx, rec_.data(), isect_, sizeof(float) * rec_.size()), "Tracer::trace");
+        auto at = [&](int plane, int64_t i) { return rec_[(size_t)(plane * n + i)]; };
+        auto at_u = [&](int plane, int64_t i) {
+            uint32_t u;
+            std::memcpy(&u, &rec_[(size_t)(plane * n + i)], 4);
+            return u;
+        };
         for (int64_t i = 0; i < n; ++i) {
             Intersection& x = intersections_begin[i];
+            x = Intersection{};
             uint32_t inst;
             std::memcpy(&inst, &host_[(size_t)(n + i)], 4);
             x.hit = inst != 0xffffffffu;
             x.instance = inst;
-            x.id = x.hit ? ids_[inst] : 0;
             x.t = host_[(size_t)i];
             x.u = host_[(size_t)(2 * n + i)];
             x.v = host_[(size_t)(3 * n + i)];
             x.offset_dim = host_[(size_t)(4 * n + i)];
+            if (!x.hit) continue;
+            x.id = ids_[inst];
+            Vec3* vs[7] = {&x.geo.p, &x.geo.n, &x.geo.dpdu, &x.geo.dpdv, &x.geo.dndu, &x.geo.dndv, &x.offset};
+            for (int k = 0; k < 7; ++k) *vs[k] = Vec3{at(3 * k, i), at(3 * k + 1, i), at(3 * k + 2, i)};
+            x.geo.u = x.u;
+            x.geo.v = x.v;
+            for (int k = 0; k < 16; ++k) x.space.m[(size_t)k] = at(21 + k, i);
+            x.light_pdf = at(37, i);
+            x.backfacing = at_u(38, i) != 0;
+            const Color col(at(40, i), at(41, i), at(42, i));
+            if (at_u(39, i) == PSY_SHADER_LAMBERT) x.surface_closure.init(LambertClosure(col));
+            else x.surface_closure.init(EmitClosure(col));
+            x.shader = (int32_t)at_u(43, i);
         }
         return (uint32_t)n;
     }
@@ -531,19 +696,22 @@ private:
         psy_ctx* ctx = Device::get().ctx();
         check(psy_malloc(ctx, sizeof(float) * PSY_WRAY_PLANES * (size_t)n, &rays_), "psy_malloc");
         check(psy_malloc(ctx, sizeof(float) * 5 * (size_t)n, &hits_), "psy_malloc");
+        check(psy_malloc(ctx, sizeof(float) * PSY_ISECT_PLANES * (size_t)n, &isect_), "psy_malloc");
         cap_ = n;
     }
     void release() {
         if (rays_) psy_free(Device::get().ctx(), rays_);
         if (hits_) psy_free(Device::get().ctx(), hits_);
-        rays_ = hits_ = nullptr;
+        if (isect_) psy_free(Device::get().ctx(), isect_);
+        rays_ = hits_ = isect_ = nullptr;
         cap_ = 0;
     }
     Scene* scene_;
     std::vector<uint64_t> ids_;
-    std::vector<float> host_;
+    std::vector<float> host_, rec_;
     void* rays_ = nullptr;
     void* hits_ = nullptr;
+    void* isect_ = nullptr;
     int64_t cap_ = 0;
 };
 

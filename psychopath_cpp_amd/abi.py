@@ -47,6 +47,7 @@ FLOATS_PER_KIND = {
 
 MISS = 0xFFFFFFFF
 PSY_STATIC_MAX_RAYS = 2097152
+PSY_ISECT_PLANES = 44
 
 
 class ObjectDesc(C.Structure):
@@ -115,7 +116,7 @@ EXPORTED_SYMBOLS = [
     "psy_ctx_create", "psy_ctx_destroy", "psy_ctx_set_trace_mode", "psy_ctx_retrace_count", "psy_ctx_trace_pass_counts", "psy_ctx_set_render_chunk", "psy_ctx_set_render_mode", "psy_ctx_set_trace_timing", "psy_ctx_set_trace_stack", "psy_ctx_set_second_pass", "psy_ctx_set_stream_steps", "psy_render_autotune", "psy_ctx_trace_timing", "psy_last_error", "psy_abi_version",
     "psy_malloc", "psy_free", "psy_copy_to_device", "psy_copy_to_host",
     "psy_scene_create", "psy_scene_destroy", "psy_scene_get_info", "psy_scene_instance_ids",
-    "psy_trace_world", "psy_trace_rays", "psy_trace_rays_static", "psy_world_to_rays",
+    "psy_trace_world", "psy_trace_rays", "psy_trace_rays_static", "psy_intersections", "psy_world_to_rays",
     "psy_parse_scene_count", "psy_parse_scene", "psy_parsed_scene_desc", "psy_parsed_scene_settings",
     "psy_parsed_scene_warnings", "psy_parsed_scene_destroy", "psy_parse_dump_tree",
     "psy_film_to_srgb8", "psy_write_png", "psy_write_pfm",

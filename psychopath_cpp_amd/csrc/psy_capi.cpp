@@ -1152,6 +1152,17 @@ int psy_trace_rays_static(psy_ctx* ctx, const psy_scene* s, const float* rays, i
     return rc ? fail(rc, "psy_trace_rays_static: launch failed") : PSY_OK;
 }
 
+int psy_intersections(psy_ctx* ctx, const psy_scene* s, const float* wrays, int64_t n, const psy_hits* hits,
+                      float* out, void* stream) {
+    if (!ctx || !s || ((!wrays || !out) && n > 0) || n < 0)
+        return fail(PSY_ERR_INVALID, "psy_intersections: bad argument");
+    if (int rc = check_hits(hits, n)) return rc;
+    if (n > 0 && !hits->aux) return fail(PSY_ERR_INVALID, "psy_intersections: hits.aux is needed (patch offsets)");
+    HIP_OK(hipSetDevice(ctx->device));
+    int rc = launch_intersections(s->dev, wrays, n, to_dev(hits), out, (hipStream_t)stream);
+    return rc ? fail(rc, "psy_intersections: launch failed") : PSY_OK;
+}
+
 int psy_world_to_rays(psy_ctx* ctx, const float* wrays, int64_t n, float* rays, void* stream) {
     if (!ctx || ((!wrays || !rays) && n > 0) || n < 0) return fail(PSY_ERR_INVALID, "psy_world_to_rays: bad argument");
     HIP_OK(hipSetDevice(ctx->device));

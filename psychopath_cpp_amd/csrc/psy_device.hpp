@@ -175,6 +175,9 @@ int trace_persist_occupancy(bool world, int stack, int* blocks_per_cu);
 // pool, then the counters.
 int static_scratch_address(int device, uint32_t** out);
 int launch_world_to_rays(const float* wrays, int64_t n, float* rays, hipStream_t st);
+// Intersection records (PSY_ISECT_PLANES planes) of the hits of world rays.
+int launch_intersections(const DevScene& s, const float* wrays, int64_t n, const DevHits& h, float* out,
+                         hipStream_t st);
 
 // ---- integrator (psy_render_dev.hpp) ----------------------------------------
 // The spectrum grid of colour upsampling (color/spectrum_grid.h) as the

@@ -225,6 +225,43 @@ __global__ void k_zero_counters(uint32_t* p) {
     if (threadIdx.x < 8 && blockIdx.x == 0) p[threadIdx.x] = 0u;
 }
 
+// Intersection records of the hits of world rays (PSY_ISECT_PLANES planes
+// of n entries; see psy_gpu.h).  Misses get closure kind 0.
+__global__ void k_intersections(DevScene s, const float* __restrict__ w, int64_t n, DevHits h,
+                                float* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t li = h.inst[i];
+    float* o = out + i;
+    if (li == 0xffffffffu) {
+        // a miss keeps Intersection's defaults (basics/intersection.hpp:24-49)
+        for (int p = 0; p < PSY_ISECT_PLANES; ++p) o[p * n] = 0.0f;
+        for (int k = 0; k < 4; ++k) o[(21 + 5 * k) * n] = 1.0f;
+        o[37 * n] = 9999.0f;
+        o[43 * n] = __int_as_float(-1);
+        return;
+    }
+    const V3 wo = mk(w[0 * n + i], w[1 * n + i], w[2 * n + i]);
+    const V3 wd = mk(w[3 * n + i], w[4 * n + i], w[5 * n + i]);
+    const float time = w[18 * n + i];
+    IsectRec R;
+    intersection_record(s, li, h.t[i], h.u[i], h.v[i], h.aux ? h.aux[i] : 0.0f, wo, wd, time, R);
+    const V3 v3s[7] = {R.p, R.n, R.dpdu, R.dpdv, R.dndu, R.dndv, R.offset};
+    for (int k = 0; k < 7; ++k) {
+        o[(3 * k + 0) * n] = v3s[k].x;
+        o[(3 * k + 1) * n] = v3s[k].y;
+        o[(3 * k + 2) * n] = v3s[k].z;
+    }
+    for (int k = 0; k < 16; ++k) o[(21 + k) * n] = R.space.m[k];
+    o[37 * n] = R.light_pdf;
+    o[38 * n] = __uint_as_float(R.backfacing);
+    o[39 * n] = __uint_as_float(R.closure);
+    o[40 * n] = R.col[0];
+    o[41 * n] = R.col[1];
+    o[42 * n] = R.col[2];
+    o[43 * n] = __int_as_float(R.shader);
+}
+
 __global__ void k_world_to_rays(const float* __restrict__ w, int64_t n, float* __restrict__ r) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -762,6 +799,13 @@ __global__ void k_zero_n(uint32_t* p, int n) {
 int launch_zero_u32(uint32_t* p, int n, hipStream_t st) {
     if (n <= 0) return 0;
     hipLaunchKernelGGL(k_zero_n, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n);
+    return check_launch();
+}
+
+int launch_intersections(const DevScene& s, const float* wrays, int64_t n, const DevHits& h, float* out,
+                         hipStream_t st) {
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_intersections, dim3(grid_for(n, 128)), dim3(128), 0, st, s, wrays, n, h, out);
     return check_launch();
 }
 

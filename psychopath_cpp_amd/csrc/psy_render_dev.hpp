@@ -675,6 +675,117 @@ PSY_DEV_FN void light_hit(const DevScene& s, const DevSpectrum& SP, uint32_t li,
 }
 
 
+// ---- Intersection records ------------------------------------------------------
+// The fields of the reference's Intersection (basics/intersection.hpp:22-49)
+// for one closest hit, as its tracer fills them: differential geometry in the
+// hit object's space (Sphere::intersect_ray, object/sphere.cpp:120-189;
+// intersect_rays_with_patch, object/patch_utils.hpp:94-131; the lights'
+// intersect_ray), the merged instance transform at the ray's time (`space`,
+// world -> object), the light pdf of a light hit, and the closure the
+// shader binds (shading/surface_shader.hpp:28-53; magenta emission without a
+// shader, tracer/tracer.cpp:248-250) as a kind and an RGB colour.
+struct IsectRec {
+    V3 p, n, dpdu, dpdv, dndu, dndv, offset;
+    M44 space;
+    float light_pdf;
+    uint32_t backfacing;
+    uint32_t closure;  // PSY_SHADER_LAMBERT / PSY_SHADER_EMIT
+    float col[3];
+    int32_t shader;    // the instance's shader, -1 for lights and none
+};
+
+// Sphere differential geometry (object/sphere.cpp:131-189): latitude /
+// longitude parameterization scaled to [0,1]^2.
+PSY_DEV_FN void sphere_dg(V3 n, float radi, V3* dpdu, V3* dpdv, V3* dndu, V3* dndv) {
+    const V3 pp = scale(n, radi);
+    const float lat_sin = sqrtf((n.x * n.x) + (n.y * n.y));
+    const float long_cos = n.x / lat_sin;
+    const float long_sin = n.y / lat_sin;
+    const float pi2 = (float)(PSY_PI_D * 2);
+    const float fpi = (float)PSY_PI_D;
+    *dpdu = scale(mk(pp.y * -1.0f, pp.x, 0.0f), pi2);
+    *dpdv = scale(mk(pp.z * long_cos, pp.z * long_sin, -radi * lat_sin), fpi);
+    const V3 d2pduu = scale(mk(pp.x, pp.y, 0.0f), (-pi2 * pi2));
+    const V3 d2pduv = scale(scale(scale(mk(-long_sin, long_cos, 0.0f), fpi), pp.z), pi2);
+    const V3 d2pdvv = scale(mk(pp.x, pp.y, pp.z), (float)(-PSY_PI_D * PSY_PI_D));
+    const float E = dot(*dpdu, *dpdu), F = dot(*dpdu, *dpdv), G = dot(*dpdv, *dpdv);
+    const float e = dot(n, d2pduu), f = dot(n, d2pduv), g = dot(n, d2pdvv);
+    const float invEGF2 = 1.0f / ((E * G) - (F * F));
+    *dndu = add(scale(*dpdu, ((f * F) - (e * G)) * invEGF2), scale(*dpdv, ((e * F) - (f * E)) * invEGF2));
+    *dndv = add(scale(*dpdu, ((g * F) - (f * G)) * invEGF2), scale(*dpdv, ((f * F) - (g * E)) * invEGF2));
+}
+
+PSY_DEV_FN void intersection_record(const DevScene& s, uint32_t li, float t, float u, float v, float aux, V3 wo,
+                                    V3 wd, float time, IsectRec& R) {
+    const DevInstance I = s.inst[li];
+    const DevObject ob = s.obj[I.object];
+    R.space = m_identity();
+    V3 o = wo, d = wd;
+    if (I.xf_count > 0) {
+        R.space = m_lerp_seq(time, s.xf + I.xf_offset, I.xf_count);
+        o = m_pos(R.space, wo);
+        d = m_dir(R.space, wd);
+    }
+    const float* od = s.odata + ob.data;
+    R.p = add(o, scale(d, t));
+    R.light_pdf = 9999.0f;  // Intersection's default where no light sets it
+    R.shader = -1;
+    R.dpdu = R.dpdv = R.dndu = R.dndv = splat(0.0f);
+    if (ob.kind == K_RECT_LIGHT) {
+        float w, h, col[3];
+        rect_light_at(od, ob.ts, time, &w, &h, col);
+        R.p = mk(o.x + (d.x * t), o.y + (d.y * t), 0.0f);
+        R.n = mk(0.0f, 0.0f, 1.0f);
+        R.backfacing = d.z > 0.0f ? 1u : 0u;
+        R.light_pdf = rect_light_pdf(w, h, o);
+        R.offset = scale(R.n, 0.000001f);
+        const double area = (double)(w * h);
+        R.closure = PSY_SHADER_EMIT;
+        for (int k = 0; k < 3; ++k) R.col[k] = ((col[k] * 0.5f) / (float)area) * s.escale[I.object];
+        return;
+    }
+    if (ob.kind == K_SPHERE_LIGHT) {
+        V3 c; float r, col[3];
+        sphere_light_at(od, ob.ts, time, &c, &r, col);
+        const double area = 4.0 * PSY_PI_D * r * r;
+        R.n = normalized(sub(R.p, c));
+        R.backfacing = dot(R.n, normalized(d)) > 0.0f ? 1u : 0u;
+        R.light_pdf = sphere_light_pdf(c, (double)r, o);
+        R.offset = scale(R.n, 0.000001f);
+        R.closure = PSY_SHADER_EMIT;
+        for (int k = 0; k < 3; ++k) R.col[k] = (col[k] / (float)area) * s.escale[I.object];
+        return;
+    }
+    if (ob.kind == K_SPHERE) {
+        V3 c; float r;
+        sphere_at(od, ob.ts, 4, time, &c, &r);
+        R.n = normalized(sub(R.p, c));
+        sphere_dg(R.n, r, &R.dpdu, &R.dpdv, &R.dndu, &R.dndv);
+        R.offset = scale(R.n, 0.000001f);
+    } else {
+        if (ob.kind == K_BILINEAR) {
+            Patch<4> pp;
+            load_patch<4>(s.odata, ob.data, pp);
+            patch_dg<4>(pp, u, v, &R.n, &R.dpdu, &R.dpdv, &R.dndu, &R.dndv);
+        } else {
+            Patch<16> pp;
+            load_patch<16>(s.odata, ob.data, pp);
+            patch_dg<16>(pp, u, v, &R.n, &R.dpdu, &R.dpdv, &R.dndu, &R.dndv);
+        }
+        R.offset = scale(R.n, aux * 1.74f);
+    }
+    R.backfacing = dot(R.n, normalized(d)) > 0.0f ? 1u : 0u;
+    R.shader = I.shader;
+    if (I.shader >= 0) {
+        const DevShader sh = s.shaders[I.shader];
+        R.closure = (uint32_t)sh.kind;
+        R.col[0] = sh.color[0]; R.col[1] = sh.color[1]; R.col[2] = sh.color[2];
+    } else {
+        R.closure = PSY_SHADER_EMIT;
+        R.col[0] = 1.0f; R.col[1] = 0.0f; R.col[2] = 1.0f;
+    }
+}
+
 // ---- adjoint (path replay) ---------------------------------------------------
 // The adjoint pass re-runs every path from its sample index with the same
 // sampler, scene and parameters, so it reproduces the forward paths exactly

@@ -47,6 +47,7 @@ def lib():
     L.psy_trace_world.argtypes = [vp, vp, vp, i64, C.POINTER(abi.Hits), u32, vp]
     L.psy_trace_rays.argtypes = [vp, vp, vp, i64, C.POINTER(abi.Hits), u32, vp]
     L.psy_trace_rays_static.argtypes = [vp, vp, vp, i64, C.POINTER(abi.Hits), u32, vp]
+    L.psy_intersections.argtypes = [vp, vp, vp, i64, C.POINTER(abi.Hits), vp, vp]
     L.psy_world_to_rays.argtypes = [vp, vp, i64, vp, vp]
     L.psy_halton.argtypes = [vp, u32, vp, i64, vp, vp]
     L.psy_ctx_set_trace_timing.argtypes = [vp, C.c_int]
@@ -280,6 +281,18 @@ def trace_rays_static(ctx, scene, rays, occlusion_only=False, hits=None):
     check(lib().psy_trace_rays_static(ctx.handle, scene.handle, _ptr(rays), n, C.byref(hs), flags, _stream()),
           "psy_trace_rays_static")
     return hits
+
+
+def intersections(ctx, scene, wrays, hits):
+    """psy_intersections: the Intersection fields of the hits of trace_world(wrays) as a
+    (PSY_ISECT_PLANES, n) float32 tensor (see include/psy_gpu.h for the planes)."""
+    _check_planes(wrays, abi.PSY_WRAY_PLANES, "world rays")
+    n = wrays.shape[1]
+    out = torch.empty((abi.PSY_ISECT_PLANES, n), dtype=torch.float32, device=wrays.device)
+    hs = hits.struct()
+    check(lib().psy_intersections(ctx.handle, scene.handle, _ptr(wrays), n, C.byref(hs), _ptr(out), _stream()),
+          "psy_intersections")
+    return out
 
 
 def world_to_rays(ctx, wrays):

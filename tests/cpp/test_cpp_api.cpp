@@ -174,6 +174,26 @@ int main(int argc, char** argv) {
         }
         f.write((const char*)film.xyz.data(), sizeof(float) * film.xyz.size());
         f.write((const char*)&integ.rays_shot, sizeof integ.rays_shot);
+        // the rest of each Intersection, in the plane order of psy_intersections
+        for (const Intersection& h : hits) {
+            float rec[PSY_ISECT_PLANES] = {};
+            if (h.hit) {
+                const Vec3* vs[7] = {&h.geo.p, &h.geo.n, &h.geo.dpdu, &h.geo.dpdv, &h.geo.dndu, &h.geo.dndv, &h.offset};
+                for (int k = 0; k < 7; ++k) {
+                    rec[3 * k] = vs[k]->x; rec[3 * k + 1] = vs[k]->y; rec[3 * k + 2] = vs[k]->z;
+                }
+                for (int k = 0; k < 16; ++k) rec[21 + k] = h.space.m[(size_t)k];
+                rec[37] = h.light_pdf;
+                const uint32_t bf = h.backfacing ? 1u : 0u;
+                const SurfaceClosure* c = h.surface_closure.get();
+                const uint32_t kind = c ? (c->is_emitter() ? (uint32_t)PSY_SHADER_EMIT : (uint32_t)PSY_SHADER_LAMBERT) : 0u;
+                std::memcpy(&rec[38], &bf, 4);
+                std::memcpy(&rec[39], &kind, 4);
+                if (c) { rec[40] = c->color().r; rec[41] = c->color().g; rec[42] = c->color().b; }
+                std::memcpy(&rec[43], &h.shader, 4);
+            }
+            f.write((const char*)rec, sizeof rec);
+        }
         std::printf("traced %u rays, rendered %lld rays\n", n, (long long)integ.rays_shot);
         return 0;
     }

@@ -56,6 +56,8 @@ def test_gpu_cpp_tracer_and_integrator(gpu_ctx):
     film_cpp = np.frombuffer(raw[off:off + 4 * W * H * 3], np.float32).reshape(H, W, 3)
     off += 4 * W * H * 3
     rays_shot = int(np.frombuffer(raw[off:off + 8], np.int64)[0])
+    off += 8
+    isect_cpp = np.frombuffer(raw[off:off + 4 * 44 * n], np.float32).reshape(n, 44)
     sc = cpp_equivalent_scene()
     g = gpu.GpuScene(gpu_ctx, sc)
     planes = np.ascontiguousarray(rays.T)
@@ -71,6 +73,12 @@ def test_gpu_cpp_tracer_and_integrator(gpu_ctx):
     ids = g.instance_ids()
     hit = inst != 0xFFFFFFFF
     assert np.array_equal(rec["id"][hit], ids[inst[hit]])
+    # the Intersection's geometry, flags and re-materialised closure are the
+    # C ABI's records of the same hits
+    isect = gpu.intersections(gpu_ctx, g, torch.from_numpy(planes).cuda(), hits).cpu().numpy().T
+    assert np.array_equal(isect_cpp[hit].view(np.uint32), isect[hit].view(np.uint32))
+    kinds = isect_cpp[hit][:, 39].view(np.uint32)
+    assert set(np.unique(kinds)) <= {1, 2} and (kinds == 1).any()
     film, _, rays_py = gpu.render(gpu_ctx, g, int(W), int(H), 4, 1)
     assert rays_shot == rays_py
     assert np.array_equal(film.cpu().numpy().view(np.uint32), film_cpp.view(np.uint32))

@@ -106,18 +106,39 @@ def cpu_baseline(scene_py, wrays, pick, min_seconds):
     return sample * reps / dt / 1e6, threads, inter, dt, reps
 
 
+PROFILE = os.path.join(ROOT, "profiles", "trace_heightfield_stream.txt")
+
+
+def _profile_value(key):
+    """A `key: value` figure of the main pass's rocprofv3 summary, or None."""
+    if not os.path.exists(PROFILE):
+        return None
+    with open(PROFILE) as f:
+        for line in f:
+            if line.startswith(key + ":"):
+                return float(line.split(":", 1)[1].strip())
+    return None
+
+
 def traffic_from_profile():
     """HBM bytes per main-pass launch measured with rocprofv3 (FETCH_SIZE and
     WRITE_SIZE, scaled by a plain copy's reading), kept beside the profile
     summary; None when that file is absent."""
-    path = os.path.join(ROOT, "profiles", "trace_heightfield_stream.txt")
-    if not os.path.exists(path):
+    return _profile_value("traffic_bytes_per_launch")
+
+
+def issue_from_profile():
+    """The main pass's instruction-issue figures from the same summary: the
+    share of time the SIMDs issue VALU instructions (VALUBusy), the active
+    lanes per VALU instruction (VALUUtilization) and the mean resident
+    waves per SIMD (MeanOccupancyPerCU / 4)."""
+    keys = ("valu_busy_pct", "valu_lane_utilization_pct", "waves_per_simd")
+    vals = {k: _profile_value(k) for k in keys}
+    if any(v is None for v in vals.values()):
         return None
-    with open(path) as f:
-        for line in f:
-            if line.startswith("traffic_bytes_per_launch:"):
-                return float(line.split(":", 1)[1].strip())
-    return None
+    vals["waves_per_simd_allowed"] = 5
+    vals["source"] = "rocprofv3 --pmc VALUBusy VALUUtilization MeanOccupancyPerCU (profiles/trace_heightfield_stream.txt)"
+    return vals
 
 
 # Hero wavelengths of every render: uniform over the reference's
@@ -390,6 +411,7 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
             "traffic": traffic_from_profile(),
+            "issue": issue_from_profile(),
         },
         "cpu_baseline": cpu,
     }

@@ -265,6 +265,8 @@ def main():
     ap.add_argument("--trace-mode", choices=("persistent", "plain"), default="plain",
                     help="main trace pass: persistent waves with lane refill, or one thread per ray")
     ap.add_argument("--refill", type=int, default=16, help="persistent mode: refill a wave once this many lanes idle")
+    ap.add_argument("--continuation", type=int, default=24,
+                    help="plain mode: a wave hands its rays to a follow-on launch once fewer than this many trace (0: never)")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="persistent mode: blocks per CU (0: occupancy)")
     ap.add_argument("--split", action="store_true",
                     help="also time the primary and the bounce half of the stream separately (stderr)")
@@ -291,6 +293,7 @@ def main():
 
     log("setting up heightfield scene")
     ctx = gpu.Context(0)
+    ctx.set_continuation(args.continuation)
     ctx.set_trace_mode(persistent=args.trace_mode == "persistent", refill_min=args.refill,
                        blocks_per_cu=args.blocks_per_cu)
     scene_py = configs.heightfield_scene()
@@ -402,9 +405,10 @@ def main():
             "leaf_instances": int(info.n_leaf_instances),
             "bytes_per_ray": BYTES_PER_RAY,
             "trace_mode": args.trace_mode,
+            "continuation_below": args.continuation,
         },
         "roofline": {
-            "kernel": "k_trace<false, 16, false> (main pass, %.3f ms/launch)" % main_avg,
+            "kernel": "k_trace<false, 16, false> + k_trace_cont<false, 16> (main pass, %.3f ms/call)" % main_avg,
             "bound": "hbm",
             "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS,

@@ -237,6 +237,14 @@ int psy_ctx_set_trace_stack(psy_ctx* ctx, int entries);
  * split-tree node expansions there goes on to the third, wave-cooperative
  * pass.  Results do not depend on either setting. */
 int psy_ctx_set_second_pass(psy_ctx* ctx, uint32_t wide_min, int32_t budget);
+/* The main pass's continuations (no counterpart in the reference, whose
+ * lanes are CPU threads): the rays of one 64-lane wave end after very
+ * different numbers of node steps, so once fewer than `below` of a wave's
+ * rays are still tracing (1..64, default 24), their traversal state goes
+ * to a continuation buffer in the context's scratch and a follow-on launch
+ * finishes them 64 to a wave; 0 turns it off.  Results are identical.
+ * psy_trace_rays_static never continues. */
+int psy_ctx_set_continuation(psy_ctx* ctx, int below);
 /* Paths per wavefront chunk of psy_render (no counterpart in the reference,
  * whose buckets are sqrt(65536/spp) pixels square): whole pixels, at least
  * one; the default 2^21 paths.  Results do not depend on it. */

@@ -135,8 +135,8 @@ int ensure_queue(psy_ctx* ctx, int64_t n) {
     const int64_t cap = std::max<int64_t>(n, 1 << 16);
     // for up to cap rays: the second pass's queue, the third pass's queue,
     // the second-pass rays' deep-node lists and the deep-node pool (3 words
-    // an entry)
-    int rc = ctx->queue.alloc((size_t)(6 * cap) * sizeof(uint32_t));
+    // an entry), then the main pass's continuation buffer
+    int rc = ctx->queue.alloc((size_t)(6 * cap + psy::cont_area_words(cap)) * sizeof(uint32_t));
     if (rc) { ctx->queue_cap = 0; return rc; }
     ctx->queue_cap = cap;
     return PSY_OK;
@@ -207,6 +207,13 @@ int psy_ctx_set_second_pass(psy_ctx* ctx, uint32_t wide_min, int32_t budget) {
     if (budget < 1) return fail(PSY_ERR_INVALID, "psy_ctx_set_second_pass: budget must be at least 1");
     ctx->deep_wide_min = wide_min;
     ctx->deep_budget = budget;
+    return PSY_OK;
+}
+
+int psy_ctx_set_continuation(psy_ctx* ctx, int below) {
+    if (!ctx) return fail(PSY_ERR_INVALID, "psy_ctx_set_continuation: NULL context");
+    if (below < 0 || below > 64) return fail(PSY_ERR_INVALID, "psy_ctx_set_continuation: below must be 0..64");
+    ctx->cont_below = below;
     return PSY_OK;
 }
 
@@ -1147,8 +1154,10 @@ int psy_trace_rays_static(psy_ctx* ctx, const psy_scene* s, const float* rays, i
     // which the trace's first kernel clears
     uint32_t* queue = ctx->static_scratch;
     uint32_t* counters = ctx->static_scratch + 6 * (int64_t)PSY_STATIC_MAX_RAYS;
+    psy::TraceTuning tune = ctx->tuning(false, false);
+    tune.cont = nullptr;  // no continuation buffer in the fixed array: waves run their rays to the end
     int rc = launch_trace(s->dev, false, (flags & PSY_TRACE_OCCLUSION_ONLY) != 0, s->stack_need, rays, n, n,
-                          to_dev(hits), queue, counters, ctx->tuning(false, false), (hipStream_t)stream);
+                          to_dev(hits), queue, counters, tune, (hipStream_t)stream);
     return rc ? fail(rc, "psy_trace_rays_static: launch failed") : PSY_OK;
 }
 

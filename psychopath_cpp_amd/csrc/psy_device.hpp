@@ -157,6 +157,11 @@ struct TraceTuning {
     // handed to the third pass
     uint32_t wide_min = DEEP_WIDE_MIN;
     int32_t deep_budget = DEEP_BUDGET;
+    // the main pass's continuation buffer (cont_words words; NULL: none)
+    // and the live-lane count below which a wave continues its rays on it
+    uint32_t* cont = nullptr;
+    int64_t cont_words = 0;
+    int cont_below = 0;
     // hipEvent_t handles recorded around the passes when the context times
     // them: before the main pass, after it, after the second pass.
     void* ev[3] = {nullptr, nullptr, nullptr};

@@ -61,6 +61,14 @@ struct DBuf {
     int upload(const std::vector<T>& v) { return upload(v.data(), v.size() * sizeof(T)); }
 };
 
+// Words of a context's continuation buffer for streams of up to cap rays:
+// room for every continued ray of a call that continues waves with fewer
+// than 24 live lanes and the 16-entry stack (48 words an entry, 0.36
+// entries per ray), or fewer than 16 and the 32-entry stack (80 words,
+// 0.23 entries per ray); a call that could need more runs without
+// continuations.
+inline int64_t cont_area_words(int64_t cap) { return 19 * cap + 10240; }
+
 }  // namespace psy
 
 struct psy_ctx {
@@ -91,6 +99,7 @@ struct psy_ctx {
     int occupancy[2] = {0, 0};    // 16-entry stack
     int occupancy32[2] = {0, 0};  // 32-entry stack
     int trace_stack = 16;         // main-pass LDS stack entries
+    int cont_below = 24;          // main pass: continue a wave's rays once fewer lanes trace (0: never)
     int mode = PSY_TRACE_MODE_PLAIN;
     int refill_min = 16;
     int blocks_per_cu = 0;  // <= 0: from occupancy
@@ -108,6 +117,11 @@ struct psy_ctx {
         t.stack = trace_stack;
         t.wide_min = deep_wide_min;
         t.deep_budget = deep_budget;
+        if (queue.p) {
+            t.cont = (uint32_t*)queue.p + 6 * queue_cap;
+            t.cont_words = psy::cont_area_words(queue_cap);
+            t.cont_below = cont_below;
+        }
         if (timing && timed) {
             for (int k = 0; k < 3; ++k) {
                 hipEvent_t e = nullptr;

@@ -80,7 +80,7 @@ template <bool WORLD, int STACK, bool COHERENT>
 __global__ __launch_bounds__(TRACE_BLOCK, main_waves(WORLD, STACK, COHERENT)) void k_trace(
     DevScene s, const float* __restrict__ rays, int64_t n, int64_t rs, DevHits h, uint32_t flags,
     uint32_t* __restrict__ queue, uint32_t* __restrict__ heads, uint32_t* __restrict__ counters, DeepPool pool,
-    const uint32_t* __restrict__ n_dev) {
+    const uint32_t* __restrict__ n_dev, ContBuf cont) {
     __shared__ int32_t stk_ref[STACK][TRACE_BLOCK];
     __shared__ float stk_near[STACK][TRACE_BLOCK];
     const int tid = threadIdx.x;
@@ -90,8 +90,11 @@ __global__ __launch_bounds__(TRACE_BLOCK, main_waves(WORLD, STACK, COHERENT)) vo
     Lane L;
     lane_init<WORLD>(s, rays, rs, (uint32_t)i, flags, L);
     if constexpr (!COHERENT) {
-        while (!lane_step<WORLD, STACK>(s, L, stk_ref, stk_near, tid, pool)) {
-        }
+        // Rays of one wave end after very different numbers of steps; once
+        // fewer than cont.below lanes are still tracing, they are written
+        // to the continuation buffer and finished by k_trace_cont, packed
+        // 64 to a wave.
+        if (lane_run<WORLD, STACK>(s, L, stk_ref, stk_near, tid, pool, cont)) return;
     } else {
         bool done = false;
         for (;;) {
@@ -103,6 +106,26 @@ __global__ __launch_bounds__(TRACE_BLOCK, main_waves(WORLD, STACK, COHERENT)) vo
                 done = true;
         }
     }
+    if (lane_finish(s, L, h)) enqueue_second(queue, heads, counters, L);
+}
+
+// The continued rays of the previous launch (`in`), one per lane: entry e
+// restores its lane, which runs to the end or is continued again into
+// `out`.  The grid covers `in`'s capacity; lanes past its entry count exit
+// at once.
+template <bool WORLD, int STACK>
+__global__ __launch_bounds__(TRACE_BLOCK, main_waves(WORLD, STACK)) void k_trace_cont(
+    DevScene s, const float* __restrict__ rays, int64_t rs, DevHits h, uint32_t flags, uint32_t* __restrict__ queue,
+    uint32_t* __restrict__ heads, uint32_t* __restrict__ counters, DeepPool pool, ContBuf in, ContBuf out) {
+    __shared__ int32_t stk_ref[STACK][TRACE_BLOCK];
+    __shared__ float stk_near[STACK][TRACE_BLOCK];
+    const int tid = threadIdx.x;
+    const uint32_t e = blockIdx.x * TRACE_BLOCK + tid;
+    const uint32_t count = min(*in.count, in.cap);
+    if (e >= count) return;
+    Lane L;
+    cont_load<WORLD, STACK>(s, rays, rs, flags, in.c, in.cap, e, L, stk_ref, stk_near, tid);
+    if (lane_run<WORLD, STACK>(s, L, stk_ref, stk_near, tid, pool, out)) return;
     if (lane_finish(s, L, h)) enqueue_second(queue, heads, counters, L);
 }
 
@@ -221,6 +244,11 @@ __global__ __launch_bounds__(TRACE_BLOCK) void k_trace_coop(DevScene s, const fl
 
 // counters of a trace call: [0] second-pass queue, [1] third-pass queue,
 // [2..3] next ray of the persistent form (u64), [4] deep-node pool
+__global__ void k_zero_n(uint32_t* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
 __global__ void k_zero_counters(uint32_t* p) {
     if (threadIdx.x < 8 && blockIdx.x == 0) p[threadIdx.x] = 0u;
 }
@@ -722,6 +750,40 @@ static void launch_deep(const DevScene& s, const float* rays, int64_t n, int64_t
                        (const uint32_t*)counters, (const uint32_t*)coop_queue);
 }
 
+// The main pass one thread per ray, then the rays it continued (when the
+// call has a continuation buffer: tune.cont, cont_words(STACK) words per
+// entry).
+template <bool W, int STACK>
+static void launch_plain(const DevScene& s, const float* rays, int64_t n, int64_t rs, const DevHits& h,
+                         uint32_t flags, uint32_t* queue, uint32_t* heads, uint32_t* counters, const DeepPool& pool,
+                         const TraceTuning& tune, int64_t blocks, hipStream_t st) {
+    // Up to CONT_LEVELS continuation buffers, each after the previous in
+    // tune.cont, each counted by its own counter (counters[5 + level]).
+    // Fewer than `below` of a wave's 64 rays are continued, so a level
+    // needs at most this many entries for a launch of m rays.
+    const int below = tune.cont_below;
+    auto need = [&](int64_t m) { return (m * (int64_t)(below - 1) + 63) / 64 + 64; };
+    ContBuf lv[CONT_LEVELS + 1];
+    int levels = 0;
+    if (tune.cont && below > 1 && below <= 64) {
+        int64_t used = 0, m = n;
+        for (; levels < CONT_LEVELS; ++levels) {
+            const int64_t cap = need(m);
+            if ((used + cap) * cont_words(STACK) > tune.cont_words || cap > (int64_t)UINT32_MAX) break;
+            lv[levels] = ContBuf{tune.cont + used * cont_words(STACK), (uint32_t)cap, below, counters + 5 + levels};
+            used += cap;
+            m = cap;
+        }
+    }
+    lv[levels] = ContBuf{nullptr, 0u, 0, counters + 5};
+    hipLaunchKernelGGL((k_trace<W, STACK, false>), dim3((unsigned)blocks), dim3(TRACE_BLOCK), 0, st, s, rays, n, rs,
+                       h, flags, queue, heads, counters, pool, tune.n_dev, lv[0]);
+    for (int l = 0; l < levels; ++l)
+        hipLaunchKernelGGL((k_trace_cont<W, STACK>), dim3((unsigned)((lv[l].cap + TRACE_BLOCK - 1) / TRACE_BLOCK)),
+                           dim3(TRACE_BLOCK), 0, st, s, rays, rs, h, flags, queue, heads, counters, pool, lv[l],
+                           lv[l + 1]);
+}
+
 template <bool W>
 static int launch_trace_w(const DevScene& s, int stack_need, const float* rays, int64_t n, int64_t rs, const DevHits& h,
                           uint32_t flags, uint32_t* queue, uint32_t* counters, const TraceTuning& tune,
@@ -733,6 +795,7 @@ static int launch_trace_w(const DevScene& s, int stack_need, const float* rays, 
     const DeepPool pool{queue + 3 * n, counters + 4, (uint32_t)std::min<int64_t>(n, (int64_t)UINT32_MAX - 2)};
     hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, st, counters);
     if (tune.ev[0]) (void)hipEventRecord((hipEvent_t)tune.ev[0], st);
+    const ContBuf no_cont{nullptr, 0u, 0, counters + 5};
     const int64_t plain_blocks = grid_for(n, TRACE_BLOCK);
     const int64_t persist_blocks = (int64_t)(tune.stack > 16 ? tune.persist_blocks32 : tune.persist_blocks);
     const int refill = std::max(1, std::min(64, tune.refill_min));
@@ -747,17 +810,15 @@ static int launch_trace_w(const DevScene& s, int stack_need, const float* rays, 
     } else if (tune.stack > 16) {
         if (tune.plain_coherent)
             hipLaunchKernelGGL((k_trace<W, 32, true>), dim3((unsigned)plain_blocks), dim3(TRACE_BLOCK), 0, st, s, rays,
-                               n, rs, h, flags, queue, heads, counters, pool, tune.n_dev);
+                               n, rs, h, flags, queue, heads, counters, pool, tune.n_dev, no_cont);
         else
-            hipLaunchKernelGGL((k_trace<W, 32, false>), dim3((unsigned)plain_blocks), dim3(TRACE_BLOCK), 0, st, s, rays,
-                               n, rs, h, flags, queue, heads, counters, pool, tune.n_dev);
+            launch_plain<W, 32>(s, rays, n, rs, h, flags, queue, heads, counters, pool, tune, plain_blocks, st);
     } else {
         if (tune.plain_coherent)
             hipLaunchKernelGGL((k_trace<W, MAIN_STACK, true>), dim3((unsigned)plain_blocks), dim3(TRACE_BLOCK), 0, st, s,
-                               rays, n, rs, h, flags, queue, heads, counters, pool, tune.n_dev);
+                               rays, n, rs, h, flags, queue, heads, counters, pool, tune.n_dev, no_cont);
         else
-            hipLaunchKernelGGL((k_trace<W, MAIN_STACK, false>), dim3((unsigned)plain_blocks), dim3(TRACE_BLOCK), 0, st,
-                               s, rays, n, rs, h, flags, queue, heads, counters, pool, tune.n_dev);
+            launch_plain<W, MAIN_STACK>(s, rays, n, rs, h, flags, queue, heads, counters, pool, tune, plain_blocks, st);
     }
     if (tune.ev[1]) (void)hipEventRecord((hipEvent_t)tune.ev[1], st);
     if (stack_need <= 16) launch_deep<W, 16>(s, rays, n, rs, h, flags, queue, heads, pool.e, counters, tune, st);
@@ -791,10 +852,6 @@ int trace_persist_occupancy(bool world, int stack, int* blocks_per_cu) {
     return e == hipSuccess ? 0 : PSY_ERR_HIP;
 }
 
-__global__ void k_zero_n(uint32_t* p, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0u;
-}
 
 int launch_zero_u32(uint32_t* p, int n, hipStream_t st) {
     if (n <= 0) return 0;

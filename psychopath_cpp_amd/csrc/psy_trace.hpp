@@ -991,6 +991,30 @@ PSY_DEV void lane_start(const DevScene& s, Lane& L, float max_t, uint32_t i) {
     }
 }
 
+// The patch-space ray of a walk through leaf I's split tree (world rays:
+// through the leaf's transform at the ray's time) and its footprint's flip
+// distances; false when its inverse direction is not finite.
+template <bool WORLD>
+PSY_DEV bool walk_ray(const DevScene& s, Lane& L, const DevInstance& I) {
+    bool finite = true;
+    if (WORLD) {
+        L.wo = L.r.o;
+        L.wdinv = L.r.dinv;
+        L.ww = L.r.w;
+        if (I.xf_count > 0) {
+            const M44 x = m_lerp_seq(L.r.time, s.xf + I.xf_offset, I.xf_count);
+            L.wo = m_pos(x, L.r.o);
+            const V3 d = m_dir(x, L.r.d);
+            L.wdinv = divv(splat(1.0f), d);
+            finite = finite3(L.wdinv);
+            L.ww = ray_widths(L.wo, d, m_dir(x, L.r.odx), m_dir(x, L.r.ody), m_dir(x, L.r.ddx), m_dir(x, L.r.ddy));
+        }
+    }
+    L.wfx = flip_x(walk_w<WORLD>(L));
+    L.wfy = flip_y(walk_w<WORLD>(L));
+    return finite;
+}
+
 // One unit of work; true when the ray is finished (or must be retraced).
 //   walking : expand heap node k of the patch's cached split tree -- test
 //             its two children (k = 0 stands for the root alone), accept
@@ -1144,27 +1168,10 @@ PSY_DEV bool lane_step(const DevScene& s, Lane& L, int32_t (*stk_ref)[TRACE_BLOC
         L.k = 0;
         L.depth = -1;
         L.trail = 0;
-        bool finite = true;
-        if (WORLD) {
-            L.wo = L.r.o;
-            L.wdinv = L.r.dinv;
-            L.ww = L.r.w;
-            if (I.xf_count > 0) {
-                const M44 x = m_lerp_seq(L.r.time, s.xf + I.xf_offset, I.xf_count);
-                L.wo = m_pos(x, L.r.o);
-                const V3 d = m_dir(x, L.r.d);
-                L.wdinv = divv(splat(1.0f), d);
-                finite = finite3(L.wdinv);
-                L.ww = ray_widths(L.wo, d, m_dir(x, L.r.odx), m_dir(x, L.r.ody), m_dir(x, L.r.ddx),
-                                  m_dir(x, L.r.ddy));
-            }
-        }
-        if (!finite) {
+        if (!walk_ray<WORLD>(s, L, I)) {
             L.state |= LANE_RETRACE;
             return true;
         }
-        L.wfx = flip_x(walk_w<WORLD>(L));
-        L.wfy = flip_y(walk_w<WORLD>(L));
         L.state |= LANE_WALKING;
         return false;
     }
@@ -1204,6 +1211,119 @@ PSY_DEV void enqueue_second(uint32_t* __restrict__ queue, uint32_t* __restrict__
     const bool listed = (L.state & (LANE_RETRACE | LANE_DEEP_MANY)) == 0;
     queue[slot] = L.idx | (listed ? QUEUE_RESUME : 0u);
     heads[slot] = L.deep_head;
+}
+
+// Continuations.  Once fewer than a set number of a main-pass wave's lanes
+// are still tracing, the wave writes those lanes' traversal state to a
+// continuation buffer and ends; a follow-on launch packs the continued
+// rays 64 to a wave and finishes them.  A lane's state is CONT_FIXED words
+// (best hit, ray index, walk state, BVH next / stack depth, flags, deep
+// list) plus its stack entries (ref, near); the ray itself is reloaded from
+// the stream and its walk ray re-derived from the walked leaf, so each ray
+// takes the same steps as it would have in one launch.  Entry e's word w is
+// at cont[w * cap + e].
+constexpr int CONT_FIXED = 16;
+// One level: the continued rays' own launch runs them to the end (a second
+// level, continuing that launch's tails again, was slower: 2.21 against
+// 2.01 ms on the 2^22-ray heightfield stream).
+#ifndef PSY_CONT_LEVELS
+#define PSY_CONT_LEVELS 1
+#endif
+constexpr int CONT_LEVELS = PSY_CONT_LEVELS;  // counted in the trace's counters[5 + level]
+// A continuation buffer: `cap` entries of cont_words(stack) words; a wave
+// continues its lanes into it once fewer than `below` of them are tracing
+// (0: never); *count counts its entries, taken a wave at a time.  (Packing
+// the continued rays in wave order instead, through a prefix sum of
+// per-wave counts, left the follow-on launch's L2 hit rate as it was and
+// cost the scan: 2.07 against 1.99 ms on the 2^22-ray heightfield stream.)
+struct ContBuf {
+    uint32_t* c;
+    uint32_t cap;
+    int below;
+    uint32_t* count;
+};
+
+constexpr int cont_words(int stack) { return CONT_FIXED + 2 * stack; }
+
+template <int STACK>
+PSY_DEV void cont_save(uint32_t* __restrict__ cont, uint32_t cap, uint32_t e, const Lane& L,
+                       int32_t (*stk_ref)[TRACE_BLOCK], float (*stk_near)[TRACE_BLOCK], int tid) {
+    uint32_t* c = cont + e;
+    const size_t m = cap;
+    c[0 * m] = f2u(L.b.t);
+    c[1 * m] = L.b.inst;
+    c[2 * m] = f2u(L.b.u);
+    c[3 * m] = f2u(L.b.v);
+    c[4 * m] = f2u(L.b.aux);
+    c[5 * m] = L.idx;
+    c[6 * m] = L.slot;
+    c[7 * m] = L.winst;
+    c[8 * m] = L.k;
+    c[9 * m] = L.trail;
+    c[10 * m] = (uint32_t)L.levels;
+    c[11 * m] = (uint32_t)L.depth;
+    c[12 * m] = (uint32_t)L.next;
+    c[13 * m] = (uint32_t)L.sp;
+    c[14 * m] = L.state;
+    c[15 * m] = L.deep_head;
+    for (int j = 0; j < L.sp; ++j) {
+        c[(CONT_FIXED + 2 * j) * m] = (uint32_t)stk_ref[j][tid];
+        c[(CONT_FIXED + 2 * j + 1) * m] = f2u(stk_near[j][tid]);
+    }
+}
+
+template <bool WORLD, int STACK>
+PSY_DEV void cont_load(const DevScene& s, const float* __restrict__ rays, int64_t rs, uint32_t flags,
+                       const uint32_t* __restrict__ cont, uint32_t cap, uint32_t e, Lane& L,
+                       int32_t (*stk_ref)[TRACE_BLOCK], float (*stk_near)[TRACE_BLOCK], int tid) {
+    const uint32_t* c = cont + e;
+    const size_t m = cap;
+    lane_init<WORLD>(s, rays, rs, c[5 * m], flags, L);
+    L.b.t = u2f(c[0 * m]);
+    L.b.inst = c[1 * m];
+    L.b.u = u2f(c[2 * m]);
+    L.b.v = u2f(c[3 * m]);
+    L.b.aux = u2f(c[4 * m]);
+    L.slot = c[6 * m];
+    L.winst = c[7 * m];
+    L.k = c[8 * m];
+    L.trail = c[9 * m];
+    L.levels = (int)c[10 * m];
+    L.depth = (int)c[11 * m];
+    L.next = (int32_t)c[12 * m];
+    L.sp = (int)c[13 * m];
+    L.state = c[14 * m];
+    L.deep_head = c[15 * m];
+    for (int j = 0; j < L.sp; ++j) {
+        stk_ref[j][tid] = (int32_t)c[(CONT_FIXED + 2 * j) * m];
+        stk_near[j][tid] = u2f(c[(CONT_FIXED + 2 * j + 1) * m]);
+    }
+    // a walking lane's leaf gave a finite walk ray when the walk started
+    if (L.state & LANE_WALKING) (void)walk_ray<WORLD>(s, L, s.inst[L.winst]);
+}
+
+// Runs lane L to the end of its ray or, once fewer than `out.below` of the
+// wave's lanes are left, writes it to `out` (all of the wave's remaining
+// lanes together, if the buffer has room for them).  True when continued.
+template <bool WORLD, int STACK>
+PSY_DEV bool lane_run(const DevScene& s, Lane& L, int32_t (*stk_ref)[TRACE_BLOCK],
+                      float (*stk_near)[TRACE_BLOCK], int tid, const DeepPool& pool, const ContBuf& out) {
+    int below = out.below;  // wave-uniform
+    for (;;) {
+        if (lane_step<WORLD, STACK>(s, L, stk_ref, stk_near, tid, pool)) return false;
+#if defined(__HIP_DEVICE_COMPILE__)
+        const int live = __popcll(__ballot(1));
+        if (live < below) {
+            const uint32_t e = wave_alloc(out.count, 1u);
+            const uint32_t first = (uint32_t)__shfl((int)e, __ffsll((long long)__ballot(1)) - 1);
+            if ((uint64_t)first + (uint64_t)live <= (uint64_t)out.cap) {
+                cont_save<STACK>(out.c, out.cap, e, L, stk_ref, stk_near, tid);
+                return true;
+            }
+            below = 0;  // no room: these lanes finish here
+        }
+#endif
+    }
 }
 
 // Second pass for a ray the main pass left with a list of cache nodes to

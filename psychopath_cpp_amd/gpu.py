@@ -53,6 +53,7 @@ def lib():
     L.psy_ctx_set_trace_timing.argtypes = [vp, C.c_int]
     L.psy_ctx_set_trace_stack.argtypes = [vp, C.c_int]
     L.psy_ctx_set_second_pass.argtypes = [vp, u32, i32]
+    L.psy_ctx_set_continuation.argtypes = [vp, C.c_int]
     L.psy_ctx_set_stream_steps.argtypes = [vp, C.c_int]
     L.psy_render_autotune.argtypes = [vp, vp, C.POINTER(abi.RenderParams), i64, C.POINTER(abi.RenderTuning)]
     L.psy_ctx_trace_timing.argtypes = [vp, vp, vp, vp]
@@ -127,6 +128,11 @@ class Context:
         (0: always wide, 0xffffffff: always narrow) and its node budget per ray before the third pass."""
         check(lib().psy_ctx_set_second_pass(self.handle, int(wide_min) & 0xFFFFFFFF, int(budget)),
               "psy_ctx_set_second_pass")
+
+    def set_continuation(self, below=24):
+        """psy_ctx_set_continuation: a main-pass wave hands its rays to a follow-on launch once fewer than
+        `below` of them are still tracing (0: never)."""
+        check(lib().psy_ctx_set_continuation(self.handle, int(below)), "psy_ctx_set_continuation")
 
     def set_trace_timing(self, enable=True):
         """psy_ctx_set_trace_timing: HIP events around each trace call's passes."""

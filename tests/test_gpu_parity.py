@@ -328,6 +328,50 @@ def test_gpu_trace_modes_identical(gpu_ctx, c1):
         ctx.close()
 
 
+def test_gpu_continuations_identical(gpu_ctx, c1):
+    """Waves that hand their last rays to the follow-on launch give the same
+    bits as waves that run every ray to the end: the C1 stream (object and
+    world rays), the instanced hierarchy with moving transforms (walks
+    re-derive their patch-space ray on restore) and the sphere scene's
+    camera rays at 1024 x 768 (rays handed on to the second pass from both
+    launches), at several thresholds and both stack depths."""
+    sc, g, o = c1
+    W, H, spp, seed = 256, 256, 2, 5
+    ctx = gpu.Context(0)
+    gi = gs = None
+    try:
+        w, _ = gpu.camera_rays(ctx, g, W, H, spp, seed)
+        ctx.set_continuation(0)
+        hp = gpu.trace_world(ctx, g, w)
+        b, valid = gpu.bounce_rays(ctx, g, w, hp, W, spp, seed, 9, 10)
+        stream = torch.cat([w, b[:, valid.bool()]], dim=1).contiguous()
+        rays = gpu.world_to_rays(ctx, stream)
+        ref = gpu.trace_rays(ctx, g, rays)
+        ref_w = gpu.trace_world(ctx, g, stream)
+        gi = gpu.GpuScene(ctx, _instanced_scene(True))
+        wi, _ = gpu.camera_rays(ctx, gi, 192, 128, 2, 7)
+        ref_i = gpu.trace_world(ctx, gi, wi)
+        gs = gpu.GpuScene(ctx, configs.sphere_scene())
+        ws, _ = gpu.camera_rays(ctx, gs, 1024, 768, 1, 1)
+        ref_s = gpu.trace_world(ctx, gs, ws)
+        second = ctx.pass_counts()[0]
+        for stack in (16, 32):
+            ctx.set_trace_stack(stack)
+            for below in (1, 16, 24, 33, 64):
+                say("continuation below", below, "stack", stack)
+                ctx.set_continuation(below)
+                assert _hits_equal(gpu.trace_rays(ctx, g, rays), ref), (below, stack)
+                assert _hits_equal(gpu.trace_world(ctx, g, stream), ref_w), (below, stack)
+                assert _hits_equal(gpu.trace_world(ctx, gi, wi), ref_i), (below, stack)
+                assert _hits_equal(gpu.trace_world(ctx, gs, ws), ref_s), (below, stack)
+                assert ctx.pass_counts()[0] == second
+    finally:
+        for x in (gi, gs):
+            if x is not None:
+                x.close()
+        ctx.close()
+
+
 def test_gpu_c1_full_stream(c1, oracle):
     """The benchmark's full C1 stream (2^22 rays: 2^21 primary at 1024 x
     1024 x 2 spp plus 2^21 first-bounce rays): every launch form of the main
@@ -348,6 +392,9 @@ def test_gpu_c1_full_stream(c1, oracle):
         for mode in ((True, 16, False), (True, 8, True), (False, 16, True)):
             ctx.set_trace_mode(persistent=mode[0], refill_min=mode[1], coherent=mode[2])
             assert _hits_equal(gpu.trace_rays(ctx, g, rays), ref), mode
+        ctx.set_trace_mode(persistent=False)
+        ctx.set_continuation(0)
+        assert _hits_equal(gpu.trace_rays(ctx, g, rays), ref), "without continuations"
         rng = np.random.default_rng(29)
         pick = np.sort(rng.choice(n, 8192, replace=False))
         w = oracle.wrays_from_planes(wrays[:, torch.from_numpy(pick).cuda()].cpu().numpy())

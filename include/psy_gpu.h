@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define PSY_ABI_VERSION 1
+#define PSY_ABI_VERSION 2
 
 /* ---- status codes ---------------------------------------------------- */
 enum psy_status {
@@ -68,11 +68,16 @@ typedef struct psy_instance_desc {
     int32_t shader_index;     /* -1 = none                                   */
 } psy_instance_desc;
 
-/* Surface shaders (shading/surface_shader.hpp:28-53). */
-enum psy_shader_kind { PSY_SHADER_LAMBERT = 1, PSY_SHADER_EMIT = 2 };
+/* Surface shaders (shading/surface_shader.hpp:28-71): EmitShader,
+ * LambertShader, and GTRShader, the Disney GTR microfacet closure
+ * (shading/surface_closure.hpp:339-608) with its roughness (0 gives a
+ * perfect mirror, a delta closure), tail shape and Fresnel blend; the
+ * three GTR fields are ignored for the other kinds. */
+enum psy_shader_kind { PSY_SHADER_LAMBERT = 1, PSY_SHADER_EMIT = 2, PSY_SHADER_GTR = 3 };
 typedef struct psy_shader_desc {
     int32_t kind;
     float color[3];  /* linear RGB with white point rgb<1,1,1> */
+    float roughness, tail_shape, fresnel;  /* PSY_SHADER_GTR */
 } psy_shader_desc;
 
 /* One Assembly (scene/assembly.hpp:74-330).  Child assemblies are indices
@@ -99,7 +104,16 @@ typedef struct psy_scene_desc {
     float background[3];
     float dice_rate;       /* Config::dice_rate (config.cpp:7), reference default 0.25 */
     float min_upoly_size;  /* Config::min_upoly_size (config.cpp:8), default 1e-5       */
+    uint32_t flags;        /* PSY_SCENE_* */
 } psy_scene_desc;
+/* RectangleLight::intersect_ray tests the hit's y against half the width
+ * (lights/rectangle_light.hpp:127), so a light taller than wide loses the
+ * part of its top beyond half its width (the light samples still cover it),
+ * and one wider than tall is hit beyond half its height wherever the
+ * light's own bounds (which stop there) are not tested first, as when it is
+ * a scene's only instance.  Scenes keep that behaviour unless they set this
+ * flag, which tests y against half the height. */
+#define PSY_SCENE_RECT_LIGHT_FIX 1u
 
 /* ---- scene files (.psy) ------------------------------------------------ */
 /* Host-only parsing of the reference's scene format into the description
@@ -112,8 +126,8 @@ typedef struct psy_scene_desc {
  * shader names up the parent chain); what it skips with a message (unknown
  * node types, instances of unknown data, unknown shader types) is skipped
  * and reported by psy_parsed_scene_warnings.  Scenes the device path cannot
- * render fail with PSY_ERR_UNSUPPORTED: an instance bound to a GTR shader,
- * lights whose motion sample lists differ in length (other than 1). */
+ * render fail with PSY_ERR_UNSUPPORTED: lights whose motion sample lists
+ * differ in length (other than 1). */
 typedef struct psy_parsed_scene psy_parsed_scene;
 typedef struct psy_render_settings {
     int32_t width, height;   /* RenderSettings Resolution (default 1 1)      */

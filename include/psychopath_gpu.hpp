@@ -39,6 +39,7 @@
 #define PSYCHOPATH_GPU_HPP
 
 #include <array>
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -207,6 +208,16 @@ public:
     int kind() const override { return PSY_SHADER_EMIT; }
 };
 
+// GTRShader (shading/surface_shader.hpp:55-71): the GTR microfacet closure;
+// roughness 0 is a perfect mirror.
+class GTRShader : public SurfaceShader {
+public:
+    GTRShader(Color c, float roughness_, float tail_shape_, float fresnel_)
+        : SurfaceShader(c), roughness(roughness_), tail_shape(tail_shape_), fresnel(fresnel_) {}
+    int kind() const override { return PSY_SHADER_GTR; }
+    float roughness, tail_shape, fresnel;
+};
+
 // ---- assembly ---------------------------------------------------------------
 class Assembly {
 public:
@@ -317,6 +328,9 @@ struct Scene {
     float dice_rate = 0.25f;        // Config::dice_rate (config.cpp:7)
     float min_upoly_size = 1e-5f;   // Config::min_upoly_size (config.cpp:8)
     int dice_depth = -1;            // dice cache depth; < 0: from the scene
+    // rectangle lights test y against their height (PSY_SCENE_RECT_LIGHT_FIX;
+    // the reference tests it against the width)
+    bool rect_light_fix = false;
 
     Scene() : root(new Assembly()) {}
     ~Scene() {
@@ -351,6 +365,11 @@ struct Scene {
                 sd.color[0] = sh->col.r;
                 sd.color[1] = sh->col.g;
                 sd.color[2] = sh->col.b;
+                if (const GTRShader* g = dynamic_cast<const GTRShader*>(sh.get())) {
+                    sd.roughness = g->roughness;
+                    sd.tail_shape = g->tail_shape;
+                    sd.fresnel = g->fresnel;
+                }
                 D->shaders.push_back(sd);
             }
             D->children.emplace_back();
@@ -429,6 +448,7 @@ struct Scene {
         d.background[2] = background_color.b;
         d.dice_rate = dice_rate;
         d.min_upoly_size = min_upoly_size;
+        d.flags = rect_light_fix ? PSY_SCENE_RECT_LIGHT_FIX : 0u;
         return D;
     }
 
@@ -441,9 +461,11 @@ struct Scene {
             handle = nullptr;
         }
         check(psy_scene_create(Device::get().ctx(), &D->desc, dice_depth, &handle), "psy_scene_create");
+        shader_table = D->shaders;
     }
 
     psy_scene* handle = nullptr;
+    std::vector<psy_shader_desc> shader_table;  // the finalized scene's shaders, by index
 };
 
 // ---- tracing ----------------------------------------------------------------
@@ -477,6 +499,7 @@ inline Vec3 v3(float x, float y, float z) { return Vec3{x, y, z}; }
 inline float dot(const Vec3& a, const Vec3& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 inline Vec3 scale(const Vec3& a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
 inline Vec3 add(const Vec3& a, const Vec3& b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline Vec3 sub(const Vec3& a, const Vec3& b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
 inline Vec3 cross(const Vec3& a, const Vec3& b) {
     return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
@@ -574,20 +597,124 @@ public:
     const Color& emitted_color() const { return col_; }
 };
 
+// GTRClosure (shading/surface_closure.hpp:339-608), validated as the
+// reference's constructor does; evaluate() applies Schlick's Fresnel per RGB
+// channel (the reference applies it per wavelength of the spectral colour).
+class GTRClosure final : public SurfaceClosure {
+public:
+    GTRClosure(Color c = Color(1.0f), float roughness = 0.05f, float tail_shape = 2.0f, float fresnel = 1.0f)
+        : SurfaceClosure(c), rough_(roughness), tail_(tail_shape), fresnel_(fresnel) {
+        rough_ = std::max(0.0f, std::min(0.9999f, rough_));
+        tail_ = std::max(0.0001f, tail_);
+        if (rough_ < 0.000244140625f) rough_ = 0.0f;
+        if (std::abs(tail_ - 1.0f) < 0.0001f) tail_ = 1.0f + 0.0001f;
+        norm_ = normalization(rough_, tail_);
+    }
+    bool is_delta() const override { return rough_ == 0.0f; }
+    bool is_emitter() const override { return false; }
+    float roughness() const { return rough_; }
+    float tail_shape() const { return tail_; }
+    float fresnel() const { return fresnel_; }
+
+    Color evaluate(const Vec3& in, const Vec3& out, const DifferentialGeometry& geo) const {
+        Vec3 nn = detail::normalized(geo.n);
+        const Vec3 aa = detail::scale(detail::normalized(in), -1.0f);
+        const Vec3 bb = detail::normalized(out);
+        const Vec3 hh = detail::normalized(detail::add(aa, bb));
+        if (detail::dot(nn, hh) < 0.0f) nn = detail::scale(nn, -1.0f);
+        const float na = clamp(detail::dot(nn, aa)), nb = clamp(detail::dot(nn, bb));
+        const float ha = clamp(detail::dot(hh, aa)), hb = clamp(detail::dot(hh, bb));
+        const float nh = clamp(detail::dot(nn, hh));
+        const float c[3] = {fres(col_.r, hb), fres(col_.g, hb), fres(col_.b, hb)};
+        if (rough_ == 0.0f) return Color(c[0], c[1], c[2]);
+        const float r2 = rough_ * rough_;
+        float D = 1.0f;
+        if (nh > 0.0f) D = norm_ / std::pow(1 + ((r2 - 1) * (nh * nh)), tail_);
+        const float f = (D * g1(na, ha, r2) * g1(nb, hb, r2)) * (float)(1.0 / detail::PI);
+        return Color(c[0] * f, c[1] * f, c[2] * f);
+    }
+    float sample_pdf(const Vec3& in, const Vec3& out, const DifferentialGeometry& geo) const {
+        Vec3 nn = detail::normalized(geo.n);
+        const Vec3 aa = detail::scale(detail::normalized(in), -1.0f);
+        const Vec3 bb = detail::normalized(out);
+        const Vec3 hh = detail::normalized(detail::scale(detail::add(aa, bb), 0.5f));
+        if (detail::dot(nn, hh) < 0.0f) nn = detail::scale(nn, -1.0f);
+        const float nh = clamp(detail::dot(nn, hh));
+        const float r2 = rough_ * rough_;
+        const float D = nh > 0.0f ? norm_ / std::pow(1 + ((r2 - 1) * (nh * nh)), tail_) : 0.0f;
+        return D * (float)(1.0 / detail::PI);
+    }
+    // sample(): a half vector from the distribution, `in` mirrored about it
+    void sample(const Vec3& in, const DifferentialGeometry& geo, float si, float sj, Vec3* out, Color* filter,
+                float* pdf) const {
+        Vec3 nn = detail::normalized(geo.n);
+        if (detail::dot(nn, in) > 0.0f) nn = detail::scale(nn, -1.0f);
+        const float r2 = rough_ * rough_;
+        const float top = 1.0f - std::pow((std::pow(r2, 1.0f - tail_) * (1.0f - si)) + si, 1.0f / (1.0f - tail_));
+        const float ct = std::sqrt(top / (1.0f - r2));
+        const float st = std::sqrt(1.0f - (ct * ct));
+        const float angle = (float)((double)sj * detail::PI * 2);
+        Vec3 tx, ty;
+        if (std::fabs(nn.x) > std::fabs(nn.y)) {
+            const float il = 1.f / std::sqrt(nn.x * nn.x + nn.z * nn.z);
+            tx = detail::v3(-nn.z * il, 0.f, nn.x * il);
+        } else {
+            const float il = 1.f / std::sqrt(nn.y * nn.y + nn.z * nn.z);
+            tx = detail::v3(0.f, nn.z * il, -nn.y * il);
+        }
+        ty = detail::cross(nn, tx);
+        const Vec3 h = detail::normalized(detail::add(
+            detail::add(detail::scale(tx, std::cos(angle) * st), detail::scale(ty, std::sin(angle) * st)),
+            detail::scale(nn, ct)));
+        *out = detail::sub(in, detail::scale(detail::scale(h, 2.0f), detail::dot(in, h)));
+        *pdf = sample_pdf(in, *out, geo);
+        *filter = evaluate(in, *out, geo);
+    }
+
+private:
+    static float clamp(float v) { return std::min(1.0f, std::max(-1.0f, v)); }
+    static float normalization(float r, float t) {
+        const float r2 = r * r;
+        const float top = (t - 1.0f) * (r2 - 1);
+        return (float)((double)top / (detail::PI * (double)(1.0f - std::pow(r2, 1.0f - t))));
+    }
+    // lerp(1 - fresnel, schlick_fresnel_from_fac(c, hb), c)
+    float fres(float c, float hb) const {
+        const float c1 = 1.0f - hb, c2 = c1 * c1;
+        const float s = c + ((1.0f - c) * c1 * c2 * c2);
+        const float a = 1.0f - fresnel_;
+        return (float)((double)s * (1.0 - (double)a) + (double)(c * a));
+    }
+    static float g1(float nx, float hx, float r2) {
+        const float nx2 = nx * nx;
+        const float a = r2 * std::sqrt((1.0f - nx2) / nx2);
+        const float b = (std::sqrt(1.0f + a * a) - 1.0f) * 0.5f;
+        return ((hx * nx) > 0.0f ? 1.0f : 0.0f) / (1.0f + b);
+    }
+    float rough_, tail_, fresnel_, norm_;
+};
+
 // SurfaceClosureUnion (shading/closure_union.hpp:16-43): storage for one
-// closure of either kind; get() is null until init().
+// closure of any kind; get() is null until init().
 class SurfaceClosureUnion {
 public:
     void init(const LambertClosure& c) { lambert_ = c; kind_ = 1; }
     void init(const EmitClosure& c) { emit_ = c; kind_ = 2; }
-    SurfaceClosure* get() { return kind_ == 1 ? (SurfaceClosure*)&lambert_ : kind_ == 2 ? (SurfaceClosure*)&emit_ : nullptr; }
+    void init(const GTRClosure& c) { gtr_ = c; kind_ = 3; }
+    SurfaceClosure* get() { return const_cast<SurfaceClosure*>(static_cast<const SurfaceClosureUnion*>(this)->get()); }
     const SurfaceClosure* get() const {
-        return kind_ == 1 ? (const SurfaceClosure*)&lambert_ : kind_ == 2 ? (const SurfaceClosure*)&emit_ : nullptr;
+        switch (kind_) {
+            case 1: return &lambert_;
+            case 2: return &emit_;
+            case 3: return &gtr_;
+            default: return nullptr;
+        }
     }
 
 private:
     LambertClosure lambert_;
     EmitClosure emit_;
+    GTRClosure gtr_;
     int kind_ = 0;
 };
 
@@ -682,7 +809,12 @@ public:
             x.light_pdf = at(37, i);
             x.backfacing = at_u(38, i) != 0;
             const Color col(at(40, i), at(41, i), at(42, i));
-            if (at_u(39, i) == PSY_SHADER_LAMBERT) x.surface_closure.init(LambertClosure(col));
+            const uint32_t kind = at_u(39, i);
+            const int32_t sh = (int32_t)at_u(43, i);
+            if (kind == PSY_SHADER_GTR && sh >= 0 && (size_t)sh < scene_->shader_table.size()) {
+                const psy_shader_desc& sd = scene_->shader_table[(size_t)sh];
+                x.surface_closure.init(GTRClosure(col, sd.roughness, sd.tail_shape, sd.fresnel));
+            } else if (kind == PSY_SHADER_LAMBERT) x.surface_closure.init(LambertClosure(col));
             else x.surface_closure.init(EmitClosure(col));
             x.shader = (int32_t)at_u(43, i);
         }

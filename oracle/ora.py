@@ -21,7 +21,7 @@ INTER_DTYPE = np.dtype([("hit", "<i4"), ("backfacing", "<i4"), ("id", "<u8"), ("
                         ("p", "<f4", 3), ("dpdu", "<f4", 3), ("dpdv", "<f4", 3), ("n", "<f4", 3),
                         ("dndu", "<f4", 3), ("dndv", "<f4", 3), ("offset", "<f4", 3),
                         ("closure", "<i4"), ("closure_col", "<f4", 3), ("closure_prob", "<f4"),
-                        ("patch_dim", "<f4")], align=True)
+                        ("patch_dim", "<f4"), ("closure_param", "<f4", 3)], align=True)
 RAY_DTYPE = np.dtype([("o", "<f4", 3), ("d", "<f4", 3), ("max_t", "<f4"), ("time", "<f4"),
                       ("owx", "<f4"), ("owy", "<f4"), ("dwx", "<f4"), ("dwy", "<f4"), ("fwx", "<f4"),
                       ("fwy", "<f4"), ("flags", "<u4")])
@@ -117,6 +117,16 @@ def lib():
         L.ora_kat_sphere_lambert.restype = f
         L.ora_kat_sphere_lambert.argtypes = [f, f]
         L.ora_kat_lambert.argtypes = [f, vp, vp, vp, vp, vp]
+        L.ora_kat_gtr_params.argtypes = [vp, vp]
+        L.ora_kat_gtr_dist.restype = f
+        L.ora_kat_gtr_dist.argtypes = [vp, f]
+        L.ora_kat_gtr_half_theta.restype = f
+        L.ora_kat_gtr_half_theta.argtypes = [vp, f]
+        L.ora_kat_gtr.argtypes = [f, vp, vp, vp, vp, vp, vp]
+        L.ora_kat_gtr_sample.argtypes = [vp, vp, vp, f, f, vp]
+        L.ora_kat_gtr_estimate.restype = f
+        L.ora_kat_gtr_estimate.argtypes = [vp, vp, vp, f, vp]
+        L.ora_kat_axis_angle.argtypes = [vp, f, vp]
         L.ora_kat_light_select.restype = C.c_int
         L.ora_kat_light_select.argtypes = [vp, vp, vp, vp, f, f, f, f, vp, vp, vp, vp]
         L.ora_kat_total_lights.restype = i64

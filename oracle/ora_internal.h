@@ -88,6 +88,7 @@ typedef struct ora_scene {
     int n_fd; float* focus;
     float background[3];
     float dice_rate, min_upoly;
+    uint32_t flags;  /* PSY_SCENE_* */
 } ora_scene;
 
 /* element_id_bits (scene/assembly.hpp:245-254) */

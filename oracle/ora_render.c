@@ -206,6 +206,14 @@ static void coordinate_system_from_vec3(v3 v1, v3* v2, v3* v3o) {
     *v3o = v3cross(v1, *v2);
 }
 
+/* zup_to_vec (utils/utils.hpp:239-248) */
+static v3 zup_to_vec(v3 from, v3 toz) {
+    toz = v3norm(toz);
+    v3 tox, toy;
+    coordinate_system_from_vec3(toz, &tox, &toy);
+    return v3add(v3add(v3scale(tox, from.x), v3scale(toy, from.y)), v3scale(toz, from.z));
+}
+
 static v3 uniform_sample_sphere(float u, float v) {
     const float z = 1.f - (2.f * u);
     const float r = sqrtf(smax(0.f, 1.f - z * z));
@@ -291,6 +299,7 @@ static float lambert_estimate(v3 in, v3 out, float cos_theta, v3 nor) {
 }
 
 /* ---- light tree sampling ------------------------------------------------- */
+struct gtr_closure;
 typedef struct {
     float n, u, v;
     v3 pos, nor, d;
@@ -300,7 +309,10 @@ typedef struct {
     v3 to_light;
     float spec[NSPEC];
     float selection_pdf, light_sample_pdf;
+    const struct gtr_closure* gtr;  /* the query's closure: GTR, or Lambert when NULL */
 } lquery;
+/* LightQuery::bsdf->estimate_eval_over_solid_angle */
+static float bsdf_estimate(const lquery* q, v3 d, float cos_theta_max);
 
 static float node_prob(const ora_assembly* a, const lquery* q, size_t index) {
     const ora_ltree_node* nd = &a->lnodes[index];
@@ -317,7 +329,7 @@ static float node_prob(const ora_assembly* a, const lquery* q, size_t index) {
         const float s2 = smin(1.0f, r2 / dist2);
         cos_theta_max = sqrtf(1.0f - s2);
     }
-    const float approx = smax(0.0f, lambert_estimate(q->d, d, cos_theta_max, q->nor));
+    const float approx = smax(0.0f, bsdf_estimate(q, d, cos_theta_max));
     return nd->energy * inv_surface_area * approx;
 }
 
@@ -549,6 +561,232 @@ static float lambert_pdf(v3 in, v3 out, v3 gn) {
     return smax(v3dot(nn, v) * (float)INV_PI, 0.0f);
 }
 
+/* ---- GTRClosure (shading/surface_closure.hpp:339-608) -------------------- */
+typedef struct gtr_closure { float col[3]; float roughness, tail_shape, fresnel, norm; } gtr_closure;
+
+/* clamp (utils/utils.hpp:42-44): std::min(b, std::max(a, v)) */
+static float uclamp(float v, float a, float b) { return smin(b, smax(a, v)); }
+
+/* GTRClosure::normalization (:376-382), the pi product in double */
+static float gtr_normalization(float r, float t) {
+    const float r2 = r * r;
+    const float top = (t - 1.0f) * (r2 - 1);
+    const double bottom = M_PI * (double)(1.0f - powf(r2, 1.0f - t));
+    return (float)((double)top / bottom);
+}
+
+/* The GTRClosure(col, roughness, tail_shape, fresnel) constructor with
+ * validate (:351-372) */
+static gtr_closure gtr_make(const float col[3], const float prm[3]) {
+    gtr_closure g;
+    memcpy(g.col, col, sizeof g.col);
+    g.roughness = smax(0.0f, smin(0.9999f, prm[0]));
+    g.tail_shape = smax(0.0001f, prm[1]);
+    g.fresnel = prm[2];
+    if (g.roughness < 0.000244140625f) g.roughness = 0.0f;
+    if (fabsf(g.tail_shape - 1.0f) < 0.0001f) g.tail_shape = 1.0f + 0.0001f;
+    g.norm = gtr_normalization(g.roughness, g.tail_shape);
+    return g;
+}
+
+/* GTRClosure::dist (:398-410) */
+static float gtr_dist(const gtr_closure* g, float nh, float rough) {
+    const float r2 = rough * rough;
+    float D = 0.0f;
+    if (nh > 0.0f) {
+        const float nh2 = nh * nh;
+        D = gtr_normalization(rough, g->tail_shape) / powf(1 + ((r2 - 1) * nh2), g->tail_shape);
+    }
+    return D;
+}
+
+/* schlick_fresnel_from_fac (:173-177) */
+static float schlick_from_fac(float f, float c) {
+    const float c1 = (1.0f - c);
+    const float c2 = c1 * c1;
+    return f + ((1.0f - f) * c1 * c2 * c2);
+}
+
+/* lerp (utils/utils.hpp:52-55) for floats: the 1.0 - alpha in double */
+static float lerp_f(float alpha, float a, float b) {
+    return (float)((double)a * (1.0 - (double)alpha) + (double)(b * alpha));
+}
+
+/* GTRClosure::evaluate (:452-517) */
+static void gtr_eval(const gtr_closure* g, v3 in, v3 out, v3 gn, float hero, float e[NSPEC]) {
+    v3 nn = v3norm(gn);
+    const v3 aa = v3scale(v3norm(in), -1.0f);
+    const v3 bb = v3norm(out);
+    const v3 hh = v3norm(v3add(aa, bb));
+    if (v3dot(nn, hh) < 0.0f) nn = v3scale(nn, -1.0f);
+    const float na = uclamp(v3dot(nn, aa), -1.0f, 1.0f);
+    const float nb = uclamp(v3dot(nn, bb), -1.0f, 1.0f);
+    const float ha = uclamp(v3dot(hh, aa), -1.0f, 1.0f);
+    const float hb = uclamp(v3dot(hh, bb), -1.0f, 1.0f);
+    const float nh = uclamp(v3dot(nn, hh), -1.0f, 1.0f);
+    const float r2 = g->roughness * g->roughness;
+    float D = 1.0f, G1 = 1.0f, G2 = 1.0f;
+    spec_from_color(g->col, hero, e);
+    for (int i = 0; i < NSPEC; ++i) e[i] = lerp_f(1.0f - g->fresnel, schlick_from_fac(e[i], hb), e[i]);
+    if (g->roughness == 0.0f) return;
+    if (nh > 0.0f) {
+        const float nh2 = nh * nh;
+        D = g->norm / powf(1 + ((r2 - 1) * nh2), g->tail_shape);
+    }
+    {
+        const float na2 = na * na;
+        const float tan_na = sqrtf((1.0f - na2) / na2);
+        const float pos = (ha * na) > 0.0f ? 1.0f : 0.0f;
+        const float a = r2 * tan_na;
+        const float b = (sqrtf(1.0f + a * a) - 1.0f) * 0.5f;
+        G1 = pos / (1.0f + b);
+    }
+    {
+        const float nb2 = nb * nb;
+        const float tan_nb = sqrtf((1.0f - nb2) / nb2);
+        const float pos = (hb * nb) > 0.0f ? 1.0f : 0.0f;
+        const float a = r2 * tan_nb;
+        const float b = (sqrtf(1.0f + a * a) - 1.0f) * 0.5f;
+        G2 = pos / (1.0f + b);
+    }
+    const float f = D * G1 * G2;
+    for (int i = 0; i < NSPEC; ++i) e[i] = (e[i] * f) * (float)INV_PI;
+}
+
+/* GTRClosure::sample_pdf (:556-571) */
+static float gtr_pdf(const gtr_closure* g, v3 in, v3 out, v3 gn) {
+    v3 nn = v3norm(gn);
+    const v3 aa = v3scale(v3norm(in), -1.0f);
+    const v3 bb = v3norm(out);
+    const v3 hh = v3norm(v3scale(v3add(aa, bb), 0.5f));
+    if (v3dot(nn, hh) < 0.0f) nn = v3scale(nn, -1.0f);
+    const float nh = uclamp(v3dot(nn, hh), -1.0f, 1.0f);
+    return gtr_dist(g, nh, g->roughness) * (float)INV_PI;
+}
+
+/* GTRClosure::half_theta_sample (:385-396) */
+static float gtr_half_theta_sample(const gtr_closure* g, float u) {
+    const float r2 = g->roughness * g->roughness;
+    const float top = 1.0f - powf((powf(r2, 1.0f - g->tail_shape) * (1.0f - u)) + u, 1.0f / (1.0f - g->tail_shape));
+    const float bottom = (1.0f - r2);
+    return sqrtf(top / bottom);
+}
+
+/* GTRClosure::sample (:418-437): the direction */
+static v3 gtr_sample_dir(const gtr_closure* g, v3 in, v3 gn, float si, float sj) {
+    v3 nn = v3norm(gn);
+    if (v3dot(nn, in) > 0.0f) nn = v3scale(nn, -1.0f);
+    const float theta_cos = gtr_half_theta_sample(g, si);
+    const float theta_sin = sqrtf(1.0f - (theta_cos * theta_cos));
+    const float angle = (float)((double)sj * M_PI * 2);
+    v3 half_dir = v3m(cosf(angle) * theta_sin, sinf(angle) * theta_sin, theta_cos);
+    half_dir = v3norm(zup_to_vec(half_dir, nn));
+    return v3sub(in, v3scale(v3scale(half_dir, 2.0f), v3dot(in, half_dir)));
+}
+
+/* GTRClosure::estimate_eval_over_solid_angle (:520-553) */
+static float gtr_estimate(const gtr_closure* g, v3 in, v3 cent, float cos_theta, v3 nor) {
+    v3 nn = v3norm(nor);
+    const v3 aa = v3scale(v3norm(in), -1.0f);
+    const v3 bb = v3norm(cent);
+    if (v3dot(nn, in) > 0.0f) nn = v3scale(nn, -1.0f);
+    const float theta = acosf(cos_theta);
+    const v3 hh = v3norm(v3add(aa, bb));
+    const float nh = uclamp(v3dot(nn, hh), -1.0f, 1.0f);
+    const float fac = gtr_dist(g, nh, smin(1.0f, sqrtf(g->roughness) + (2.0f * theta / (float)M_PI)));
+    return fac * smin(1.0f - cos_theta, 1.0f) * (float)INV_PI;
+}
+
+/* calc_uv_differentials (basics/differential_geometry.hpp:145-156) */
+static void uv_differentials(v3 dp, v3 dpdu, v3 dpdv, float* du, float* dv) {
+    const float dpdu_ilen = 1.0f / v3len(dpdu);
+    const v3 dpdu_n = v3scale(dpdu, dpdu_ilen);
+    const float dpdv_ilen = 1.0f / v3len(dpdv);
+    const v3 dpdv_n = v3scale(dpdv, dpdv_ilen);
+    *du = v3dot(dp, dpdu_n) * dpdu_ilen;
+    *dv = v3dot(dp, dpdv_n) * dpdv_ilen;
+}
+
+/* The rotation about `axis` by `angle`, v' = v M, by Rodrigues' formula as
+ * Imath's Matrix44::setAxisAngle lays it out (make_axis_angle_transform,
+ * math/transform.hpp:221-226; Imath is not part of the reference tree) */
+static mat4 axis_angle(v3 axis, float angle) {
+    const v3 u = v3norm(axis);
+    const float sn = sinf(angle), cs = cosf(angle), oc = 1 - cs;
+    mat4 m = mat4_identity();
+    m.m[0][0] = u.x * u.x * oc + cs;
+    m.m[0][1] = u.x * u.y * oc + u.z * sn;
+    m.m[0][2] = u.x * u.z * oc - u.y * sn;
+    m.m[1][0] = u.x * u.y * oc - u.z * sn;
+    m.m[1][1] = u.y * u.y * oc + cs;
+    m.m[1][2] = u.y * u.z * oc + u.x * sn;
+    m.m[2][0] = u.x * u.z * oc + u.y * sn;
+    m.m[2][1] = u.y * u.z * oc - u.x * sn;
+    m.m[2][2] = u.z * u.z * oc + cs;
+    return m;
+}
+
+/* reflect_ray_direction_differential (basics/differential_geometry.hpp:113-117) */
+static v3 reflect_dd(v3 normal, v3 normal_d, v3 d, v3 dd) {
+    const float ddn = v3dot(dd, normal) + v3dot(d, normal_d);
+    const v3 tmp = v3add(v3scale(normal_d, v3dot(d, normal)), v3scale(normal, ddn));
+    return v3sub(dd, v3scale(tmp, 2.0f));
+}
+
+/* GTRClosure::propagate_differentials (:574-606) and clamp_dd
+ * (basics/differential_geometry.hpp:125-135); gn, dpdu.. world space */
+static void gtr_differentials(float t, const ora_wray* in, v3 gn, v3 dpdu, v3 dpdv, v3 dndu, v3 dndv,
+                              ora_wray* out) {
+    v3 nn = v3norm(gn);
+    const v3 hh = v3norm(v3add(v3norm(v3scale(va(in->d), -1.0f)), v3norm(va(out->d))));
+    const v3 odx = transfer_origin_diff(t, nn, va(in->d), va(in->odx), va(in->ddx));
+    const v3 ody = transfer_origin_diff(t, nn, va(in->d), va(in->ody), va(in->ddy));
+    float dux, dvx, duy, dvy;
+    uv_differentials(odx, dpdu, dpdv, &dux, &dvx);
+    uv_differentials(ody, dpdu, dpdv, &duy, &dvy);
+    v3 dndx = v3add(v3scale(dndu, dux), v3scale(dndv, dvx));
+    v3 dndy = v3add(v3scale(dndu, duy), v3scale(dndv, dvy));
+    if (v3dot(nn, hh) < 0.0f) {
+        nn = v3scale(nn, -1.0f);
+        dndx = v3scale(dndx, -1.0f);
+        dndy = v3scale(dndy, -1.0f);
+    }
+    const v3 axis = v3norm(v3cross(nn, hh));
+    const float angle = acosf(uclamp(v3dot(nn, hh), 0.0f, 1.0f));
+    const mat4 xf = axis_angle(axis, angle);
+    const v3 dhdx = mat4_dir(&xf, dndx);
+    const v3 dhdy = mat4_dir(&xf, dndy);
+    v3 ddx = reflect_dd(hh, dhdx, va(in->d), va(in->ddx));
+    v3 ddy = reflect_dd(hh, dhdy, va(in->d), va(in->ddy));
+    const float len_d = v3len(va(out->d));
+    const float len_dx = v3len(ddx);
+    const float len_dy = v3len(ddy);
+    if ((len_dx / len_d) > 0.9f) ddx = v3scale(ddx, 0.9f * len_d / len_dx);
+    if ((len_dy / len_d) > 0.9f) ddy = v3scale(ddy, 0.9f * len_d / len_dy);
+    set3(out->odx, odx);
+    set3(out->ody, ody);
+    set3(out->ddx, ddx);
+    set3(out->ddy, ddy);
+}
+
+static float bsdf_estimate(const lquery* q, v3 d, float cos_theta_max) {
+    return q->gtr ? gtr_estimate(q->gtr, q->d, d, cos_theta_max, q->nor)
+                  : lambert_estimate(q->d, d, cos_theta_max, q->nor);
+}
+
+/* world-space dp/du, dp/dv, dn/du, dn/dv of a stored hit
+ * (DifferentialGeometry::transformed_from, basics/differential_geometry.hpp:20-40) */
+static void hit_world_derivs(const ora_inter* h, v3* dpdu, v3* dpdv, v3* dndu, v3* dndv) {
+    mat4 sp;
+    memcpy(&sp.m[0][0], h->space, sizeof(float) * 16);
+    const mat4 inv = ora_mat4_inverse(&sp);
+    *dpdu = mat4_dir(&inv, va(h->dpdu));
+    *dpdv = mat4_dir(&inv, va(h->dpdv));
+    const float il = 1.0f / v3len(mat4_dir_transpose(&sp, va(h->n)));
+    *dndu = v3scale(mat4_dir_transpose(&sp, va(h->dndu)), il);
+    *dndv = v3scale(mat4_dir_transpose(&sp, va(h->dndv)), il);
+}
+
 static void init_path(path_t* p, uint32_t off, short x, short y) {
     memset(p, 0, sizeof *p);
     p->off = off;
@@ -573,8 +811,23 @@ static void next_ray(const ora_scene* s, path_t* p, int width, int height, ora_w
         const float sa = next_dim(p), sb = next_dim(p), sc = next_dim(p);
         ora_camera_ray_from_dims(s, rx, ry, dx, dy, sc, sa, sb, ray);
         p->time = ray->time;
+    } else if (p->step % 2 && p->inter.closure == PSY_SHADER_GTR &&
+               gtr_make(p->inter.closure_col, p->inter.closure_param).roughness == 0.0f) {
+        /* shadow ray of a delta closure (a GTR mirror): no light sample and
+         * no sampler dimensions; the default WorldRay the reference then
+         * traces has no effect (update_path ignores its result), so a ray
+         * of zero light and length is traced here */
+        for (int i = 0; i < NSPEC; ++i) p->lcol[i] = 0.0f;
+        v3 gp, gn, po;
+        hit_world(&p->inter, &gp, &gn, &po);
+        set3(ray->o, v3add(gp, po));
+        set3(ray->d, v3m(1e-20f, 1e-20f, 1e-20f));
+        ray->time = p->time;
+        ray->type = PSY_RAY_OCCLUSION;
     } else if (p->step % 2) {
-        /* shadow ray: the Lambert closure is never a delta distribution */
+        /* shadow ray towards a light sample */
+        const int is_gtr = p->inter.closure == PSY_SHADER_GTR;
+        const gtr_closure g = is_gtr ? gtr_make(p->inter.closure_col, p->inter.closure_param) : (gtr_closure){{0}};
         v3 gp, gn, po;
         hit_world(&p->inter, &gp, &gn, &po);
         v3 lq_nor = gn;
@@ -587,12 +840,14 @@ static void next_ray(const ora_scene* s, path_t* p, int width, int height, ora_w
         lq.xform = mat4_identity();
         lq.selection_pdf = 1.0f;
         lq.light_sample_pdf = 1.0f;
+        lq.gtr = is_gtr ? &g : NULL;
         const ora_assembly* root = &s->asms[0];
         if (!light_tree_sample(root, &lq)) {
             for (int i = 0; i < NSPEC; ++i) p->lcol[i] = 0.0f;
             lq.to_light = v3m(0.0f, 0.0f, 1.0f);
         } else {
-            const float bsdf_pdf = lambert_pdf(va(p->prev_ray.d), lq.to_light, gn);
+            const float bsdf_pdf = is_gtr ? gtr_pdf(&g, va(p->prev_ray.d), lq.to_light, gn)
+                                          : lambert_pdf(va(p->prev_ray.d), lq.to_light, gn);
             const float a2 = lq.light_sample_pdf * lq.light_sample_pdf;
             const float b2 = bsdf_pdf * bsdf_pdf;
             const float mis_inv_pdf = (a2 / (a2 + b2)) / lq.light_sample_pdf;
@@ -603,7 +858,37 @@ static void next_ray(const ora_scene* s, path_t* p, int width, int height, ora_w
         set3(ray->d, v3scale(v3sub(lq.to_light, po), 0.9995f));
         ray->time = p->time;
         ray->type = PSY_RAY_OCCLUSION;
-        lambert_differentials(p->inter.t, &p->prev_ray, gn, ray);
+        if (is_gtr) {
+            v3 dpdu, dpdv, dndu, dndv;
+            hit_world_derivs(&p->inter, &dpdu, &dpdv, &dndu, &dndv);
+            gtr_differentials(p->inter.t, &p->prev_ray, gn, dpdu, dpdv, dndu, dndv, ray);
+        } else {
+            lambert_differentials(p->inter.t, &p->prev_ray, gn, ray);
+        }
+    } else if (p->inter.closure == PSY_SHADER_GTR) {
+        /* bounce ray: GTRClosure::sample */
+        const gtr_closure g = gtr_make(p->inter.closure_col, p->inter.closure_param);
+        const float su = next_dim(p), sv = next_dim(p);
+        v3 gp, gn, po, dpdu, dpdv, dndu, dndv;
+        hit_world(&p->inter, &gp, &gn, &po);
+        hit_world_derivs(&p->inter, &dpdu, &dpdv, &dndu, &dndv);
+        const v3 out = gtr_sample_dir(&g, va(p->prev_ray.d), gn, su, sv);
+        float pdf = gtr_pdf(&g, va(p->prev_ray.d), out, gn);
+        float filt[NSPEC];
+        gtr_eval(&g, va(p->prev_ray.d), out, gn, p->wavelength, filt);
+        set3(ray->o, v3add(gp, po));
+        set3(ray->d, out);
+        ray->time = p->time;
+        ray->type = 2;
+        gtr_differentials(p->inter.t, &p->prev_ray, gn, dpdu, dpdv, dndu, dndv, ray);
+        if (g.roughness != 0.0f) {
+            if (pdf < 0.00001f) pdf = 0.00001f;
+            for (int i = 0; i < NSPEC; ++i) p->fcol[i] *= filt[i] / pdf;
+            p->last_pdf = pdf;
+        } else {
+            for (int i = 0; i < NSPEC; ++i) p->fcol[i] *= filt[i];
+            p->last_pdf = 0.0f;
+        }
     } else {
         /* bounce ray: LambertClosure::sample */
         const float su = next_dim(p), sv = next_dim(p);
@@ -627,7 +912,13 @@ static void update_path(const ora_scene* s, path_t* p, const ora_wray* ray, cons
             v3 gp, gn, po;
             hit_world(&p->inter, &gp, &gn, &po);
             float fac[NSPEC];
-            lambert_eval(p->inter.closure_col, va(p->prev_ray.d), va(ray->d), gn, p->wavelength, fac);
+            if (p->inter.closure == PSY_SHADER_GTR) {
+                const gtr_closure g = gtr_make(p->inter.closure_col, p->inter.closure_param);
+                if (g.roughness == 0.0f) goto next;  /* a delta closure adds no light sample */
+                gtr_eval(&g, va(p->prev_ray.d), va(ray->d), gn, p->wavelength, fac);
+            } else {
+                lambert_eval(p->inter.closure_col, va(p->prev_ray.d), va(ray->d), gn, p->wavelength, fac);
+            }
             for (int i = 0; i < NSPEC; ++i) p->col[i] += p->fcol[i] * p->lcol[i] * fac[i];
         }
     } else {
@@ -655,6 +946,7 @@ static void update_path(const ora_scene* s, path_t* p, const ora_wray* ray, cons
             for (int i = 0; i < NSPEC; ++i) p->col[i] += p->fcol[i] * bg[i];
         }
     }
+next:
     p->step++;
     if (p->step == p->path_length * 2) p->done = 1;
 }
@@ -846,3 +1138,48 @@ int ora_kat_light_select(const ora_scene* s, const float pos[3], const float nor
 }
 
 int64_t ora_kat_total_lights(const ora_scene* s) { return (int64_t)s->asms[0].total_lights; }
+
+/* ---- GTR closure pieces, for the oracle's own checks --------------------- */
+/* GTRClosure(col, prm) validated: roughness, tail shape, normalization */
+void ora_kat_gtr_params(const float prm[3], float out[3]) {
+    const float c[3] = {1.0f, 1.0f, 1.0f};
+    const gtr_closure g = gtr_make(c, prm);
+    out[0] = g.roughness; out[1] = g.tail_shape; out[2] = g.norm;
+}
+/* dist(nh, roughness) */
+float ora_kat_gtr_dist(const float prm[3], float nh) {
+    const float c[3] = {1.0f, 1.0f, 1.0f};
+    const gtr_closure g = gtr_make(c, prm);
+    return gtr_dist(&g, nh, g.roughness);
+}
+float ora_kat_gtr_half_theta(const float prm[3], float u) {
+    const float c[3] = {1.0f, 1.0f, 1.0f};
+    const gtr_closure g = gtr_make(c, prm);
+    return gtr_half_theta_sample(&g, u);
+}
+/* evaluate (first wavelength of the flat spectrum of a grey colour) and
+ * sample_pdf of in -> out about normal n */
+void ora_kat_gtr(float grey, const float prm[3], const float in[3], const float out[3], const float n[3],
+                 float* eval, float* pdf) {
+    const float c[3] = {grey, grey, grey};
+    const gtr_closure g = gtr_make(c, prm);
+    float e[NSPEC];
+    gtr_eval(&g, va(in), va(out), va(n), 500.0f, e);
+    *eval = e[0];
+    *pdf = gtr_pdf(&g, va(in), va(out), va(n));
+}
+void ora_kat_gtr_sample(const float prm[3], const float in[3], const float n[3], float si, float sj, float out[3]) {
+    const float c[3] = {1.0f, 1.0f, 1.0f};
+    const gtr_closure g = gtr_make(c, prm);
+    set3(out, gtr_sample_dir(&g, va(in), va(n), si, sj));
+}
+float ora_kat_gtr_estimate(const float prm[3], const float in[3], const float cent[3], float cos_theta,
+                           const float nor[3]) {
+    const float c[3] = {1.0f, 1.0f, 1.0f};
+    const gtr_closure g = gtr_make(c, prm);
+    return gtr_estimate(&g, va(in), va(cent), cos_theta, va(nor));
+}
+void ora_kat_axis_angle(const float axis[3], float angle, float m[16]) {
+    const mat4 x = axis_angle(va(axis), angle);
+    memcpy(m, &x.m[0][0], sizeof(float) * 16);
+}

@@ -428,6 +428,7 @@ ora_scene* ora_scene_create(const psy_scene_desc* d) {
     ora_scene* s = (ora_scene*)calloc(1, sizeof(ora_scene));
     s->n_asm = d->n_assemblies;
     s->asms = (ora_assembly*)calloc((size_t)s->n_asm, sizeof(ora_assembly));
+    s->flags = d->flags;
     s->n_shaders = d->n_shaders;
     s->shaders = (psy_shader_desc*)malloc(sizeof(psy_shader_desc) * (d->n_shaders + 1));
     if (d->n_shaders) memcpy(s->shaders, d->shaders, sizeof(psy_shader_desc) * d->n_shaders);

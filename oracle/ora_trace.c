@@ -216,6 +216,9 @@ static void shade(const tracer* tr, ora_inter* in, int shader) {
         const psy_shader_desc* sd = &tr->scene->shaders[shader];
         in->closure = sd->kind;
         memcpy(in->closure_col, sd->color, sizeof(float) * 3);
+        in->closure_param[0] = sd->roughness;
+        in->closure_param[1] = sd->tail_shape;
+        in->closure_param[2] = sd->fresnel;
     } else {
         in->closure = PSY_SHADER_EMIT;
         in->closure_col[0] = 1.0f; in->closure_col[1] = 0.0f; in->closure_col[2] = 1.0f;
@@ -349,8 +352,9 @@ float ora_rect_light_pdf(float w, float h, v3 arr) {
     return 1.0f / (a1 + a2);
 }
 
-/* RectangleLight::intersect_ray (lights/rectangle_light.hpp:110-147) */
-static int rect_light_intersect(const ora_object* ob, const ora_ray* ray, ora_inter* in) {
+/* RectangleLight::intersect_ray (lights/rectangle_light.hpp:110-147);
+ * `fix`: test y against the height (PSY_SCENE_RECT_LIGHT_FIX) */
+static int rect_light_intersect(const ora_object* ob, const ora_ray* ray, int fix, ora_inter* in) {
     if (ray->d.z == 0.0f) return 0;
     float w, h, col[3];
     rect_dims(ob, ray->time, &w, &h, col);
@@ -358,8 +362,8 @@ static int rect_light_intersect(const ora_object* ob, const ora_ray* ray, ora_in
     if (t <= 0.0f || t > ray->max_t) return 0;
     const float x = ray->o.x + (ray->d.x * t);
     const float y = ray->o.y + (ray->d.y * t);
-    /* the y test uses the width, as the reference does (:127) */
-    if (x >= (w * -0.5f) && x <= (w * 0.5f) && y >= (h * -0.5f) && y <= (w * 0.5f)) {
+    /* the y test uses the width, as the reference does (:127), unless fixed */
+    if (x >= (w * -0.5f) && x <= (w * 0.5f) && y >= (h * -0.5f) && y <= ((fix ? h : w) * 0.5f)) {
         in->t = t;
         set3(in->p, v3m(x, y, 0.0f));
         set3(in->n, v3m(0.0f, 0.0f, 1.0f));
@@ -726,7 +730,7 @@ static void trace_surface_or_light(tracer* tr, const ora_object* ob, ora_ray* rb
         int hit;
         if (ob->kind == PSY_OBJ_SPHERE) hit = sphere_intersect(ob, r, in);
         else if (ob->kind == PSY_OBJ_SPHERE_LIGHT) hit = sphere_light_intersect(ob, r, in);
-        else hit = rect_light_intersect(ob, r, in);
+        else hit = rect_light_intersect(ob, r, (tr->scene->flags & PSY_SCENE_RECT_LIGHT_FIX) != 0, in);
         if (!hit) continue;
         in->hit = 1;
         in->id = tr->element_id;

@@ -42,6 +42,7 @@ typedef struct {
     float closure_col[3];
     float closure_prob;
     float patch_dim;      /* max_dim of the accepted patch leaf (0 if none) */
+    float closure_param[3];  /* GTR: roughness, tail shape, Fresnel as the shader gives them */
 } ora_inter;
 
 /* Ray with derived footprint (the product's PSY_RAY_PLANES fields) */
@@ -166,6 +167,20 @@ int ora_kat_light_select(const ora_scene* s, const float pos[3], const float nor
                          float n, float u, float v, uint64_t* id, float* selection_pdf, float* light_sample_pdf,
                          float to_light[3]);
 int64_t ora_kat_total_lights(const ora_scene* s);
+/* GTRClosure (shading/surface_closure.hpp:339-608); prm = roughness, tail
+ * shape, Fresnel as a GTRShader gives them: validated roughness, tail shape
+ * and normalization; dist; half_theta_sample; evaluate (grey colour, first
+ * wavelength) with sample_pdf; sample's direction; the solid-angle estimate;
+ * and the axis-angle rotation its differentials use (row-vector matrix) */
+void ora_kat_gtr_params(const float prm[3], float out[3]);
+float ora_kat_gtr_dist(const float prm[3], float nh);
+float ora_kat_gtr_half_theta(const float prm[3], float u);
+void ora_kat_gtr(float grey, const float prm[3], const float in[3], const float out[3], const float n[3],
+                 float* eval, float* pdf);
+void ora_kat_gtr_sample(const float prm[3], const float in[3], const float n[3], float si, float sj, float out[3]);
+float ora_kat_gtr_estimate(const float prm[3], const float in[3], const float cent[3], float cos_theta,
+                           const float nor[3]);
+void ora_kat_axis_angle(const float axis[3], float angle, float m[16]);
 
 #ifdef __cplusplus
 }

@@ -22,6 +22,8 @@ PSY_INSTANCE_ASSEMBLY = 1
 
 PSY_SHADER_LAMBERT = 1
 PSY_SHADER_EMIT = 2
+PSY_SHADER_GTR = 3
+PSY_SCENE_RECT_LIGHT_FIX = 1
 
 PSY_WRAY_PLANES = 20
 PSY_RAY_PLANES = 15
@@ -60,7 +62,8 @@ class InstanceDesc(C.Structure):
 
 
 class ShaderDesc(C.Structure):
-    _fields_ = [("kind", C.c_int32), ("color", C.c_float * 3)]
+    _fields_ = [("kind", C.c_int32), ("color", C.c_float * 3), ("roughness", C.c_float), ("tail_shape", C.c_float),
+                ("fresnel", C.c_float)]
 
 
 class AssemblyDesc(C.Structure):
@@ -86,7 +89,7 @@ class SceneDesc(C.Structure):
     _fields_ = [("assemblies", C.POINTER(AssemblyDesc)), ("n_assemblies", C.c_int32),
                 ("shaders", C.POINTER(ShaderDesc)), ("n_shaders", C.c_int32),
                 ("camera", CameraDesc), ("background", C.c_float * 3),
-                ("dice_rate", C.c_float), ("min_upoly_size", C.c_float)]
+                ("dice_rate", C.c_float), ("min_upoly_size", C.c_float), ("flags", C.c_uint32)]
 
 
 class Hits(C.Structure):

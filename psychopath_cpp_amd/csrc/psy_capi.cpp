@@ -124,6 +124,25 @@ namespace {
 namespace psy {
 // Grows the per-context trace scratch to n entries.  Calls on one context
 // must not overlap in time (they share this scratch).
+// GTRClosure::validate and normalization (shading/surface_closure.hpp:
+// 351-382): roughness clamped to [0, 0.9999] and taken as 0 below 2^-12 (a
+// perfect mirror), tail shape at least 1e-4 and pushed off 1 by 1e-4, and
+// the distribution's normalization (t - 1)(r^2 - 1) / (pi (1 - r^2^(1-t))),
+// its pi product in double as in the reference.
+void gtr_validate(float roughness, float tail_shape, float fresnel, DevShader& o) {
+    float r = std::max(0.0f, std::min(0.9999f, roughness));
+    float t = std::max(0.0001f, tail_shape);
+    if (r < 0.000244140625f) r = 0.0f;
+    if (std::abs(t - 1.0f) < 0.0001f) t = 1.0f + 0.0001f;
+    const float r2 = r * r;
+    const float top = (t - 1.0f) * (r2 - 1);
+    const double bottom = M_PI * (double)(1.0f - std::pow(r2, 1.0f - t));
+    o.roughness = r;
+    o.tail_shape = t;
+    o.fresnel = fresnel;
+    o.norm = (float)((double)top / bottom);
+}
+
 int ensure_queue(psy_ctx* ctx, int64_t n) {
     if (!ctx->counters.p) {
         int rc = ctx->counters.alloc(sizeof(uint32_t) * 8);
@@ -909,8 +928,14 @@ int prepare_scene(const psy_scene_desc* d, int dice_depth, HostScene& H) {
     // ---- shaders, camera (basics/camera.hpp:23-69) -------------------------
     H.shaders.resize(d->n_shaders);
     for (int i = 0; i < d->n_shaders; ++i) {
-        H.shaders[i].kind = d->shaders[i].kind;
-        for (int k = 0; k < 3; ++k) H.shaders[i].color[k] = d->shaders[i].color[k];
+        const psy_shader_desc& sd = d->shaders[i];
+        if (sd.kind != PSY_SHADER_LAMBERT && sd.kind != PSY_SHADER_EMIT && sd.kind != PSY_SHADER_GTR)
+            return fail(PSY_ERR_INVALID, "psy_scene_create: shader " + std::to_string(i) + " has an unknown kind");
+        DevShader& o = H.shaders[i];
+        o.kind = sd.kind;
+        for (int k = 0; k < 3; ++k) o.color[k] = sd.color[k];
+        o.roughness = o.tail_shape = o.fresnel = o.norm = 0.0f;
+        if (sd.kind == PSY_SHADER_GTR) gtr_validate(sd.roughness, sd.tail_shape, sd.fresnel, o);
     }
     const psy_camera_desc& c = d->camera;
     DevCamera& cam = H.cam;
@@ -1008,6 +1033,7 @@ int psy_scene_create(psy_ctx* ctx, const psy_scene_desc* d, int dice_depth, psy_
     s.dice_rate = H.dice_rate;
     s.min_upoly = H.min_upoly;
     s.n_inst = (int32_t)nl;
+    s.flags = d->flags;
 
     psy_scene_info& info = sc->info;
     info.n_leaf_instances = (int64_t)nl;

@@ -58,7 +58,10 @@ struct alignas(16) DevObject {
 };
 static_assert(sizeof(DevObject) == 32, "object record must be 32 bytes");
 
-struct DevShader { int32_t kind; float color[3]; };
+// A surface shader; GTR ones carry GTRClosure's validated roughness and
+// tail shape, its Fresnel blend and its normalization factor
+// (shading/surface_closure.hpp:351-382), computed once on the host.
+struct DevShader { int32_t kind; float color[3]; float roughness, tail_shape, fresnel, norm; };
 
 constexpr int MAX_CAM_SAMPLES = 8;
 struct DevCamera {
@@ -86,6 +89,7 @@ struct DevScene {
     float dice_rate;
     float min_upoly;
     int32_t n_inst;
+    uint32_t flags;           // PSY_SCENE_*
 };
 
 // Halton tables (dimension d >= 1): perm table offset, pow_base, terms.
@@ -207,7 +211,7 @@ struct RenderLaunch {
     float bg[3];
     DevSpectrum spec;
 };
-constexpr int PATH_PLANES = 46;  // float planes of a path state
+constexpr int PATH_PLANES = 58;  // float planes of a path state
 // Path buffers of one chunk of `cap` paths: states (PATH_PLANES), two world
 // ray streams, hits, two active lists, counters (u32 x 4).
 struct RenderBuffers {

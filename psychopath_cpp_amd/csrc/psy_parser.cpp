@@ -286,9 +286,7 @@ namespace {
 
 using PS = psy_parsed_scene;
 
-// a shader binding: index into PS::shaders, -1 for none, -2 for a GTR
-// shader (not renderable here)
-constexpr int SHADER_GTR = -2;
+// a shader binding: index into PS::shaders, -1 for none
 
 struct Builder {
     PS& ps;
@@ -342,7 +340,25 @@ struct Builder {
             index = (int)ps.shaders.size();
             ps.shaders.push_back(d);
         } else if (type->leaf == "GTR") {
-            index = SHADER_GTR;
+            // the reference's defaults and fields (parser/parser.cpp:511-546):
+            // the first number of Roughness, TailShape and Fresnel
+            psy_shader_desc d{};
+            d.kind = PSY_SHADER_GTR;
+            d.color[0] = d.color[1] = d.color[2] = 0.9f;
+            d.roughness = 0.1f;
+            d.tail_shape = 2.0f;
+            d.fresnel = 0.25f;
+            for (const DtNode& c : n.children) {
+                if (c.type == "Color") {
+                    color3(c.leaf, d.color);
+                } else if (c.type == "Roughness" || c.type == "TailShape" || c.type == "Fresnel") {
+                    const std::vector<float> v = floats(c.leaf);
+                    if (v.empty()) continue;
+                    (c.type == "Roughness" ? d.roughness : c.type == "TailShape" ? d.tail_shape : d.fresnel) = v[0];
+                }
+            }
+            index = (int)ps.shaders.size();
+            ps.shaders.push_back(d);
         } else {
             warn(n, "unknown surface shader type '" + type->leaf + "'");
         }
@@ -511,8 +527,6 @@ struct Builder {
                 if (shader == -1) warn(c, "bound surface shader " + c.leaf + " does not exist");
             }
         }
-        if (shader == SHADER_GTR)
-            return fail(n, "Instance of " + name + " binds a GTR surface shader, which the device path does not shade");
         PS::Asm& A = ps.asms[a];
         psy_instance_desc d{};
         d.transform_index = (int32_t)(A.transforms.size() / 16);

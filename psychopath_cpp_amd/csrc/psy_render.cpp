@@ -761,6 +761,10 @@ int psy_adjoint(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, 
         return fail(PSY_ERR_UNSUPPORTED, "psy_adjoint: the scene has non-grey colours; the parameters are grey "
                     "albedos and emission scales");
     if (!d_film_xyz) return fail(PSY_ERR_INVALID, "psy_adjoint: NULL film gradient");
+    for (const DevShader& sh : s->host_shaders)
+        if (sh.kind == PSY_SHADER_GTR)
+            return fail(PSY_ERR_UNSUPPORTED, "psy_adjoint: the scene has a GTR shader; the adjoint differentiates "
+                        "Lambert albedos and emission scales only");
     if (prm->path_length > ADJ_MAX_VERTS)
         return fail(PSY_ERR_UNSUPPORTED, "psy_adjoint: path_length above 8 (scattering vertices kept per path)");
     const int64_t n_albedo = (int64_t)s->host_shaders.size();

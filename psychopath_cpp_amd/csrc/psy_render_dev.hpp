@@ -58,7 +58,11 @@ enum : int {
     P_PDDX = 39,
     P_PDDY = 42,
     P_SIDX = 45,     // sample index within the pixel
-    P_COUNT = 46
+    P_DPDU = 46,     // stored hit's world-space dp/du, dp/dv, dn/du, dn/dv (GTR shaders only)
+    P_DPDV = 49,
+    P_DNDU = 52,
+    P_DNDV = 55,
+    P_COUNT = 58
 };
 static_assert(P_COUNT == PATH_PLANES, "path state layout");
 constexpr uint32_t STEP_BACKFACING = 1u << 8;
@@ -327,6 +331,193 @@ PSY_DEV float lambert_pdf(V3 in, V3 out, V3 gn) {
     return smax(dot(nn, v) * (float)(1.0 / PSY_PI_D), 0.0f);
 }
 
+// ---- GTR microfacet closure (shading/surface_closure.hpp:339-608) -----------
+// The shader's roughness, tail shape and normalization come validated from
+// the host (GTRClosure::validate).  A roughness of 0 is a perfect mirror:
+// a delta closure, whose paths take no light samples.
+
+// clamp(v, a, b) of utils/utils.hpp:42-44: std::min(b, std::max(a, v))
+PSY_DEV float uclamp(float v, float a, float b) { return smin(b, smax(a, v)); }
+
+// GTRClosure::normalization (:376-382), its pi product in double
+PSY_DEV_FN float gtr_normalization(float r, float t) {
+    const float r2 = r * r;
+    const float top = (t - 1.0f) * (r2 - 1);
+    const double bottom = PSY_PI_D * (double)(1.0f - powf(r2, 1.0f - t));
+    return (float)((double)top / bottom);
+}
+
+// GTRClosure::dist (:398-410) for roughness `rough`, whose normalization
+// is `norm`
+PSY_DEV_FN float gtr_dist(float nh, float rough, float tail, float norm) {
+    const float r2 = rough * rough;
+    float D = 0.0f;
+    if (nh > 0.0f) {
+        const float nh2 = nh * nh;
+        D = norm / powf(1 + ((r2 - 1) * nh2), tail);
+    }
+    return D;
+}
+
+// schlick_fresnel_from_fac (:173-177)
+PSY_DEV float schlick_from_fac(float f, float c) {
+    const float c1 = (1.0f - c);
+    const float c2 = c1 * c1;
+    return f + ((1.0f - f) * c1 * c2 * c2);
+}
+
+// GTRClosure::evaluate (:452-517) at the 4 wavelengths of `hero`
+PSY_DEV_FN void gtr_eval(const DevSpectrum& S, const DevShader& g, V3 in, V3 out, V3 gn, float hero,
+                         float e[NSPEC]) {
+    V3 nn = normalized(gn);
+    const V3 aa = scale(normalized(in), -1.0f);
+    const V3 bb = normalized(out);
+    const V3 hh = normalized(add(aa, bb));
+    if (dot(nn, hh) < 0.0f) nn = scale(nn, -1.0f);
+    const float na = uclamp(dot(nn, aa), -1.0f, 1.0f);
+    const float nb = uclamp(dot(nn, bb), -1.0f, 1.0f);
+    const float ha = uclamp(dot(hh, aa), -1.0f, 1.0f);
+    const float hb = uclamp(dot(hh, bb), -1.0f, 1.0f);
+    const float nh = uclamp(dot(nn, hh), -1.0f, 1.0f);
+    const float r2 = g.roughness * g.roughness;
+    color_spec4(S, g.color[0], g.color[1], g.color[2], hero, e);
+    for (int k = 0; k < NSPEC; ++k) e[k] = lerpf(1.0f - g.fresnel, schlick_from_fac(e[k], hb), e[k]);
+    if (g.roughness == 0.0f) return;  // a mirror: the Fresnel-weighted colour
+    float D = 1.0f;
+    if (nh > 0.0f) {
+        const float nh2 = nh * nh;
+        D = g.norm / powf(1 + ((r2 - 1) * nh2), g.tail_shape);
+    }
+    const float na2 = na * na;
+    const float tan_na = sqrtf((1.0f - na2) / na2);
+    const float g1_pos = (ha * na) > 0.0f ? 1.0f : 0.0f;
+    const float g1_a = r2 * tan_na;
+    const float g1_b = (sqrtf(1.0f + g1_a * g1_a) - 1.0f) * 0.5f;
+    const float G1 = g1_pos / (1.0f + g1_b);
+    const float nb2 = nb * nb;
+    const float tan_nb = sqrtf((1.0f - nb2) / nb2);
+    const float g2_pos = (hb * nb) > 0.0f ? 1.0f : 0.0f;
+    const float g2_a = r2 * tan_nb;
+    const float g2_b = (sqrtf(1.0f + g2_a * g2_a) - 1.0f) * 0.5f;
+    const float G2 = g2_pos / (1.0f + g2_b);
+    const float f = D * G1 * G2;
+    for (int k = 0; k < NSPEC; ++k) e[k] = (e[k] * f) * (float)(1.0 / PSY_PI_D);
+}
+
+// GTRClosure::sample_pdf (:556-571)
+PSY_DEV_FN float gtr_pdf(const DevShader& g, V3 in, V3 out, V3 gn) {
+    V3 nn = normalized(gn);
+    const V3 aa = scale(normalized(in), -1.0f);
+    const V3 bb = normalized(out);
+    const V3 hh = normalized(scale(add(aa, bb), 0.5f));
+    if (dot(nn, hh) < 0.0f) nn = scale(nn, -1.0f);
+    const float nh = uclamp(dot(nn, hh), -1.0f, 1.0f);
+    return gtr_dist(nh, g.roughness, g.tail_shape, g.norm) * (float)(1.0 / PSY_PI_D);
+}
+
+// GTRClosure::sample (:418-437): a half vector from the distribution
+// (half_theta_sample, :385-396), the incoming direction mirrored about it
+PSY_DEV_FN V3 gtr_sample_dir(const DevShader& g, V3 in, V3 gn, float si, float sj) {
+    V3 nn = normalized(gn);
+    if (dot(nn, in) > 0.0f) nn = scale(nn, -1.0f);
+    const float r2 = g.roughness * g.roughness;
+    const float top =
+        1.0f - powf((powf(r2, 1.0f - g.tail_shape) * (1.0f - si)) + si, 1.0f / (1.0f - g.tail_shape));
+    const float bottom = (1.0f - r2);
+    const float theta_cos = sqrtf(top / bottom);
+    const float theta_sin = sqrtf(1.0f - (theta_cos * theta_cos));
+    const float angle = (float)((double)sj * PSY_PI_D * 2);
+    V3 half_dir = mk(cosf(angle) * theta_sin, sinf(angle) * theta_sin, theta_cos);
+    half_dir = normalized(zup_to_vec(half_dir, nn));
+    return sub(in, scale(scale(half_dir, 2.0f), dot(in, half_dir)));
+}
+
+// GTRClosure::estimate_eval_over_solid_angle (:520-553): the distribution
+// at the half vector towards the centre, widened by the cone's angle
+PSY_DEV_FN float gtr_estimate(float rough, float tail, V3 in, V3 cent, float cos_theta, V3 nor) {
+    V3 nn = normalized(nor);
+    const V3 aa = scale(normalized(in), -1.0f);
+    const V3 bb = normalized(cent);
+    if (dot(nn, in) > 0.0f) nn = scale(nn, -1.0f);
+    const float theta = acosf(cos_theta);
+    const V3 hh = normalized(add(aa, bb));
+    const float nh = uclamp(dot(nn, hh), -1.0f, 1.0f);
+    const float r = smin(1.0f, sqrtf(rough) + (2.0f * theta / (float)PSY_PI_D));
+    const float fac = gtr_dist(nh, r, tail, gtr_normalization(r, tail));
+    return fac * smin(1.0f - cos_theta, 1.0f) * (float)(1.0 / PSY_PI_D);
+}
+
+// calc_uv_differentials (basics/differential_geometry.hpp:145-156)
+PSY_DEV void uv_differentials(V3 dp, V3 dpdu, V3 dpdv, float* du, float* dv) {
+    const float dpdu_ilen = 1.0f / length(dpdu);
+    const V3 dpdu_n = scale(dpdu, dpdu_ilen);
+    const float dpdv_ilen = 1.0f / length(dpdv);
+    const V3 dpdv_n = scale(dpdv, dpdv_ilen);
+    *du = dot(dp, dpdu_n) * dpdu_ilen;
+    *dv = dot(dp, dpdv_n) * dpdv_ilen;
+}
+
+// The rotation by `angle` about `axis` as a row-vector matrix (v' = v M),
+// Rodrigues' formula in the element order of Imath's Matrix44::setAxisAngle,
+// which make_axis_angle_transform (math/transform.hpp:221-226) calls; the
+// axis is normalized once more there.
+PSY_DEV M44 axis_angle_matrix(V3 axis, float angle) {
+    const V3 u = normalized(axis);
+    const float sn = sinf(angle), cs = cosf(angle);
+    const float oc = 1 - cs;
+    M44 m = m_identity();
+    m.m[0] = u.x * u.x * oc + cs;
+    m.m[1] = u.x * u.y * oc + u.z * sn;
+    m.m[2] = u.x * u.z * oc - u.y * sn;
+    m.m[4] = u.x * u.y * oc - u.z * sn;
+    m.m[5] = u.y * u.y * oc + cs;
+    m.m[6] = u.y * u.z * oc + u.x * sn;
+    m.m[8] = u.x * u.z * oc + u.y * sn;
+    m.m[9] = u.y * u.z * oc - u.x * sn;
+    m.m[10] = u.z * u.z * oc + cs;
+    return m;
+}
+
+// reflect_ray_direction_differential (basics/differential_geometry.hpp:113-117)
+PSY_DEV V3 reflect_dd(V3 normal, V3 normal_d, V3 d, V3 dd) {
+    const float ddn = dot(dd, normal) + dot(d, normal_d);
+    const V3 tmp = add(scale(normal_d, dot(d, normal)), scale(normal, ddn));
+    return sub(dd, scale(tmp, 2.0f));
+}
+
+// GTRClosure::propagate_differentials (:574-606) with clamp_dd
+// (basics/differential_geometry.hpp:125-135); geo is the hit's world-space
+// normal and derivatives.
+PSY_DEV_FN void gtr_differentials(float t, V3 in_d, V3 in_odx, V3 in_ody, V3 in_ddx, V3 in_ddy, V3 gn, V3 dpdu,
+                                  V3 dpdv, V3 dndu, V3 dndv, V3 out_d, V3* odx, V3* ody, V3* ddx, V3* ddy) {
+    V3 nn = normalized(gn);
+    const V3 hh = normalized(add(normalized(scale(in_d, -1.0f)), normalized(out_d)));
+    *odx = transfer_origin_diff(t, nn, in_d, in_odx, in_ddx);
+    *ody = transfer_origin_diff(t, nn, in_d, in_ody, in_ddy);
+    float dux, dvx, duy, dvy;
+    uv_differentials(*odx, dpdu, dpdv, &dux, &dvx);
+    uv_differentials(*ody, dpdu, dpdv, &duy, &dvy);
+    V3 dndx = add(scale(dndu, dux), scale(dndv, dvx));
+    V3 dndy = add(scale(dndu, duy), scale(dndv, dvy));
+    if (dot(nn, hh) < 0.0f) {
+        nn = scale(nn, -1.0f);
+        dndx = scale(dndx, -1.0f);
+        dndy = scale(dndy, -1.0f);
+    }
+    const V3 axis = normalized(cross(nn, hh));
+    const float angle = acosf(uclamp(dot(nn, hh), 0.0f, 1.0f));
+    const M44 xf = axis_angle_matrix(axis, angle);
+    const V3 dhdx = m_dir(xf, dndx);
+    const V3 dhdy = m_dir(xf, dndy);
+    *ddx = reflect_dd(hh, dhdx, in_d, in_ddx);
+    *ddy = reflect_dd(hh, dhdy, in_d, in_ddy);
+    const float len_d = length(out_d);
+    const float len_dx = length(*ddx);
+    const float len_dy = length(*ddy);
+    if ((len_dx / len_d) > 0.9f) *ddx = scale(*ddx, 0.9f * len_d / len_dx);
+    if ((len_dy / len_d) > 0.9f) *ddy = scale(*ddy, 0.9f * len_d / len_dy);
+}
+
 // ---- lights -----------------------------------------------------------------
 PSY_DEV void rect_light_at(const float* d, int ts, float time, float* w, float* h, float col[3]) {
     float na = 0.0f;
@@ -394,6 +585,10 @@ struct LQuery {
     float scale;      // that emission scale
     int32_t obj;      // the sampled light object
     float selection_pdf, light_sample_pdf;
+    // the closure whose estimate weights the tree's nodes: Lambert, or GTR
+    // with this validated roughness and tail shape
+    int32_t bsdf;
+    float g_rough, g_tail;
 };
 
 PSY_DEV_FN void sphere_light_sample(const DevSpectrum& SP, const float* od, int ts, LQuery& q, float* pdf) {
@@ -498,7 +693,9 @@ PSY_DEV_FN float node_prob(const DevLights& L, const LQuery& q, int index) {
         const float s2 = smin(1.0f, r2 / dist2);
         cos_theta_max = sqrtf(1.0f - s2);
     }
-    const float approx = smax(0.0f, lambert_estimate(q.d, d, cos_theta_max, q.nor));
+    const float est = q.bsdf == PSY_SHADER_GTR ? gtr_estimate(q.g_rough, q.g_tail, q.d, d, cos_theta_max, q.nor)
+                                               : lambert_estimate(q.d, d, cos_theta_max, q.nor);
+    const float approx = smax(0.0f, est);
     return nd.energy * inv_surface_area * approx;
 }
 
@@ -882,11 +1079,13 @@ PSY_DEV bool render_update_path(const RenderLaunch& A, int64_t first, const Path
         for (int k = 0; k < NSPEC; ++k) { col[k] = P.f(P_COL + k, i); fcol[k] = P.f(P_FCOL + k, i); }
         if (step & 1) {
             // result of a shadow ray: unoccluded -> add the light sample
-            if (!hit) {
-                const uint32_t sh = P.u(P_SHADER, i);
-                const float* alb = s.shaders[sh].color;
+            // (the adjoint takes Lambert and Emit shaders only: its scene has no GTR)
+            const DevShader SH = s.shaders[P.u(P_SHADER, i)];
+            const bool sgtr = !Adj::on && SH.kind == PSY_SHADER_GTR;
+            if (!hit && !(sgtr && SH.roughness == 0.0f)) {
                 float fac[NSPEC];
-                lambert_eval(A.spec, alb, P.v3(P_PD, i), rd, P.v3(P_GN, i), P.f(P_WL, i), fac);
+                if (sgtr) gtr_eval(A.spec, SH, P.v3(P_PD, i), rd, P.v3(P_GN, i), P.f(P_WL, i), fac);
+                else lambert_eval(A.spec, SH.color, P.v3(P_PD, i), rd, P.v3(P_GN, i), P.f(P_WL, i), fac);
                 for (int k = 0; k < NSPEC; ++k) col[k] += fcol[k] * P.f(P_LCOL + k, i) * fac[k];
                 if constexpr (Adj::on) {
                     const Adj& Z = adj;
@@ -949,6 +1148,17 @@ PSY_DEV bool render_update_path(const RenderLaunch& A, int64_t first, const Path
                 P.set3(P_POFF, i, po);
                 P.f(P_T, i) = t;
                 P.setu(P_SHADER, i, (uint32_t)I.shader);
+                if (!Adj::on && s.shaders[I.shader].kind == PSY_SHADER_GTR) {
+                    // the rest of the world-space DifferentialGeometry
+                    // (basics/differential_geometry.hpp:20-40), which the
+                    // GTR closure's differentials need
+                    IsectRec R;
+                    intersection_record(s, hinst, t, hr.u, hr.v, hr.aux, ro, rd, time, R);
+                    P.set3(P_DPDU, i, m_dir(inv, R.dpdu));
+                    P.set3(P_DPDV, i, m_dir(inv, R.dpdv));
+                    P.set3(P_DNDU, i, scale(m_dir_transpose(space, R.dndu), il));
+                    P.set3(P_DNDV, i, scale(m_dir_transpose(space, R.dndv), il));
+                }
                 if constexpr (Adj::on) adj.Q.setu(Q_CHAIN + (step >> 1), i, (uint32_t)I.shader);
                 P.set3(P_PD, i, rd);
                 P.set3(P_PODX, i, ray.odx);
@@ -975,8 +1185,23 @@ PSY_DEV bool render_update_path(const RenderLaunch& A, int64_t first, const Path
             const V3 gp = P.v3(P_GP, i), gn = P.v3(P_GN, i), po = P.v3(P_POFF, i), pd = P.v3(P_PD, i);
             const float ht = P.f(P_T, i);
             const float time = P.f(P_TIME, i);
-            const float* alb = s.shaders[P.u(P_SHADER, i)].color;
-            if (nstep & 1) {
+            const DevShader SH = s.shaders[P.u(P_SHADER, i)];
+            const float* alb = SH.color;
+            const bool gtr = !Adj::on && SH.kind == PSY_SHADER_GTR;
+            const bool delta = gtr && SH.roughness == 0.0f;  // GTRClosure::is_delta
+            if ((nstep & 1) && delta) {
+                // a delta closure takes no light sample (and no sampler
+                // dimensions): a shadow ray of zero light, whose result is
+                // not used
+                for (int k = 0; k < NSPEC; ++k) P.f(P_LCOL + k, i) = 0.0f;
+                if constexpr (Adj::on) {
+                    adj.Q.f(Q_LB, i) = 0.0f;
+                    adj.Q.setu(Q_LOBJ, i, 0xffffffffu);
+                }
+                no = add(gp, po);
+                nd = mk(1e-20f, 1e-20f, 1e-20f);
+                ntype = PSY_RAY_OCCLUSION;
+            } else if (nstep & 1) {
                 // shadow ray towards a light sample
                 V3 lq_nor = gn;
                 if (stepw & STEP_BACKFACING) lq_nor = scale(lq_nor, -1.0f);
@@ -999,12 +1224,15 @@ PSY_DEV bool render_update_path(const RenderLaunch& A, int64_t first, const Path
                 q.obj = -1;
                 q.selection_pdf = 1.0f;
                 q.light_sample_pdf = 1.0f;
+                q.bsdf = gtr ? PSY_SHADER_GTR : PSY_SHADER_LAMBERT;
+                q.g_rough = SH.roughness;
+                q.g_tail = SH.tail_shape;
                 float lcol[NSPEC] = {0.0f, 0.0f, 0.0f, 0.0f}, lbase = 0.0f;
                 if (!light_tree_sample(s, A.L, A.spec, q)) {
                     q.to_light = mk(0.0f, 0.0f, 1.0f);
                     q.obj = -1;
                 } else {
-                    const float bsdf_pdf = lambert_pdf(pd, q.to_light, gn);
+                    const float bsdf_pdf = gtr ? gtr_pdf(SH, pd, q.to_light, gn) : lambert_pdf(pd, q.to_light, gn);
                     const float a2 = q.light_sample_pdf * q.light_sample_pdf;
                     const float b2 = bsdf_pdf * bsdf_pdf;
                     const float mis_inv_pdf = (a2 / (a2 + b2)) / q.light_sample_pdf;
@@ -1021,6 +1249,24 @@ PSY_DEV bool render_update_path(const RenderLaunch& A, int64_t first, const Path
                 no = add(gp, po);
                 nd = scale(sub(q.to_light, po), 0.9995f);
                 ntype = PSY_RAY_OCCLUSION;
+            } else if (gtr) {
+                // bounce ray: GTRClosure::sample
+                const float su = next_dim(A, off, dim);
+                const float sv = next_dim(A, off, dim);
+                nd = gtr_sample_dir(SH, pd, gn, su, sv);
+                float pdf = gtr_pdf(SH, pd, nd, gn);
+                float filt[NSPEC];
+                gtr_eval(A.spec, SH, pd, nd, gn, P.f(P_WL, i), filt);
+                no = add(gp, po);
+                ntype = 2u;  // WorldRay::R_DIFFUSE
+                if (!delta) {
+                    if (pdf < 0.00001f) pdf = 0.00001f;
+                    for (int k = 0; k < NSPEC; ++k) fcol[k] *= filt[k] / pdf;
+                    P.f(P_LASTPDF, i) = pdf;
+                } else {
+                    for (int k = 0; k < NSPEC; ++k) fcol[k] *= filt[k];
+                    P.f(P_LASTPDF, i) = 0.0f;
+                }
             } else {
                 // bounce ray: LambertClosure::sample
                 float su = next_dim(A, off, dim);
@@ -1044,17 +1290,26 @@ PSY_DEV bool render_update_path(const RenderLaunch& A, int64_t first, const Path
                 if constexpr (Adj::on) adj.Q.f(Q_B, i) *= lambert_eval1(pd, nd, gn) / pdf;
                 P.f(P_LASTPDF, i) = pdf;
             }
-            // LambertClosure::propagate_differentials
-            const float len = length(nd);
-            const V3 gnn = normalized(gn);
-            V3 x, y;
-            coordinate_system_from_vec3(nd, &x, &y);
-            x = normalized(x);
-            y = normalized(y);
-            nodx = transfer_origin_diff(ht, gnn, pd, P.v3(P_PODX, i), P.v3(P_PDDX, i));
-            nody = transfer_origin_diff(ht, gnn, pd, P.v3(P_PODY, i), P.v3(P_PDDY, i));
-            nddx = sdiv(scale(x, 0.15f), len);
-            nddy = sdiv(scale(y, 0.15f), len);
+            if (gtr) {
+                // GTRClosure::propagate_differentials (a delta closure's
+                // zero-light shadow ray keeps zero differentials)
+                if (!((nstep & 1) && delta))
+                    gtr_differentials(ht, pd, P.v3(P_PODX, i), P.v3(P_PODY, i), P.v3(P_PDDX, i), P.v3(P_PDDY, i), gn,
+                                      P.v3(P_DPDU, i), P.v3(P_DPDV, i), P.v3(P_DNDU, i), P.v3(P_DNDV, i), nd, &nodx,
+                                      &nody, &nddx, &nddy);
+            } else {
+                // LambertClosure::propagate_differentials
+                const float len = length(nd);
+                const V3 gnn = normalized(gn);
+                V3 x, y;
+                coordinate_system_from_vec3(nd, &x, &y);
+                x = normalized(x);
+                y = normalized(y);
+                nodx = transfer_origin_diff(ht, gnn, pd, P.v3(P_PODX, i), P.v3(P_PDDX, i));
+                nody = transfer_origin_diff(ht, gnn, pd, P.v3(P_PODY, i), P.v3(P_PDDY, i));
+                nddx = sdiv(scale(x, 0.15f), len);
+                nddy = sdiv(scale(y, 0.15f), len);
+            }
             ntime = time;
             P.setu(P_DIM, i, dim);
             alive = true;

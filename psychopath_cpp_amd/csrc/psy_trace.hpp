@@ -568,8 +568,9 @@ PSY_DEV void sphere_at(const float* d, int ts, int stride, float time, V3* cent,
     *radi = lerpf(na, a[3], b[3]);
 }
 
-// RectangleLight::intersect_ray (lights/rectangle_light.hpp:110-147)
-PSY_DEV bool rect_t(const float* d, int ts, float time, V3 o, V3 dir, V3 dinv, float max_t,
+// RectangleLight::intersect_ray (lights/rectangle_light.hpp:110-147);
+// `fix`: test y against the height (PSY_SCENE_RECT_LIGHT_FIX)
+PSY_DEV bool rect_t(const float* d, int ts, float time, V3 o, V3 dir, V3 dinv, float max_t, bool fix,
                                        float* t_out) {
     if (dir.z == 0.0f) return false;
     float wd, ht;
@@ -587,8 +588,8 @@ PSY_DEV bool rect_t(const float* d, int ts, float time, V3 o, V3 dir, V3 dinv, f
     if (t <= 0.0f || t > max_t) return false;
     const float x = o.x + (dir.x * t);
     const float y = o.y + (dir.y * t);
-    // the y test compares against the width, as the reference does
-    if (x >= (wd * -0.5f) && x <= (wd * 0.5f) && y >= (ht * -0.5f) && y <= (wd * 0.5f)) {
+    // the y test compares against the width, as the reference does, unless fixed
+    if (x >= (wd * -0.5f) && x <= (wd * 0.5f) && y >= (ht * -0.5f) && y <= ((fix ? ht : wd) * 0.5f)) {
         *t_out = t;
         return true;
     }
@@ -674,7 +675,7 @@ PSY_DEV bool intersect_leaf(const DevScene& s, uint32_t li, const DevInstance& I
             return false;
         }
         case K_RECT_LIGHT:
-            if (rect_t(od, ob.ts, r.time, o, d, dinv, b.t, &t)) {
+            if (rect_t(od, ob.ts, r.time, o, d, dinv, b.t, (s.flags & PSY_SCENE_RECT_LIGHT_FIX) != 0u, &t)) {
                 b.t = t; b.inst = li; b.aux = 0.0f; b.u = 0.0f; b.v = 0.0f;
                 return r.occl;
             }

@@ -82,7 +82,7 @@ def lib():
     L.psy_scene_get_params.argtypes = [vp, vp, vp]
     L.psy_scene_set_params.argtypes = [vp, vp, i64, vp, i64]
     L.psy_adjoint.argtypes = [vp, vp, C.POINTER(abi.RenderParams), vp, vp, vp, C.POINTER(i64), vp]
-    if L.psy_abi_version() != 1:
+    if L.psy_abi_version() != 2:
         raise PsyError("libpsy_gpu.so ABI version mismatch")
     _lib = L
     return L

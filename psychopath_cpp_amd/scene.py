@@ -126,17 +126,24 @@ def _fptr(arr):
 
 
 class Scene:
-    def __init__(self, camera, root, background=(0.0, 0.0, 0.0), dice_rate=0.25, min_upoly_size=0.00001):
+    def __init__(self, camera, root, background=(0.0, 0.0, 0.0), dice_rate=0.25, min_upoly_size=0.00001,
+                 rect_light_fix=False):
         self.camera = camera
         self.root = root
         self.background = tuple(float(b) for b in background)
         self.dice_rate = float(dice_rate)
         self.min_upoly_size = float(min_upoly_size)
+        # PSY_SCENE_RECT_LIGHT_FIX: rectangle lights test y against their
+        # height (the reference tests it against the width)
+        self.rect_light_fix = bool(rect_light_fix)
         self.shaders = []
         self.shader_map = {}
 
-    def add_surface_shader(self, name, kind, color):
-        self.shaders.append((int(kind), tuple(float(c) for c in color)))
+    def add_surface_shader(self, name, kind, color, roughness=0.1, tail_shape=2.0, fresnel=0.25):
+        """A LambertShader, EmitShader or GTRShader (shading/surface_shader.hpp:28-71); the last three
+        arguments are the GTR shader's (parser/parser.cpp:511-546 defaults)."""
+        self.shaders.append((int(kind), tuple(float(c) for c in color), float(roughness), float(tail_shape),
+                             float(fresnel)))
         self.shader_map[name] = len(self.shaders) - 1
         return True
 
@@ -176,8 +183,8 @@ class Scene:
                                           len(a.xforms), subs.ctypes.data_as(C.POINTER(C.c_int32)),
                                           len(a.assemblies))
         shaders = (abi.ShaderDesc * max(1, len(self.shaders)))()
-        for i, (kind, col) in enumerate(self.shaders):
-            shaders[i] = abi.ShaderDesc(kind, (C.c_float * 3)(*col))
+        for i, (kind, col, rough, tail, fres) in enumerate(self.shaders):
+            shaders[i] = abi.ShaderDesc(kind, (C.c_float * 3)(*col), rough, tail, fres)
         cam = self.camera
         cxf = np.ascontiguousarray(np.array(cam.transforms, np.float32).reshape(-1, 16))
         fovs = np.array(cam.fovs or [0.0], np.float32)
@@ -187,7 +194,8 @@ class Scene:
         cdesc = abi.CameraDesc(_fptr(cxf), len(cam.transforms), _fptr(fovs), len(cam.fovs), _fptr(aps),
                                len(cam.aperture_radii), _fptr(fds), len(cam.focus_distances))
         desc = abi.SceneDesc(adescs, len(order), shaders, len(self.shaders), cdesc,
-                             (C.c_float * 3)(*self.background), self.dice_rate, self.min_upoly_size)
+                             (C.c_float * 3)(*self.background), self.dice_rate, self.min_upoly_size,
+                             abi.PSY_SCENE_RECT_LIGHT_FIX if self.rect_light_fix else 0)
         return SceneHandle(desc, keep)
 
 

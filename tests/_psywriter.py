@@ -73,7 +73,11 @@ def write_psy(sc, width=64, height=48, spp=4, seed=1, name="frame"):
     out += cam + ["    }",
                   f"    World {{ BackgroundShader {{ Type [Color] Color [{nums(sc.background)}] }} }}",
                   "    Assembly {"]
-    for i, (kind, col) in enumerate(sc.shaders):
+    for i, (kind, col, rough, tail, fres) in enumerate(sc.shaders):
+        if kind == abi.PSY_SHADER_GTR:
+            out.append(f"        SurfaceShader $s{i} {{ Type [GTR] Color [{nums(col)}] Roughness [{num(rough)}] "
+                       f"TailShape [{num(tail)}] Fresnel [{num(fres)}] }}")
+            continue
         t = "Lambert" if kind == abi.PSY_SHADER_LAMBERT else "Emit"
         out.append(f"        SurfaceShader $s{i} {{ Type [{t}] Color [{nums(col)}] }}")
     _assembly(sc, sc.root, [], 2, out)

@@ -174,7 +174,7 @@ def test_cpu_build_adjoint_matches_finite_differences():
     sc = configs.adjoint_scene(n_spheres=16, n_lights=12)
     n_sh, n_l = hostbuild.param_counts(sc.to_desc())
     W, H, S = 32, 24, 4
-    alb0 = np.array([c[0] for _, c in sc.shaders], np.float32)
+    alb0 = np.array([c[1][0] for c in sc.shaders], np.float32)
     alb, esc = J.perturbed(alb0, n_l, seed=0)
     o = ora.OracleScene(sc)
 

@@ -20,7 +20,7 @@ def say(*a):
 
 
 def _albedo0(sc):
-    return np.array([c[0] for _, c in sc.shaders], np.float32)
+    return np.array([c[1][0] for c in sc.shaders], np.float32)
 
 
 def test_gpu_adjoint_matches_oracle_fd(gpu_ctx, oracle):

@@ -129,12 +129,17 @@ def test_scene_settings_camera_and_world():
 def test_assembly_objects_instances_and_warnings():
     p = psyfile.ParsedScene(SCENE)
     d = p.desc()
-    # shaders: the renderable ones in order; Lambert without Color is 0.9 grey
-    assert d.n_shaders == 3
-    sh = [(d.shaders[i].kind, tuple(d.shaders[i].color)) for i in range(3)]
+    # shaders: the known types in order; Lambert without Color is 0.9 grey;
+    # GTR takes the reference parser's defaults for the fields it lacks
+    # (parser/parser.cpp:512-515: colour 0.9, roughness 0.1, tail 2, Fresnel 0.25)
+    assert d.n_shaders == 4
+    sh = [(d.shaders[i].kind, tuple(d.shaders[i].color)) for i in range(4)]
     assert sh[0] == (abi.PSY_SHADER_LAMBERT, tuple(_f([0.8, 0.1, 0.1])))
     assert sh[1] == (abi.PSY_SHADER_EMIT, (4.0, 3.0, 2.0))
     assert sh[2] == (abi.PSY_SHADER_LAMBERT, tuple(_f([0.9, 0.9, 0.9])))
+    assert sh[3] == (abi.PSY_SHADER_GTR, tuple(_f([0.9, 0.9, 0.9])))
+    g = d.shaders[3]
+    assert (g.roughness, g.tail_shape, g.fresnel) == tuple(_f([0.2, 2.0, 0.25]))
     assert d.n_assemblies == 2
     A = d.assemblies[0]
     kinds = [A.objects[i].kind for i in range(A.n_objects)]
@@ -169,7 +174,6 @@ def test_assembly_objects_instances_and_warnings():
 
 
 @pytest.mark.parametrize("bad,msg", [
-    ("Instance { Data [$ball] SurfaceShaderBind [$shiny] }", "GTR"),
     ("SphereLight $l2 { Location [0 0 1] Location [0 0 2] Radius [1] Radius [2] Radius [3] Color [1 1 1] }",
      "motion sample counts"),
     ("SphereLight $l3 { Radius [1] Color [1 1 1] }", "needs Location"),
@@ -254,7 +258,10 @@ def test_reference_example_file():
     d = p.desc()
     assert tuple(d.background) == tuple(_f([0.8, 0.8, 0.8]))
     assert d.camera.n_transforms == 2 and d.camera.fovs[0] == DEG * np.float32(0.785398163)
-    assert d.n_shaders == 1 and d.shaders[0].kind == abi.PSY_SHADER_LAMBERT
+    # $grey_diffuse, then $mirror: a GTR of roughness 0 (a perfect mirror)
+    assert d.n_shaders == 2 and d.shaders[0].kind == abi.PSY_SHADER_LAMBERT
+    m = d.shaders[1]
+    assert m.kind == abi.PSY_SHADER_GTR and (m.roughness, m.tail_shape, m.fresnel) == tuple(_f([0.0, 2.0, 0.25]))
     A = d.assemblies[0]
     assert A.n_objects == 1 and A.objects[0].kind == abi.PSY_OBJ_SPHERE_LIGHT
     assert [A.instances[i].type for i in range(A.n_instances)] == [abi.PSY_INSTANCE_OBJECT, abi.PSY_INSTANCE_ASSEMBLY]

@@ -421,6 +421,34 @@ typedef struct psy_render_params {
 int psy_render(psy_ctx* ctx, const psy_scene* scene, const psy_render_params* params,
                float* film_xyz, float* sample_xyz, int64_t* rays_traced, void* stream);
 
+/* Adaptive sampling: PathTraceIntegrator(scene, film, spp, spp_max,
+ * variance_max, seed, thread_count) + integrate() (integrator/
+ * path_trace_integrator.cpp:28-90, 282-384) with its Film statistics
+ * (film/film.hpp:135-165).  The image is cut into the reference's square
+ * buckets (psy_bucket_size: sqrt(65536 / spp), at most
+ * sqrt(width * height / (4 thread_count)), at least 1; row-major bucket
+ * order); a bucket takes passes of spp samples per pixel while its largest
+ * X variance estimate exceeds variance_max and fewer than spp_max (at least
+ * spp) samples were taken.  Each pass of the reference draws the same
+ * samples (Sampler(x, y, s, seed), s = 0 .. spp - 1), so they are rendered
+ * once and each pass adds them to the film statistics again: the film and
+ * pass counts are the reference's, rays_traced is one pass's.
+ *   film_xyz     : device, width*height*3, or NULL: the film's per-pixel
+ *                  sum / count (CIE XYZ, as psy_render's film)
+ *   variance_xyz : device, width*height*3, or NULL: variance_estimate
+ *                  (+inf below 2 samples)
+ *   passes       : device, one int32 per bucket, or NULL */
+int32_t psy_bucket_size(int32_t width, int32_t height, int32_t spp, int32_t thread_count);
+int psy_render_adaptive(psy_ctx* ctx, const psy_scene* scene, const psy_render_params* params,
+                        int32_t spp_max, float variance_max, int32_t thread_count,
+                        float* film_xyz, float* variance_xyz, int32_t* passes,
+                        int64_t* rays_traced, void* stream);
+/* The film part alone, over per-sample values (sample_xyz: device, the
+ * layout psy_render writes) and a given bucket size. */
+int psy_film_adaptive(psy_ctx* ctx, const float* sample_xyz, int32_t width, int32_t height, int32_t spp,
+                      int32_t spp_max, float variance_max, int32_t bucket_size,
+                      float* film_xyz, float* variance_xyz, int32_t* passes, void* stream);
+
 /* The render autotuner (no counterpart in the reference): times psy_render
  * on every pixel at an spp that keeps to max_samples samples (<= 0: 2^23),
  * by coordinate descent over the render form (PSY_RENDER_STREAM or

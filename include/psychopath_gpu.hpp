@@ -857,15 +857,13 @@ struct Film {
 
 class PathTraceIntegrator {
 public:
-    // variance_max and spp_max (adaptive sampling) are not used: every
-    // pixel takes spp samples; thread_count is the reference's CPU threads.
+    // spp_max and variance_max drive the reference's adaptive loop over
+    // buckets (psy_render_adaptive); thread_count sets the bucket size, as
+    // in the reference, and nothing else.
     PathTraceIntegrator(Scene* scene_, Film* image_, int spp_, int spp_max_, float variance_max_, uint32_t seed_,
                         int thread_count_ = 1, std::function<void()> callback_ = std::function<void()>())
-        : scene(scene_), image(image_), spp(spp_), seed(seed_), callback(std::move(callback_)) {
-        (void)spp_max_;
-        (void)variance_max_;
-        (void)thread_count_;
-    }
+        : scene(scene_), image(image_), spp(spp_), spp_max(spp_max_), variance_max(variance_max_), seed(seed_),
+          thread_count(thread_count_), callback(std::move(callback_)) {}
     int path_length = 4;  // the reference integrator's bounces
 
     void integrate() {
@@ -881,7 +879,11 @@ public:
         p.seed = seed;
         p.path_length = path_length;
         int64_t rays = 0;
-        int rc = psy_render(ctx, scene->handle, &p, (float*)film, nullptr, &rays, nullptr);
+        // with spp_max above spp, the adaptive loop (whose single pass is
+        // psy_render's film)
+        int rc = spp_max > spp ? psy_render_adaptive(ctx, scene->handle, &p, spp_max, variance_max, thread_count,
+                                                     (float*)film, nullptr, nullptr, &rays, nullptr)
+                               : psy_render(ctx, scene->handle, &p, (float*)film, nullptr, &rays, nullptr);
         if (rc == PSY_OK) rc = psy_copy_to_host(ctx, image->xyz.data(), film, bytes);
         psy_free(ctx, film);
         check(rc, "PathTraceIntegrator::integrate");
@@ -891,8 +893,10 @@ public:
 
     Scene* scene;
     Film* image;
-    int spp;
+    int spp, spp_max;
+    float variance_max;
     uint32_t seed;
+    int thread_count;
     std::function<void()> callback;
     int64_t rays_shot = 0;  // Global::Stats::rays_shot of the last integrate()
 };

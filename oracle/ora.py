@@ -127,6 +127,7 @@ def lib():
         L.ora_kat_gtr_estimate.restype = f
         L.ora_kat_gtr_estimate.argtypes = [vp, vp, vp, f, vp]
         L.ora_kat_axis_angle.argtypes = [vp, f, vp]
+        L.ora_film_adaptive.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, f, C.c_int, vp, vp, vp]
         L.ora_kat_light_select.restype = C.c_int
         L.ora_kat_light_select.argtypes = [vp, vp, vp, vp, f, f, f, f, vp, vp, vp, vp]
         L.ora_kat_total_lights.restype = i64
@@ -407,3 +408,17 @@ def planes_from_wrays(w):
     p[18] = w["time"]
     p[19] = w["type"].astype(np.uint32).view(np.float32)
     return p
+
+
+def film_adaptive(samples, spp_max, variance_max, bucket_size):
+    """ora_film_adaptive over samples of shape (H, W, spp, 3): (film (H, W, 3),
+    variance (H, W, 3), passes per bucket in row-major bucket order)."""
+    s = np.ascontiguousarray(samples, np.float32)
+    H, W, spp, _ = s.shape
+    film = np.zeros((H, W, 3), np.float32)
+    var = np.zeros((H, W, 3), np.float32)
+    nb = ((W + bucket_size - 1) // bucket_size) * ((H + bucket_size - 1) // bucket_size)
+    passes = np.zeros(nb, np.int32)
+    lib().ora_film_adaptive(_p(s), W, H, spp, int(spp_max), np.float32(variance_max), int(bucket_size), _p(film),
+                            _p(var), _p(passes))
+    return film, var, passes

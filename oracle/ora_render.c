@@ -1183,3 +1183,77 @@ void ora_kat_axis_angle(const float axis[3], float angle, float m[16]) {
     const mat4 x = axis_angle(va(axis), angle);
     memcpy(m, &x.m[0][0], sizeof(float) * 16);
 }
+
+/* ---- adaptive sampling film ------------------------------------------------
+ * Film::add_sample / variance_estimate (film/film.hpp:135-165) in the
+ * adaptive loop of PathTraceIntegrator::render_blocks
+ * (integrator/path_trace_integrator.cpp:296-376), over per-sample XYZ values
+ * (width*height*spp*3, ordered (y, x, s)) that every pass of a bucket adds
+ * again (each pass draws Sampler(x, y, s, seed) for s = 0 .. spp - 1).
+ * Buckets are bucket_size squares in row-major order; film = sum / count,
+ * var = variance_estimate, passes one int per bucket. */
+static float film_hcol(float n) {
+    if (n < 0.0f) n = 0.0f;
+    return powf(n, 1.0f / 2.2f);
+}
+static float film_mmax(float a, float b) { return a > b ? a : b; }
+
+void ora_film_adaptive(const float* samples, int W, int H, int spp, int spp_max, float var_max, int bs,
+                       float* film, float* var, int32_t* passes) {
+    const int nbx = (W + bs - 1) / bs, nby = (H + bs - 1) / bs;
+    const size_t P = (size_t)W * H;
+    float* sum = (float*)calloc(P * 3, sizeof(float));
+    float* vp = (float*)calloc(P * 3, sizeof(float));
+    float* vf = (float*)calloc(P * 3, sizeof(float));
+    uint16_t* acc = (uint16_t*)calloc(P, sizeof(uint16_t));
+    for (int by = 0; by < nby; ++by) {
+        for (int bx = 0; bx < nbx; ++bx) {
+            const int x0 = bx * bs, y0 = by * bs;
+            const int bw = W - x0 < bs ? W - x0 : bs, bh = H - y0 < bs ? H - y0 : bs;
+            float max_var = 9999999999.0f, samp_it = 0.0f;
+            int np = 0;
+            while (max_var > var_max && samp_it < (float)spp_max) {
+                /* add_sample in the reference's order: x, then y, then s */
+                for (int x = x0; x < x0 + bw; ++x) {
+                    for (int y = y0; y < y0 + bh; ++y) {
+                        const size_t pix = (size_t)y * W + x;
+                        for (int s = 0; s < spp; ++s) {
+                            const float* v = samples + (pix * spp + s) * 3;
+                            int ok = 1;
+                            for (int c = 0; c < 3; ++c)
+                                if (isnan(v[c]) || !isfinite(v[c])) ok = 0;
+                            if (!ok) continue;
+                            for (int c = 0; c < 3; ++c) sum[pix * 3 + c] += v[c];
+                            acc[pix]++;
+                            const uint16_t k = acc[pix];
+                            for (int c = 0; c < 3; ++c) {
+                                const float avg = film_hcol(sum[pix * 3 + c] / (float)k);
+                                if (k > 1) vf[pix * 3 + c] += fabsf(vp[pix * 3 + c] - avg) * (float)(k - 1);
+                                vp[pix * 3 + c] = avg;
+                            }
+                        }
+                    }
+                }
+                samp_it += (float)spp;
+                ++np;
+                max_var = 0.0f;
+                for (int x = x0; x < x0 + bw; ++x)
+                    for (int y = y0; y < y0 + bh; ++y) {
+                        const size_t pix = (size_t)y * W + x;
+                        const float ve = acc[pix] < 2 ? INFINITY
+                                                      : (vf[pix * 3] / (float)(acc[pix] - 1)) / (float)sqrt((double)acc[pix]);
+                        max_var = film_mmax(max_var, ve);
+                    }
+            }
+            if (passes) passes[by * nbx + bx] = np;
+        }
+    }
+    for (size_t pix = 0; pix < P; ++pix)
+        for (int c = 0; c < 3; ++c) {
+            if (film) film[pix * 3 + c] = acc[pix] ? sum[pix * 3 + c] / (float)acc[pix] : 0.0f;
+            if (var)
+                var[pix * 3 + c] = acc[pix] < 2 ? INFINITY
+                                                : (vf[pix * 3 + c] / (float)(acc[pix] - 1)) / (float)sqrt((double)acc[pix]);
+        }
+    free(sum); free(vp); free(vf); free(acc);
+}

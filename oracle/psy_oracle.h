@@ -181,6 +181,11 @@ void ora_kat_gtr_sample(const float prm[3], const float in[3], const float n[3],
 float ora_kat_gtr_estimate(const float prm[3], const float in[3], const float cent[3], float cos_theta,
                            const float nor[3]);
 void ora_kat_axis_angle(const float axis[3], float angle, float m[16]);
+/* The adaptive sampling loop's film (film/film.hpp:135-165 in
+ * integrator/path_trace_integrator.cpp:296-376) over per-sample XYZ values
+ * that every pass of a bucket adds again */
+void ora_film_adaptive(const float* samples, int W, int H, int spp, int spp_max, float var_max, int bs,
+                       float* film, float* var, int32_t* passes);
 
 #ifdef __cplusplus
 }

@@ -185,6 +185,9 @@ int trace_persist_occupancy(bool world, int stack, int* blocks_per_cu);
 int static_scratch_address(int device, uint32_t** out);
 int launch_world_to_rays(const float* wrays, int64_t n, float* rays, hipStream_t st);
 // Intersection records (PSY_ISECT_PLANES planes) of the hits of world rays.
+// Film::add_sample / variance_estimate passes per bucket (state: 10 W*H floats)
+int launch_film_adaptive(const float* samples, int W, int H, int spp, int bs, int spp_max, float var_max, float* state,
+                         float* film, float* var_out, int32_t* passes, hipStream_t st);
 int launch_intersections(const DevScene& s, const float* wrays, int64_t n, const DevHits& h, float* out,
                          hipStream_t st);
 

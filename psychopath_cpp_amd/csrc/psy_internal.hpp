@@ -146,6 +146,7 @@ struct psy_ctx {
     }
     // integrator scratch (path states, ray / hit streams, active lists)
     psy::DBuf render_scratch;
+    psy::DBuf film_state, film_samples;  // psy_film_adaptive / psy_render_adaptive
     int64_t render_chunk = 1 << 21;  // paths per chunk
     int render_mode = PSY_RENDER_WAVEFRONT;
     int render_refill = 16;          // fused mode: refill a wave at this many idle lanes

@@ -1086,4 +1086,107 @@ int static_scratch_address(int device, uint32_t** out) {
     return 0;
 }
 
+
+// ---- adaptive sampling film ---------------------------------------------------
+// Film::add_sample and Film::variance_estimate (film/film.hpp:135-165) in
+// the loop of PathTraceIntegrator::render_blocks
+// (integrator/path_trace_integrator.cpp:296-376): a bucket takes passes of
+// spp samples per pixel while the largest X variance estimate over its
+// pixels exceeds variance_max and fewer than spp_max samples were taken.
+// Every pass of the reference draws its samples from Sampler(x, y, s,
+// seed) with s = 0 .. spp - 1, the same sampler offsets every time, so each
+// pass adds the same samples: they are rendered once and added pass after
+// pass here.  One block per bucket; per-pixel state in 10 planes of W*H
+// (XYZ sum, count as uint16 bits, last gamma mean, the running variance sum).
+constexpr int FILM_THREADS = 256;
+
+PSY_DEV float film_hcol(float n) {
+    if (n < 0.0f) n = 0.0f;
+    return powf(n, 1.0f / 2.2f);
+}
+PSY_DEV float film_mmax(float a, float b) { return a > b ? a : b; }  // film/film.hpp:56-61
+
+__global__ __launch_bounds__(FILM_THREADS) void k_film_adaptive(const float* __restrict__ samples, int W, int H,
+                                                                int spp, int bs, int nbx, int spp_max, float var_max,
+                                                                float* __restrict__ st, float* __restrict__ film,
+                                                                float* __restrict__ var_out, int32_t* __restrict__ passes) {
+    __shared__ float red[FILM_THREADS];
+    const int b = blockIdx.x;
+    const int x0 = (b % nbx) * bs, y0 = (b / nbx) * bs;
+    const int bw = min(W - x0, bs), bh = min(H - y0, bs);
+    const int npix = bw * bh;
+    const int64_t P = (int64_t)W * H;
+    for (int q = threadIdx.x; q < npix; q += FILM_THREADS) {
+        const int64_t pix = (int64_t)(y0 + q / bw) * W + x0 + q % bw;
+        for (int k = 0; k < 10; ++k) st[k * P + pix] = 0.0f;
+    }
+    float max_var = 9999999999.0f;  // X component of the bucket's max_variance
+    float samp_it = 0.0f;
+    int np = 0;
+    while (max_var > var_max && samp_it < (float)spp_max) {
+        float m = 0.0f;
+        for (int q = threadIdx.x; q < npix; q += FILM_THREADS) {
+            const int64_t pix = (int64_t)(y0 + q / bw) * W + x0 + q % bw;
+            float sum[3], vp[3], vf[3];
+            for (int c = 0; c < 3; ++c) {
+                sum[c] = st[c * P + pix];
+                vp[c] = st[(4 + c) * P + pix];
+                vf[c] = st[(7 + c) * P + pix];
+            }
+            uint16_t acc = (uint16_t)f2u(st[3 * P + pix]);
+            const float* sp = samples + pix * spp * 3;
+            for (int s = 0; s < spp; ++s) {
+                const float v0 = sp[3 * s], v1 = sp[3 * s + 1], v2 = sp[3 * s + 2];
+                // skip NaN and infinite samples
+                if (!(fabsf(v0) < INFINITY && fabsf(v1) < INFINITY && fabsf(v2) < INFINITY)) continue;
+                sum[0] += v0; sum[1] += v1; sum[2] += v2;
+                acc = (uint16_t)(acc + 1);
+                const float k = (float)acc;
+                for (int c = 0; c < 3; ++c) {
+                    const float avg = film_hcol(sum[c] / k);
+                    if (acc > 1) vf[c] += fabsf(vp[c] - avg) * (float)(acc - 1);
+                    vp[c] = avg;
+                }
+            }
+            for (int c = 0; c < 3; ++c) {
+                st[c * P + pix] = sum[c];
+                st[(4 + c) * P + pix] = vp[c];
+                st[(7 + c) * P + pix] = vf[c];
+            }
+            st[3 * P + pix] = u2f((uint32_t)acc);
+            const float ve = acc < 2 ? INFINITY : (vf[0] / (float)(acc - 1)) / (float)sqrt((double)acc);
+            m = film_mmax(m, ve);
+        }
+        red[threadIdx.x] = m;
+        __syncthreads();
+        for (int d = FILM_THREADS / 2; d > 0; d >>= 1) {
+            if ((int)threadIdx.x < d) red[threadIdx.x] = film_mmax(red[threadIdx.x], red[threadIdx.x + d]);
+            __syncthreads();
+        }
+        max_var = red[0];
+        __syncthreads();
+        samp_it += (float)spp;
+        ++np;
+    }
+    for (int q = threadIdx.x; q < npix; q += FILM_THREADS) {
+        const int64_t pix = (int64_t)(y0 + q / bw) * W + x0 + q % bw;
+        const uint16_t acc = (uint16_t)f2u(st[3 * P + pix]);
+        for (int c = 0; c < 3; ++c) {
+            if (film) film[pix * 3 + c] = acc ? st[c * P + pix] / (float)acc : 0.0f;
+            if (var_out)
+                var_out[pix * 3 + c] =
+                    acc < 2 ? INFINITY : (st[(7 + c) * P + pix] / (float)(acc - 1)) / (float)sqrt((double)acc);
+        }
+    }
+    if (threadIdx.x == 0 && passes) passes[b] = np;
+}
+
+int launch_film_adaptive(const float* samples, int W, int H, int spp, int bs, int spp_max, float var_max, float* state,
+                         float* film, float* var_out, int32_t* passes, hipStream_t st) {
+    const int nbx = (W + bs - 1) / bs, nby = (H + bs - 1) / bs;
+    hipLaunchKernelGGL(k_film_adaptive, dim3((unsigned)(nbx * nby)), dim3(FILM_THREADS), 0, st, samples, W, H, spp,
+                       bs, nbx, spp_max, var_max, state, film, var_out, passes);
+    return check_launch();
+}
+
 }  // namespace psy

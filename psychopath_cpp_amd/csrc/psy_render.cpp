@@ -670,6 +670,50 @@ int psy_render(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, f
     return rc ? rc : rr;
 }
 
+// PathTraceIntegrator::integrate's bucket size
+// (integrator/path_trace_integrator.cpp:28-34)
+int32_t psy_bucket_size(int32_t width, int32_t height, int32_t spp, int32_t thread_count) {
+    if (width <= 0 || height <= 0 || spp <= 0 || thread_count <= 0) return -1;
+    const int max_bucket_size = (int)std::sqrt((width * height) / (thread_count * 4.0f));
+    int bucket_size = (int)std::sqrt(static_cast<float>(65536) / spp);  // Config::samples_per_bucket
+    bucket_size = std::min(max_bucket_size, bucket_size);
+    bucket_size = std::max(1, bucket_size);
+    return bucket_size;
+}
+
+int psy_film_adaptive(psy_ctx* ctx, const float* sample_xyz, int32_t width, int32_t height, int32_t spp,
+                      int32_t spp_max, float variance_max, int32_t bucket_size, float* film_xyz, float* variance_xyz,
+                      int32_t* passes, void* stream) {
+    if (!ctx || !sample_xyz) return fail(PSY_ERR_INVALID, "psy_film_adaptive: NULL argument");
+    if (width <= 0 || height <= 0 || spp <= 0 || bucket_size <= 0)
+        return fail(PSY_ERR_INVALID, "psy_film_adaptive: width, height, spp and bucket_size must be positive");
+    if ((int64_t)width * height > ((int64_t)1 << 31) / 10)
+        return fail(PSY_ERR_INVALID, "psy_film_adaptive: image too large");
+    HIP_OK(hipSetDevice(ctx->device));
+    hipStream_t st = (hipStream_t)stream;
+    if (int rc = scratch_order(ctx, st)) return rc;  // film_state is the context's, like the trace scratch
+    if (int rc = ctx->film_state.reserve(sizeof(float) * 10 * (size_t)width * height)) return rc;
+    const int rc = launch_film_adaptive(sample_xyz, width, height, spp, bucket_size, spp_max, variance_max,
+                                        (float*)ctx->film_state.p, film_xyz, variance_xyz, passes, st);
+    if (rc) return fail(rc, "psy_film_adaptive: launch failed");
+    return scratch_release(ctx, st);
+}
+
+int psy_render_adaptive(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, int32_t spp_max,
+                        float variance_max, int32_t thread_count, float* film_xyz, float* variance_xyz,
+                        int32_t* passes, int64_t* rays_traced, void* stream) {
+    if (int rc = check_render(ctx, s, prm, "psy_render_adaptive")) return rc;
+    if (thread_count <= 0) return fail(PSY_ERR_INVALID, "psy_render_adaptive: thread_count must be positive");
+    HIP_OK(hipSetDevice(ctx->device));
+    const size_t bytes = sizeof(float) * 3 * (size_t)prm->width * prm->height * prm->spp;
+    if (int rc = ctx->film_samples.reserve(bytes)) return rc;
+    int rc = psy_render(ctx, s, prm, nullptr, (float*)ctx->film_samples.p, rays_traced, stream);
+    if (rc) return rc;
+    const int32_t bs = psy_bucket_size(prm->width, prm->height, prm->spp, thread_count);
+    return psy_film_adaptive(ctx, (const float*)ctx->film_samples.p, prm->width, prm->height, prm->spp,
+                             std::max(spp_max, prm->spp), variance_max, bs, film_xyz, variance_xyz, passes, stream);
+}
+
 // The render autotuner: coordinate descent over the render form and its
 // paths in flight (wavefront chunk or stream pool), the main pass's launch
 // form and LDS stack depth, and the stream form's steps per captured graph,

@@ -54,6 +54,12 @@ def lib():
     L.psy_ctx_set_trace_stack.argtypes = [vp, C.c_int]
     L.psy_ctx_set_second_pass.argtypes = [vp, u32, i32]
     L.psy_ctx_set_continuation.argtypes = [vp, C.c_int]
+    L.psy_bucket_size.restype = C.c_int32
+    L.psy_bucket_size.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_int32]
+    L.psy_render_adaptive.argtypes = [vp, vp, C.POINTER(abi.RenderParams), C.c_int32, C.c_float, C.c_int32, vp, vp,
+                                      vp, C.POINTER(i64), vp]
+    L.psy_film_adaptive.argtypes = [vp, vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_float, C.c_int32, vp, vp,
+                                    vp, vp]
     L.psy_ctx_set_stream_steps.argtypes = [vp, C.c_int]
     L.psy_render_autotune.argtypes = [vp, vp, C.POINTER(abi.RenderParams), i64, C.POINTER(abi.RenderTuning)]
     L.psy_ctx_trace_timing.argtypes = [vp, vp, vp, vp]
@@ -413,6 +419,41 @@ def render(ctx, scene, width, height, spp, seed=1, path_length=4, film=True, sam
     check(lib().psy_render(ctx.handle, scene.handle, C.byref(prm), _ptr(f), _ptr(s), C.byref(rays), _stream()),
           "psy_render")
     return f, s, rays.value
+
+
+def bucket_size(width, height, spp, thread_count):
+    """psy_bucket_size: PathTraceIntegrator::integrate's bucket size."""
+    return int(lib().psy_bucket_size(int(width), int(height), int(spp), int(thread_count)))
+
+
+def render_adaptive(ctx, scene, width, height, spp, spp_max, variance_max, thread_count=1, seed=1, path_length=4):
+    """psy_render_adaptive: (film (H, W, 3), variance (H, W, 3), passes per bucket, rays_traced) of the
+    reference's adaptive sampling loop with spp_max and variance_max."""
+    dev = torch.device("cuda", ctx.device)
+    f = torch.empty((height, width, 3), dtype=torch.float32, device=dev)
+    v = torch.empty((height, width, 3), dtype=torch.float32, device=dev)
+    bs = bucket_size(width, height, spp, thread_count)
+    nb = ((width + bs - 1) // bs) * ((height + bs - 1) // bs)
+    p = torch.empty(nb, dtype=torch.int32, device=dev)
+    prm = abi.RenderParams(width, height, spp, seed, path_length, 0)
+    rays = C.c_int64()
+    check(lib().psy_render_adaptive(ctx.handle, scene.handle, C.byref(prm), int(spp_max), C.c_float(variance_max),
+                                    int(thread_count), _ptr(f), _ptr(v), _ptr(p), C.byref(rays), _stream()),
+          "psy_render_adaptive")
+    return f, v, p, rays.value
+
+
+def film_adaptive(ctx, samples, spp_max, variance_max, bucket_size_):
+    """psy_film_adaptive over a (H, W, spp, 3) CUDA float32 tensor of samples: (film, variance, passes)."""
+    samples = samples.contiguous()
+    H, W, spp = int(samples.shape[0]), int(samples.shape[1]), int(samples.shape[2])
+    f = torch.empty((H, W, 3), dtype=torch.float32, device=samples.device)
+    v = torch.empty((H, W, 3), dtype=torch.float32, device=samples.device)
+    nb = ((W + bucket_size_ - 1) // bucket_size_) * ((H + bucket_size_ - 1) // bucket_size_)
+    p = torch.empty(nb, dtype=torch.int32, device=samples.device)
+    check(lib().psy_film_adaptive(ctx.handle, _ptr(samples), W, H, spp, int(spp_max), C.c_float(variance_max),
+                                  int(bucket_size_), _ptr(f), _ptr(v), _ptr(p), _stream()), "psy_film_adaptive")
+    return f, v, p
 
 
 def adjoint(ctx, scene, d_film, spp, seed=1, path_length=4):

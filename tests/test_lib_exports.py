@@ -12,7 +12,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 def header_symbols():
     with open(os.path.join(ROOT, "include", "psy_gpu.h")) as f:
         text = f.read()
-    return sorted(set(re.findall(r"^\s*(?:int|void|const char\*|const psy_\w+\*)\s+(psy_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|int32_t|void|const char\*|const psy_\w+\*)\s+(psy_\w+)\s*\(", text, re.M)))
 
 
 def test_header_symbols_listed_in_abi_module():

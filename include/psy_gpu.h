@@ -397,8 +397,16 @@ typedef struct psy_render_params {
     int32_t spp;            /* samples per pixel                                */
     uint32_t seed;          /* ImageSampler seed (parser default 1)             */
     int32_t path_length;    /* bounces; the reference integrator uses 4         */
-    int32_t reserved;
+    uint32_t flags;         /* PSY_RENDER_* below                               */
 } psy_render_params;
+/* Sample every path's dimensions from Sobol::sample (sampling/sobol.hpp:
+ * 40-53) over the matrices set by psy_sobol_set_matrices, instead of the
+ * reference's Halton Sampler (sampling/image_sampler.hpp:16-38); no
+ * counterpart in the reference, whose integrator samples with Halton only.
+ * A path keeps the Sampler's index (hash of pixel and seed, plus the
+ * sample number) and dimension order; dimension d takes Sobol dimension
+ * d at that index, unscrambled, rounded to float and kept below 1. */
+#define PSY_RENDER_SOBOL 1u
 
 /* Replaces: PathTraceIntegrator(scene, film, spp, spp, 0, seed, threads)
  * + integrate() (integrator/path_trace_integrator.hpp:103-120,

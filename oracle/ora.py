@@ -85,6 +85,8 @@ def lib():
         L.ora_bounce_rays.argtypes = [vp, vp, vp, i64, i64, C.c_int, C.c_int, u32, u32, u32, vp, vp]
         L.ora_render.restype = i64
         L.ora_render.argtypes = [vp, C.c_int, C.c_int, C.c_int, u32, C.c_int, C.c_int, vp]
+        L.ora_render_sobol.restype = i64
+        L.ora_render_sobol.argtypes = [vp, C.c_int, C.c_int, C.c_int, u32, C.c_int, C.c_int, vp, vp, u32]
         L.ora_color_to_spectrum.restype = C.c_float
         L.ora_color_to_spectrum.argtypes = [vp]
         L.ora_spectrum_set_grid.restype = C.c_int
@@ -362,10 +364,16 @@ class OracleScene:
         lib().ora_camera_rays(self.handle, width, height, spp, seed, first, n, _p(out), _p(wl))
         return out, wl
 
-    def render(self, width, height, spp, seed, threads=8, path_length=4):
-        """Per-sample film contributions (height, width, spp, 3) and rays traced."""
+    def render(self, width, height, spp, seed, threads=8, path_length=4, sobol_matrices=None):
+        """Per-sample film contributions (height, width, spp, 3) and rays traced; with sobol_matrices
+        (uint64, 52 columns a dimension) every path dimension comes from Sobol::sample."""
         out = np.zeros((height, width, spp, 3), np.float32)
-        rays = lib().ora_render(self.handle, width, height, spp, seed, int(path_length), int(threads), _p(out))
+        if sobol_matrices is None:
+            rays = lib().ora_render(self.handle, width, height, spp, seed, int(path_length), int(threads), _p(out))
+        else:
+            m = np.ascontiguousarray(sobol_matrices, np.uint64)
+            rays = lib().ora_render_sobol(self.handle, width, height, spp, seed, int(path_length), int(threads),
+                                          _p(out), _p(m), m.size)
         if rays < 0:
             raise RuntimeError("oracle render: chromatic colours are not restated")
         return out, int(rays)

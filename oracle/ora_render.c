@@ -491,7 +491,19 @@ typedef struct {
     int done;
 } path_t;
 
-static inline float next_dim(path_t* p) { return ora_sampler_get(p->off, p->dim++); }
+/* the Sobol form of the sampler (the device's PSY_RENDER_SOBOL; no
+ * counterpart in the reference): Sobol::sample (sampling/sobol.hpp:40-53)
+ * of dimension d at the Sampler's index, unscrambled, as a float below 1;
+ * set for the length of one ora_render_sobol call */
+static const uint64_t* g_sobol_m = NULL;
+static uint32_t g_sobol_cols = 0;
+static inline float next_dim(path_t* p) {
+    if (g_sobol_m) {
+        const float v = (float)ora_sobol(g_sobol_m, g_sobol_cols, p->dim++, (uint64_t)p->off, 0);
+        return v < 1.0f ? v : 0.99999994f;
+    }
+    return ora_sampler_get(p->off, p->dim++);
+}
 
 static float faster_ln(float x) {
     uint32_t i;
@@ -1055,6 +1067,16 @@ int64_t ora_render(const ora_scene* s, int width, int height, int spp, uint32_t 
     pthread_mutex_destroy(&J.mu);
     if (g_chromatic_seen) return -1;
     return J.rays;
+}
+
+int64_t ora_render_sobol(const ora_scene* s, int width, int height, int spp, uint32_t seed, int path_length,
+                         int threads, float* path_xyz, const uint64_t* m, uint32_t n_cols) {
+    g_sobol_m = m;
+    g_sobol_cols = n_cols;
+    const int64_t r = ora_render(s, width, height, spp, seed, path_length, threads, path_xyz);
+    g_sobol_m = NULL;
+    g_sobol_cols = 0;
+    return r;
 }
 
 /* ---- entry points of the oracle's own checks (tests/test_oracle_pins.py) */

@@ -114,6 +114,10 @@ void ora_bounce_rays(const ora_scene* s, const ora_wray* in, const ora_inter* hi
  * achromatic colours only). */
 int64_t ora_render(const ora_scene* s, int width, int height, int spp, uint32_t seed,
                    int path_length, int threads, float* path_xyz);
+/* The same with every path dimension from Sobol::sample over matrices m
+ * (n_cols columns) at the Sampler's index (the device's PSY_RENDER_SOBOL) */
+int64_t ora_render_sobol(const ora_scene* s, int width, int height, int spp, uint32_t seed, int path_length,
+                         int threads, float* path_xyz, const uint64_t* m, uint32_t n_cols);
 /* Color_to_SpectralSample value for an achromatic colour */
 float ora_color_to_spectrum(const float c[3]);
 /* The spectrum grid (color/spectrum_grid.h) as data: cells of 8 ints

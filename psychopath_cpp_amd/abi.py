@@ -24,6 +24,7 @@ PSY_SHADER_LAMBERT = 1
 PSY_SHADER_EMIT = 2
 PSY_SHADER_GTR = 3
 PSY_SCENE_RECT_LIGHT_FIX = 1
+PSY_RENDER_SOBOL = 1
 
 PSY_WRAY_PLANES = 20
 PSY_RAY_PLANES = 15
@@ -99,7 +100,7 @@ class Hits(C.Structure):
 
 class RenderParams(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32), ("seed", C.c_uint32),
-                ("path_length", C.c_int32), ("reserved", C.c_int32)]
+                ("path_length", C.c_int32), ("flags", C.c_uint32)]
 
 
 class RenderTuning(C.Structure):

@@ -209,6 +209,8 @@ struct RenderLaunch {
     DevCamera cam;
     const HaltonDim* dims;
     const uint16_t* perm;
+    const uint64_t* sobol_m;  // PSY_RENDER_SOBOL: the Sobol matrices (sobol_cols columns), else null
+    uint32_t sobol_cols;
     int32_t width, height, spp, path_length;
     uint32_t seed;
     float bg[3];

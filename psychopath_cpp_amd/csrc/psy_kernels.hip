@@ -29,15 +29,7 @@ __global__ void k_sobol(const uint64_t* m, uint32_t n_cols, uint32_t dim, const 
                         uint64_t scramble, double* out) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    uint64_t index = idx[i];
-    uint64_t r = scramble & ((1ull << 52) - 1ull);
-    uint32_t c = dim * 52u;
-    while (index) {
-        if ((index & 1ull) && c < n_cols) r ^= m[c];
-        index >>= 1;
-        ++c;
-    }
-    out[i] = (double)r * (1.0 / 4503599627370496.0);
+    out[i] = sobol_value(m, n_cols, dim, idx[i], scramble);
 }
 
 __global__ void k_pixel_samples(const HaltonDim* dims, const uint16_t* perm, int x0, int y0, int w, int h,

@@ -387,6 +387,10 @@ int check_render(const psy_ctx* ctx, const psy_scene* s, const psy_render_params
         return fail(PSY_ERR_INVALID, std::string(who) + ": image sides are limited to 32767 pixels (short pixel coordinates)");
     if (prm->path_length < 1 || prm->path_length > 32)
         return fail(PSY_ERR_INVALID, std::string(who) + ": path_length must be in 1..32");
+    if ((prm->flags & ~PSY_RENDER_SOBOL) != 0u)
+        return fail(PSY_ERR_INVALID, std::string(who) + ": unknown render flags");
+    if ((prm->flags & PSY_RENDER_SOBOL) && !ctx->sobol_m.p)
+        return fail(PSY_ERR_INVALID, std::string(who) + ": PSY_RENDER_SOBOL needs the Sobol matrices (psy_sobol_set_matrices)");
     if (s->chromatic && !ctx->spec.cells)
         return fail(PSY_ERR_UNSUPPORTED,
                     std::string(who) + ": the scene has non-grey colours (or surfaces without a shader, which emit "
@@ -442,6 +446,9 @@ RenderLaunch make_launch(const psy_ctx* ctx, const psy_scene* s, const psy_rende
     R.cam = s->cam;
     R.dims = (const HaltonDim*)ctx->halton_dims.p;
     R.perm = (const uint16_t*)ctx->halton_perm.p;
+    const bool sobol = (prm->flags & PSY_RENDER_SOBOL) != 0u;
+    R.sobol_m = sobol ? (const uint64_t*)ctx->sobol_m.p : nullptr;
+    R.sobol_cols = sobol ? ctx->sobol_dims * 52u : 0u;
     R.width = prm->width;
     R.height = prm->height;
     R.spp = prm->spp;

@@ -767,6 +767,7 @@ PSY_DEV_FN bool light_tree_sample(const DevScene& s, const DevLights& L, const D
 // ---- path kernels -------------------------------------------------------------
 
 PSY_DEV float next_dim(const RenderLaunch& A, uint32_t off, uint32_t& dim) {
+    if (A.sobol_m) return sobol_sampler_get(A.sobol_m, A.sobol_cols, off, dim++);
     return sampler_get(A.dims, A.perm, off, dim++);
 }
 

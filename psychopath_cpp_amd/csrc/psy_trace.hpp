@@ -78,6 +78,26 @@ PSY_DEV float sampler_get(const HaltonDim* dims, const uint16_t* perm, uint32_t 
     return halton_sample(dims, perm, d, off);
 }
 
+// Sobol::sample (sampling/sobol.hpp:40-53): the index's bits select matrix
+// columns of dimension `dim` (52 columns a dimension; columns past the end
+// of the table read as zero), XORed into the 52-bit scramble
+PSY_DEV double sobol_value(const uint64_t* m, uint32_t n_cols, uint32_t dim, uint64_t index, uint64_t scramble) {
+    uint64_t r = scramble & ((1ull << 52) - 1ull);
+    uint32_t c = dim * 52u;
+    while (index) {
+        if ((index & 1ull) && c < n_cols) r ^= m[c];
+        index >>= 1;
+        ++c;
+    }
+    return (double)r * (1.0 / 4503599627370496.0);
+}
+// The Sobol form of a path's sample (PSY_RENDER_SOBOL): dimension `dim` at
+// the Sampler's index, unscrambled, as a float below 1
+PSY_DEV float sobol_sampler_get(const uint64_t* m, uint32_t n_cols, uint32_t offset, uint32_t dim) {
+    const float v = (float)sobol_value(m, n_cols, dim, (uint64_t)offset, 0ull);
+    return v < 1.0f ? v : 0.99999994f;
+}
+
 
 
 // ---------------------------------------------------------------------------

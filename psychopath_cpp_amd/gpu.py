@@ -405,16 +405,17 @@ def bounce_rays(ctx, scene, wrays, hits, width, spp, seed, dim_u=9, dim_v=10):
     return out, valid
 
 
-def render(ctx, scene, width, height, spp, seed=1, path_length=4, film=True, samples=False):
+def render(ctx, scene, width, height, spp, seed=1, path_length=4, film=True, samples=False, sobol=False):
     """psy_render: the path-traced image (PathTraceIntegrator::integrate).
 
     Returns (film, samples, rays_traced): film is a (height, width, 3) CIE XYZ
     tensor (per-pixel mean), samples a (height, width, spp, 3) tensor of every
-    sample's film contribution; either is None when not requested."""
+    sample's film contribution; either is None when not requested.  sobol:
+    PSY_RENDER_SOBOL (the context's Sobol matrices instead of Halton)."""
     dev = torch.device("cuda", ctx.device)
     f = torch.empty((height, width, 3), dtype=torch.float32, device=dev) if film else None
     s = torch.empty((height, width, spp, 3), dtype=torch.float32, device=dev) if samples else None
-    prm = abi.RenderParams(width, height, spp, seed, path_length, 0)
+    prm = abi.RenderParams(width, height, spp, seed, path_length, abi.PSY_RENDER_SOBOL if sobol else 0)
     rays = C.c_int64()
     check(lib().psy_render(ctx.handle, scene.handle, C.byref(prm), _ptr(f), _ptr(s), C.byref(rays), _stream()),
           "psy_render")

@@ -59,6 +59,7 @@ int build(const psy_scene_desc* d, int dice_depth, HostBuilt& B) {
     s.dice_rate = H.dice_rate;
     s.min_upoly = H.min_upoly;
     s.n_inst = (int32_t)H.insts.size();
+    s.flags = d->flags;
     DevLights& L = B.L;
     L.nodes = H.l_nodes.data();
     L.bounds = H.l_bounds.data();
@@ -223,6 +224,8 @@ int hb_render_params(const psy_scene_desc* d, int dice_depth, const psy_render_p
     make_halton_tables(dims, perm);
     R.dims = dims.data();
     R.perm = perm.data();
+    R.sobol_m = nullptr;  // the Halton sampler
+    R.sobol_cols = 0;
     R.width = prm->width;
     R.height = prm->height;
     R.spp = prm->spp;

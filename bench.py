@@ -214,6 +214,43 @@ def batching_forms():
     return out
 
 
+def full_config_renders():
+    """Renders of the four scene configurations at their own sizes, each
+    with the autotuned setting (psy_render_autotune): Mrays/s and ms per
+    render.  The lights are RGB (the reference's lights carry an RGB colour;
+    C3's two-Gaussian spectra have no counterpart there)."""
+    import torch
+    from psychopath_cpp_amd import configs, gpu
+    out = []
+    for name, make, W, H, spp, pl in (
+            ("C0 64 spheres + ground, 2 lights, 256x256, 16 spp, path length 4", configs.sphere_scene, 256, 256, 16, 4),
+            ("C1 heightfield 64x64 bilinear patches, 1024x1024, 64 spp, path length 4", configs.heightfield_scene,
+             1024, 1024, 64, 4),
+            ("C2 4096 bicubic patches instanced 16x with motion, DoF, 1920x1080, 32 spp, path length 6",
+             configs.bicubic_sphere_scene, 1920, 1080, 32, 6),
+            ("C3 sphere scene + 10000 sphere lights, 1024x1024, 64 spp, path length 4", configs.many_light_scene,
+             1024, 1024, 64, 4)):
+        log(f"full config: {name}")
+        ctx = gpu.Context(0)
+        g = gpu.GpuScene(ctx, make())
+        t0 = time.perf_counter()
+        tune = gpu.autotune_render(ctx, g, W, H, spp, path_length=pl)
+        tune_s = time.perf_counter() - t0
+        rate, film = _render_rate(gpu, ctx, g, W, H, spp, pl, reps=1)
+        torch.cuda.synchronize()
+        log(f"full config: {rate:.1f} Mrays/s ({tune})")
+        out.append({"workload": name, "mrays": round(rate, 1),
+                    "autotuned": {"form": RENDER_FORMS.get(int(tune["mode"]), str(tune["mode"])),
+                                  "paths": int(tune["pool"]),
+                                  "trace": TRACE_MODES.get(int(tune["trace_mode"]), str(tune["trace_mode"])),
+                                  "stack": int(tune["stack"])},
+                    "tune_seconds": round(tune_s, 2),
+                    "film_mean_xyz": [round(float(v), 6) for v in film.mean(dim=(0, 1)).cpu().tolist()]})
+        g.close()
+        ctx.close()
+    return out
+
+
 def adjoint_timing():
     """Adjoint / forward time of the differentiable render."""
     import numpy as np
@@ -261,6 +298,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="minimum CPU baseline time (repeats the sample)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip the batching and adjoint measurements")
+    ap.add_argument("--full-configs", action="store_true",
+                    help="only render the four scene configurations at their own sizes (autotuned) and print them")
     ap.add_argument("--dice-depth", type=int, default=-1)
     ap.add_argument("--trace-mode", choices=("persistent", "plain"), default="plain",
                     help="main trace pass: persistent waves with lane refill, or one thread per ray")
@@ -273,6 +312,9 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES (calibrates rocprofv3 byte counters)")
     args = ap.parse_args()
+    if args.full_configs:
+        print(json.dumps({"full_config_renders": full_config_renders()}), flush=True)
+        return 0
     if args.copy_probe:
         import torch
         src = torch.empty(args.copy_probe // 4, dtype=torch.float32, device="cuda").fill_(1.0)

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define PSY_ABI_VERSION 2
+#define PSY_ABI_VERSION 3
 
 /* ---- status codes ---------------------------------------------------- */
 enum psy_status {
@@ -47,13 +47,24 @@ enum psy_object_kind {
     PSY_OBJ_BILINEAR = 2,        /* per time sample: v1 v2 v3 v4, xyz each      (12) */
     PSY_OBJ_BICUBIC = 3,         /* per time sample: 16 control points, xyz     (48) */
     PSY_OBJ_SPHERE_LIGHT = 4,    /* per time sample: cx cy cz r  R G B            (7) */
-    PSY_OBJ_RECTANGLE_LIGHT = 5  /* per time sample: width height  R G B          (5) */
+    PSY_OBJ_RECTANGLE_LIGHT = 5, /* per time sample: width height  R G B          (5) */
+    PSY_OBJ_SUBDIVISION = 6      /* per time sample: n_vertices control points, xyz   */
 };
 
+/* A SubdivisionSurface (object/subdivision_surface.hpp:15-71) is its
+ * control mesh: n_vertices points per time sample in `data`, and n_faces
+ * faces of face_vert_counts[f] (>= 3) vertices each, listed in
+ * face_vert_indices.  psy_scene_create turns it into the bicubic patches of
+ * psy_subdivision_patches; every hit on them carries the instance's id.
+ * The four mesh fields are read for this kind only. */
 typedef struct psy_object_desc {
-    int32_t kind;          /* enum psy_object_kind              */
-    int32_t time_samples;  /* >= 1                              */
-    const float* data;     /* time_samples * floats-per-kind    */
+    int32_t kind;          /* enum psy_object_kind                      */
+    int32_t time_samples;  /* >= 1                                      */
+    const float* data;     /* time_samples * floats-per-kind            */
+    int32_t n_vertices;    /* PSY_OBJ_SUBDIVISION: points per sample    */
+    int32_t n_faces;       /* PSY_OBJ_SUBDIVISION                       */
+    const int32_t* face_vert_counts;   /* PSY_OBJ_SUBDIVISION: n_faces  */
+    const int32_t* face_vert_indices;  /* PSY_OBJ_SUBDIVISION           */
 } psy_object_desc;
 
 enum psy_instance_type { PSY_INSTANCE_OBJECT = 0, PSY_INSTANCE_ASSEMBLY = 1 };
@@ -326,6 +337,21 @@ int psy_scene_get_info(const psy_scene* scene, psy_scene_info* info);
 /* InstanceID (basics/instance_id.hpp:9-40) of every flattened leaf, as the
  * reference's Tracer would report it in Intersection::id (host memory). */
 int psy_scene_instance_ids(const psy_scene* scene, uint64_t* out, int64_t n);
+
+/* Replaces: SubdivisionSurface::finalize (object/subdivision_surface.cpp:
+ * 85-233), the Catmull-Clark patch extraction it leaves to OpenSubdiv.
+ * The limit surface of a control mesh (boundary edges creased, boundary
+ * vertices smooth) as bicubic Bezier patches: 48 floats each, 16 control
+ * points in rows along v, in PSY_OBJ_BICUBIC's layout.  Faces are refined
+ * adaptively around extraordinary vertices down to level 5; each face left
+ * irregular there gets an end cap matching the limit position and tangents
+ * at its corners.  The mesh must be manifold with consistently oriented
+ * faces of 3 or more distinct vertices.  Host only, no device is used.
+ * patches == NULL: only *n_patches is set; else max_patches must be at
+ * least the count. */
+int psy_subdivision_patches(const float* verts, int32_t n_verts, const int32_t* face_vert_counts,
+                            int32_t n_faces, const int32_t* face_vert_indices, float* patches,
+                            int64_t max_patches, int64_t* n_patches);
 
 /* ---- tracing ------------------------------------------------------------ */
 /* Replaces: uint32_t Tracer::trace(const WorldRay*, const WorldRay*,

@@ -142,6 +142,59 @@ private:
     std::vector<std::array<Vec3, 16>> v_;
 };
 
+// SubdivisionSurface (object/subdivision_surface.hpp:15-71): a Catmull-Clark
+// control mesh, verts_per_motion_sample points per time sample.  The scene
+// turns it into the limit surface's bicubic patches (patches() returns them,
+// as SubdivisionSurface::finalize fills `patches`); every hit on them
+// carries the instance's id.  The device path takes one time sample.
+class SubdivisionSurface : public Object {
+public:
+    SubdivisionSurface() = default;
+    void set_verts(std::vector<Vec3>&& verts, int verts_per_motion_sample) {
+        if (verts_per_motion_sample < 0 || (verts_per_motion_sample == 0 && !verts.empty()) ||
+            (verts_per_motion_sample > 0 && verts.size() % (size_t)verts_per_motion_sample != 0))
+            throw std::invalid_argument("SubdivisionSurface: vertex count is not a multiple of the points per sample");
+        verts_ = std::move(verts);
+        per_ = verts_per_motion_sample;
+    }
+    void set_face_vert_counts(std::vector<int>&& counts) { counts_.assign(counts.begin(), counts.end()); }
+    void set_face_vert_indices(std::vector<int>&& indices) { indices_.assign(indices.begin(), indices.end()); }
+    int kind() const override { return PSY_OBJ_SUBDIVISION; }
+    int time_samples() const override { return per_ > 0 ? (int)(verts_.size() / (size_t)per_) : 1; }
+    std::vector<float> data() const override {
+        std::vector<float> d;
+        for (const Vec3& p : verts_) d.insert(d.end(), {p.x, p.y, p.z});
+        return d;
+    }
+    int verts_per_motion_sample() const { return per_; }
+    const std::vector<int32_t>& face_vert_counts() const { return counts_; }
+    const std::vector<int32_t>& face_vert_indices() const { return indices_; }
+    // The first time sample's limit surface as bicubic Bezier patches
+    // (psy_subdivision_patches; host only).
+    std::vector<std::array<Vec3, 16>> patches() const {
+        const std::vector<float> d = data();
+        int64_t n = 0;
+        check(psy_subdivision_patches(d.data(), per_, counts_.data(), (int32_t)counts_.size(), indices_.data(),
+                                      nullptr, 0, &n));
+        std::vector<float> f((size_t)n * 48);
+        check(psy_subdivision_patches(d.data(), per_, counts_.data(), (int32_t)counts_.size(), indices_.data(),
+                                      f.data(), n, &n));
+        std::vector<std::array<Vec3, 16>> out((size_t)n);
+        for (int64_t k = 0; k < n; ++k)
+            for (int i = 0; i < 16; ++i)
+                out[(size_t)k][i] = Vec3{f[48 * k + 3 * i], f[48 * k + 3 * i + 1], f[48 * k + 3 * i + 2]};
+        return out;
+    }
+
+private:
+    static void check(int rc) {
+        if (rc != PSY_OK) throw std::runtime_error(std::string("psy_subdivision_patches: ") + psy_last_error());
+    }
+    std::vector<Vec3> verts_;
+    int per_ = 0;
+    std::vector<int32_t> counts_, indices_;
+};
+
 class SphereLight : public Object {
 public:
     SphereLight(std::vector<Vec3> positions, std::vector<float> radii, std::vector<Color> colors)
@@ -398,6 +451,12 @@ struct Scene {
                 od.kind = A->objects_[i]->kind();
                 od.time_samples = A->objects_[i]->time_samples();
                 od.data = D->object_data[a].data() + data_off[i];
+                if (const SubdivisionSurface* sd = dynamic_cast<const SubdivisionSurface*>(A->objects_[i].get())) {
+                    od.n_vertices = sd->verts_per_motion_sample();
+                    od.n_faces = (int32_t)sd->face_vert_counts().size();
+                    od.face_vert_counts = sd->face_vert_counts().data();
+                    od.face_vert_indices = sd->face_vert_indices().data();
+                }
                 D->objects[a].push_back(od);
             }
             for (const auto& in : A->instances_) {

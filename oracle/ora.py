@@ -130,6 +130,10 @@ def lib():
         L.ora_kat_gtr_estimate.argtypes = [vp, vp, vp, f, vp]
         L.ora_kat_axis_angle.argtypes = [vp, f, vp]
         L.ora_film_adaptive.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, f, C.c_int, vp, vp, vp]
+        L.ora_subdiv_patches_ex.restype = i64
+        L.ora_subdiv_patches_ex.argtypes = [vp, C.c_int, vp, C.c_int, vp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                            C.POINTER(C.c_void_p)]
+        L.ora_free.argtypes = [vp]
         L.ora_kat_light_select.restype = C.c_int
         L.ora_kat_light_select.argtypes = [vp, vp, vp, vp, f, f, f, f, vp, vp, vp, vp]
         L.ora_kat_total_lights.restype = i64
@@ -430,3 +434,22 @@ def film_adaptive(samples, spp_max, variance_max, bucket_size):
     lib().ora_film_adaptive(_p(s), W, H, spp, int(spp_max), np.float32(variance_max), int(bucket_size), _p(film),
                             _p(var), _p(passes))
     return film, var, passes
+
+
+def subdiv_patches(verts, face_vert_counts, face_vert_indices, max_level=5, min_level=0, prune=True,
+                   cap_all=False):
+    """ora_subdiv_patches_ex: Bezier patches (n, 16, 3) of a Catmull-Clark
+    control mesh; None when the oracle refuses the mesh."""
+    v = np.ascontiguousarray(np.asarray(verts, np.float32).reshape(-1, 3))
+    cnt = np.ascontiguousarray(np.asarray(face_vert_counts, np.int32).reshape(-1))
+    idx = np.ascontiguousarray(np.asarray(face_vert_indices, np.int32).reshape(-1))
+    out = C.c_void_p()
+    n = lib().ora_subdiv_patches_ex(_p(v), v.shape[0], _p(cnt), cnt.size, _p(idx), int(max_level), int(min_level),
+                                    int(bool(prune)), int(bool(cap_all)), C.byref(out))
+    if n < 0:
+        return None
+    res = np.zeros((n, 16, 3), np.float32)
+    if n:
+        C.memmove(res.ctypes.data, out.value, n * 48 * 4)
+    lib().ora_free(out)
+    return res

@@ -27,12 +27,26 @@ static inline int ray_is_occl(const ora_ray* r) { return (r->id_and_flags & RAY_
 static inline int ray_is_done(const ora_ray* r) { return (r->id_and_flags & RAY_DONE) != 0; }
 static inline uint32_t ray_id(const ora_ray* r) { return r->id_and_flags & RAY_IDMASK; }
 
+/* SubdivisionSurface::Node (object/subdivision_surface.hpp:16-20) with one
+ * time sample's bounds; patch >= 0 marks a leaf */
+typedef struct {
+    bbox b;
+    int child[2];
+    int patch;
+} ora_sd_node;
+
 typedef struct ora_object {
     int kind, ts;
     float escale;     /* emission scale of a light object (1); not in the light tree */
-    float* data;      /* ts * floats-per-kind */
+    float* data;      /* ts * floats-per-kind; a subdivision surface's control points */
     bbox* bounds;     /* ts */
     int n_bounds;
+    /* PSY_OBJ_SUBDIVISION: control point count, Bicubic patches, patch BVH */
+    int n_verts;
+    int n_patches;
+    struct ora_object* patches;
+    ora_sd_node* sd_nodes;
+    int sd_root;
 } ora_object;
 
 /* BVH::Node (accel/bvh.hpp:46-55) */

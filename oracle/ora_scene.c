@@ -55,7 +55,93 @@ static bbox bound_points(const float* p, int n) {
     return b;
 }
 
+/* SubdivisionSurface::build_bvh_recursive (object/subdivision_surface.cpp:
+ * 295-354): split the nodes' centres at the middle of the axis picked by
+ * "y if longer than x, then z if longer than y", std::partition order */
+typedef struct {
+    ora_sd_node* n;
+    int count;
+} sd_build;
+
+static int sd_center_below(const ora_sd_node* a, int axis, float pmid) {
+    const v3 c = v3scale(v3add(a->b.min, a->b.max), 0.5f);
+    return v3get(c, axis) < pmid;
+}
+
+static int sd_recurse(sd_build* B, int* order, int begin, int end) {
+    if (begin + 1 == end) return order[begin];
+    bbox cb = bbox_empty();
+    for (int i = begin; i < end; ++i) {
+        const ora_sd_node* q = &B->n[order[i]];
+        const v3 c = v3scale(v3add(q->b.min, q->b.max), 0.5f);
+        cb.min = v3m(cb.min.x < c.x ? cb.min.x : c.x, cb.min.y < c.y ? cb.min.y : c.y, cb.min.z < c.z ? cb.min.z : c.z);
+        cb.max = v3m(cb.max.x > c.x ? cb.max.x : c.x, cb.max.y > c.y ? cb.max.y : c.y, cb.max.z > c.z ? cb.max.z : c.z);
+    }
+    const v3 ext = v3sub(cb.max, cb.min);
+    int axis = 0;
+    if (ext.y > ext.x) axis = 1;
+    if (ext.z > ext.y) axis = 2;
+    const float pmid = (v3get(cb.min, axis) + v3get(cb.max, axis)) * 0.5f;
+    /* std::partition for bidirectional iterators */
+    int first = begin, last = end;
+    for (;;) {
+        while (first != last && sd_center_below(&B->n[order[first]], axis, pmid)) ++first;
+        if (first == last) break;
+        --last;
+        while (first != last && !sd_center_below(&B->n[order[last]], axis, pmid)) --last;
+        if (first == last) break;
+        const int t = order[first]; order[first] = order[last]; order[last] = t;
+        ++first;
+    }
+    int mid = first;
+    if (mid == begin) ++mid;
+    const int me = B->count++;
+    const int c0 = sd_recurse(B, order, begin, mid);
+    const int c1 = sd_recurse(B, order, mid, end);
+    B->n[me].child[0] = c0;
+    B->n[me].child[1] = c1;
+    B->n[me].patch = -1;
+    B->n[me].b = bbox_merge(B->n[c0].b, B->n[c1].b);
+    return me;
+}
+
+static void object_finalize(ora_object* o);
+
+/* SubdivisionSurface::finalize (object/subdivision_surface.cpp:85-233):
+ * patches, bounds of the control points, patch BVH (:237-292) */
+static void subdiv_finalize(ora_object* o) {
+    o->n_bounds = 1;
+    o->bounds = (bbox*)malloc(sizeof(bbox));
+    bbox bb = bbox_empty();
+    for (int i = 0; i < o->n_verts; ++i) {
+        const v3 q = v3m(o->data[3 * i], o->data[3 * i + 1], o->data[3 * i + 2]);
+        bb.min = v3min(bb.min, q);
+        bb.max = v3max(bb.max, q);
+    }
+    o->bounds[0] = bb;
+    o->sd_root = -1;
+    if (o->n_patches == 0) return;
+    sd_build B;
+    B.n = (ora_sd_node*)calloc(2 * (size_t)o->n_patches, sizeof(ora_sd_node));
+    B.count = o->n_patches;
+    int* order = (int*)malloc(sizeof(int) * (size_t)o->n_patches);
+    for (int i = 0; i < o->n_patches; ++i) {
+        object_finalize(&o->patches[i]);
+        B.n[i].b = o->patches[i].bounds[0];
+        B.n[i].child[0] = B.n[i].child[1] = -1;
+        B.n[i].patch = i;
+        order[i] = i;
+    }
+    o->sd_root = sd_recurse(&B, order, 0, o->n_patches);
+    o->sd_nodes = B.n;
+    free(order);
+}
+
 static void object_finalize(ora_object* o) {
+    if (o->kind == PSY_OBJ_SUBDIVISION) {
+        subdiv_finalize(o);
+        return;
+    }
     const int nf = ora_kind_floats(o->kind);
     if (o->kind == PSY_OBJ_SPHERE_LIGHT) {
         /* positions and radii share the time-sample count here, so the
@@ -425,6 +511,7 @@ static mat4 load_mat(const float* f) {
 
 ora_scene* ora_scene_create(const psy_scene_desc* d) {
     if (!d || d->n_assemblies < 1) return NULL;
+    int ok = 1;
     ora_scene* s = (ora_scene*)calloc(1, sizeof(ora_scene));
     s->n_asm = d->n_assemblies;
     s->asms = (ora_assembly*)calloc((size_t)s->n_asm, sizeof(ora_assembly));
@@ -450,6 +537,27 @@ ora_scene* ora_scene_create(const psy_scene_desc* d) {
             o->kind = od->kind;
             o->escale = 1.0f;
             o->ts = od->time_samples;
+            if (od->kind == PSY_OBJ_SUBDIVISION) {
+                float* pp = NULL;
+                const int64_t np = od->time_samples == 1 ? ora_subdiv_patches(od->data, od->n_vertices,
+                    od->face_vert_counts, od->n_faces, od->face_vert_indices, &pp) : -1;
+                if (np < 0) { ok = 0; continue; }
+                o->n_verts = od->n_vertices;
+                o->data = (float*)malloc(sizeof(float) * (3 * (size_t)od->n_vertices + 1));
+                memcpy(o->data, od->data, sizeof(float) * 3 * (size_t)od->n_vertices);
+                o->n_patches = (int)np;
+                o->patches = (ora_object*)calloc((size_t)np + 1, sizeof(ora_object));
+                for (int64_t k = 0; k < np; ++k) {
+                    ora_object* q = &o->patches[k];
+                    q->kind = PSY_OBJ_BICUBIC;
+                    q->ts = 1;
+                    q->escale = 1.0f;
+                    q->data = (float*)malloc(sizeof(float) * 48);
+                    memcpy(q->data, pp + 48 * k, sizeof(float) * 48);
+                }
+                ora_free(pp);
+                continue;
+            }
             const int nf = ora_kind_floats(od->kind) * od->time_samples;
             o->data = (float*)malloc(sizeof(float) * (nf + 1));
             memcpy(o->data, od->data, sizeof(float) * nf);
@@ -490,6 +598,10 @@ ora_scene* ora_scene_create(const psy_scene_desc* d) {
     memcpy(s->background, d->background, sizeof(float) * 3);
     s->dice_rate = d->dice_rate;
     s->min_upoly = d->min_upoly_size;
+    if (!ok) {
+        ora_scene_destroy(s);
+        return NULL;
+    }
     assembly_finalize(&s->asms[0]);
     for (int k = 1; k < s->n_asm; ++k) assembly_finalize(&s->asms[k]);
     return s;
@@ -499,7 +611,12 @@ void ora_scene_destroy(ora_scene* s) {
     if (!s) return;
     for (int k = 0; k < s->n_asm; ++k) {
         ora_assembly* a = &s->asms[k];
-        for (int i = 0; i < a->n_obj; ++i) { free(a->obj[i].data); free(a->obj[i].bounds); }
+        for (int i = 0; i < a->n_obj; ++i) {
+            ora_object* o = &a->obj[i];
+            for (int k = 0; k < o->n_patches; ++k) { free(o->patches[k].data); free(o->patches[k].bounds); }
+            free(o->patches); free(o->sd_nodes);
+            free(o->data); free(o->bounds);
+        }
         for (size_t i = 0; i < a->n_lnodes; ++i) free(a->lnodes[i].bounds);
         free(a->lnodes); free(a->lbounds);
         free(a->obj); free(a->inst); free(a->xf); free(a->sub); free(a->nodes); free(a->bounds);

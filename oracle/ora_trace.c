@@ -859,6 +859,43 @@ static int next_object(bvh4_trav* t, ora_ray** rb, ora_ray** re, size_t* idx) {
     return 0;
 }
 
+/* the node test of SubdivisionSurface::intersect_rays: the node's bounds
+ * (one time sample) against the ray */
+static int sd_node_pred(void* vctx, ora_ray* ray) {
+    const ora_sd_node* n = (const ora_sd_node*)vctx;
+    float h0, h1;
+    return bbox_intersect(&n->b, ray->o, ray->d_inv, ray->max_t, &h0, &h1);
+}
+
+/* SubdivisionSurface::intersect_rays (object/subdivision_surface.cpp:41-83):
+ * the patch BVH walked with ray partitions, each leaf's patch diced as a
+ * Bicubic under the instance's id and shader */
+static void intersect_subdiv(tracer* tr, const ora_object* ob, ora_ray* rb, ora_ray* re) {
+    if (ob->sd_root < 0) return;
+    int node_stack[64];
+    ora_ray* end_stack[64];
+    int sp = 0;
+    node_stack[0] = ob->sd_root;
+    end_stack[0] = re;
+    while (sp >= 0) {
+        const ora_sd_node* n = &ob->sd_nodes[node_stack[sp]];
+        if (n->patch < 0) {
+            end_stack[sp] = mutable_partition(rb, end_stack[sp], sd_node_pred, (void*)n);
+            if (rb != end_stack[sp] && sp + 1 < 64) {
+                node_stack[sp + 1] = n->child[0];
+                node_stack[sp] = n->child[1];
+                end_stack[sp + 1] = end_stack[sp];
+                ++sp;
+            } else {
+                --sp;
+            }
+        } else {
+            intersect_patch(tr, &ob->patches[n->patch], rb, end_stack[sp]);
+            --sp;
+        }
+    }
+}
+
 /* Tracer::trace_assembly (tracer/tracer.cpp:125-219) */
 static void trace_assembly(tracer* tr, const ora_assembly* a, ora_ray* rays, ora_ray* end) {
     bvh4_trav t;
@@ -899,6 +936,7 @@ static void trace_assembly(tracer* tr, const ora_assembly* a, ora_ray* rays, ora
         if (inst->type == PSY_INSTANCE_OBJECT) {
             const ora_object* ob = &a->obj[inst->data_index];
             if (ob->kind == PSY_OBJ_BILINEAR || ob->kind == PSY_OBJ_BICUBIC) intersect_patch(tr, ob, hb, he);
+            else if (ob->kind == PSY_OBJ_SUBDIVISION) intersect_subdiv(tr, ob, hb, he);
             else trace_surface_or_light(tr, ob, hb, he);
         } else {
             trace_assembly(tr, a->sub[inst->data_index], hb, he);

@@ -185,6 +185,19 @@ void ora_kat_gtr_sample(const float prm[3], const float in[3], const float n[3],
 float ora_kat_gtr_estimate(const float prm[3], const float in[3], const float cent[3], float cos_theta,
                            const float nor[3]);
 void ora_kat_axis_angle(const float axis[3], float angle, float m[16]);
+/* SubdivisionSurface::finalize's patch extraction (object/subdivision_surface.cpp:
+ * 85-233): Bezier patches, 48 floats each, into *out (free with ora_free);
+ * returns the count, or -1 for a mesh it refuses.  The _ex form takes the
+ * deepest level (5 in the reference), the level below which every active face
+ * is split, whether to refine only the faces within two rings of the split
+ * ones (else every face), and whether every active face of the deepest level
+ * becomes an end cap. */
+int64_t ora_subdiv_patches(const float* verts, int n_verts, const int32_t* counts, int n_faces,
+                           const int32_t* indices, float** out);
+int64_t ora_subdiv_patches_ex(const float* verts, int n_verts, const int32_t* counts, int n_faces,
+                              const int32_t* indices, int max_level, int min_level, int prune, int cap_all,
+                              float** out);
+void ora_free(void* p);
 /* The adaptive sampling loop's film (film/film.hpp:135-165 in
  * integrator/path_trace_integrator.cpp:296-376) over per-sample XYZ values
  * that every pass of a bucket adds again */

@@ -16,6 +16,7 @@ PSY_OBJ_BILINEAR = 2
 PSY_OBJ_BICUBIC = 3
 PSY_OBJ_SPHERE_LIGHT = 4
 PSY_OBJ_RECTANGLE_LIGHT = 5
+PSY_OBJ_SUBDIVISION = 6
 
 PSY_INSTANCE_OBJECT = 0
 PSY_INSTANCE_ASSEMBLY = 1
@@ -54,7 +55,9 @@ PSY_ISECT_PLANES = 44
 
 
 class ObjectDesc(C.Structure):
-    _fields_ = [("kind", C.c_int32), ("time_samples", C.c_int32), ("data", C.POINTER(C.c_float))]
+    _fields_ = [("kind", C.c_int32), ("time_samples", C.c_int32), ("data", C.POINTER(C.c_float)),
+                ("n_vertices", C.c_int32), ("n_faces", C.c_int32),
+                ("face_vert_counts", C.POINTER(C.c_int32)), ("face_vert_indices", C.POINTER(C.c_int32))]
 
 
 class InstanceDesc(C.Structure):
@@ -119,7 +122,7 @@ class SceneInfo(C.Structure):
 EXPORTED_SYMBOLS = [
     "psy_ctx_create", "psy_ctx_destroy", "psy_ctx_set_trace_mode", "psy_ctx_retrace_count", "psy_ctx_trace_pass_counts", "psy_ctx_set_render_chunk", "psy_ctx_set_render_mode", "psy_ctx_set_trace_timing", "psy_ctx_set_trace_stack", "psy_ctx_set_second_pass", "psy_ctx_set_continuation", "psy_ctx_set_stream_steps", "psy_render_autotune", "psy_render_adaptive", "psy_film_adaptive", "psy_bucket_size", "psy_ctx_trace_timing", "psy_last_error", "psy_abi_version",
     "psy_malloc", "psy_free", "psy_copy_to_device", "psy_copy_to_host",
-    "psy_scene_create", "psy_scene_destroy", "psy_scene_get_info", "psy_scene_instance_ids",
+    "psy_scene_create", "psy_scene_destroy", "psy_scene_get_info", "psy_scene_instance_ids", "psy_subdivision_patches",
     "psy_trace_world", "psy_trace_rays", "psy_trace_rays_static", "psy_intersections", "psy_world_to_rays",
     "psy_parse_scene_count", "psy_parse_scene", "psy_parsed_scene_desc", "psy_parsed_scene_settings",
     "psy_parsed_scene_warnings", "psy_parsed_scene_destroy", "psy_parse_dump_tree",

@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpsy_gpu.so")
 
-SOURCES = ["psy_kernels.hip", "psy_capi.cpp", "psy_render.cpp", "psy_parser.cpp", "psy_film.cpp"]
+SOURCES = ["psy_kernels.hip", "psy_capi.cpp", "psy_render.cpp", "psy_parser.cpp", "psy_film.cpp", "psy_subdiv.cpp"]
 
 # -ffp-contract=off and IEEE div/sqrt: results must match the reference's
 # uncontracted SSE float arithmetic bit for bit.  Packed FP32 instructions

@@ -436,6 +436,9 @@ struct FlatLeaf {
 struct Flattener {
     const psy_scene_desc* d;
     std::vector<int32_t> obj_base;           // first global object of each assembly
+    // per global object: its first patch object and patch count when it is
+    // a subdivision surface (first -1 otherwise)
+    std::vector<int32_t> sub_first, sub_count;
     std::vector<std::vector<M44>> frames;
     std::vector<FlatLeaf> leaves;
     int depth_guard = 0;
@@ -465,7 +468,14 @@ struct Flattener {
             if (in.shader_index >= d->n_shaders) { err = "shader index out of range"; return false; }
             if (in.type == PSY_INSTANCE_OBJECT) {
                 if (in.data_index < 0 || in.data_index >= ad.n_objects) { err = "object index out of range"; return false; }
-                leaves.push_back(FlatLeaf{obj_base[a] + in.data_index, fr, sh, id});
+                const int g = obj_base[a] + in.data_index;
+                if (sub_first[g] >= 0) {
+                    // SubdivisionSurface::intersect_rays passes the instance's
+                    // id to every patch (object/subdivision_surface.cpp:41-83)
+                    for (int k = 0; k < sub_count[g]; ++k) leaves.push_back(FlatLeaf{sub_first[g] + k, fr, sh, id});
+                } else {
+                    leaves.push_back(FlatLeaf{g, fr, sh, id});
+                }
             } else if (in.type == PSY_INSTANCE_ASSEMBLY) {
                 if (in.data_index < 0 || in.data_index >= ad.n_assemblies) { err = "sub-assembly index out of range"; return false; }
                 const int sub = ad.assemblies[in.data_index];
@@ -702,11 +712,29 @@ int prepare_scene(const psy_scene_desc* d, int dice_depth, HostScene& H) {
     Flattener fl;
     fl.d = d;
     std::vector<Box> obox;  // union over time samples
+    std::vector<std::pair<int32_t, std::vector<float>>> subdiv;  // (global object, its patches)
     for (int a = 0; a < d->n_assemblies; ++a) {
         const psy_assembly_desc& ad = d->assemblies[a];
         fl.obj_base.push_back((int32_t)objs.size());
         for (int i = 0; i < ad.n_objects; ++i) {
             const psy_object_desc& od = ad.objects[i];
+            if (od.kind == PSY_OBJ_SUBDIVISION) {
+                if (od.time_samples != 1)
+                    return fail(PSY_ERR_UNSUPPORTED, "psy_scene_create: deforming subdivision surfaces (time_samples > 1) are not supported on the device path");
+                if (od.n_vertices < 0 || od.n_faces < 0 || (od.n_vertices > 0 && !od.data))
+                    return fail(PSY_ERR_INVALID, "psy_scene_create: subdivision surface without its control mesh");
+                std::vector<float> patches;
+                std::string err;
+                if (int rc = subdivide_patches(od.data, od.n_vertices, od.face_vert_counts, od.n_faces, od.face_vert_indices, patches, err))
+                    return fail(rc, "psy_scene_create: subdivision surface: " + err);
+                subdiv.emplace_back((int32_t)objs.size(), std::move(patches));
+                DevObject o{};
+                o.kind = od.kind;
+                o.ts = 1;
+                obox.push_back(Box{});
+                objs.push_back(o);
+                continue;
+            }
             const int nf = floats_per_kind(od.kind);
             if (nf == 0) return fail(PSY_ERR_INVALID, "psy_scene_create: unknown object kind");
             if (od.time_samples < 1 || !od.data) return fail(PSY_ERR_INVALID, "psy_scene_create: object without data");
@@ -747,6 +775,27 @@ int prepare_scene(const psy_scene_desc* d, int dice_depth, HostScene& H) {
             objs.push_back(o);
         }
     }
+    // the patches of the subdivision surfaces, after every described object
+    fl.sub_first.assign(objs.size(), -1);
+    fl.sub_count.assign(objs.size(), 0);
+    for (const auto& sp : subdiv) {
+        const size_t np = sp.second.size() / 48;
+        fl.sub_first[sp.first] = (int32_t)objs.size();
+        fl.sub_count[sp.first] = (int32_t)np;
+        for (size_t k = 0; k < np; ++k) {
+            const float* p = sp.second.data() + 48 * k;
+            DevObject o{};
+            o.kind = PSY_OBJ_BICUBIC;
+            o.ts = 1;
+            o.data = (int32_t)odata.size();
+            odata.insert(odata.end(), p, p + 48);
+            Box b;
+            for (int q = 0; q < 16; ++q) b.growp(p + 3 * q);
+            obox.push_back(b);
+            objs.push_back(o);
+        }
+    }
+    if (odata.size() > (size_t)INT32_MAX) return fail(PSY_ERR_UNSUPPORTED, "psy_scene_create: object data above 2^31 floats");
     if (!fl.run(0, -1, 0, -1)) return fail(PSY_ERR_INVALID, "psy_scene_create: " + fl.err);
     if (int rc = prepare_lights(d, fl.obj_base, H)) return rc;
     for (const FlatLeaf& L : fl.leaves) {

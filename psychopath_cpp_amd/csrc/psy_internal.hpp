@@ -216,6 +216,10 @@ struct HostScene {
     std::vector<int32_t> l_sub;
 };
 
+// Catmull-Clark limit surface of a control mesh as bicubic Bezier patches,
+// 48 floats each (psy_subdiv.cpp); err holds the reason of a nonzero status.
+int subdivide_patches(const float* verts, int32_t n_verts, const int32_t* counts, int32_t n_faces,
+                      const int32_t* indices, std::vector<float>& out, std::string& err);
 // Host part of psy_scene_create (psy_capi.cpp); no device calls.
 int prepare_scene(const psy_scene_desc* d, int dice_depth, HostScene& H);
 // Light trees of every assembly (psy_render.cpp); obj_base: first global

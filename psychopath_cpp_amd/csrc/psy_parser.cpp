@@ -6,8 +6,8 @@
 //   DataTree (lexer, parse_node, print_tree)   parser/data_tree.cpp:1-400
 //   Parser::parse_next_frame, parse_matrix, parse_camera, parse_assembly,
 //   parse_bilinear_patch, parse_bicubic_patch, parse_surface_shader,
-//   parse_sphere_light, parse_rectangle_light, parse_sphere
-//                                              parser/parser.cpp:37-662
+//   parse_sphere_light, parse_rectangle_light, parse_sphere,
+//   parse_subdivision_surface                  parser/parser.cpp:37-662
 //   Assembly name lookups (object/assembly maps per assembly, shaders
 //   searched up the parent chain)              scene/assembly.hpp:105-190
 //
@@ -265,6 +265,9 @@ struct psy_parsed_scene {
     struct Asm {
         std::vector<psy_object_desc> objects;
         std::vector<std::vector<float>> object_data;
+        // a subdivision surface's face vertex counts and indices (empty for
+        // the other kinds)
+        std::vector<std::vector<int32_t>> object_counts, object_indices;
         std::vector<psy_instance_desc> instances;
         std::vector<float> transforms;
         std::vector<int32_t> children;  // scene indices of the child assemblies
@@ -370,6 +373,8 @@ struct Builder {
         PS::Asm& A = ps.asms[a];
         const int idx = (int)A.objects.size();
         A.object_data.push_back(std::move(data));
+        A.object_counts.emplace_back();
+        A.object_indices.emplace_back();
         psy_object_desc o{};
         o.kind = kind;
         o.time_samples = ts;
@@ -394,6 +399,42 @@ struct Builder {
         }
         if (ts == 0) return fail(n, n.type + " " + n.name + " has no Vertices");
         add_object(a, n, kind, ts, std::move(data));
+        return true;
+    }
+
+    // Parser::parse_subdivision_surface (parser/parser.cpp:416-468): each
+    // Vertices leaf adds its whole xyz triples, the first one fixing the
+    // points per time sample and the total cut to a multiple of it; the
+    // last FaceVertCounts and FaceVertIndices leaves win
+    bool subdivision(int a, const DtNode& n) {
+        std::vector<float> verts;
+        int per = 0;
+        std::vector<int> counts, indices;
+        for (const DtNode& c : n.children) {
+            if (c.type == "Vertices") {
+                const std::vector<float> v = floats(c.leaf);
+                const size_t whole = v.size() / 3;
+                verts.insert(verts.end(), v.begin(), v.begin() + 3 * whole);
+                if (per == 0) per = (int)whole;
+                if (per > 0) verts.resize(verts.size() - verts.size() % (3 * (size_t)per));
+            } else if (c.type == "FaceVertCounts") {
+                if (!ints(c.leaf, counts)) return fail(c, "FaceVertCounts: integer out of range");
+            } else if (c.type == "FaceVertIndices") {
+                if (!ints(c.leaf, indices)) return fail(c, "FaceVertIndices: integer out of range");
+            }
+        }
+        int64_t total = 0;
+        for (int k : counts) total += k;
+        if (total != (int64_t)indices.size())
+            return fail(n, "SubdivisionSurface " + n.name + ": FaceVertIndices holds " + std::to_string(indices.size()) +
+                               " entries, FaceVertCounts sum to " + std::to_string(total));
+        const int ts = per > 0 ? (int)(verts.size() / (3 * (size_t)per)) : 1;
+        const int idx = add_object(a, n, PSY_OBJ_SUBDIVISION, ts, std::move(verts));
+        PS::Asm& A = ps.asms[a];
+        A.object_counts[idx].assign(counts.begin(), counts.end());
+        A.object_indices[idx].assign(indices.begin(), indices.end());
+        A.objects[idx].n_vertices = per;
+        A.objects[idx].n_faces = (int32_t)counts.size();
         return true;
     }
 
@@ -494,7 +535,7 @@ struct Builder {
             } else if (c.type == "BicubicPatch") {
                 ok = patch(a, c, 48, PSY_OBJ_BICUBIC);
             } else if (c.type == "SubdivisionSurface") {
-                warn(c, "SubdivisionSurface " + c.name + " is not supported on the device path; skipped");
+                ok = subdivision(a, c);
             } else if (c.type == "Sphere") {
                 ok = sphere(a, c);
             } else if (c.type == "SurfaceShader") {
@@ -636,7 +677,11 @@ void finish_desc(PS& ps) {
     ps.asm_desc.resize(ps.asms.size());
     for (size_t a = 0; a < ps.asms.size(); ++a) {
         PS::Asm& A = ps.asms[a];
-        for (size_t i = 0; i < A.objects.size(); ++i) A.objects[i].data = A.object_data[i].data();
+        for (size_t i = 0; i < A.objects.size(); ++i) {
+            A.objects[i].data = A.object_data[i].data();
+            A.objects[i].face_vert_counts = A.object_counts[i].data();
+            A.objects[i].face_vert_indices = A.object_indices[i].data();
+        }
         psy_assembly_desc& d = ps.asm_desc[a];
         d.objects = A.objects.data();
         d.n_objects = (int32_t)A.objects.size();

@@ -44,6 +44,7 @@ def lib():
     L.psy_scene_destroy.argtypes = [vp]
     L.psy_scene_get_info.argtypes = [vp, C.POINTER(abi.SceneInfo)]
     L.psy_scene_instance_ids.argtypes = [vp, vp, i64]
+    L.psy_subdivision_patches.argtypes = [vp, i32, vp, i32, vp, vp, i64, C.POINTER(i64)]
     L.psy_trace_world.argtypes = [vp, vp, vp, i64, C.POINTER(abi.Hits), u32, vp]
     L.psy_trace_rays.argtypes = [vp, vp, vp, i64, C.POINTER(abi.Hits), u32, vp]
     L.psy_trace_rays_static.argtypes = [vp, vp, vp, i64, C.POINTER(abi.Hits), u32, vp]
@@ -88,10 +89,28 @@ def lib():
     L.psy_scene_get_params.argtypes = [vp, vp, vp]
     L.psy_scene_set_params.argtypes = [vp, vp, i64, vp, i64]
     L.psy_adjoint.argtypes = [vp, vp, C.POINTER(abi.RenderParams), vp, vp, vp, C.POINTER(i64), vp]
-    if L.psy_abi_version() != 2:
+    if L.psy_abi_version() != 3:
         raise PsyError("libpsy_gpu.so ABI version mismatch")
     _lib = L
     return L
+
+
+def subdivision_patches(verts, face_vert_counts, face_vert_indices):
+    """SubdivisionSurface::finalize (object/subdivision_surface.cpp:85-233):
+    the Catmull-Clark limit surface of a control mesh as (n, 16, 3) bicubic
+    Bezier control points (rows along v).  Host only; no GPU is needed."""
+    L = lib()
+    v = np.ascontiguousarray(np.asarray(verts, np.float32).reshape(-1, 3))
+    cnt = np.ascontiguousarray(np.asarray(face_vert_counts, np.int32).reshape(-1))
+    idx = np.ascontiguousarray(np.asarray(face_vert_indices, np.int32).reshape(-1))
+    if int(cnt.sum()) != idx.size:
+        raise PsyError("face_vert_indices must hold sum(face_vert_counts) entries")
+    n = C.c_int64(0)
+    args = (v.ctypes.data, v.shape[0], cnt.ctypes.data, cnt.size, idx.ctypes.data)
+    check(L.psy_subdivision_patches(*args, None, 0, C.byref(n)), "psy_subdivision_patches")
+    out = np.zeros((max(1, n.value), 16, 3), np.float32)
+    check(L.psy_subdivision_patches(*args, out.ctypes.data, n.value, C.byref(n)), "psy_subdivision_patches")
+    return out[:n.value]
 
 
 def check(rc, what):

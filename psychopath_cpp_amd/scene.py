@@ -52,6 +52,25 @@ def Bicubic(*points_per_sample):
     return Object(abi.PSY_OBJ_BICUBIC, [np.asarray(p, np.float32).reshape(48) for p in points_per_sample])
 
 
+class SubdivisionSurface:
+    """object/subdivision_surface.hpp:15-71: a Catmull-Clark control mesh
+    (vertices (n,3), per-face vertex counts, concatenated face vertex
+    indices) with one time sample."""
+
+    kind = abi.PSY_OBJ_SUBDIVISION
+
+    def __init__(self, verts, face_vert_counts, face_vert_indices):
+        self.data = np.ascontiguousarray(np.asarray(verts, np.float32).reshape(-1, 3))
+        self.face_vert_counts = np.ascontiguousarray(np.asarray(face_vert_counts, np.int32).reshape(-1))
+        self.face_vert_indices = np.ascontiguousarray(np.asarray(face_vert_indices, np.int32).reshape(-1))
+        if int(self.face_vert_counts.sum()) != self.face_vert_indices.size:
+            raise ValueError("face_vert_indices must hold sum(face_vert_counts) entries")
+
+    @property
+    def time_samples(self):
+        return 1
+
+
 def SphereLight(center, radius, color):
     """lights/sphere_light.hpp: positions, radii, colours per sample."""
     c = np.atleast_2d(np.asarray(center, np.float32))
@@ -170,7 +189,14 @@ class Scene:
             objs = (abi.ObjectDesc * max(1, len(a.objects)))()
             for i, o in enumerate(a.objects):
                 keep.append(o.data)
-                objs[i] = abi.ObjectDesc(o.kind, o.time_samples, _fptr(o.data))
+                if o.kind == abi.PSY_OBJ_SUBDIVISION:
+                    keep.extend([o.face_vert_counts, o.face_vert_indices])
+                    ip = C.POINTER(C.c_int32)
+                    objs[i] = abi.ObjectDesc(o.kind, 1, _fptr(o.data), o.data.shape[0], o.face_vert_counts.size,
+                                             o.face_vert_counts.ctypes.data_as(ip),
+                                             o.face_vert_indices.ctypes.data_as(ip))
+                else:
+                    objs[i] = abi.ObjectDesc(o.kind, o.time_samples, _fptr(o.data))
             insts = (abi.InstanceDesc * max(1, len(a.instances)))()
             for i, (kind, idx, ti, tc, sh) in enumerate(a.instances):
                 shi = -1 if sh is None else self.shader_map[sh]

@@ -36,6 +36,10 @@ def _assembly(sc, a, names, indent, out):
         elif o.kind == abi.PSY_OBJ_SPHERE_LIGHT:
             props = " ".join(f"Location [{nums(r[:3])}] Radius [{num(r[3])}] Color [{nums(r[4:7])}]" for r in d)
             out.append(f"{pad}SphereLight {nm} {{ {props} }}")
+        elif o.kind == abi.PSY_OBJ_SUBDIVISION:
+            out.append(f"{pad}SubdivisionSurface {nm} {{ Vertices [{nums(d)}] "
+                       f"FaceVertCounts [{' '.join(str(int(c)) for c in o.face_vert_counts)}] "
+                       f"FaceVertIndices [{' '.join(str(int(i)) for i in o.face_vert_indices)}] }}")
         elif o.kind == abi.PSY_OBJ_RECTANGLE_LIGHT:
             props = " ".join(f"Dimensions [{nums(r[:2])}] Color [{nums(r[2:5])}]" for r in d)
             out.append(f"{pad}RectangleLight {nm} {{ {props} }}")

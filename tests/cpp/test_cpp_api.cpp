@@ -44,18 +44,39 @@ static void print_desc(const psy_scene_desc& d) {
         std::printf("assembly %d objects %d instances %d transforms %d children %d xfsum %.6f\n", a, A.n_objects,
                     A.n_instances, A.n_transforms, A.n_assemblies, xs);
         for (int i = 0; i < A.n_objects; ++i) {
-            const int nf = A.objects[i].kind == PSY_OBJ_SPHERE ? 4 : A.objects[i].kind == PSY_OBJ_BILINEAR ? 12
-                           : A.objects[i].kind == PSY_OBJ_BICUBIC ? 48 : A.objects[i].kind == PSY_OBJ_SPHERE_LIGHT ? 7 : 5;
+            const psy_object_desc& o = A.objects[i];
+            const int nf = o.kind == PSY_OBJ_SPHERE ? 4 : o.kind == PSY_OBJ_BILINEAR ? 12
+                           : o.kind == PSY_OBJ_BICUBIC ? 48 : o.kind == PSY_OBJ_SPHERE_LIGHT ? 7
+                           : o.kind == PSY_OBJ_SUBDIVISION ? 3 * o.n_vertices : 5;
             double sum = 0.0;
-            for (int k = 0; k < nf * A.objects[i].time_samples; ++k) sum += A.objects[i].data[k];
-            std::printf("object %d %d kind %d ts %d sum %.6f\n", a, i, A.objects[i].kind,
-                        A.objects[i].time_samples, sum);
+            for (int k = 0; k < nf * o.time_samples; ++k) sum += o.data[k];
+            std::printf("object %d %d kind %d ts %d sum %.6f\n", a, i, o.kind, o.time_samples, sum);
+            if (o.kind == PSY_OBJ_SUBDIVISION) {
+                long long sc = 0, si = 0;
+                for (int f = 0; f < o.n_faces; ++f) sc += o.face_vert_counts[f];
+                for (long long k = 0; k < sc; ++k) si += o.face_vert_indices[k] * (k + 1);
+                std::printf("mesh %d %d verts %d faces %d counts %lld indices %lld\n", a, i, o.n_vertices, o.n_faces,
+                            sc, si);
+            }
         }
         for (int i = 0; i < A.n_instances; ++i)
             std::printf("instance %d %d type %d data %d xf %d+%d shader %d\n", a, i, A.instances[i].type,
                         A.instances[i].data_index, A.instances[i].transform_index,
                         A.instances[i].transform_count, A.instances[i].shader_index);
     }
+}
+
+// A Catmull-Clark cube of half-size 0.6: its limit surface is a rounded blob.
+static std::unique_ptr<Object> make_blob() {
+    std::unique_ptr<SubdivisionSurface> s(new SubdivisionSurface());
+    std::vector<Vec3> v;
+    for (int x = 0; x < 2; ++x)
+        for (int y = 0; y < 2; ++y)
+            for (int z = 0; z < 2; ++z) v.push_back(Vec3{x ? 0.6f : -0.6f, y ? 0.6f : -0.6f, z ? 0.6f : -0.6f});
+    s->set_verts(std::move(v), 8);
+    s->set_face_vert_counts({4, 4, 4, 4, 4, 4});
+    s->set_face_vert_indices({0, 2, 6, 4, 1, 5, 7, 3, 0, 4, 5, 1, 2, 3, 7, 6, 0, 1, 3, 2, 4, 6, 7, 5});
+    return std::unique_ptr<Object>(s.release());
 }
 
 static std::unique_ptr<Scene> make_scene() {
@@ -81,6 +102,8 @@ static std::unique_ptr<Scene> make_scene() {
             cp[j * 4 + i] = Vec3{-3.0f + (float)i, 2.0f + (float)j, (i == 1 || i == 2) && (j == 1 || j == 2) ? 1.5f : 0.2f};
     root.add_object("hill", std::unique_ptr<Object>(new Bicubic(cp)));
     root.create_object_instance("hill", {translate(0.5f, 0.0f, 0.0f)}, root.get_surface_shader("grey"));
+    root.add_object("blob", make_blob());
+    root.create_object_instance("blob", {translate(2.5f, 2.0f, 0.8f)}, root.get_surface_shader("grey"));
     std::unique_ptr<Assembly> lamp(new Assembly());
     lamp->add_object("bulb", std::unique_ptr<Object>(new SphereLight({{0.0f, 0.0f, 0.0f}}, {0.4f}, {Color(30.0f)})));
     lamp->create_object_instance("bulb", {}, nullptr);
@@ -121,6 +144,16 @@ int main(int argc, char** argv) {
         f.write((const char*)hdr, sizeof hdr);
         f.write((const char*)film.xyz.data(), sizeof(float) * film.xyz.size());
         std::printf("rendered %s %dx%d spp %d\n", r->scene->name.c_str(), r->res_x, r->res_y, r->spp);
+        return 0;
+    }
+    if (mode == "subdiv") {
+        // SubdivisionSurface::patches on the host: count and checksum
+        std::unique_ptr<Object> b = make_blob();
+        const auto P = static_cast<SubdivisionSurface*>(b.get())->patches();
+        double sum = 0.0;
+        for (size_t k = 0; k < P.size(); ++k)
+            for (int i = 0; i < 16; ++i) sum += (P[k][i].x + 2.0 * P[k][i].y + 3.0 * P[k][i].z) * (double)(k % 7 + 1);
+        std::printf("patches %zu sum %.17g\n", P.size(), sum);
         return 0;
     }
     if (mode == "nodevice") {

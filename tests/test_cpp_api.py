@@ -12,11 +12,19 @@ import pytest
 
 from psychopath_cpp_amd import abi
 from psychopath_cpp_amd.scene import (Assembly, Bicubic, Bilinear, Camera, RectangleLight, Scene, Sphere,
-                                      SphereLight, translate)
+                                      SphereLight, SubdivisionSurface, translate)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CAM = np.array([1.0, -0.0, 0.0, 0.0, 0.0, 0.4472135901451111, 0.8944271802902222, 0.0, 0.0, 0.8944271802902222,
                 -0.4472135901451111, 0.0, 0.0, -9.0, 5.0, 1.0], np.float32).reshape(4, 4)
+
+
+def blob_mesh():
+    """The Catmull-Clark cube of half-size 0.6 test_cpp_api.cpp builds."""
+    v = np.array([[0.6 if x else -0.6, 0.6 if y else -0.6, 0.6 if z else -0.6]
+                  for x in range(2) for y in range(2) for z in range(2)], np.float32)
+    idx = [0, 2, 6, 4, 1, 5, 7, 3, 0, 4, 5, 1, 2, 3, 7, 6, 0, 1, 3, 2, 4, 6, 7, 5]
+    return v, [4] * 6, idx
 
 
 def cpp_equivalent_scene():
@@ -32,6 +40,8 @@ def cpp_equivalent_scene():
     cp = [(-3.0 + i, 2.0 + j, 1.5 if (i in (1, 2) and j in (1, 2)) else 0.2) for j in range(4) for i in range(4)]
     root.add_object("hill", Bicubic(cp))
     root.create_object_instance("hill", [translate(0.5, 0.0, 0.0)], "grey")
+    root.add_object("blob", SubdivisionSurface(*blob_mesh()))
+    root.create_object_instance("blob", [translate(2.5, 2.0, 0.8)], "grey")
     lamp = Assembly()
     lamp.add_object("bulb", SphereLight((0.0, 0.0, 0.0), 0.4, (30.0, 30.0, 30.0)))
     lamp.create_object_instance("bulb", [], None)
@@ -65,9 +75,14 @@ def _describe_python(sc):
                      f"children {A.n_assemblies} xfsum {xs:.6f}")
         for i in range(A.n_objects):
             o = A.objects[i]
-            n = abi.FLOATS_PER_KIND[o.kind] * o.time_samples
+            per = 3 * o.n_vertices if o.kind == abi.PSY_OBJ_SUBDIVISION else abi.FLOATS_PER_KIND[o.kind]
+            n = per * o.time_samples
             s = sum(float(o.data[k]) for k in range(n))
             lines.append(f"object {a} {i} kind {o.kind} ts {o.time_samples} sum {s:.6f}")
+            if o.kind == abi.PSY_OBJ_SUBDIVISION:
+                sc_ = sum(int(o.face_vert_counts[f]) for f in range(o.n_faces))
+                si = sum(int(o.face_vert_indices[k]) * (k + 1) for k in range(sc_))
+                lines.append(f"mesh {a} {i} verts {o.n_vertices} faces {o.n_faces} counts {sc_} indices {si}")
         for i in range(A.n_instances):
             x = A.instances[i]
             lines.append(f"instance {a} {i} type {x.type} data {x.data_index} xf {x.transform_index}+"
@@ -108,3 +123,15 @@ def test_cpp_interface_reports_missing_device():
     else:
         assert out.startswith("error psy_scene_create") or out.startswith("error psy_ctx_create")
         assert "no HIP device" in out
+
+
+def test_cpp_subdivision_patches_match_python():
+    """SubdivisionSurface::patches (the C++ class over
+    psy_subdivision_patches) gives the patches the Python wrapper gives."""
+    from psychopath_cpp_amd import gpu
+    out = subprocess.run([cpp_binary(), "subdiv"], check=True, stdout=subprocess.PIPE, text=True).stdout.split()
+    P = gpu.subdivision_patches(*blob_mesh()).astype(np.float64)
+    w = (np.arange(len(P)) % 7 + 1)[:, None]
+    s = float(((P[..., 0] + 2.0 * P[..., 1] + 3.0 * P[..., 2]) * w).sum())
+    assert int(out[1]) == len(P)
+    assert float(out[3]) == pytest.approx(s, rel=1e-10)

@@ -258,3 +258,31 @@ def test_cpu_build_skewed_layout_bvh_depth():
     agree, t_ok, hits = compare(hb, inter)
     assert hits > 50
     assert agree >= 0.9999 and t_ok >= 0.9999
+
+
+@pytest.mark.parametrize("motion", [False, True])
+def test_cpu_build_subdivision_surfaces(motion):
+    """Catmull-Clark surfaces (a box instanced three times, one moving, a
+    prism in a sub-assembly, a fan of triangles, an L-shaped sheet) against
+    the oracle's SubdivisionSurface::intersect_rays over its patch BVH."""
+    from tests import _meshes as Msh
+    sc = Msh.subdiv_scene(motion)
+    o = ora.OracleScene(sc)
+    w, _ = o.camera_rays(96, 64, 2, 1)
+    hb = hostbuild.trace(sc, ora.planes_from_wrays(w), world=True, dice_depth=10)
+    inter = o.trace(w, threads=8, chunk=8192)
+    agree, t_ok, hits = compare(hb, inter)
+    assert hits > 0.3 * len(w)
+    assert agree >= 0.9999 and t_ok >= 0.9999
+
+
+def test_cpu_build_subdivision_render_matches_oracle():
+    from tests import _meshes as Msh
+    sc = Msh.subdiv_scene(False)
+    W, H, spp = 32, 24, 4
+    hs, hrays = hostbuild.render(sc, W, H, spp, seed=1)
+    os_, orays = ora.OracleScene(sc).render(W, H, spp, 1, threads=8)
+    worst, close = R.tile_check(hs, os_)
+    assert np.isfinite(hs).all() and hs.mean() > 0.0
+    assert close >= 0.95 and worst <= 3.0
+    assert abs(hrays - orays) <= 0.01 * orays

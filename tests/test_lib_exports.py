@@ -28,7 +28,7 @@ def test_library_exports_every_header_symbol():
     missing = [s for s in header_symbols() if not hasattr(lib, s)]
     assert not missing, missing
     lib.psy_abi_version.restype = C.c_int
-    assert lib.psy_abi_version() == 2
+    assert lib.psy_abi_version() == 3
 
 
 def test_oracle_is_not_linked_by_product():

@@ -144,7 +144,9 @@ def test_assembly_objects_instances_and_warnings():
     A = d.assemblies[0]
     kinds = [A.objects[i].kind for i in range(A.n_objects)]
     assert kinds == [abi.PSY_OBJ_SPHERE, abi.PSY_OBJ_SPHERE, abi.PSY_OBJ_BILINEAR, abi.PSY_OBJ_BICUBIC,
-                     abi.PSY_OBJ_SPHERE_LIGHT, abi.PSY_OBJ_RECTANGLE_LIGHT]
+                     abi.PSY_OBJ_SPHERE_LIGHT, abi.PSY_OBJ_RECTANGLE_LIGHT, abi.PSY_OBJ_SUBDIVISION]
+    # a control mesh of one point and no faces: an empty surface
+    assert (A.objects[6].n_vertices, A.objects[6].n_faces, A.objects[6].time_samples) == (1, 0, 1)
 
     def data(o, nf):
         return np.ctypeslib.as_array(o.data, (o.time_samples * nf,)).copy()
@@ -161,16 +163,16 @@ def test_assembly_objects_instances_and_warnings():
     inst = [(A.instances[i].type, A.instances[i].data_index, A.instances[i].transform_index,
              A.instances[i].transform_count, A.instances[i].shader_index) for i in range(A.n_instances)]
     assert inst == [(0, 0, 0, 0, 2), (0, 2, 0, 1, -1), (0, 3, 1, 0, 1), (0, 4, 1, 0, -1), (0, 5, 1, 2, -1),
-                    (1, 0, 3, 1, -1)]
+                    (1, 0, 3, 1, -1), (0, 6, 4, 0, -1)]
     assert A.n_transforms == 4 and A.n_assemblies == 1 and A.assemblies[0] == 1
     B = d.assemblies[1]
     # $red found up the parent chain; $ball is not in $child's scope
     assert B.n_objects == 1 and B.n_instances == 1 and B.instances[0].shader_index == 0
     assert np.array_equal(data(B.objects[0], 4), _f([0, 0, 0, 2]))
     w = p.warnings
-    for msg in ("unknown surface shader type 'OSL'", "SubdivisionSurface $subd", "$ball, which does not exist",
-                "$nothing, which does not exist", "$subd, which does not exist"):
+    for msg in ("unknown surface shader type 'OSL'", "$ball, which does not exist", "$nothing, which does not exist"):
         assert msg in w, msg
+    assert "$subd" not in w
 
 
 @pytest.mark.parametrize("bad,msg", [
@@ -193,7 +195,16 @@ def _compare_assembly(pa, pb, da, db):
     for i in range(A.n_objects):
         oa, ob = A.objects[i], B.objects[i]
         assert (oa.kind, oa.time_samples) == (ob.kind, ob.time_samples)
-        nf = abi.FLOATS_PER_KIND[oa.kind] * oa.time_samples
+        if oa.kind == abi.PSY_OBJ_SUBDIVISION:
+            assert (oa.n_vertices, oa.n_faces) == (ob.n_vertices, ob.n_faces)
+            nf = 3 * oa.n_vertices * oa.time_samples
+            cnt = np.ctypeslib.as_array(oa.face_vert_counts, (oa.n_faces,))
+            assert np.array_equal(cnt, np.ctypeslib.as_array(ob.face_vert_counts, (ob.n_faces,)))
+            ni = int(cnt.sum())
+            assert np.array_equal(np.ctypeslib.as_array(oa.face_vert_indices, (ni,)),
+                                  np.ctypeslib.as_array(ob.face_vert_indices, (ni,)))
+        else:
+            nf = abi.FLOATS_PER_KIND[oa.kind] * oa.time_samples
         assert np.array_equal(np.ctypeslib.as_array(oa.data, (nf,)), np.ctypeslib.as_array(ob.data, (nf,)))
     for i in range(A.n_instances):
         ia, ib = A.instances[i], B.instances[i]
@@ -214,12 +225,14 @@ def round_trip(sc):
 
 
 @pytest.mark.parametrize("name", ["spheres", "heightfield", "bicubic_motion", "instanced", "instanced_lights",
-                                  "chromatic", "many_lights"])
+                                  "chromatic", "many_lights", "subdivision"])
 def test_round_trip_gives_the_same_description(name):
+    from tests import _meshes
     sc = {"spheres": configs.sphere_scene, "heightfield": lambda: configs.heightfield_scene(grid=8),
           "bicubic_motion": lambda: configs.bicubic_sphere_scene(grid=4, n_outer=2, n_inner=2),
           "instanced": lambda: G._instanced_scene(True), "instanced_lights": R._instanced_lights_scene,
-          "chromatic": R._chromatic_scene, "many_lights": lambda: configs.many_light_scene(n_lights=50)}[name]()
+          "chromatic": R._chromatic_scene, "many_lights": lambda: configs.many_light_scene(n_lights=50),
+          "subdivision": lambda: _meshes.subdiv_scene(True)}[name]()
     p = round_trip(sc)
     handle = sc.to_desc()  # keeps the buffers the description points into
     want = handle.desc
@@ -267,3 +280,34 @@ def test_reference_example_file():
     assert [A.instances[i].type for i in range(A.n_instances)] == [abi.PSY_INSTANCE_OBJECT, abi.PSY_INSTANCE_ASSEMBLY]
     assert d.n_assemblies == 2 and d.assemblies[1].n_instances == 0
     assert p.warnings.count("\n") == 4
+
+
+def test_subdivision_surface_fields():
+    """Parser::parse_subdivision_surface (parser/parser.cpp:416-468): whole
+    xyz triples of every Vertices leaf, the first leaf fixing the points per
+    time sample; the last FaceVertCounts / FaceVertIndices leaves win."""
+    text = """Scene $s { Output { Path [] } RenderSettings { Resolution [4 4] SamplesPerPixel [1] Seed [1] }
+      Camera { Fov [40] Transform [1 0 0 0 0 1 0 0 0 0 1 0 0 0 0 1] }
+      World { BackgroundShader { Type [Color] Color [0 0 0] } }
+      Assembly {
+        SubdivisionSurface $m {
+          Vertices [0 0 0  1 0 0  1 1 0  0 1 0  9]
+          FaceVertCounts [3 3]
+          FaceVertCounts [4]
+          FaceVertIndices [0 1 2 3]
+          Vertices [0 0 1  1 0 1  1 1 1  0 1 1  2 2 2]
+        }
+        Instance { Data [$m] }
+      }
+    }"""
+    p = psyfile.ParsedScene(text)
+    A = p.desc().assemblies[0]
+    o = A.objects[0]
+    assert (o.kind, o.n_vertices, o.time_samples, o.n_faces) == (abi.PSY_OBJ_SUBDIVISION, 4, 2, 1)
+    v = np.ctypeslib.as_array(o.data, (24,))
+    assert v[12 + 2] == 1.0 and v[-1] == 1.0   # the fifth point of the second leaf is cut
+    assert list(np.ctypeslib.as_array(o.face_vert_counts, (1,))) == [4]
+    assert list(np.ctypeslib.as_array(o.face_vert_indices, (4,))) == [0, 1, 2, 3]
+    bad = text.replace("FaceVertIndices [0 1 2 3]", "FaceVertIndices [0 1 2]")
+    with pytest.raises(gpu.PsyError, match="FaceVertIndices"):
+        psyfile.ParsedScene(bad)

@@ -251,6 +251,92 @@ def full_config_renders():
     return out
 
 
+def subdivision_stream(args):
+    """The primary + first-bounce stream (2^21 + 2^21 rays) through the
+    Catmull-Clark scene (configs.subdivision_scene: the heightfield as one
+    control mesh, 25 instanced rounded boxes): trace Mrays/s with the main
+    and second pass timed by HIP events, the host's patch extraction, and
+    the oracle on a sample of the same rays.  Each step is one
+    psy_trace_world call (world rays in, closest hits out)."""
+    import numpy as np
+    import torch
+    from oracle import ora
+    from psychopath_cpp_amd import configs, gpu
+    scene_py = configs.subdivision_scene()
+    field = scene_py.root.objects[0]
+    t0 = time.perf_counter()
+    reps = 0
+    while time.perf_counter() - t0 < 0.5 or reps < 3:
+        patches = gpu.subdivision_patches(field.data, field.face_vert_counts, field.face_vert_indices)
+        reps += 1
+    extract_ms = (time.perf_counter() - t0) / reps * 1e3
+    t0 = time.perf_counter()
+    reps = 0
+    while time.perf_counter() - t0 < 0.5 or reps < 3:
+        ora.subdiv_patches(field.data, field.face_vert_counts, field.face_vert_indices)
+        reps += 1
+    ora_extract_ms = (time.perf_counter() - t0) / reps * 1e3
+    ctx = gpu.Context(0)
+    ctx.set_continuation(args.continuation)
+    t0 = time.perf_counter()
+    g = gpu.GpuScene(ctx, scene_py, dice_depth=args.dice_depth)
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - t0
+    info = g.info
+    log(f"subdivision scene: {len(patches)} field patches ({extract_ms:.2f} ms to extract), "
+        f"{info.n_leaf_instances} leaves, dice depth {info.dice_depth}, built in {build_s:.2f}s")
+    wrays, n_valid = make_stream(ctx, g)
+    # the boxes are transformed patch instances: their ray footprints come
+    # from world rays (psy_trace_world, Tracer::trace's own interface)
+    hits = gpu.HitBuffers(STREAM_RAYS, wrays.device, aux=False)
+    for _ in range(args.warmup):
+        gpu.trace_world(ctx, g, wrays, hits=hits)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ctx.set_trace_timing(True)
+    ctx.trace_timing()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        gpu.trace_world(ctx, g, wrays, hits=hits)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    main_ms, second_ms, calls = ctx.trace_timing()
+    ctx.set_trace_timing(False)
+    ms = ev0.elapsed_time(ev1) / args.steps
+    mrays = STREAM_RAYS / (ms / 1e3) / 1e6
+    log(f"subdivision stream: {ms:.3f} ms/step = {mrays:.1f} Mrays/s; main {main_ms / max(calls, 1):.3f} ms, "
+        f"second {second_ms / max(calls, 1):.3f} ms per call, {ctx.retrace_count()} rays in the second pass")
+    out = {"workload": "subdivision_scene: 64x64-quad Catmull-Clark heightfield + 25 rounded boxes, "
+                       "2^21 primary + 2^21 bounce rays",
+           "field_patches": len(patches), "leaves": int(info.n_leaf_instances), "dice_depth": int(info.dice_depth),
+           "extract_ms": round(extract_ms, 2), "oracle_extract_ms": round(ora_extract_ms, 2),
+           "scene_build_s": round(build_s, 3), "ms_per_step": round(ms, 4), "mrays": round(mrays, 1),
+           "main_pass_ms": round(main_ms / max(calls, 1), 4), "second_pass_ms": round(second_ms / max(calls, 1), 4),
+           "second_pass_rays": int(ctx.retrace_count()), "valid_bounces": n_valid}
+    if not args.no_cpu:
+        sample = min(args.cpu_sample, STREAM_RAYS)
+        pick = mixed_sample(STREAM_RAYS, sample)
+        cpu_mrays, threads, inter, dt, reps = cpu_baseline(scene_py, wrays, pick, args.cpu_seconds)
+        ids = g.instance_ids()
+        tp = torch.from_numpy(pick).to(hits.inst.device)
+        gi = hits.inst[tp].cpu().numpy().view(np.uint32)
+        gt = hits.t[tp].cpu().numpy()
+        ghit = gi != 0xFFFFFFFF
+        ok = ghit == (inter["hit"] != 0)
+        both = ghit & (inter["hit"] != 0)
+        ok[both] &= ids[gi[both]] == inter["id"][both]
+        ok[both] &= np.abs(gt[both] - inter["t"][both]) <= 1e-5 * np.abs(inter["t"][both])
+        log(f"subdivision cpu baseline: {cpu_mrays:.3f} Mrays/s on {threads} threads; parity {ok.mean():.6f}")
+        out["cpu_baseline"] = {"value": round(cpu_mrays, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+                               "sample": f"{sample} rays of the stream (half primary, half bounce), {reps} passes"}
+        out["parity_on_cpu_sample"] = round(float(ok.mean()), 6)
+    g.close()
+    ctx.close()
+    return out
+
+
 def adjoint_timing():
     """Adjoint / forward time of the differentiable render."""
     import numpy as np
@@ -300,6 +386,8 @@ def main():
     ap.add_argument("--no-forms", action="store_true", help="skip the batching and adjoint measurements")
     ap.add_argument("--full-configs", action="store_true",
                     help="only render the four scene configurations at their own sizes (autotuned) and print them")
+    ap.add_argument("--subdiv", action="store_true",
+                    help="only time the ray stream through the Catmull-Clark scene (configs.subdivision_scene)")
     ap.add_argument("--dice-depth", type=int, default=-1)
     ap.add_argument("--trace-mode", choices=("persistent", "plain"), default="plain",
                     help="main trace pass: persistent waves with lane refill, or one thread per ray")
@@ -314,6 +402,9 @@ def main():
     args = ap.parse_args()
     if args.full_configs:
         print(json.dumps({"full_config_renders": full_config_renders()}), flush=True)
+        return 0
+    if args.subdiv:
+        print(json.dumps({"subdivision_stream": subdivision_stream(args)}), flush=True)
         return 0
     if args.copy_probe:
         import torch

@@ -25,13 +25,19 @@ restated here, so a seed fully determines a scene.
   many_light_scene  sphere_scene plus N sphere lights (radii log-uniform in
                     [0.01, 0.2], 0.5-3 above the ground, grey emission scale
                     U[0.1, 10]).
+  subdivision_scene heightfield_scene's 65x65 vertices as one Catmull-Clark
+                    control mesh (an open sheet of 64x64 quads) with 25
+                    rounded boxes on it (one control mesh, a cube of 4x4
+                    quads a side, instanced at random rotations about z),
+                    the heightfield's light and camera.
 """
 import math
 
 import numpy as np
 
 from . import abi
-from .scene import Assembly, Bicubic, Bilinear, Camera, RectangleLight, Scene, Sphere, SphereLight, look_at, translate
+from .scene import (Assembly, Bicubic, Bilinear, Camera, RectangleLight, Scene, Sphere, SphereLight,
+                    SubdivisionSurface, look_at, translate)
 
 
 class JKiss:
@@ -241,4 +247,67 @@ def many_light_scene(seed=1, n_lights=10000, big_lights=2, **kw):
         e = rng.uniform(0.1, 10.0)
         root.add_object(f"ml{i}", SphereLight(c, r, (e, e, e)))
         root.create_object_instance(f"ml{i}", [], None)
+    return sc
+
+
+def cube_mesh(k=4, half=1.0):
+    """Closed quad control mesh of a cube with k x k quads a side, faces
+    oriented outwards: (verts (n,3) float32, face vertex counts, indices)."""
+    ids = {}
+    verts = []
+
+    def vid(p):
+        if p not in ids:
+            ids[p] = len(verts)
+            verts.append([(c / k * 2.0 - 1.0) * half for c in p])
+        return ids[p]
+    faces = []
+    for axis in range(3):
+        for side in (0, k):
+            u, v = (axis + 1) % 3, (axis + 2) % 3
+            for a in range(k):
+                for b in range(k):
+                    quad = []
+                    for (da, db) in ((0, 0), (1, 0), (1, 1), (0, 1)):
+                        p = [0, 0, 0]
+                        p[axis] = side
+                        p[u] = a + da
+                        p[v] = b + db
+                        quad.append(vid(tuple(p)))
+                    faces.append(quad if side == k else quad[::-1])
+    return (np.asarray(verts, np.float32), np.full(len(faces), 4, np.int32),
+            np.asarray([i for f in faces for i in f], np.int32))
+
+
+def subdivision_scene(seed=7, grid=64, extent=10.0, dice_rate=1.0, n_boxes=5, light_pos=(0.0, -3.0, 13.5),
+                      light_size=2.0, light_color=(20.0, 20.0, 20.0), fov_deg=36.0,
+                      camera_eye=(0.0, -1.5, 11.5), camera_target=(0.0, 0.3, 0.0)):
+    """Catmull-Clark surfaces: the heightfield as one control mesh and
+    rounded boxes standing on it (object/subdivision_surface.hpp)."""
+    X, Y, Z = heightfield(seed, grid, extent)
+    root = Assembly()
+    sc = Scene(Camera([look_at(camera_eye, camera_target)], [math.radians(fov_deg)]), root,
+               background=(0.0, 0.0, 0.0), dice_rate=dice_rate)
+    sc.add_surface_shader("ground", abi.PSY_SHADER_LAMBERT, (0.5, 0.5, 0.5))
+    sc.add_surface_shader("box", abi.PSY_SHADER_LAMBERT, (0.7, 0.7, 0.7))
+    V = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    n = grid + 1
+    F = [[j * n + i, j * n + i + 1, (j + 1) * n + i + 1, (j + 1) * n + i] for j in range(grid) for i in range(grid)]
+    root.add_object("field", SubdivisionSurface(V, [4] * len(F), np.asarray(F, np.int32).ravel()))
+    root.create_object_instance("field", [], "ground")
+    root.add_object("box", SubdivisionSurface(*cube_mesh(4, 0.45)))
+    rng = JKiss(seed + 1)
+    for b in range(n_boxes * n_boxes):
+        j, i = divmod(b, n_boxes)
+        x = (i + 0.5) / n_boxes * extent * 0.8 - extent * 0.4
+        y = (j + 0.5) / n_boxes * extent * 0.8 - extent * 0.4
+        gi = int(round((x + extent / 2) / extent * grid))
+        gj = int(round((y + extent / 2) / extent * grid))
+        z = float(Z[gj, gi]) + 0.35
+        a = rng.uniform(0.0, 2 * math.pi)
+        c, s_ = math.cos(a), math.sin(a)
+        rot = np.array([[c, s_, 0, 0], [-s_, c, 0, 0], [0, 0, 1, 0], [0, 0, 0, 1]], np.float32)
+        root.create_object_instance("box", [translate(x, y, z) @ rot], "box")
+    root.add_object("light", RectangleLight(light_size, light_size, light_color))
+    root.create_object_instance("light", [translate(*light_pos)], None)
     return sc

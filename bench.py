@@ -189,6 +189,7 @@ def batching_forms():
         gpu.set_render_chunk(ctx, 1 << 22)
         wide, f_wide = _render_rate(gpu, ctx, g, W, H, spp, pl)
         log(f"batching: wavefront, 2^22-path chunks: {wide:.1f} Mrays/s")
+        d0 = int(g.info.dice_depth)
         t0 = time.perf_counter()
         tune = gpu.autotune_render(ctx, g, W, H, spp, path_length=pl)
         tune_s = time.perf_counter() - t0
@@ -204,6 +205,7 @@ def batching_forms():
                                   "paths": int(tune["pool"]),
                                   "trace": TRACE_MODES.get(int(tune["trace_mode"]), str(tune["trace_mode"])),
                                   "stack": int(tune["stack"]), "steps_per_graph": int(tune["steps_per_graph"]),
+                                  "dice_depth": int(tune["dice_depth"]), "scene_dice_depth": d0,
                                   "tune_seconds": round(tune_s, 2)},
                     "speedup_vs_plain": round(tuned / plain, 2),
                     "speedup_vs_default_wavefront": round(tuned / dflt, 2),
@@ -233,6 +235,7 @@ def full_config_renders():
         log(f"full config: {name}")
         ctx = gpu.Context(0)
         g = gpu.GpuScene(ctx, make())
+        d0 = int(g.info.dice_depth)
         t0 = time.perf_counter()
         tune = gpu.autotune_render(ctx, g, W, H, spp, path_length=pl)
         tune_s = time.perf_counter() - t0
@@ -243,7 +246,8 @@ def full_config_renders():
                     "autotuned": {"form": RENDER_FORMS.get(int(tune["mode"]), str(tune["mode"])),
                                   "paths": int(tune["pool"]),
                                   "trace": TRACE_MODES.get(int(tune["trace_mode"]), str(tune["trace_mode"])),
-                                  "stack": int(tune["stack"])},
+                                  "stack": int(tune["stack"]), "dice_depth": int(tune["dice_depth"]),
+                                  "scene_dice_depth": d0},
                     "tune_seconds": round(tune_s, 2),
                     "film_mean_xyz": [round(float(v), 6) for v in film.mean(dim=(0, 1)).cpu().tolist()]})
         g.close()

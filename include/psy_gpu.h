@@ -337,6 +337,12 @@ int psy_scene_get_info(const psy_scene* scene, psy_scene_info* info);
 /* InstanceID (basics/instance_id.hpp:9-40) of every flattened leaf, as the
  * reference's Tracer would report it in Intersection::id (host memory). */
 int psy_scene_instance_ids(const psy_scene* scene, uint64_t* out, int64_t n);
+/* Re-dices the scene: rebuilds its dice cache at dice_depth (as in
+ * psy_scene_create; < 0 the memory budget's depth).  Waits for the device
+ * first.  Hits and renders do not change: a split deeper than the cache is
+ * diced by the second pass.  No counterpart in the reference, which splits
+ * per ray batch; psy_render_autotune uses it to pick the depth. */
+int psy_scene_set_dice_depth(psy_scene* scene, int dice_depth);
 
 /* Replaces: SubdivisionSurface::finalize (object/subdivision_surface.cpp:
  * 85-233), the Catmull-Clark patch extraction it leaves to OpenSubdiv.
@@ -488,9 +494,11 @@ int psy_film_adaptive(psy_ctx* ctx, const float* sample_xyz, int32_t width, int3
  * by coordinate descent over the render form (PSY_RENDER_STREAM or
  * PSY_RENDER_WAVEFRONT) with its paths in flight (render chunk,
  * 2^19..2^23), the main pass's launch form (psy_ctx_set_trace_mode), its
- * LDS stack (psy_ctx_set_trace_stack) and the stream form's steps per graph
- * (psy_ctx_set_stream_steps), and leaves the fastest setting on the
- * context.  `out` (or NULL) receives it. */
+ * LDS stack (psy_ctx_set_trace_stack), the stream form's steps per graph
+ * (psy_ctx_set_stream_steps) and the scene's dicing depth
+ * (psy_scene_set_dice_depth: the cache 4 and 2 levels shallower and 2
+ * deeper, within 16 GiB), and leaves the fastest setting on the context
+ * and the scene.  `out` (or NULL) receives it. */
 typedef struct psy_render_tuning {
     int32_t mode;             /* PSY_RENDER_STREAM or PSY_RENDER_WAVEFRONT */
     int32_t trace_mode;       /* PSY_TRACE_MODE_*                        */
@@ -499,9 +507,10 @@ typedef struct psy_render_tuning {
     int32_t steps_per_graph;
     int32_t trials;           /* renders timed                           */
     int64_t pool;             /* paths in flight                         */
+    int32_t dice_depth;       /* the scene's dice-cache depth             */
     double mrays_per_s;       /* of the best trial                       */
 } psy_render_tuning;
-int psy_render_autotune(psy_ctx* ctx, const psy_scene* scene, const psy_render_params* params,
+int psy_render_autotune(psy_ctx* ctx, psy_scene* scene, const psy_render_params* params,
                         int64_t max_samples, psy_render_tuning* out);
 
 /* ---- differentiation ---------------------------------------------------- */

@@ -699,6 +699,72 @@ bool invert_d(const M44& m, double inv[16]) {
 
 namespace psy {
 
+int dice_layout(size_t n_patches, int dice_depth, int& levels, int64_t& per, std::string& err) {
+    levels = dice_depth;
+    if (levels < 0) {
+        // Up to 24 levels while the cache stays within 2 GiB (scenes of a few
+        // large patches, e.g. a ground plane diced to pixel size), else up to
+        // 18 levels within 16 GiB (under 6% of an MI355X's HBM); rays that
+        // need deeper splits take the second pass
+        const double gib = 1024.0 * 1024.0 * 1024.0;
+        auto bytes = [&](int l) { return (double)((int64_t)1 << (l + 1)) * 32.0 * (double)n_patches; };
+        levels = 24;
+        while (levels > 18 && bytes(levels) > 2.0 * gib) --levels;
+        while (levels > 0 && bytes(levels) > 16.0 * gib) --levels;
+    }
+    if (levels > 24) {
+        err = "dice_depth above 24";
+        return PSY_ERR_INVALID;
+    }
+    per = (int64_t)1 << (levels + 1);  // slot: 1-based heap, entry 0 unused
+    // the main pass addresses the cache with 32-bit node indices
+    if ((double)per * (double)n_patches > 4294967295.0) {
+        err = "dice cache above 2^32 nodes; lower dice_depth";
+        return PSY_ERR_INVALID;
+    }
+    return PSY_OK;
+}
+
+int rebuild_dice(psy_scene* sc, int dice_depth) {
+    int levels = 0;
+    int64_t per = 2;
+    std::string err;
+    if (int rc = dice_layout(sc->patch_ids.size(), dice_depth, levels, per, err))
+        return fail(rc, "psy_scene_set_dice_depth: " + err);
+    HIP_OK(hipSetDevice(sc->device));
+    for (size_t k = 0; k < sc->patch_ids.size(); ++k) {
+        sc->host_objs[sc->patch_ids[k]].dice_levels = levels;
+        sc->host_objs[sc->patch_ids[k]].dice = (int64_t)k * per;
+    }
+    for (DevInstance& I : sc->host_insts) {
+        I.dice_levels = sc->host_objs[I.object].dice_levels;
+        I.dice = sc->host_objs[I.object].dice;
+    }
+    // no launch of any stream may still read the old cache or leaves
+    HIP_OK(hipDeviceSynchronize());
+    int rc;
+    if ((rc = sc->inst.upload(sc->host_insts))) return rc;
+    if ((rc = sc->obj.upload(sc->host_objs))) return rc;
+    const size_t old_bytes = sc->dice.bytes;
+    const size_t dice_bytes = (size_t)per * sc->patch_ids.size() * 32;
+    if ((rc = sc->dice.alloc(dice_bytes))) return rc;
+    if (!sc->patch_ids.empty()) {
+        DBuf pid;
+        if ((rc = pid.upload(sc->patch_ids.data(), sc->patch_ids.size() * sizeof(int32_t)))) return rc;
+        rc = launch_build_dice((const float*)sc->odata.p, (const DevObject*)sc->obj.p, (const int32_t*)pid.p,
+                               (int)sc->patch_ids.size(), levels, (float4*)sc->dice.p, nullptr);
+        if (rc) return fail(rc, "psy_scene_set_dice_depth: dice cache build launch failed");
+        HIP_OK(hipDeviceSynchronize());
+    }
+    sc->dev.inst = (const DevInstance*)sc->inst.p;
+    sc->dev.obj = (const DevObject*)sc->obj.p;
+    sc->dev.dice = (const float4*)sc->dice.p;
+    sc->info.dice_depth = levels;
+    sc->info.dice_cache_bytes = (int64_t)dice_bytes;
+    sc->info.device_bytes += (int64_t)dice_bytes - (int64_t)old_bytes;
+    return PSY_OK;
+}
+
 int prepare_scene(const psy_scene_desc* d, int dice_depth, HostScene& H) {
     if (!d) return fail(PSY_ERR_INVALID, "psy_scene_create: NULL argument");
     if (d->n_assemblies < 1 || !d->assemblies) return fail(PSY_ERR_INVALID, "psy_scene_create: no root assembly");
@@ -937,23 +1003,10 @@ int prepare_scene(const psy_scene_desc* d, int dice_depth, HostScene& H) {
     std::vector<int32_t>& patch_ids = H.patch_ids;
     for (size_t i = 0; i < objs.size(); ++i)
         if (objs[i].kind == PSY_OBJ_BILINEAR || objs[i].kind == PSY_OBJ_BICUBIC) patch_ids.push_back((int32_t)i);
-    int levels = dice_depth;
-    if (levels < 0) {
-        // Up to 24 levels while the cache stays within 2 GiB (scenes of a few
-        // large patches, e.g. a ground plane diced to pixel size), else up to
-        // 18 levels within 16 GiB (under 6% of an MI355X's HBM); rays that
-        // need deeper splits take the second pass
-        const double gib = 1024.0 * 1024.0 * 1024.0;
-        auto bytes = [&](int l) { return (double)((int64_t)1 << (l + 1)) * 32.0 * (double)patch_ids.size(); };
-        levels = 24;
-        while (levels > 18 && bytes(levels) > 2.0 * gib) --levels;
-        while (levels > 0 && bytes(levels) > 16.0 * gib) --levels;
-    }
-    if (levels > 24) return fail(PSY_ERR_INVALID, "psy_scene_create: dice_depth above 24");
-    const int64_t per = (int64_t)1 << (levels + 1);  // slot: 1-based heap, entry 0 unused
-    // the main pass addresses the cache with 32-bit node indices
-    if ((double)per * (double)patch_ids.size() > 4294967295.0)
-        return fail(PSY_ERR_INVALID, "psy_scene_create: dice cache above 2^32 nodes; lower dice_depth");
+    int levels = 0;
+    int64_t per = 2;
+    std::string derr;
+    if (int rc = dice_layout(patch_ids.size(), dice_depth, levels, per, derr)) return fail(rc, "psy_scene_create: " + derr);
     H.levels = levels;
     H.per = per;
     for (size_t k = 0; k < patch_ids.size(); ++k) {
@@ -1053,6 +1106,9 @@ int psy_scene_create(psy_ctx* ctx, const psy_scene_desc* d, int dice_depth, psy_
     if ((rc = sc->escale_dev.upload(H.escale))) return rc;
     if ((rc = sc->lparam_dev.upload(H.light_param))) return rc;
     sc->host_shaders = H.shaders;
+    sc->host_insts = H.insts;
+    sc->host_objs = H.objs;
+    sc->patch_ids = H.patch_ids;
     sc->light_objects = H.light_objects;
     sc->escale = H.escale;
     if ((rc = upload_lights(H, sc.get()))) return rc;
@@ -1096,6 +1152,11 @@ int psy_scene_create(psy_ctx* ctx, const psy_scene_desc* d, int dice_depth, psy_
                                   sc->shaders.bytes + dice_bytes);
     *out = sc.release();
     return PSY_OK;
+}
+
+int psy_scene_set_dice_depth(psy_scene* s, int dice_depth) {
+    if (!s) return fail(PSY_ERR_INVALID, "psy_scene_set_dice_depth: NULL scene");
+    return rebuild_dice(s, dice_depth);
 }
 
 int psy_scene_param_counts(const psy_scene* s, int64_t* n_albedo, int64_t* n_emission) {

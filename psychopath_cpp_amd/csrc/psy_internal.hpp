@@ -165,6 +165,11 @@ struct psy_scene {
     // differentiable parameters: shader albedos (host copy of the table),
     // emission scales of the light objects
     std::vector<psy::DevShader> host_shaders;
+    // leaves, objects and the patch objects in dice-cache slot order, kept
+    // to re-dice the scene at another depth (psy_scene_set_dice_depth)
+    std::vector<psy::DevInstance> host_insts;
+    std::vector<psy::DevObject> host_objs;
+    std::vector<int32_t> patch_ids;
     std::vector<int32_t> light_objects;
     std::vector<float> escale;
     psy::DBuf escale_dev, lparam_dev;
@@ -220,6 +225,12 @@ struct HostScene {
 // 48 floats each (psy_subdiv.cpp); err holds the reason of a nonzero status.
 int subdivide_patches(const float* verts, int32_t n_verts, const int32_t* counts, int32_t n_faces,
                       const int32_t* indices, std::vector<float>& out, std::string& err);
+// Dice-cache depth for n patch objects: dice_depth, or below 0 the memory
+// budget's (up to 24 levels within 2 GiB, else up to 18 within 16 GiB);
+// per = nodes per patch slot.
+int dice_layout(size_t n_patches, int dice_depth, int& levels, int64_t& per, std::string& err);
+// Rebuilds a scene's dice cache at another depth (psy_capi.cpp).
+int rebuild_dice(psy_scene* sc, int dice_depth);
 // Host part of psy_scene_create (psy_capi.cpp); no device calls.
 int prepare_scene(const psy_scene_desc* d, int dice_depth, HostScene& H);
 // Light trees of every assembly (psy_render.cpp); obj_base: first global

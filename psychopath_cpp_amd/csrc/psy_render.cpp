@@ -726,7 +726,7 @@ int psy_render_adaptive(psy_ctx* ctx, const psy_scene* s, const psy_render_param
 // form and LDS stack depth, and the stream form's steps per captured graph,
 // timing psy_render on a sample budget of the image (every pixel at reduced
 // spp).  Leaves the best setting on the context.
-int psy_render_autotune(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, int64_t max_samples,
+int psy_render_autotune(psy_ctx* ctx, psy_scene* s, const psy_render_params* prm, int64_t max_samples,
                         psy_render_tuning* out) {
     if (int rc = check_render(ctx, s, prm, "psy_render_autotune")) return rc;
     HIP_OK(hipSetDevice(ctx->device));
@@ -786,6 +786,27 @@ int psy_render_autotune(psy_ctx* ctx, const psy_scene* s, const psy_render_param
     if (!rc && best.tmode == PSY_TRACE_MODE_PLAIN) { Cfg t = best; t.tmode = PSY_TRACE_MODE_PLAIN_COHERENT; rc = consider(t); }
     if (best.mode == PSY_RENDER_STREAM)
         for (int it : {2, 8, 16}) if (!rc && it != best.iters) { Cfg t = best; t.iters = it; rc = consider(t); }
+    // dicing depth: the scene's dice cache re-built 4 and 2 levels shallower
+    // and 2 deeper (within 16 GiB and the 2^32-node cap); rays that need
+    // deeper splits than the cache holds go to the second pass
+    const int s_levels = s->info.dice_depth;
+    int best_levels = s_levels;
+    if (!rc && !s->patch_ids.empty()) {
+        const double cap = 16.0 * 1024.0 * 1024.0 * 1024.0;
+        apply(best);
+        for (int d : {s_levels - 4, s_levels - 2, s_levels + 2}) {
+            if (d < 0 || d > 24) continue;
+            if ((double)((int64_t)1 << (d + 1)) * 32.0 * (double)s->patch_ids.size() > cap) continue;
+            if ((rc = rebuild_dice(s, d))) break;
+            double m = 0.0;
+            if ((rc = timed(best, &m))) break;
+            if (m > best_m) { best_m = m; best_levels = d; }
+        }
+        if (rc || s->info.dice_depth != best_levels) {
+            const int rrc = rebuild_dice(s, rc ? s_levels : best_levels);
+            if (!rc) rc = rrc;
+        }
+    }
     if (rc) {
         ctx->render_mode = s_mode; ctx->mode = s_tmode; ctx->refill_min = s_refill; ctx->trace_stack = s_stack;
         ctx->render_graph_iters = s_iters; ctx->render_chunk = s_chunk;
@@ -800,6 +821,7 @@ int psy_render_autotune(psy_ctx* ctx, const psy_scene* s, const psy_render_param
         out->steps_per_graph = best.iters;
         out->trials = trials;
         out->pool = best.pool;
+        out->dice_depth = best_levels;
         out->mrays_per_s = best_m;
     }
     return PSY_OK;

@@ -45,6 +45,7 @@ def lib():
     L.psy_scene_get_info.argtypes = [vp, C.POINTER(abi.SceneInfo)]
     L.psy_scene_instance_ids.argtypes = [vp, vp, i64]
     L.psy_subdivision_patches.argtypes = [vp, i32, vp, i32, vp, vp, i64, C.POINTER(i64)]
+    L.psy_scene_set_dice_depth.argtypes = [vp, C.c_int]
     L.psy_trace_world.argtypes = [vp, vp, vp, i64, C.POINTER(abi.Hits), u32, vp]
     L.psy_trace_rays.argtypes = [vp, vp, vp, i64, C.POINTER(abi.Hits), u32, vp]
     L.psy_trace_rays_static.argtypes = [vp, vp, vp, i64, C.POINTER(abi.Hits), u32, vp]
@@ -207,9 +208,18 @@ class GpuScene:
         check(lib().psy_scene_create(ctx.handle, self._desc.ptr(), int(dice_depth), C.byref(h)),
               "psy_scene_create")
         self.handle = h
+        self.refresh_info()
+
+    def refresh_info(self):
         info = abi.SceneInfo()
-        check(lib().psy_scene_get_info(h, C.byref(info)), "psy_scene_get_info")
+        check(lib().psy_scene_get_info(self.handle, C.byref(info)), "psy_scene_get_info")
         self.info = info
+
+    def set_dice_depth(self, depth):
+        """psy_scene_set_dice_depth: rebuild the dice cache at `depth` (< 0:
+        the memory budget's depth)."""
+        check(lib().psy_scene_set_dice_depth(self.handle, int(depth)), "psy_scene_set_dice_depth")
+        self.refresh_info()
 
     def param_counts(self):
         """psy_scene_param_counts: (shader albedos, light emission scales)."""
@@ -517,4 +527,5 @@ def autotune_render(ctx, scene, width, height, spp, seed=1, path_length=4, max_s
     out = abi.RenderTuning()
     check(lib().psy_render_autotune(ctx.handle, scene.handle, C.byref(prm), int(max_samples), C.byref(out)),
           "psy_render_autotune")
+    scene.refresh_info()
     return {f: getattr(out, f) for f, _ in abi.RenderTuning._fields_}

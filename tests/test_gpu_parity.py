@@ -272,6 +272,27 @@ def test_gpu_dice_depth_fallback_matches(gpu_ctx, oracle, c1):
     assert torch.equal(h1.u, h2.u) and torch.equal(h1.v, h2.v)
 
 
+def test_gpu_set_dice_depth_keeps_the_hits(gpu_ctx, c1):
+    """psy_scene_set_dice_depth re-dices the scene in place: other depths,
+    and back to the budget's, give bit-identical hits (splits below the cache
+    are diced by the second pass)."""
+    sc, g, o = c1
+    w, _ = gpu.camera_rays(gpu_ctx, g, 96, 96, 2, 3)
+    base = gpu.trace_world(gpu_ctx, g, w)
+    d0 = int(g.info.dice_depth)
+    try:
+        for d in (4, 9, max(0, d0 - 1)):
+            g.set_dice_depth(d)
+            assert g.info.dice_depth == d
+            h = gpu.trace_world(gpu_ctx, g, w)
+            say("dice depth", d, "second pass", gpu_ctx.retrace_count())
+            assert _hits_equal(h, base)
+    finally:
+        g.set_dice_depth(-1)
+    assert g.info.dice_depth == d0
+    assert _hits_equal(gpu.trace_world(gpu_ctx, g, w), base)
+
+
 def test_gpu_empty_and_single_instance_scenes(gpu_ctx, oracle):
     root = Assembly()
     sc = Scene(Camera([look_at((0, -5, 0), (0, 0, 0))], [np.radians(40)]), root)

@@ -296,6 +296,8 @@ def test_gpu_render_autotune(gpu_ctx):
         assert t["mode"] in (abi.PSY_RENDER_STREAM, abi.PSY_RENDER_WAVEFRONT)
         assert t["trials"] >= 4 and t["mrays_per_s"] > 0
         assert t["stack"] in (16, 32) and t["steps_per_graph"] % 2 == 0
+        # the scene is left at the dicing depth the tuner picked
+        assert t["dice_depth"] == g.info.dice_depth and 0 <= t["dice_depth"] <= 24
         f_s, _, r_s = gpu.render(ctx, g, 64, 48, 4, 1)
         assert r_s == r_wf and torch.equal(f_s, f_wf)
     finally:

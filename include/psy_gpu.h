@@ -338,8 +338,9 @@ int psy_scene_get_info(const psy_scene* scene, psy_scene_info* info);
  * reference's Tracer would report it in Intersection::id (host memory). */
 int psy_scene_instance_ids(const psy_scene* scene, uint64_t* out, int64_t n);
 /* Re-dices the scene: rebuilds its dice cache at dice_depth (as in
- * psy_scene_create; < 0 the memory budget's depth).  Waits for the device
- * first.  Hits and renders do not change: a split deeper than the cache is
+ * psy_scene_create; < 0 the memory budget's depth), beside the old one
+ * (which is freed after), and waits for the device; on failure the scene is
+ * unchanged.  Hits and renders do not change: a split deeper than the cache is
  * diced by the second pass.  No counterpart in the reference, which splits
  * per ray batch; psy_render_autotune uses it to pick the depth. */
 int psy_scene_set_dice_depth(psy_scene* scene, int dice_depth);

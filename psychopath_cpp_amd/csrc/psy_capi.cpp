@@ -732,30 +732,41 @@ int rebuild_dice(psy_scene* sc, int dice_depth) {
     if (int rc = dice_layout(sc->patch_ids.size(), dice_depth, levels, per, err))
         return fail(rc, "psy_scene_set_dice_depth: " + err);
     HIP_OK(hipSetDevice(sc->device));
+    // the new leaves, objects and cache are built beside the old ones and
+    // swapped in only once all of them exist, so a failure leaves the scene
+    // as it was
+    std::vector<DevObject> objs = sc->host_objs;
+    std::vector<DevInstance> insts = sc->host_insts;
     for (size_t k = 0; k < sc->patch_ids.size(); ++k) {
-        sc->host_objs[sc->patch_ids[k]].dice_levels = levels;
-        sc->host_objs[sc->patch_ids[k]].dice = (int64_t)k * per;
+        objs[sc->patch_ids[k]].dice_levels = levels;
+        objs[sc->patch_ids[k]].dice = (int64_t)k * per;
     }
-    for (DevInstance& I : sc->host_insts) {
-        I.dice_levels = sc->host_objs[I.object].dice_levels;
-        I.dice = sc->host_objs[I.object].dice;
+    for (DevInstance& I : insts) {
+        I.dice_levels = objs[I.object].dice_levels;
+        I.dice = objs[I.object].dice;
     }
-    // no launch of any stream may still read the old cache or leaves
-    HIP_OK(hipDeviceSynchronize());
+    DBuf inst, obj, dice;
     int rc;
-    if ((rc = sc->inst.upload(sc->host_insts))) return rc;
-    if ((rc = sc->obj.upload(sc->host_objs))) return rc;
-    const size_t old_bytes = sc->dice.bytes;
+    if ((rc = inst.upload(insts))) return rc;
+    if ((rc = obj.upload(objs))) return rc;
     const size_t dice_bytes = (size_t)per * sc->patch_ids.size() * 32;
-    if ((rc = sc->dice.alloc(dice_bytes))) return rc;
+    if ((rc = dice.alloc(dice_bytes))) return rc;
     if (!sc->patch_ids.empty()) {
         DBuf pid;
         if ((rc = pid.upload(sc->patch_ids.data(), sc->patch_ids.size() * sizeof(int32_t)))) return rc;
-        rc = launch_build_dice((const float*)sc->odata.p, (const DevObject*)sc->obj.p, (const int32_t*)pid.p,
-                               (int)sc->patch_ids.size(), levels, (float4*)sc->dice.p, nullptr);
+        rc = launch_build_dice((const float*)sc->odata.p, (const DevObject*)obj.p, (const int32_t*)pid.p,
+                               (int)sc->patch_ids.size(), levels, (float4*)dice.p, nullptr);
         if (rc) return fail(rc, "psy_scene_set_dice_depth: dice cache build launch failed");
-        HIP_OK(hipDeviceSynchronize());
     }
+    // no launch of any stream may still read the old cache or leaves, and
+    // the new cache must be complete
+    HIP_OK(hipDeviceSynchronize());
+    const size_t old_bytes = sc->dice.bytes;
+    std::swap(sc->inst.p, inst.p); std::swap(sc->inst.bytes, inst.bytes);
+    std::swap(sc->obj.p, obj.p); std::swap(sc->obj.bytes, obj.bytes);
+    std::swap(sc->dice.p, dice.p); std::swap(sc->dice.bytes, dice.bytes);
+    sc->host_objs.swap(objs);
+    sc->host_insts.swap(insts);
     sc->dev.inst = (const DevInstance*)sc->inst.p;
     sc->dev.obj = (const DevObject*)sc->obj.p;
     sc->dev.dice = (const float4*)sc->dice.p;

@@ -919,6 +919,9 @@ struct Lane {
     V3 wo, wdinv;
     RayW ww;
     float wfx, wfy;  // flip distances of the patch-space footprint
+    // the leaf transforms wo .. wfy were derived for: the first transform's
+    // index, -2 for an untransformed leaf, -1 for none yet
+    int32_t wframe;
     uint32_t winst, k, trail;
     int levels, depth;
     // BVH: the nearest hit child, visited next without a stack round trip
@@ -1004,6 +1007,7 @@ PSY_DEV void lane_start(const DevScene& s, Lane& L, float max_t, uint32_t i) {
     L.sp = 0;
     L.state = 0;
     L.deep_head = DEEP_NONE;
+    L.wframe = -1;  // read by world rays only
     // the main pass's box tests assume a finite inverse direction; the
     // second pass follows the reference's NaN behaviour exactly
     if (!finite3(L.r.dinv)) {
@@ -1014,9 +1018,16 @@ PSY_DEV void lane_start(const DevScene& s, Lane& L, float max_t, uint32_t i) {
 
 // The patch-space ray of a walk through leaf I's split tree (world rays:
 // through the leaf's transform at the ray's time) and its footprint's flip
-// distances; false when its inverse direction is not finite.
+// distances; false when its inverse direction is not finite.  They depend
+// on the leaf's transforms alone, so the patches of one instance (a
+// subdivision surface's, an instanced assembly's) reuse them: a ray that
+// meets several patches of a transformed instance derives them once.
 template <bool WORLD>
 PSY_DEV bool walk_ray(const DevScene& s, Lane& L, const DevInstance& I) {
+    // object rays keep only the two flip distances, cheaper to re-derive
+    // than to hold in a register through the BVH walk
+    const int32_t frame = I.xf_count > 0 ? I.xf_offset : -2;
+    if (WORLD && frame == L.wframe) return true;
     bool finite = true;
     if (WORLD) {
         L.wo = L.r.o;
@@ -1033,6 +1044,7 @@ PSY_DEV bool walk_ray(const DevScene& s, Lane& L, const DevInstance& I) {
     }
     L.wfx = flip_x(walk_w<WORLD>(L));
     L.wfy = flip_y(walk_w<WORLD>(L));
+    if (WORLD) L.wframe = finite ? frame : -1;
     return finite;
 }
 

@@ -57,7 +57,6 @@
 #include <cmath>
 #include <cstdint>
 #include <string>
-#include <unordered_map>
 #include <vector>
 
 #include "psy_internal.hpp"
@@ -103,11 +102,6 @@ struct Level {
     int rot(int h) const { return twin[prev(h)]; }
 };
 
-struct Out {
-    std::vector<float>* patches;
-    std::string err;
-};
-
 bool build_topology(Level& L, bool full, std::string& err) {
     const int nf = L.nf();
     const size_t nh = L.fv.size();
@@ -115,26 +109,34 @@ bool build_topology(Level& L, bool full, std::string& err) {
     L.hface.resize(nh);
     for (int f = 0; f < nf; ++f)
         for (int h = L.off[f]; h < L.off[f + 1]; ++h) L.hface[h] = f;
-    std::unordered_map<uint64_t, int32_t> dir;
-    dir.reserve(nh * 2);
-    for (size_t h = 0; h < nh; ++h) {
-        const uint64_t key = ((uint64_t)(uint32_t)L.org((int)h) << 32) | (uint32_t)L.dst((int)h);
-        if (!dir.emplace(key, (int32_t)h).second) {
-            err = "an edge is used twice in the same direction (a non-manifold edge or inconsistently oriented faces)";
-            return false;
-        }
-    }
-    L.twin.assign(nh, -1);
-    for (size_t h = 0; h < nh; ++h) {
-        const uint64_t key = ((uint64_t)(uint32_t)L.dst((int)h) << 32) | (uint32_t)L.org((int)h);
-        auto it = dir.find(key);
-        if (it != dir.end()) L.twin[h] = it->second;
-    }
-    std::vector<int32_t> outc(nv, 0), lowest(nv, -1);
+    // outgoing half-edges of every vertex, in half-edge order (CSR): a
+    // twin is found among the few half-edges leaving its edge's far end
+    std::vector<int32_t> outc(nv, 0), lowest(nv, -1), out_off(nv + 1, 0), out_h(nh);
     for (size_t h = 0; h < nh; ++h) {
         const int v = L.org((int)h);
         if (lowest[v] < 0) lowest[v] = (int32_t)h;
         ++outc[v];
+    }
+    for (size_t v = 0; v < nv; ++v) out_off[v + 1] = out_off[v] + outc[v];
+    {
+        std::vector<int32_t> fill(out_off.begin(), out_off.end() - 1);
+        for (size_t h = 0; h < nh; ++h) out_h[fill[L.org((int)h)]++] = (int32_t)h;
+    }
+    L.twin.assign(nh, -1);
+    for (size_t h = 0; h < nh; ++h) {
+        const int a = L.org((int)h), b = L.dst((int)h);
+        for (int k = out_off[a]; k < out_off[a + 1]; ++k) {
+            const int g = out_h[k];
+            if (g != (int)h && L.dst(g) == b) {
+                err = "an edge is used twice in the same direction (a non-manifold edge or inconsistently oriented faces)";
+                return false;
+            }
+        }
+        for (int k = out_off[b]; k < out_off[b + 1]; ++k)
+            if (L.dst(out_h[k]) == a) {
+                L.twin[h] = out_h[k];
+                break;
+            }
     }
     L.vfirst.assign(nv, -1);
     L.vfan.assign(nv, 0);

@@ -71,3 +71,25 @@ def test_gpu_subdivision_render_matches_oracle(gpu_ctx, oracle):
     assert close >= 0.95 and worst <= 3.0
     assert abs(rays - orays) <= 0.01 * orays
     torch.cuda.synchronize()
+
+
+def test_gpu_parsed_subdivision_scene_matches_oracle(gpu_ctx, oracle):
+    """The scene written as .psy text (SubdivisionSurface nodes with
+    Vertices, FaceVertCounts, FaceVertIndices) and read back by the
+    parser renders as the oracle renders the parsed description."""
+    from psychopath_cpp_amd import psyfile
+    from tests import _psywriter
+    sc = M.subdiv_scene(False)
+    p = psyfile.ParsedScene(_psywriter.write_psy(sc))
+    assert p.warnings == ""
+    W, H, spp = 48, 32, 4
+    g = gpu.GpuScene(gpu_ctx, p)
+    o = oracle.OracleScene(p)
+    _, samples, rays = gpu.render(gpu_ctx, g, W, H, spp, 1, samples=True)
+    torch.cuda.synchronize()
+    gs = samples.cpu().numpy()
+    os_, orays = o.render(W, H, spp, 1, threads=8)
+    worst, close = tile_check(gs, os_)
+    say("parsed subdivision render: rays", rays, orays, "close", close, "worst tile z", worst)
+    assert gs.mean() > 0.0 and close >= 0.95 and worst <= 3.0
+    assert abs(rays - orays) <= 0.01 * orays

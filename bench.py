@@ -180,15 +180,17 @@ def batching_forms():
         ctx = gpu.Context(0)
         g = gpu.GpuScene(ctx, make())
         gpu.set_render_mode(ctx)
+        gpu.set_render_streams(ctx, 1)
         gpu.set_render_chunk(ctx, REF_BUCKET_PATHS)
         plain, f_plain = _render_rate(gpu, ctx, g, W, H, spp, pl, reps=1)
         log(f"batching [{name[:40]}]: plain wavefront, {REF_BUCKET_PATHS}-path buckets: {plain:.1f} Mrays/s")
+        gpu.set_render_streams(ctx, 2)
         gpu.set_render_chunk(ctx, 1 << 21)
         dflt, f_dflt = _render_rate(gpu, ctx, g, W, H, spp, pl)
-        log(f"batching: wavefront, library default 2^21-path chunks: {dflt:.1f} Mrays/s")
+        log(f"batching: wavefront, library default 2^21-path chunks, two in flight: {dflt:.1f} Mrays/s")
         gpu.set_render_chunk(ctx, 1 << 22)
         wide, f_wide = _render_rate(gpu, ctx, g, W, H, spp, pl)
-        log(f"batching: wavefront, 2^22-path chunks: {wide:.1f} Mrays/s")
+        log(f"batching: wavefront, 2^22-path chunks, two in flight: {wide:.1f} Mrays/s")
         d0 = int(g.info.dice_depth)
         t0 = time.perf_counter()
         tune = gpu.autotune_render(ctx, g, W, H, spp, path_length=pl)
@@ -199,13 +201,14 @@ def batching_forms():
         out.append({"workload": name, "plain_mrays": round(plain, 2),
                     "plain_form": f"wavefront, {REF_BUCKET_PATHS}-path buckets, host sync per step",
                     "default_wavefront_mrays": round(dflt, 2),
-                    "default_form": "wavefront, the library's default 2^21-path chunks",
+                    "default_form": "wavefront, the library's default 2^21-path chunks, two in flight",
                     "wavefront_2p22_mrays": round(wide, 2), "autotuned_mrays": round(tuned, 2),
                     "autotuned": {"form": RENDER_FORMS.get(int(tune["mode"]), str(tune["mode"])),
                                   "paths": int(tune["pool"]),
                                   "trace": TRACE_MODES.get(int(tune["trace_mode"]), str(tune["trace_mode"])),
                                   "stack": int(tune["stack"]), "steps_per_graph": int(tune["steps_per_graph"]),
                                   "dice_depth": int(tune["dice_depth"]), "scene_dice_depth": d0,
+                                  "streams": int(tune["streams"]),
                                   "tune_seconds": round(tune_s, 2)},
                     "speedup_vs_plain": round(tuned / plain, 2),
                     "speedup_vs_default_wavefront": round(tuned / dflt, 2),
@@ -247,7 +250,7 @@ def full_config_renders():
                                   "paths": int(tune["pool"]),
                                   "trace": TRACE_MODES.get(int(tune["trace_mode"]), str(tune["trace_mode"])),
                                   "stack": int(tune["stack"]), "dice_depth": int(tune["dice_depth"]),
-                                  "scene_dice_depth": d0},
+                                  "scene_dice_depth": d0, "streams": int(tune["streams"])},
                     "tune_seconds": round(tune_s, 2),
                     "film_mean_xyz": [round(float(v), 6) for v in film.mean(dim=(0, 1)).cpu().tolist()]})
         g.close()

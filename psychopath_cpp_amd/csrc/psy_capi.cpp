@@ -143,23 +143,25 @@ void gtr_validate(float roughness, float tail_shape, float fresnel, DevShader& o
     o.norm = (float)((double)top / bottom);
 }
 
-int ensure_queue(psy_ctx* ctx, int64_t n) {
-    if (!ctx->counters.p) {
-        int rc = ctx->counters.alloc(sizeof(uint32_t) * 8);
+int ensure_trace_scratch(DBuf& queue, DBuf& counters, int64_t& queue_cap, int64_t n) {
+    if (!counters.p) {
+        int rc = counters.alloc(sizeof(uint32_t) * 8);
         if (rc) return rc;
     }
-    if (n <= ctx->queue_cap) return PSY_OK;
+    if (n <= queue_cap) return PSY_OK;
     HIP_OK(hipDeviceSynchronize());
-    ctx->queue.release();
+    queue.release();
     const int64_t cap = std::max<int64_t>(n, 1 << 16);
     // for up to cap rays: the second pass's queue, the third pass's queue,
     // the second-pass rays' deep-node lists and the deep-node pool (3 words
     // an entry), then the main pass's continuation buffer
-    int rc = ctx->queue.alloc((size_t)(6 * cap + psy::cont_area_words(cap)) * sizeof(uint32_t));
-    if (rc) { ctx->queue_cap = 0; return rc; }
-    ctx->queue_cap = cap;
+    int rc = queue.alloc((size_t)(6 * cap + psy::cont_area_words(cap)) * sizeof(uint32_t));
+    if (rc) { queue_cap = 0; return rc; }
+    queue_cap = cap;
     return PSY_OK;
 }
+
+int ensure_queue(psy_ctx* ctx, int64_t n) { return ensure_trace_scratch(ctx->queue, ctx->counters, ctx->queue_cap, n); }
 
 static int capturing(hipStream_t st, bool* yes) {
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
@@ -280,6 +282,13 @@ int psy_ctx_set_render_chunk(psy_ctx* ctx, int64_t paths) {
     if (!ctx) return fail(PSY_ERR_INVALID, "psy_ctx_set_render_chunk: NULL context");
     if (paths < 1 || paths > (int64_t)INT32_MAX) return fail(PSY_ERR_INVALID, "psy_ctx_set_render_chunk: paths not in 1..2^31-1");
     ctx->render_chunk = paths;
+    return PSY_OK;
+}
+
+int psy_ctx_set_render_streams(psy_ctx* ctx, int streams) {
+    if (!ctx) return fail(PSY_ERR_INVALID, "psy_ctx_set_render_streams: NULL context");
+    if (streams != 1 && streams != 2) return fail(PSY_ERR_INVALID, "psy_ctx_set_render_streams: streams must be 1 or 2");
+    ctx->render_streams = streams;
     return PSY_OK;
 }
 

@@ -143,6 +143,11 @@ struct psy_ctx {
         for (void* e : timing_events) (void)hipEventDestroy((hipEvent_t)e);
         if (scratch_done) (void)hipEventDestroy(scratch_done);
         if (capture_stream) (void)hipStreamDestroy(capture_stream);
+        if (render_stream2) (void)hipStreamDestroy(render_stream2);
+        if (render_join) (void)hipEventDestroy(render_join);
+        for (hipEvent_t e : render_done)
+            if (e) (void)hipEventDestroy(e);
+        if (host_counts) (void)hipHostFree(host_counts);
     }
     // integrator scratch (path states, ray / hit streams, active lists)
     psy::DBuf render_scratch;
@@ -154,6 +159,20 @@ struct psy_ctx {
     int render_occupancy = 0;
     int render_graph_iters = 4;      // stream mode: steps per captured graph
     hipStream_t capture_stream = nullptr;  // for graph capture (never runs work)
+    // The wavefront form's chunks in flight (1 or 2).  The second chunk runs
+    // on render_stream2 with its own scratch, trace queue and counters;
+    // render_join orders that stream after the caller's at the start of a
+    // render and the caller's after it at the end.
+    int render_streams = 2;
+    hipStream_t render_stream2 = nullptr;
+    hipEvent_t render_join = nullptr;
+    hipEvent_t render_done[2] = {nullptr, nullptr};
+    psy::DBuf render_scratch2, queue2, counters2;
+    int64_t queue2_cap = 0;
+    // pinned words the wavefront form reads each chunk's next active count
+    // into (word 0: first chunk, word 16: second), so that one chunk's read
+    // back does not wait for the other chunk's queued work
+    uint32_t* host_counts = nullptr;
 };
 
 struct psy_scene {
@@ -241,6 +260,8 @@ int upload_lights(const HostScene& H, psy_scene* sc);
 void make_halton_tables(std::vector<HaltonDim>& dims, std::vector<uint16_t>& perm);
 // Grows the context's trace scratch (second-pass queue) to n rays.
 int ensure_queue(psy_ctx* ctx, int64_t n);
+// The same for a given queue / counters pair and its capacity in rays.
+int ensure_trace_scratch(DBuf& queue, DBuf& counters, int64_t& queue_cap, int64_t n);
 // Orders a call that uses the context's trace scratch on stream st after
 // the previous such call (made on any stream): scratch_order before the
 // call's first launch, scratch_release after its last.  Streams under graph

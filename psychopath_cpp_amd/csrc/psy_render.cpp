@@ -372,10 +372,14 @@ namespace {
 
 // Chunking and scratch layout of one render call: chunks of whole pixels,
 // at most ctx->render_chunk paths; `extra` floats after the render scratch.
+// With slots == 2 (the wavefront form on two streams) B2 and
+// chunk_samples2 are the second chunk's, in the context's second scratch.
 struct RenderSetup {
     int64_t spp, n_pix, total, chunk, cap;
-    RenderBuffers B;
+    int slots = 1;
+    RenderBuffers B, B2;
     float* chunk_samples;
+    float* chunk_samples2 = nullptr;
     float* extra;
 };
 
@@ -408,34 +412,48 @@ void plan_render(const psy_ctx* ctx, const psy_render_params* prm, RenderSetup& 
     U.cap = (U.chunk + 63) / 64 * 64;       // storage stride, keeps every array 256-byte aligned
 }
 
-// After plan_render: the scratch, with `extra_floats` after the render's own.
+// Floats of one chunk's scratch: paths, 2 ray streams, hits, active lists,
+// counters, and the chunk samples when the render keeps its own.
+int64_t slot_floats(int64_t cap, bool own_samples) {
+    return (int64_t)PATH_PLANES * cap + 2 * 20 * cap + 5 * cap + 2 * cap + 64 + (own_samples ? 3 * cap : 0);
+}
+
+// Lays one chunk's scratch out from `base`; returns the float after it.
+float* carve_slot(float* base, int64_t cap, bool own_samples, RenderBuffers& B, float** samples) {
+    B.cap = cap;
+    B.paths = base;
+    B.rays[0] = base + (int64_t)PATH_PLANES * cap;
+    B.rays[1] = B.rays[0] + 20 * cap;
+    float* h = B.rays[1] + 20 * cap;
+    B.hits = DevHits{h, (uint32_t*)(h + cap), h + 2 * cap, h + 3 * cap, h + 4 * cap};
+    B.act[0] = (uint32_t*)(h + 5 * cap);
+    B.act[1] = B.act[0] + cap;
+    B.count = B.act[1] + cap;
+    *samples = (float*)(B.count + 64);
+    return *samples + (own_samples ? 3 * cap : 0);
+}
+
+// After plan_render: the scratch, with `extra_floats` after the render's
+// own, and the second chunk's when U.slots == 2.
 int alloc_render(psy_ctx* ctx, bool own_samples, int64_t extra_floats, RenderSetup& U) {
     const int64_t cap = U.cap;
     if (cap > (int64_t)INT32_MAX) return fail(PSY_ERR_UNSUPPORTED, "psy_render: more than 2^31 paths per chunk");
-    // scratch (floats): paths, 2 ray streams, hits, active lists, counters,
-    // chunk samples, extra
-    const int64_t f_paths = (int64_t)PATH_PLANES * cap;
-    const int64_t f_rays = 20 * cap;
-    const int64_t f_hits = 5 * cap;
-    const int64_t f_act = cap;
-    const int64_t f_samp = own_samples ? 3 * cap : 0;
+    const int64_t f_slot = slot_floats(cap, own_samples);
     const int64_t f_extra = (extra_floats + 63) / 64 * 64;
-    const int64_t f_total = f_paths + 2 * f_rays + f_hits + 2 * f_act + 64 + f_samp + f_extra;
-    if (int rc = ctx->render_scratch.reserve((size_t)f_total * sizeof(float))) return rc;
+    if (int rc = ctx->render_scratch.reserve((size_t)(f_slot + f_extra) * sizeof(float))) return rc;
     if (int rc = ensure_queue(ctx, cap)) return rc;
-    float* base = (float*)ctx->render_scratch.p;
-    RenderBuffers& B = U.B;
-    B.cap = cap;
-    B.paths = base;
-    B.rays[0] = base + f_paths;
-    B.rays[1] = B.rays[0] + f_rays;
-    float* h = B.rays[1] + f_rays;
-    B.hits = DevHits{h, (uint32_t*)(h + cap), h + 2 * cap, h + 3 * cap, h + 4 * cap};
-    B.act[0] = (uint32_t*)(h + f_hits);
-    B.act[1] = B.act[0] + cap;
-    B.count = B.act[1] + cap;
-    U.chunk_samples = (float*)(B.count + 64);
-    U.extra = U.chunk_samples + f_samp;
+    U.extra = carve_slot((float*)ctx->render_scratch.p, cap, own_samples, U.B, &U.chunk_samples);
+    if (!ctx->host_counts)
+        HIP_OK(hipHostMalloc((void**)&ctx->host_counts, 32 * sizeof(uint32_t), hipHostMallocDefault));
+    if (U.slots == 2) {
+        if (int rc = ctx->render_scratch2.reserve((size_t)f_slot * sizeof(float))) return rc;
+        if (int rc = ensure_trace_scratch(ctx->queue2, ctx->counters2, ctx->queue2_cap, cap)) return rc;
+        if (!ctx->render_stream2) HIP_OK(hipStreamCreateWithFlags(&ctx->render_stream2, hipStreamNonBlocking));
+        if (!ctx->render_join) HIP_OK(hipEventCreateWithFlags(&ctx->render_join, hipEventDisableTiming));
+        for (hipEvent_t& e : ctx->render_done)
+            if (!e) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        carve_slot((float*)ctx->render_scratch2.p, cap, own_samples, U.B2, &U.chunk_samples2);
+    }
     return PSY_OK;
 }
 
@@ -459,44 +477,132 @@ RenderLaunch make_launch(const psy_ctx* ctx, const psy_scene* s, const psy_rende
     return R;
 }
 
+// One chunk in flight of the wavefront form: its scratch, trace scratch,
+// stream and pinned count word, and where its paths are.
+struct ChunkSlot {
+    RenderBuffers B;
+    float* samples;  // own chunk samples (unused when the caller keeps every sample)
+    uint32_t* queue;
+    uint32_t* counters;
+    int64_t queue_cap;
+    hipStream_t st;
+    hipEvent_t done;  // recorded after each step's count read back (two slots)
+    uint32_t* count_host;
+    int64_t first = 0, n = 0, n_active = 0;  // n == 0: idle
+    int cur = 0, step = 0;
+};
+
 // The wavefront loop (integrator/path_trace_integrator.cpp:321-349) over
 // every chunk: init, then per step trace the active stream, shade it and
-// compact the paths that go on.  With `adj`, the shading step is the
-// adjoint pass's (same paths, plus gradients) and no samples are written.
+// compact the paths that go on.  With `adj` (one entry per slot), the
+// shading step is the adjoint pass's (same paths, plus gradients) and no
+// samples are written.
+// With U.slots == 2 two chunks are in flight, the second on the context's
+// second stream: the host polls both slots' step events and, for the one
+// that has ended, reads its next active count and queues its next step (or
+// its resolve and the next chunk), so each chunk's short late steps run
+// beside the other chunk's work.  Every path is computed as with one slot,
+// so the samples are the same.
 int run_wavefront(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, const RenderLaunch& R,
                   const RenderSetup& U, float* film_xyz, float* sample_xyz, const AdjointBuffers* adj,
                   int64_t* traced_out, hipStream_t st) {
-    const RenderBuffers& B = U.B;
-    int64_t traced = 0;
-    for (int64_t first = 0; first < U.total; first += U.chunk) {
-        const int64_t n = std::min(U.chunk, U.total - first);
-        float* samp = sample_xyz ? sample_xyz : U.chunk_samples;
-        const int64_t sbase = sample_xyz ? 0 : first;
-        int rc = launch_render_init(R, first, n, B, st);
-        if (!rc && adj) rc = launch_adjoint_init(R, B, *adj, n, st);
-        if (rc) return fail(rc, "psy_render: init launch failed");
-        int64_t n_active = n;
-        int cur = 0;
-        for (int step = 0; step < 2 * prm->path_length && n_active > 0; ++step) {
-            traced += n_active;
-            rc = launch_trace(s->dev, true, false, s->stack_need, B.rays[cur], n_active, B.cap, B.hits,
-                              (uint32_t*)ctx->queue.p, (uint32_t*)ctx->counters.p, ctx->tuning(true), st);
-            if (rc) return fail(rc, "psy_render: trace launch failed");
-            if ((rc = launch_zero_u32(B.count + (cur ^ 1), 1, st))) return fail(rc, "psy_render: launch failed");
-            rc = adj ? launch_render_adjoint(R, first, B, *adj, cur, n_active, st)
-                     : launch_render_update(R, first, B, cur, n_active, samp, sbase, st);
-            if (rc) return fail(rc, "psy_render: update launch failed");
-            uint32_t next = 0;
-            HIP_OK(hipMemcpyAsync(&next, B.count + (cur ^ 1), sizeof next, hipMemcpyDeviceToHost, st));
-            HIP_OK(hipStreamSynchronize(st));
-            n_active = next;
-            cur ^= 1;
-        }
-        if (film_xyz && !adj) {
-            rc = launch_render_resolve(samp, sbase, first / U.spp, n / U.spp, (int)U.spp, film_xyz, st);
-            if (rc) return fail(rc, "psy_render: resolve launch failed");
-        }
+    const int n_slots = U.slots;
+    ChunkSlot S[2];
+    S[0].B = U.B;
+    S[0].samples = U.chunk_samples;
+    S[0].queue = (uint32_t*)ctx->queue.p;
+    S[0].counters = (uint32_t*)ctx->counters.p;
+    S[0].queue_cap = ctx->queue_cap;
+    S[0].st = st;
+    S[0].count_host = ctx->host_counts;
+    S[0].done = ctx->render_done[0];
+    if (n_slots == 2) {
+        S[1].B = U.B2;
+        S[1].samples = U.chunk_samples2;
+        S[1].queue = (uint32_t*)ctx->queue2.p;
+        S[1].counters = (uint32_t*)ctx->counters2.p;
+        S[1].queue_cap = ctx->queue2_cap;
+        S[1].st = ctx->render_stream2;
+        S[1].count_host = ctx->host_counts + 16;
+        S[1].done = ctx->render_done[1];
+        // the second stream starts after the caller's queued work
+        HIP_OK(hipEventRecord(ctx->render_join, st));
+        HIP_OK(hipStreamWaitEvent(ctx->render_stream2, ctx->render_join, 0));
     }
+    int64_t traced = 0, next_first = 0;
+    auto step = [&](ChunkSlot& c) -> int {
+        traced += c.n_active;
+        TraceTuning t = ctx->tuning(true);
+        t.cont = c.queue + 6 * c.queue_cap;
+        t.cont_words = cont_area_words(c.queue_cap);
+        int rc = launch_trace(s->dev, true, false, s->stack_need, c.B.rays[c.cur], c.n_active, c.B.cap, c.B.hits,
+                              c.queue, c.counters, t, c.st);
+        if (rc) return fail(rc, "psy_render: trace launch failed");
+        if ((rc = launch_zero_u32(c.B.count + (c.cur ^ 1), 1, c.st))) return fail(rc, "psy_render: launch failed");
+        float* samp = sample_xyz ? sample_xyz : c.samples;
+        const int64_t sbase = sample_xyz ? 0 : c.first;
+        rc = adj ? launch_render_adjoint(R, c.first, c.B, adj[&c - S], c.cur, c.n_active, c.st)
+                 : launch_render_update(R, c.first, c.B, c.cur, c.n_active, samp, sbase, c.st);
+        if (rc) return fail(rc, "psy_render: update launch failed");
+        HIP_OK(hipMemcpyAsync(c.count_host, c.B.count + (c.cur ^ 1), sizeof(uint32_t), hipMemcpyDeviceToHost, c.st));
+        if (n_slots == 2) HIP_OK(hipEventRecord(c.done, c.st));
+        return PSY_OK;
+    };
+    // the slot's next chunk and its first step; no chunks left: idle
+    auto begin = [&](ChunkSlot& c) -> int {
+        c.n = std::min(U.chunk, U.total - next_first);
+        if (c.n <= 0) { c.n = 0; return PSY_OK; }
+        c.first = next_first;
+        next_first += c.n;
+        int rc = launch_render_init(R, c.first, c.n, c.B, c.st);
+        if (!rc && adj) rc = launch_adjoint_init(R, c.B, adj[&c - S], c.n, c.st);
+        if (rc) return fail(rc, "psy_render: init launch failed");
+        c.n_active = c.n;
+        c.cur = 0;
+        c.step = 0;
+        return step(c);
+    };
+    auto run = [&]() -> int {
+        for (int k = 0; k < n_slots; ++k)
+            if (int rc = begin(S[k])) return rc;
+        for (int k = 0; S[0].n > 0 || S[1].n > 0; k = (k + 1) % n_slots) {
+            ChunkSlot& c = S[k];
+            if (c.n == 0) continue;
+            if (n_slots == 2) {
+                // whichever chunk's step ends first goes on first
+                const hipError_t e = hipEventQuery(c.done);
+                if (e == hipErrorNotReady) {
+                    (void)hipGetLastError();  // not an error: the step is still running
+                    continue;
+                }
+                HIP_OK(e);
+            } else {
+                HIP_OK(hipStreamSynchronize(c.st));
+            }
+            c.n_active = *c.count_host;
+            c.cur ^= 1;
+            if (++c.step < 2 * prm->path_length && c.n_active > 0) {
+                if (int rc = step(c)) return rc;
+                continue;
+            }
+            if (film_xyz && !adj) {
+                const float* samp = sample_xyz ? sample_xyz : c.samples;
+                const int64_t sbase = sample_xyz ? 0 : c.first;
+                if (int rc = launch_render_resolve(samp, sbase, c.first / U.spp, c.n / U.spp, (int)U.spp, film_xyz, c.st))
+                    return fail(rc, "psy_render: resolve launch failed");
+            }
+            if (int rc = begin(c)) return rc;
+        }
+        return PSY_OK;
+    };
+    const int rc = run();
+    if (n_slots == 2) {
+        // the caller's stream goes on after the second stream's work, also
+        // after a failed launch
+        HIP_OK(hipEventRecord(ctx->render_join, ctx->render_stream2));
+        HIP_OK(hipStreamWaitEvent(st, ctx->render_join, 0));
+    }
+    if (rc) return rc;
     if (traced_out) *traced_out = traced;
     return PSY_OK;
 }
@@ -656,6 +762,7 @@ int render_forms(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm,
         if (int rc = alloc_render(ctx, false, 64 + U.cap + (sample_xyz ? 0 : 3 * window), U)) return rc;
         return run_stream(ctx, s, prm, R, U, window, film_xyz, sample_xyz, rays_traced, st);
     }
+    if (ctx->render_mode == PSY_RENDER_WAVEFRONT && ctx->render_streams == 2 && U.total > U.chunk) U.slots = 2;
     if (int rc = alloc_render(ctx, sample_xyz == nullptr, 0, U)) return rc;
     if (ctx->render_mode == PSY_RENDER_FUSED) return run_fused(ctx, s, prm, R, U, film_xyz, sample_xyz, rays_traced, st);
     return run_wavefront(ctx, s, prm, R, U, film_xyz, sample_xyz, nullptr, rays_traced, st);
@@ -737,9 +844,9 @@ int psy_render_autotune(psy_ctx* ctx, psy_scene* s, const psy_render_params* prm
     const int64_t total = n_pix * p.spp;
     // saved so that a failed trial leaves the context as it was
     const int s_mode = ctx->render_mode, s_tmode = ctx->mode, s_refill = ctx->refill_min, s_stack = ctx->trace_stack,
-              s_iters = ctx->render_graph_iters;
+              s_iters = ctx->render_graph_iters, s_streams = ctx->render_streams;
     const int64_t s_chunk = ctx->render_chunk;
-    struct Cfg { int mode; int64_t pool; int tmode, refill, stack, iters; };
+    struct Cfg { int mode; int64_t pool; int tmode, refill, stack, iters, streams; };
     auto apply = [&](const Cfg& c) {
         ctx->render_mode = c.mode;
         ctx->render_chunk = c.pool;
@@ -747,16 +854,23 @@ int psy_render_autotune(psy_ctx* ctx, psy_scene* s, const psy_render_params* prm
         ctx->refill_min = c.refill;
         ctx->trace_stack = c.stack;
         ctx->render_graph_iters = c.iters;
+        ctx->render_streams = c.streams;
     };
     int trials = 0;
+    const int64_t full = n_pix * prm->spp;
     auto timed = [&](const Cfg& c, double* mrays) -> int {
         apply(c);
+        // two wavefront chunks in flight are timed on at least two chunks
+        // (whole-image spp, within the full render's)
+        psy_render_params q = p;
+        if (c.mode == PSY_RENDER_WAVEFRONT && c.streams == 2)
+            q.spp = (int32_t)std::min<int64_t>(prm->spp, std::max<int64_t>(p.spp, (2 * c.pool + n_pix - 1) / n_pix));
         int64_t rays = 0;
-        int rc = psy_render(ctx, s, &p, nullptr, nullptr, &rays, nullptr);  // warm-up: buffers, code
+        int rc = psy_render(ctx, s, &q, nullptr, nullptr, &rays, nullptr);  // warm-up: buffers, code
         if (rc) return rc;
         HIP_OK(hipDeviceSynchronize());
         const auto t0 = std::chrono::steady_clock::now();
-        rc = psy_render(ctx, s, &p, nullptr, nullptr, &rays, nullptr);
+        rc = psy_render(ctx, s, &q, nullptr, nullptr, &rays, nullptr);
         if (rc) return rc;
         HIP_OK(hipDeviceSynchronize());
         const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -764,7 +878,7 @@ int psy_render_autotune(psy_ctx* ctx, psy_scene* s, const psy_render_params* prm
         ++trials;
         return PSY_OK;
     };
-    Cfg best{PSY_RENDER_STREAM, std::min<int64_t>((int64_t)1 << 21, total), PSY_TRACE_MODE_PLAIN, 16, 16, 4};
+    Cfg best{PSY_RENDER_STREAM, std::min<int64_t>((int64_t)1 << 21, total), PSY_TRACE_MODE_PLAIN, 16, 16, 4, 2};
     double best_m = 0.0;
     int rc = timed(best, &best_m);
     auto consider = [&](Cfg c) -> int {
@@ -774,11 +888,14 @@ int psy_render_autotune(psy_ctx* ctx, psy_scene* s, const psy_render_params* prm
         return PSY_OK;
     };
     // form and paths in flight: powers of two from 2^19, up to the workload
+    // (the sample for the stream form, the full render for the wavefront
+    // form's two chunks in flight)
     for (int mode : {PSY_RENDER_STREAM, PSY_RENDER_WAVEFRONT}) {
+        const int64_t upto = mode == PSY_RENDER_WAVEFRONT ? full : total;
         for (int64_t pool = (int64_t)1 << 19; !rc && pool <= ((int64_t)1 << 23); pool <<= 1) {
-            const int64_t c = std::min(pool, total);
+            const int64_t c = std::min(pool, upto);
             if (mode != best.mode || c != best.pool) { Cfg t = best; t.mode = mode; t.pool = c; rc = consider(t); }
-            if (c == total) break;
+            if (c == upto) break;
         }
     }
     if (!rc) { Cfg t = best; t.tmode = PSY_TRACE_MODE_PERSISTENT; t.refill = 32; rc = consider(t); }
@@ -786,6 +903,7 @@ int psy_render_autotune(psy_ctx* ctx, psy_scene* s, const psy_render_params* prm
     if (!rc && best.tmode == PSY_TRACE_MODE_PLAIN) { Cfg t = best; t.tmode = PSY_TRACE_MODE_PLAIN_COHERENT; rc = consider(t); }
     if (best.mode == PSY_RENDER_STREAM)
         for (int it : {2, 8, 16}) if (!rc && it != best.iters) { Cfg t = best; t.iters = it; rc = consider(t); }
+    if (!rc && best.mode == PSY_RENDER_WAVEFRONT) { Cfg t = best; t.streams = 1; rc = consider(t); }
     // dicing depth: the scene's dice cache re-built 4 and 2 levels shallower
     // and 2 deeper (within 16 GiB and the 2^32-node cap); rays that need
     // deeper splits than the cache holds go to the second pass
@@ -809,7 +927,7 @@ int psy_render_autotune(psy_ctx* ctx, psy_scene* s, const psy_render_params* prm
     }
     if (rc) {
         ctx->render_mode = s_mode; ctx->mode = s_tmode; ctx->refill_min = s_refill; ctx->trace_stack = s_stack;
-        ctx->render_graph_iters = s_iters; ctx->render_chunk = s_chunk;
+        ctx->render_graph_iters = s_iters; ctx->render_chunk = s_chunk; ctx->render_streams = s_streams;
         return rc;
     }
     apply(best);
@@ -822,6 +940,7 @@ int psy_render_autotune(psy_ctx* ctx, psy_scene* s, const psy_render_params* prm
         out->trials = trials;
         out->pool = best.pool;
         out->dice_depth = best_levels;
+        out->streams = best.streams;
         out->mrays_per_s = best_m;
     }
     return PSY_OK;
@@ -856,24 +975,32 @@ int psy_adjoint(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, 
     J.rows = std::max(1, 2 * ctx->n_cu);
     J.d_film = d_film_xyz;
     J.lparam = (const int32_t*)s->lparam_dev.p;
-    // extra scratch: adjoint path state, gradient vector, partial rows
+    // extra scratch: adjoint path state per chunk in flight, gradient
+    // vector, partial rows per chunk in flight (a block of one chunk's
+    // launch adds into the row of its index, so the two chunks' launches,
+    // which may overlap, keep rows of their own)
     RenderSetup U;
     plan_render(ctx, prm, U);
+    if (ctx->render_streams == 2 && U.total > U.chunk) U.slots = 2;
     const int64_t f_q = (int64_t)ADJ_STATE_PLANES * U.cap;
     const int64_t f_grad = (n_params + 63) / 64 * 64;
     const int64_t f_part = J.lds ? (int64_t)J.rows * n_params : 0;
-    if (int rc = alloc_render(ctx, true, f_q + f_grad + f_part, U)) return rc;
+    if (int rc = alloc_render(ctx, true, U.slots * (f_q + f_part) + f_grad, U)) return rc;
     J.q = U.extra;
-    J.grad = J.q + f_q;
+    J.grad = J.q + U.slots * f_q;
     J.partials = J.grad + f_grad;
+    AdjointBuffers JS[2] = {J, J};
+    JS[1].q = J.q + f_q;
+    JS[1].partials = J.partials + f_part;
+    J.rows *= U.slots;  // the reduction sums the rows of every chunk in flight
     if (int rc = scratch_order(ctx, st)) return rc;
-    int rc = launch_fill_f32(J.grad, f_grad + f_part, 0.0f, st);
+    int rc = launch_fill_f32(J.grad, f_grad + U.slots * f_part, 0.0f, st);
     if (rc) return fail(rc, "psy_adjoint: launch failed");
     // the replay takes the flat grey spectrum: paths do not depend on
     // colours, and every contribution is then the same at the 4 wavelengths
     RenderLaunch R = make_launch(ctx, s, prm);
     R.spec = DevSpectrum{};
-    if ((rc = run_wavefront(ctx, s, prm, R, U, nullptr, nullptr, &J, rays_traced, st))) return rc;
+    if ((rc = run_wavefront(ctx, s, prm, R, U, nullptr, nullptr, JS, rays_traced, st))) return rc;
     if ((rc = launch_adjoint_reduce(J, d_albedo, d_emission, st))) return fail(rc, "psy_adjoint: launch failed");
     return scratch_release(ctx, st);
 }

@@ -39,6 +39,7 @@ def lib():
     L.psy_ctx_retrace_count.argtypes = [vp, C.POINTER(i64)]
     L.psy_ctx_trace_pass_counts.argtypes = [vp, C.POINTER(i64), C.POINTER(i64)]
     L.psy_ctx_set_render_chunk.argtypes = [vp, i64]
+    L.psy_ctx_set_render_streams.argtypes = [vp, C.c_int]
     L.psy_ctx_set_render_mode.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.psy_scene_create.argtypes = [vp, C.POINTER(abi.SceneDesc), C.c_int, C.POINTER(vp)]
     L.psy_scene_destroy.argtypes = [vp]
@@ -508,6 +509,11 @@ def set_render_chunk(ctx, paths):
     check(lib().psy_ctx_set_render_chunk(ctx.handle, int(paths)), "psy_ctx_set_render_chunk")
 
 
+def set_render_streams(ctx, streams):
+    """psy_ctx_set_render_streams: wavefront chunks in flight (1 or 2)."""
+    check(lib().psy_ctx_set_render_streams(ctx.handle, int(streams)), "psy_ctx_set_render_streams")
+
+
 def set_render_mode(ctx, fused=False, refill_min=16, blocks_per_cu=0, stream=False):
     """psy_ctx_set_render_mode: wavefront (default), fused persistent paths,
     or the stream form (stream=True)."""
@@ -521,8 +527,8 @@ def set_stream_steps(ctx, steps):
 
 
 def autotune_render(ctx, scene, width, height, spp, seed=1, path_length=4, max_samples=0):
-    """psy_render_autotune: leaves the fastest stream-form setting on ctx and
-    returns it as a dict."""
+    """psy_render_autotune: leaves the fastest render setting on ctx (and
+    the fastest dicing depth on the scene) and returns it as a dict."""
     prm = abi.RenderParams(width, height, spp, seed, path_length, 0)
     out = abi.RenderTuning()
     check(lib().psy_render_autotune(ctx.handle, scene.handle, C.byref(prm), int(max_samples), C.byref(out)),

@@ -50,6 +50,37 @@ def test_gpu_adjoint_matches_oracle_fd(gpu_ctx, oracle):
     J.check(grad, fd, "GPU adjoint vs oracle FD (32x24x4)", say)
 
 
+def test_gpu_adjoint_two_streams_equals_one(gpu_ctx):
+    """Two chunks in flight (psy_ctx_set_render_streams) replay the same
+    paths: the same ray count and the gradients of one chunk at a time, up
+    to the order of the float sums (1e-4 relative to the largest), for 29
+    and for 1257 parameters (each chunk's blocks summing into LDS rows of
+    their own)."""
+    for n_spheres, n_lights in ((16, 12), (256, 1000)):
+        sc = configs.adjoint_scene(n_spheres=n_spheres, n_lights=n_lights)
+        g = gpu.GpuScene(gpu_ctx, sc)
+        W, H, S = 48, 40, 4
+        film, _, _ = gpu.render(gpu_ctx, g, W, H, S, 1)
+        d = torch.from_numpy(J.d_loss(film.cpu().numpy(), 0.5 * film.cpu().numpy())).cuda()
+        out = []
+        for streams in (1, 2):
+            ctx = gpu.Context(0)
+            try:
+                gpu.set_render_streams(ctx, streams)
+                gpu.set_render_chunk(ctx, 150 * S)  # 13 chunks, the last 120 pixels
+                ga, ge, rays = gpu.adjoint(ctx, g, d, S, 1)
+                out.append((np.concatenate([ga.cpu().numpy(), ge.cpu().numpy()]), rays))
+            finally:
+                ctx.close()
+        (g1, r1), (g2, r2) = out
+        say("adjoint streams 1 vs 2:", n_spheres, "spheres", n_lights, "lights, rays", r1, r2,
+            "max diff", float(np.abs(g1 - g2).max()), "scale", float(np.abs(g1).max()))
+        assert r1 == r2
+        assert np.abs(g1).max() > 0
+        assert np.abs(g1 - g2).max() <= 1e-4 * np.abs(g1).max()
+        g.close()
+
+
 def _c4(gpu_ctx, W, H, S):
     sc = configs.adjoint_scene()
     g = gpu.GpuScene(gpu_ctx, sc)

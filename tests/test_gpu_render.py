@@ -283,6 +283,42 @@ def test_gpu_render_stream_equals_wavefront(gpu_ctx, name):
         ctx.close()
 
 
+@pytest.mark.parametrize("name", ["spheres", "heightfield", "instanced_lights"])
+def test_gpu_render_two_streams_equals_one(gpu_ctx, name):
+    """Two wavefront chunks in flight (psy_ctx_set_render_streams) give the
+    samples and film of one, bit for bit, and the same ray count: for two
+    whole chunks, an even and an odd number of chunks with a short last one,
+    one chunk (which runs on one stream), samples kept or not, and on a
+    caller's stream other than the default."""
+    sc = {"spheres": configs.sphere_scene, "heightfield": lambda: configs.heightfield_scene(grid=16),
+          "instanced_lights": _instanced_lights_scene}[name]()
+    g = gpu.GpuScene(gpu_ctx, sc)
+    f_wf, s_wf, r_wf = gpu.render(gpu_ctx, g, 40, 24, 5, 3, film=True, samples=True)
+    ctx = gpu.Context(0)
+    try:
+        with pytest.raises(gpu.PsyError, match="1 or 2"):
+            gpu.set_render_streams(ctx, 3)
+        gpu.set_render_streams(ctx, 2)
+        # 960 pixels x 5 spp: 2 chunks, 4 (the last 120 pixels), 138 (the last 1), 1
+        for chunk in (480 * 5, 280 * 5, 7 * 5, 1 << 21):
+            gpu.set_render_chunk(ctx, chunk)
+            f_2, s_2, r_2 = gpu.render(ctx, g, 40, 24, 5, 3, film=True, samples=True)
+            say(name, "two streams, chunk", chunk, "rays", r_2, r_wf)
+            assert r_2 == r_wf
+            assert torch.equal(s_2, s_wf) and torch.equal(f_2, f_wf)
+            f_2, _, r_2 = gpu.render(ctx, g, 40, 24, 5, 3, film=True, samples=False)
+            assert r_2 == r_wf and torch.equal(f_2, f_wf)
+        gpu.set_render_chunk(ctx, 280 * 5)
+        side = torch.cuda.Stream()
+        with torch.cuda.stream(side):
+            f_2, _, r_2 = gpu.render(ctx, g, 40, 24, 5, 3, film=True, samples=False)
+            f_copy = f_2.clone()  # ordered on the caller's stream after the render
+        side.synchronize()
+        assert r_2 == r_wf and torch.equal(f_copy, f_wf)
+    finally:
+        ctx.close()
+
+
 def test_gpu_render_autotune(gpu_ctx):
     """psy_render_autotune leaves a setting on the context that renders the
     same image."""
@@ -298,6 +334,7 @@ def test_gpu_render_autotune(gpu_ctx):
         assert t["stack"] in (16, 32) and t["steps_per_graph"] % 2 == 0
         # the scene is left at the dicing depth the tuner picked
         assert t["dice_depth"] == g.info.dice_depth and 0 <= t["dice_depth"] <= 24
+        assert t["streams"] in (1, 2)
         f_s, _, r_s = gpu.render(ctx, g, 64, 48, 4, 1)
         assert r_s == r_wf and torch.equal(f_s, f_wf)
     finally:

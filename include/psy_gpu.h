@@ -275,13 +275,13 @@ int psy_ctx_set_continuation(psy_ctx* ctx, int below);
  * one; the default 2^21 paths.  Results do not depend on it. */
 int psy_ctx_set_render_chunk(psy_ctx* ctx, int64_t paths);
 /* Chunks of PSY_RENDER_WAVEFRONT in flight at once (no counterpart in the
- * reference): 1 or 2 (default).  With 2, a render of more than one chunk
- * runs a second chunk on a stream of the context's own, with scratch of its
- * own (a second set of path, ray and trace buffers: twice the render
- * scratch), and the host queues each chunk's next step as soon as that
- * chunk's previous one ends, so each chunk's short late bounces run beside
- * the other's work.  The caller's stream waits for both at the end.  Same
- * samples as with 1. */
+ * reference): 1..4, default 2.  With k > 1, a render of more than one chunk
+ * runs up to k chunks at once, each beyond the first on a stream of the
+ * context's own with scratch of its own (a set of path, ray and trace
+ * buffers per chunk: k times the render scratch), and the host queues each
+ * chunk's next step as soon as that chunk's previous one ends, so each
+ * chunk's short late bounces run beside the others' work.  The caller's
+ * stream waits for all of them at the end.  Same samples as with 1. */
 int psy_ctx_set_render_streams(psy_ctx* ctx, int streams);
 /* Form of psy_render (no counterpart in the reference).
  * PSY_RENDER_WAVEFRONT (default): one path per thread per step; each step
@@ -507,9 +507,9 @@ int psy_film_adaptive(psy_ctx* ctx, const float* sample_xyz, int32_t width, int3
  * LDS stack (psy_ctx_set_trace_stack), the stream form's steps per graph
  * (psy_ctx_set_stream_steps) and the scene's dicing depth
  * (psy_scene_set_dice_depth: the cache 4 and 2 levels shallower and 2
- * deeper, within 16 GiB) and, for the wavefront form, one or two chunks in
- * flight (psy_ctx_set_render_streams; two are timed on a sample of at least
- * two chunks, up to the full spp), and leaves the fastest setting on the
+ * deeper, within 16 GiB) and, for the wavefront form, 1 to 4 chunks in
+ * flight (psy_ctx_set_render_streams; k are timed on a sample of at least k
+ * chunks, up to the full spp), and leaves the fastest setting on the
  * context
  * and the scene.  `out` (or NULL) receives it. */
 typedef struct psy_render_tuning {
@@ -521,7 +521,7 @@ typedef struct psy_render_tuning {
     int32_t trials;           /* renders timed                           */
     int64_t pool;             /* paths in flight                         */
     int32_t dice_depth;       /* the scene's dice-cache depth             */
-    int32_t streams;          /* wavefront chunks in flight (1 or 2)     */
+    int32_t streams;          /* wavefront chunks in flight (1..4)       */
     double mrays_per_s;       /* of the best trial                       */
 } psy_render_tuning;
 int psy_render_autotune(psy_ctx* ctx, psy_scene* scene, const psy_render_params* params,

@@ -287,7 +287,8 @@ int psy_ctx_set_render_chunk(psy_ctx* ctx, int64_t paths) {
 
 int psy_ctx_set_render_streams(psy_ctx* ctx, int streams) {
     if (!ctx) return fail(PSY_ERR_INVALID, "psy_ctx_set_render_streams: NULL context");
-    if (streams != 1 && streams != 2) return fail(PSY_ERR_INVALID, "psy_ctx_set_render_streams: streams must be 1 or 2");
+    if (streams < 1 || streams > psy_ctx::MAX_RENDER_SLOTS)
+        return fail(PSY_ERR_INVALID, "psy_ctx_set_render_streams: streams must be in 1..4");
     ctx->render_streams = streams;
     return PSY_OK;
 }

@@ -143,7 +143,8 @@ struct psy_ctx {
         for (void* e : timing_events) (void)hipEventDestroy((hipEvent_t)e);
         if (scratch_done) (void)hipEventDestroy(scratch_done);
         if (capture_stream) (void)hipStreamDestroy(capture_stream);
-        if (render_stream2) (void)hipStreamDestroy(render_stream2);
+        for (hipStream_t x : slot_stream)
+            if (x) (void)hipStreamDestroy(x);
         if (render_join) (void)hipEventDestroy(render_join);
         for (hipEvent_t e : render_done)
             if (e) (void)hipEventDestroy(e);
@@ -159,19 +160,22 @@ struct psy_ctx {
     int render_occupancy = 0;
     int render_graph_iters = 4;      // stream mode: steps per captured graph
     hipStream_t capture_stream = nullptr;  // for graph capture (never runs work)
-    // The wavefront form's chunks in flight (1 or 2).  The second chunk runs
-    // on render_stream2 with its own scratch, trace queue and counters;
-    // render_join orders that stream after the caller's at the start of a
-    // render and the caller's after it at the end.
+    // The wavefront form's chunks in flight (1..MAX_RENDER_SLOTS).  Chunk
+    // k >= 1 runs on slot_stream[k - 1] with scratch, trace queue and
+    // counters of its own; render_join orders those streams after the
+    // caller's at the start of a render and the caller's after them at the
+    // end; render_done[k] is recorded after each of chunk k's steps.
+    static constexpr int MAX_RENDER_SLOTS = 4;
     int render_streams = 2;
-    hipStream_t render_stream2 = nullptr;
+    hipStream_t slot_stream[MAX_RENDER_SLOTS - 1] = {};
     hipEvent_t render_join = nullptr;
-    hipEvent_t render_done[2] = {nullptr, nullptr};
-    psy::DBuf render_scratch2, queue2, counters2;
-    int64_t queue2_cap = 0;
+    hipEvent_t render_done[MAX_RENDER_SLOTS] = {};
+    psy::DBuf slot_scratch[MAX_RENDER_SLOTS - 1], slot_queue[MAX_RENDER_SLOTS - 1],
+        slot_counters[MAX_RENDER_SLOTS - 1];
+    int64_t slot_queue_cap[MAX_RENDER_SLOTS - 1] = {};
     // pinned words the wavefront form reads each chunk's next active count
-    // into (word 0: first chunk, word 16: second), so that one chunk's read
-    // back does not wait for the other chunk's queued work
+    // into (word 16 k for chunk k), so that one chunk's read back does not
+    // wait for the other chunks' queued work
     uint32_t* host_counts = nullptr;
 };
 

@@ -372,16 +372,24 @@ namespace {
 
 // Chunking and scratch layout of one render call: chunks of whole pixels,
 // at most ctx->render_chunk paths; `extra` floats after the render scratch.
-// With slots == 2 (the wavefront form on two streams) B2 and
-// chunk_samples2 are the second chunk's, in the context's second scratch.
+// With slots > 1 (the wavefront form on several streams) Bx[k - 1] and
+// samples_x[k - 1] are chunk k's, in the context's scratch of slot k.
 struct RenderSetup {
     int64_t spp, n_pix, total, chunk, cap;
     int slots = 1;
-    RenderBuffers B, B2;
+    RenderBuffers B;
     float* chunk_samples;
-    float* chunk_samples2 = nullptr;
+    RenderBuffers Bx[psy_ctx::MAX_RENDER_SLOTS - 1];
+    float* samples_x[psy_ctx::MAX_RENDER_SLOTS - 1] = {};
     float* extra;
 };
+
+// Chunks in flight for a render of U's chunking on the context: one chunk
+// needs no more, nor does a form other than the wavefront.
+int render_slots(const psy_ctx* ctx, const RenderSetup& U) {
+    const int64_t chunks = (U.total + U.chunk - 1) / U.chunk;
+    return (int)std::min<int64_t>(ctx->render_streams, chunks);
+}
 
 int check_render(const psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, const char* who) {
     if (!ctx || !s || !prm) return fail(PSY_ERR_INVALID, std::string(who) + ": NULL argument");
@@ -434,7 +442,7 @@ float* carve_slot(float* base, int64_t cap, bool own_samples, RenderBuffers& B, 
 }
 
 // After plan_render: the scratch, with `extra_floats` after the render's
-// own, and the second chunk's when U.slots == 2.
+// own, and that of chunks 1..U.slots-1.
 int alloc_render(psy_ctx* ctx, bool own_samples, int64_t extra_floats, RenderSetup& U) {
     const int64_t cap = U.cap;
     if (cap > (int64_t)INT32_MAX) return fail(PSY_ERR_UNSUPPORTED, "psy_render: more than 2^31 paths per chunk");
@@ -444,15 +452,19 @@ int alloc_render(psy_ctx* ctx, bool own_samples, int64_t extra_floats, RenderSet
     if (int rc = ensure_queue(ctx, cap)) return rc;
     U.extra = carve_slot((float*)ctx->render_scratch.p, cap, own_samples, U.B, &U.chunk_samples);
     if (!ctx->host_counts)
-        HIP_OK(hipHostMalloc((void**)&ctx->host_counts, 32 * sizeof(uint32_t), hipHostMallocDefault));
-    if (U.slots == 2) {
-        if (int rc = ctx->render_scratch2.reserve((size_t)f_slot * sizeof(float))) return rc;
-        if (int rc = ensure_trace_scratch(ctx->queue2, ctx->counters2, ctx->queue2_cap, cap)) return rc;
-        if (!ctx->render_stream2) HIP_OK(hipStreamCreateWithFlags(&ctx->render_stream2, hipStreamNonBlocking));
+        HIP_OK(hipHostMalloc((void**)&ctx->host_counts, 16 * psy_ctx::MAX_RENDER_SLOTS * sizeof(uint32_t),
+                             hipHostMallocDefault));
+    if (U.slots > 1) {
         if (!ctx->render_join) HIP_OK(hipEventCreateWithFlags(&ctx->render_join, hipEventDisableTiming));
         for (hipEvent_t& e : ctx->render_done)
             if (!e) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-        carve_slot((float*)ctx->render_scratch2.p, cap, own_samples, U.B2, &U.chunk_samples2);
+    }
+    for (int k = 0; k + 1 < U.slots; ++k) {
+        if (int rc = ctx->slot_scratch[k].reserve((size_t)f_slot * sizeof(float))) return rc;
+        if (int rc = ensure_trace_scratch(ctx->slot_queue[k], ctx->slot_counters[k], ctx->slot_queue_cap[k], cap))
+            return rc;
+        if (!ctx->slot_stream[k]) HIP_OK(hipStreamCreateWithFlags(&ctx->slot_stream[k], hipStreamNonBlocking));
+        carve_slot((float*)ctx->slot_scratch[k].p, cap, own_samples, U.Bx[k], &U.samples_x[k]);
     }
     return PSY_OK;
 }
@@ -497,17 +509,17 @@ struct ChunkSlot {
 // compact the paths that go on.  With `adj` (one entry per slot), the
 // shading step is the adjoint pass's (same paths, plus gradients) and no
 // samples are written.
-// With U.slots == 2 two chunks are in flight, the second on the context's
-// second stream: the host polls both slots' step events and, for the one
-// that has ended, reads its next active count and queues its next step (or
-// its resolve and the next chunk), so each chunk's short late steps run
-// beside the other chunk's work.  Every path is computed as with one slot,
-// so the samples are the same.
+// With U.slots > 1 that many chunks are in flight, chunk k >= 1 on the
+// context's stream of slot k: the host polls the slots' step events and,
+// for each one that has ended, reads its next active count and queues its
+// next step (or its resolve and the next chunk), so each chunk's short late
+// steps run beside the other chunks' work.  Every path is computed as with
+// one slot, so the samples are the same.
 int run_wavefront(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, const RenderLaunch& R,
                   const RenderSetup& U, float* film_xyz, float* sample_xyz, const AdjointBuffers* adj,
                   int64_t* traced_out, hipStream_t st) {
     const int n_slots = U.slots;
-    ChunkSlot S[2];
+    ChunkSlot S[psy_ctx::MAX_RENDER_SLOTS];
     S[0].B = U.B;
     S[0].samples = U.chunk_samples;
     S[0].queue = (uint32_t*)ctx->queue.p;
@@ -516,18 +528,18 @@ int run_wavefront(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm
     S[0].st = st;
     S[0].count_host = ctx->host_counts;
     S[0].done = ctx->render_done[0];
-    if (n_slots == 2) {
-        S[1].B = U.B2;
-        S[1].samples = U.chunk_samples2;
-        S[1].queue = (uint32_t*)ctx->queue2.p;
-        S[1].counters = (uint32_t*)ctx->counters2.p;
-        S[1].queue_cap = ctx->queue2_cap;
-        S[1].st = ctx->render_stream2;
-        S[1].count_host = ctx->host_counts + 16;
-        S[1].done = ctx->render_done[1];
-        // the second stream starts after the caller's queued work
-        HIP_OK(hipEventRecord(ctx->render_join, st));
-        HIP_OK(hipStreamWaitEvent(ctx->render_stream2, ctx->render_join, 0));
+    if (n_slots > 1) HIP_OK(hipEventRecord(ctx->render_join, st));
+    for (int k = 1; k < n_slots; ++k) {
+        S[k].B = U.Bx[k - 1];
+        S[k].samples = U.samples_x[k - 1];
+        S[k].queue = (uint32_t*)ctx->slot_queue[k - 1].p;
+        S[k].counters = (uint32_t*)ctx->slot_counters[k - 1].p;
+        S[k].queue_cap = ctx->slot_queue_cap[k - 1];
+        S[k].st = ctx->slot_stream[k - 1];
+        S[k].count_host = ctx->host_counts + 16 * k;
+        S[k].done = ctx->render_done[k];
+        // the slot's stream starts after the caller's queued work
+        HIP_OK(hipStreamWaitEvent(S[k].st, ctx->render_join, 0));
     }
     int64_t traced = 0, next_first = 0;
     auto step = [&](ChunkSlot& c) -> int {
@@ -545,7 +557,7 @@ int run_wavefront(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm
                  : launch_render_update(R, c.first, c.B, c.cur, c.n_active, samp, sbase, c.st);
         if (rc) return fail(rc, "psy_render: update launch failed");
         HIP_OK(hipMemcpyAsync(c.count_host, c.B.count + (c.cur ^ 1), sizeof(uint32_t), hipMemcpyDeviceToHost, c.st));
-        if (n_slots == 2) HIP_OK(hipEventRecord(c.done, c.st));
+        if (n_slots > 1) HIP_OK(hipEventRecord(c.done, c.st));
         return PSY_OK;
     };
     // the slot's next chunk and its first step; no chunks left: idle
@@ -565,10 +577,15 @@ int run_wavefront(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm
     auto run = [&]() -> int {
         for (int k = 0; k < n_slots; ++k)
             if (int rc = begin(S[k])) return rc;
-        for (int k = 0; S[0].n > 0 || S[1].n > 0; k = (k + 1) % n_slots) {
+        auto busy = [&]() {
+            for (int k = 0; k < n_slots; ++k)
+                if (S[k].n > 0) return true;
+            return false;
+        };
+        for (int k = 0; busy(); k = (k + 1) % n_slots) {
             ChunkSlot& c = S[k];
             if (c.n == 0) continue;
-            if (n_slots == 2) {
+            if (n_slots > 1) {
                 // whichever chunk's step ends first goes on first
                 const hipError_t e = hipEventQuery(c.done);
                 if (e == hipErrorNotReady) {
@@ -596,10 +613,10 @@ int run_wavefront(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm
         return PSY_OK;
     };
     const int rc = run();
-    if (n_slots == 2) {
-        // the caller's stream goes on after the second stream's work, also
-        // after a failed launch
-        HIP_OK(hipEventRecord(ctx->render_join, ctx->render_stream2));
+    // the caller's stream goes on after the other slots' work, also after a
+    // failed launch (each wait takes the event as last recorded)
+    for (int k = 1; k < n_slots; ++k) {
+        HIP_OK(hipEventRecord(ctx->render_join, S[k].st));
         HIP_OK(hipStreamWaitEvent(st, ctx->render_join, 0));
     }
     if (rc) return rc;
@@ -762,7 +779,7 @@ int render_forms(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm,
         if (int rc = alloc_render(ctx, false, 64 + U.cap + (sample_xyz ? 0 : 3 * window), U)) return rc;
         return run_stream(ctx, s, prm, R, U, window, film_xyz, sample_xyz, rays_traced, st);
     }
-    if (ctx->render_mode == PSY_RENDER_WAVEFRONT && ctx->render_streams == 2 && U.total > U.chunk) U.slots = 2;
+    if (ctx->render_mode == PSY_RENDER_WAVEFRONT) U.slots = render_slots(ctx, U);
     if (int rc = alloc_render(ctx, sample_xyz == nullptr, 0, U)) return rc;
     if (ctx->render_mode == PSY_RENDER_FUSED) return run_fused(ctx, s, prm, R, U, film_xyz, sample_xyz, rays_traced, st);
     return run_wavefront(ctx, s, prm, R, U, film_xyz, sample_xyz, nullptr, rays_traced, st);
@@ -860,11 +877,12 @@ int psy_render_autotune(psy_ctx* ctx, psy_scene* s, const psy_render_params* prm
     const int64_t full = n_pix * prm->spp;
     auto timed = [&](const Cfg& c, double* mrays) -> int {
         apply(c);
-        // two wavefront chunks in flight are timed on at least two chunks
+        // k wavefront chunks in flight are timed on at least k chunks
         // (whole-image spp, within the full render's)
         psy_render_params q = p;
-        if (c.mode == PSY_RENDER_WAVEFRONT && c.streams == 2)
-            q.spp = (int32_t)std::min<int64_t>(prm->spp, std::max<int64_t>(p.spp, (2 * c.pool + n_pix - 1) / n_pix));
+        if (c.mode == PSY_RENDER_WAVEFRONT && c.streams > 1)
+            q.spp = (int32_t)std::min<int64_t>(
+                prm->spp, std::max<int64_t>(p.spp, (c.streams * c.pool + n_pix - 1) / n_pix));
         int64_t rays = 0;
         int rc = psy_render(ctx, s, &q, nullptr, nullptr, &rays, nullptr);  // warm-up: buffers, code
         if (rc) return rc;
@@ -903,7 +921,15 @@ int psy_render_autotune(psy_ctx* ctx, psy_scene* s, const psy_render_params* prm
     if (!rc && best.tmode == PSY_TRACE_MODE_PLAIN) { Cfg t = best; t.tmode = PSY_TRACE_MODE_PLAIN_COHERENT; rc = consider(t); }
     if (best.mode == PSY_RENDER_STREAM)
         for (int it : {2, 8, 16}) if (!rc && it != best.iters) { Cfg t = best; t.iters = it; rc = consider(t); }
-    if (!rc && best.mode == PSY_RENDER_WAVEFRONT) { Cfg t = best; t.streams = 1; rc = consider(t); }
+    // wavefront chunks in flight: 1, 3 and 4 against the search's 2, then
+    // the best count at half the chunk
+    if (best.mode == PSY_RENDER_WAVEFRONT) {
+        for (int k : {1, 3, 4})
+            if (!rc && k != best.streams) { Cfg t = best; t.streams = k; rc = consider(t); }
+        if (!rc && best.streams > 1 && best.pool / 2 >= ((int64_t)1 << 19)) {
+            Cfg t = best; t.pool = best.pool / 2; rc = consider(t);
+        }
+    }
     // dicing depth: the scene's dice cache re-built 4 and 2 levels shallower
     // and 2 deeper (within 16 GiB and the 2^32-node cap); rays that need
     // deeper splits than the cache holds go to the second pass
@@ -981,7 +1007,7 @@ int psy_adjoint(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, 
     // which may overlap, keep rows of their own)
     RenderSetup U;
     plan_render(ctx, prm, U);
-    if (ctx->render_streams == 2 && U.total > U.chunk) U.slots = 2;
+    U.slots = render_slots(ctx, U);
     const int64_t f_q = (int64_t)ADJ_STATE_PLANES * U.cap;
     const int64_t f_grad = (n_params + 63) / 64 * 64;
     const int64_t f_part = J.lds ? (int64_t)J.rows * n_params : 0;
@@ -989,9 +1015,12 @@ int psy_adjoint(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm, 
     J.q = U.extra;
     J.grad = J.q + U.slots * f_q;
     J.partials = J.grad + f_grad;
-    AdjointBuffers JS[2] = {J, J};
-    JS[1].q = J.q + f_q;
-    JS[1].partials = J.partials + f_part;
+    AdjointBuffers JS[psy_ctx::MAX_RENDER_SLOTS];
+    for (int k = 0; k < U.slots; ++k) {
+        JS[k] = J;
+        JS[k].q = J.q + k * f_q;
+        JS[k].partials = J.partials + k * f_part;
+    }
     J.rows *= U.slots;  // the reduction sums the rows of every chunk in flight
     if (int rc = scratch_order(ctx, st)) return rc;
     int rc = launch_fill_f32(J.grad, f_grad + U.slots * f_part, 0.0f, st);

@@ -1342,6 +1342,9 @@ template <bool WORLD, int STACK>
 PSY_DEV bool lane_run(const DevScene& s, Lane& L, int32_t (*stk_ref)[TRACE_BLOCK],
                       float (*stk_near)[TRACE_BLOCK], int tid, const DeepPool& pool, const ContBuf& out) {
     int below = out.below;  // wave-uniform
+#if !defined(__HIP_DEVICE_COMPILE__)
+    (void)below;  // the host pass has no wave intrinsics
+#endif
     for (;;) {
         if (lane_step<WORLD, STACK>(s, L, stk_ref, stk_near, tid, pool)) return false;
 #if defined(__HIP_DEVICE_COMPILE__)

@@ -510,7 +510,7 @@ def set_render_chunk(ctx, paths):
 
 
 def set_render_streams(ctx, streams):
-    """psy_ctx_set_render_streams: wavefront chunks in flight (1 or 2)."""
+    """psy_ctx_set_render_streams: wavefront chunks in flight (1..4)."""
     check(lib().psy_ctx_set_render_streams(ctx.handle, int(streams)), "psy_ctx_set_render_streams")
 
 

@@ -284,10 +284,10 @@ def test_gpu_render_stream_equals_wavefront(gpu_ctx, name):
 
 
 @pytest.mark.parametrize("name", ["spheres", "heightfield", "instanced_lights"])
-def test_gpu_render_two_streams_equals_one(gpu_ctx, name):
-    """Two wavefront chunks in flight (psy_ctx_set_render_streams) give the
-    samples and film of one, bit for bit, and the same ray count: for two
-    whole chunks, an even and an odd number of chunks with a short last one,
+def test_gpu_render_chunks_in_flight_equal_one(gpu_ctx, name):
+    """Two to four wavefront chunks in flight (psy_ctx_set_render_streams)
+    give the samples and film of one, bit for bit, and the same ray count:
+    for as many chunks as streams, more chunks with a short last one, and
     one chunk (which runs on one stream), samples kept or not, and on a
     caller's stream other than the default."""
     sc = {"spheres": configs.sphere_scene, "heightfield": lambda: configs.heightfield_scene(grid=16),
@@ -296,18 +296,21 @@ def test_gpu_render_two_streams_equals_one(gpu_ctx, name):
     f_wf, s_wf, r_wf = gpu.render(gpu_ctx, g, 40, 24, 5, 3, film=True, samples=True)
     ctx = gpu.Context(0)
     try:
-        with pytest.raises(gpu.PsyError, match="1 or 2"):
-            gpu.set_render_streams(ctx, 3)
-        gpu.set_render_streams(ctx, 2)
-        # 960 pixels x 5 spp: 2 chunks, 4 (the last 120 pixels), 138 (the last 1), 1
-        for chunk in (480 * 5, 280 * 5, 7 * 5, 1 << 21):
-            gpu.set_render_chunk(ctx, chunk)
-            f_2, s_2, r_2 = gpu.render(ctx, g, 40, 24, 5, 3, film=True, samples=True)
-            say(name, "two streams, chunk", chunk, "rays", r_2, r_wf)
-            assert r_2 == r_wf
-            assert torch.equal(s_2, s_wf) and torch.equal(f_2, f_wf)
-            f_2, _, r_2 = gpu.render(ctx, g, 40, 24, 5, 3, film=True, samples=False)
-            assert r_2 == r_wf and torch.equal(f_2, f_wf)
+        for bad in (0, 5):
+            with pytest.raises(gpu.PsyError, match="1..4"):
+                gpu.set_render_streams(ctx, bad)
+        # 960 pixels x 5 spp: 2 chunks, 3 (the last 160 pixels), 4 (the last
+        # 120), 138 (the last 1), 1
+        for streams in (2, 3, 4):
+            gpu.set_render_streams(ctx, streams)
+            for chunk in (480 * 5, 400 * 5, 280 * 5, 7 * 5, 1 << 21):
+                gpu.set_render_chunk(ctx, chunk)
+                f_2, s_2, r_2 = gpu.render(ctx, g, 40, 24, 5, 3, film=True, samples=True)
+                say(name, streams, "streams, chunk", chunk, "rays", r_2, r_wf)
+                assert r_2 == r_wf
+                assert torch.equal(s_2, s_wf) and torch.equal(f_2, f_wf)
+                f_2, _, r_2 = gpu.render(ctx, g, 40, 24, 5, 3, film=True, samples=False)
+                assert r_2 == r_wf and torch.equal(f_2, f_wf)
         gpu.set_render_chunk(ctx, 280 * 5)
         side = torch.cuda.Stream()
         with torch.cuda.stream(side):
@@ -334,7 +337,7 @@ def test_gpu_render_autotune(gpu_ctx):
         assert t["stack"] in (16, 32) and t["steps_per_graph"] % 2 == 0
         # the scene is left at the dicing depth the tuner picked
         assert t["dice_depth"] == g.info.dice_depth and 0 <= t["dice_depth"] <= 24
-        assert t["streams"] in (1, 2)
+        assert 1 <= t["streams"] <= 4
         f_s, _, r_s = gpu.render(ctx, g, 64, 48, 4, 1)
         assert r_s == r_wf and torch.equal(f_s, f_wf)
     finally:

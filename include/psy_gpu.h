@@ -509,8 +509,8 @@ int psy_film_adaptive(psy_ctx* ctx, const float* sample_xyz, int32_t width, int3
  * (psy_scene_set_dice_depth: the cache 4 and 2 levels shallower and 2
  * deeper, within 16 GiB) and, for the wavefront form, 1 to 4 chunks in
  * flight (psy_ctx_set_render_streams; k are timed on a sample of at least k
- * chunks, up to the full spp), and leaves the fastest setting on the
- * context
+ * chunks, up to the full spp) and the whole image as 2 to 4 chunks all in
+ * flight, and leaves the fastest setting on the context
  * and the scene.  `out` (or NULL) receives it. */
 typedef struct psy_render_tuning {
     int32_t mode;             /* PSY_RENDER_STREAM or PSY_RENDER_WAVEFRONT */

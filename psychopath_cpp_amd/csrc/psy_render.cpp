@@ -887,12 +887,18 @@ int psy_render_autotune(psy_ctx* ctx, psy_scene* s, const psy_render_params* prm
         int rc = psy_render(ctx, s, &q, nullptr, nullptr, &rays, nullptr);  // warm-up: buffers, code
         if (rc) return rc;
         HIP_OK(hipDeviceSynchronize());
-        const auto t0 = std::chrono::steady_clock::now();
-        rc = psy_render(ctx, s, &q, nullptr, nullptr, &rays, nullptr);
-        if (rc) return rc;
-        HIP_OK(hipDeviceSynchronize());
-        const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        *mrays = (double)rays / std::max(dt, 1e-9) / 1e6;
+        // the fastest of up to 3 timed renders, stopping once 30 ms are spent
+        double fastest = 1e30, spent = 0.0;
+        for (int rep = 0; rep < 3 && (rep == 0 || spent < 0.03); ++rep) {
+            const auto t0 = std::chrono::steady_clock::now();
+            rc = psy_render(ctx, s, &q, nullptr, nullptr, &rays, nullptr);
+            if (rc) return rc;
+            HIP_OK(hipDeviceSynchronize());
+            const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            fastest = std::min(fastest, dt);
+            spent += dt;
+        }
+        *mrays = (double)rays / std::max(fastest, 1e-9) / 1e6;
         ++trials;
         return PSY_OK;
     };
@@ -922,12 +928,20 @@ int psy_render_autotune(psy_ctx* ctx, psy_scene* s, const psy_render_params* prm
     if (best.mode == PSY_RENDER_STREAM)
         for (int it : {2, 8, 16}) if (!rc && it != best.iters) { Cfg t = best; t.iters = it; rc = consider(t); }
     // wavefront chunks in flight: 1, 3 and 4 against the search's 2, then
-    // the best count at half the chunk
+    // the best count at half the chunk, then the whole image as k chunks all
+    // in flight (k = 2..4, chunks within the search's 2^23 paths; such a
+    // trial renders the whole image)
     if (best.mode == PSY_RENDER_WAVEFRONT) {
         for (int k : {1, 3, 4})
             if (!rc && k != best.streams) { Cfg t = best; t.streams = k; rc = consider(t); }
         if (!rc && best.streams > 1 && best.pool / 2 >= ((int64_t)1 << 19)) {
             Cfg t = best; t.pool = best.pool / 2; rc = consider(t);
+        }
+        for (int k = 2; k <= psy_ctx::MAX_RENDER_SLOTS && !rc; ++k) {
+            const int64_t pool = (n_pix + k - 1) / k * prm->spp;
+            if (pool > ((int64_t)1 << 23) || pool >= full) continue;
+            if (pool == best.pool && k == best.streams) continue;
+            Cfg t = best; t.pool = pool; t.streams = k; rc = consider(t);
         }
     }
     // dicing depth: the scene's dice cache re-built 4 and 2 levels shallower

@@ -425,6 +425,8 @@ __device__ __forceinline__ void compact_push(bool alive, uint32_t i, const WRay&
     }
 }
 
+// PAR: the step parity of every path in the stream (0 or 1), or -1 (mixed).
+template <int PAR>
 __global__ __launch_bounds__(RENDER_BLOCK) void k_render_update(RenderLaunch A, int64_t first, PathBuf P,
                                                                 const uint32_t* __restrict__ act_in,
                                                                 const float* __restrict__ rays_in, DevHits hits,
@@ -439,7 +441,7 @@ __global__ __launch_bounds__(RENDER_BLOCK) void k_render_update(RenderLaunch A, 
     WRay nr;
     if (j < n_in) {
         i = act_in[j];
-        alive = render_update_one(A, first, P, i, j, rays_in, hits, rcap, sample_xyz, sbase, nr);
+        alive = render_update_one<NoAdjoint, PAR>(A, first, P, i, j, rays_in, hits, rcap, sample_xyz, sbase, nr);
     }
     compact_push(alive, i, nr, act_out, rays_out, rcap, count_out);
 }
@@ -929,14 +931,19 @@ int launch_render_init(const RenderLaunch& R, int64_t first, int64_t n, const Re
 }
 
 int launch_render_update(const RenderLaunch& R, int64_t first, const RenderBuffers& B, int cur, int64_t n_in,
-                         float* sample_xyz, int64_t sbase, hipStream_t st) {
+                         int step, float* sample_xyz, int64_t sbase, hipStream_t st) {
     if (n_in <= 0) return 0;
     PathBuf P{B.paths, B.cap};
     const int nxt = cur ^ 1;
-    hipLaunchKernelGGL(k_render_update, dim3((unsigned)((n_in + RENDER_BLOCK - 1) / RENDER_BLOCK)),
-                       dim3(RENDER_BLOCK), 0, st, R, first, P, (const uint32_t*)B.act[cur],
-                       (const float*)B.rays[cur], B.hits, n_in, B.cap, B.act[nxt], B.rays[nxt], B.count + nxt,
-                       sample_xyz, sbase);
+    const dim3 grid((unsigned)((n_in + RENDER_BLOCK - 1) / RENDER_BLOCK));
+    if (step & 1)
+        hipLaunchKernelGGL(k_render_update<1>, grid, dim3(RENDER_BLOCK), 0, st, R, first, P,
+                           (const uint32_t*)B.act[cur], (const float*)B.rays[cur], B.hits, n_in, B.cap, B.act[nxt],
+                           B.rays[nxt], B.count + nxt, sample_xyz, sbase);
+    else
+        hipLaunchKernelGGL(k_render_update<0>, grid, dim3(RENDER_BLOCK), 0, st, R, first, P,
+                           (const uint32_t*)B.act[cur], (const float*)B.rays[cur], B.hits, n_in, B.cap, B.act[nxt],
+                           B.rays[nxt], B.count + nxt, sample_xyz, sbase);
     return hipGetLastError() == hipSuccess ? 0 : PSY_ERR_HIP;
 }
 
@@ -1012,9 +1019,9 @@ int launch_render_update_io(const RenderLaunch& R, int64_t first, float* paths, 
                             float* rays_out, uint32_t* count_out, float* sample_xyz, int64_t sbase, hipStream_t st) {
     if (n_in <= 0) return 0;
     PathBuf P{paths, cap};
-    hipLaunchKernelGGL(k_render_update, dim3((unsigned)((n_in + RENDER_BLOCK - 1) / RENDER_BLOCK)), dim3(RENDER_BLOCK),
-                       0, st, R, first, P, act_in, rays_in, hits, n_in, cap, act_out, rays_out, count_out, sample_xyz,
-                       sbase);
+    hipLaunchKernelGGL(k_render_update<-1>, dim3((unsigned)((n_in + RENDER_BLOCK - 1) / RENDER_BLOCK)),
+                       dim3(RENDER_BLOCK), 0, st, R, first, P, act_in, rays_in, hits, n_in, cap, act_out, rays_out,
+                       count_out, sample_xyz, sbase);
     return hipGetLastError() == hipSuccess ? 0 : PSY_ERR_HIP;
 }
 

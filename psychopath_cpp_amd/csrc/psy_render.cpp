@@ -554,7 +554,7 @@ int run_wavefront(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm
         float* samp = sample_xyz ? sample_xyz : c.samples;
         const int64_t sbase = sample_xyz ? 0 : c.first;
         rc = adj ? launch_render_adjoint(R, c.first, c.B, adj[&c - S], c.cur, c.n_active, c.st)
-                 : launch_render_update(R, c.first, c.B, c.cur, c.n_active, samp, sbase, c.st);
+                 : launch_render_update(R, c.first, c.B, c.cur, c.n_active, c.step, samp, sbase, c.st);
         if (rc) return fail(rc, "psy_render: update launch failed");
         HIP_OK(hipMemcpyAsync(c.count_host, c.B.count + (c.cur ^ 1), sizeof(uint32_t), hipMemcpyDeviceToHost, c.st));
         if (n_slots > 1) HIP_OK(hipEventRecord(c.done, c.st));

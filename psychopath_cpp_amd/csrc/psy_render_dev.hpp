@@ -1060,7 +1060,10 @@ PSY_DEV int64_t path_sample(const RenderLaunch& A, const PathBuf& P, uint32_t i)
 // update_path for the ray just traced (slot j of the stream) of path i, then
 // next_ray_for_path if the path goes on (returns true, the ray in `nr`);
 // a finished path writes its film contribution to sample_xyz.
-template <class Adj>
+// PAR: the parity of every path's step when the launch fixes it (0 or 1: a
+// wavefront step, where all paths of the stream are at the same step, so
+// each launch compiles only its half), else -1 (read per path).
+template <class Adj, int PAR = -1>
 PSY_DEV bool render_update_path(const RenderLaunch& A, int64_t first, const PathBuf& P, uint32_t i, const WRay& ray,
                                 const HitRec& hr, float* sample_xyz, int64_t sbase, WRay& nr, const Adj& adj) {
     bool alive = false;
@@ -1072,13 +1075,14 @@ PSY_DEV bool render_update_path(const RenderLaunch& A, int64_t first, const Path
         const DevScene& s = A.s;
         uint32_t stepw = P.u(P_STEP, i);
         const int step = (int)(stepw & 0xffu);
+        const bool odd = PAR < 0 ? (step & 1) != 0 : PAR == 1;
         bool done = false;
         const uint32_t hinst = hr.inst;
         const bool hit = hinst != 0xffffffffu;
         const V3 rd = ray.d;
         float col[NSPEC], fcol[NSPEC];
         for (int k = 0; k < NSPEC; ++k) { col[k] = P.f(P_COL + k, i); fcol[k] = P.f(P_FCOL + k, i); }
-        if (step & 1) {
+        if (odd) {
             // result of a shadow ray: unoccluded -> add the light sample
             // (the adjoint takes Lambert and Emit shaders only: its scene has no GTR)
             const DevShader SH = s.shaders[P.u(P_SHADER, i)];
@@ -1177,6 +1181,7 @@ PSY_DEV bool render_update_path(const RenderLaunch& A, int64_t first, const Path
                 adj_contrib(s, adj, i, step >> 1, adj.Q.f(Q_W, i) * (adj.Q.f(Q_B, i) * bg[0]), -1);
         }
         const int nstep = step + 1;
+        const bool next_odd = !odd;  // nstep & 1
         if (nstep == A.path_length * 2) done = true;
 
         if (!done) {
@@ -1190,7 +1195,7 @@ PSY_DEV bool render_update_path(const RenderLaunch& A, int64_t first, const Path
             const float* alb = SH.color;
             const bool gtr = !Adj::on && SH.kind == PSY_SHADER_GTR;
             const bool delta = gtr && SH.roughness == 0.0f;  // GTRClosure::is_delta
-            if ((nstep & 1) && delta) {
+            if (next_odd && delta) {
                 // a delta closure takes no light sample (and no sampler
                 // dimensions): a shadow ray of zero light, whose result is
                 // not used
@@ -1202,7 +1207,7 @@ PSY_DEV bool render_update_path(const RenderLaunch& A, int64_t first, const Path
                 no = add(gp, po);
                 nd = mk(1e-20f, 1e-20f, 1e-20f);
                 ntype = PSY_RAY_OCCLUSION;
-            } else if (nstep & 1) {
+            } else if (next_odd) {
                 // shadow ray towards a light sample
                 V3 lq_nor = gn;
                 if (stepw & STEP_BACKFACING) lq_nor = scale(lq_nor, -1.0f);
@@ -1294,7 +1299,7 @@ PSY_DEV bool render_update_path(const RenderLaunch& A, int64_t first, const Path
             if (gtr) {
                 // GTRClosure::propagate_differentials (a delta closure's
                 // zero-light shadow ray keeps zero differentials)
-                if (!((nstep & 1) && delta))
+                if (!(next_odd && delta))
                     gtr_differentials(ht, pd, P.v3(P_PODX, i), P.v3(P_PODY, i), P.v3(P_PDDX, i), P.v3(P_PDDY, i), gn,
                                       P.v3(P_DPDU, i), P.v3(P_DPDV, i), P.v3(P_DNDU, i), P.v3(P_DNDV, i), nd, &nodx,
                                       &nody, &nddx, &nddy);
@@ -1361,7 +1366,7 @@ PSY_DEV void lane_wray(const Lane& L, uint32_t type, WRay& w) {
 }
 
 // render_update_path for slot j of a ray / hit stream.
-template <class Adj = NoAdjoint>
+template <class Adj = NoAdjoint, int PAR = -1>
 PSY_DEV bool render_update_one(const RenderLaunch& A, int64_t first, const PathBuf& P, uint32_t i, int64_t j,
                                const float* rays_in, const DevHits& hits, int64_t rcap, float* sample_xyz,
                                int64_t sbase, WRay& nr, const Adj& adj = Adj()) {
@@ -1373,7 +1378,7 @@ PSY_DEV bool render_update_one(const RenderLaunch& A, int64_t first, const PathB
     hr.u = hits.u[j];
     hr.v = hits.v[j];
     hr.aux = hits.aux ? hits.aux[j] : 0.0f;
-    return render_update_path(A, first, P, i, ray, hr, sample_xyz, sbase, nr, adj);
+    return render_update_path<Adj, PAR>(A, first, P, i, ray, hr, sample_xyz, sbase, nr, adj);
 }
 
 // The adjoint state of a new path: w = dLoss/dC from its pixel's film

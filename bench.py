@@ -197,7 +197,15 @@ def batching_forms():
         tune_s = time.perf_counter() - t0
         tuned, f_tuned = _render_rate(gpu, ctx, g, W, H, spp, pl)
         log(f"batching: autotuned {tune} in {tune_s:.2f} s: {tuned:.1f} Mrays/s")
-        same = bool(torch.equal(f_plain, f_tuned) and torch.equal(f_plain, f_wide) and torch.equal(f_plain, f_dflt))
+        # forms give identical images at one dicing depth; a re-diced scene
+        # (the tuner's depth) can differ where two micro-patches tie in t
+        f_ref = f_plain
+        if int(tune["dice_depth"]) != d0:
+            ref_ctx = gpu.Context(0)
+            f_ref, _, _ = gpu.render(ref_ctx, g, W, H, spp, 1, pl)  # library defaults, the tuned depth
+            ref_ctx.close()
+        same = bool(torch.equal(f_ref, f_tuned) and torch.equal(f_plain, f_wide) and torch.equal(f_plain, f_dflt))
+        depth_rel = float(((f_tuned - f_plain).abs() / (f_plain.abs() + 1e-6)).max())
         out.append({"workload": name, "plain_mrays": round(plain, 2),
                     "plain_form": f"wavefront, {REF_BUCKET_PATHS}-path buckets, host sync per step",
                     "default_wavefront_mrays": round(dflt, 2),
@@ -208,12 +216,13 @@ def batching_forms():
                                   "trace": TRACE_MODES.get(int(tune["trace_mode"]), str(tune["trace_mode"])),
                                   "stack": int(tune["stack"]), "steps_per_graph": int(tune["steps_per_graph"]),
                                   "dice_depth": int(tune["dice_depth"]), "scene_dice_depth": d0,
-                                  "streams": int(tune["streams"]),
+                                  "streams": int(tune["streams"]), "shade_waves": int(tune["shade_waves"]),
                                   "tune_seconds": round(tune_s, 2)},
                     "speedup_vs_plain": round(tuned / plain, 2),
                     "speedup_vs_default_wavefront": round(tuned / dflt, 2),
                     "speedup_vs_2p22_wavefront": round(tuned / wide, 2),
-                    "images_identical": same, "hero_wavelength_nm": HERO_WAVELENGTH_NM})
+                    "images_identical": same, "film_max_rel_vs_scene_depth": depth_rel,
+                    "hero_wavelength_nm": HERO_WAVELENGTH_NM})
         g.close()
         ctx.close()
     return out
@@ -250,7 +259,8 @@ def full_config_renders():
                                   "paths": int(tune["pool"]),
                                   "trace": TRACE_MODES.get(int(tune["trace_mode"]), str(tune["trace_mode"])),
                                   "stack": int(tune["stack"]), "dice_depth": int(tune["dice_depth"]),
-                                  "scene_dice_depth": d0, "streams": int(tune["streams"])},
+                                  "scene_dice_depth": d0, "streams": int(tune["streams"]),
+                                  "shade_waves": int(tune["shade_waves"])},
                     "tune_seconds": round(tune_s, 2),
                     "film_mean_xyz": [round(float(v), 6) for v in film.mean(dim=(0, 1)).cpu().tolist()]})
         g.close()

@@ -283,6 +283,13 @@ int psy_ctx_set_render_chunk(psy_ctx* ctx, int64_t paths);
  * chunk's short late bounces run beside the others' work.  The caller's
  * stream waits for all of them at the end.  Same samples as with 1. */
 int psy_ctx_set_render_streams(psy_ctx* ctx, int streams);
+/* Register budget of PSY_RENDER_WAVEFRONT's shading kernel for the steps
+ * that take a hit and sample a light (no counterpart in the reference):
+ * 0 (default) lets the compiler use what the light sampling needs (2 waves
+ * per SIMD); 4 holds it to 128 VGPRs for 4 waves per SIMD, spilling the
+ * rest to scratch, which pays where the light tree is deep (many lights)
+ * and costs elsewhere.  Same samples either way. */
+int psy_ctx_set_render_shade_waves(psy_ctx* ctx, int waves);
 /* Form of psy_render (no counterpart in the reference).
  * PSY_RENDER_WAVEFRONT (default): one path per thread per step; each step
  * traces the whole active stream, then shades it and compacts the paths
@@ -349,9 +356,13 @@ int psy_scene_instance_ids(const psy_scene* scene, uint64_t* out, int64_t n);
 /* Re-dices the scene: rebuilds its dice cache at dice_depth (as in
  * psy_scene_create; < 0 the memory budget's depth), beside the old one
  * (which is freed after), and waits for the device; on failure the scene is
- * unchanged.  Hits and renders do not change: a split deeper than the cache is
- * diced by the second pass.  No counterpart in the reference, which splits
- * per ray batch; psy_render_autotune uses it to pick the depth. */
+ * unchanged.  A split deeper than the cache is diced by the second pass, so
+ * every hit keeps its distance and instance; only where two micro-patches
+ * give exactly the same distance can the depth change which one supplies
+ * (u, v) (the first found is kept, and the second pass finds a deferred
+ * subtree after the rest of the walk).  No counterpart in the reference,
+ * which splits per ray batch; psy_render_autotune uses it to pick the
+ * depth. */
 int psy_scene_set_dice_depth(psy_scene* scene, int dice_depth);
 
 /* Replaces: SubdivisionSurface::finalize (object/subdivision_surface.cpp:
@@ -509,8 +520,10 @@ int psy_film_adaptive(psy_ctx* ctx, const float* sample_xyz, int32_t width, int3
  * (psy_scene_set_dice_depth: the cache 4 and 2 levels shallower and 2
  * deeper, within 16 GiB) and, for the wavefront form, 1 to 4 chunks in
  * flight (psy_ctx_set_render_streams; k are timed on a sample of at least k
- * chunks, up to the full spp) and the whole image as 2 to 4 chunks all in
- * flight, and leaves the fastest setting on the context
+ * chunks, up to the full spp), the whole image as 2 to 4 chunks all in
+ * flight and the shading kernel's register budget
+ * (psy_ctx_set_render_shade_waves), and leaves the fastest setting on the
+ * context
  * and the scene.  `out` (or NULL) receives it. */
 typedef struct psy_render_tuning {
     int32_t mode;             /* PSY_RENDER_STREAM or PSY_RENDER_WAVEFRONT */
@@ -522,6 +535,7 @@ typedef struct psy_render_tuning {
     int64_t pool;             /* paths in flight                         */
     int32_t dice_depth;       /* the scene's dice-cache depth             */
     int32_t streams;          /* wavefront chunks in flight (1..4)       */
+    int32_t shade_waves;      /* psy_ctx_set_render_shade_waves          */
     double mrays_per_s;       /* of the best trial                       */
 } psy_render_tuning;
 int psy_render_autotune(psy_ctx* ctx, psy_scene* scene, const psy_render_params* params,

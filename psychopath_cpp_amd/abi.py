@@ -109,7 +109,8 @@ class RenderParams(C.Structure):
 class RenderTuning(C.Structure):
     _fields_ = [("mode", C.c_int32), ("trace_mode", C.c_int32), ("refill_min", C.c_int32), ("stack", C.c_int32),
                 ("steps_per_graph", C.c_int32), ("trials", C.c_int32), ("pool", C.c_int64),
-                ("dice_depth", C.c_int32), ("streams", C.c_int32), ("mrays_per_s", C.c_double)]
+                ("dice_depth", C.c_int32), ("streams", C.c_int32),
+                ("shade_waves", C.c_int32), ("mrays_per_s", C.c_double)]
 
 
 class SceneInfo(C.Structure):
@@ -120,7 +121,7 @@ class SceneInfo(C.Structure):
 
 # Every symbol the C ABI exports (include/psy_gpu.h).
 EXPORTED_SYMBOLS = [
-    "psy_ctx_create", "psy_ctx_destroy", "psy_ctx_set_trace_mode", "psy_ctx_retrace_count", "psy_ctx_trace_pass_counts", "psy_ctx_set_render_chunk", "psy_ctx_set_render_streams", "psy_ctx_set_render_mode", "psy_ctx_set_trace_timing", "psy_ctx_set_trace_stack", "psy_ctx_set_second_pass", "psy_ctx_set_continuation", "psy_ctx_set_stream_steps", "psy_render_autotune", "psy_render_adaptive", "psy_film_adaptive", "psy_bucket_size", "psy_ctx_trace_timing", "psy_last_error", "psy_abi_version",
+    "psy_ctx_create", "psy_ctx_destroy", "psy_ctx_set_trace_mode", "psy_ctx_retrace_count", "psy_ctx_trace_pass_counts", "psy_ctx_set_render_chunk", "psy_ctx_set_render_streams", "psy_ctx_set_render_shade_waves", "psy_ctx_set_render_mode", "psy_ctx_set_trace_timing", "psy_ctx_set_trace_stack", "psy_ctx_set_second_pass", "psy_ctx_set_continuation", "psy_ctx_set_stream_steps", "psy_render_autotune", "psy_render_adaptive", "psy_film_adaptive", "psy_bucket_size", "psy_ctx_trace_timing", "psy_last_error", "psy_abi_version",
     "psy_malloc", "psy_free", "psy_copy_to_device", "psy_copy_to_host",
     "psy_scene_create", "psy_scene_destroy", "psy_scene_get_info", "psy_scene_instance_ids", "psy_scene_set_dice_depth", "psy_subdivision_patches",
     "psy_trace_world", "psy_trace_rays", "psy_trace_rays_static", "psy_intersections", "psy_world_to_rays",

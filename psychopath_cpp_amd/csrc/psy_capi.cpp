@@ -285,6 +285,13 @@ int psy_ctx_set_render_chunk(psy_ctx* ctx, int64_t paths) {
     return PSY_OK;
 }
 
+int psy_ctx_set_render_shade_waves(psy_ctx* ctx, int waves) {
+    if (!ctx) return fail(PSY_ERR_INVALID, "psy_ctx_set_render_shade_waves: NULL context");
+    if (waves != 0 && waves != 4) return fail(PSY_ERR_INVALID, "psy_ctx_set_render_shade_waves: waves must be 0 or 4");
+    ctx->render_shade_waves = waves;
+    return PSY_OK;
+}
+
 int psy_ctx_set_render_streams(psy_ctx* ctx, int streams) {
     if (!ctx) return fail(PSY_ERR_INVALID, "psy_ctx_set_render_streams: NULL context");
     if (streams < 1 || streams > psy_ctx::MAX_RENDER_SLOTS)

@@ -230,8 +230,9 @@ struct RenderBuffers {
 int launch_render_init(const RenderLaunch& R, int64_t first, int64_t n, const RenderBuffers& B, hipStream_t st);
 // Finished paths write their film XYZ to sample_xyz[3 * (sample - sbase)].
 // Every path of the stream is at step `step` (the wavefront loop's).
+// shade_waves 4: an even step's kernel is the one held to 4 waves per SIMD.
 int launch_render_update(const RenderLaunch& R, int64_t first, const RenderBuffers& B, int cur, int64_t n_in,
-                         int step, float* sample_xyz, int64_t sbase, hipStream_t st);
+                         int step, int shade_waves, float* sample_xyz, int64_t sbase, hipStream_t st);
 // Stream form: device-side state of a pool of path slots.
 struct StreamState {
     uint32_t count[2];         // ping-pong stream lengths

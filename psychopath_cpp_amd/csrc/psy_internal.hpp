@@ -159,6 +159,7 @@ struct psy_ctx {
     int render_blocks_per_cu = 0;    // fused mode: <= 0 from occupancy
     int render_occupancy = 0;
     int render_graph_iters = 4;      // stream mode: steps per captured graph
+    int render_shade_waves = 0;      // wavefront hit-step shading: 0 compiler's registers, 4 held to 4 waves
     hipStream_t capture_stream = nullptr;  // for graph capture (never runs work)
     // The wavefront form's chunks in flight (1..MAX_RENDER_SLOTS).  Chunk
     // k >= 1 runs on slot_stream[k - 1] with scratch, trace queue and

@@ -425,16 +425,15 @@ __device__ __forceinline__ void compact_push(bool alive, uint32_t i, const WRay&
     }
 }
 
-// PAR: the step parity of every path in the stream (0 or 1), or -1 (mixed).
+// The shading step of one stream slot.  PAR: the step parity of every path
+// in the stream (0 or 1), or -1 (mixed).
 template <int PAR>
-__global__ __launch_bounds__(RENDER_BLOCK) void k_render_update(RenderLaunch A, int64_t first, PathBuf P,
-                                                                const uint32_t* __restrict__ act_in,
-                                                                const float* __restrict__ rays_in, DevHits hits,
-                                                                int64_t n_in, int64_t rcap,
-                                                                uint32_t* __restrict__ act_out,
-                                                                float* __restrict__ rays_out,
-                                                                uint32_t* __restrict__ count_out,
-                                                                float* __restrict__ sample_xyz, int64_t sbase) {
+__device__ __forceinline__ void render_update_slot(const RenderLaunch& A, int64_t first, const PathBuf& P,
+                                                   const uint32_t* __restrict__ act_in,
+                                                   const float* __restrict__ rays_in, const DevHits& hits,
+                                                   int64_t n_in, int64_t rcap, uint32_t* __restrict__ act_out,
+                                                   float* __restrict__ rays_out, uint32_t* __restrict__ count_out,
+                                                   float* __restrict__ sample_xyz, int64_t sbase) {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool alive = false;
     uint32_t i = 0;
@@ -444,6 +443,31 @@ __global__ __launch_bounds__(RENDER_BLOCK) void k_render_update(RenderLaunch A, 
         alive = render_update_one<NoAdjoint, PAR>(A, first, P, i, j, rays_in, hits, rcap, sample_xyz, sbase, nr);
     }
     compact_push(alive, i, nr, act_out, rays_out, rcap, count_out);
+}
+
+template <int PAR>
+__global__ __launch_bounds__(RENDER_BLOCK) void k_render_update(RenderLaunch A, int64_t first, PathBuf P,
+                                                                const uint32_t* __restrict__ act_in,
+                                                                const float* __restrict__ rays_in, DevHits hits,
+                                                                int64_t n_in, int64_t rcap,
+                                                                uint32_t* __restrict__ act_out,
+                                                                float* __restrict__ rays_out,
+                                                                uint32_t* __restrict__ count_out,
+                                                                float* __restrict__ sample_xyz, int64_t sbase) {
+    render_update_slot<PAR>(A, first, P, act_in, rays_in, hits, n_in, rcap, act_out, rays_out, count_out, sample_xyz,
+                            sbase);
+}
+
+// The even (hit, then light sample) step with registers held to 128 for 4
+// waves per SIMD, spilling the rest to scratch: the light tree's descent
+// is a chain of dependent loads, which more waves hide where a scene has
+// many lights.  Same arithmetic as k_render_update<0>.
+__global__ __launch_bounds__(RENDER_BLOCK) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_render_update_w4(
+    RenderLaunch A, int64_t first, PathBuf P, const uint32_t* __restrict__ act_in, const float* __restrict__ rays_in,
+    DevHits hits, int64_t n_in, int64_t rcap, uint32_t* __restrict__ act_out, float* __restrict__ rays_out,
+    uint32_t* __restrict__ count_out, float* __restrict__ sample_xyz, int64_t sbase) {
+    render_update_slot<0>(A, first, P, act_in, rays_in, hits, n_in, rcap, act_out, rays_out, count_out, sample_xyz,
+                          sbase);
 }
 
 // ---- stream form ---------------------------------------------------------------
@@ -931,12 +955,16 @@ int launch_render_init(const RenderLaunch& R, int64_t first, int64_t n, const Re
 }
 
 int launch_render_update(const RenderLaunch& R, int64_t first, const RenderBuffers& B, int cur, int64_t n_in,
-                         int step, float* sample_xyz, int64_t sbase, hipStream_t st) {
+                         int step, int shade_waves, float* sample_xyz, int64_t sbase, hipStream_t st) {
     if (n_in <= 0) return 0;
     PathBuf P{B.paths, B.cap};
     const int nxt = cur ^ 1;
     const dim3 grid((unsigned)((n_in + RENDER_BLOCK - 1) / RENDER_BLOCK));
-    if (step & 1)
+    if (!(step & 1) && shade_waves == 4)
+        hipLaunchKernelGGL(k_render_update_w4, grid, dim3(RENDER_BLOCK), 0, st, R, first, P,
+                           (const uint32_t*)B.act[cur], (const float*)B.rays[cur], B.hits, n_in, B.cap, B.act[nxt],
+                           B.rays[nxt], B.count + nxt, sample_xyz, sbase);
+    else if (step & 1)
         hipLaunchKernelGGL(k_render_update<1>, grid, dim3(RENDER_BLOCK), 0, st, R, first, P,
                            (const uint32_t*)B.act[cur], (const float*)B.rays[cur], B.hits, n_in, B.cap, B.act[nxt],
                            B.rays[nxt], B.count + nxt, sample_xyz, sbase);

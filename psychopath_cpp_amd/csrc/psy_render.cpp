@@ -554,7 +554,8 @@ int run_wavefront(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm
         float* samp = sample_xyz ? sample_xyz : c.samples;
         const int64_t sbase = sample_xyz ? 0 : c.first;
         rc = adj ? launch_render_adjoint(R, c.first, c.B, adj[&c - S], c.cur, c.n_active, c.st)
-                 : launch_render_update(R, c.first, c.B, c.cur, c.n_active, c.step, samp, sbase, c.st);
+                 : launch_render_update(R, c.first, c.B, c.cur, c.n_active, c.step, ctx->render_shade_waves, samp,
+                                        sbase, c.st);
         if (rc) return fail(rc, "psy_render: update launch failed");
         HIP_OK(hipMemcpyAsync(c.count_host, c.B.count + (c.cur ^ 1), sizeof(uint32_t), hipMemcpyDeviceToHost, c.st));
         if (n_slots > 1) HIP_OK(hipEventRecord(c.done, c.st));
@@ -861,9 +862,9 @@ int psy_render_autotune(psy_ctx* ctx, psy_scene* s, const psy_render_params* prm
     const int64_t total = n_pix * p.spp;
     // saved so that a failed trial leaves the context as it was
     const int s_mode = ctx->render_mode, s_tmode = ctx->mode, s_refill = ctx->refill_min, s_stack = ctx->trace_stack,
-              s_iters = ctx->render_graph_iters, s_streams = ctx->render_streams;
+              s_iters = ctx->render_graph_iters, s_streams = ctx->render_streams, s_shade = ctx->render_shade_waves;
     const int64_t s_chunk = ctx->render_chunk;
-    struct Cfg { int mode; int64_t pool; int tmode, refill, stack, iters, streams; };
+    struct Cfg { int mode; int64_t pool; int tmode, refill, stack, iters, streams, shade; };
     auto apply = [&](const Cfg& c) {
         ctx->render_mode = c.mode;
         ctx->render_chunk = c.pool;
@@ -872,6 +873,7 @@ int psy_render_autotune(psy_ctx* ctx, psy_scene* s, const psy_render_params* prm
         ctx->trace_stack = c.stack;
         ctx->render_graph_iters = c.iters;
         ctx->render_streams = c.streams;
+        ctx->render_shade_waves = c.shade;
     };
     int trials = 0;
     const int64_t full = n_pix * prm->spp;
@@ -902,7 +904,7 @@ int psy_render_autotune(psy_ctx* ctx, psy_scene* s, const psy_render_params* prm
         ++trials;
         return PSY_OK;
     };
-    Cfg best{PSY_RENDER_STREAM, std::min<int64_t>((int64_t)1 << 21, total), PSY_TRACE_MODE_PLAIN, 16, 16, 4, 2};
+    Cfg best{PSY_RENDER_STREAM, std::min<int64_t>((int64_t)1 << 21, total), PSY_TRACE_MODE_PLAIN, 16, 16, 4, 2, 0};
     double best_m = 0.0;
     int rc = timed(best, &best_m);
     auto consider = [&](Cfg c) -> int {
@@ -943,6 +945,8 @@ int psy_render_autotune(psy_ctx* ctx, psy_scene* s, const psy_render_params* prm
             if (pool == best.pool && k == best.streams) continue;
             Cfg t = best; t.pool = pool; t.streams = k; rc = consider(t);
         }
+        // the hit step's shading held to 4 waves per SIMD
+        if (!rc) { Cfg t = best; t.shade = 4; rc = consider(t); }
     }
     // dicing depth: the scene's dice cache re-built 4 and 2 levels shallower
     // and 2 deeper (within 16 GiB and the 2^32-node cap); rays that need
@@ -968,6 +972,7 @@ int psy_render_autotune(psy_ctx* ctx, psy_scene* s, const psy_render_params* prm
     if (rc) {
         ctx->render_mode = s_mode; ctx->mode = s_tmode; ctx->refill_min = s_refill; ctx->trace_stack = s_stack;
         ctx->render_graph_iters = s_iters; ctx->render_chunk = s_chunk; ctx->render_streams = s_streams;
+        ctx->render_shade_waves = s_shade;
         return rc;
     }
     apply(best);
@@ -981,6 +986,7 @@ int psy_render_autotune(psy_ctx* ctx, psy_scene* s, const psy_render_params* prm
         out->pool = best.pool;
         out->dice_depth = best_levels;
         out->streams = best.streams;
+        out->shade_waves = best.shade;
         out->mrays_per_s = best_m;
     }
     return PSY_OK;

@@ -40,6 +40,7 @@ def lib():
     L.psy_ctx_trace_pass_counts.argtypes = [vp, C.POINTER(i64), C.POINTER(i64)]
     L.psy_ctx_set_render_chunk.argtypes = [vp, i64]
     L.psy_ctx_set_render_streams.argtypes = [vp, C.c_int]
+    L.psy_ctx_set_render_shade_waves.argtypes = [vp, C.c_int]
     L.psy_ctx_set_render_mode.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.psy_scene_create.argtypes = [vp, C.POINTER(abi.SceneDesc), C.c_int, C.POINTER(vp)]
     L.psy_scene_destroy.argtypes = [vp]
@@ -512,6 +513,12 @@ def set_render_chunk(ctx, paths):
 def set_render_streams(ctx, streams):
     """psy_ctx_set_render_streams: wavefront chunks in flight (1..4)."""
     check(lib().psy_ctx_set_render_streams(ctx.handle, int(streams)), "psy_ctx_set_render_streams")
+
+
+def set_render_shade_waves(ctx, waves):
+    """psy_ctx_set_render_shade_waves: 0 (compiler's registers) or 4 (the
+    hit step's shading held to 4 waves per SIMD)."""
+    check(lib().psy_ctx_set_render_shade_waves(ctx.handle, int(waves)), "psy_ctx_set_render_shade_waves")
 
 
 def set_render_mode(ctx, fused=False, refill_min=16, blocks_per_cu=0, stream=False):

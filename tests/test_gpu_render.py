@@ -322,6 +322,31 @@ def test_gpu_render_chunks_in_flight_equal_one(gpu_ctx, name):
         ctx.close()
 
 
+@pytest.mark.parametrize("name", ["spheres", "heightfield", "instanced_lights"])
+def test_gpu_render_capped_shading_equals_default(gpu_ctx, name):
+    """The hit step's shading kernel held to 4 waves per SIMD
+    (psy_ctx_set_render_shade_waves) spills registers but computes the same
+    samples, bit for bit, with one and with several chunks in flight."""
+    sc = {"spheres": configs.sphere_scene, "heightfield": lambda: configs.heightfield_scene(grid=16),
+          "instanced_lights": _instanced_lights_scene}[name]()
+    g = gpu.GpuScene(gpu_ctx, sc)
+    f_wf, s_wf, r_wf = gpu.render(gpu_ctx, g, 40, 24, 5, 3, film=True, samples=True)
+    ctx = gpu.Context(0)
+    try:
+        with pytest.raises(gpu.PsyError, match="0 or 4"):
+            gpu.set_render_shade_waves(ctx, 2)
+        gpu.set_render_shade_waves(ctx, 4)
+        for streams, chunk in ((1, 1 << 21), (3, 280 * 5)):
+            gpu.set_render_streams(ctx, streams)
+            gpu.set_render_chunk(ctx, chunk)
+            f_4, s_4, r_4 = gpu.render(ctx, g, 40, 24, 5, 3, film=True, samples=True)
+            say(name, "shade waves 4, streams", streams, "rays", r_4, r_wf)
+            assert r_4 == r_wf
+            assert torch.equal(s_4, s_wf) and torch.equal(f_4, f_wf)
+    finally:
+        ctx.close()
+
+
 def test_gpu_render_autotune(gpu_ctx):
     """psy_render_autotune leaves a setting on the context that renders the
     same image."""
@@ -338,6 +363,7 @@ def test_gpu_render_autotune(gpu_ctx):
         # the scene is left at the dicing depth the tuner picked
         assert t["dice_depth"] == g.info.dice_depth and 0 <= t["dice_depth"] <= 24
         assert 1 <= t["streams"] <= 4
+        assert t["shade_waves"] in (0, 4)
         f_s, _, r_s = gpu.render(ctx, g, 64, 48, 4, 1)
         assert r_s == r_wf and torch.equal(f_s, f_wf)
     finally:

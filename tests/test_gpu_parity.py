@@ -293,6 +293,31 @@ def test_gpu_set_dice_depth_keeps_the_hits(gpu_ctx, c1):
     assert _hits_equal(gpu.trace_world(gpu_ctx, g, w), base)
 
 
+def test_gpu_dice_depth_changes_only_tied_uv(gpu_ctx, c1):
+    """Over the full 1024x1024x2 camera stream of C1, a shallower dice cache
+    keeps every hit's distance and instance bit for bit.  Only where two
+    micro-patches give exactly the same distance may (u, v) differ, because
+    the second pass walks a deferred subtree after the rest of the walk and
+    the first hit found is kept: at most 1 in 10^4 rays (23 of 2^21 at depth
+    12 when measured)."""
+    sc, g, o = c1
+    w, _ = gpu.camera_rays(gpu_ctx, g, 1024, 1024, 2, 1)
+    base = gpu.trace_world(gpu_ctx, g, w)
+    # bit patterns (a miss may carry a NaN distance)
+    b = {k: getattr(base, k).view(torch.int32).clone() for k in ("t", "inst", "u", "v")}
+    n = w.shape[1]
+    try:
+        for d in (12, 8):
+            g.set_dice_depth(d)
+            h = gpu.trace_world(gpu_ctx, g, w)
+            assert torch.equal(h.t.view(torch.int32), b["t"]) and torch.equal(h.inst, b["inst"])
+            uv = int(((h.u.view(torch.int32) != b["u"]) | (h.v.view(torch.int32) != b["v"])).sum())
+            say("dice depth", d, "hits with another (u, v):", uv, "of", n)
+            assert uv <= n // 10000
+    finally:
+        g.set_dice_depth(-1)
+
+
 def test_gpu_empty_and_single_instance_scenes(gpu_ctx, oracle):
     root = Assembly()
     sc = Scene(Camera([look_at((0, -5, 0), (0, 0, 0))], [np.radians(40)]), root)

@@ -266,8 +266,9 @@ struct AdjointBuffers {
 int launch_fill_f32(float* p, int64_t n, float v, hipStream_t st);
 int launch_adjoint_init(const RenderLaunch& R, const RenderBuffers& B, const AdjointBuffers& J, int64_t n,
                         hipStream_t st);
+// Every path of the stream is at step `step`.
 int launch_render_adjoint(const RenderLaunch& R, int64_t first, const RenderBuffers& B, const AdjointBuffers& J,
-                          int cur, int64_t n_in, hipStream_t st);
+                          int cur, int64_t n_in, int step, hipStream_t st);
 int launch_adjoint_reduce(const AdjointBuffers& J, float* d_albedo, float* d_emission, hipStream_t st);
 int launch_render_resolve(const float* sample_xyz, int64_t sbase, int64_t first_pixel, int64_t n_pixels, int spp,
                           float* film, hipStream_t st);

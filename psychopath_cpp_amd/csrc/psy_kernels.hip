@@ -575,8 +575,9 @@ __global__ __launch_bounds__(RENDER_BLOCK) void k_adjoint_init(RenderLaunch A, P
 // every active path (the forward step, identical decisions) plus the
 // gradients of its contributions.  A fixed grid walks the stream; with LDS
 // sums each block adds its row into partials[blockIdx.x] at the end (rows are
-// owned by block index, and launches on one stream do not overlap).
-template <bool LDS>
+// owned by block index, and launches on one stream do not overlap).  PAR:
+// the step parity of every path in the stream, as in k_render_update.
+template <bool LDS, int PAR>
 __global__ __launch_bounds__(RENDER_BLOCK) void k_render_adjoint(
     RenderLaunch A, int64_t first, PathBuf P, PathBuf Q, const uint32_t* __restrict__ act_in,
     const float* __restrict__ rays_in, DevHits hits, int64_t n_in, int64_t rcap, uint32_t* __restrict__ act_out,
@@ -596,7 +597,7 @@ __global__ __launch_bounds__(RENDER_BLOCK) void k_render_adjoint(
         WRay nr;
         if (j < n_in) {
             i = act_in[j];
-            alive = render_update_one(A, first, P, i, j, rays_in, hits, rcap, nullptr, 0, nr, acc);
+            alive = render_update_one<AdjAcc, PAR>(A, first, P, i, j, rays_in, hits, rcap, nullptr, 0, nr, acc);
         }
         compact_push(alive, i, nr, act_out, rays_out, rcap, count_out);
     }
@@ -1006,22 +1007,28 @@ int launch_adjoint_init(const RenderLaunch& R, const RenderBuffers& B, const Adj
     return hipGetLastError() == hipSuccess ? 0 : PSY_ERR_HIP;
 }
 
-int launch_render_adjoint(const RenderLaunch& R, int64_t first, const RenderBuffers& B, const AdjointBuffers& J,
-                          int cur, int64_t n_in, hipStream_t st) {
-    if (n_in <= 0) return 0;
+template <bool LDS, int PAR>
+static void launch_adjoint_kernel(const RenderLaunch& R, int64_t first, const RenderBuffers& B,
+                                  const AdjointBuffers& J, int cur, int64_t n_in, unsigned grid, hipStream_t st) {
     const int nxt = cur ^ 1;
+    const PathBuf P{B.paths, B.cap}, Q{J.q, B.cap};
+    hipLaunchKernelGGL((k_render_adjoint<LDS, PAR>), dim3(grid), dim3(RENDER_BLOCK),
+                       LDS ? (size_t)J.n_params * sizeof(float) : 0, st, R, first, P, Q, (const uint32_t*)B.act[cur],
+                       (const float*)B.rays[cur], B.hits, n_in, B.cap, B.act[nxt], B.rays[nxt], B.count + nxt,
+                       J.n_albedo, J.n_params, J.lparam, J.grad, J.partials);
+}
+
+int launch_render_adjoint(const RenderLaunch& R, int64_t first, const RenderBuffers& B, const AdjointBuffers& J,
+                          int cur, int64_t n_in, int step, hipStream_t st) {
+    if (n_in <= 0) return 0;
     const int64_t want = (n_in + RENDER_BLOCK - 1) / RENDER_BLOCK;
     const unsigned grid = (unsigned)std::min<int64_t>(want, J.rows);
-    const PathBuf P{B.paths, B.cap}, Q{J.q, B.cap};
     if (J.lds) {
-        hipLaunchKernelGGL(k_render_adjoint<true>, dim3(grid), dim3(RENDER_BLOCK), (size_t)J.n_params * sizeof(float),
-                           st, R, first, P, Q, (const uint32_t*)B.act[cur], (const float*)B.rays[cur], B.hits, n_in,
-                           B.cap, B.act[nxt], B.rays[nxt], B.count + nxt, J.n_albedo, J.n_params, J.lparam, J.grad,
-                           J.partials);
+        if (step & 1) launch_adjoint_kernel<true, 1>(R, first, B, J, cur, n_in, grid, st);
+        else launch_adjoint_kernel<true, 0>(R, first, B, J, cur, n_in, grid, st);
     } else {
-        hipLaunchKernelGGL(k_render_adjoint<false>, dim3(grid), dim3(RENDER_BLOCK), 0, st, R, first, P, Q,
-                           (const uint32_t*)B.act[cur], (const float*)B.rays[cur], B.hits, n_in, B.cap, B.act[nxt],
-                           B.rays[nxt], B.count + nxt, J.n_albedo, J.n_params, J.lparam, J.grad, J.partials);
+        if (step & 1) launch_adjoint_kernel<false, 1>(R, first, B, J, cur, n_in, grid, st);
+        else launch_adjoint_kernel<false, 0>(R, first, B, J, cur, n_in, grid, st);
     }
     return hipGetLastError() == hipSuccess ? 0 : PSY_ERR_HIP;
 }

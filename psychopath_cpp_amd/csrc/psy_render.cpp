@@ -553,7 +553,7 @@ int run_wavefront(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm
         if ((rc = launch_zero_u32(c.B.count + (c.cur ^ 1), 1, c.st))) return fail(rc, "psy_render: launch failed");
         float* samp = sample_xyz ? sample_xyz : c.samples;
         const int64_t sbase = sample_xyz ? 0 : c.first;
-        rc = adj ? launch_render_adjoint(R, c.first, c.B, adj[&c - S], c.cur, c.n_active, c.st)
+        rc = adj ? launch_render_adjoint(R, c.first, c.B, adj[&c - S], c.cur, c.n_active, c.step, c.st)
                  : launch_render_update(R, c.first, c.B, c.cur, c.n_active, c.step, ctx->render_shade_waves, samp,
                                         sbase, c.st);
         if (rc) return fail(rc, "psy_render: update launch failed");

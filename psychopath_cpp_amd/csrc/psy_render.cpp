@@ -950,6 +950,12 @@ int psy_render_autotune(psy_ctx* ctx, psy_scene* s, const psy_render_params* prm
         // the other stack depth again, with the chunks now in flight (more
         // chunks in flight favour the 16-entry stack's higher occupancy)
         if (!rc) { Cfg t = best; t.stack = best.stack == 16 ? 32 : 16; rc = consider(t); }
+        // and the other one-thread-per-ray launch form
+        if (!rc && (best.tmode == PSY_TRACE_MODE_PLAIN || best.tmode == PSY_TRACE_MODE_PLAIN_COHERENT)) {
+            Cfg t = best;
+            t.tmode = best.tmode == PSY_TRACE_MODE_PLAIN ? PSY_TRACE_MODE_PLAIN_COHERENT : PSY_TRACE_MODE_PLAIN;
+            rc = consider(t);
+        }
     }
     // dicing depth: the scene's dice cache re-built 4 and 2 levels shallower
     // and 2 deeper (within 16 GiB and the 2^32-node cap); rays that need

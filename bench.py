@@ -268,6 +268,38 @@ def full_config_renders():
     return out
 
 
+def render_profile():
+    """One render each of C1 (1024x1024, 16 spp, path length 4) and C2
+    (960x540, 8 spp, path length 6) after a warm-up render, with one
+    wavefront chunk in flight so that kernel durations do not overlap: the
+    integrator's time per kernel under a kernel-trace profiler."""
+    import torch
+    from psychopath_cpp_amd import configs, gpu
+    out = []
+    for name, make, W, H, spp, pl, stack, depth in (
+            ("C1 heightfield 1024x1024, 16 spp, path length 4", configs.heightfield_scene, 1024, 1024, 16, 4, 16, -1),
+            ("C2 bicubic motion 960x540, 8 spp, path length 6", configs.bicubic_sphere_scene, 960, 540, 8, 6, 32, 12)):
+        ctx = gpu.Context(0)
+        g = gpu.GpuScene(ctx, make(), dice_depth=depth)
+        gpu.set_render_mode(ctx)
+        ctx.set_trace_mode(persistent=False, coherent=True)
+        ctx.set_trace_stack(stack)
+        gpu.set_render_streams(ctx, 1)
+        gpu.set_render_chunk(ctx, 1 << 23)
+        gpu.render(ctx, g, W, H, spp, 1, pl)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, _, rays = gpu.render(ctx, g, W, H, spp, 1, pl)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        log(f"render profile [{name}]: {rays} rays in {dt * 1e3:.2f} ms")
+        out.append({"workload": name, "rays": int(rays), "ms": round(dt * 1e3, 2),
+                    "mrays": round(rays / dt / 1e6, 1), "dice_depth": int(g.info.dice_depth)})
+        g.close()
+        ctx.close()
+    return out
+
+
 def subdivision_stream(args):
     """The primary + first-bounce stream (2^21 + 2^21 rays) through the
     Catmull-Clark scene (configs.subdivision_scene: the heightfield as one
@@ -414,6 +446,9 @@ def main():
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="persistent mode: blocks per CU (0: occupancy)")
     ap.add_argument("--split", action="store_true",
                     help="also time the primary and the bounce half of the stream separately (stderr)")
+    ap.add_argument("--render-profile", action="store_true",
+                    help="only render C1 (1024x1024x16) and C2 (960x540x8) once each with one wavefront chunk in "
+                         "flight, for a per-kernel profile of the integrator")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device-to-device copy of BYTES (calibrates rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -422,6 +457,9 @@ def main():
         return 0
     if args.subdiv:
         print(json.dumps({"subdivision_stream": subdivision_stream(args)}), flush=True)
+        return 0
+    if args.render_profile:
+        print(json.dumps({"render_profile": render_profile()}), flush=True)
         return 0
     if args.copy_probe:
         import torch
@@ -435,7 +473,7 @@ def main():
         return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box has one GPU; "
-              "the path shards by image tiles or ray ranges, see DESIGN.md)", file=sys.stderr)
+              "the path would shard by image tiles or ray ranges)", file=sys.stderr)
         return 2
 
     import torch

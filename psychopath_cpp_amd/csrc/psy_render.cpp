@@ -498,7 +498,7 @@ struct ChunkSlot {
     uint32_t* counters;
     int64_t queue_cap;
     hipStream_t st;
-    hipEvent_t done;  // recorded after each step's count read back (two slots)
+    hipEvent_t done;  // recorded after each step's count read back (several slots)
     uint32_t* count_host;
     int64_t first = 0, n = 0, n_active = 0;  // n == 0: idle
     int cur = 0, step = 0;
@@ -597,7 +597,7 @@ int run_wavefront(psy_ctx* ctx, const psy_scene* s, const psy_render_params* prm
             } else {
                 HIP_OK(hipStreamSynchronize(c.st));
             }
-            c.n_active = *c.count_host;
+            c.n_active = *(volatile const uint32_t*)c.count_host;  // written by the copy engine
             c.cur ^= 1;
             if (++c.step < 2 * prm->path_length && c.n_active > 0) {
                 if (int rc = step(c)) return rc;

@@ -281,7 +281,9 @@ int psy_ctx_set_render_chunk(psy_ctx* ctx, int64_t paths);
  * buffers per chunk: k times the render scratch), and the host queues each
  * chunk's next step as soon as that chunk's previous one ends, so each
  * chunk's short late bounces run beside the others' work.  The caller's
- * stream waits for all of them at the end.  Same samples as with 1. */
+ * stream waits for all of them at the end.  While such a render runs, the
+ * calling thread polls the chunks' events (one host core stays busy).
+ * Same samples as with 1. */
 int psy_ctx_set_render_streams(psy_ctx* ctx, int streams);
 /* Register budget of PSY_RENDER_WAVEFRONT's shading kernel for the steps
  * that take a hit and sample a light (no counterpart in the reference):

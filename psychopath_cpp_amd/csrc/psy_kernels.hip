@@ -577,8 +577,11 @@ __global__ __launch_bounds__(RENDER_BLOCK) void k_adjoint_init(RenderLaunch A, P
 // sums each block adds its row into partials[blockIdx.x] at the end (rows are
 // owned by block index, and launches on one stream do not overlap).  PAR:
 // the step parity of every path in the stream, as in k_render_update.
+// Held to 2 waves per SIMD: the even step otherwise takes 256 VGPRs and 24
+// AGPRs (1 wave); at 256 registers it spills 80-112 B per lane and runs
+// the C4 adjoint in 8.9 ms against 10.8.
 template <bool LDS, int PAR>
-__global__ __launch_bounds__(RENDER_BLOCK) void k_render_adjoint(
+__global__ __launch_bounds__(RENDER_BLOCK) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_render_adjoint(
     RenderLaunch A, int64_t first, PathBuf P, PathBuf Q, const uint32_t* __restrict__ act_in,
     const float* __restrict__ rays_in, DevHits hits, int64_t n_in, int64_t rcap, uint32_t* __restrict__ act_out,
     float* __restrict__ rays_out, uint32_t* __restrict__ count_out, int32_t n_albedo, int32_t n_params,
